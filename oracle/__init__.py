@@ -1,0 +1,369 @@
+"""CPU FP64 oracle for the QXMD Gamma-point H|psi> / band-update path.
+
+TEST INFRASTRUCTURE ONLY.  Used by tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` as the checker; the product package
+``qxmd_amd`` never imports it.
+
+The arithmetic lives in ``qxoracle.c`` (plain C, FP64, each routine citing the
+reference file:line it restates); this module loads ``liboracle.so`` through
+ctypes and offers numpy-shaped wrappers.  Pinning status of every function is
+listed in ``PINNING`` below.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+# How each oracle function is pinned (checked by tests/test_oracle_*.py).
+PINNING = {
+    "get_mesh": "reference get_mesh/chkftm (oracle/_ref) + grid sizes 30/60/90/120",
+    "strvec": "bit-exact vs reference strvec/set_rciprl (oracle/_ref); lattice counts",
+    "ijkgd": "bit-exact vs reference reclat (oracle/_ref); point-reflection invariant",
+    "rnd00": "reference rnd00 (oracle/_ref); closed-form Lehmer values",
+    "rfft3/fft3": "numpy.fft (fftn / irfftn) with FFTW sign and 1/N convention",
+    "hcnonloc": "constant-V closed form, single plane wave, hermiticity, numpy FFT",
+    "ekinetic": "reference ekinetic2 (oracle/_ref)",
+    "hckbpp_g": "reference hckbpp_g / hckbpp_ib_g (oracle/_ref); GEMM identity",
+    "enkbpp_g": "reference enkbpp_g2 (oracle/_ref)",
+    "schmidt": "reference schmidt2 (oracle/_ref); Psi^T Psi = I",
+    "hmatrix": "reference hmatrix2 (oracle/_ref)",
+    "eigh": "reference RsEIGQR (oracle/_ref) and numpy.linalg.eigh",
+    "untrycore": "reference untrycore (oracle/_ref)",
+    "softchg": "integral = sum occ; numpy FFT of psi(r)^2",
+    "prpcg": "reference prpcg (oracle/_ref)",
+    "eigen_bkl": "Rayleigh quotient decrease, residual invariants; per-step pieces above",
+}
+
+
+def build(force: bool = False) -> str:
+    so = os.path.join(_HERE, "liboracle.so")
+    src = os.path.join(_HERE, "qxoracle.c")
+    if force or not os.path.exists(so) or os.path.getmtime(so) < os.path.getmtime(src):
+        subprocess.check_call(["make", "-s", "-C", _HERE], stdout=subprocess.DEVNULL)
+    return so
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        so = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(so):
+            build()
+        _LIB = C.CDLL(so)
+        _declare(_LIB)
+    return _LIB
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i = C.c_int
+_d = C.c_double
+_vp = C.c_void_p
+
+
+class Ham(C.Structure):
+    _fields_ = [
+        ("nplw", _i), ("nplwex", _i), ("n1", _i), ("n2", _i), ("n3", _i),
+        ("ijkgd_c", _vp), ("glocal", _vp), ("dkgnrm", _vp), ("eqlmr", _vp),
+        ("lmmax", _i), ("natom", _i), ("lmx_atom", _vp), ("vclalp_atom", _vp),
+    ]
+
+
+def _declare(L):
+    L.qxo_rciprl.argtypes = [_dp, _dp, C.POINTER(_d)]
+    L.qxo_fft_size.argtypes = [_i, _i]
+    L.qxo_fft_size.restype = _i
+    L.qxo_get_mesh.argtypes = [_dp, _d, _ip]
+    L.qxo_strvec_capacity.argtypes = [_dp, _d]
+    L.qxo_strvec_capacity.restype = _i
+    L.qxo_strvec.argtypes = [_dp, _d, _d, _d, _d, _d, _i, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp,
+                             _dp, C.POINTER(_d)]
+    L.qxo_strvec.restype = _i
+    L.qxo_ijkgd.argtypes = [_i, _ip, _ip, _ip, _ip, _ip]
+    L.qxo_dkgnrm.argtypes = [_i, _dp, _dp]
+    L.qxo_rnd00.argtypes = [C.POINTER(_d)]
+    L.qxo_rnd00.restype = _d
+    L.qxo_cfft3.argtypes = [_i, _dp, _i, _i, _i]
+    L.qxo_rfft3.argtypes = [_i, _dp, _dp, _i, _i, _i]
+    L.qxo_fft3.argtypes = [_i, _dp, _dp, _i, _i, _i]
+    L.qxo_wvg2r.argtypes = [_dp, _i, _vp, _i, _i, _i, _dp, _dp]
+    L.qxo_hcnonloc.argtypes = [_dp, _dp, _dp, _i, _vp, _i, _i, _i, _dp, _dp]
+    L.qxo_ekinetic.argtypes = [_dp, _dp, _dp, _i, _i, _vp, _vp, _vp]
+    L.qxo_hckbpp_g.argtypes = [_dp, _i, _i, _i, _ip, _dp, _dp, _dp, _i, _vp]
+    L.qxo_enkbpp_g.argtypes = [_dp, _i, _i, _ip, _dp, _dp, _i, _d]
+    L.qxo_enkbpp_g.restype = _d
+    L.qxo_rsdecmp.argtypes = [_dp, _i, _i]
+    L.qxo_rsinvcd.argtypes = [_dp, _i, _i, _dp]
+    L.qxo_schmidt.argtypes = [_dp, _i, _i, _dp]
+    L.qxo_schmidt.restype = _i
+    L.qxo_hmatrix.argtypes = [_dp, _dp, _i, _i, _dp, _dp]
+    L.qxo_eigh.argtypes = [_dp, _i, _i, _dp]
+    L.qxo_untrycore.argtypes = [_dp, _i, _i, _dp, _i, _dp]
+    L.qxo_softchg.argtypes = [_dp, _i, _i, _i, _dp, _d, _vp, _i, _i, _i, _dp]
+    L.qxo_prpcg.argtypes = [_d, _dp, _i, _dp]
+    L.qxo_hpsi.argtypes = [C.POINTER(Ham), _dp, _dp, _i, _vp]
+    L.qxo_eigen_bkl.argtypes = [C.POINTER(Ham), _dp, _dp, _dp, _dp, _i, _i, _i, _i, _d, _d, _d,
+                                _d, _d, _dp, _ip]
+    L.qxo_scf_band_step.argtypes = [C.POINTER(Ham), _dp, _dp, _dp, _dp, _i, _i, _d, _vp,
+                                    C.POINTER(_d), C.POINTER(_d), _vp]
+    L.qxo_scf_band_step.restype = _i
+
+
+def _f(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# ---------------------------------------------------------------- grid / sphere
+
+def rciprl(a):
+    """a: (3,3) with lattice vectors as columns, i.e. a[:, i] = vector i."""
+    af = _f(np.asarray(a, np.float64).T.ravel())  # column-major storage
+    b = np.zeros(9)
+    v = _d()
+    lib().qxo_rciprl(af, b, C.byref(v))
+    return b.reshape(3, 3).T, v.value
+
+
+def fft_size(n, first_dim=True):
+    return lib().qxo_fft_size(int(n), 1 if first_dim else 0)
+
+
+def get_mesh(hcell, ecutdens):
+    nd = np.zeros(3, np.int32)
+    lib().qxo_get_mesh(_f(np.asarray(hcell, np.float64).T.ravel()), float(ecutdens), nd)
+    return tuple(int(x) for x in nd)
+
+
+def strvec(rba, ecutdens, ecut, ecutsoft=None, ecutc=None, ecutorth=None):
+    """G-vector generation and stable |G| sort.  Returns a dict with the counts
+    and the sorted arrays (index 0 = G=0)."""
+    ecutsoft = ecutdens if ecutsoft is None else ecutsoft
+    ecutc = ecut if ecutc is None else ecutc
+    ecutorth = ecut if ecutorth is None else ecutorth
+    a = _f(np.asarray(rba, np.float64).T.ravel())
+    cap = lib().qxo_strvec_capacity(a, float(ecutdens))
+    counts = np.zeros(6, np.int32)
+    nga, ngb, ngc = (np.zeros(cap, np.int32) for _ in range(3))
+    gx, gy, gz, rec = (np.zeros(cap) for _ in range(4))
+    cross = np.zeros(9)
+    rvol = _d()
+    num = lib().qxo_strvec(a, float(ecutdens), float(ecut), float(ecutsoft), float(ecutc),
+                           float(ecutorth), cap, counts, nga, ngb, ngc, gx, gy, gz, rec, cross,
+                           C.byref(rvol))
+    assert num > 0
+    return dict(num=int(counts[0]), nplw5=int(counts[1]), nplw=int(counts[2]),
+                nplw2=int(counts[3]), nplwc=int(counts[4]), nplw7=int(counts[5]),
+                nga=nga, ngb=ngb, ngc=ngc, gx=gx, gy=gy, gz=gz, recnrm=rec,
+                cross=cross.reshape(3, 3).T, rvol=rvol.value)
+
+
+def ijkgd(nga, ngb, ngc, nplw5, nd):
+    out = np.zeros(2 * nplw5 + 1, np.int32)
+    lib().qxo_ijkgd(int(nplw5), _f(nga, np.int32), _f(ngb, np.int32), _f(ngc, np.int32),
+                    _f(nd, np.int32), out)
+    return out
+
+
+def dkgnrm(recnrm, nplwc):
+    out = np.zeros(2 * (nplwc + 1))
+    lib().qxo_dkgnrm(int(nplwc), _f(recnrm), out)
+    return out
+
+
+def rnd00_sequence(seed, count):
+    s = _d(float(seed))
+    return np.array([lib().qxo_rnd00(C.byref(s)) for _ in range(count)]), s.value
+
+
+# ---------------------------------------------------------------- FFT
+
+def rfft3(key, fft3x, fft3y, nd):
+    x, y = _f(fft3x).copy(), _f(fft3y).copy()
+    lib().qxo_rfft3(int(key), x, y, *[int(v) for v in nd])
+    return x, y
+
+
+def fft3(key, fft3x, fft3y, nd):
+    x, y = _f(fft3x).copy(), _f(fft3y).copy()
+    lib().qxo_fft3(int(key), x, y, *[int(v) for v in nd])
+    return x, y
+
+
+def _cptr(arr, offset=0):
+    return C.c_void_p(arr.ctypes.data + offset * arr.itemsize)
+
+
+def wvg2r(cgjr, nplw, ijk, nd):
+    N = int(np.prod(nd))
+    x, y = np.zeros(N), np.zeros(N)
+    lib().qxo_wvg2r(_f(cgjr), int(nplw), _cptr(ijk, (len(ijk) - 1) // 2), *[int(v) for v in nd],
+                    x, y)
+    return x
+
+
+def hcnonloc(glocal, cgjr, rhcr, nplw, ijk, nd):
+    N = int(np.prod(nd))
+    r = _f(rhcr).copy()
+    x, y = np.zeros(N), np.zeros(N)
+    lib().qxo_hcnonloc(_f(glocal), _f(cgjr), r, int(nplw), _cptr(ijk, (len(ijk) - 1) // 2),
+                       *[int(v) for v in nd], x, y)
+    return r
+
+
+def ekinetic(dkg, cgjr, occ=None):
+    """cgjr: (nb, nplwex).  Returns (rhcr, ekib, ekint)."""
+    c = _f(cgjr)
+    nb, n = c.shape
+    r = np.zeros_like(c)
+    ekib = np.zeros(nb)
+    ekint = np.zeros(1)
+    occ_a = _f(occ) if occ is not None else np.zeros(nb)
+    lib().qxo_ekinetic(_f(dkg), c, r, n, nb, _cptr(occ_a), _cptr(ekib), _cptr(ekint))
+    return r, ekib, float(ekint[0])
+
+
+def hckbpp_g(eqlmr, lmx_atom, vclalp_atom, cgjr, rhcr):
+    """eqlmr: (natom, lmmax, nplwex); vclalp_atom: (natom, lmmax); cgjr/rhcr: (nb, nplwex).
+    Returns (rhcr + KB, slmir[nb, natom, lmmax])."""
+    e = _f(eqlmr)
+    natom, lmmax, n = e.shape
+    c = _f(cgjr)
+    nb = c.shape[0]
+    r = _f(rhcr).copy()
+    sl = np.zeros((nb, natom, lmmax))
+    lib().qxo_hckbpp_g(e, n, lmmax, natom, _f(lmx_atom, np.int32), _f(vclalp_atom), c, r, nb,
+                       _cptr(sl))
+    return r, sl
+
+
+def enkbpp_g(slmir, lmx_atom, clalp_atom, occ, rvol):
+    sl = _f(slmir)
+    nb, natom, lmmax = sl.shape
+    return lib().qxo_enkbpp_g(sl, lmmax, natom, _f(lmx_atom, np.int32), _f(clalp_atom), _f(occ),
+                              nb, float(rvol))
+
+
+def rsdecmp_rsinvcd(s):
+    """Cholesky + inverse of the lower factor; s symmetric (n, n).  Returns Linv (lower)."""
+    n = s.shape[0]
+    a = _f(np.asarray(s).T).copy()  # column-major
+    lib().qxo_rsdecmp(a, n, n)
+    w = np.zeros(n)
+    lib().qxo_rsinvcd(a, n, n, w)
+    return a.reshape(n, n).T
+
+
+def schmidt(cgjr):
+    c = _f(cgjr).copy()
+    nb, n = c.shape
+    s = np.zeros(nb * nb)
+    ok = lib().qxo_schmidt(c, n, nb, s)
+    return c, bool(ok)
+
+
+def hmatrix(cgjr, rhcr):
+    c, r = _f(cgjr), _f(rhcr)
+    nb, n = c.shape
+    h = np.zeros(nb * nb)
+    eig = np.zeros(nb)
+    lib().qxo_hmatrix(c, r, n, nb, h, eig)
+    return h.reshape(nb, nb).T.copy(), eig
+
+
+def eigh(a):
+    n = a.shape[0]
+    m = _f(np.asarray(a).T).copy()
+    w = np.zeros(n)
+    lib().qxo_eigh(m, n, n, w)
+    return w, m.reshape(n, n).T.copy()
+
+
+def untrycore(gdcr, u):
+    g = _f(gdcr).copy()
+    nb, n = g.shape
+    work = np.zeros_like(g)
+    lib().qxo_untrycore(g, n, nb, _f(np.asarray(u).T), nb, work)
+    return g
+
+
+def softchg(cgjr, nplw, occ, rvol, ijk, nd):
+    c = _f(cgjr)
+    nb, n = c.shape
+    rho = np.zeros(int(np.prod(nd)))
+    lib().qxo_softchg(c, n, int(nplw), nb, _f(occ), float(rvol), _cptr(ijk, (len(ijk) - 1) // 2),
+                      *[int(v) for v in nd], rho)
+    return rho
+
+
+def prpcg(ekib, dkg):
+    out = np.zeros(len(dkg))
+    lib().qxo_prpcg(float(ekib), _f(dkg), len(dkg), out)
+    return out
+
+
+# ---------------------------------------------------------------- Hamiltonian
+
+class Hamiltonian:
+    """Holds the arrays a ``Ham`` struct points to (keeps them alive)."""
+
+    def __init__(self, nplw, nd, ijk, glocal, dkg, eqlmr=None, lmx_atom=None, vclalp_atom=None):
+        self.nplw = int(nplw)
+        self.nplwex = 2 * (self.nplw + 1)
+        self.nd = tuple(int(v) for v in nd)
+        self.ijk = _f(ijk, np.int32)
+        self.glocal = _f(glocal)
+        self.dkg = _f(dkg)[: self.nplwex].copy()
+        self.eqlmr = None if eqlmr is None else _f(eqlmr)
+        self.lmx = None if lmx_atom is None else _f(lmx_atom, np.int32)
+        self.vcl = None if vclalp_atom is None else _f(vclalp_atom)
+        h = Ham()
+        h.nplw, h.nplwex = self.nplw, self.nplwex
+        h.n1, h.n2, h.n3 = self.nd
+        h.ijkgd_c = _cptr(self.ijk, (len(self.ijk) - 1) // 2)
+        h.glocal = _cptr(self.glocal)
+        h.dkgnrm = _cptr(self.dkg)
+        if self.eqlmr is not None:
+            h.eqlmr = _cptr(self.eqlmr)
+            h.natom, h.lmmax = self.eqlmr.shape[0], self.eqlmr.shape[1]
+            h.lmx_atom = _cptr(self.lmx)
+            h.vclalp_atom = _cptr(self.vcl)
+        else:
+            h.eqlmr = None
+            h.natom = h.lmmax = 0
+        self.h = h
+
+    def hpsi(self, psi):
+        p = _f(psi)
+        out = np.zeros_like(p)
+        lib().qxo_hpsi(C.byref(self.h), p, out, p.shape[0], None)
+        return out
+
+    def eigen_bkl(self, eig, gdcr, rhcr, ekib, noband=0, itermx=4, iteremx=4, tolera=1e-10,
+                  threinn=0.0, wdinner=0.9, roughfzan=0.3, roughfeig=0.3):
+        e, g, r = _f(eig).copy(), _f(gdcr).copy(), _f(rhcr).copy()
+        nb = g.shape[0]
+        bzan = np.zeros(nb)
+        nriter = np.zeros(nb, np.int32)
+        lib().qxo_eigen_bkl(C.byref(self.h), e, g, r, _f(ekib), nb, int(noband), int(itermx),
+                            int(iteremx), tolera, threinn, wdinner, roughfzan, roughfeig, bzan,
+                            nriter)
+        return e, g, r, bzan, nriter
+
+    def scf_band_step(self, gdcr, occ, rvol, noband=0, clalp_atom=None, want_rho=True):
+        g = _f(gdcr).copy()
+        nb = g.shape[0]
+        r = np.zeros_like(g)
+        eig = np.zeros(nb)
+        rho = np.zeros(int(np.prod(self.nd))) if want_rho else None
+        ek, en = _d(), _d()
+        cl = _f(clalp_atom) if clalp_atom is not None else None
+        ok = lib().qxo_scf_band_step(C.byref(self.h), g, r, eig, _f(occ), nb, int(noband),
+                                     float(rvol), _cptr(rho) if rho is not None else None,
+                                     C.byref(ek), C.byref(en), _cptr(cl) if cl is not None else None)
+        return dict(psi=g, hpsi=r, eig=eig, rho=rho, ekin=ek.value, enl=en.value, ok=bool(ok))

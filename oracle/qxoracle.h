@@ -1,0 +1,97 @@
+/*
+ * qxoracle.h -- CPU FP64 oracle for the QXMD Gamma-point H|psi> / band-update path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm, used as the checker by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py.  The product (qxmd_amd/) never links or calls it.
+ *
+ * Conventions: all arrays use the reference's Fortran layouts, 0-based in C.
+ *   - Wavefunctions are real-packed half-sphere vectors of nplwex = 2*(nplw+1)
+ *     doubles, (Re, Im) interleaved for G index 0..nplw (Sources/reclat.f90:90).
+ *     Band-major blocks: psi(ig, ib) at psi[ig + nplwex*ib].
+ *   - FFT grids are x-fastest: ijk = i + n1*(j + n2*k) (Sources/fft3.f90:166-224).
+ *   - ijkgd maps are the reference's 1-based values; ijkgd_c points at the m=0
+ *     entry so ijkgd_c[m] is valid for -nplw5 <= m <= nplw5 (Sources/reclat.f90:259-276).
+ */
+#ifndef QXORACLE_H
+#define QXORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- lattice / grid rule ---------------------------------------------- */
+void qxo_rciprl(const double a[9], double b[9], double *v);
+int  qxo_fft_size(int n, int first_dim);
+void qxo_get_mesh(const double hcell[9], double ecutdens, int nd[3]);
+
+/* ---- G sphere (strvec + vsgar/vsoe + set_rciprl) ------------------------ */
+int  qxo_strvec_capacity(const double rba[9], double ecutdens);
+int  qxo_strvec(const double rba[9], double ecutdens, double ecut, double ecutsoft,
+                double ecutc, double ecutorth, int cap, int counts[6],
+                int *nga, int *ngb, int *ngc, double *gx, double *gy, double *gz,
+                double *recnrm, double cross[9], double *rvol);
+void qxo_ijkgd(int nplw5, const int *nga, const int *ngb, const int *ngc,
+               const int nd[3], int *ijkgd /* 2*nplw5+1 entries */);
+void qxo_dkgnrm(int nplwc, const double *recnrm, double *dkgnrm);
+double qxo_rnd00(double *seed);
+
+/* ---- FFT (FFTW3 semantics of rfft3 / fft3) ------------------------------ */
+void qxo_cfft3(int sign, double *xy /* complex interleaved, N */, int n1, int n2, int n3);
+void qxo_rfft3(int key, double *fft3x, double *fft3y, int n1, int n2, int n3);
+void qxo_fft3(int key, double *fft3x, double *fft3y, int n1, int n2, int n3);
+
+/* ---- local H|psi>, kinetic ---------------------------------------------- */
+void qxo_wvg2r(const double *cgjr, int nplw, const int *ijkgd_c, int n1, int n2, int n3,
+               double *fft3x, double *fft3y);
+void qxo_hcnonloc(const double *glocal, const double *cgjr, double *rhcr, int nplw,
+                  const int *ijkgd_c, int n1, int n2, int n3, double *fft3x, double *fft3y);
+void qxo_ekinetic(const double *dkgnrm, const double *cgjr, double *rhcr, int nplwex, int nb,
+                  const double *occ, double *ekib, double *ekint);
+
+/* ---- Kleinman-Bylander nonlocal term ------------------------------------ */
+void qxo_hckbpp_g(const double *eqlmr, int nplwex, int lmmax, int natom, const int *lmx_atom,
+                  const double *vclalp_atom, const double *cgjr, double *rhcr, int nb,
+                  double *slmir);
+double qxo_enkbpp_g(const double *slmir, int lmmax, int natom, const int *lmx_atom,
+                    const double *clalp_atom, const double *occ, int nb, double rvol);
+
+/* ---- dense band algebra -------------------------------------------------- */
+void qxo_rsdecmp(double *sr, int n, int lda);
+void qxo_rsinvcd(double *sr, int n, int lda, double *work);
+int  qxo_schmidt(double *cgjr, int nplwex, int nband, double *smat /* nband*nband */);
+void qxo_hmatrix(const double *cgjr, const double *rhcr, int nplwex, int nband,
+                 double *dmtrxr, double *eig);
+int  qxo_eigh(double *a, int n, int lda, double *w);
+void qxo_untrycore(double *gdcr, int nplwex, int nband, const double *u, int ldu, double *work);
+void qxo_softchg(const double *cgjr, int nplwex, int nplw, int nband, const double *occ,
+                 double rvol, const int *ijkgd_c, int n1, int n2, int n3, double *rho);
+void qxo_prpcg(double ekib, const double *dkgnrm, int nplwex, double *prcd);
+
+/* ---- BKL conjugate-gradient band solver ---------------------------------- */
+typedef struct {
+  int nplw, nplwex, n1, n2, n3;
+  const int *ijkgd_c;        /* centered map, valid for -nplw..nplw        */
+  const double *glocal;      /* local potential on the FFT grid (Ry)       */
+  const double *dkgnrm;      /* |G|^2 duplicated for (Re, Im)              */
+  const double *eqlmr;       /* KB projectors (nplwex, lmmax, natom) or 0  */
+  int lmmax, natom;
+  const int *lmx_atom;
+  const double *vclalp_atom; /* (lmmax, natom): clalp/Omega              */
+} qxo_ham;
+
+void qxo_hpsi(const qxo_ham *h, const double *psi, double *hpsi, int nb, double *slmir);
+void qxo_eigen_bkl(const qxo_ham *h, double *eig, double *gdcr, double *rhcr,
+                   const double *ekib, int nband, int noband, int itermx, int iteremx,
+                   double tolera, double threinn, double wdinner, double roughfzan,
+                   double roughfeig, double *bzan, int *nriter);
+
+/* One SCF iteration of the band part with a frozen local potential:
+ * energy pass, subspace rotation, BKL CG, Gram-Schmidt, density. */
+int qxo_scf_band_step(const qxo_ham *h, double *gdcr, double *rhcr, double *eig,
+                      const double *occ, int nband, int noband, double rvol,
+                      double *rho, double *ekin, double *enl, const double *clalp_atom);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

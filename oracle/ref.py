@@ -1,0 +1,182 @@
+"""ctypes access to oracle/_ref/libqxref.so: the reference's own FFT-free
+routines, compiled from /root/reference by oracle/ref.mk (test infrastructure
+only; used to pin the C oracle).  ``available()`` is False where the library
+was not built (e.g. no flang); tests that need it skip then.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(_HERE, "_ref", "libqxref.so")
+_LIB = None
+
+
+def available() -> bool:
+    return os.path.exists(SO)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        # RTLD_LAZY: the two FFT symbols the reference takes from FFTW-backed
+        # Sources/fft3.f90 stay unresolved and are never called from here.
+        _LIB = C.CDLL(SO, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+    return _LIB
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="F_CONTIGUOUS")
+
+
+def _F(a, dtype=np.float64):
+    return np.asfortranarray(a, dtype=dtype)
+
+
+def get_mesh(hcell, ecutdens):
+    nd = np.zeros(3, np.int32)
+    L = lib()
+    L.qxref_get_mesh.argtypes = [_dp, C.c_double, _ip]
+    L.qxref_get_mesh(_F(hcell), float(ecutdens), nd)
+    return tuple(int(x) for x in nd)
+
+
+def gsphere(rba, ecut, ecutdens, nd, cap):
+    L = lib()
+    L.qxref_gsphere.argtypes = [_dp, C.c_double, C.c_double, C.c_int, _ip, _ip, _ip, _ip, _dp,
+                                _ip, _ip]
+    counts = np.zeros(6, np.int32)
+    nga, ngb, ngc = (np.zeros(cap, np.int32) for _ in range(3))
+    rec = np.zeros(cap)
+    ijk = np.zeros(2 * cap + 1, np.int32)
+    L.qxref_gsphere(_F(rba), float(ecut), float(ecutdens), int(cap), counts, nga, ngb, ngc, rec,
+                    _F(np.asarray(nd), np.int32), ijk)
+    n5 = int(counts[1])
+    return dict(kfft0=int(counts[0]), nplw5=n5, nplw=int(counts[2]), nplw2=int(counts[3]),
+                nplwc=int(counts[4]), nplw7=int(counts[5]), nga=nga[: n5 + 1],
+                ngb=ngb[: n5 + 1], ngc=ngc[: n5 + 1], recnrm=rec[: n5 + 1],
+                ijkgd=ijk[: 2 * n5 + 1])
+
+
+def rnd00(seed, n):
+    L = lib()
+    L.qxref_rnd00.argtypes = [C.POINTER(C.c_double), C.c_int, _dp]
+    s = C.c_double(float(seed))
+    out = np.zeros(n)
+    L.qxref_rnd00(C.byref(s), int(n), out)
+    return out, s.value
+
+
+def rseigqr(a):
+    """Returns (w, vectors as columns)."""
+    L = lib()
+    n = a.shape[0]
+    L.qxref_rseigqr.argtypes = [_dp, C.c_int, _dp]
+    m = _F(a).copy(order="F")
+    w = np.zeros(n)
+    L.qxref_rseigqr(m, n, w)
+    return w, np.asarray(m)
+
+
+def icholesky(s):
+    L = lib()
+    n = s.shape[0]
+    L.qxref_icholesky.argtypes = [_dp, C.c_int]
+    m = _F(s).copy(order="F")
+    L.qxref_icholesky(m, n)
+    return np.asarray(m)
+
+
+def schmidt2(cgjr):
+    """cgjr: (nb, nplwex) band rows.  Returns (orthonormalised copy, lcholesky)."""
+    L = lib()
+    c = _F(np.asarray(cgjr).T).copy(order="F")
+    n, nb = c.shape
+    ok = C.c_int(0)
+    L.qxref_schmidt2.argtypes = [_dp, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    L.qxref_schmidt2(c, n, nb, C.byref(ok))
+    return np.ascontiguousarray(c.T), bool(ok.value)
+
+
+def hmatrix2(cgjr, rhcr):
+    L = lib()
+    c = _F(np.asarray(cgjr).T)
+    r = _F(np.asarray(rhcr).T)
+    n, nb = c.shape
+    h = np.zeros((nb, nb), order="F")
+    eig = np.zeros(nb)
+    L.qxref_hmatrix2.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp, _dp]
+    L.qxref_hmatrix2(c, r, n, nb, h, eig)
+    return np.asarray(h), eig
+
+
+def untrycore(gdcr, u):
+    L = lib()
+    g = _F(np.asarray(gdcr).T).copy(order="F")
+    n, nb = g.shape
+    L.qxref_untrycore.argtypes = [_dp, C.c_int, C.c_int, _dp]
+    L.qxref_untrycore(g, n, nb, _F(u))
+    return np.ascontiguousarray(g.T)
+
+
+def prpcg(ekib, dkg):
+    L = lib()
+    out = np.zeros(len(dkg))
+    L.qxref_prpcg.argtypes = [C.c_double, _dp, C.c_int, _dp]
+    L.qxref_prpcg(float(ekib), _F(dkg), len(dkg), out)
+    return out
+
+
+def ekinetic2(cgjr, occ, dkg):
+    L = lib()
+    c = _F(np.asarray(cgjr).T)
+    n, nb = c.shape
+    r = np.zeros((n, nb), order="F")
+    ekib = np.zeros(nb)
+    ekint = C.c_double(0.0)
+    L.qxref_ekinetic2.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp,
+                                  C.POINTER(C.c_double)]
+    L.qxref_ekinetic2(c, r, n, nb, _F(occ), _F(dkg), ekib, C.byref(ekint))
+    return np.ascontiguousarray(r.T), ekib, ekint.value
+
+
+def kb_setup(eqlmr, lmx_atom, vclalp_atom, clalp_atom, nbmax):
+    """eqlmr: (natom, lmmax, nplwex) (C order) -> Fortran (nplwex, lmmax, natom)."""
+    L = lib()
+    e = np.asarray(eqlmr)
+    natom, lmmax, n = e.shape
+    eF = _F(np.transpose(e, (2, 1, 0)))
+    L.qxref_kb_setup.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _ip, _dp, _dp, C.c_int]
+    L.qxref_kb_setup(eF, n, lmmax, natom, _F(lmx_atom, np.int32), _F(np.asarray(vclalp_atom).T),
+                     _F(np.asarray(clalp_atom).T), int(nbmax))
+    return natom, lmmax
+
+
+def hckbpp_g(cgjr, rhcr, natom, lmmax):
+    L = lib()
+    c = _F(np.asarray(cgjr).T)
+    r = _F(np.asarray(rhcr).T).copy(order="F")
+    n, nb = c.shape
+    sl = np.zeros((lmmax, natom, nb), order="F")
+    L.qxref_hckbpp_g.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int, _dp]
+    L.qxref_hckbpp_g(c, r, n, nb, natom, sl)
+    return np.ascontiguousarray(r.T), np.ascontiguousarray(np.transpose(sl, (2, 1, 0)))
+
+
+def hckbpp_ib_g(cgjr1, rhcr1, natom):
+    L = lib()
+    c = _F(cgjr1)
+    r = _F(rhcr1).copy()
+    L.qxref_hckbpp_ib_g.argtypes = [_dp, _dp, C.c_int, C.c_int]
+    L.qxref_hckbpp_ib_g(c, r, len(c), natom)
+    return r
+
+
+def enkbpp_g2(nband, natom, occ, rvol):
+    L = lib()
+    L.qxref_enkbpp_g2.argtypes = [C.c_int, C.c_int, _dp, C.c_double]
+    L.qxref_enkbpp_g2.restype = C.c_double
+    return L.qxref_enkbpp_g2(int(nband), int(natom), _F(occ), float(rvol))

@@ -1,0 +1,335 @@
+!-----------------------------------------------------------------------
+! ref_harness.f90 -- C-callable wrappers around the reference QXMD
+! routines that need no FFT, so the CPU oracle can be pinned against the
+! reference's own code (test infrastructure only).
+!
+! Built by oracle/ref.mk together with the reference sources compiled
+! where they lie; nothing from the reference is copied.  Every wrapper
+! sets the reference's module state the way the single-rank serial build
+! (no_comm) would, then calls the reference subroutine unchanged.
+!-----------------------------------------------------------------------
+
+subroutine qxref_get_mesh(hcell, ecutdens, nd) bind(C, name="qxref_get_mesh")
+  use iso_c_binding
+  implicit none
+  real(c_double), intent(in) :: hcell(3,3)
+  real(c_double), value :: ecutdens
+  integer(c_int) :: nd(3)
+  integer :: nfile(2), nd1v(3), nd1vks(3)
+  real*8 :: h(3,3), ec
+  nfile = 6
+  h = hcell
+  ec = ecutdens
+  call get_mesh(nfile, 0, 1, ec, h, nd1v, nd1vks)
+  nd = nd1vks
+end subroutine
+
+! strvec + set_rciprl + reclat: G list in sorted order and the ijkgd map.
+subroutine qxref_gsphere(rba_in, ecut_in, ecutdens_in, cap, counts, nga_o, ngb_o, ngc_o, &
+&                        recnrm_o, nd, ijkgd_o) bind(C, name="qxref_gsphere")
+  use iso_c_binding
+  use pwlda_pw, only: nga, ngb, ngc, recnrm
+  use reclat_variables, only: rv_lnc => lnoncollinear
+  implicit none
+  real(c_double), intent(in) :: rba_in(3,3)
+  real(c_double), value :: ecut_in, ecutdens_in
+  integer(c_int), value :: cap
+  integer(c_int) :: counts(6), nga_o(0:cap-1), ngb_o(0:cap-1), ngc_o(0:cap-1), nd(3)
+  real(c_double) :: recnrm_o(0:cap-1)
+  integer(c_int) :: ijkgd_o(*)
+  integer :: nfile(2), nplw5, nplw, nplw2, nplwc, nplw7, nplwcs, nplw3
+  integer :: nplw5ex, nplwex, nplw2ex, nplw3ex, nplwcex, nplw7ex, kfft0d
+  integer :: kfft1, kfft2, kfft3, kfft0, kfft1b, kfft2b, kfft3b, kfft0b
+  integer :: kmax1, kmax2, kmax3, kmax1d, kmax2d, kmax3d, kmax1b, kmax2b, kmax3b
+  integer :: kmax1cs, kmax2cs, kmax3cs, lplh, i
+  real*8 :: rba(3,3), rvol, cross(3,3), rk1d(3), pwscale, ecut, ecutdens, ecutsoft, ecutc, ecutorth
+  integer, allocatable :: ijkgd(:), ijkg(:)
+  real*8 :: gbox(0:0), abg(0:0)
+  integer :: ngbox(0:0), ijkgb(0:0)
+  nfile = 6
+  rba = rba_in
+  ecut = ecut_in
+  ecutdens = ecutdens_in
+  ecutsoft = ecutdens
+  ecutc = ecut
+  ecutorth = ecut
+  pwscale = 1.2d0
+  rv_lnc = .false.
+  nplw5 = 0
+  call strvec(nfile, 0, 1, nplw5, nplw, nplw2, nplwc, nplw7, nplwcs, &
+&             ecutdens, ecut, ecutsoft, ecutc, ecutorth, rba, rvol, cross, rk1d, &
+&             .true., pwscale, .false., .false.)
+  counts(1) = 0
+  counts(2) = nplw5
+  counts(3) = nplw
+  counts(4) = nplw2
+  counts(5) = nplwc
+  counts(6) = nplw7
+  if (nplw5 + 1 > cap) then
+    counts(1) = -1
+    return
+  end if
+  do i = 0, nplw5
+    nga_o(i) = nga(i)
+    ngb_o(i) = ngb(i)
+    ngc_o(i) = ngc(i)
+    recnrm_o(i) = recnrm(i)
+  end do
+  nplw3 = 0
+  nplw5ex = 2*(nplw5+1); nplwex = 2*(nplw+1); nplw2ex = 2*(nplw2+1)
+  nplw3ex = 2; nplwcex = 2*(nplwc+1); nplw7ex = 2*(nplw7+1)
+  kfft0d = nd(1)*nd(2)*nd(3)
+  lplh = nplw
+  allocate(ijkgd(-nplw5:nplw5), ijkg(-nplw2:nplw2))
+  kfft1b = 0; kfft2b = 0; kfft3b = 0; kfft0b = 0
+  call reclat(nfile, 0, 1, nd, rba, ecut, ecutdens, ecutsoft, ecutorth, &
+&     nplw5, nplw, nplw2, nplw3, nplwc, nplw7, nplwcs, &
+&     nplw5ex, nplwex, nplw2ex, nplw3ex, nplwcex, nplw7ex, kfft0d, &
+&     nga, ngb, ngc, recnrm, ijkgd, nplw5, lplh, &
+&     kfft1, kfft2, kfft3, kfft0, ijkg, &
+&     kfft1b, kfft2b, kfft3b, kfft0b, &
+&     gbox, gbox, gbox, ngbox, ngbox, ngbox, abg, ijkgb, &
+&     kmax1, kmax2, kmax3, kmax1d, kmax2d, kmax3d, &
+&     kmax1b, kmax2b, kmax3b, kmax1cs, kmax2cs, kmax3cs)
+  do i = -nplw5, nplw5
+    ijkgd_o(i + nplw5 + 1) = ijkgd(i)
+  end do
+  counts(1) = kfft0
+  deallocate(ijkgd, ijkg)
+end subroutine
+
+subroutine qxref_rnd00(seed, n, out) bind(C, name="qxref_rnd00")
+  use iso_c_binding
+  implicit none
+  real(c_double) :: seed
+  integer(c_int), value :: n
+  real(c_double) :: out(n)
+  integer :: i
+  real*8 :: rn, s
+  s = seed
+  do i = 1, n
+    call rnd00(rn, s)
+    out(i) = rn
+  end do
+  seed = s
+end subroutine
+
+subroutine qxref_rseigqr(a, n, w) bind(C, name="qxref_rseigqr")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: n
+  real(c_double) :: a(n,n), w(n)
+  real*8, allocatable :: w1r(:), w1i(:), work(:)
+  character*5 :: check
+  allocate(w1r(n), w1i(n), work(n))
+  check = 'OK   '
+  call RsEIGQR(a, w, n, n, 1.0d-15, 1, 2, check, w1r, w1i, work)
+  deallocate(w1r, w1i, work)
+end subroutine
+
+subroutine qxref_icholesky(s, n) bind(C, name="qxref_icholesky")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: n
+  real(c_double) :: s(n,n)
+  real*8, allocatable :: work(:)
+  allocate(work(n))
+  call rsdecmp(s, n, n)
+  call rsinvcd(s, work, n, n)
+  deallocate(work)
+end subroutine
+
+subroutine qxref_schmidt2(cgjr, nplwex, nband, ok) bind(C, name="qxref_schmidt2")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: nplwex, nband
+  real(c_double) :: cgjr(nplwex, nband)
+  integer(c_int) :: ok
+  integer :: nfile(2), npnod, i, nbncnt(1), nbndsp(1), lcnt(1), ldsp(1), mcnt(1), mdsp(1), jd(1)
+  integer, allocatable :: iod(:)
+  real*8, allocatable :: prod(:), prod2(:), dbuf(:), dmtrxr(:,:)
+  real*8 :: ct0, bijr(1), bm1r(1), rvol
+  integer :: nhk1(1), nhk2(1)
+  logical :: lvandi(1), lc
+  nfile = 6
+  npnod = nplwex/2
+  allocate(iod(nband), prod(nband), prod2(nband), dbuf(nplwex), dmtrxr(nband, nband))
+  do i = 1, nband
+    iod(i) = i
+  end do
+  nbncnt(1) = nband; nbndsp(1) = 0
+  ct0 = 0.d0; rvol = 1.d0; nhk1 = 1; nhk2 = 1; lvandi = .false.
+  call setlstop(nfile, 0, 1, .false.)
+  call schmidt2(nfile, 0, 1, ct0, .false., cgjr, 1, npnod, npnod, nband, 1, nband, nband, &
+&     nbncnt, nbndsp, iod, iod, prod, prod2, dbuf, .false., .false., .false., .false., &
+&     dmtrxr, nband, 1, lcnt, ldsp, mcnt, mdsp, jd, bijr, bm1r, rvol, 1, nhk1, nhk2, 1, lvandi)
+  call getlcholesky(nfile, 0, 1, lc)
+  ok = 0
+  if (lc) ok = 1
+  deallocate(iod, prod, prod2, dbuf, dmtrxr)
+end subroutine
+
+subroutine qxref_hmatrix2(cgjr, rhcr, nplwex, nband, dmtrxr, eig) bind(C, name="qxref_hmatrix2")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: nplwex, nband
+  real(c_double) :: cgjr(nplwex, nband), rhcr(nplwex, nband), dmtrxr(nband, nband), eig(nband)
+  integer :: nfile(2), nbncnt(1), nbndsp(1), npnod
+  integer, allocatable :: iod(:)
+  real*8, allocatable :: prod(:), prodr(:), bx0r(:)
+  nfile = 6
+  npnod = nplwex/2
+  allocate(iod(nband), prod(nband), prodr(nband), bx0r(nband))
+  nbncnt(1) = nband; nbndsp(1) = 0
+  call hmatrix2(nfile, 0, 1, eig, cgjr, rhcr, 1, npnod, npnod, nband, 1, nband, nband, &
+&     nbncnt, nbndsp, iod, bx0r, dmtrxr, nband, prod, prodr)
+  deallocate(iod, prod, prodr, bx0r)
+end subroutine
+
+subroutine qxref_untrycore(gdcr, nplwex, nband, u) bind(C, name="qxref_untrycore")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: nplwex, nband
+  real(c_double) :: gdcr(nplwex, nband), u(nband, nband)
+  integer :: nfile(2), nbncnt(1), nbndsp(1), npnod, iblock
+  real*8, allocatable :: prod(:), prodr(:), w1i(:), tmpr(:,:)
+  nfile = 6
+  npnod = nplwex/2
+  iblock = int(1.d+07/8/nband)
+  if (iblock < 100) iblock = 100
+  allocate(prod(nband), prodr(nband), w1i(nband), tmpr(iblock, nband))
+  nbncnt(1) = nband; nbndsp(1) = 0
+  call untrycore(nfile, 0, 1, gdcr, 1, npnod, npnod, nband, 1, nband, nband, nbncnt, nbndsp, &
+&     prod, prodr, w1i, tmpr, iblock, u, nband)
+  deallocate(prod, prodr, w1i, tmpr)
+end subroutine
+
+subroutine qxref_prpcg(ekib, dkgnrm, nplwex, prcd) bind(C, name="qxref_prpcg")
+  use iso_c_binding
+  implicit none
+  real(c_double), value :: ekib
+  integer(c_int), value :: nplwex
+  real(c_double) :: dkgnrm(nplwex), prcd(nplwex)
+  integer :: nfile(2)
+  real*8 :: ek(1)
+  nfile = 6
+  ek(1) = ekib
+  call prpcg(nfile, 0, 1, 1, prcd, ek, dkgnrm, nplwex)
+end subroutine
+
+subroutine qxref_ekinetic2(cgjr, rhcr, nplwex, nband, occ, dkgnrm, ekib, ekint) &
+&   bind(C, name="qxref_ekinetic2")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: nplwex, nband
+  real(c_double) :: cgjr(nplwex, nband), rhcr(nplwex, nband), occ(nband), dkgnrm(nplwex)
+  real(c_double) :: ekib(nband), ekint
+  integer :: nfile(2), i
+  integer, allocatable :: iod(:)
+  real*8 :: ek
+  nfile = 6
+  allocate(iod(nband))
+  do i = 1, nband
+    iod(i) = i
+  end do
+  ek = 0.d0
+  call ekinetic2(nfile, 0, 1, ek, ekib, occ, cgjr, rhcr, nplwex, nplwex/2 - 1, &
+&     nband, 1, nband, nband, iod, dkgnrm, 1)
+  ekint = ek
+  deallocate(iod)
+end subroutine
+
+! KB state: one species per atom so that lmx and vclalp may vary per atom.
+subroutine qxref_kb_setup(eqlmr, nplwex, lmmax_in, natom, lmx_atom, vclalp_atom, clalp_atom, &
+&   nbmax) bind(C, name="qxref_kb_setup")
+  use iso_c_binding
+  use nlpp_variables, only: lmx, clalp, lmmax
+  use nl_variables, only: eqlmr_, vclalp, slmir, zzzr, zzzi, lbigmm2
+  implicit none
+  integer(c_int), value :: nplwex, lmmax_in, natom, nbmax
+  real(c_double) :: eqlmr(nplwex, lmmax_in, natom)
+  integer(c_int) :: lmx_atom(natom)
+  real(c_double) :: vclalp_atom(lmmax_in, natom), clalp_atom(lmmax_in, natom)
+  if (allocated(eqlmr_)) deallocate(eqlmr_)
+  if (allocated(vclalp)) deallocate(vclalp)
+  if (allocated(clalp)) deallocate(clalp)
+  if (allocated(lmx)) deallocate(lmx)
+  if (allocated(slmir)) deallocate(slmir)
+  if (allocated(zzzr)) deallocate(zzzr)
+  if (allocated(zzzi)) deallocate(zzzi)
+  lmmax = lmmax_in
+  allocate(eqlmr_(nplwex, lmmax_in, natom), vclalp(lmmax_in, natom), clalp(lmmax_in, natom))
+  allocate(lmx(natom), slmir(lmmax_in, natom, nbmax), zzzr(0:nplwex/2), zzzi(0:nplwex/2))
+  eqlmr_ = eqlmr
+  vclalp = vclalp_atom
+  clalp = clalp_atom
+  lmx = lmx_atom
+  slmir = 0.d0
+  lbigmm2 = .true.
+end subroutine
+
+subroutine qxref_hckbpp_g(cgjr, rhcr, nplwex, nband, natom, slmir_o) bind(C, name="qxref_hckbpp_g")
+  use iso_c_binding
+  use nlpp_variables, only: lmmax
+  use nl_variables, only: slmir
+  implicit none
+  integer(c_int), value :: nplwex, nband, natom
+  real(c_double) :: cgjr(nplwex, nband), rhcr(nplwex, nband)
+  real(c_double) :: slmir_o(lmmax, natom, nband)
+  integer :: nfile(2), nhk(natom), i
+  logical :: lkb(natom), lkg(natom)
+  real*8 :: yc(1), ys(1)
+  nfile = 6
+  do i = 1, natom
+    nhk(i) = i
+  end do
+  lkb = .true.
+  lkg = .false.
+  call hckbpp_g(nfile, 0, 1, cgjr, rhcr, nplwex, nplwex/2 - 1, nband, 1, nband, nband, &
+&     natom, nhk, nhk, natom, lkb, lkg, yc, ys, 0)
+  slmir_o = slmir(:, :, 1:nband)
+end subroutine
+
+subroutine qxref_hckbpp_ib_g(cgjr, rhcr, nplwex, natom) bind(C, name="qxref_hckbpp_ib_g")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: nplwex, natom
+  real(c_double) :: cgjr(nplwex), rhcr(nplwex)
+  integer :: nfile(2), nhk(natom), i
+  logical :: lkb(natom), lkg(natom)
+  real*8 :: yc(1), ys(1)
+  nfile = 6
+  do i = 1, natom
+    nhk(i) = i
+  end do
+  lkb = .true.
+  lkg = .false.
+  call hckbpp_ib_g(nfile, 0, 1, cgjr, rhcr, nplwex, nplwex/2 - 1, &
+&     natom, nhk, nhk, natom, lkb, lkg, yc, ys, 0)
+end subroutine
+
+function qxref_enkbpp_g2(nband, natom, occ, rvol) bind(C, name="qxref_enkbpp_g2")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: nband, natom
+  real(c_double) :: occ(nband)
+  real(c_double), value :: rvol
+  real(c_double) :: qxref_enkbpp_g2
+  integer :: nfile(2), nhk(natom), i, iod(nband)
+  logical :: lkb(natom), lkg(natom)
+  real*8 :: e, rv
+  nfile = 6
+  do i = 1, natom
+    nhk(i) = i
+  end do
+  do i = 1, nband
+    iod(i) = i
+  end do
+  lkb = .true.
+  lkg = .false.
+  e = 0.d0
+  rv = rvol
+  call enkbpp_g2(nfile, 0, 1, e, nband, 1, nband, nband, occ, iod, natom, nhk, nhk, natom, &
+&     lkb, lkg, rv)
+  qxref_enkbpp_g2 = e
+end function

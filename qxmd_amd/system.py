@@ -1,0 +1,169 @@
+"""Synthetic diamond-Si cells and the boundary inputs of the H|psi> path.
+
+Host-side setup only (numpy): it builds what QXMD's Fortran modules own before
+the hot path runs -- the cell, the seeded wavefunctions, a local potential on
+the FFT grid and Kleinman-Bylander projector tables in the reference's
+real-packed layout.  The G-sphere itself comes from ``qxmd_amd.gsphere``.
+
+Random numbers follow the reference's own generators: the Lehmer ``rnd00``
+(Sources/funcs.f90:1673-1689) and the polar Box-Muller ``rndgaus``
+(Sources/funcs.f90:1586-1619), each with an explicit seed.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+A0_SI = 10.26  # conventional diamond-Si lattice constant, bohr
+DIAMOND_FRAC = np.array([
+    [0.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0],
+    [0.25, 0.25, 0.25], [0.25, 0.75, 0.75], [0.75, 0.25, 0.75], [0.75, 0.75, 0.25]])
+
+
+class Lehmer:
+    """rnd00: x <- 16807 x mod (2^31-1) evaluated in FP64, u = x / (2^31-1)."""
+
+    def __init__(self, seed: float):
+        self.x = float(seed)
+
+    def uniform(self, n: int) -> np.ndarray:
+        out = np.empty(n)
+        x = self.x
+        for i in range(n):
+            x = math.fmod(16807.0 * x, 2147483647.0)
+            out[i] = x / 2147483647.0
+        self.x = x
+        return out
+
+    def gauss(self, n: int) -> np.ndarray:
+        """rndgaus pairs: (v2*fac, v1*fac) per accepted (v1, v2)."""
+        out = []
+        while len(out) < n:
+            while True:
+                v1 = 2.0 * self.uniform(1)[0] - 1.0
+                v2 = 2.0 * self.uniform(1)[0] - 1.0
+                rsq = v1 * v1 + v2 * v2
+                if rsq < 1.0:
+                    break
+            fac = math.sqrt(-2.0 * math.log(rsq) / rsq)
+            out.append(v2 * fac)
+            out.append(v1 * fac)
+        return np.array(out[:n])
+
+
+@dataclass
+class SiCell:
+    ncell: int
+    hcell: np.ndarray          # (3,3), columns are the lattice vectors (bohr)
+    tau: np.ndarray            # (natom, 3) Cartesian positions (bohr)
+    natom: int = field(init=False)
+
+    def __post_init__(self):
+        self.natom = self.tau.shape[0]
+
+    @property
+    def volume(self) -> float:
+        return float(abs(np.linalg.det(self.hcell)))
+
+
+def diamond_si(ncell: int, seed: float = 20261202.0, sigma: float = 0.02) -> SiCell:
+    """ncell^3 conventional diamond cells, a = 10.26*ncell bohr, atoms displaced
+    by seeded Gaussian noise of width sigma (bohr)."""
+    a = A0_SI * ncell
+    frac = []
+    for i in range(ncell):
+        for j in range(ncell):
+            for k in range(ncell):
+                frac.append((DIAMOND_FRAC + np.array([i, j, k])) / ncell)
+    frac = np.concatenate(frac, axis=0)
+    tau = frac * a
+    if sigma > 0:
+        tau = tau + sigma * Lehmer(seed).gauss(tau.size).reshape(tau.shape)
+    return SiCell(ncell=ncell, hcell=np.eye(3) * a, tau=tau)
+
+
+def initial_wavefunctions(gs, nband: int, seed: float = 25.0) -> np.ndarray:
+    """c(G) = (u1 + i u2) / (1 + |G|^2), u uniform in (-0.5, 0.5) from rnd00,
+    Im c(0) = 0, in the reference G order; returns (nband, nplwex) real-packed."""
+    nplw = gs.nplw
+    rng = Lehmer(seed)
+    u = rng.uniform(nband * 2 * (nplw + 1)).reshape(nband, nplw + 1, 2) - 0.5
+    damp = 1.0 / (1.0 + gs.recnrm[: nplw + 1])
+    psi = (u * damp[None, :, None]).reshape(nband, 2 * (nplw + 1))
+    psi[:, 1] = 0.0
+    return np.ascontiguousarray(psi)
+
+
+def structure_factors(gs, tau: np.ndarray, nplw: int):
+    """ycos/ysin(ig, ia) = cos/sin(G_ig . tau_ia) for ig = 0..nplw
+    (the quantities stscfn_pw builds, Sources/pwlda.f90:5711-5854)."""
+    g = np.stack([gs.gx[: nplw + 1], gs.gy[: nplw + 1], gs.gz[: nplw + 1]], axis=1)
+    ph = g @ tau.T  # (nplw+1, natom)
+    return np.cos(ph), np.sin(ph)
+
+
+def model_local_potential(gs, cell: SiCell, depth: float = 1.2, width: float = 1.1,
+                          shift: float = 0.4) -> np.ndarray:
+    """A smooth local potential on the FFT grid (Ry): Gaussian wells of the given
+    depth and width at every atom plus a constant.  Stands in for
+    V_H + V_xc + V_loc,ps (glocal), which the SCF glue assembles."""
+    n1, n2, n3 = gs.nd
+    grid = np.stack(np.meshgrid(np.arange(n1) / n1, np.arange(n2) / n2, np.arange(n3) / n3,
+                                indexing="ij"), axis=-1)  # (n1,n2,n3,3) fractional
+    r = grid @ cell.hcell.T  # Cartesian
+    v = np.full((n1, n2, n3), shift)
+    L = np.diag(cell.hcell)
+    for t in cell.tau:
+        d = r - t
+        d -= np.round(d / L) * L  # minimum image (orthorhombic cells)
+        v -= depth * np.exp(-np.sum(d * d, axis=-1) / (2.0 * width * width))
+    # x fastest: ijk = i + n1*(j + n2*k)
+    return np.ascontiguousarray(np.transpose(v, (2, 1, 0)).ravel())
+
+
+def model_kb_projectors(gs, cell: SiCell, rc: float = 1.6, lmmax: int = 4,
+                        clalp_sp=(2.6, 1.3)):
+    """s + p Kleinman-Bylander projectors in the real-packed layout that setbgq
+    writes (Sources/nlkbpp.f90:2135-2187):
+        even l: eqlmr_(2ig+1) =  cos(G.tau) f(G),  eqlmr_(2ig+2) = -sin(G.tau) f(G)
+        odd  l: eqlmr_(2ig+1) = -sin(G.tau) f(G),  eqlmr_(2ig+2) = -cos(G.tau) f(G)
+    with f = beta_l(|G|) Y_lm(G^) from Gaussian model radial functions.
+    Returns eqlmr (natom, lmmax, nplwex), lmx per atom, vclalp and clalp
+    (natom, lmmax) with vclalp = clalp / Omega (Sources/nlkbpp.f90:1262)."""
+    nplw = gs.nplw
+    gx, gy, gz = gs.gx[: nplw + 1], gs.gy[: nplw + 1], gs.gz[: nplw + 1]
+    q = np.sqrt(gs.recnrm[: nplw + 1])
+    qs = np.where(q > 0, q, 1.0)
+    omega = cell.volume
+    norm = 1.0 / math.sqrt(omega)
+    bs = norm * 4.0 * math.pi * rc ** 1.5 * np.exp(-0.5 * (q * rc) ** 2)
+    bp = norm * 4.0 * math.pi * rc ** 2.5 * q * np.exp(-0.5 * (q * rc) ** 2)
+    y00 = 1.0 / math.sqrt(4.0 * math.pi)
+    c1 = math.sqrt(3.0 / (4.0 * math.pi))
+    f = [bs * y00, bp * c1 * gx / qs, bp * c1 * gy / qs, bp * c1 * gz / qs]
+    ycos, ysin = structure_factors(gs, cell.tau, nplw)
+    nat = cell.natom
+    eq = np.zeros((nat, lmmax, 2 * (nplw + 1)))
+    for l in range(4):
+        fl = f[l]
+        if l == 0:  # even l
+            eq[:, l, 0::2] = (ycos * fl[:, None]).T
+            eq[:, l, 1::2] = (-ysin * fl[:, None]).T
+        else:
+            eq[:, l, 0::2] = (-ysin * fl[:, None]).T
+            eq[:, l, 1::2] = (-ycos * fl[:, None]).T
+    lmx = np.full(nat, 4, np.int32)
+    clalp = np.zeros((nat, lmmax))
+    clalp[:, 0] = clalp_sp[0]
+    clalp[:, 1:4] = clalp_sp[1]
+    return np.ascontiguousarray(eq), lmx, clalp / omega, clalp
+
+
+def occupations(nband: int, nel: int) -> np.ndarray:
+    """lfermi=1 insulator filling: 2 electrons in each of the lowest nel/2 bands
+    (Sources/fermi.f90:135-137)."""
+    occ = np.zeros(nband)
+    occ[: nel // 2] = 2.0
+    return occ

@@ -1,0 +1,134 @@
+"""Pin the C oracle against the reference's own routines (oracle/_ref, built
+with flang from /root/reference by oracle/ref.mk).  Skipped where that library
+was not built (the GPU box has no reference tree)."""
+import numpy as np
+import pytest
+
+import oracle as O
+import oracle.ref as R
+from qxmd_amd.system import (diamond_si, initial_wavefunctions, model_kb_projectors,
+                             occupations)
+
+pytestmark = pytest.mark.skipif(not R.available(), reason="oracle/_ref not built")
+
+
+@pytest.mark.parametrize("ncell", [1, 2, 3])
+def test_gsphere_bitexact(ncell):
+    cell = diamond_si(ncell)
+    nd = R.get_mesh(cell.hcell, 80.0)
+    assert nd == O.get_mesh(cell.hcell, 80.0)
+    assert nd == (30 * ncell,) * 3
+    s = O.strvec(cell.hcell, 80.0, 20.0)
+    r = R.gsphere(cell.hcell, 20.0, 80.0, nd, s["num"] + 1)
+    n5 = r["nplw5"]
+    for k in ("nplw5", "nplw", "nplw2", "nplwc", "nplw7"):
+        assert s[k] == r[k], k
+    for k in ("nga", "ngb", "ngc"):
+        assert np.array_equal(s[k][: n5 + 1], r[k]), k
+    assert np.array_equal(s["recnrm"][: n5 + 1], r["recnrm"])  # bit-exact FP64
+    ij = O.ijkgd(s["nga"], s["ngb"], s["ngc"], n5, nd)
+    assert np.array_equal(ij, r["ijkgd"])
+
+
+def test_gsphere_noncubic_cell():
+    # an orthorhombic, slightly sheared cell exercises the per-axis kmax and
+    # the table rounding of get_mesh for unequal dimensions
+    h = np.array([[10.26, 0.3, 0.0], [0.0, 13.1, 0.2], [0.0, 0.0, 8.7]])
+    nd = R.get_mesh(h, 80.0)
+    assert nd == O.get_mesh(h, 80.0)
+    s = O.strvec(h, 80.0, 20.0)
+    r = R.gsphere(h, 20.0, 80.0, nd, s["num"] + 1)
+    n5 = r["nplw5"]
+    assert s["nplw"] == r["nplw"] and s["nplw5"] == n5
+    for k in ("nga", "ngb", "ngc"):
+        assert np.array_equal(s[k][: n5 + 1], r[k]), k
+    assert np.array_equal(O.ijkgd(s["nga"], s["ngb"], s["ngc"], n5, nd), r["ijkgd"])
+
+
+def test_rnd00():
+    a, sa = R.rnd00(25.0, 1000)
+    b, sb = O.rnd00_sequence(25.0, 1000)
+    assert np.array_equal(a, b) and sa == sb
+
+
+def _psi(sph, nb, seed=25.0):
+    return initial_wavefunctions(sph, nb, seed)
+
+
+def test_ekinetic(si8):
+    psi = _psi(si8, 6)
+    occ = occupations(6, 8)
+    r1, e1, t1 = O.ekinetic(si8.dkg, psi, occ)
+    r2, e2, t2 = R.ekinetic2(psi, occ, si8.dkg)
+    assert np.array_equal(r1, r2)
+    np.testing.assert_allclose(e1, e2, rtol=1e-14)
+    assert abs(t1 - t2) <= 1e-14 * abs(t2)
+
+
+def test_prpcg(si8):
+    for ek in (0.7, 3.1):
+        np.testing.assert_allclose(O.prpcg(ek, si8.dkg), R.prpcg(ek, si8.dkg), rtol=1e-15)
+
+
+def test_hckbpp(si8):
+    nb = 5
+    psi = _psi(si8, nb)
+    rh0 = _psi(si8, nb, seed=77.0)
+    eq, lmx, vcl, cl = model_kb_projectors(si8, si8.cell)
+    lmx = lmx.copy()
+    lmx[1] = 1          # exercise a per-atom projector count
+    natom, lmmax = R.kb_setup(eq, lmx, vcl, cl, nb)
+    r_ref, sl_ref = R.hckbpp_g(psi, rh0, natom, lmmax)
+    r_or, sl_or = O.hckbpp_g(eq, lmx, vcl, psi, rh0)
+    np.testing.assert_allclose(r_or, r_ref, rtol=1e-12, atol=1e-14)
+    for ia in range(natom):
+        np.testing.assert_allclose(sl_or[:, ia, : lmx[ia]], sl_ref[:, ia, : lmx[ia]],
+                                   rtol=1e-12, atol=1e-15)
+    occ = np.array([2.0, 2.0, 2.0, 0.0, 1.0])
+    e_ref = R.enkbpp_g2(nb, natom, occ, si8.cell.volume)
+    e_or = O.enkbpp_g(sl_or, lmx, cl, occ, si8.cell.volume)
+    assert abs(e_or - e_ref) <= 1e-12 * abs(e_ref)
+    # single band entry point hckbpp_ib_g
+    r1 = R.hckbpp_ib_g(psi[2], rh0[2], natom)
+    np.testing.assert_allclose(r1, r_ref[2], rtol=1e-12, atol=1e-14)
+
+
+def test_schmidt(si8):
+    psi = _psi(si8, 12)
+    a, oka = O.schmidt(psi)
+    b, okb = R.schmidt2(psi)
+    assert oka and okb
+    np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-13)
+
+
+def test_icholesky():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((40, 25))
+    s = x.T @ x + 0.1 * np.eye(25)
+    li_ref = R.icholesky(s)
+    li = O.rsdecmp_rsinvcd(s)
+    np.testing.assert_allclose(np.tril(li), np.tril(li_ref), rtol=1e-12, atol=1e-14)
+
+
+def test_hmatrix_untrycore(si8):
+    psi = _psi(si8, 9)
+    h = _psi(si8, 9, seed=5.0)
+    a, ea = O.hmatrix(psi, h)
+    b, eb = R.hmatrix2(psi, h)
+    np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(ea, eb, rtol=1e-13)
+    u = np.linalg.qr(np.random.default_rng(1).standard_normal((9, 9)))[0]
+    np.testing.assert_allclose(O.untrycore(psi, u), R.untrycore(psi, u), rtol=1e-12, atol=1e-15)
+
+
+def test_eigh_vs_rseigqr():
+    rng = np.random.default_rng(11)
+    for n in (7, 40):
+        x = rng.standard_normal((n, n))
+        a = x + x.T
+        w_ref, v_ref = R.rseigqr(a)
+        w, v = O.eigh(a)
+        np.testing.assert_allclose(w, w_ref, rtol=1e-11, atol=1e-12)
+        # eigenvectors agree up to sign (eigenvalues are simple here)
+        dots = np.abs(np.sum(v * v_ref, axis=0))
+        np.testing.assert_allclose(dots, 1.0, atol=1e-9)
