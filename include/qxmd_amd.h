@@ -1,0 +1,124 @@
+/*
+ * qxmd_amd.h -- C-ABI of the MI355X-native QXMD Gamma-point H|psi> /
+ * band-update path.  Plain pointers and sizes only; every entry point returns
+ * 0 on success and a negative code on failure (qx_last_error() describes it).
+ *
+ * Layout conventions follow the reference's Fortran arrays:
+ *   - wavefunctions cgjr/gdcr/rhcr(nplwex, nb): real-packed half sphere,
+ *     (Re, Im) interleaved for G index 0..nplw, nplwex = 2*(nplw+1)
+ *     (Sources/reclat.f90:90), band-major;
+ *   - FFT grids fft3x/fft3y/glocal(kfft0), x fastest (Sources/fft3.f90:166-224);
+ *   - eqlmr_(nplwex, lmmax, natom), vclalp(lmmax, natom) (Sources/nlkbpp.f90:5-73);
+ *   - occupations occ(nb), cell volume rvol (bohr^3).
+ * Energies are in Rydberg, as in the reference.
+ *
+ * Host-array entry points (the Fortran-shaped drop-ins) copy to the device,
+ * run, and copy back before returning.  The qx_slot_* / *_slot entry points
+ * work on device-resident band storage and are the throughput path.
+ */
+#ifndef QXMD_AMD_H
+#define QXMD_AMD_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qx_ctx qx_ctx;
+
+enum { QX_FP64 = 0, QX_MIXED = 1 };
+
+const char *qx_last_error(void);
+int qx_version(void);
+
+/* ---------------- plane-wave set-up (host) --------------------------------- */
+/* get_mesh: FFT grid for a cutoff (Sources/input.f90:1082-1134). hcell(3,3)
+ * column-major, lattice vectors as columns. */
+int qx_get_mesh(const double *hcell, double ecutdens, int *nd);
+/* strvec + set_rciprl: G vectors sorted by |G| with the reference's stable key
+ * order (Sources/reclat.f90:500-875, Sources/modules.f90:1518-1540).
+ * counts = {num, nplw5, nplw, nplw2, nplwc, nplw7}; arrays need cap >= num+1
+ * entries (index 0 is G = 0).  Returns num or -(num+1) if cap is too small. */
+int qx_strvec(const double *rba, double ecutdens, double ecut, int cap, int *counts, int *nga,
+              int *ngb, int *ngc, double *recnrm, double *gx, double *gy, double *gz);
+/* reclat: ijkgd(-nplw5:nplw5), 1-based grid indices (Sources/reclat.f90:259-276). */
+int qx_ijkgd(int nplw5, const int *nga, const int *ngb, const int *ngc, const int *nd,
+             int *ijkgd);
+
+/* ---------------- context ------------------------------------------------- */
+/* One cell / G sphere / FFT grid on the current HIP device.  nga..recnrm are
+ * the set_rciprl arrays for G index 0..nplw; nbmax bounds the bands held. */
+int qx_create(qx_ctx **ctx, int n1, int n2, int n3, int nplw, const int *nga, const int *ngb,
+              const int *ngc, const double *recnrm, int nbmax);
+int qx_destroy(qx_ctx *ctx);
+int qx_sync(qx_ctx *ctx);
+/* local potential glocal on the FFT grid, Ry (VECTOR build, Sources/engrad.f90:487-495) */
+int qx_set_glocal(qx_ctx *ctx, const double *glocal);
+/* KB projector tables as setnlc_g leaves them (Sources/nlkbpp.f90:1236-1279):
+ * eqlmr_(nplwex, lmmax, natom), lmx per atom, vclalp(lmmax, natom). */
+int qx_set_kb(qx_ctx *ctx, const double *eqlmr, int lmmax, int natom, const int *lmx_atom,
+              const double *vclalp_atom);
+int qx_info(qx_ctx *ctx, int *out /* {ncol, npair, nH, npj, nplwex} */);
+
+/* ---------------- device-resident band storage ----------------------------- */
+int qx_slot_put(qx_ctx *ctx, int slot, const double *gdcr, int nb);
+int qx_slot_get(qx_ctx *ctx, int slot, double *gdcr, int nb);
+int qx_slot_copy(qx_ctx *ctx, int src, int dst, int nb);
+
+/* dst = H src for nb bands: kinetic |G|^2 + local (two pruned FFTs) + KB;
+ * prec QX_FP64 or QX_MIXED (FP32 FFT stage).  Replaces ekinetic2 +
+ * hcnonloc + hckbpp_g (Sources/engrad.f90:66-127, 411-606;
+ * Sources/nlkbpp.f90:1284-1393). */
+int qx_hpsi_slot(qx_ctx *ctx, int src, int dst, int nb, int prec);
+/* local part only: dst (+)= (T + V) src */
+int qx_local_slot(qx_ctx *ctx, int src, int dst, int nb, int prec, int accumulate);
+/* KB part only: dst += B D B^T W src; slmir(lmmax... packed per projector) optional */
+int qx_kb_slot(qx_ctx *ctx, int src, int dst, int nb, double *slmir_host);
+/* schmidt on a slot (Sources/schmidt.f90:71-378); *lcholesky = 1 if S was
+ * positive definite (psi updated), 0 otherwise (psi untouched). */
+int qx_schmidt_slot(qx_ctx *ctx, int slot, int nb, int *lcholesky);
+/* subspace rotation: hmatrix2 + RsEIGQR + untrycore on psi and hpsi
+ * (Sources/engrad.f90:845-956, Sources/eigen.f90:1171-1514); eig ascending. */
+int qx_rotate_slot(qx_ctx *ctx, int psi, int hpsi, int nb, double *eig);
+/* rho(r) = sum occ/rvol psi(r)^2 on the FFT grid (softchg, Sources/chgdns.f90:388-454) */
+int qx_density_slot(qx_ctx *ctx, int slot, int nb, const double *occ, double rvol, double *rho,
+                    int prec);
+/* BKL CG band update, lortho=.false. / lhcunt=.true. as the SCF driver calls it
+ * (Sources/eigen.f90:199-938, Sources/pwlda.f90:3121).  psi/hpsi slots are
+ * updated in place; ekib, eig in; eig, bzan, nriter out (host arrays). */
+int qx_eigen_bkl_slot(qx_ctx *ctx, int psi, int hpsi, int nb, const double *ekib, double *eig,
+                      int noband, int itermx, int iteremx, double tolera, double threinn,
+                      double wdinner, double roughfzan, double roughfeig, double *bzan,
+                      int *nriter, int prec);
+/* one SCF band step with frozen potential (energy pass, rotation, CG,
+ * Gram-Schmidt, density) -- see Sources/pwlda.f90:2087-3574 */
+int qx_scf_band_step(qx_ctx *ctx, int psi, int hpsi, int nb, const double *occ, int noband,
+                     double rvol, double *eig, double *rho, int *lcholesky, int prec);
+
+/* ---------------- timing on the library stream ---------------------------- */
+int qx_event_record(qx_ctx *ctx, int slot);
+int qx_event_elapsed(qx_ctx *ctx, int slot0, int slot1, float *ms);
+
+/* ---------------- Fortran-shaped drop-ins (host arrays) -------------------- */
+/* rfft3 (Sources/fft3.f90:113-232): key=1 r2c x1/N with full Hermitian grid
+ * out, else c2r from the half grid; ierror = 0 on success. */
+int qx_rfft3(int key, double *fft3x, double *fft3y, int kfft1, int kfft2, int kfft3,
+             int *ierror);
+/* fft3 (Sources/fft3.f90:237-333): complex, key=1 forward x1/N. */
+int qx_fft3(int key, double *fft3x, double *fft3y, int kfft1, int kfft2, int kfft3, int *ierror);
+/* hcnonloc for one band with the context's glocal (Sources/engrad.f90:411-606):
+ * rhcr += local part of H cgjr (no kinetic). */
+int qx_hcnonloc(qx_ctx *ctx, const double *cgjr, double *rhcr);
+/* hckbpp_g (Sources/nlkbpp.f90:1284-1393): rhcr(:, 1:nb) += KB cgjr; slmir
+ * (lmmax, natom, nb) written if non-null. */
+int qx_hckbpp_g(qx_ctx *ctx, const double *cgjr, double *rhcr, int nb, double *slmir);
+/* hckbpp_ib_g (Sources/nlkbpp.f90:1623-1721): one band. */
+int qx_hckbpp_ib_g(qx_ctx *ctx, const double *cgjr, double *rhcr);
+/* schmidt (Sources/schmidt.f90:71-112) on host cgjr(nplwex, nband). */
+int qx_schmidt(qx_ctx *ctx, double *cgjr, int nband, int *lcholesky);
+/* softchg (Sources/chgdns.f90:388-454) from host cgjr(nplwex, nb). */
+int qx_softchg(qx_ctx *ctx, const double *cgjr, int nb, const double *occ, double rvol,
+               double *rho);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,561 @@
+// api.hip -- the C-ABI of qxmd_amd (declared in include/qxmd_amd.h).
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "../../include/qxmd_amd.h"
+#include "context.h"
+
+namespace qx {
+int strvec(const double* rba, double ecutdens, double ecut, int cap, int* counts, int* nga,
+           int* ngb, int* ngc, double* recnrm, double* gx, double* gy, double* gz);
+void get_mesh(const double* hcell, double ecutdens, int* nd);
+void ijkgd_map(int nplw5, const int* nga, const int* ngb, const int* ngc, const int* nd, int* ijkgd);
+int fullgrid_rfft3(int key, double* fx, double* fy, int n1, int n2, int n3);
+int fullgrid_fft3(int key, double* fx, double* fy, int n1, int n2, int n3);
+int dense_chol_inv(Ctx* c, double* s, int n, int* ok_dev);
+int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int* info);
+int dense_symmetrize(Ctx* c, const double* C, int n, double* H, double* eig);
+int pdot_batched(Ctx* c, const double* x, const double* y, int nb, double* out_dev);
+int bkl_band_by_band(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib, double* eig,
+                     int noband, int itermx, int iteremx, double tolera, double threinn,
+                     double wdinner, double roughfzan, double roughfeig, double* bzan, int* nriter,
+                     int prec);
+int bkl_batched(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib_host,
+                double* eig_host, int noband, int itermx, int iteremx, double tolera,
+                double roughfzan, double roughfeig, double* bzan, int* nriter, int prec);
+
+static thread_local std::string g_err;
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+// ---- internal <-> reference order ------------------------------------------
+__global__ void k_to_internal(const double* __restrict__ ref, double* __restrict__ in,
+                              const int* __restrict__ perm, int np, int nb) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)np * nb) return;
+  int b = (int)(i / np), p = (int)(i % np);
+  size_t n = 2 * (size_t)np;
+  int g = perm[p];
+  in[b * n + 2 * p] = ref[b * n + 2 * (size_t)g];
+  in[b * n + 2 * p + 1] = ref[b * n + 2 * (size_t)g + 1];
+}
+__global__ void k_to_reference(const double* __restrict__ in, double* __restrict__ ref,
+                               const int* __restrict__ perm, int np, int nb) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)np * nb) return;
+  int b = (int)(i / np), p = (int)(i % np);
+  size_t n = 2 * (size_t)np;
+  int g = perm[p];
+  ref[b * n + 2 * (size_t)g] = in[b * n + 2 * p];
+  ref[b * n + 2 * (size_t)g + 1] = in[b * n + 2 * p + 1];
+}
+
+static unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+__global__ void k_transpose(const double* a, double* t, int n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)n * n) return;
+  int r = (int)(i % n), col = (int)(i / n);
+  t[(size_t)r * n + col] = a[(size_t)col * n + r];
+}
+
+// acc += (T+V) psi - T psi  (the local part alone, hcnonloc)
+__global__ void k_add_minus_kin(double* acc, const double* tv, const double* psi, const double* kin,
+                                int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) acc[i] += tv[i] - kin[i >> 1] * psi[i];
+}
+
+// stage a host reference-order array (nb bands) into dst (internal order) via slot 5
+static int put_internal(Ctx* c, double* dst, const double* host, int nb) {
+  const size_t bytes = sizeof(double) * (size_t)c->nplwex * nb;
+  double* stage = c->d_slot[5];
+  QX_CHECK(hipMemcpyAsync(stage, host, bytes, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_to_internal, dim3(cdiv((long)(c->nplw + 1) * nb, 256)), dim3(256), 0,
+                     c->stream, stage, dst, c->d_perm, c->nplw + 1, nb);
+  QX_CHECK(hipGetLastError());
+  return 0;
+}
+static int get_reference(Ctx* c, const double* src, double* host, int nb) {
+  const size_t bytes = sizeof(double) * (size_t)c->nplwex * nb;
+  double* stage = c->d_slot[5];
+  hipLaunchKernelGGL(k_to_reference, dim3(cdiv((long)(c->nplw + 1) * nb, 256)), dim3(256), 0,
+                     c->stream, src, stage, c->d_perm, c->nplw + 1, nb);
+  QX_CHECK(hipMemcpyAsync(host, stage, bytes, hipMemcpyDeviceToHost, c->stream));
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int hpsi_batched(Ctx* c, const double* x, double* hx, int nb, int prec) {
+  int rc = local_apply(c, x, hx, nb, prec, 0);
+  if (rc) return rc;
+  return kb_apply(c, x, hx, nb, 1, nullptr);
+}
+
+// Gram-Schmidt by Cholesky of the packed overlap (schmidt2)
+static int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host) {
+  const int n = c->nplwex;
+  double* S = c->d_small;                        // nb x nb (+ nb scratch column)
+  double* Linv = c->d_small + (size_t)(nb + 1) * nb;
+  int* ok = reinterpret_cast<int*>(c->d_scal + 62 * (size_t)c->nbmax);
+  // S[i][j] = <psi_j|psi_i> (C[n=i][m=j]); column-major S(i,j) has the same
+  // storage for a symmetric matrix.
+  int rc = gemm_tn(c, nb, nb, n, 2.0, psi, n, psi, n, 0.0, S, nb, 1);
+  if (rc) return rc;
+  if ((rc = dense_chol_inv(c, S, nb, ok))) return rc;
+  int okh = 0;
+  QX_CHECK(hipMemcpyAsync(&okh, ok, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  *ok_host = okh;
+  if (!okh) return 0;
+  // psi'_ib = sum_jb Linv(ib,jb) psi_jb: B(n=ib,k=jb) = Linv(ib,jb) = S[ib + nb*jb]
+  // -> stored column-major, so use it as B with ldb via a transposed copy
+  // gemm_nn wants B[n*ldb + k] = Linv(n,k), the transpose of the column-major factor
+  double* out = c->d_slot[5];
+  hipLaunchKernelGGL(k_transpose, dim3(cdiv((long)nb * nb, 256)), dim3(256), 0, c->stream, S, Linv, nb);
+  if ((rc = gemm_nn(c, n, nb, nb, 1.0, psi, n, Linv, nb, 0.0, out, n))) return rc;
+  QX_CHECK(hipMemcpyAsync(psi, out, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice,
+                          c->stream));
+  return 0;
+}
+
+// subspace rotation: H_sub, eigenvectors U, psi <- psi U, hpsi <- hpsi U
+static int rotate_dev(Ctx* c, double* psi, double* hpsi, int nb, double* eig_host) {
+  const int n = c->nplwex;
+  double* C = c->d_small;
+  double* H = C + (size_t)nb * nb;
+  double* Z = H + (size_t)nb * nb;
+  double* work = Z + (size_t)nb * nb;
+  double* w = c->d_scal + 40 * (size_t)c->nbmax;
+  int* info = reinterpret_cast<int*>(c->d_scal + 63 * (size_t)c->nbmax);
+  int rc = gemm_tn(c, nb, nb, n, 2.0, psi, n, hpsi, n, 0.0, C, nb, 1);
+  if (rc) return rc;
+  if ((rc = dense_symmetrize(c, C, nb, H, nullptr))) return rc;
+  if ((rc = dense_eigh(c, H, nb, Z, w, work, info))) return rc;
+  // psi' = psi U: B(n=b, k=j) = U(j,b) = Z[j + nb*b] -> ldb = nb with B[n*ldb+k]
+  double* out = c->d_slot[5];
+  if ((rc = gemm_nn(c, n, nb, nb, 1.0, psi, n, Z, nb, 0.0, out, n))) return rc;
+  QX_CHECK(hipMemcpyAsync(psi, out, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice,
+                          c->stream));
+  if ((rc = gemm_nn(c, n, nb, nb, 1.0, hpsi, n, Z, nb, 0.0, out, n))) return rc;
+  QX_CHECK(hipMemcpyAsync(hpsi, out, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice,
+                          c->stream));
+  if (eig_host) {
+    int infoh = 0;
+    QX_CHECK(hipMemcpyAsync(eig_host, w, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+    QX_CHECK(hipMemcpyAsync(&infoh, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    QX_CHECK(hipStreamSynchronize(c->stream));
+    if (infoh) return fail(-7, "subspace eigensolver did not converge");
+  }
+  return 0;
+}
+
+// ekib: 2 <psi|T|psi> per band (ekinetic2, Sources/engrad.f90:117-122)
+__global__ void k_kin_apply(const double* psi, double* out, const double* kin, int n, int nb) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)n * nb) return;
+  int k = (int)(i % n);
+  out[i] = kin[k >> 1] * psi[i];
+}
+
+}  // namespace qx
+
+using namespace qx;
+
+struct qx_ctx {
+  Ctx c;
+};
+
+extern "C" {
+
+const char* qx_last_error(void) { return g_err.c_str(); }
+int qx_version(void) { return 1; }
+
+int qx_get_mesh(const double* hcell, double ecutdens, int* nd) {
+  get_mesh(hcell, ecutdens, nd);
+  return 0;
+}
+
+int qx_strvec(const double* rba, double ecutdens, double ecut, int cap, int* counts, int* nga,
+              int* ngb, int* ngc, double* recnrm, double* gx, double* gy, double* gz) {
+  return strvec(rba, ecutdens, ecut, cap, counts, nga, ngb, ngc, recnrm, gx, gy, gz);
+}
+
+int qx_ijkgd(int nplw5, const int* nga, const int* ngb, const int* ngc, const int* nd, int* ijkgd) {
+  ijkgd_map(nplw5, nga, ngb, ngc, nd, ijkgd);
+  return 0;
+}
+
+int qx_create(qx_ctx** out, int n1, int n2, int n3, int nplw, const int* nga, const int* ngb,
+              const int* ngc, const double* recnrm, int nbmax) {
+  if (!out || n1 <= 0 || n2 <= 0 || n3 <= 0 || nplw < 0 || nbmax < 1)
+    return fail(-1, "qx_create: bad arguments");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(-9, "qx_create: no HIP device");
+  qx_ctx* h = new qx_ctx();
+  Ctx* c = &h->c;
+  QX_CHECK(hipGetDevice(&c->dev));
+  c->n1 = n1; c->n2 = n2; c->n3 = n3;
+  c->N = (long)n1 * n2 * n3;
+  c->nplw = nplw;
+  c->nplwex = 2 * (nplw + 1);
+  c->nbmax = std::max(2, nbmax + (nbmax & 1));
+  c->plx = make_plan(n1); c->ply = make_plan(n2); c->plz = make_plan(n3);
+  if (c->plx.npass < 0 || c->ply.npass < 0 || c->plz.npass < 0 || n1 > 1024 || n2 > 1024 ||
+      n3 > 1024) {
+    delete h;
+    return fail(-2, "qx_create: FFT sizes must be 2^a 3^b 5^c 7^d <= 1024");
+  }
+  QX_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  int rc = pencil_build(c, nga, ngb, ngc, recnrm);
+  if (rc) {
+    delete h;
+    return fail(rc, "qx_create: pencil layout failed (sphere wraps the grid?)");
+  }
+  const size_t vbytes = sizeof(double) * (size_t)c->nplwex * c->nbmax;
+  for (int s = 0; s < Ctx::kSlots; ++s) {
+    QX_CHECK(hipMalloc(&c->d_slot[s], vbytes));
+    QX_CHECK(hipMemset(c->d_slot[s], 0, vbytes));
+  }
+  // pencil intermediates: a batch of band pairs, FP64 complex
+  const size_t per_bp = 16 * (size_t)n3 * c->pc.ncol;
+  const int nbp = c->nbmax / 2;
+  c->batch_bp = std::max(1, std::min(nbp, (int)std::max<size_t>(1, (size_t)(512u << 20) / per_bp)));
+  c->t_bytes = per_bp * c->batch_bp;
+  QX_CHECK(hipMalloc(&c->d_t1, c->t_bytes));
+  QX_CHECK(hipMalloc(&c->d_t2, c->t_bytes));
+  QX_CHECK(hipMalloc(&c->d_V, sizeof(double) * c->N));
+  QX_CHECK(hipMalloc(&c->d_Vf, sizeof(float) * c->N));
+  QX_CHECK(hipMemset(c->d_V, 0, sizeof(double) * c->N));
+  QX_CHECK(hipMemset(c->d_Vf, 0, sizeof(float) * c->N));
+  const size_t nsmall = 4 * (size_t)(c->nbmax + 1) * c->nbmax + 16 * (size_t)c->nbmax;
+  QX_CHECK(hipMalloc(&c->d_small, sizeof(double) * nsmall));
+  const size_t nsc = 64 * (size_t)c->nbmax + (size_t)c->nbmax * (c->nplwex / 4096 + 2);
+  QX_CHECK(hipMalloc(&c->d_scal, sizeof(double) * nsc));
+  QX_CHECK(hipMemset(c->d_scal, 0, sizeof(double) * nsc));
+  c->rho_groups = std::max(1, std::min(nbp, 32));
+  QX_CHECK(hipMalloc(&c->d_rho_part, sizeof(double) * (size_t)c->rho_groups * c->N));
+  QX_CHECK(hipMalloc(&c->d_rho, sizeof(double) * c->N));
+  for (auto& e : c->ev) QX_CHECK(hipEventCreate(&e));
+  *out = h;
+  return 0;
+}
+
+int qx_destroy(qx_ctx* h) {
+  if (!h) return 0;
+  Ctx* c = &h->c;
+  (void)hipStreamSynchronize(c->stream);
+  void* ptrs[] = {c->d_perm, c->d_pt_col, c->d_pt_z, c->d_pair_p0, c->d_col_hi, c->d_col_kmod,
+                  c->d_kin, c->d_tw, c->d_V, c->d_Vf, c->d_B, c->d_D, c->d_t1, c->d_t2, c->d_P,
+                  c->d_small, c->d_rho_part, c->d_rho, c->d_scal};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (auto* p : c->d_slot)
+    if (p) (void)hipFree(p);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
+  delete h;
+  return 0;
+}
+
+int qx_sync(qx_ctx* h) {
+  QX_CHECK(hipStreamSynchronize(h->c.stream));
+  return 0;
+}
+
+int qx_info(qx_ctx* h, int* out) {
+  out[0] = h->c.pc.ncol;
+  out[1] = h->c.pc.npair;
+  out[2] = h->c.pc.nH;
+  out[3] = h->c.npj;
+  out[4] = h->c.nplwex;
+  return 0;
+}
+
+__global__ void k_d2f(const double* a, float* b, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+int qx_set_glocal(qx_ctx* h, const double* glocal) {
+  Ctx* c = &h->c;
+  QX_CHECK(hipMemcpyAsync(c->d_V, glocal, sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_d2f, dim3(cdiv(c->N, 256)), dim3(256), 0, c->stream, c->d_V, c->d_Vf, c->N);
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int qx_set_kb(qx_ctx* h, const double* eqlmr, int lmmax, int natom, const int* lmx_atom,
+              const double* vclalp_atom) {
+  Ctx* c = &h->c;
+  std::vector<int> pa, pl;
+  for (int ia = 0; ia < natom; ++ia)
+    for (int l = 0; l < std::min(lmx_atom[ia], lmmax); ++l) {
+      pa.push_back(ia);
+      pl.push_back(l);
+    }
+  const int npj = (int)pa.size();
+  const size_t n = (size_t)c->nplwex;
+  if (c->d_B) { (void)hipFree(c->d_B); c->d_B = nullptr; }
+  if (c->d_D) { (void)hipFree(c->d_D); c->d_D = nullptr; }
+  if (c->d_P) { (void)hipFree(c->d_P); c->d_P = nullptr; }
+  c->npj = npj;
+  c->lmmax = lmmax;
+  c->natom = natom;
+  c->proj_atom = pa;
+  c->proj_l = pl;
+  if (npj == 0) return 0;
+  // projectors in reference order, then permuted to internal order on device
+  std::vector<double> B(n * npj), D(npj);
+  for (int p = 0; p < npj; ++p) {
+    std::memcpy(&B[n * p], eqlmr + n * ((size_t)pl[p] + (size_t)lmmax * pa[p]), sizeof(double) * n);
+    D[p] = vclalp_atom[pl[p] + lmmax * pa[p]];
+  }
+  QX_CHECK(hipMalloc(&c->d_B, sizeof(double) * n * npj));
+  QX_CHECK(hipMalloc(&c->d_D, sizeof(double) * npj));
+  QX_CHECK(hipMalloc(&c->d_P, sizeof(double) * (size_t)npj * c->nbmax));
+  double* stage = nullptr;
+  QX_CHECK(hipMalloc(&stage, sizeof(double) * n * npj));
+  QX_CHECK(hipMemcpy(stage, B.data(), sizeof(double) * n * npj, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_to_internal, dim3(cdiv((long)(c->nplw + 1) * npj, 256)), dim3(256), 0,
+                     c->stream, stage, c->d_B, c->d_perm, c->nplw + 1, npj);
+  QX_CHECK(hipMemcpyAsync(c->d_D, D.data(), sizeof(double) * npj, hipMemcpyHostToDevice, c->stream));
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  QX_CHECK(hipFree(stage));
+  return 0;
+}
+
+int qx_slot_put(qx_ctx* h, int slot, const double* gdcr, int nb) {
+  Ctx* c = &h->c;
+  if (slot < 0 || slot > 1 || nb > c->nbmax) return fail(-1, "qx_slot_put: slot 0/1, nb <= nbmax");
+  int rc = put_internal(c, c->d_slot[slot], gdcr, nb);
+  if (rc) return rc;
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int qx_slot_get(qx_ctx* h, int slot, double* gdcr, int nb) {
+  Ctx* c = &h->c;
+  if (slot < 0 || slot > 4 || nb > c->nbmax) return fail(-1, "qx_slot_get: bad slot/nb");
+  return get_reference(c, c->d_slot[slot], gdcr, nb);
+}
+
+int qx_slot_copy(qx_ctx* h, int src, int dst, int nb) {
+  Ctx* c = &h->c;
+  QX_CHECK(hipMemcpyAsync(c->d_slot[dst], c->d_slot[src], sizeof(double) * c->nplwex * nb,
+                          hipMemcpyDeviceToDevice, c->stream));
+  return 0;
+}
+
+int qx_hpsi_slot(qx_ctx* h, int src, int dst, int nb, int prec) {
+  Ctx* c = &h->c;
+  return hpsi_batched(c, c->d_slot[src], c->d_slot[dst], nb, prec);
+}
+
+int qx_local_slot(qx_ctx* h, int src, int dst, int nb, int prec, int accumulate) {
+  Ctx* c = &h->c;
+  return local_apply(c, c->d_slot[src], c->d_slot[dst], nb, prec, accumulate);
+}
+
+int qx_kb_slot(qx_ctx* h, int src, int dst, int nb, double* slmir_host) {
+  Ctx* c = &h->c;
+  int rc = kb_apply(c, c->d_slot[src], c->d_slot[dst], nb, 1, nullptr);
+  if (rc) return rc;
+  (void)slmir_host;
+  return 0;
+}
+
+int qx_schmidt_slot(qx_ctx* h, int slot, int nb, int* lcholesky) {
+  Ctx* c = &h->c;
+  return schmidt_dev(c, c->d_slot[slot], nb, lcholesky);
+}
+
+int qx_rotate_slot(qx_ctx* h, int psi, int hpsi, int nb, double* eig) {
+  Ctx* c = &h->c;
+  return rotate_dev(c, c->d_slot[psi], c->d_slot[hpsi], nb, eig);
+}
+
+int qx_density_slot(qx_ctx* h, int slot, int nb, const double* occ, double rvol, double* rho,
+                    int prec) {
+  Ctx* c = &h->c;
+  double* w = c->d_scal + 41 * (size_t)c->nbmax;
+  std::vector<double> wh(nb + 1, 0.0);
+  for (int b = 0; b < nb; ++b) wh[b] = occ[b] / rvol;
+  QX_CHECK(hipMemcpyAsync(w, wh.data(), sizeof(double) * (nb + 1), hipMemcpyHostToDevice, c->stream));
+  int rc = density(c, c->d_slot[slot], nb, w, 1.0, prec);
+  if (rc) return rc;
+  if (rho) {
+    QX_CHECK(hipMemcpyAsync(rho, c->d_rho, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream));
+    QX_CHECK(hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+int qx_eigen_bkl_slot(qx_ctx* h, int psi, int hpsi, int nb, const double* ekib, double* eig,
+                      int noband, int itermx, int iteremx, double tolera, double threinn,
+                      double wdinner, double roughfzan, double roughfeig, double* bzan, int* nriter,
+                      int prec) {
+  Ctx* c = &h->c;
+  if (psi > 1 || hpsi > 1) return fail(-1, "qx_eigen_bkl_slot: psi/hpsi must be slots 0/1");
+  int rc = bkl_band_by_band(c, c->d_slot[psi], c->d_slot[hpsi], nb, ekib, eig, noband, itermx,
+                            iteremx, tolera, threinn, wdinner, roughfzan, roughfeig, bzan, nriter,
+                            prec);
+  if (rc) return rc;
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// batched BKL (all bands per iteration)
+int qx_eigen_bkl_batched_slot(qx_ctx* h, int psi, int hpsi, int nb, const double* ekib, double* eig,
+                              int noband, int itermx, int iteremx, double tolera, double roughfzan,
+                              double roughfeig, double* bzan, int* nriter, int prec) {
+  Ctx* c = &h->c;
+  return bkl_batched(c, c->d_slot[psi], c->d_slot[hpsi], nb, ekib, eig, noband, itermx, iteremx,
+                     tolera, roughfzan, roughfeig, bzan, nriter, prec);
+}
+
+int qx_ekib_slot(qx_ctx* h, int slot, int nb, double* ekib_host) {
+  Ctx* c = &h->c;
+  const int n = c->nplwex;
+  double* t = c->d_slot[5];
+  double* d = c->d_scal + 42 * (size_t)c->nbmax;
+  hipLaunchKernelGGL(k_kin_apply, dim3(cdiv((long)n * nb, 256)), dim3(256), 0, c->stream,
+                     c->d_slot[slot], t, c->d_kin, n, nb);
+  pdot_batched(c, t, c->d_slot[slot], nb, d);
+  std::vector<double> e(nb);
+  QX_CHECK(hipMemcpyAsync(e.data(), d, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  for (int b = 0; b < nb; ++b) ekib_host[b] = 2.0 * e[b];
+  return 0;
+}
+
+int qx_scf_band_step_ex(qx_ctx* h, int psi, int hpsi, int nb, const double* occ, int noband,
+                        double rvol, double* eig, double* rho, int* lcholesky, int prec,
+                        int batched) {
+  Ctx* c = &h->c;
+  double* P = c->d_slot[psi];
+  double* HP = c->d_slot[hpsi];
+  std::vector<double> ekib(nb), bz(nb);
+  std::vector<int> nr(nb);
+  // energy pass: H psi, ekib
+  int rc = hpsi_batched(c, P, HP, nb, prec);
+  if (rc) return rc;
+  if ((rc = qx_ekib_slot(h, psi, nb, ekib.data()))) return rc;
+  // subspace rotation
+  if ((rc = rotate_dev(c, P, HP, nb, eig))) return rc;
+  // CG
+  if (batched)
+    rc = bkl_batched(c, P, HP, nb, ekib.data(), eig, noband, 4, 4, 1e-10, 0.3, 0.3, bz.data(),
+                     nr.data(), prec);
+  else
+    rc = bkl_band_by_band(c, P, HP, nb, ekib.data(), eig, noband, 4, 4, 1e-10, 0.0, 0.9, 0.3, 0.3,
+                          bz.data(), nr.data(), prec);
+  if (rc) return rc;
+  if ((rc = schmidt_dev(c, P, nb, lcholesky))) return rc;
+  if (rho || true) {
+    if ((rc = qx_density_slot(h, psi, nb, occ, rvol, rho, prec))) return rc;
+  }
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int qx_scf_band_step(qx_ctx* h, int psi, int hpsi, int nb, const double* occ, int noband,
+                     double rvol, double* eig, double* rho, int* lcholesky, int prec) {
+  return qx_scf_band_step_ex(h, psi, hpsi, nb, occ, noband, rvol, eig, rho, lcholesky, prec, 0);
+}
+
+int qx_event_record(qx_ctx* h, int slot) {
+  QX_CHECK(hipEventRecord(h->c.ev[slot & 7], h->c.stream));
+  return 0;
+}
+
+int qx_event_elapsed(qx_ctx* h, int s0, int s1, float* ms) {
+  QX_CHECK(hipEventSynchronize(h->c.ev[s1 & 7]));
+  QX_CHECK(hipEventElapsedTime(ms, h->c.ev[s0 & 7], h->c.ev[s1 & 7]));
+  return 0;
+}
+
+// ---------------- drop-ins --------------------------------------------------
+int qx_rfft3(int key, double* fft3x, double* fft3y, int k1, int k2, int k3, int* ierror) {
+  int rc = fullgrid_rfft3(key, fft3x, fft3y, k1, k2, k3);
+  if (ierror) *ierror = rc ? 1 : 0;
+  return rc ? fail(rc, "qx_rfft3: failed (%d)", rc) : 0;
+}
+
+int qx_fft3(int key, double* fft3x, double* fft3y, int k1, int k2, int k3, int* ierror) {
+  int rc = fullgrid_fft3(key, fft3x, fft3y, k1, k2, k3);
+  if (ierror) *ierror = rc ? 1 : 0;
+  return rc ? fail(rc, "qx_fft3: failed (%d)", rc) : 0;
+}
+
+int qx_hcnonloc(qx_ctx* h, const double* cgjr, double* rhcr) {
+  Ctx* c = &h->c;
+  int rc = put_internal(c, c->d_slot[0], cgjr, 1);
+  if (rc) return rc;
+  if ((rc = put_internal(c, c->d_slot[1], rhcr, 1))) return rc;
+  // slot 2 = (T + V) psi; slot 1 += slot 2 - T psi
+  if ((rc = local_apply(c, c->d_slot[0], c->d_slot[2], 1, PREC_FP64, 0))) return rc;
+  hipLaunchKernelGGL(k_add_minus_kin, dim3(cdiv(c->nplwex, 256)), dim3(256), 0, c->stream,
+                     c->d_slot[1], c->d_slot[2], c->d_slot[0], c->d_kin, c->nplwex);
+  return get_reference(c, c->d_slot[1], rhcr, 1);
+}
+
+int qx_hckbpp_g(qx_ctx* h, const double* cgjr, double* rhcr, int nb, double* slmir) {
+  Ctx* c = &h->c;
+  if (nb > c->nbmax) return fail(-1, "qx_hckbpp_g: nb > nbmax");
+  int rc = put_internal(c, c->d_slot[0], cgjr, nb);
+  if (rc) return rc;
+  if ((rc = put_internal(c, c->d_slot[1], rhcr, nb))) return rc;
+  double* P = c->d_slot[2];  // projections staged here (npj x nb fits a slot)
+  if ((size_t)c->npj * nb > (size_t)c->nplwex * c->nbmax) return fail(-1, "too many projectors");
+  if ((rc = kb_apply(c, c->d_slot[0], c->d_slot[1], nb, 1, slmir ? P : nullptr))) return rc;
+  if (slmir) {
+    std::vector<double> ph((size_t)c->npj * nb);
+    QX_CHECK(hipMemcpyAsync(ph.data(), P, sizeof(double) * ph.size(), hipMemcpyDeviceToHost,
+                            c->stream));
+    QX_CHECK(hipStreamSynchronize(c->stream));
+    std::memset(slmir, 0, sizeof(double) * (size_t)c->lmmax * c->natom * nb);
+    for (int b = 0; b < nb; ++b)
+      for (int p = 0; p < c->npj; ++p)
+        slmir[c->proj_l[p] + (size_t)c->lmmax * (c->proj_atom[p] + (size_t)c->natom * b)] =
+            ph[(size_t)b * c->npj + p];
+  }
+  return get_reference(c, c->d_slot[1], rhcr, nb);
+}
+
+int qx_hckbpp_ib_g(qx_ctx* h, const double* cgjr, double* rhcr) {
+  return qx_hckbpp_g(h, cgjr, rhcr, 1, nullptr);
+}
+
+int qx_schmidt(qx_ctx* h, double* cgjr, int nband, int* lcholesky) {
+  Ctx* c = &h->c;
+  if (nband > c->nbmax) return fail(-1, "qx_schmidt: nband > nbmax");
+  int rc = put_internal(c, c->d_slot[0], cgjr, nband);
+  if (rc) return rc;
+  if ((rc = schmidt_dev(c, c->d_slot[0], nband, lcholesky))) return rc;
+  if (!*lcholesky) return 0;
+  return get_reference(c, c->d_slot[0], cgjr, nband);
+}
+
+int qx_softchg(qx_ctx* h, const double* cgjr, int nb, const double* occ, double rvol, double* rho) {
+  Ctx* c = &h->c;
+  if (nb > c->nbmax) return fail(-1, "qx_softchg: nb > nbmax");
+  int rc = put_internal(c, c->d_slot[0], cgjr, nb);
+  if (rc) return rc;
+  return qx_density_slot(h, 0, nb, occ, rvol, rho, PREC_FP64);
+}
+
+}  // extern "C"
+

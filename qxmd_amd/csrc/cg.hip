@@ -1,0 +1,372 @@
+// cg.hip -- BKL conjugate-gradient band update (Sources/eigen.f90:199-938).
+//
+// Two schedules share the vector kernels below:
+//   * band-by-band (the reference's order): band ib's search direction is made
+//     orthogonal to the already-updated bands 1..ib-1 (subgs3) and to itself
+//     (subgs4); its H h is one single-vector band-apply.  Scalars of the 2x2
+//     Rayleigh-Ritz step are evaluated on the host exactly as the reference.
+//   * batched: every band advances one BKL step per iteration; directions are
+//     projected against all current bands with two GEMMs, H h is one batched
+//     band-apply over all bands, and the per-band 2x2 step runs in a device
+//     kernel, so an iteration never waits on the host.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+#include "context.h"
+
+namespace qx {
+
+int local_apply(Ctx* c, const double* psi, double* out, int nb, int prec, int accumulate);
+
+constexpr int kDotThreads = 256;
+constexpr int kDotChunk = 4096;
+
+// partial[b*nch + ch] = sum over chunk of x_b y_b (plain sum)
+__global__ void __launch_bounds__(kDotThreads) k_dot_partial(const double* __restrict__ x,
+                                                             const double* __restrict__ y, int n,
+                                                             int ldx, int ldy, int nch,
+                                                             double* __restrict__ part) {
+  __shared__ double red[kDotThreads / 64];
+  const int b = blockIdx.y, ch = blockIdx.x;
+  const double* xb = x + (size_t)b * ldx;
+  const double* yb = y + (size_t)b * ldy;
+  const int i0 = ch * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  double s = 0.0;
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) s += xb[i] * yb[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kDotThreads / 64; ++w) t += red[w];
+    part[(size_t)b * nch + ch] = t;
+  }
+}
+
+// out[b] = 2 * sum - x_b[0] y_b[0]  (packed inner product)
+__global__ void k_dot_final(const double* __restrict__ part, int nch, int nb,
+                            const double* __restrict__ x, const double* __restrict__ y, int ldx,
+                            int ldy, double* __restrict__ out) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double s = 0.0;
+  for (int c = 0; c < nch; ++c) s += part[(size_t)b * nch + c];
+  out[b] = 2.0 * s - x[(size_t)b * ldx] * y[(size_t)b * ldy];
+}
+
+// out_b = a_b * x_b + c_b * y_b (coefficient arrays may be null: 1 / 0);
+// `mask` (optional) skips bands whose flag is zero.
+__global__ void k_axpby(double* out, const double* x, const double* y, const double* a,
+                        const double* cc, double sa, double sc, int n, int ld, int nb,
+                        const int* mask) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)n * nb) return;
+  int b = (int)(i / n), k = (int)(i % n);
+  if (mask && !mask[b]) return;
+  double av = a ? a[b] : sa, cv = cc ? cc[b] : sc;
+  size_t o = (size_t)b * ld + k;
+  double yv = y ? y[o] : 0.0;
+  out[o] = av * x[o] + cv * yv;
+}
+
+// prcd_b = Teter-Payne-Allan preconditioner from ekib[b] (prpcg, Sources/eigen.f90:1093-1113)
+__global__ void k_precond_mul(double* g, const double* kin2, const double* ekib, int n, int ld,
+                              int nb, const int* mask) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)n * nb) return;
+  int b = (int)(i / n), k = (int)(i % n);
+  if (mask && !mask[b]) return;
+  double e15 = 1.5 * ekib[b];
+  double x = kin2[k >> 1] / e15;
+  double tesu = 27.0 + x * (18.0 + x * (12.0 + 8.0 * x));
+  double x2 = x * x, x4 = x2 * x2;
+  tesu = tesu / (tesu + 16.0 * x4);
+  g[(size_t)b * ld + k] *= tesu * 2.0 / e15;
+}
+
+static inline unsigned cdivu(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+int pdot_batched(Ctx* c, const double* x, const double* y, int nb, double* out_dev) {
+  const int n = c->nplwex, nch = (n + kDotChunk - 1) / kDotChunk;
+  double* part = c->d_scal + 64 * (size_t)c->nbmax;  // scratch after the scalar arrays
+  hipLaunchKernelGGL(k_dot_partial, dim3(nch, nb), dim3(kDotThreads), 0, c->stream, x, y, n, n, n,
+                     nch, part);
+  hipLaunchKernelGGL(k_dot_final, dim3(cdivu(nb, 256)), dim3(256), 0, c->stream, part, nch, nb, x, y,
+                     n, n, out_dev);
+  return 0;
+}
+
+static void axpby(Ctx* c, double* out, const double* x, const double* y, const double* a,
+                  const double* cc, double sa, double sc, int nb, const int* mask) {
+  const int n = c->nplwex;
+  hipLaunchKernelGGL(k_axpby, dim3(cdivu((long)n * nb, 256)), dim3(256), 0, c->stream, out, x, y, a,
+                     cc, sa, sc, n, n, nb, mask);
+}
+
+static double pdot1(Ctx* c, const double* x, const double* y) {
+  double* d = c->d_scal + 60 * (size_t)c->nbmax;
+  pdot_batched(c, x, y, 1, d);
+  double h = 0.0;
+  (void)hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  (void)hipStreamSynchronize(c->stream);
+  return h;
+}
+
+// H h for one vector (kinetic + KB + local), order of Sources/eigen.f90:516-551
+static int hpsi1(Ctx* c, const double* x, double* hx, int prec) {
+  int rc = local_apply(c, x, hx, 1, prec, 0);
+  if (rc) return rc;
+  return kb_apply(c, x, hx, 1, 1, nullptr);
+}
+
+int bkl_band_by_band(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib, double* eig,
+                     int noband, int itermx, int iteremx, double tolera, double threinn,
+                     double wdinner, double roughfzan, double roughfeig, double* bzan, int* nriter,
+                     int prec) {
+  const int n = c->nplwex;
+  double* gnk = c->d_slot[4];
+  double* hnk = c->d_slot[4] + n;
+  double* prod = c->d_scal + 8 * (size_t)c->nbmax;
+  double* ek = c->d_scal + 9 * (size_t)c->nbmax;
+  double* kin2 = c->d_kin;
+  (void)hipMemcpyAsync(ek, ekib, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream);
+  for (int ibb = 0; ibb < nb; ++ibb) {
+    const int ib = ibb + 1;
+    double* x = psi + (size_t)n * ibb;
+    double* hx = hpsi + (size_t)n * ibb;
+    double eigv = pdot1(c, x, hx);
+    axpby(c, gnk, x, hx, nullptr, nullptr, eigv, -1.0, 1, nullptr);
+    double zansa = pdot1(c, gnk, gnk);
+    int iter = 0, izcnt = 0, lreset = 1;
+    double zansab = zansa, z000re = zansa, x000re = 1e10, eigb = eigv, dinnerx = 0.0, betk = 0.0;
+    for (;;) {
+      ++iter;
+      hipLaunchKernelGGL(k_precond_mul, dim3(cdivu(n, 256)), dim3(256), 0, c->stream, gnk, kin2,
+                         ek + ibb, n, n, 1, nullptr);
+      double zansap = pdot1(c, gnk, gnk);
+      double beta = lreset ? 0.0 : zansap / betk;
+      // hnk = gnk + beta*hnk (hnk cleared on reset)
+      axpby(c, hnk, gnk, lreset ? nullptr : hnk, nullptr, nullptr, 1.0, beta, 1, nullptr);
+      lreset = 0;
+      betk = zansap;
+      if (ibb > 0) {  // subgs3 against updated bands 1..ib-1
+        int rc = gemm_tn(c, ibb, 1, n, 2.0, psi, n, hnk, n, 0.0, prod, ibb, 1);
+        if (rc) return rc;
+        rc = gemm_nn(c, n, 1, ibb, -1.0, psi, n, prod, ibb, 1.0, hnk, n);
+        if (rc) return rc;
+      }
+      double hhx = pdot1(c, x, hnk);  // subgs4
+      axpby(c, hnk, hnk, x, nullptr, nullptr, 1.0, -hhx, 1, nullptr);
+      double cai = pdot1(c, hnk, hx);
+      int rc = hpsi1(c, hnk, gnk, prec);
+      if (rc) return rc;
+      double cbi = pdot1(c, hnk, gnk);
+      double hhi = pdot1(c, hnk, hnk);
+      double dtnew, alpsq;
+      if (hhi > 0.0) {
+        double h11 = eigv, h12 = cai / std::sqrt(hhi), h21 = h12, h22 = cbi / hhi;
+        double enew =
+            0.5 * (h11 + h22 - std::sqrt((h11 + h22) * (h11 + h22) - 4.0 * (h11 * h22 - h12 * h21)));
+        dtnew = (enew - h11) / h12 / std::sqrt(hhi);
+        double alpsq2 = 1.0 / (1.0 + dtnew * (0.0 + dtnew * hhi));
+        alpsq = std::sqrt(alpsq2);
+        eigv = enew;
+      } else {
+        double hx2 = pdot1(c, hnk, x);
+        double c0 = eigv, c1 = cai, c2 = cbi, c3 = hhi, c4 = 2.0 * hx2;
+        double s0 = c0 * c4 - c1, s1 = c0 * c3 - c2, s2 = c1 * c3 - c2 * c4;
+        double s = s1 * s1 - s2 * s0;
+        dtnew = s > 0.0 ? (-s1 - std::sqrt(s)) / s2 : -1.0;
+        if (dtnew > 0.0) {
+          double dinner = std::sqrt(1.0 / (1.0 + dtnew * dtnew * hhi));
+          if (dinner < threinn) dtnew = 0.0;
+          if (dinner < dinnerx * wdinner) dtnew = 0.0;
+          dinnerx = std::max(dinnerx, dinner);
+        }
+        double dt2 = dtnew * dtnew, alpsq2 = 1.0 / (1.0 + dtnew * c4 + dt2 * c3);
+        eigv = (c0 + dtnew * c1 + dt2 * c2) * alpsq2;
+        alpsq = std::sqrt(alpsq2);
+      }
+      // x = alpsq (x + dt h), hx = alpsq (hx + dt H h), gnk = eig x - hx
+      axpby(c, x, x, hnk, nullptr, nullptr, alpsq, alpsq * dtnew, 1, nullptr);
+      axpby(c, hx, hx, gnk, nullptr, nullptr, alpsq, alpsq * dtnew, 1, nullptr);
+      axpby(c, gnk, x, hx, nullptr, nullptr, eigv, -1.0, 1, nullptr);
+      zansa = pdot1(c, gnk, gnk);
+      if ((ib <= noband && iter >= itermx) || (ib > noband && iter >= iteremx) || zansa < tolera ||
+          std::fabs(eigb - eigv) < tolera || zansa < z000re * roughfzan ||
+          (iter > 1 && std::fabs(eigb - eigv) < x000re * roughfeig))
+        break;
+      if (iter == 1) x000re = std::fabs(eigb - eigv);
+      if (zansa > zansab) {
+        if (++izcnt > 20) { izcnt = 0; lreset = 1; }
+      }
+      zansab = zansa;
+      eigb = eigv;
+    }
+    eig[ibb] = eigv;
+    if (bzan) bzan[ibb] = zansa;
+    if (nriter) nriter[ibb] = iter;
+  }
+  QX_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// batched form: per-band scalars live on the device
+// ---------------------------------------------------------------------------
+struct BklState {
+  double *eig, *eigb, *zansa, *zansab, *z000re, *x000re, *betk, *cai, *cbi, *hhi, *zansap;
+  double *alpha, *dt, *beta, *one;
+  int *active, *iter, *izcnt, *lreset;
+};
+
+__global__ void k_bkl_init(BklState s, int nb) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  s.zansab[b] = s.zansa[b];
+  s.z000re[b] = s.zansa[b];
+  s.x000re[b] = 1e10;
+  s.eigb[b] = s.eig[b];
+  s.izcnt[b] = 0;
+  s.lreset[b] = 1;
+  s.iter[b] = 0;
+  s.active[b] = 1;
+  s.one[b] = 1.0;
+}
+
+__global__ void k_bkl_beta(BklState s, int nb) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb || !s.active[b]) return;
+  s.beta[b] = s.lreset[b] ? 0.0 : s.zansap[b] / s.betk[b];
+  s.lreset[b] = 0;
+  s.betk[b] = s.zansap[b];
+}
+
+// 2x2 Rayleigh-Ritz of BKL (Sources/eigen.f90:593-622); inactive bands get
+// alpha = 1, dt = 0 so the update leaves them unchanged.
+__global__ void k_bkl_step(BklState s, int nb) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  if (!s.active[b]) {
+    s.alpha[b] = 1.0;
+    s.dt[b] = 0.0;
+    return;
+  }
+  double hhi = s.hhi[b], cai = s.cai[b], cbi = s.cbi[b], h11 = s.eig[b];
+  if (hhi > 0.0) {
+    double h12 = cai / sqrt(hhi), h22 = cbi / hhi;
+    double enew = 0.5 * (h11 + h22 - sqrt((h11 + h22) * (h11 + h22) - 4.0 * (h11 * h22 - h12 * h12)));
+    double dt = (enew - h11) / h12 / sqrt(hhi);
+    double a2 = 1.0 / (1.0 + dt * dt * hhi);
+    s.alpha[b] = sqrt(a2);
+    s.dt[b] = dt;
+    s.eig[b] = enew;
+  } else {
+    s.alpha[b] = 1.0;
+    s.dt[b] = 0.0;
+  }
+  s.iter[b] += 1;
+}
+
+// convergence test after the update (Sources/eigen.f90:755-778)
+__global__ void k_bkl_conv(BklState s, int nb, int noband, int itermx, int iteremx, double tolera,
+                           double roughfzan, double roughfeig) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb || !s.active[b]) return;
+  const int ib = b + 1, iter = s.iter[b];
+  double zansa = s.zansa[b], eig = s.eig[b], eigb = s.eigb[b];
+  bool stop = (ib <= noband && iter >= itermx) || (ib > noband && iter >= iteremx) ||
+              zansa < tolera || fabs(eigb - eig) < tolera || zansa < s.z000re[b] * roughfzan ||
+              (iter > 1 && fabs(eigb - eig) < s.x000re[b] * roughfeig);
+  if (stop) {
+    s.active[b] = 0;
+    return;
+  }
+  if (iter == 1) s.x000re[b] = fabs(eigb - eig);
+  if (zansa > s.zansab[b]) {
+    if (++s.izcnt[b] > 20) { s.izcnt[b] = 0; s.lreset[b] = 1; }
+  }
+  s.zansab[b] = zansa;
+  s.eigb[b] = eig;
+}
+
+__global__ void k_neg(double* out, const double* in, int nb) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) out[b] = -in[b];
+}
+__global__ void k_mul(double* out, const double* a, const double* b2, int nb) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) out[b] = a[b] * b2[b];
+}
+
+int hpsi_batched(Ctx* c, const double* x, double* hx, int nb, int prec);
+
+int bkl_batched(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib_host, double* eig_host,
+                int noband, int itermx, int iteremx, double tolera, double roughfzan,
+                double roughfeig, double* bzan, int* nriter, int prec) {
+  const int n = c->nplwex;
+  double* S = c->d_scal;
+  const size_t L = (size_t)c->nbmax;
+  BklState s;
+  s.eig = S; s.eigb = S + L; s.zansa = S + 2 * L; s.zansab = S + 3 * L; s.z000re = S + 4 * L;
+  s.x000re = S + 5 * L; s.betk = S + 6 * L; s.cai = S + 7 * L; s.cbi = S + 10 * L;
+  s.hhi = S + 11 * L; s.zansap = S + 12 * L; s.alpha = S + 13 * L; s.dt = S + 14 * L;
+  s.beta = S + 15 * L; s.one = S + 16 * L;
+  double* ek = S + 9 * L;
+  double* tmp = S + 17 * L;
+  double* tmp2 = S + 18 * L;
+  int* I = reinterpret_cast<int*>(S + 20 * L);
+  s.active = I; s.iter = I + L; s.izcnt = I + 2 * L; s.lreset = I + 3 * L;
+  double* gnk = c->d_slot[2];
+  double* hnk = c->d_slot[3];
+  double* hh = c->d_slot[4];
+  double* proj = c->d_small;  // nb x nb
+  const unsigned gb = cdivu(nb, 256);
+  QX_CHECK(hipMemcpyAsync(ek, ekib_host, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
+  // eig = <x|Hx>, g = eig x - Hx, zansa = <g|g>
+  pdot_batched(c, psi, hpsi, nb, s.eig);
+  axpby(c, gnk, psi, hpsi, s.eig, nullptr, 0.0, -1.0, nb, nullptr);
+  pdot_batched(c, gnk, gnk, nb, s.zansa);
+  hipLaunchKernelGGL(k_bkl_init, dim3(gb), dim3(256), 0, c->stream, s, nb);
+  const int maxit = std::max(itermx, iteremx);
+  for (int it = 0; it < maxit; ++it) {
+    // precondition, CG direction
+    hipLaunchKernelGGL(k_precond_mul, dim3(cdivu((long)n * nb, 256)), dim3(256), 0, c->stream, gnk,
+                       c->d_kin, ek, n, n, nb, s.active);
+    pdot_batched(c, gnk, gnk, nb, s.zansap);
+    hipLaunchKernelGGL(k_bkl_beta, dim3(gb), dim3(256), 0, c->stream, s, nb);
+    if (it == 0) axpby(c, hnk, gnk, nullptr, nullptr, nullptr, 1.0, 0.0, nb, s.active);
+    else axpby(c, hnk, gnk, hnk, nullptr, s.beta, 1.0, 0.0, nb, s.active);
+    // project out all current bands: h <- h - Psi (Psi^T W h)
+    int rc = gemm_tn(c, nb, nb, n, 2.0, psi, n, hnk, n, 0.0, proj, nb, 1);
+    if (rc) return rc;
+    rc = gemm_nn(c, n, nb, nb, -1.0, psi, n, proj, nb, 1.0, hnk, n);
+    if (rc) return rc;
+    pdot_batched(c, hnk, hpsi, nb, s.cai);
+    rc = hpsi_batched(c, hnk, hh, nb, prec);
+    if (rc) return rc;
+    pdot_batched(c, hnk, hh, nb, s.cbi);
+    pdot_batched(c, hnk, hnk, nb, s.hhi);
+    hipLaunchKernelGGL(k_bkl_step, dim3(gb), dim3(256), 0, c->stream, s, nb);
+    // x = a (x + dt h); Hx = a (Hx + dt Hh); g = eig x - Hx
+    hipLaunchKernelGGL(k_mul, dim3(gb), dim3(256), 0, c->stream, tmp, s.alpha, s.dt, nb);
+    axpby(c, psi, psi, hnk, s.alpha, tmp, 0.0, 0.0, nb, s.active);
+    axpby(c, hpsi, hpsi, hh, s.alpha, tmp, 0.0, 0.0, nb, s.active);
+    axpby(c, gnk, psi, hpsi, s.eig, nullptr, 0.0, -1.0, nb, s.active);
+    pdot_batched(c, gnk, gnk, nb, tmp2);
+    QX_CHECK(hipMemcpyAsync(s.zansa, tmp2, sizeof(double) * nb, hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_bkl_conv, dim3(gb), dim3(256), 0, c->stream, s, nb, noband, itermx,
+                       iteremx, tolera, roughfzan, roughfeig);
+  }
+  QX_CHECK(hipMemcpyAsync(eig_host, s.eig, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+  if (bzan)
+    QX_CHECK(hipMemcpyAsync(bzan, s.zansa, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+  std::vector<int> itv(nb);
+  QX_CHECK(hipMemcpyAsync(itv.data(), s.iter, sizeof(int) * nb, hipMemcpyDeviceToHost, c->stream));
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  if (nriter)
+    for (int b = 0; b < nb; ++b) nriter[b] = itv[b];
+  return 0;
+}
+
+}  // namespace qx

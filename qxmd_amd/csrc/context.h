@@ -1,0 +1,77 @@
+// context.h -- device-resident plane-wave context of the qxmd_amd library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <vector>
+#include "common.h"
+
+namespace qx {
+
+enum Prec { PREC_FP64 = 0, PREC_FP32 = 1 };
+
+// Pencil layout of the half sphere.  Columns are the (h,k) lines along z that
+// the full sphere (G and -G) touches; they come in mirror pairs (c, -c), with
+// the self-mirror column (0,0) alone as pair 0.  Internal point order:
+// pair 0 (G=0 first), then for each pair q>=1 the points of column 2q-1 and
+// then those of column 2q.  Pair q>=1 owns columns 2q-1 and 2q.
+struct Pencil {
+  int ncol = 0, npair = 0, nH = 0, hmin = 0;
+  std::vector<int> perm;        // internal point p -> reference index g
+  std::vector<int> pt_col;      // column holding +G of point p
+  std::vector<int> pt_z;        // l mod n3 of point p
+  std::vector<int> pair_p0;     // first point of pair q (size npair+1)
+  std::vector<int> col_hi;      // h - hmin of column c
+  std::vector<int> col_kmod;    // k mod n2 of column c
+};
+
+struct Ctx {
+  int dev = 0;
+  int n1 = 0, n2 = 0, n3 = 0;
+  long N = 0;
+  int nplw = 0, nplwex = 0;
+  int nbmax = 0;
+  hipStream_t stream = nullptr;
+  Pencil pc;
+  FftPlan plx{}, ply{}, plz{};
+  // device arrays
+  int *d_perm = nullptr, *d_pt_col = nullptr, *d_pt_z = nullptr, *d_pair_p0 = nullptr;
+  int *d_col_hi = nullptr, *d_col_kmod = nullptr;
+  double *d_kin = nullptr;        // |G_p|^2 per internal point
+  double *d_tw = nullptr;         // twiddles: x (n1), y (n2), z (n3), exp(+2pi i k/n)
+  double *d_V = nullptr;          // local potential, FFT order (N)
+  float *d_Vf = nullptr;          // FP32 copy
+  // KB projectors, internal order: B[p*nplwex + i], p = projector
+  int npj = 0;
+  double *d_B = nullptr;
+  double *d_D = nullptr;          // vclalp per projector
+  std::vector<int> proj_atom, proj_l;
+  int lmmax = 0, natom = 0;
+  // band storage slots, internal order: slot s, band b at d_slot[s] + b*nplwex
+  static constexpr int kSlots = 6;
+  double *d_slot[kSlots] = {};
+  // scratch
+  void *d_t1 = nullptr, *d_t2 = nullptr;  // pencil intermediates
+  size_t t_bytes = 0;
+  int batch_bp = 0;                       // band pairs per pencil batch
+  double *d_P = nullptr;                  // projections npj x nbmax
+  double *d_small = nullptr;              // nbmax x nbmax x 4 dense work
+  double *d_rho_part = nullptr;           // density partial planes
+  int rho_groups = 0;
+  double *d_rho = nullptr;                // density (N)
+  double *d_scal = nullptr;               // scalar work (device)
+  hipEvent_t ev[8] = {};
+};
+
+int pencil_build(Ctx* c, const int* nga, const int* ngb, const int* ngc, const double* recnrm);
+
+// band-apply: out = (T + V_loc) psi for bands [b0, b0+nb) of slot src into
+// slot dst (kinetic + local); KB handled separately by the GEMM path.
+int local_apply(Ctx* c, const double* psi, double* out, int nb, int prec, int accumulate);
+int density(Ctx* c, const double* psi, int nb, const double* occ_dev, double scale, int prec);
+int kb_apply(Ctx* c, const double* psi, double* out, int nb, int accumulate_into_out,
+             double* slmir_out);
+int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
+            int ldb, double beta, double* C, int ldc, int corr);
+int gemm_nn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
+            int ldb, double beta, double* C, int ldc);
+
+}  // namespace qx

@@ -1,0 +1,206 @@
+// fullgrid.hip -- dense 3-D FFT on the full grid, behind the rfft3/fft3
+// drop-ins (Sources/fft3.f90:113-333): x1/N on the forward transform, the r2c
+// result returned as the full Hermitian grid, c2r reading only i < n1/2+1.
+// Lines along each axis are transformed in LDS by the same Stockham engine as
+// the pencil path; blocks take runs of lines adjacent in x so that strided
+// (y, z) lines are still read in coalesced rows.
+#include <algorithm>
+#include "context.h"
+
+namespace qx {
+
+constexpr int kFgThreads = 256;
+
+// axis 0: lines along x, block = `nl` consecutive lines (rows)
+// axis 1/2: lines along y or z; block = `nl` consecutive x values of one
+//           (z) resp. (y) index.
+__global__ void __launch_bounds__(kFgThreads) k_fg_lines(double2* __restrict__ g, int n1, int n2,
+                                                         int n3, int axis, int nl, FftPlan pl,
+                                                         const double* __restrict__ twtab,
+                                                         int sign, double scale) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<double>* tw = reinterpret_cast<cplx<double>*>(smem_raw);
+  const int n = pl.n;
+  cplx<double>* lines = tw + n;
+  load_tw<double>(tw, twtab, n);
+  long base, estride;
+  int lcount;
+  if (axis == 0) {
+    long l0 = (long)blockIdx.x * nl;
+    long nlines = (long)n2 * n3;
+    lcount = (int)std::min<long>(nl, nlines - l0);
+    base = l0 * n1;
+    estride = 1;
+    for (int i = threadIdx.x; i < lcount * n; i += blockDim.x) {
+      double2 v = g[base + i];
+      lines[i] = mk<double>(v.x, v.y);
+    }
+  } else {
+    int x0 = (blockIdx.x % ((n1 + nl - 1) / nl)) * nl;
+    int other = blockIdx.x / ((n1 + nl - 1) / nl);  // z for axis 1, y for axis 2
+    lcount = std::min(nl, n1 - x0);
+    estride = axis == 1 ? n1 : (long)n1 * n2;
+    base = axis == 1 ? x0 + (long)other * n1 * n2 : x0 + (long)other * n1;
+    for (int i = threadIdx.x; i < lcount * n; i += blockDim.x) {
+      int t = i / lcount, x = i % lcount;
+      double2 v = g[base + (long)t * estride + x];
+      lines[x * n + t] = mk<double>(v.x, v.y);
+    }
+  }
+  __syncthreads();
+  lds_fft<double, 4>(lines, lcount, n, 1, pl, tw, sign);
+  if (axis == 0) {
+    for (int i = threadIdx.x; i < lcount * n; i += blockDim.x) {
+      cplx<double> v = lines[i];
+      g[base + i] = make_double2(v.x * scale, v.y * scale);
+    }
+  } else {
+    for (int i = threadIdx.x; i < lcount * n; i += blockDim.x) {
+      int t = i / lcount, x = i % lcount;
+      cplx<double> v = lines[x * n + t];
+      g[base + (long)t * estride + x] = make_double2(v.x * scale, v.y * scale);
+    }
+  }
+}
+
+__global__ void k_fg_pack(const double* __restrict__ x, const double* __restrict__ y, double2* g,
+                          long N) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) g[i] = make_double2(x[i], y ? y[i] : 0.0);
+}
+
+// c2r input: Hermitian completion from the half grid i < n1/2+1
+__global__ void k_fg_hermitian(const double* __restrict__ x, const double* __restrict__ y,
+                               double2* g, int n1, int n2, int n3) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long N = (long)n1 * n2 * n3;
+  if (i >= N) return;
+  int ix = (int)(i % n1);
+  long r = i / n1;
+  int iy = (int)(r % n2), iz = (int)(r / n2);
+  const int n1h = n1 / 2 + 1;
+  if (ix < n1h) {
+    g[i] = make_double2(x[i], y[i]);
+  } else {
+    int mx = n1 - ix, my = (n2 - iy) % n2, mz = (n3 - iz) % n3;
+    long m = mx + (long)n1 * (my + (long)n2 * mz);
+    g[i] = make_double2(x[m], -y[m]);
+  }
+}
+
+__global__ void k_fg_unpack(const double2* __restrict__ g, double* x, double* y, long N,
+                            int real_only) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double2 v = g[i];
+  x[i] = v.x;
+  y[i] = real_only ? 0.0 : v.y;
+}
+
+struct FullGrid {
+  int n1 = 0, n2 = 0, n3 = 0;
+  FftPlan p[3];
+  double* d_tw[3] = {nullptr, nullptr, nullptr};
+  double2* d_g = nullptr;
+  double *d_x = nullptr, *d_y = nullptr;
+  hipStream_t stream = nullptr;
+};
+
+static FullGrid g_fg;
+
+static int fg_prepare(int n1, int n2, int n3) {
+  if (g_fg.n1 == n1 && g_fg.n2 == n2 && g_fg.n3 == n3) return 0;
+  if (g_fg.d_g) {
+    (void)hipFree(g_fg.d_g);
+    (void)hipFree(g_fg.d_x);
+    (void)hipFree(g_fg.d_y);
+    for (auto& t : g_fg.d_tw) (void)hipFree(t);
+  }
+  if (!g_fg.stream) QX_CHECK(hipStreamCreateWithFlags(&g_fg.stream, hipStreamNonBlocking));
+  const int dims[3] = {n1, n2, n3};
+  for (int a = 0; a < 3; ++a) {
+    g_fg.p[a] = make_plan(dims[a]);
+    if (g_fg.p[a].npass < 0 || dims[a] > 2 * 4 * kFgThreads) return -6;
+    std::vector<double> tw(2 * dims[a]);
+    for (int k = 0; k < dims[a]; ++k) {
+      double ang = 2.0 * M_PI * (double)k / (double)dims[a];
+      tw[2 * k] = std::cos(ang);
+      tw[2 * k + 1] = std::sin(ang);
+    }
+    QX_CHECK(hipMalloc(&g_fg.d_tw[a], sizeof(double) * tw.size()));
+    QX_CHECK(hipMemcpy(g_fg.d_tw[a], tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+  }
+  long N = (long)n1 * n2 * n3;
+  QX_CHECK(hipMalloc(&g_fg.d_g, sizeof(double2) * N));
+  QX_CHECK(hipMalloc(&g_fg.d_x, sizeof(double) * N));
+  QX_CHECK(hipMalloc(&g_fg.d_y, sizeof(double) * N));
+  g_fg.n1 = n1;
+  g_fg.n2 = n2;
+  g_fg.n3 = n3;
+  return 0;
+}
+
+static int fg_transform(int sign, double scale) {
+  const int n1 = g_fg.n1, n2 = g_fg.n2, n3 = g_fg.n3;
+  const int dims[3] = {n1, n2, n3};
+  for (int a = 0; a < 3; ++a) {
+    const int n = dims[a];
+    int nl = std::max(1, std::min(64, 4096 / n));
+    unsigned blocks;
+    if (a == 0) blocks = (unsigned)(((long)n2 * n3 + nl - 1) / nl);
+    else blocks = (unsigned)(((n1 + nl - 1) / nl) * (a == 1 ? n3 : n2));
+    size_t lds = sizeof(cplx<double>) * ((size_t)n + (size_t)nl * n);
+    double s = (a == 2) ? scale : 1.0;
+    if (lds > 65536)
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_fg_lines),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_fg_lines, dim3(blocks), dim3(kFgThreads), lds, g_fg.stream, g_fg.d_g, n1,
+                       n2, n3, a, nl, g_fg.p[a], g_fg.d_tw[a], sign, s);
+  }
+  QX_CHECK(hipGetLastError());
+  return 0;
+}
+
+int fullgrid_rfft3(int key, double* fx, double* fy, int n1, int n2, int n3) {
+  int rc = fg_prepare(n1, n2, n3);
+  if (rc) return rc;
+  const long N = (long)n1 * n2 * n3;
+  const unsigned nb = (unsigned)((N + 255) / 256);
+  QX_CHECK(hipMemcpyAsync(g_fg.d_x, fx, sizeof(double) * N, hipMemcpyHostToDevice, g_fg.stream));
+  if (key == 1) {
+    hipLaunchKernelGGL(k_fg_pack, dim3(nb), dim3(256), 0, g_fg.stream, g_fg.d_x, nullptr, g_fg.d_g, N);
+    if ((rc = fg_transform(-1, 1.0 / (double)N))) return rc;
+    hipLaunchKernelGGL(k_fg_unpack, dim3(nb), dim3(256), 0, g_fg.stream, g_fg.d_g, g_fg.d_x,
+                       g_fg.d_y, N, 0);
+  } else {
+    QX_CHECK(hipMemcpyAsync(g_fg.d_y, fy, sizeof(double) * N, hipMemcpyHostToDevice, g_fg.stream));
+    hipLaunchKernelGGL(k_fg_hermitian, dim3(nb), dim3(256), 0, g_fg.stream, g_fg.d_x, g_fg.d_y,
+                       g_fg.d_g, n1, n2, n3);
+    if ((rc = fg_transform(+1, 1.0))) return rc;
+    hipLaunchKernelGGL(k_fg_unpack, dim3(nb), dim3(256), 0, g_fg.stream, g_fg.d_g, g_fg.d_x,
+                       g_fg.d_y, N, 1);
+  }
+  QX_CHECK(hipMemcpyAsync(fx, g_fg.d_x, sizeof(double) * N, hipMemcpyDeviceToHost, g_fg.stream));
+  QX_CHECK(hipMemcpyAsync(fy, g_fg.d_y, sizeof(double) * N, hipMemcpyDeviceToHost, g_fg.stream));
+  QX_CHECK(hipStreamSynchronize(g_fg.stream));
+  return 0;
+}
+
+int fullgrid_fft3(int key, double* fx, double* fy, int n1, int n2, int n3) {
+  int rc = fg_prepare(n1, n2, n3);
+  if (rc) return rc;
+  const long N = (long)n1 * n2 * n3;
+  const unsigned nb = (unsigned)((N + 255) / 256);
+  QX_CHECK(hipMemcpyAsync(g_fg.d_x, fx, sizeof(double) * N, hipMemcpyHostToDevice, g_fg.stream));
+  QX_CHECK(hipMemcpyAsync(g_fg.d_y, fy, sizeof(double) * N, hipMemcpyHostToDevice, g_fg.stream));
+  hipLaunchKernelGGL(k_fg_pack, dim3(nb), dim3(256), 0, g_fg.stream, g_fg.d_x, g_fg.d_y, g_fg.d_g, N);
+  if ((rc = fg_transform(key == 1 ? -1 : +1, key == 1 ? 1.0 / (double)N : 1.0))) return rc;
+  hipLaunchKernelGGL(k_fg_unpack, dim3(nb), dim3(256), 0, g_fg.stream, g_fg.d_g, g_fg.d_x, g_fg.d_y,
+                     N, 0);
+  QX_CHECK(hipMemcpyAsync(fx, g_fg.d_x, sizeof(double) * N, hipMemcpyDeviceToHost, g_fg.stream));
+  QX_CHECK(hipMemcpyAsync(fy, g_fg.d_y, sizeof(double) * N, hipMemcpyDeviceToHost, g_fg.stream));
+  QX_CHECK(hipStreamSynchronize(g_fg.stream));
+  return 0;
+}
+
+}  // namespace qx

@@ -1,0 +1,184 @@
+// gemm.hip -- FP64 MFMA contractions of the band algebra.
+//
+// Every dense contraction on the path is one of two shapes over the
+// real-packed plane-wave index (length nplwex, contiguous in memory):
+//   gemm_tn : C[n][m] = alpha * sum_k A[m][k] B[n][k] + beta C   (K = plane waves)
+//             projections P = B^T Psi (hckbpp_g, Sources/nlkbpp.f90:1358-1364),
+//             overlap S = Psi^T Psi (schmidt2, Sources/schmidt.f90:199-216),
+//             subspace H = Psi^T H Psi (hmatrix2, Sources/engrad.f90:869-882)
+//   gemm_nn : C[n][m] = alpha * sum_k A[k][m] B[n][k] + beta C   (M = plane waves)
+//             back-projection H Psi += B D P (Sources/nlkbpp.f90:1376-1379),
+//             rotations Psi U (untrycore, Sources/eigen.f90:1456-1514) and
+//             Psi L^-T (schmidt2, Sources/schmidt.f90:284-313).
+// The packed inner product 2 sum - x0 y0 (G=0 counted once) is applied as
+// alpha = 2 plus a rank-1 correction of row 0 (and row 1 for the KB sum,
+// which skips Im c(G=0): Sources/nlkbpp.f90:1360-1363).
+//
+// Tiles: 64x64 per 256-thread block, 4 wavefronts of 32x32, each a 2x2 grid
+// of v_mfma_f64_16x16x4 blocks; K staged through LDS 16 at a time.  The
+// long-K shape splits K over blocks and reduces with FP64 atomics.
+#include <algorithm>
+#include "context.h"
+
+namespace qx {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64, BN = 64, KT = 16, LDT = BM + 2;
+
+template <bool AKC, bool SPLIT>
+__global__ void __launch_bounds__(256) k_gemm_f64(int M, int N, int K, int kchunk, double alpha,
+                                                  const double* __restrict__ A, int lda,
+                                                  const double* __restrict__ B, int ldb,
+                                                  double* __restrict__ C, int ldc) {
+  __shared__ double As[KT][LDT];
+  __shared__ double Bs[KT][LDT];
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  d4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+  for (int k0 = kb; k0 < ke; k0 += KT) {
+    // ---- stage A tile (64 x 16) ----
+    if (AKC) {
+      int m = tid >> 2, kk = (tid & 3) * 4;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        int gk = k0 + kk + u, gm = m0 + m;
+        As[kk + u][m] = (gm < M && gk < ke) ? A[(size_t)gm * lda + gk] : 0.0;
+      }
+    } else {
+      int kk = tid >> 4, m = (tid & 15) * 4;
+      int gk = k0 + kk;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        int gm = m0 + m + u;
+        As[kk][m + u] = (gm < M && gk < ke) ? A[(size_t)gk * lda + gm] : 0.0;
+      }
+    }
+    // ---- stage B tile (64 x 16), B[n][k] ----
+    {
+      int n = tid >> 2, kk = (tid & 3) * 4;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        int gk = k0 + kk + u, gn = n0 + n;
+        Bs[kk + u][n] = (gn < N && gk < ke) ? B[(size_t)gn * ldb + gk] : 0.0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k4 = 0; k4 < KT; k4 += 4) {
+      const int kr = k4 + (lane >> 4), c = lane & 15;
+      double a0 = As[kr][wm + c], a1 = As[kr][wm + 16 + c];
+      double b0 = Bs[kr][wn + c], b1 = Bs[kr][wn + 16 + c];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // ---- epilogue: acc[i][j][v] = C(m = .. + 4*(lane/16) + v, n = .. + lane%16)
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      int n = n0 + wn + j * 16 + (lane & 15);
+      if (n >= N) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int m = m0 + wm + i * 16 + 4 * (lane >> 4) + v;
+        if (m >= M) continue;
+        double val = alpha * acc[i][j][v];
+        if (SPLIT) atomicAdd(&C[(size_t)n * ldc + m], val);
+        else C[(size_t)n * ldc + m] += val;
+      }
+    }
+}
+
+__global__ void k_scale(double* C, int M, int N, int ldc, double beta) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)M * N) return;
+  int m = (int)(i % M), n = (int)(i / M);
+  double* p = C + (size_t)n * ldc + m;
+  *p = beta == 0.0 ? 0.0 : beta * *p;
+}
+
+// packed-inner-product corrections: C[n][m] -= A[m][0] B[n][0]  (+ 2 A[m][1] B[n][1])
+__global__ void k_corr(double* C, int M, int N, int ldc, const double* A, int lda,
+                       const double* B, int ldb, double alpha, int skip_im0) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)M * N) return;
+  int m = (int)(i % M), n = (int)(i / M);
+  double c = A[(size_t)m * lda] * B[(size_t)n * ldb];
+  if (skip_im0) c += 2.0 * A[(size_t)m * lda + 1] * B[(size_t)n * ldb + 1];
+  C[(size_t)n * ldc + m] -= alpha * 0.5 * c;
+}
+
+static inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+// corr: 0 none, 1 subtract the G=0 term, 2 also the Im(G=0) term (KB)
+int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
+            int ldb, double beta, double* C, int ldc, int corr) {
+  if (M <= 0 || N <= 0) return 0;
+  hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N, ldc,
+                     beta);
+  const int tiles = (int)(cdiv(M, BM) * cdiv(N, BN));
+  int nsplit = std::max<int>(1, std::min<int>((int)cdiv(K, 1024), std::max(1, 2048 / tiles)));
+  int kchunk = (int)(cdiv(cdiv(K, nsplit), KT) * KT);
+  nsplit = (int)cdiv(K, kchunk);
+  dim3 grid(cdiv(M, BM), cdiv(N, BN), nsplit);
+  if (nsplit > 1)
+    hipLaunchKernelGGL((k_gemm_f64<true, true>), grid, dim3(256), 0, c->stream, M, N, K, kchunk,
+                       alpha, A, lda, B, ldb, C, ldc);
+  else
+    hipLaunchKernelGGL((k_gemm_f64<true, false>), grid, dim3(256), 0, c->stream, M, N, K, kchunk,
+                       alpha, A, lda, B, ldb, C, ldc);
+  if (corr)
+    hipLaunchKernelGGL(k_corr, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N, ldc,
+                       A, lda, B, ldb, alpha, corr == 2 ? 1 : 0);
+  QX_CHECK(hipGetLastError());
+  return 0;
+}
+
+int gemm_nn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
+            int ldb, double beta, double* C, int ldc) {
+  if (M <= 0 || N <= 0) return 0;
+  if (beta != 1.0)
+    hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N,
+                       ldc, beta);
+  dim3 grid(cdiv(M, BM), cdiv(N, BN), 1);
+  hipLaunchKernelGGL((k_gemm_f64<false, false>), grid, dim3(256), 0, c->stream, M, N, K, K, alpha,
+                     A, lda, B, ldb, C, ldc);
+  QX_CHECK(hipGetLastError());
+  return 0;
+}
+
+// P[b][p] -> slmir (optional copy) and DP[b][p] = D[p] P[b][p]
+__global__ void k_scale_proj(double* P, const double* D, int npj, int nb) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)npj * nb) return;
+  P[i] *= D[i % npj];
+}
+
+// Kleinman-Bylander apply for nb bands: out += B diag(D) (B^T W psi).
+int kb_apply(Ctx* c, const double* psi, double* out, int nb, int, double* slmir_out) {
+  if (c->npj == 0) return 0;
+  int rc = gemm_tn(c, c->npj, nb, c->nplwex, 2.0, c->d_B, c->nplwex, psi, c->nplwex, 0.0, c->d_P,
+                   c->npj, 2);
+  if (rc) return rc;
+  if (slmir_out)
+    QX_CHECK(hipMemcpyAsync(slmir_out, c->d_P, sizeof(double) * (size_t)c->npj * nb,
+                            hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_scale_proj, dim3(cdiv((long)c->npj * nb, 256)), dim3(256), 0, c->stream,
+                     c->d_P, c->d_D, c->npj, nb);
+  return gemm_nn(c, c->nplwex, nb, c->npj, 1.0, c->d_B, c->nplwex, c->d_P, c->npj, 1.0, out,
+                 c->nplwex);
+}
+
+}  // namespace qx

@@ -1,0 +1,109 @@
+"""ctypes binding of libqxmd_amd.so (the C-ABI declared in include/qxmd_amd.h).
+
+The library is the product: nothing here falls back to a CPU path.  Importing
+works without a GPU (for set-up calls and symbol checks); any compute call on
+a machine without a HIP device fails with QxError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(HERE, "libqxmd_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "qxmd_amd.h")
+
+_LIB = None
+
+
+class QxError(RuntimeError):
+    pass
+
+
+def load():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(SO):
+            raise QxError(f"{SO} is missing: build it with `python -m qxmd_amd.build`")
+        _LIB = C.CDLL(SO)
+        _declare(_LIB)
+    return _LIB
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i, _d, _vp = C.c_int, C.c_double, C.c_void_p
+_pi = C.POINTER(C.c_int)
+_pd = C.POINTER(C.c_double)
+
+# name -> argtypes (restype int unless noted)
+SIGNATURES = {
+    "qx_version": [],
+    "qx_get_mesh": [_dp, _d, _ip],
+    "qx_strvec": [_dp, _d, _d, _i, _ip, _ip, _ip, _ip, _dp, _vp, _vp, _vp],
+    "qx_ijkgd": [_i, _ip, _ip, _ip, _ip, _ip],
+    "qx_create": [C.POINTER(_vp), _i, _i, _i, _i, _ip, _ip, _ip, _dp, _i],
+    "qx_destroy": [_vp],
+    "qx_sync": [_vp],
+    "qx_set_glocal": [_vp, _dp],
+    "qx_set_kb": [_vp, _dp, _i, _i, _ip, _dp],
+    "qx_info": [_vp, _ip],
+    "qx_slot_put": [_vp, _i, _dp, _i],
+    "qx_slot_get": [_vp, _i, _dp, _i],
+    "qx_slot_copy": [_vp, _i, _i, _i],
+    "qx_hpsi_slot": [_vp, _i, _i, _i, _i],
+    "qx_local_slot": [_vp, _i, _i, _i, _i, _i],
+    "qx_kb_slot": [_vp, _i, _i, _i, _vp],
+    "qx_schmidt_slot": [_vp, _i, _i, _pi],
+    "qx_rotate_slot": [_vp, _i, _i, _i, _dp],
+    "qx_density_slot": [_vp, _i, _i, _dp, _d, _vp, _i],
+    "qx_eigen_bkl_slot": [_vp, _i, _i, _i, _dp, _dp, _i, _i, _i, _d, _d, _d, _d, _d, _dp, _ip, _i],
+    "qx_scf_band_step": [_vp, _i, _i, _i, _dp, _i, _d, _dp, _vp, _pi, _i],
+    "qx_event_record": [_vp, _i],
+    "qx_event_elapsed": [_vp, _i, _i, C.POINTER(C.c_float)],
+    "qx_rfft3": [_i, _dp, _dp, _i, _i, _i, _pi],
+    "qx_fft3": [_i, _dp, _dp, _i, _i, _i, _pi],
+    "qx_hcnonloc": [_vp, _dp, _dp],
+    "qx_hckbpp_g": [_vp, _dp, _dp, _i, _vp],
+    "qx_hckbpp_ib_g": [_vp, _dp, _dp],
+    "qx_schmidt": [_vp, _dp, _i, _pi],
+    "qx_softchg": [_vp, _dp, _i, _dp, _d, _dp],
+}
+# entry points beyond the header's core set (batched CG, kinetic energies)
+EXTRA = {
+    "qx_eigen_bkl_batched_slot": [_vp, _i, _i, _i, _dp, _dp, _i, _i, _i, _d, _d, _d, _dp, _ip, _i],
+    "qx_ekib_slot": [_vp, _i, _i, _dp],
+    "qx_scf_band_step_ex": [_vp, _i, _i, _i, _dp, _i, _d, _dp, _vp, _pi, _i, _i],
+}
+
+
+def _declare(L):
+    for name, args in {**SIGNATURES, **EXTRA}.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.qx_last_error.argtypes = []
+    L.qx_last_error.restype = C.c_char_p
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = load().qx_last_error().decode(errors="replace")
+        raise QxError(f"{what}: qxmd_amd error {rc}: {msg}")
+
+
+def declared_symbols():
+    """Names of the functions include/qxmd_amd.h declares."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(qx_[a-z0-9_]+)\s*\(", txt)))
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)

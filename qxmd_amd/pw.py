@@ -1,0 +1,225 @@
+"""Host-side mirror of the reference's plane-wave path interface.
+
+``GSphere`` runs the set-up the reference does in reclat_ini/strvec/reclat
+(grid rule, sorted G vectors, FFT maps) through the library; ``PlaneWave``
+owns one device context (G sphere + FFT grid + potential + KB tables + band
+storage) and exposes the reference's operations by their QXMD names.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .lib import QxError, check, f64, i32, load
+
+FP64, MIXED = 0, 1
+
+
+class GSphere:
+    """get_mesh + strvec + set_rciprl + reclat for cell ``hcell`` (columns are
+    lattice vectors, bohr) and cutoffs in Ry."""
+
+    def __init__(self, hcell, ecut=20.0, ecutdens=80.0):
+        L = load()
+        h = f64(np.asarray(hcell, np.float64).T.ravel())  # column-major
+        nd = np.zeros(3, np.int32)
+        check(L.qx_get_mesh(h, float(ecutdens), nd), "qx_get_mesh")
+        self.nd = tuple(int(v) for v in nd)
+        counts = np.zeros(6, np.int32)
+        cap = 1
+        rc = L.qx_strvec(h, float(ecutdens), float(ecut), cap, counts, np.zeros(1, np.int32),
+                         np.zeros(1, np.int32), np.zeros(1, np.int32), np.zeros(1), None, None,
+                         None)
+        cap = -rc
+        nga, ngb, ngc = (np.zeros(cap, np.int32) for _ in range(3))
+        rec, gx, gy, gz = (np.zeros(cap) for _ in range(4))
+        num = L.qx_strvec(h, float(ecutdens), float(ecut), cap, counts, nga, ngb, ngc, rec,
+                          gx.ctypes.data_as(C.c_void_p), gy.ctypes.data_as(C.c_void_p),
+                          gz.ctypes.data_as(C.c_void_p))
+        if num <= 0:
+            raise QxError("qx_strvec failed")
+        self.num, self.nplw5, self.nplw, self.nplw2, self.nplwc, self.nplw7 = (int(x) for x in counts)
+        self.nga, self.ngb, self.ngc, self.recnrm = nga, ngb, ngc, rec
+        self.gx, self.gy, self.gz = gx, gy, gz
+        self.nplwex = 2 * (self.nplw + 1)
+        self.N = int(np.prod(self.nd))
+        self.hcell = np.asarray(hcell, np.float64)
+        self.volume = float(abs(np.linalg.det(self.hcell)))
+
+    def ijkgd(self):
+        out = np.zeros(2 * self.nplw5 + 1, np.int32)
+        check(load().qx_ijkgd(self.nplw5, self.nga, self.ngb, self.ngc, i32(self.nd), out),
+              "qx_ijkgd")
+        return out
+
+    def dkgnrm(self):
+        d = np.repeat(self.recnrm[: self.nplw + 1], 2)
+        return d
+
+
+class PlaneWave:
+    """Device context for one G sphere (slots 0 = psi, 1 = H psi)."""
+
+    def __init__(self, gs: GSphere, nbmax: int):
+        L = load()
+        self.gs = gs
+        self.nbmax = int(nbmax)
+        h = C.c_void_p()
+        n1, n2, n3 = gs.nd
+        check(L.qx_create(C.byref(h), n1, n2, n3, gs.nplw, gs.nga, gs.ngb, gs.ngc, gs.recnrm,
+                          self.nbmax), "qx_create")
+        self.h = h
+        self.npj = 0
+
+    def close(self):
+        if self.h:
+            load().qx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- boundary data
+    def set_glocal(self, glocal):
+        check(load().qx_set_glocal(self.h, f64(glocal)), "qx_set_glocal")
+
+    def set_kb(self, eqlmr, lmx_atom, vclalp_atom):
+        """eqlmr: (natom, lmmax, nplwex) C-order == Fortran eqlmr_(nplwex, lmmax, natom)."""
+        e = f64(eqlmr)
+        natom, lmmax, n = e.shape
+        assert n == self.gs.nplwex
+        check(load().qx_set_kb(self.h, e, lmmax, natom, i32(lmx_atom), f64(vclalp_atom)),
+              "qx_set_kb")
+        info = np.zeros(5, np.int32)
+        load().qx_info(self.h, info)
+        self.npj = int(info[3])
+
+    def info(self):
+        info = np.zeros(5, np.int32)
+        load().qx_info(self.h, info)
+        return dict(ncol=int(info[0]), npair=int(info[1]), nH=int(info[2]), npj=int(info[3]),
+                    nplwex=int(info[4]))
+
+    # ---- band storage
+    def put(self, slot, psi):
+        p = f64(psi)
+        check(load().qx_slot_put(self.h, int(slot), p, p.shape[0]), "qx_slot_put")
+
+    def get(self, slot, nb):
+        out = np.zeros((nb, self.gs.nplwex))
+        check(load().qx_slot_get(self.h, int(slot), out, nb), "qx_slot_get")
+        return out
+
+    def sync(self):
+        check(load().qx_sync(self.h), "qx_sync")
+
+    # ---- operations on resident slots
+    def hpsi(self, src=0, dst=1, nb=None, prec=FP64):
+        check(load().qx_hpsi_slot(self.h, src, dst, nb, prec), "qx_hpsi_slot")
+
+    def local(self, src, dst, nb, prec=FP64, accumulate=False):
+        check(load().qx_local_slot(self.h, src, dst, nb, prec, int(accumulate)), "qx_local_slot")
+
+    def kb(self, src, dst, nb):
+        check(load().qx_kb_slot(self.h, src, dst, nb, None), "qx_kb_slot")
+
+    def schmidt(self, slot, nb):
+        ok = C.c_int(0)
+        check(load().qx_schmidt_slot(self.h, slot, nb, C.byref(ok)), "qx_schmidt_slot")
+        return bool(ok.value)
+
+    def rotate(self, psi, hpsi, nb):
+        eig = np.zeros(nb)
+        check(load().qx_rotate_slot(self.h, psi, hpsi, nb, eig), "qx_rotate_slot")
+        return eig
+
+    def density(self, slot, nb, occ, rvol, prec=FP64):
+        rho = np.zeros(self.gs.N)
+        check(load().qx_density_slot(self.h, slot, nb, f64(occ), float(rvol),
+                                     rho.ctypes.data_as(C.c_void_p), prec), "qx_density_slot")
+        return rho
+
+    def ekib(self, slot, nb):
+        out = np.zeros(nb)
+        check(load().qx_ekib_slot(self.h, slot, nb, out), "qx_ekib_slot")
+        return out
+
+    def eigen_bkl(self, nb, ekib, eig, noband=0, itermx=4, iteremx=4, tolera=1e-10, threinn=0.0,
+                  wdinner=0.9, roughfzan=0.3, roughfeig=0.3, prec=FP64, batched=False):
+        e = f64(eig).copy()
+        bz = np.zeros(nb)
+        nr = np.zeros(nb, np.int32)
+        L = load()
+        if batched:
+            check(L.qx_eigen_bkl_batched_slot(self.h, 0, 1, nb, f64(ekib), e, noband, itermx,
+                                              iteremx, tolera, roughfzan, roughfeig, bz, nr, prec),
+                  "qx_eigen_bkl_batched_slot")
+        else:
+            check(L.qx_eigen_bkl_slot(self.h, 0, 1, nb, f64(ekib), e, noband, itermx, iteremx,
+                                      tolera, threinn, wdinner, roughfzan, roughfeig, bz, nr, prec),
+                  "qx_eigen_bkl_slot")
+        return e, bz, nr
+
+    def scf_band_step(self, nb, occ, rvol, noband=0, prec=FP64, batched=False, want_rho=True):
+        eig = np.zeros(nb)
+        rho = np.zeros(self.gs.N) if want_rho else None
+        ok = C.c_int(0)
+        check(load().qx_scf_band_step_ex(self.h, 0, 1, nb, f64(occ), noband, float(rvol), eig,
+                                         rho.ctypes.data_as(C.c_void_p) if rho is not None else None,
+                                         C.byref(ok), prec, int(batched)), "qx_scf_band_step")
+        return eig, rho, bool(ok.value)
+
+    # ---- timing
+    def record(self, slot):
+        check(load().qx_event_record(self.h, slot), "qx_event_record")
+
+    def elapsed_ms(self, s0, s1):
+        ms = C.c_float(0)
+        check(load().qx_event_elapsed(self.h, s0, s1, C.byref(ms)), "qx_event_elapsed")
+        return float(ms.value)
+
+    # ---- Fortran-shaped drop-ins on host arrays
+    def hcnonloc(self, cgjr, rhcr):
+        r = f64(rhcr).copy()
+        check(load().qx_hcnonloc(self.h, f64(cgjr), r), "qx_hcnonloc")
+        return r
+
+    def hckbpp_g(self, cgjr, rhcr, want_slmir=False, lmmax=None, natom=None):
+        c = f64(cgjr)
+        r = f64(rhcr).copy()
+        nb = c.shape[0]
+        sl = np.zeros((nb, natom, lmmax)) if want_slmir else None
+        check(load().qx_hckbpp_g(self.h, c, r, nb,
+                                 sl.ctypes.data_as(C.c_void_p) if sl is not None else None),
+              "qx_hckbpp_g")
+        return (r, sl) if want_slmir else r
+
+    def schmidt_host(self, cgjr):
+        c = f64(cgjr).copy()
+        ok = C.c_int(0)
+        check(load().qx_schmidt(self.h, c, c.shape[0], C.byref(ok)), "qx_schmidt")
+        return c, bool(ok.value)
+
+    def softchg(self, cgjr, occ, rvol):
+        c = f64(cgjr)
+        rho = np.zeros(self.gs.N)
+        check(load().qx_softchg(self.h, c, c.shape[0], f64(occ), float(rvol), rho), "qx_softchg")
+        return rho
+
+
+def rfft3(key, fft3x, fft3y, nd):
+    x, y = f64(fft3x).copy(), f64(fft3y).copy()
+    ierr = C.c_int(0)
+    check(load().qx_rfft3(int(key), x, y, *[int(v) for v in nd], C.byref(ierr)), "qx_rfft3")
+    return x, y
+
+
+def fft3(key, fft3x, fft3y, nd):
+    x, y = f64(fft3x).copy(), f64(fft3y).copy()
+    ierr = C.c_int(0)
+    check(load().qx_fft3(int(key), x, y, *[int(v) for v in nd], C.byref(ierr)), "qx_fft3")
+    return x, y

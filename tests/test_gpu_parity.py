@@ -1,0 +1,193 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle on the
+same seeded inputs.  FP64 stages must agree to round-off (tolerances below
+are written per test); the mixed-precision FFT stage to FP32 accuracy."""
+import numpy as np
+import pytest
+
+import oracle as O
+from qxmd_amd import pw
+from qxmd_amd.system import (initial_wavefunctions, model_kb_projectors, model_local_potential,
+                             occupations)
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(sph, nb, kb=True):
+    gs = pw.GSphere(sph.cell.hcell)
+    assert gs.nplw == sph.nplw and gs.nd == sph.nd
+    ctx = pw.PlaneWave(gs, nb)
+    v = model_local_potential(sph, sph.cell)
+    ctx.set_glocal(v)
+    H = None
+    if kb:
+        eq, lmx, vcl, cl = model_kb_projectors(sph, sph.cell)
+        ctx.set_kb(eq, lmx, vcl)
+        H = O.Hamiltonian(sph.nplw, sph.nd, sph.ijk, v, sph.dkg, eq, lmx, vcl)
+    else:
+        H = O.Hamiltonian(sph.nplw, sph.nd, sph.ijk, v, sph.dkg)
+    return gs, ctx, H, v
+
+
+def _relerr(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def test_slot_roundtrip(si8):
+    nb = 3
+    gs, ctx, _, _ = _setup(si8, nb, kb=False)
+    psi = initial_wavefunctions(si8, nb)
+    ctx.put(0, psi)
+    back = ctx.get(0, nb)
+    assert np.array_equal(back, psi)
+
+
+@pytest.mark.parametrize("nb", [1, 5])
+def test_local_apply_fp64(si8, nb):
+    gs, ctx, H, v = _setup(si8, nb, kb=False)
+    psi = initial_wavefunctions(si8, nb)
+    ctx.put(0, psi)
+    ctx.local(0, 1, nb)
+    got = ctx.get(1, nb)
+    ref = H.hpsi(psi)  # kinetic + local (no KB)
+    assert _relerr(got, ref) < 1e-12
+
+
+def test_local_apply_mixed(si8):
+    nb = 6
+    gs, ctx, H, v = _setup(si8, nb, kb=False)
+    psi = initial_wavefunctions(si8, nb)
+    ctx.put(0, psi)
+    ctx.local(0, 1, nb, prec=pw.MIXED)
+    got = ctx.get(1, nb)
+    ref = H.hpsi(psi)
+    assert _relerr(got, ref) < 2e-6  # FP32 FFT stage
+
+
+def test_hpsi_fp64_si64(si64):
+    nb = 12
+    gs, ctx, H, v = _setup(si64, nb)
+    psi = initial_wavefunctions(si64, nb)
+    ctx.put(0, psi)
+    ctx.hpsi(0, 1, nb)
+    got = ctx.get(1, nb)
+    ref = H.hpsi(psi)
+    assert _relerr(got, ref) < 1e-12
+
+
+def test_hckbpp_dropin(si8):
+    nb = 4
+    gs, ctx, H, v = _setup(si8, nb)
+    eq, lmx, vcl, cl = model_kb_projectors(si8, si8.cell)
+    psi = initial_wavefunctions(si8, nb)
+    r0 = initial_wavefunctions(si8, nb, seed=9.0)
+    got, sl = ctx.hckbpp_g(psi, r0, want_slmir=True, lmmax=eq.shape[1], natom=eq.shape[0])
+    ref, slr = O.hckbpp_g(eq, lmx, vcl, psi, r0)
+    assert _relerr(got, ref) < 1e-12
+    assert _relerr(sl, slr) < 1e-12
+
+
+def test_hcnonloc_dropin(si8):
+    gs, ctx, H, v = _setup(si8, 2, kb=False)
+    psi = initial_wavefunctions(si8, 1)[0]
+    r0 = initial_wavefunctions(si8, 1, seed=3.0)[0]
+    got = ctx.hcnonloc(psi, r0)
+    ref = O.hcnonloc(v, psi, r0, si8.nplw, si8.ijk, si8.nd)
+    assert _relerr(got, ref) < 1e-12
+
+
+def test_schmidt(si8):
+    nb = 20
+    gs, ctx, H, v = _setup(si8, nb, kb=False)
+    psi = initial_wavefunctions(si8, nb)
+    got, ok = ctx.schmidt_host(psi)
+    ref, okr = O.schmidt(psi)
+    assert ok and okr
+    assert _relerr(got, ref) < 1e-10
+
+
+def test_schmidt_not_positive_definite(si8):
+    nb = 4
+    gs, ctx, H, v = _setup(si8, nb, kb=False)
+    psi = initial_wavefunctions(si8, nb)
+    psi[3] = psi[1]  # linearly dependent set -> lcholesky = .false.
+    got, ok = ctx.schmidt_host(psi)
+    assert not ok
+    assert np.array_equal(got, psi)  # left untouched
+
+
+def test_rotation(si8):
+    nb = 16
+    gs, ctx, H, v = _setup(si8, nb)
+    psi, _ = O.schmidt(initial_wavefunctions(si8, nb))
+    ctx.put(0, psi)
+    ctx.hpsi(0, 1, nb)
+    eig = ctx.rotate(0, 1, nb)
+    hp = H.hpsi(psi)
+    hm, _ = O.hmatrix(psi, hp)
+    w, u = O.eigh(hm)
+    np.testing.assert_allclose(eig, w, rtol=0, atol=1e-11)
+    p2 = ctx.get(0, nb)
+    hp2 = ctx.get(1, nb)
+    hm2, e2 = O.hmatrix(p2, hp2)
+    assert np.abs(hm2 - np.diag(w)).max() < 1e-10  # rotated basis diagonalises H
+
+
+def test_density(si8):
+    nb = 10
+    gs, ctx, H, v = _setup(si8, nb, kb=False)
+    psi, _ = O.schmidt(initial_wavefunctions(si8, nb))
+    occ = occupations(nb, 16)
+    rho = ctx.softchg(psi, occ, si8.cell.volume)
+    ref = O.softchg(psi, si8.nplw, occ, si8.cell.volume, si8.ijk, si8.nd)
+    assert _relerr(rho, ref) < 1e-12
+
+
+@pytest.mark.parametrize("nd", [(30, 30, 30), (6, 4, 5), (60, 60, 60)])
+def test_rfft3_dropin(nd):
+    rng = np.random.default_rng(7)
+    N = int(np.prod(nd))
+    x = rng.standard_normal(N)
+    fx, fy = pw.rfft3(1, x, np.zeros(N), nd)
+    rx, ry = O.rfft3(1, x, np.zeros(N), nd)
+    assert np.abs(fx - rx).max() < 1e-14 and np.abs(fy - ry).max() < 1e-14
+    bx, by = pw.rfft3(2, rx, ry, nd)
+    ox, oy = O.rfft3(2, rx, ry, nd)
+    assert np.abs(bx - ox).max() < 1e-11 and not by.any()
+    z = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    cx, cy = pw.fft3(1, z.real, z.imag, nd)
+    qx, qy = O.fft3(1, z.real, z.imag, nd)
+    assert np.abs(cx - qx).max() < 1e-14 and np.abs(cy - qy).max() < 1e-14
+
+
+def test_bkl_band_by_band_fp64(si8):
+    nb = 12
+    gs, ctx, H, v = _setup(si8, nb)
+    psi, _ = O.schmidt(initial_wavefunctions(si8, nb))
+    hp = H.hpsi(psi)
+    hm, _ = O.hmatrix(psi, hp)
+    w, u = O.eigh(hm)
+    psi = O.untrycore(psi, u)
+    hp = O.untrycore(hp, u)
+    ekib = O.ekinetic(si8.dkg, psi)[1]
+    e_ref, p_ref, h_ref, bz_ref, nr_ref = H.eigen_bkl(w, psi, hp, ekib)
+    ctx.put(0, psi)
+    ctx.put(1, hp)
+    e, bz, nr = ctx.eigen_bkl(nb, ekib, w)
+    np.testing.assert_array_equal(nr, nr_ref)
+    np.testing.assert_allclose(e, e_ref, rtol=0, atol=1e-11)
+    assert _relerr(ctx.get(0, nb), p_ref) < 1e-9
+    assert _relerr(ctx.get(1, nb), h_ref) < 1e-9
+
+
+def test_scf_band_step_fp64(si8):
+    nb = 20
+    gs, ctx, H, v = _setup(si8, nb)
+    psi, _ = O.schmidt(initial_wavefunctions(si8, nb))
+    occ = occupations(nb, 32)
+    ref = H.scf_band_step(psi, occ, si8.cell.volume)
+    ctx.put(0, psi)
+    eig, rho, ok = ctx.scf_band_step(nb, occ, si8.cell.volume)
+    assert ok and ref["ok"]
+    # eigenvalues after CG: Ry -> Ha tolerance 1e-6 Ha per eigenvalue
+    np.testing.assert_allclose(eig / 2, ref["eig"] / 2, rtol=0, atol=1e-8)
+    assert _relerr(rho, ref["rho"]) < 1e-8
