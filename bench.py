@@ -1,0 +1,227 @@
+#!/usr/bin/env python
+"""Benchmark: Gamma-point H|psi> band-applies/s and SCF band-step time on the
+216-atom Si cell (3x3x3 diamond, a = 30.78 bohr, 20/80 Ry, 90^3 grid, 21909
+half-sphere plane waves, 540 bands, 864 KB projectors), mixed precision vs FP64.
+
+A step is one batched H|psi> over all 540 bands (540 band-applies: kinetic,
+sphere-pruned FFT pair with V(r), KB projection and back-projection) with the
+inputs resident in HBM.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--ncell 3] [--no-scf] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+
+
+def build_inputs(ncell, nband):
+    from qxmd_amd import pw
+    from qxmd_amd.system import (diamond_si, initial_wavefunctions, model_kb_projectors,
+                                 model_local_potential)
+    cell = diamond_si(ncell)
+    gs = pw.GSphere(cell.hcell)
+    v = model_local_potential(gs, cell)
+    eq, lmx, vcl, cl = model_kb_projectors(gs, cell)
+    psi = initial_wavefunctions(gs, nband)
+    return cell, gs, v, eq, lmx, vcl, cl, psi
+
+
+def fft_stage_bytes(gs, s):
+    """Algorithmic FFT-stage bytes per band-apply: 8*N_h*s_c + N*s_r + 2*(nplw+1)*s_c
+    (two real 3-D transforms as two HBM passes each, V(r) read, sphere I/O)."""
+    n1, n2, n3 = gs.nd
+    N = n1 * n2 * n3
+    Nh = (n1 // 2 + 1) * n2 * n3
+    return 8 * Nh * 2 * s + N * s + 2 * (gs.nplw + 1) * 2 * s
+
+
+def time_region(ctx, fn, reps):
+    ctx.sync()
+    ctx.record(0)
+    for _ in range(reps):
+        fn()
+    ctx.record(1)
+    ctx.sync()
+    return ctx.elapsed_ms(0, 1) / reps
+
+
+def cpu_worker(ncell, budget_s):
+    """Oracle H|psi> (FP64) on the host for as many bands as fit the budget."""
+    import oracle as O
+    cell, gs, v, eq, lmx, vcl, cl, psi = build_inputs(ncell, 64)
+    H = O.Hamiltonian(gs.nplw, gs.nd, gs.ijkgd(), v, gs.dkgnrm(), eq, lmx, vcl)
+    H.hpsi(psi[:1])  # warm the twiddle tables
+    t0 = time.perf_counter()
+    n = 0
+    while n < 64:
+        H.hpsi(psi[n:n + 2])
+        n += 2
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    print(json.dumps({"bands": n, "seconds": dt}))
+
+
+def cpu_baseline(ncell, threads, budget_s=12.0):
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker",
+                        "--ncell", str(ncell), "--cpu-budget", str(budget_s)],
+                       env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       timeout=600)
+    if r.returncode != 0:
+        return None
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    return d
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ncell", type=int, default=3)
+    ap.add_argument("--nband", type=int, default=0)
+    ap.add_argument("--no-scf", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-worker", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.cpu_worker:
+        cpu_worker(args.ncell, args.cpu_budget)
+        return
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs the single-GPU path only (one rank, one MI355X); "
+              "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    from qxmd_amd import pw
+    from qxmd_amd.system import occupations
+    natom = 8 * args.ncell ** 3
+    nband = args.nband or int(2.5 * natom)  # 20 / 160 / 540 / 1280
+    nel = 4 * natom
+    t_setup = time.perf_counter()
+    cell, gs, v, eq, lmx, vcl, cl, psi0 = build_inputs(args.ncell, nband)
+    ctx = pw.PlaneWave(gs, nband)
+    ctx.set_glocal(v)
+    ctx.set_kb(eq, lmx, vcl)
+    ctx.put(0, psi0)
+    assert ctx.schmidt(0, nband)
+    psi_orth = ctx.get(0, nband)
+    t_setup = time.perf_counter() - t_setup
+    print(f"# setup {t_setup:.1f}s: Si{natom}, grid {gs.nd}, nplw {gs.nplw}, nband {nband}, "
+          f"npj {ctx.npj}", file=sys.stderr, flush=True)
+
+    def hpsi(prec):
+        return lambda: ctx.hpsi(0, 1, nband, prec)
+
+    # warmup, then the timed region: K full H|psi> passes, mixed precision
+    for _ in range(args.warmup):
+        hpsi(pw.MIXED)()
+        hpsi(pw.FP64)()
+    ms_mixed = time_region(ctx, hpsi(pw.MIXED), args.steps)
+    print(f"# H|psi> mixed: {ms_mixed:.3f} ms/step", file=sys.stderr, flush=True)
+    ms_fp64 = time_region(ctx, hpsi(pw.FP64), args.steps)
+    print(f"# H|psi> fp64 : {ms_fp64:.3f} ms/step", file=sys.stderr, flush=True)
+    # stage timings (same stream, HIP events)
+    reps = max(3, args.steps // 2)
+    ms_loc32 = time_region(ctx, lambda: ctx.local(0, 1, nband, pw.MIXED), reps)
+    ms_loc64 = time_region(ctx, lambda: ctx.local(0, 1, nband, pw.FP64), reps)
+    ms_kb = time_region(ctx, lambda: ctx.kb(0, 1, nband), reps)
+    print(f"# stages: local fp32 {ms_loc32:.3f} ms, local fp64 {ms_loc64:.3f} ms, "
+          f"KB fp64 {ms_kb:.3f} ms", file=sys.stderr, flush=True)
+
+    # roofline of the dominant stage of the mixed step
+    nplwex = gs.nplwex
+    kb_flops = 4.0 * ctx.npj * nplwex * nband
+    fft_bytes32 = fft_stage_bytes(gs, 4) * nband
+    if ms_kb >= ms_loc32:
+        ach = kb_flops / (ms_kb * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": FP64_PEAK_TFS,
+                "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFS, 4), "traffic": None,
+                "kernel": "k_gemm_f64 (KB projection + back-projection)"}
+    else:
+        ach = fft_bytes32 / (ms_loc32 * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_zinv+k_plane+k_zfwd (FP32 local band-apply)"}
+
+    extra = {}
+    if not args.no_scf:
+        occ = occupations(nband, nel)
+        res = {}
+        for name, prec, batched in (("fp64_batched", pw.FP64, True),
+                                    ("mixed_batched", pw.MIXED, True),
+                                    ("fp64_band_by_band", pw.FP64, False)):
+            ctx.put(0, psi_orth)
+            ctx.sync()
+            t0 = time.perf_counter()
+            eig, rho, ok = ctx.scf_band_step(nband, occ, cell.volume, prec=prec, batched=batched,
+                                             want_rho=True)
+            dt = time.perf_counter() - t0
+            res[name] = (dt, eig, ok)
+            print(f"# SCF band step {name}: {dt * 1e3:.1f} ms ok={ok}", file=sys.stderr,
+                  flush=True)
+        e64 = np.sum(occ * np.sort(res["fp64_batched"][1])) / 2.0
+        emx = np.sum(occ * np.sort(res["mixed_batched"][1])) / 2.0
+        extra = {
+            "scf_step_ms": {k: round(v[0] * 1e3, 2) for k, v in res.items()},
+            "mixed_vs_fp64_band_energy_err_ha_per_atom": float(abs(emx - e64) / natom),
+            "mixed_vs_fp64_max_eig_err_ha": float(np.abs(np.sort(res["mixed_batched"][1]) -
+                                                         np.sort(res["fp64_batched"][1])).max() / 2),
+        }
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        d = cpu_baseline(args.ncell, threads)
+        if d:
+            cpu = {"value": round(d["bands"] / d["seconds"], 3), "unit": "band-applies/s",
+                   "cores": threads, "kind": "port",
+                   "sample": f"oracle FP64 H|psi> (C, OpenMP) on {d['bands']} bands of the same "
+                             f"Si{natom} cell, {d['seconds']:.1f} s"}
+
+    value = nband / (ms_mixed * 1e-3)
+    line = {
+        "metric": "H|psi> band-applies/s + SCF step time, Si216 mixed vs FP64 at 1e-6 Ha/atom",
+        "value": round(value, 1),
+        "unit": "band-applies/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_mixed, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 FFT + fp64 MFMA (mixed)",
+        "data": "synthetic",
+        "config": {"workload": f"Si{natom} Gamma H|psi>, {nband} bands, grid {gs.nd[0]}^3, "
+                               f"nplw {gs.nplw}, {ctx.npj} KB projectors",
+                   "natom": natom, "nband": nband, "grid": list(gs.nd), "nplw": gs.nplw,
+                   "npj": ctx.npj, "ecut_ry": 20.0, "ecutdens_ry": 80.0},
+        "fp64_band_applies_per_s": round(nband / (ms_fp64 * 1e-3), 1),
+        "fp64_ms_per_step": round(ms_fp64, 4),
+        "stage_ms": {"local_fp32": round(ms_loc32, 4), "local_fp64": round(ms_loc64, 4),
+                     "kb_fp64": round(ms_kb, 4)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -231,7 +232,9 @@ int qx_create(qx_ctx** out, int n1, int n2, int n3, int nplw, const int* nga, co
   // pencil intermediates: a batch of band pairs, FP64 complex
   const size_t per_bp = 16 * (size_t)n3 * c->pc.ncol;
   const int nbp = c->nbmax / 2;
-  c->batch_bp = std::max(1, std::min(nbp, (int)std::max<size_t>(1, (size_t)(512u << 20) / per_bp)));
+  size_t cap_bytes = (size_t)512 << 20;  // pencil intermediates per buffer
+  if (const char* e = std::getenv("QX_PENCIL_MB")) cap_bytes = (size_t)std::max(1, std::atoi(e)) << 20;
+  c->batch_bp = std::max(1, std::min(nbp, (int)std::max<size_t>(1, cap_bytes / per_bp)));
   c->t_bytes = per_bp * c->batch_bp;
   QX_CHECK(hipMalloc(&c->d_t1, c->t_bytes));
   QX_CHECK(hipMalloc(&c->d_t2, c->t_bytes));

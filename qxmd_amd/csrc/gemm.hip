@@ -83,7 +83,8 @@ __global__ void __launch_bounds__(256) k_gemm_f64(int M, int N, int K, int kchun
     }
     __syncthreads();
   }
-  // ---- epilogue: acc[i][j][v] = C(m = .. + 4*(lane/16) + v, n = .. + lane%16)
+  // ---- epilogue: v_mfma_f64_16x16x4 leaves D(i, j) in lane j + 16*(i%4),
+  // register i/4, i.e. acc[i][j][v] = C(m = .. + lane/16 + 4*v, n = .. + lane%16)
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -92,7 +93,7 @@ __global__ void __launch_bounds__(256) k_gemm_f64(int M, int N, int K, int kchun
       if (n >= N) continue;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        int m = m0 + wm + i * 16 + 4 * (lane >> 4) + v;
+        int m = m0 + wm + i * 16 + (lane >> 4) + 4 * v;
         if (m >= M) continue;
         double val = alpha * acc[i][j][v];
         if (SPLIT) atomicAdd(&C[(size_t)n * ldc + m], val);

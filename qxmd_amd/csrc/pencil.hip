@@ -454,14 +454,15 @@ static int density_t(Ctx* c, const double* psi, int nb, const double* w) {
   lds_optin(k_zinv<T>, lz);
   lds_optin(k_plane_rho<T>, lp);
   int nparts = 0;
+  const int nbatch = (nbp + batch - 1) / batch;
+  const int groups_per_batch = std::max(1, c->rho_groups / nbatch);
+  if (groups_per_batch * nbatch > c->rho_groups) return -4;
   for (int bp0 = 0; bp0 < nbp; bp0 += batch) {
     const int nbb = std::min(batch, nbp - bp0);
     PencilArgs a = make_args(c, nb, bp0);
     a.rows = rows;
     hipLaunchKernelGGL(k_zinv<T>, dim3(gz.x, nbb), dim3(kZThreads), lz, c->stream, a, psi, t1);
-    const int groups_max = c->rho_groups - nparts;
-    if (groups_max < 1) return -4;
-    int ng = std::min(groups_max, nbb);
+    int ng = std::min(groups_per_batch, nbb);
     int per = (nbb + ng - 1) / ng;
     ng = (nbb + per - 1) / per;
     hipLaunchKernelGGL(k_plane_rho<T>, dim3(c->n3, ng), dim3(kPlaneThreads), lp, c->stream, a, t1,

@@ -28,13 +28,29 @@ class Lehmer:
     def __init__(self, seed: float):
         self.x = float(seed)
 
+    M = 2147483647
+    A = 16807
+
     def uniform(self, n: int) -> np.ndarray:
+        """n draws.  16807*x < 2^46 is exact in FP64, so the FP64 fmod of the
+        reference equals integer arithmetic; blocks of draws are generated
+        from the multipliers 16807^i mod m (exact in int64)."""
         out = np.empty(n)
-        x = self.x
-        for i in range(n):
-            x = math.fmod(16807.0 * x, 2147483647.0)
-            out[i] = x / 2147483647.0
-        self.x = x
+        x = int(self.x)
+        blk = 4096
+        mult = np.empty(blk, np.int64)
+        a = 1
+        for i in range(blk):
+            a = (a * self.A) % self.M
+            mult[i] = a
+        done = 0
+        while done < n:
+            k = min(blk, n - done)
+            xs = (mult[:k] * x) % self.M
+            out[done:done + k] = xs / float(self.M)
+            x = int(xs[k - 1])
+            done += k
+        self.x = float(x)
         return out
 
     def gauss(self, n: int) -> np.ndarray:
@@ -110,17 +126,21 @@ def model_local_potential(gs, cell: SiCell, depth: float = 1.2, width: float = 1
     depth and width at every atom plus a constant.  Stands in for
     V_H + V_xc + V_loc,ps (glocal), which the SCF glue assembles."""
     n1, n2, n3 = gs.nd
-    grid = np.stack(np.meshgrid(np.arange(n1) / n1, np.arange(n2) / n2, np.arange(n3) / n3,
-                                indexing="ij"), axis=-1)  # (n1,n2,n3,3) fractional
-    r = grid @ cell.hcell.T  # Cartesian
-    v = np.full((n1, n2, n3), shift)
     L = np.diag(cell.hcell)
+    if np.abs(cell.hcell - np.diag(L)).max() > 0:
+        raise ValueError("model_local_potential expects an orthorhombic cell")
+    axes = [np.arange(n) / n * L[i] for i, n in enumerate((n1, n2, n3))]
+    v = np.full((n3, n2, n1), shift)
     for t in cell.tau:
-        d = r - t
-        d -= np.round(d / L) * L  # minimum image (orthorhombic cells)
-        v -= depth * np.exp(-np.sum(d * d, axis=-1) / (2.0 * width * width))
-    # x fastest: ijk = i + n1*(j + n2*k)
-    return np.ascontiguousarray(np.transpose(v, (2, 1, 0)).ravel())
+        # minimum image per axis; the Gaussian factorises over x, y, z
+        e = []
+        for i in range(3):
+            d = axes[i] - t[i]
+            d -= np.round(d / L[i]) * L[i]
+            e.append(np.exp(-d * d / (2.0 * width * width)))
+        v -= depth * (e[2][:, None, None] * e[1][None, :, None] * e[0][None, None, :])
+    # stored (z, y, x): x fastest, ijk = i + n1*(j + n2*k)
+    return np.ascontiguousarray(v.ravel())
 
 
 def model_kb_projectors(gs, cell: SiCell, rc: float = 1.6, lmmax: int = 4,

@@ -191,3 +191,19 @@ def test_scf_band_step_fp64(si8):
     # eigenvalues after CG: Ry -> Ha tolerance 1e-6 Ha per eigenvalue
     np.testing.assert_allclose(eig / 2, ref["eig"] / 2, rtol=0, atol=1e-8)
     assert _relerr(rho, ref["rho"]) < 1e-8
+
+
+def test_multi_batch_pencil(si64, monkeypatch):
+    """Force several band-pair batches through the pencil buffers (1 MiB cap):
+    local apply and density must not depend on the batching."""
+    monkeypatch.setenv("QX_PENCIL_MB", "1")
+    nb = 13
+    gs, ctx, H, v = _setup(si64, nb, kb=False)
+    psi, _ = O.schmidt(initial_wavefunctions(si64, nb))
+    ctx.put(0, psi)
+    ctx.local(0, 1, nb)
+    assert _relerr(ctx.get(1, nb), H.hpsi(psi)) < 1e-12
+    occ = occupations(nb, 20)
+    rho = ctx.density(0, nb, occ, si64.cell.volume)
+    ref = O.softchg(psi, si64.nplw, occ, si64.cell.volume, si64.ijk, si64.nd)
+    assert _relerr(rho, ref) < 1e-12
