@@ -114,6 +114,13 @@ int qx_hckbpp_g(qx_ctx *ctx, const double *cgjr, double *rhcr, int nb, double *s
 int qx_hckbpp_ib_g(qx_ctx *ctx, const double *cgjr, double *rhcr);
 /* schmidt (Sources/schmidt.f90:71-112) on host cgjr(nplwex, nband). */
 int qx_schmidt(qx_ctx *ctx, double *cgjr, int nband, int *lcholesky);
+/* RsEIGQR with ifvec=1, ifsrt=2 (Sources/funcs.f90:380-764): a(n,n) column-major,
+ * lower triangle used; eigenvectors returned in the columns of a, w ascending.
+ * Runs in FP64 on the device; n <= nbmax of the context. */
+int qx_rseigqr(qx_ctx *ctx, double *a, int n, double *w);
+/* icholesky = rsdecmp + rsinvcd (Sources/schmidt.f90:383-544): s(n,n) column-major
+ * lower triangle -> L^-1 (lower, upper zeroed); ok = 1 when positive definite. */
+int qx_icholesky(qx_ctx *ctx, double *s, int n, int *ok);
 /* softchg (Sources/chgdns.f90:388-454) from host cgjr(nplwex, nb). */
 int qx_softchg(qx_ctx *ctx, const double *cgjr, int nb, const double *occ, double rvol,
                double *rho);

@@ -16,7 +16,7 @@ void get_mesh(const double* hcell, double ecutdens, int* nd);
 void ijkgd_map(int nplw5, const int* nga, const int* ngb, const int* ngc, const int* nd, int* ijkgd);
 int fullgrid_rfft3(int key, double* fx, double* fy, int n1, int n2, int n3);
 int fullgrid_fft3(int key, double* fx, double* fy, int n1, int n2, int n3);
-int dense_chol_inv(Ctx* c, double* s, int n, int* ok_dev);
+int dense_chol_inv(Ctx* c, double* s, int n, double* x, int* ok, int* flag);
 int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int* info);
 int dense_symmetrize(Ctx* c, const double* C, int n, double* H, double* eig);
 int pdot_batched(Ctx* c, const double* x, const double* y, int nb, double* out_dev);
@@ -63,13 +63,6 @@ __global__ void k_to_reference(const double* __restrict__ in, double* __restrict
 
 static unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
-__global__ void k_transpose(const double* a, double* t, int n) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long)n * n) return;
-  int r = (int)(i % n), col = (int)(i / n);
-  t[(size_t)r * n + col] = a[(size_t)col * n + r];
-}
-
 // acc += (T+V) psi - T psi  (the local part alone, hcnonloc)
 __global__ void k_add_minus_kin(double* acc, const double* tv, const double* psi, const double* kin,
                                 int n) {
@@ -106,24 +99,22 @@ int hpsi_batched(Ctx* c, const double* x, double* hx, int nb, int prec) {
 // Gram-Schmidt by Cholesky of the packed overlap (schmidt2)
 static int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host) {
   const int n = c->nplwex;
-  double* S = c->d_small;                        // nb x nb (+ nb scratch column)
-  double* Linv = c->d_small + (size_t)(nb + 1) * nb;
+  double* S = c->d_small;                         // nb x nb overlap, then L
+  double* Linv = c->d_small + (size_t)(nb + 1) * nb;  // row-major L^-1
   int* ok = reinterpret_cast<int*>(c->d_scal + 62 * (size_t)c->nbmax);
-  // S[i][j] = <psi_j|psi_i> (C[n=i][m=j]); column-major S(i,j) has the same
-  // storage for a symmetric matrix.
+  int* flag = ok + 2;
+  // S[i][j] = <psi_j|psi_i> (C[n=i][m=j]); symmetric, so its storage is also
+  // the column-major S(i,j).
   int rc = gemm_tn(c, nb, nb, n, 2.0, psi, n, psi, n, 0.0, S, nb, 1);
   if (rc) return rc;
-  if ((rc = dense_chol_inv(c, S, nb, ok))) return rc;
+  if ((rc = dense_chol_inv(c, S, nb, Linv, ok, flag))) return rc;
   int okh = 0;
   QX_CHECK(hipMemcpyAsync(&okh, ok, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   QX_CHECK(hipStreamSynchronize(c->stream));
   *ok_host = okh;
   if (!okh) return 0;
-  // psi'_ib = sum_jb Linv(ib,jb) psi_jb: B(n=ib,k=jb) = Linv(ib,jb) = S[ib + nb*jb]
-  // -> stored column-major, so use it as B with ldb via a transposed copy
-  // gemm_nn wants B[n*ldb + k] = Linv(n,k), the transpose of the column-major factor
+  // psi'_ib = sum_jb Linv(ib,jb) psi_jb: gemm_nn's B[n*ldb + k] = Linv(n,k)
   double* out = c->d_slot[5];
-  hipLaunchKernelGGL(k_transpose, dim3(cdiv((long)nb * nb, 256)), dim3(256), 0, c->stream, S, Linv, nb);
   if ((rc = gemm_nn(c, n, nb, nb, 1.0, psi, n, Linv, nb, 0.0, out, n))) return rc;
   QX_CHECK(hipMemcpyAsync(psi, out, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice,
                           c->stream));
@@ -152,11 +143,8 @@ static int rotate_dev(Ctx* c, double* psi, double* hpsi, int nb, double* eig_hos
   QX_CHECK(hipMemcpyAsync(hpsi, out, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice,
                           c->stream));
   if (eig_host) {
-    int infoh = 0;
     QX_CHECK(hipMemcpyAsync(eig_host, w, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
-    QX_CHECK(hipMemcpyAsync(&infoh, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     QX_CHECK(hipStreamSynchronize(c->stream));
-    if (infoh) return fail(-7, "subspace eigensolver did not converge");
   }
   return 0;
 }
@@ -558,6 +546,53 @@ int qx_softchg(qx_ctx* h, const double* cgjr, int nb, const double* occ, double 
   int rc = put_internal(c, c->d_slot[0], cgjr, nb);
   if (rc) return rc;
   return qx_density_slot(h, 0, nb, occ, rvol, rho, PREC_FP64);
+}
+
+
+// RsEIGQR with ifvec=1, ifsrt=2 (Sources/funcs.f90:380-764) on a host matrix:
+// a(n,n) column-major (lower triangle used) -> eigenvectors in its columns,
+// w ascending.
+int qx_rseigqr(qx_ctx* h, double* a, int n, double* w) {
+  Ctx* c = &h->c;
+  if (n < 1 || n > c->nbmax) return fail(-1, "qx_rseigqr: need 1 <= n <= nbmax");
+  double* C = c->d_small;
+  double* H = C + (size_t)n * n;
+  double* Z = H + (size_t)n * n;
+  double* work = Z + (size_t)n * n;
+  double* wd = c->d_scal + 40 * (size_t)c->nbmax;
+  // symmetric fill from the lower triangle: C[hi*n+lo] = a(hi, lo)
+  std::vector<double> ct((size_t)n * n);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) ct[(size_t)(i > j ? i : j) * n + (i > j ? j : i)] = a[i + (size_t)n * j];
+  QX_CHECK(hipMemcpyAsync(C, ct.data(), sizeof(double) * ct.size(), hipMemcpyHostToDevice, c->stream));
+  int rc = dense_symmetrize(c, C, n, H, nullptr);
+  if (rc) return rc;
+  if ((rc = dense_eigh(c, H, n, Z, wd, work, nullptr))) return rc;
+  QX_CHECK(hipMemcpyAsync(a, Z, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
+  QX_CHECK(hipMemcpyAsync(w, wd, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// icholesky (Sources/schmidt.f90:383-423): s(n,n) column-major, lower triangle
+// of a symmetric positive-definite matrix -> L^-1 in the lower triangle (upper
+// zeroed); *ok = 1 when the matrix was positive definite.
+int qx_icholesky(qx_ctx* h, double* s, int n, int* ok) {
+  Ctx* c = &h->c;
+  if (n < 1 || n > c->nbmax) return fail(-1, "qx_icholesky: need 1 <= n <= nbmax");
+  double* S = c->d_small;
+  double* X = c->d_small + (size_t)(n + 1) * n;
+  int* okd = reinterpret_cast<int*>(c->d_scal + 62 * (size_t)c->nbmax);
+  QX_CHECK(hipMemcpyAsync(S, s, sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice, c->stream));
+  int rc = dense_chol_inv(c, S, n, X, okd, okd + 2);
+  if (rc) return rc;
+  std::vector<double> xr((size_t)n * n);
+  QX_CHECK(hipMemcpyAsync(xr.data(), X, sizeof(double) * xr.size(), hipMemcpyDeviceToHost, c->stream));
+  QX_CHECK(hipMemcpyAsync(ok, okd, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) s[i + (size_t)n * j] = xr[(size_t)i * n + j];  // row-major -> col-major
+  return 0;
 }
 
 }  // extern "C"

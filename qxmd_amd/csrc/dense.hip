@@ -1,22 +1,28 @@
-// dense.hip -- FP64 small dense linear algebra on the GPU for the band update:
-//   * Cholesky S = L L^T and the explicit inverse L^-1 (the steps of rsdecmp
-//     and rsinvcd, Sources/schmidt.f90:428-544), with the positive-definite
-//     check of schmidt2 (Sources/schmidt.f90:256-267);
-//   * the symmetric eigenproblem of the subspace Hamiltonian (role of
+// dense.hip -- FP64 dense linear algebra of the band update, on the GPU.
+//
+//   * Cholesky S = L L^T and the explicit inverse L^-1 (the role of rsdecmp /
+//     rsinvcd, Sources/schmidt.f90:428-544) with the positive-definite check of
+//     schmidt2 (Sources/schmidt.f90:256-267), blocked by 64: the diagonal block
+//     is factored and inverted in LDS, the panel below and the trailing matrix
+//     are updated by many workgroups, and L^-1 is assembled block row by block
+//     row, X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj.
+//   * The symmetric eigenproblem of the subspace Hamiltonian (the role of
 //     RsEIGQR, Sources/funcs.f90:380-764): Householder tridiagonalisation,
-//     implicit QL with Wilkinson shifts, back-transformation, and the
-//     reference's stable ascending sort.
-// Matrices here are nb x nb with nb <= a few thousand; each routine runs in a
-// single 1024-thread workgroup on L2-resident data (latency-bound stage).
+//     eigenvalues by bisection on Sturm counts (one thread per eigenvalue),
+//     eigenvectors by inverse iteration with modified Gram-Schmidt inside each
+//     cluster of close eigenvalues (one workgroup per cluster), and the
+//     back-transformation by the stored reflectors (parallel over columns).
+//     Eigenvalues come out ascending with their vectors, as RsEIGQR's sort.
 #include <cmath>
+#include <vector>
 #include "context.h"
 
 namespace qx {
 
+constexpr int NBK = 64;           // block size of the Cholesky / inverse
 constexpr int kDenseThreads = 1024;
 
 __device__ inline double block_sum(double v, double* red) {
-  // wave reduction then across waves
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   __syncthreads();
@@ -31,254 +37,539 @@ __device__ inline double block_sum(double v, double* red) {
   return red[32];
 }
 
-// S (column-major, lower triangle valid) -> L^-1 in the lower triangle, upper
-// zeroed.  ok[0] = 1 if every diagonal of L^-1 exceeds 1e-30 (lcholesky).
-// S(i,j) at s[i + n*j].
-__global__ void __launch_bounds__(kDenseThreads) k_chol_inv(double* s, int n, int* ok) {
-  __shared__ double red[40];
-  const double zero = 1e-30;
-#define S_(i, j) s[(size_t)(i) + (size_t)n * (j)]
-  // ---- rsdecmp: column i of L from the already-finished columns k < i
-  for (int i = 0; i < n; ++i) {
-    double part = 0.0;
-    for (int k = threadIdx.x; k < i; k += blockDim.x) part += S_(i, k) * S_(i, k);
-    double d = S_(i, i) - block_sum(part, red);
-    const bool pos = d > zero;
-    double lii = pos ? sqrt(d) : 0.0;
-    for (int j = i + 1 + threadIdx.x; j < n; j += blockDim.x) {
-      if (pos) {
-        double t = S_(j, i);
-        for (int k = 0; k < i; ++k) t -= S_(j, k) * S_(i, k);
-        S_(j, i) = t / lii;
+// ===========================================================================
+// Cholesky and inverse.  S(i,j) at s[i + n*j] (lower triangle used).
+// X = L^-1 is written row-major: X(i,j) at x[i*n + j], zero above the diagonal.
+// ===========================================================================
+
+// factor the diagonal block [k0, k0+kb): L_kk into S (lower), L_kk^-1 into X
+__global__ void __launch_bounds__(256) k_potrf_blk(double* s, int n, int k0, int kb, double* x,
+                                                   int* fail) {
+  __shared__ double a[NBK][NBK + 1];
+  __shared__ double xi[NBK][NBK + 1];
+  const int t = threadIdx.x;
+  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
+    int i = idx % kb, j = idx / kb;
+    a[i][j] = (i >= j) ? s[(size_t)(k0 + i) + (size_t)n * (k0 + j)] : 0.0;
+  }
+  __syncthreads();
+  for (int j = 0; j < kb; ++j) {
+    if (t == 0) {
+      double d = a[j][j];
+      if (d > 1e-30) {
+        a[j][j] = sqrt(d);
       } else {
-        S_(j, i) = 0.0;
+        a[j][j] = 0.0;
+        fail[0] = 1;
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0) S_(i, i) = lii;
+    const double ljj = a[j][j];
+    for (int i = j + 1 + t; i < kb; i += blockDim.x) a[i][j] = ljj > 0.0 ? a[i][j] / ljj : 0.0;
+    __syncthreads();
+    const int m = kb - j - 1;
+    for (int idx = t; idx < m * m; idx += blockDim.x) {
+      int i = j + 1 + idx % m, c = j + 1 + idx / m;
+      if (i >= c) a[i][c] -= a[i][j] * a[c][j];
+    }
     __syncthreads();
   }
-  // ---- rsinvcd: scale columns, then columns of the inverse from the right
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    double d = S_(i, i);
-    if (fabs(d) > zero) {
-      double r = 1.0 / d;
-      for (int j = i + 1; j < n; ++j) S_(j, i) *= r;
-      S_(i, i) = r;
+  // inverse of the lower factor: column c by forward substitution
+  if (t < kb) {
+    const int c = t;
+    for (int i = 0; i < kb; ++i) {
+      if (i < c) {
+        xi[i][c] = 0.0;
+        continue;
+      }
+      double v = (i == c) ? 1.0 : 0.0;
+      for (int k = c; k < i; ++k) v -= a[i][k] * xi[k][c];
+      xi[i][c] = a[i][i] > 0.0 ? v / a[i][i] : 0.0;
     }
   }
   __syncthreads();
-  for (int j = n - 2; j >= 0; --j) {
-    // work(i) = sum_{k=j+1}^{n-1} S(i,k) S(k,j), i > j; S(i,k) = 0 for k > i
-    for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) {
-      double w = 0.0;
-      for (int k = j + 1; k <= i; ++k) w += S_(i, k) * S_(k, j);
-      // columns k > j are final; S(k,j) for k > j is read by other threads
-      // before this column is overwritten below
-      s[(size_t)n * n + i] = w;  // scratch column after the matrix
-    }
-    __syncthreads();
-    for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) S_(i, j) = -s[(size_t)n * n + i];
-    __syncthreads();
+  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
+    int i = idx % kb, j = idx / kb;
+    if (i >= j) s[(size_t)(k0 + i) + (size_t)n * (k0 + j)] = a[i][j];
+    x[(size_t)(k0 + i) * n + (k0 + j)] = xi[i][j];
   }
-  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-    int i = idx % n, j = idx / n;
-    if (i < j) S_(i, j) = 0.0;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int good = 1;
-    for (int b = 0; b < n; ++b)
-      if (!(S_(b, b) > zero)) good = 0;
-    ok[0] = good;
-  }
-#undef S_
 }
 
-// Symmetric eigensolver.  a: n x n column-major, full symmetric (both halves);
-// on exit eigenvectors in the columns of z (n x n), eigenvalues ascending in w.
-// work: >= 4n doubles.  Householder vectors are kept in a's lower triangle.
-__global__ void __launch_bounds__(kDenseThreads) k_eigh(double* a, int n, double* z, double* w,
-                                                        double* work, int* info) {
+// panel: L(r, k0+c) = sum_{t<=c} S(r, k0+t) X_kk(c, t), rows r in [k0+kb, n)
+__global__ void __launch_bounds__(256) k_trsm_panel(double* s, int n, int k0, int kb,
+                                                    const double* x) {
+  __shared__ double di[NBK][NBK + 1];  // X_kk(c, t)
+  __shared__ double rs[NBK][NBK + 1];  // S(r, k0+t)
+  const int r0 = k0 + kb + blockIdx.x * NBK;
+  const int nr = min(NBK, n - r0);
+  const int t = threadIdx.x;
+  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
+    int c = idx / kb, tt = idx % kb;
+    di[c][tt] = x[(size_t)(k0 + c) * n + (k0 + tt)];
+  }
+  for (int idx = t; idx < nr * kb; idx += blockDim.x) {
+    int r = idx % nr, tt = idx / nr;
+    rs[r][tt] = s[(size_t)(r0 + r) + (size_t)n * (k0 + tt)];
+  }
+  __syncthreads();
+  for (int idx = t; idx < nr * kb; idx += blockDim.x) {
+    int r = idx % nr, c = idx / nr;
+    double v = 0.0;
+    for (int tt = 0; tt <= c; ++tt) v += rs[r][tt] * di[c][tt];
+    s[(size_t)(r0 + r) + (size_t)n * (k0 + c)] = v;
+  }
+}
+
+// trailing update of the lower triangle: S(i,j) -= sum_t L(i,k0+t) L(j,k0+t)
+__global__ void __launch_bounds__(256) k_syrk_update(double* s, int n, int k0, int kb) {
+  const int bi = blockIdx.x, bj = blockIdx.y;
+  if (bj > bi) return;
+  __shared__ double li[NBK][NBK + 1];
+  __shared__ double lj[NBK][NBK + 1];
+  const int base = k0 + kb;
+  const int i0 = base + bi * NBK, j0 = base + bj * NBK;
+  const int ni = min(NBK, n - i0), nj = min(NBK, n - j0);
+  const int t = threadIdx.x;
+  for (int idx = t; idx < ni * kb; idx += blockDim.x) {
+    int r = idx % ni, tt = idx / ni;
+    li[r][tt] = s[(size_t)(i0 + r) + (size_t)n * (k0 + tt)];
+  }
+  for (int idx = t; idx < nj * kb; idx += blockDim.x) {
+    int r = idx % nj, tt = idx / nj;
+    lj[r][tt] = s[(size_t)(j0 + r) + (size_t)n * (k0 + tt)];
+  }
+  __syncthreads();
+  for (int idx = t; idx < ni * nj; idx += blockDim.x) {
+    int r = idx % ni, c = idx / ni;
+    if (i0 + r < j0 + c) continue;
+    double v = 0.0;
+    for (int tt = 0; tt < kb; ++tt) v += li[r][tt] * lj[c][tt];
+    s[(size_t)(i0 + r) + (size_t)n * (j0 + c)] -= v;
+  }
+}
+
+// block row i of X: X_ij = -X_ii * sum_{k=j}^{i-1} L_ik X_kj for every j < i
+__global__ void __launch_bounds__(256) k_trtri_row(const double* s, double* x, int n, int bi) {
+  __shared__ double la[NBK][NBK + 1];
+  __shared__ double xb[NBK][NBK + 1];
+  const int bj = blockIdx.x;
+  if (bj >= bi) return;
+  const int i0 = bi * NBK, j0 = bj * NBK;
+  const int ni = min(NBK, n - i0), nj = min(NBK, n - j0);
+  const int t = threadIdx.x;
+  double acc[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+  for (int bk = bj; bk < bi; ++bk) {
+    const int k0 = bk * NBK, nk = min(NBK, n - k0);
+    __syncthreads();
+    for (int idx = t; idx < ni * nk; idx += blockDim.x) {
+      int r = idx % ni, c = idx / ni;
+      la[r][c] = s[(size_t)(i0 + r) + (size_t)n * (k0 + c)];
+    }
+    for (int idx = t; idx < nk * nj; idx += blockDim.x) {
+      int r = idx / nj, c = idx % nj;
+      xb[r][c] = x[(size_t)(k0 + r) * n + (j0 + c)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      int idx = t + q * 256;
+      int r = idx / NBK, c = idx % NBK;
+      if (r < ni && c < nj) {
+        double v = 0.0;
+        for (int kk = 0; kk < nk; ++kk) v += la[r][kk] * xb[kk][c];
+        acc[q] += v;
+      }
+    }
+  }
+  // multiply by -X_ii
+  __syncthreads();
+  for (int idx = t; idx < ni * ni; idx += blockDim.x) {
+    int r = idx / ni, c = idx % ni;
+    la[r][c] = x[(size_t)(i0 + r) * n + (i0 + c)];
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    int idx = t + q * 256;
+    int r = idx / NBK, c = idx % NBK;
+    if (r < ni && c < nj) xb[r][c] = acc[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    int idx = t + q * 256;
+    int r = idx / NBK, c = idx % NBK;
+    if (r < ni && c < nj) {
+      double v = 0.0;
+      for (int kk = 0; kk <= r; ++kk) v += la[r][kk] * xb[kk][c];
+      x[(size_t)(i0 + r) * n + (j0 + c)] = -v;
+    }
+  }
+}
+
+__global__ void k_zero_upper_rm(double* x, int n) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n * n) return;
+  int i = (int)(idx / n), j = (int)(idx % n);
+  if (j > i) x[idx] = 0.0;
+}
+
+__global__ void k_flag_clear(int* f) { f[0] = 0; }
+__global__ void k_flag_to_ok(const int* fail, int* ok) { ok[0] = fail[0] ? 0 : 1; }
+
+// s: n x n column-major (lower valid; modified), x: n x n row-major output
+// L^-1; ok[0] = lcholesky.  flag: device int scratch.
+int dense_chol_inv(Ctx* c, double* s, int n, double* x, int* ok, int* flag) {
+  hipStream_t st = c->stream;
+  hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(1), 0, st, flag);
+  const int nblk = (n + NBK - 1) / NBK;
+  for (int b = 0; b < nblk; ++b) {
+    const int k0 = b * NBK, kb = std::min(NBK, n - k0);
+    hipLaunchKernelGGL(k_potrf_blk, dim3(1), dim3(256), 0, st, s, n, k0, kb, x, flag);
+    const int rem = n - k0 - kb;
+    if (rem > 0) {
+      const int rb = (rem + NBK - 1) / NBK;
+      hipLaunchKernelGGL(k_trsm_panel, dim3(rb), dim3(256), 0, st, s, n, k0, kb, x);
+      hipLaunchKernelGGL(k_syrk_update, dim3(rb, rb), dim3(256), 0, st, s, n, k0, kb);
+    }
+  }
+  hipLaunchKernelGGL(k_zero_upper_rm, dim3((unsigned)(((long)n * n + 255) / 256)), dim3(256), 0, st,
+                     x, n);
+  for (int bi = 1; bi < nblk; ++bi)
+    hipLaunchKernelGGL(k_trtri_row, dim3(bi), dim3(256), 0, st, s, x, n, bi);
+  hipLaunchKernelGGL(k_flag_to_ok, dim3(1), dim3(1), 0, st, flag, ok);
+  QX_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ===========================================================================
+// Symmetric eigensolver
+// ===========================================================================
+
+// Householder tridiagonalisation in one workgroup.  a: n x n column-major,
+// full symmetric storage.  On exit d (diag), e (e[i] couples i, i+1) and the
+// unit reflector of step k in a(k+1:n, k) (zero when skipped).  Each step
+// makes one coalesced pass over the trailing block: it applies the rank-2
+// update of the step and accumulates the next step's matrix-vector product.
+__global__ void __launch_bounds__(kDenseThreads) k_tridiag(double* a, int n, double* d, double* e,
+                                                           double* vwork) {
   __shared__ double red[40];
-  __shared__ double sh[8];
-  double* d = work;          // diagonal
-  double* e = work + n;      // off-diagonal (e[i] couples i, i+1)
-  double* p = work + 2 * n;  // matvec
-  double* v = work + 3 * n;  // Householder vector
+  double* v = vwork;            // current reflector (n)
+  double* w = vwork + n;        // rank-2 partner (n)
+  double* p = vwork + 2 * n;    // matvec accumulators (n)
+  double* vn = vwork + 3 * n;   // next reflector (n)
 #define A_(i, j) a[(size_t)(i) + (size_t)n * (j)]
-#define Z_(i, j) z[(size_t)(i) + (size_t)n * (j)]
-  // ---- tridiagonalisation: column k annihilated below k+1
-  for (int k = 0; k + 2 < n; ++k) {
-    const int m = n - k - 1;  // length of x = A[k+1:, k]
+  const int t = threadIdx.x;
+  // reflector for column k from the current column values (already updated)
+  auto make_reflector = [&](int k, double* vout, double& alpha, bool& skip) {
+    const int m = n - k - 1;
     double part = 0.0;
-    for (int i = threadIdx.x; i < m; i += blockDim.x) {
-      double x = A_(k + 1 + i, k);
-      part += x * x;
+    for (int i = t; i < m; i += blockDim.x) {
+      double xv = A_(k + 1 + i, k);
+      part += xv * xv;
     }
     double nx2 = block_sum(part, red);
     double x0 = A_(k + 1, k);
-    double alpha = nx2 > 0.0 ? (x0 >= 0.0 ? -sqrt(nx2) : sqrt(nx2)) : 0.0;
-    // v = x - alpha e1, |v|^2 = nx2 - 2 alpha x0 + alpha^2
+    alpha = nx2 > 0.0 ? (x0 >= 0.0 ? -sqrt(nx2) : sqrt(nx2)) : 0.0;
     double vn2 = nx2 - 2.0 * alpha * x0 + alpha * alpha;
-    const bool skip = !(vn2 > 1e-300);
-    double vinv = skip ? 0.0 : 1.0 / sqrt(vn2);
-    for (int i = threadIdx.x; i < m; i += blockDim.x) {
-      double x = A_(k + 1 + i, k);
-      v[i] = (i == 0 ? x - alpha : x) * vinv;
+    skip = !(vn2 > 1e-300);
+    double inv = skip ? 0.0 : 1.0 / sqrt(vn2);
+    for (int i = t; i < m; i += blockDim.x) {
+      double xv = A_(k + 1 + i, k);
+      vout[i] = (i == 0 ? xv - alpha : xv) * inv;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      d[k] = A_(k, k);
-      e[k] = skip ? x0 : alpha;
+  };
+  if (n < 3) {
+    if (t == 0) {
+      for (int i = 0; i < n; ++i) d[i] = A_(i, i);
+      if (n == 2) e[0] = A_(1, 0);
+      e[n - 1] = 0.0;
     }
-    if (!skip) {
-      // p = A_sub v  (A_sub = A[k+1:, k+1:], symmetric, full storage)
-      for (int i = threadIdx.x; i < m; i += blockDim.x) {
-        double t = 0.0;
-        for (int j = 0; j < m; ++j) t += A_(k + 1 + j, k + 1 + i) * v[j];
-        p[i] = t;
-      }
-      __syncthreads();
-      part = 0.0;
-      for (int i = threadIdx.x; i < m; i += blockDim.x) part += v[i] * p[i];
-      double K = block_sum(part, red);
-      // w = 2 (p - K v);  A_sub -= v w^T + w v^T
-      for (int i = threadIdx.x; i < m; i += blockDim.x) p[i] = 2.0 * (p[i] - K * v[i]);
-      __syncthreads();
-      for (int idx = threadIdx.x; idx < m * m; idx += blockDim.x) {
-        int i = idx % m, j = idx / m;
-        A_(k + 1 + i, k + 1 + j) -= v[i] * p[j] + p[i] * v[j];
-      }
+    return;
+  }
+  double alpha;
+  bool skip;
+  make_reflector(0, v, alpha, skip);
+  // p = A(1:, 1:) v
+  {
+    const int m = n - 1;
+    for (int i = t; i < m; i += blockDim.x) {
+      double acc = 0.0;
+      for (int j = 0; j < m; ++j) acc += A_(1 + i, 1 + j) * v[j];
+      p[i] = acc;
     }
-    __syncthreads();
-    // keep v (unit) in column k below the diagonal for the back-transformation
-    for (int i = threadIdx.x; i < m; i += blockDim.x) A_(k + 1 + i, k) = skip ? 0.0 : v[i];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    if (n >= 2) {
-      d[n - 2] = A_(n - 2, n - 2);
-      e[n - 2] = A_(n - 1, n - 2);
-      // last column: no reflection, mark vector zero
+  for (int k = 0; k + 2 < n; ++k) {
+    const int m = n - k - 1;
+    if (t == 0) {
+      d[k] = A_(k, k);
+      e[k] = skip ? A_(k + 1, k) : alpha;
     }
+    if (!skip) {
+      double part = 0.0;
+      for (int i = t; i < m; i += blockDim.x) part += v[i] * p[i];
+      double K = block_sum(part, red);
+      for (int i = t; i < m; i += blockDim.x) w[i] = 2.0 * (p[i] - K * v[i]);
+      __syncthreads();
+      // column k+1 below the diagonal after this step's update (it feeds the
+      // next reflector, or e[n-2] on the last step)
+      for (int i = t; i < m - 1; i += blockDim.x)
+        A_(k + 2 + i, k + 1) -= v[1 + i] * w[0] + w[1 + i] * v[0];
+      __syncthreads();
+    }
+    double alpha_n = 0.0;
+    bool skip_n = true;
+    if (k + 3 < n) make_reflector(k + 1, vn, alpha_n, skip_n);
+    // fused pass: update the trailing block (columns k+2.. and the diagonal
+    // element k+1) and accumulate p_next = A'(k+2:, k+2:) vn
+    const int mn = m - 1;  // size of the next trailing block
+    if (!skip) {
+      if (t == 0) A_(k + 1, k + 1) -= 2.0 * v[0] * w[0];
+      for (int i = t; i < mn; i += blockDim.x) {
+        // column k+1 rows below (symmetric copy) handled above; row k+1 of
+        // the next block is not needed later
+        double acc = 0.0;
+        const double vi = v[1 + i], wi = w[1 + i];
+        for (int j = 0; j < mn; ++j) {
+          double aij = A_(k + 2 + i, k + 2 + j) - vi * w[1 + j] - wi * v[1 + j];
+          A_(k + 2 + i, k + 2 + j) = aij;
+          acc += aij * vn[j];
+        }
+        p[i] = acc;
+      }
+    } else if (mn > 0) {
+      for (int i = t; i < mn; i += blockDim.x) {
+        double acc = 0.0;
+        for (int j = 0; j < mn; ++j) acc += A_(k + 2 + i, k + 2 + j) * vn[j];
+        p[i] = acc;
+      }
+    }
+    __syncthreads();
+    // keep v in column k, move vn -> v
+    for (int i = t; i < m; i += blockDim.x) A_(k + 1 + i, k) = skip ? 0.0 : v[i];
+    __syncthreads();
+    for (int i = t; i < mn; i += blockDim.x) v[i] = vn[i];
+    __syncthreads();
+    alpha = alpha_n;
+    skip = skip_n;
+  }
+  if (t == 0) {
+    d[n - 2] = A_(n - 2, n - 2);
+    e[n - 2] = A_(n - 1, n - 2);
     d[n - 1] = A_(n - 1, n - 1);
     e[n - 1] = 0.0;
   }
-  if (n >= 2)
-    for (int i = threadIdx.x; i < 1; i += blockDim.x) A_(n - 1, n - 2) = 0.0;
-  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-    int i = idx % n, j = idx / n;
-    Z_(i, j) = (i == j) ? 1.0 : 0.0;
+#undef A_
+}
+
+// Sturm count: number of eigenvalues of T below sigma
+__device__ inline int sturm_count(const double* d, const double* e, int n, double sigma,
+                                  double pivmin) {
+  int cnt = 0;
+  double q = d[0] - sigma;
+  if (fabs(q) < pivmin) q = -pivmin;
+  if (q < 0.0) ++cnt;
+  for (int i = 1; i < n; ++i) {
+    q = d[i] - sigma - e[i - 1] * e[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q < 0.0) ++cnt;
   }
-  __syncthreads();
-  // ---- implicit QL on (d, e); rotations of each sweep applied to rows of Z
-  __shared__ double cs[2 * 4096];
-  int fail = 0;
-  for (int l = 0; l < n; ++l) {
-    int iter = 0;
-    for (;;) {
-      // find small subdiagonal
-      __shared__ int msh;
-      if (threadIdx.x == 0) {
-        int m = l;
-        for (; m < n - 1; ++m) {
-          double dd = fabs(d[m]) + fabs(d[m + 1]);
-          if (fabs(e[m]) <= 1e-16 * dd) break;
-        }
-        msh = m;
+  return cnt;
+}
+
+// one thread per eigenvalue index j: bisection to full precision
+__global__ void k_bisect(const double* d, const double* e, int n, double* w, double lo0, double hi0,
+                         double pivmin) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double lo = lo0, hi = hi0;
+  for (int it = 0; it < 200; ++it) {
+    double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (hi - lo <= 4.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+    if (sturm_count(d, e, n, mid, pivmin) > j) hi = mid;
+    else lo = mid;
+  }
+  w[j] = 0.5 * (lo + hi);
+}
+
+// Inverse iteration for the eigenvalues of one cluster [c0, c1) in one
+// workgroup; vectors are orthogonalised against the cluster's earlier ones.
+// z: n x n column-major (vector j in column j).  lu: 5n doubles per block.
+__global__ void __launch_bounds__(256) k_inviter(const double* d, const double* e, int n,
+                                                 const double* w, const int* cstart, int ncl,
+                                                 double* z, double* luwork, double tnorm) {
+  __shared__ double red[40];
+  const int cl = blockIdx.x;
+  if (cl >= ncl) return;
+  const int c0 = cstart[cl], c1 = cstart[cl + 1];
+  double* lu = luwork + (size_t)cl * 5 * n;
+  double* dd = lu;             // U diagonal
+  double* du = lu + n;         // U first superdiagonal
+  double* du2 = lu + 2 * n;    // U second superdiagonal (from pivoting)
+  double* dl = lu + 3 * n;     // multipliers
+  double* piv = lu + 4 * n;    // 1 if rows i, i+1 swapped
+  const double eps = 2.220446049250313e-16;
+  const double pert = eps * tnorm;
+  for (int j = c0; j < c1; ++j) {
+    double* x = z + (size_t)n * j;
+    const double lam = w[j];
+    // LU with partial pivoting of T - lam I (one thread; O(n))
+    if (threadIdx.x == 0) {
+      for (int i = 0; i < n; ++i) {
+        dd[i] = d[i] - lam;
+        du[i] = i < n - 1 ? e[i] : 0.0;
+        du2[i] = 0.0;
       }
-      __syncthreads();
-      const int m = msh;
-      if (m == l) break;
-      if (++iter > 60) { fail = 1; break; }
-      // thread 0: one QL sweep from m-1 down to l, record rotations
-      if (threadIdx.x == 0) {
-        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-        double r = hypot(g, 1.0);
-        g = d[m] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
-        double s = 1.0, c = 1.0, pp = 0.0;
-        int i;
-        int early = 0;
-        for (i = m - 1; i >= l; --i) {
-          double f = s * e[i], b = c * e[i];
-          r = hypot(f, g);
-          e[i + 1] = r;
-          if (r == 0.0) {
-            d[i + 1] -= pp;
-            e[m] = 0.0;
-            early = 1;
-            cs[2 * (i - l)] = 1.0;
-            cs[2 * (i - l) + 1] = 0.0;
-            break;
+      for (int i = 0; i < n - 1; ++i) {
+        double sub = e[i];
+        if (fabs(dd[i]) >= fabs(sub)) {
+          piv[i] = 0.0;
+          double f = dd[i] != 0.0 ? sub / dd[i] : 0.0;
+          dl[i] = f;
+          dd[i + 1] -= f * du[i];
+        } else {
+          piv[i] = 1.0;
+          double f = dd[i] / sub;
+          dl[i] = f;
+          dd[i] = sub;
+          double tmp = du[i];
+          du[i] = dd[i + 1];
+          dd[i + 1] = tmp - f * dd[i + 1];
+          if (i < n - 2) {
+            du2[i] = du[i + 1];
+            du[i + 1] = -f * du[i + 1];
           }
-          s = f / r;
-          c = g / r;
-          g = d[i + 1] - pp;
-          r = (d[i] - g) * s + 2.0 * c * b;
-          pp = s * r;
-          d[i + 1] = g + pp;
-          g = c * r - b;
-          cs[2 * (i - l)] = c;
-          cs[2 * (i - l) + 1] = s;
-        }
-        if (!early) {
-          d[l] -= pp;
-          e[l] = g;
-          e[m] = 0.0;
-        }
-        sh[0] = early ? (double)(i) : (double)(l - 1);  // last rotation index applied
-      }
-      __syncthreads();
-      const int ilast = (int)sh[0];
-      // apply rotations i = m-1 .. ilast+1 (plus the r==0 one as identity) to Z rows
-      for (int row = threadIdx.x; row < n; row += blockDim.x) {
-        for (int i = m - 1; i > ilast; --i) {
-          double c = cs[2 * (i - l)], s = cs[2 * (i - l) + 1];
-          double f = Z_(row, i + 1);
-          double zi = Z_(row, i);
-          Z_(row, i + 1) = s * zi + c * f;
-          Z_(row, i) = c * zi - s * f;
         }
       }
-      __syncthreads();
+      for (int i = 0; i < n; ++i)
+        if (fabs(dd[i]) < pert) dd[i] = dd[i] < 0.0 ? -pert : pert;
     }
-    if (fail) break;
-  }
-  // ---- back-transformation Z <- H_0 ... H_{n-3} Z
-  for (int k = n - 3; k >= 0; --k) {
-    const int m = n - k - 1;
-    for (int col = threadIdx.x; col < n; col += blockDim.x) {
-      double t = 0.0;
-      for (int i = 0; i < m; ++i) t += A_(k + 1 + i, k) * Z_(k + 1 + i, col);
-      t *= 2.0;
-      if (t != 0.0)
-        for (int i = 0; i < m; ++i) Z_(k + 1 + i, col) -= t * A_(k + 1 + i, k);
+    // deterministic start vector
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      unsigned h = (unsigned)(i * 2654435761u) ^ (unsigned)(j * 40503u + 12345u);
+      h ^= h >> 13;
+      h *= 0x5bd1e995u;
+      h ^= h >> 15;
+      x[i] = 0.5 + (double)(h & 0xffff) / 65536.0;
     }
     __syncthreads();
-  }
-  // ---- stable ascending sort of (d, Z columns): ranks, then permute via a
-  if (threadIdx.x == 0) info[0] = fail;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    double di = d[i];
-    int rank = 0;
-    for (int j = 0; j < n; ++j) {
-      double dj = d[j];
-      if (dj < di || (dj == di && j < i)) ++rank;
+    for (int it = 0; it < 3; ++it) {
+      if (threadIdx.x == 0) {
+        // solve L U x = P b: forward with the recorded row swaps, then back
+        for (int i = 0; i < n - 1; ++i) {
+          if (piv[i] == 0.0) {
+            x[i + 1] -= dl[i] * x[i];
+          } else {
+            double tmp = x[i];
+            x[i] = x[i + 1];
+            x[i + 1] = tmp - dl[i] * x[i + 1];
+          }
+        }
+        // scale to avoid overflow: track the largest entry
+        double big = 0.0;
+        for (int i = n - 1; i >= 0; --i) {
+          double v = x[i];
+          if (i + 1 < n) v -= du[i] * x[i + 1];
+          if (i + 2 < n) v -= du2[i] * x[i + 2];
+          x[i] = v / dd[i];
+          big = fmax(big, fabs(x[i]));
+          if (big > 1e150) {  // rescale the whole system (solved part and rhs)
+            for (int q = 0; q < n; ++q) x[q] *= 1e-150;
+            big *= 1e-150;
+          }
+        }
+      }
+      __syncthreads();
+      // orthogonalise against the cluster's finished vectors (MGS)
+      for (int k = c0; k < j; ++k) {
+        const double* zk = z + (size_t)n * k;
+        double part = 0.0;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) part += zk[i] * x[i];
+        double dot = block_sum(part, red);
+        for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] -= dot * zk[i];
+        __syncthreads();
+      }
+      double part = 0.0;
+      for (int i = threadIdx.x; i < n; i += blockDim.x) part += x[i] * x[i];
+      double nrm = sqrt(block_sum(part, red));
+      double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+      for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] *= inv;
+      __syncthreads();
     }
-    p[i] = (double)rank;
   }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-    int row = idx % n, col = idx / n;
-    A_(row, (int)p[col]) = Z_(row, col);
+}
+
+// back-transformation Z <- H_0 ... H_{n-3} Z, 16 columns per block, 16 lanes
+// per column over the rows
+__global__ void __launch_bounds__(256) k_backtrans(const double* a, int n, double* z) {
+  const int col = blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int lane = threadIdx.x & 15;
+  const bool active = col < n;
+  double* zc = z + (size_t)n * (active ? col : 0);
+  for (int k = n - 3; k >= 0; --k) {
+    const double* vk = a + (size_t)n * k + (k + 1);  // reflector rows k+1..n-1
+    const int m = n - k - 1;
+    double part = 0.0;
+    if (active)
+      for (int i = lane; i < m; i += 16) part += vk[i] * zc[k + 1 + i];
+    for (int o = 8; o > 0; o >>= 1) part += __shfl_xor(part, o, 16);
+    const double tv = 2.0 * part;
+    if (active && tv != 0.0)
+      for (int i = lane; i < m; i += 16) zc[k + 1 + i] -= tv * vk[i];
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < n; i += blockDim.x) w[(int)p[i]] = d[i];
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) z[idx] = a[idx];
-#undef A_
-#undef Z_
+}
+
+// host driver.  a: n x n column-major (symmetric, destroyed), z: eigenvectors
+// (columns), w: eigenvalues ascending; work >= 12n doubles; iwork >= n+1 ints
+// (device).  Cluster boundaries are found on the host from w.
+int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int* info) {
+  (void)info;
+  hipStream_t st = c->stream;
+  double* d = work;
+  double* e = work + n;
+  double* vw = work + 2 * n;  // 4n
+  hipLaunchKernelGGL(k_tridiag, dim3(1), dim3(kDenseThreads), 0, st, a, n, d, e, vw);
+  std::vector<double> hd(n), he(n);
+  QX_CHECK(hipMemcpyAsync(hd.data(), d, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  QX_CHECK(hipMemcpyAsync(he.data(), e, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  QX_CHECK(hipStreamSynchronize(st));
+  double lo = 0.0, hi = 0.0, tnorm = 0.0, emax2 = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double r = (i > 0 ? std::fabs(he[i - 1]) : 0.0) + (i < n - 1 ? std::fabs(he[i]) : 0.0);
+    lo = i == 0 ? hd[i] - r : std::min(lo, hd[i] - r);
+    hi = i == 0 ? hd[i] + r : std::max(hi, hd[i] + r);
+    tnorm = std::max(tnorm, std::fabs(hd[i]) + r);
+    if (i < n - 1) emax2 = std::max(emax2, he[i] * he[i]);
+  }
+  const double eps = 2.220446049250313e-16;
+  const double safmin = 2.2250738585072014e-308;
+  const double pivmin = std::max(safmin * std::max(1.0, emax2), safmin);
+  lo -= 2.0 * eps * tnorm + 4.0 * pivmin;
+  hi += 2.0 * eps * tnorm + 4.0 * pivmin;
+  hipLaunchKernelGGL(k_bisect, dim3((n + 127) / 128), dim3(128), 0, st, d, e, n, w, lo, hi, pivmin);
+  std::vector<double> hw(n);
+  QX_CHECK(hipMemcpyAsync(hw.data(), w, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  QX_CHECK(hipStreamSynchronize(st));
+  // clusters: consecutive eigenvalues closer than 1e-3 ||T|| are handled
+  // together (the orthogonality tolerance of the classical inverse iteration)
+  std::vector<int> cs;
+  cs.push_back(0);
+  for (int j = 1; j < n; ++j)
+    if (hw[j] - hw[j - 1] > 1e-3 * tnorm) cs.push_back(j);
+  cs.push_back(n);
+  const int ncl = (int)cs.size() - 1;
+  int* dcs = reinterpret_cast<int*>(work + 6 * n);  // n+1 ints
+  QX_CHECK(hipMemcpyAsync(dcs, cs.data(), sizeof(int) * cs.size(), hipMemcpyHostToDevice, st));
+  double* lu = c->d_rho_part;  // >= 5 n * ncl doubles of scratch
+  if ((size_t)5 * n * ncl > (size_t)c->rho_groups * c->N) return -8;
+  hipLaunchKernelGGL(k_inviter, dim3(ncl), dim3(256), 0, st, d, e, n, w, dcs, ncl, z, lu, tnorm);
+  hipLaunchKernelGGL(k_backtrans, dim3((n + 15) / 16), dim3(256), 0, st, a, n, z);
+  QX_CHECK(hipGetLastError());
+  return 0;
 }
 
 // H[i][j] for i >= j taken from C[n=i][m=j] and mirrored (hmatrix2 stores the
@@ -290,19 +581,6 @@ __global__ void k_symmetrize_lower(const double* C, int n, double* H, double* ei
   int hi = i > j ? i : j, lo = i > j ? j : i;
   H[(size_t)i + (size_t)n * j] = C[(size_t)hi * n + lo];
   if (i == j && eig) eig[i] = C[(size_t)i * n + i];
-}
-
-int dense_chol_inv(Ctx* c, double* s, int n, int* ok_dev) {
-  hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(kDenseThreads), 0, c->stream, s, n, ok_dev);
-  QX_CHECK(hipGetLastError());
-  return 0;
-}
-
-int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int* info) {
-  if (n > 4096) return -5;
-  hipLaunchKernelGGL(k_eigh, dim3(1), dim3(kDenseThreads), 0, c->stream, a, n, z, w, work, info);
-  QX_CHECK(hipGetLastError());
-  return 0;
 }
 
 int dense_symmetrize(Ctx* c, const double* C, int n, double* H, double* eig) {

@@ -70,6 +70,8 @@ SIGNATURES = {
     "qx_hckbpp_ib_g": [_vp, _dp, _dp],
     "qx_schmidt": [_vp, _dp, _i, _pi],
     "qx_softchg": [_vp, _dp, _i, _dp, _d, _dp],
+    "qx_rseigqr": [_vp, _vp, _i, _dp],
+    "qx_icholesky": [_vp, _vp, _i, _pi],
 }
 # entry points beyond the header's core set (batched CG, kinetic energies)
 EXTRA = {

@@ -204,6 +204,22 @@ class PlaneWave:
         check(load().qx_schmidt(self.h, c, c.shape[0], C.byref(ok)), "qx_schmidt")
         return c, bool(ok.value)
 
+    def rseigqr(self, a):
+        """Eigen-decomposition of a symmetric matrix on the device (FP64)."""
+        n = a.shape[0]
+        m = np.asfortranarray(np.asarray(a, np.float64)).copy(order="F")
+        w = np.zeros(n)
+        check(load().qx_rseigqr(self.h, m.ctypes.data_as(C.c_void_p), n, w), "qx_rseigqr")
+        return w, np.asarray(m)
+
+    def icholesky(self, s):
+        n = s.shape[0]
+        m = np.asfortranarray(np.asarray(s, np.float64)).copy(order="F")
+        ok = C.c_int(0)
+        check(load().qx_icholesky(self.h, m.ctypes.data_as(C.c_void_p), n, C.byref(ok)),
+              "qx_icholesky")
+        return np.asarray(m), bool(ok.value)
+
     def softchg(self, cgjr, occ, rvol):
         c = f64(cgjr)
         rho = np.zeros(self.gs.N)

@@ -207,3 +207,50 @@ def test_multi_batch_pencil(si64, monkeypatch):
     rho = ctx.density(0, nb, occ, si64.cell.volume)
     ref = O.softchg(psi, si64.nplw, occ, si64.cell.volume, si64.ijk, si64.nd)
     assert _relerr(rho, ref) < 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 65, 200])
+def test_icholesky(si8, n):
+    gs, ctx, H, v = _setup(si8, max(n, 2), kb=False)
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((n + 5, n))
+    s = x.T @ x + 0.05 * np.eye(n)
+    li, ok = ctx.icholesky(s)
+    assert ok
+    ref = O.rsdecmp_rsinvcd(s)
+    assert _relerr(np.tril(li), np.tril(ref)) < 1e-11
+    assert np.abs(np.triu(li, 1)).max() == 0.0
+
+
+def test_icholesky_not_positive_definite(si8):
+    gs, ctx, H, v = _setup(si8, 80, kb=False)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((80, 40))
+    s = x @ x.T  # rank 40 < 80
+    _, ok = ctx.icholesky(s)
+    assert not ok
+
+
+@pytest.mark.parametrize("case", ["random", "degenerate", "clustered"])
+def test_rseigqr(si8, case):
+    n = 150
+    gs, ctx, H, v = _setup(si8, n, kb=False)
+    rng = np.random.default_rng(11)
+    q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    if case == "random":
+        x = rng.standard_normal((n, n))
+        a = x + x.T
+    else:
+        lam = np.sort(rng.uniform(-3, 3, n))
+        if case == "degenerate":
+            lam[10:13] = lam[10]       # triply degenerate
+            lam[50:52] = lam[50]
+        else:
+            lam[20:60] = lam[20] + 1e-9 * np.arange(40)  # tight cluster
+        a = (q * lam) @ q.T
+        a = 0.5 * (a + a.T)
+    w, z = ctx.rseigqr(a)
+    wr, zr = O.eigh(a)
+    np.testing.assert_allclose(w, wr, rtol=0, atol=1e-11 * np.abs(wr).max())
+    assert np.abs(z.T @ z - np.eye(n)).max() < 1e-10
+    assert np.abs(a @ z - z * w).max() < 1e-10 * np.abs(wr).max()
