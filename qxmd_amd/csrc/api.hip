@@ -248,7 +248,7 @@ int qx_destroy(qx_ctx* h) {
   Ctx* c = &h->c;
   (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {c->d_perm, c->d_pt_col, c->d_pt_z, c->d_pair_p0, c->d_col_hi, c->d_col_kmod,
-                  c->d_kin, c->d_tw, c->d_V, c->d_Vf, c->d_B, c->d_D, c->d_t1, c->d_t2, c->d_P,
+                  c->d_kin, c->d_tw, c->d_V, c->d_Vf, c->d_Vp, c->d_Vpf, c->d_B, c->d_D, c->d_t1, c->d_t2, c->d_P,
                   c->d_small, c->d_rho_part, c->d_rho, c->d_scal};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -284,6 +284,8 @@ int qx_set_glocal(qx_ctx* h, const double* glocal) {
   Ctx* c = &h->c;
   QX_CHECK(hipMemcpyAsync(c->d_V, glocal, sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_d2f, dim3(cdiv(c->N, 256)), dim3(256), 0, c->stream, c->d_V, c->d_Vf, c->N);
+  int rc = fast_permute_v(c);
+  if (rc) return rc;
   QX_CHECK(hipStreamSynchronize(c->stream));
   return 0;
 }

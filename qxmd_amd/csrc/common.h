@@ -29,16 +29,16 @@ __host__ __device__ inline cplx<T> mk(T a, T b) {
   return r;
 }
 template <typename T>
-__device__ inline cplx<T> cadd(cplx<T> a, cplx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline cplx<T> cadd(cplx<T> a, cplx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
 template <typename T>
-__device__ inline cplx<T> csub(cplx<T> a, cplx<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline cplx<T> csub(cplx<T> a, cplx<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
 template <typename T>
-__device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b) {
+__host__ __device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b) {
   return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 // multiply by sign*i
 template <typename T>
-__device__ inline cplx<T> cmul_si(cplx<T> a, int sign) {
+__host__ __device__ inline cplx<T> cmul_si(cplx<T> a, int sign) {
   return sign > 0 ? mk<T>(-a.y, a.x) : mk<T>(a.y, -a.x);
 }
 
@@ -72,7 +72,7 @@ struct Dft;
 
 template <typename T>
 struct Dft<T, 2> {
-  __device__ static void run(cplx<T>* v, int) {
+  __host__ __device__ static void run(cplx<T>* v, int) {
     cplx<T> a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
@@ -80,7 +80,7 @@ struct Dft<T, 2> {
 };
 template <typename T>
 struct Dft<T, 4> {
-  __device__ static void run(cplx<T>* v, int sign) {
+  __host__ __device__ static void run(cplx<T>* v, int sign) {
     cplx<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
     cplx<T> t2 = cadd(v[1], v[3]), t3 = cmul_si(csub(v[1], v[3]), sign);
     v[0] = cadd(t0, t2);
@@ -91,7 +91,7 @@ struct Dft<T, 4> {
 };
 template <typename T>
 struct Dft<T, 3> {
-  __device__ static void run(cplx<T>* v, int sign) {
+  __host__ __device__ static void run(cplx<T>* v, int sign) {
     const T c1 = T(-0.5), s1 = T(0.86602540378443864676372317075294);
     cplx<T> a = v[0], b = v[1], c = v[2];
     cplx<T> s = cadd(b, c), d = csub(b, c);
@@ -104,7 +104,7 @@ struct Dft<T, 3> {
 };
 template <typename T>
 struct Dft<T, 5> {
-  __device__ static void run(cplx<T>* v, int sign) {
+  __host__ __device__ static void run(cplx<T>* v, int sign) {
     const T c1 = T(0.30901699437494742410229341718282);   // cos(2pi/5)
     const T c2 = T(-0.80901699437494742410229341718282);  // cos(4pi/5)
     const T s1 = T(0.95105651629515357211643933337938);   // sin(2pi/5)
@@ -125,7 +125,7 @@ struct Dft<T, 5> {
 };
 template <typename T>
 struct Dft<T, 7> {
-  __device__ static void run(cplx<T>* v, int sign) {
+  __host__ __device__ static void run(cplx<T>* v, int sign) {
     const T C[3] = {T(0.62348980185873353053), T(-0.22252093395631440429),
                     T(-0.90096886790241912624)};
     const T S[3] = {T(0.78183148246802980871), T(0.97492791218182360702),
@@ -156,6 +156,88 @@ struct Dft<T, 7> {
     for (int q = 0; q < 7; ++q) v[q] = y[q];
   }
 };
+
+
+// ---------------------------------------------------------------------------
+// Compile-time twiddles and composite codelets (N = P*M <= 16) for the
+// two-step line transforms.  cos/sin are evaluated by constexpr Taylor series
+// on [-pi, pi], so every constant is folded at compile time.
+// ---------------------------------------------------------------------------
+constexpr double kPiCE = 3.14159265358979323846264338327950288;
+constexpr double ce_cos_red(double x) {
+  double term = 1.0, sum = 1.0, x2 = x * x;
+  for (int i = 1; i < 30; ++i) {
+    term *= -x2 / ((2.0 * i - 1.0) * (2.0 * i));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double ce_sin_red(double x) {
+  double term = x, sum = x, x2 = x * x;
+  for (int i = 1; i < 30; ++i) {
+    term *= -x2 / ((2.0 * i) * (2.0 * i + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double ce_angle(int k, int n) {
+  int kk = ((k % n) + n) % n;
+  double x = 2.0 * kPiCE * (double)kk / (double)n;
+  return x > kPiCE ? x - 2.0 * kPiCE : x;
+}
+template <int N>
+struct TwTable {
+  double c[N], s[N];
+  constexpr TwTable() : c(), s() {
+    for (int k = 0; k < N; ++k) {
+      c[k] = ce_cos_red(ce_angle(k, N));
+      s[k] = ce_sin_red(ce_angle(k, N));
+    }
+  }
+};
+template <int N>
+struct TwHolder {
+  static constexpr TwTable<N> tab = TwTable<N>();
+};
+
+// Cooley-Tukey in registers: N = P*M, natural order in and out
+template <typename T, int P, int M>
+struct DftCT {
+  __host__ __device__ static void run(cplx<T>* x, int sign) {
+    constexpr int N = P * M;
+    cplx<T> y[P][M];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) y[p][m] = x[p + P * m];
+      Dft<T, M>::run(y[p], sign);
+    }
+#pragma unroll
+    for (int p = 1; p < P; ++p)
+#pragma unroll
+      for (int k = 1; k < M; ++k) {
+        const T c = (T)TwHolder<N>::tab.c[(p * k) % N];
+        const T sn = (T)TwHolder<N>::tab.s[(p * k) % N];
+        y[p][k] = cmul(y[p][k], mk<T>(c, sign > 0 ? sn : -sn));
+      }
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      cplx<T> z[P];
+#pragma unroll
+      for (int p = 0; p < P; ++p) z[p] = y[p][k];
+      Dft<T, P>::run(z, sign);
+#pragma unroll
+      for (int q = 0; q < P; ++q) x[k + M * q] = z[q];
+    }
+  }
+};
+template <typename T> struct Dft<T, 6> { __host__ __device__ static void run(cplx<T>* v, int s) { DftCT<T, 2, 3>::run(v, s); } };
+template <typename T> struct Dft<T, 8> { __host__ __device__ static void run(cplx<T>* v, int s) { DftCT<T, 2, 4>::run(v, s); } };
+template <typename T> struct Dft<T, 9> { __host__ __device__ static void run(cplx<T>* v, int s) { DftCT<T, 3, 3>::run(v, s); } };
+template <typename T> struct Dft<T, 10> { __host__ __device__ static void run(cplx<T>* v, int s) { DftCT<T, 2, 5>::run(v, s); } };
+template <typename T> struct Dft<T, 12> { __host__ __device__ static void run(cplx<T>* v, int s) { DftCT<T, 3, 4>::run(v, s); } };
+template <typename T> struct Dft<T, 15> { __host__ __device__ static void run(cplx<T>* v, int s) { DftCT<T, 3, 5>::run(v, s); } };
+template <typename T> struct Dft<T, 16> { __host__ __device__ static void run(cplx<T>* v, int s) { DftCT<T, 4, 4>::run(v, s); } };
 
 // ---------------------------------------------------------------------------
 // In-LDS batched Stockham FFT.  Lines start at buf + line*lstride with

@@ -39,6 +39,8 @@ struct Ctx {
   double *d_tw = nullptr;         // twiddles: x (n1), y (n2), z (n3), exp(+2pi i k/n)
   double *d_V = nullptr;          // local potential, FFT order (N)
   float *d_Vf = nullptr;          // FP32 copy
+  double *d_Vp = nullptr;         // slot-permuted copies for the fast pencil path
+  float *d_Vpf = nullptr;
   // KB projectors, internal order: B[p*nplwex + i], p = projector
   int npj = 0;
   double *d_B = nullptr;
@@ -67,6 +69,10 @@ int pencil_build(Ctx* c, const int* nga, const int* ngb, const int* ngc, const d
 // slot dst (kinetic + local); KB handled separately by the GEMM path.
 int local_apply(Ctx* c, const double* psi, double* out, int nb, int prec, int accumulate);
 int density(Ctx* c, const double* psi, int nb, const double* occ_dev, double scale, int prec);
+bool fast_ok(const Ctx* c);
+int fast_permute_v(Ctx* c);
+int local_apply_fast(Ctx* c, const double* psi, double* out, int nb, int prec, int accumulate);
+int density_fast(Ctx* c, const double* psi, int nb, const double* w, int prec);
 int kb_apply(Ctx* c, const double* psi, double* out, int nb, int accumulate_into_out,
              double* slmir_out);
 int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,

@@ -1,0 +1,571 @@
+// pencil_fast.hip -- the band-apply / density pencil FFTs for cubic grids whose
+// edge N factors as NA*NB with both factors <= 16 (30 = 6*5, 60 = 6*10,
+// 90 = 9*10, 120 = 12*10, 64 = 8*8, 96 = 12*8, 128 = 16*8).
+//
+// A line transform of length N is done in two register steps with a single
+// LDS exchange between them:
+//   NP ("natural -> permuted"): step A, thread b < NB: DFT_NA over the NB-strided
+//       elements b + NB*a, twiddle W_N^(b*ka); step B, thread ka < NA: DFT_NB over
+//       the contiguous run NB*ka + b.  Index ka + NA*kb ends up at slot kb + NB*ka.
+//   PN ("permuted -> natural"): the same two steps in reverse order.
+// Every inverse transform of the band-apply is NP and every forward one PN, so
+// the digit-transposed intermediate order never needs a reordering pass: the
+// z planes of T1/T2 and the y/x slots of a plane are kept in slot order and
+// V(r) is stored pre-permuted.  In x the inverse step B, the V(r) multiply and
+// the forward step B' act on the same registers.
+#include <algorithm>
+#include <cstdlib>
+#include "context.h"
+
+namespace qx {
+
+constexpr int kFastThreads = 256;
+// column pairs per z block: 16, or 8 where 32 complex lines of length N would
+// pass 48 KiB of static LDS
+__host__ __device__ constexpr int zpairs(int N, int elem_bytes) {
+  return 2 * 16 * N * elem_bytes <= 49152 ? 16 : 8;
+}
+
+__host__ __device__ inline int slot_phys(int s, int NA, int NB) { return s / NB + NA * (s % NB); }
+
+struct FastArgs {
+  int N, nH, hmax, npair, ncol, nplwex, nb, bp0, rows;
+  const int *pt_col, *pt_z, *pair_p0, *col_hi, *col_kmod;
+  const double* kin;
+  const double* tw;  // exp(+2 pi i k / N), k < N
+};
+
+template <typename T>
+__device__ inline cplx<T> twid(const cplx<T>* tw, int k, int sign) {
+  cplx<T> w = tw[k];
+  if (sign < 0) w.y = -w.y;
+  return w;
+}
+
+// ---- line steps (one thread, line in LDS with unit stride) ----------------
+template <typename T, int NA, int NB>
+__device__ inline void np_stepA(cplx<T>* L, int b, const cplx<T>* tw, int sign) {
+  cplx<T> v[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) v[a] = L[b + NB * a];
+  Dft<T, NA>::run(v, sign);
+#pragma unroll
+  for (int ka = 1; ka < NA; ++ka)
+    if (b) v[ka] = cmul(v[ka], twid(tw, b * ka, sign));
+#pragma unroll
+  for (int ka = 0; ka < NA; ++ka) L[b + NB * ka] = v[ka];
+}
+template <typename T, int NA, int NB>
+__device__ inline void np_stepB(cplx<T>* L, int ka, int sign) {
+  cplx<T> v[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) v[b] = L[NB * ka + b];
+  Dft<T, NB>::run(v, sign);
+#pragma unroll
+  for (int kb = 0; kb < NB; ++kb) L[NB * ka + kb] = v[kb];
+}
+template <typename T, int NA, int NB>
+__device__ inline void pn_stepB(cplx<T>* L, int ka, const cplx<T>* tw, int sign) {
+  cplx<T> v[NB];
+#pragma unroll
+  for (int kb = 0; kb < NB; ++kb) v[kb] = L[NB * ka + kb];
+  Dft<T, NB>::run(v, sign);
+#pragma unroll
+  for (int b = 1; b < NB; ++b)
+    if (ka) v[b] = cmul(v[b], twid(tw, b * ka, sign));
+#pragma unroll
+  for (int b = 0; b < NB; ++b) L[NB * ka + b] = v[b];
+}
+template <typename T, int NA, int NB>
+__device__ inline void pn_stepA(cplx<T>* L, int b, int sign) {
+  cplx<T> v[NA];
+#pragma unroll
+  for (int ka = 0; ka < NA; ++ka) v[ka] = L[b + NB * ka];
+  Dft<T, NA>::run(v, sign);
+#pragma unroll
+  for (int a = 0; a < NA; ++a) L[b + NB * a] = v[a];
+}
+
+__device__ inline int mirror_col_f(int col) {
+  return col == 0 ? 0 : ((col & 1) ? col + 1 : col - 1);
+}
+
+// ---- z inverse: sphere -> T1[bp][z slot][col] ------------------------------
+template <typename T, int NA, int NB>
+__global__ void __launch_bounds__(kFastThreads) k_zinv_f(FastArgs a, const double* __restrict__ psi,
+                                                         cplx<T>* __restrict__ t1) {
+  constexpr int N = NA * NB;
+  constexpr int PB = zpairs(N, (int)sizeof(cplx<T>));
+  __shared__ cplx<T> tw[N];
+  __shared__ cplx<T> lines[2 * PB][N];
+  const int q0 = blockIdx.x * PB, q1 = min(q0 + PB, a.npair);
+  const int bp = blockIdx.y;
+  const int ba = 2 * (a.bp0 + bp), bb = ba + 1;
+  const int c0 = q0 == 0 ? 0 : 2 * q0 - 1, c1 = 2 * q1 - 1, ncl = c1 - c0;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = mk<T>((T)a.tw[2 * k], (T)a.tw[2 * k + 1]);
+  for (int i = threadIdx.x; i < ncl * N; i += blockDim.x) lines[i / N][i % N] = mk<T>(0, 0);
+  __syncthreads();
+  const double* pa = psi + (size_t)ba * a.nplwex;
+  const double* pb = psi + (size_t)bb * a.nplwex;
+  const bool hasb = bb < a.nb;
+  const int p0 = a.pair_p0[q0], p1 = a.pair_p0[q1];
+  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    double ar = pa[2 * p], ai = pa[2 * p + 1];
+    double br = hasb ? pb[2 * p] : 0.0, bi = hasb ? pb[2 * p + 1] : 0.0;
+    int col = a.pt_col[p] - c0, z = a.pt_z[p];
+    if (p == 0) {
+      lines[col][0] = mk<T>((T)ar, (T)br);
+    } else {
+      lines[col][z] = mk<T>((T)(ar - bi), (T)(ai + br));
+      lines[mirror_col_f(col + c0) - c0][z == 0 ? 0 : N - z] = mk<T>((T)(ar + bi), (T)(br - ai));
+    }
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < ncl * NB; it += blockDim.x)
+    np_stepA<T, NA, NB>(lines[it / NB], it % NB, tw, +1);
+  __syncthreads();
+  for (int it = threadIdx.x; it < ncl * NA; it += blockDim.x)
+    np_stepB<T, NA, NB>(lines[it / NA], it % NA, +1);
+  __syncthreads();
+  cplx<T>* out = t1 + (size_t)bp * N * a.ncol;
+  for (int i = threadIdx.x; i < ncl * N; i += blockDim.x) {
+    int zs = i / ncl, cl = i % ncl;
+    out[(size_t)zs * a.ncol + c0 + cl] = lines[cl][zs];
+  }
+}
+
+// ---- plane: y inverse, x inverse, x V, x forward, y forward ----------------
+template <typename T, int NA, int NB>
+__device__ inline void plane_y_inverse(cplx<T>* A, int nH, const cplx<T>* tw) {
+  constexpr int N = NA * NB;
+  for (int it = threadIdx.x; it < nH * NB; it += blockDim.x)
+    np_stepA<T, NA, NB>(A + (it / NB) * N, it % NB, tw, +1);
+  __syncthreads();
+  for (int it = threadIdx.x; it < nH * NA; it += blockDim.x)
+    np_stepB<T, NA, NB>(A + (it / NA) * N, it % NA, +1);
+  __syncthreads();
+}
+
+// x step A for row slot ys: gather the natural-x inputs (|h| <= hmax) from A
+template <typename T, int NA, int NB>
+__device__ inline void x_stepA(const cplx<T>* A, cplx<T>* R, int ys, int b, int hmax,
+                               const cplx<T>* tw) {
+  constexpr int N = NA * NB;
+  cplx<T> v[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    int n = b + NB * a;
+    int h = n <= hmax ? n : (n >= N - hmax ? n - N : N);
+    v[a] = h < N ? A[(h + hmax) * N + ys] : mk<T>(0, 0);
+  }
+  Dft<T, NA>::run(v, +1);
+#pragma unroll
+  for (int ka = 1; ka < NA; ++ka)
+    if (b) v[ka] = cmul(v[ka], twid(tw, b * ka, +1));
+#pragma unroll
+  for (int ka = 0; ka < NA; ++ka) R[b + NB * ka] = v[ka];
+}
+
+// x step A': back to natural x, keep |h| <= hmax into A
+template <typename T, int NA, int NB>
+__device__ inline void x_stepAp(cplx<T>* A, const cplx<T>* R, int ys, int b, int hmax) {
+  constexpr int N = NA * NB;
+  cplx<T> v[NA];
+#pragma unroll
+  for (int ka = 0; ka < NA; ++ka) v[ka] = R[b + NB * ka];
+  Dft<T, NA>::run(v, -1);
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    int n = b + NB * a;
+    int h = n <= hmax ? n : (n >= N - hmax ? n - N : N);
+    if (h < N) A[(h + hmax) * N + ys] = v[a];
+  }
+}
+
+template <typename T, typename VT, int NA, int NB>
+__global__ void __launch_bounds__(kFastThreads) k_plane_f(FastArgs a, const VT* __restrict__ Vp,
+                                                          const cplx<T>* __restrict__ t1,
+                                                          cplx<T>* __restrict__ t2) {
+  constexpr int N = NA * NB;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* A = tw + N;           // [nH][N]
+  cplx<T>* R = A + a.nH * N;     // [rows][N]
+  const int zs = blockIdx.x, bp = blockIdx.y;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = mk<T>((T)a.tw[2 * k], (T)a.tw[2 * k + 1]);
+  for (int i = threadIdx.x; i < a.nH * N; i += blockDim.x) A[i] = mk<T>(0, 0);
+  __syncthreads();
+  const cplx<T>* in = t1 + ((size_t)bp * N + zs) * a.ncol;
+  for (int cl = threadIdx.x; cl < a.ncol; cl += blockDim.x)
+    A[a.col_hi[cl] * N + a.col_kmod[cl]] = in[cl];
+  __syncthreads();
+  plane_y_inverse<T, NA, NB>(A, a.nH, tw);
+  const VT* Vz = Vp + (size_t)zs * N * N;
+  for (int y0 = 0; y0 < N; y0 += a.rows) {
+    const int rows = min(a.rows, N - y0);
+    for (int it = threadIdx.x; it < rows * NB; it += blockDim.x) {
+      int r = it / NB;
+      x_stepA<T, NA, NB>(A, R + r * N, y0 + r, it % NB, a.hmax, tw);
+    }
+    __syncthreads();
+    for (int it = threadIdx.x; it < rows * NA; it += blockDim.x) {
+      const int r = it / NA, ka = it % NA;
+      cplx<T>* L = R + r * N + NB * ka;
+      const VT* vrow = Vz + (size_t)(y0 + r) * N + NB * ka;
+      cplx<T> v[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) v[b] = L[b];
+      Dft<T, NB>::run(v, +1);
+#pragma unroll
+      for (int kb = 0; kb < NB; ++kb) {
+        T s = (T)vrow[kb];
+        v[kb].x *= s;
+        v[kb].y *= s;
+      }
+      Dft<T, NB>::run(v, -1);
+#pragma unroll
+      for (int b = 1; b < NB; ++b)
+        if (ka) v[b] = cmul(v[b], twid(tw, b * ka, -1));
+#pragma unroll
+      for (int b = 0; b < NB; ++b) L[b] = v[b];
+    }
+    __syncthreads();
+    for (int it = threadIdx.x; it < rows * NB; it += blockDim.x) {
+      int r = it / NB;
+      x_stepAp<T, NA, NB>(A, R + r * N, y0 + r, it % NB, a.hmax);
+    }
+    __syncthreads();
+  }
+  for (int it = threadIdx.x; it < a.nH * NA; it += blockDim.x)
+    pn_stepB<T, NA, NB>(A + (it / NA) * N, it % NA, tw, -1);
+  __syncthreads();
+  for (int it = threadIdx.x; it < a.nH * NB; it += blockDim.x)
+    pn_stepA<T, NA, NB>(A + (it / NB) * N, it % NB, -1);
+  __syncthreads();
+  cplx<T>* out = t2 + ((size_t)bp * N + zs) * a.ncol;
+  for (int cl = threadIdx.x; cl < a.ncol; cl += blockDim.x)
+    out[cl] = A[a.col_hi[cl] * N + a.col_kmod[cl]];
+}
+
+// ---- z forward: T2 -> split bands, x1/N, + |G|^2 psi --------------------
+template <typename T, int NA, int NB>
+__global__ void __launch_bounds__(kFastThreads) k_zfwd_f(FastArgs a, const double* __restrict__ psi,
+                                                         const cplx<T>* __restrict__ t2,
+                                                         double* __restrict__ out, int accumulate,
+                                                         double invN) {
+  constexpr int N = NA * NB;
+  constexpr int PB = zpairs(N, (int)sizeof(cplx<T>));
+  __shared__ cplx<T> tw[N];
+  __shared__ cplx<T> lines[2 * PB][N];
+  const int q0 = blockIdx.x * PB, q1 = min(q0 + PB, a.npair);
+  const int bp = blockIdx.y;
+  const int ba = 2 * (a.bp0 + bp), bb = ba + 1;
+  const int c0 = q0 == 0 ? 0 : 2 * q0 - 1, c1 = 2 * q1 - 1, ncl = c1 - c0;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = mk<T>((T)a.tw[2 * k], (T)a.tw[2 * k + 1]);
+  const cplx<T>* in = t2 + (size_t)bp * N * a.ncol;
+  for (int i = threadIdx.x; i < ncl * N; i += blockDim.x) {
+    int zs = i / ncl, cl = i % ncl;
+    lines[cl][zs] = in[(size_t)zs * a.ncol + c0 + cl];
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < ncl * NA; it += blockDim.x)
+    pn_stepB<T, NA, NB>(lines[it / NA], it % NA, tw, -1);
+  __syncthreads();
+  for (int it = threadIdx.x; it < ncl * NB; it += blockDim.x)
+    pn_stepA<T, NA, NB>(lines[it / NB], it % NB, -1);
+  __syncthreads();
+  const double* pa = psi + (size_t)ba * a.nplwex;
+  const double* pb = psi + (size_t)bb * a.nplwex;
+  double* oa = out + (size_t)ba * a.nplwex;
+  double* ob = out + (size_t)bb * a.nplwex;
+  const bool hasb = bb < a.nb;
+  const int p0 = a.pair_p0[q0], p1 = a.pair_p0[q1];
+  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    int col = a.pt_col[p] - c0, z = a.pt_z[p];
+    cplx<T> zp = lines[col][z];
+    cplx<T> zm = lines[mirror_col_f(col + c0) - c0][z == 0 ? 0 : N - z];
+    double Ar = 0.5 * ((double)zp.x + (double)zm.x) * invN;
+    double Ai = 0.5 * ((double)zp.y - (double)zm.y) * invN;
+    double Br = 0.5 * ((double)zp.y + (double)zm.y) * invN;
+    double Bi = -0.5 * ((double)zp.x - (double)zm.x) * invN;
+    double kin = a.kin[p];
+    double r0 = kin * pa[2 * p] + Ar, r1 = kin * pa[2 * p + 1] + Ai;
+    if (accumulate) { r0 += oa[2 * p]; r1 += oa[2 * p + 1]; }
+    oa[2 * p] = r0;
+    oa[2 * p + 1] = r1;
+    if (hasb) {
+      double s0 = kin * pb[2 * p] + Br, s1 = kin * pb[2 * p + 1] + Bi;
+      if (accumulate) { s0 += ob[2 * p]; s1 += ob[2 * p + 1]; }
+      ob[2 * p] = s0;
+      ob[2 * p + 1] = s1;
+    }
+  }
+}
+
+// ---- density: y inverse, x inverse, accumulate |psi|^2 in slot order --------
+template <typename T, int NA, int NB>
+__global__ void __launch_bounds__(kFastThreads) k_plane_rho_f(FastArgs a,
+                                                              const cplx<T>* __restrict__ t1,
+                                                              int nbp_batch, int bp_per_group,
+                                                              const double* __restrict__ w,
+                                                              double* __restrict__ rho_part) {
+  constexpr int N = NA * NB;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* A = tw + N;
+  cplx<T>* R = A + a.nH * N;
+  const int zs = blockIdx.x, grp = blockIdx.y;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = mk<T>((T)a.tw[2 * k], (T)a.tw[2 * k + 1]);
+  double* rp = rho_part + ((size_t)grp * N + zs) * N * N;
+  const int bpe = min(nbp_batch, (grp + 1) * bp_per_group);
+  for (int bp = grp * bp_per_group; bp < bpe; ++bp) {
+    const int ba = 2 * (a.bp0 + bp), bb = ba + 1;
+    const double wa = w[ba], wb = bb < a.nb ? w[bb] : 0.0;
+    const bool first = bp == grp * bp_per_group;
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.nH * N; i += blockDim.x) A[i] = mk<T>(0, 0);
+    __syncthreads();
+    const cplx<T>* in = t1 + ((size_t)bp * N + zs) * a.ncol;
+    for (int cl = threadIdx.x; cl < a.ncol; cl += blockDim.x)
+      A[a.col_hi[cl] * N + a.col_kmod[cl]] = in[cl];
+    __syncthreads();
+    plane_y_inverse<T, NA, NB>(A, a.nH, tw);
+    for (int y0 = 0; y0 < N; y0 += a.rows) {
+      const int rows = min(a.rows, N - y0);
+      for (int it = threadIdx.x; it < rows * NB; it += blockDim.x) {
+        int r = it / NB;
+        x_stepA<T, NA, NB>(A, R + r * N, y0 + r, it % NB, a.hmax, tw);
+      }
+      __syncthreads();
+      for (int it = threadIdx.x; it < rows * NA; it += blockDim.x) {
+        const int r = it / NA, ka = it % NA;
+        const cplx<T>* L = R + r * N + NB * ka;
+        cplx<T> v[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) v[b] = L[b];
+        Dft<T, NB>::run(v, +1);
+        double* dst = rp + (size_t)(y0 + r) * N + NB * ka;
+#pragma unroll
+        for (int kb = 0; kb < NB; ++kb) {
+          double re = (double)v[kb].x, im = (double)v[kb].y;
+          double acc = first ? 0.0 : dst[kb];
+          dst[kb] = acc + re * re * wa + im * im * wb;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// sum the group partials and undo the slot permutation of all three axes
+__global__ void k_rho_reduce_f(const double* __restrict__ part, int ngrp, int N, int NA, int NB,
+                               double* __restrict__ rho) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long NN = (long)N * N * N;
+  if (i >= NN) return;
+  int xs = (int)(i % N);
+  long r = i / N;
+  int ys = (int)(r % N), zs = (int)(r / N);
+  double s = 0.0;
+  for (int g = 0; g < ngrp; ++g) s += part[(size_t)g * NN + i];
+  long phys = slot_phys(xs, NA, NB) + (long)N * (slot_phys(ys, NA, NB) + (long)N * slot_phys(zs, NA, NB));
+  rho[phys] = s;
+}
+
+// V_perm[zs][ys][xs] = V[phys(zs)][phys(ys)][phys(xs)]
+__global__ void k_permute_v(const double* __restrict__ V, double* __restrict__ Vp,
+                            float* __restrict__ Vpf, int N, int NA, int NB) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long NN = (long)N * N * N;
+  if (i >= NN) return;
+  int xs = (int)(i % N);
+  long r = i / N;
+  int ys = (int)(r % N), zs = (int)(r / N);
+  long phys = slot_phys(xs, NA, NB) + (long)N * (slot_phys(ys, NA, NB) + (long)N * slot_phys(zs, NA, NB));
+  double v = V[phys];
+  Vp[i] = v;
+  Vpf[i] = (float)v;
+}
+
+// --------------------------------------------------------------------------
+// host side
+// --------------------------------------------------------------------------
+bool fast_split(int n, int* na, int* nb) {
+  switch (n) {
+    case 30: *na = 6; *nb = 5; return true;
+    case 60: *na = 6; *nb = 10; return true;
+    case 64: *na = 8; *nb = 8; return true;
+    case 90: *na = 9; *nb = 10; return true;
+    case 96: *na = 12; *nb = 8; return true;
+    case 120: *na = 12; *nb = 10; return true;
+    case 128: *na = 16; *nb = 8; return true;
+    default: return false;
+  }
+}
+
+bool fast_ok(const Ctx* c) {
+  int na, nb;
+  // the x pass folds h and h - N onto one natural index, so the sphere's x
+  // extent must fit in one period
+  return c->n1 == c->n2 && c->n2 == c->n3 && fast_split(c->n1, &na, &nb) && c->d_Vp != nullptr &&
+         c->pc.nH <= c->n1 && std::getenv("QX_GENERIC_FFT") == nullptr;
+}
+
+int fast_permute_v(Ctx* c) {
+  int na, nb;
+  if (!(c->n1 == c->n2 && c->n2 == c->n3 && fast_split(c->n1, &na, &nb))) return 0;
+  if (!c->d_Vp) {
+    QX_CHECK(hipMalloc(&c->d_Vp, sizeof(double) * c->N));
+    QX_CHECK(hipMalloc(&c->d_Vpf, sizeof(float) * c->N));
+  }
+  hipLaunchKernelGGL(k_permute_v, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, c->d_V,
+                     c->d_Vp, c->d_Vpf, c->n1, na, nb);
+  QX_CHECK(hipGetLastError());
+  return 0;
+}
+
+static FastArgs fast_args(Ctx* c, int nb, int bp0, int rows) {
+  FastArgs a{};
+  a.N = c->n1;
+  a.nH = c->pc.nH;
+  a.hmax = (c->pc.nH - 1) / 2;
+  a.npair = c->pc.npair;
+  a.ncol = c->pc.ncol;
+  a.nplwex = c->nplwex;
+  a.nb = nb;
+  a.bp0 = bp0;
+  a.rows = rows;
+  a.pt_col = c->d_pt_col;
+  a.pt_z = c->d_pt_z;
+  a.pair_p0 = c->d_pair_p0;
+  a.col_hi = c->d_col_hi;
+  a.col_kmod = c->d_col_kmod;
+  a.kin = c->d_kin;
+  a.tw = c->d_tw;  // first n1 entries: exp(+2 pi i k/N)
+  return a;
+}
+
+template <typename T>
+static size_t fast_plane_lds(int N, int nH, int rows) {
+  return sizeof(cplx<T>) * ((size_t)N + (size_t)nH * N + (size_t)rows * N);
+}
+
+template <typename T>
+static int fast_rows(int N, int nH) {
+  // LDS budget of the plane kernel: the sphere tile A is fixed, the rows of
+  // the x pass are chunked to fit (two blocks per CU in FP32)
+  size_t budget = sizeof(T) == 4 ? 76 * 1024 : 120 * 1024;
+  if (const char* e = std::getenv("QX_FAST_LDS_KB")) budget = (size_t)std::atoi(e) * 1024;
+  int rows = N;
+  while (rows > 2 && fast_plane_lds<T>(N, nH, rows) > budget) rows = (rows + 1) / 2;
+  if (fast_plane_lds<T>(N, nH, rows) > 150 * 1024) return -1;
+  return rows;
+}
+
+template <typename T, int NA, int NB>
+static int local_fast_nn(Ctx* c, const double* psi, double* out, int nb, int accumulate) {
+  constexpr int N = NA * NB;
+  const int nbp = (nb + 1) / 2;
+  const int rows = fast_rows<T>(N, c->pc.nH);
+  if (rows < 1) return -3;
+  const size_t per_bp = sizeof(cplx<T>) * (size_t)N * c->pc.ncol;
+  const int batch = (int)std::max<size_t>(1, std::min<size_t>(c->t_bytes / per_bp, (size_t)nbp));
+  constexpr int PB = zpairs(N, (int)sizeof(cplx<T>));
+  const unsigned gz = (unsigned)((c->pc.npair + PB - 1) / PB);
+  const size_t lp = fast_plane_lds<T>(N, c->pc.nH, rows);
+  if (lp > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_plane_f<T, T, NA, NB>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lp);
+  cplx<T>* t1 = reinterpret_cast<cplx<T>*>(c->d_t1);
+  cplx<T>* t2 = reinterpret_cast<cplx<T>*>(c->d_t2);
+  const T* Vp = sizeof(T) == 8 ? reinterpret_cast<const T*>(c->d_Vp) : reinterpret_cast<const T*>(c->d_Vpf);
+  for (int bp0 = 0; bp0 < nbp; bp0 += batch) {
+    const int nbb = std::min(batch, nbp - bp0);
+    FastArgs a = fast_args(c, nb, bp0, rows);
+    hipLaunchKernelGGL((k_zinv_f<T, NA, NB>), dim3(gz, nbb), dim3(kFastThreads), 0, c->stream, a,
+                       psi, t1);
+    hipLaunchKernelGGL((k_plane_f<T, T, NA, NB>), dim3(N, nbb), dim3(kFastThreads), lp, c->stream,
+                       a, Vp, t1, t2);
+    hipLaunchKernelGGL((k_zfwd_f<T, NA, NB>), dim3(gz, nbb), dim3(kFastThreads), 0, c->stream, a,
+                       psi, t2, out, accumulate, 1.0 / (double)c->N);
+  }
+  QX_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <typename T, int NA, int NB>
+static int density_fast_nn(Ctx* c, const double* psi, int nb, const double* w) {
+  constexpr int N = NA * NB;
+  const int nbp = (nb + 1) / 2;
+  const int rows = fast_rows<T>(N, c->pc.nH);
+  if (rows < 1) return -3;
+  const size_t per_bp = sizeof(cplx<T>) * (size_t)N * c->pc.ncol;
+  const int batch = (int)std::max<size_t>(1, std::min<size_t>(c->t_bytes / per_bp, (size_t)nbp));
+  constexpr int PB = zpairs(N, (int)sizeof(cplx<T>));
+  const unsigned gz = (unsigned)((c->pc.npair + PB - 1) / PB);
+  const size_t lp = fast_plane_lds<T>(N, c->pc.nH, rows);
+  if (lp > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_plane_rho_f<T, NA, NB>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lp);
+  cplx<T>* t1 = reinterpret_cast<cplx<T>*>(c->d_t1);
+  const int nbatch = (nbp + batch - 1) / batch;
+  const int groups_per_batch = std::max(1, c->rho_groups / nbatch);
+  if (groups_per_batch * nbatch > c->rho_groups) return -4;
+  int nparts = 0;
+  for (int bp0 = 0; bp0 < nbp; bp0 += batch) {
+    const int nbb = std::min(batch, nbp - bp0);
+    FastArgs a = fast_args(c, nb, bp0, rows);
+    hipLaunchKernelGGL((k_zinv_f<T, NA, NB>), dim3(gz, nbb), dim3(kFastThreads), 0, c->stream, a,
+                       psi, t1);
+    int ng = std::min(groups_per_batch, nbb);
+    int per = (nbb + ng - 1) / ng;
+    ng = (nbb + per - 1) / per;
+    hipLaunchKernelGGL((k_plane_rho_f<T, NA, NB>), dim3(N, ng), dim3(kFastThreads), lp, c->stream,
+                       a, t1, nbb, per, w, c->d_rho_part + (size_t)nparts * c->N);
+    nparts += ng;
+  }
+  hipLaunchKernelGGL(k_rho_reduce_f, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream,
+                     c->d_rho_part, nparts, N, NA, NB, c->d_rho);
+  QX_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+static int local_fast_t(Ctx* c, const double* psi, double* out, int nb, int accumulate) {
+  switch (c->n1) {
+    case 30: return local_fast_nn<T, 6, 5>(c, psi, out, nb, accumulate);
+    case 60: return local_fast_nn<T, 6, 10>(c, psi, out, nb, accumulate);
+    case 64: return local_fast_nn<T, 8, 8>(c, psi, out, nb, accumulate);
+    case 90: return local_fast_nn<T, 9, 10>(c, psi, out, nb, accumulate);
+    case 96: return local_fast_nn<T, 12, 8>(c, psi, out, nb, accumulate);
+    case 120: return local_fast_nn<T, 12, 10>(c, psi, out, nb, accumulate);
+    case 128: return local_fast_nn<T, 16, 8>(c, psi, out, nb, accumulate);
+    default: return -10;
+  }
+}
+
+template <typename T>
+static int density_fast_t(Ctx* c, const double* psi, int nb, const double* w) {
+  switch (c->n1) {
+    case 30: return density_fast_nn<T, 6, 5>(c, psi, nb, w);
+    case 60: return density_fast_nn<T, 6, 10>(c, psi, nb, w);
+    case 64: return density_fast_nn<T, 8, 8>(c, psi, nb, w);
+    case 90: return density_fast_nn<T, 9, 10>(c, psi, nb, w);
+    case 96: return density_fast_nn<T, 12, 8>(c, psi, nb, w);
+    case 120: return density_fast_nn<T, 12, 10>(c, psi, nb, w);
+    case 128: return density_fast_nn<T, 16, 8>(c, psi, nb, w);
+    default: return -10;
+  }
+}
+
+int local_apply_fast(Ctx* c, const double* psi, double* out, int nb, int prec, int accumulate) {
+  if (prec == PREC_FP32) return local_fast_t<float>(c, psi, out, nb, accumulate);
+  return local_fast_t<double>(c, psi, out, nb, accumulate);
+}
+
+int density_fast(Ctx* c, const double* psi, int nb, const double* w, int prec) {
+  if (prec == PREC_FP32) return density_fast_t<float>(c, psi, nb, w);
+  return density_fast_t<double>(c, psi, nb, w);
+}
+
+}  // namespace qx

@@ -16,10 +16,10 @@ def pytest_configure(config):
 class Sphere:
     """G sphere for a synthetic Si cell as the oracle computes it."""
 
-    def __init__(self, ncell, ecut=20.0, ecutdens=80.0):
+    def __init__(self, ncell, ecut=20.0, ecutdens=80.0, cell=None):
         import oracle as O
         from qxmd_amd.system import diamond_si
-        self.cell = diamond_si(ncell)
+        self.cell = diamond_si(ncell) if cell is None else cell
         self.nd = O.get_mesh(self.cell.hcell, ecutdens)
         s = O.strvec(self.cell.hcell, ecutdens, ecut)
         self.__dict__.update(s)

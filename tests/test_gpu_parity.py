@@ -254,3 +254,50 @@ def test_rseigqr(si8, case):
     np.testing.assert_allclose(w, wr, rtol=0, atol=1e-11 * np.abs(wr).max())
     assert np.abs(z.T @ z - np.eye(n)).max() < 1e-10
     assert np.abs(a @ z - z * w).max() < 1e-10 * np.abs(wr).max()
+
+
+def test_generic_fft_path(si64, monkeypatch):
+    """The generic Stockham pencil path (used for grids without a two-factor
+    split) must agree with the oracle too."""
+    monkeypatch.setenv("QX_GENERIC_FFT", "1")
+    nb = 7
+    gs, ctx, H, v = _setup(si64, nb, kb=False)
+    psi, _ = O.schmidt(initial_wavefunctions(si64, nb))
+    ctx.put(0, psi)
+    ctx.local(0, 1, nb)
+    assert _relerr(ctx.get(1, nb), H.hpsi(psi)) < 1e-12
+    occ = occupations(nb, 10)
+    rho = ctx.density(0, nb, occ, si64.cell.volume)
+    ref = O.softchg(psi, si64.nplw, occ, si64.cell.volume, si64.ijk, si64.nd)
+    assert _relerr(rho, ref) < 1e-12
+
+
+def test_noncubic_cell():
+    """Orthorhombic two-cell supercell: grid 30 x 30 x 60 goes through the
+    generic pencil path."""
+    import conftest
+    from qxmd_amd.system import SiCell, diamond_si
+    c1 = diamond_si(1)
+    tau = np.concatenate([c1.tau, c1.tau + np.array([0.0, 0.0, 10.26])])
+    sph = conftest.Sphere(1, cell=SiCell(ncell=1, hcell=np.diag([10.26, 10.26, 20.52]), tau=tau))
+    assert sph.nd == (30, 30, 60)
+    nb = 5
+    gs, ctx, H, v = _setup(sph, nb, kb=True)
+    psi = initial_wavefunctions(sph, nb)
+    ctx.put(0, psi)
+    ctx.hpsi(0, 1, nb)
+    assert _relerr(ctx.get(1, nb), H.hpsi(psi)) < 1e-12
+
+
+@pytest.mark.parametrize("prec", [pw.FP64, pw.MIXED])
+def test_fast_path_si216_sample(prec):
+    """The 90^3 two-step (9 x 10) path on the 216-atom cell, a few bands."""
+    import conftest
+    sph = conftest.Sphere(3)
+    nb = 6
+    gs, ctx, H, v = _setup(sph, nb, kb=False)
+    psi = initial_wavefunctions(sph, nb)
+    ctx.put(0, psi)
+    ctx.local(0, 1, nb, prec=prec)
+    tol = 1e-12 if prec == pw.FP64 else 3e-6
+    assert _relerr(ctx.get(1, nb), H.hpsi(psi)) < tol
