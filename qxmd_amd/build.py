@@ -20,7 +20,7 @@ OUT = os.path.join(HERE, "libqxmd_amd.so")
 BLD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("QX_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["pencil.hip", "pencil_fast.hip", "gemm.hip", "dense.hip", "fullgrid.hip", "cg.hip", "api.hip"]
+HIP_SOURCES = ["pencil.hip", "pencil_fast.hip", "gemm.hip", "dense.hip", "eigh.hip", "fullgrid.hip", "cg.hip", "api.hip"]
 HOST_SOURCES = ["gsphere.cpp"]
 
 HIPCC_FLAGS = [

@@ -108,11 +108,12 @@ static int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host) {
   int rc = gemm_tn(c, nb, nb, n, 2.0, psi, n, psi, n, 0.0, S, nb, 1);
   if (rc) return rc;
   if ((rc = dense_chol_inv(c, S, nb, Linv, ok, flag))) return rc;
-  int okh = 0;
-  QX_CHECK(hipMemcpyAsync(&okh, ok, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  int okh[2] = {0, 0};
+  QX_CHECK(hipMemcpyAsync(okh, ok, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   QX_CHECK(hipStreamSynchronize(c->stream));
-  *ok_host = okh;
-  if (!okh) return 0;
+  if (okh[1]) return -20;  // a grid barrier of the Cholesky gave up
+  *ok_host = okh[0];
+  if (!okh[0]) return 0;
   // psi'_ib = sum_jb Linv(ib,jb) psi_jb: gemm_nn's B[n*ldb + k] = Linv(n,k)
   double* out = c->d_slot[5];
   if ((rc = gemm_nn(c, n, nb, nb, 1.0, psi, n, Linv, nb, 0.0, out, n))) return rc;
@@ -238,6 +239,10 @@ int qx_create(qx_ctx** out, int n1, int n2, int n3, int nplw, const int* nga, co
   c->rho_groups = std::max(1, std::min(nbp, 32));
   QX_CHECK(hipMalloc(&c->d_rho_part, sizeof(double) * (size_t)c->rho_groups * c->N));
   QX_CHECK(hipMalloc(&c->d_rho, sizeof(double) * c->N));
+  QX_CHECK(hipMalloc(&c->d_sync, sizeof(int) * 16));
+  QX_CHECK(hipMemset(c->d_sync, 0, sizeof(int) * 16));
+  c->dwork_len = 8 * (size_t)c->nbmax * c->nbmax + 16 * (size_t)c->nbmax + 64;
+  QX_CHECK(hipMalloc(&c->d_dwork, sizeof(double) * c->dwork_len));
   for (auto& e : c->ev) QX_CHECK(hipEventCreate(&e));
   *out = h;
   return 0;
@@ -249,7 +254,7 @@ int qx_destroy(qx_ctx* h) {
   (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {c->d_perm, c->d_pt_col, c->d_pt_z, c->d_pair_p0, c->d_col_hi, c->d_col_kmod,
                   c->d_kin, c->d_tw, c->d_V, c->d_Vf, c->d_Vp, c->d_Vpf, c->d_B, c->d_D, c->d_t1, c->d_t2, c->d_P,
-                  c->d_small, c->d_rho_part, c->d_rho, c->d_scal};
+                  c->d_small, c->d_rho_part, c->d_rho, c->d_scal, c->d_sync, c->d_dwork};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto* p : c->d_slot)
@@ -589,9 +594,12 @@ int qx_icholesky(qx_ctx* h, double* s, int n, int* ok) {
   int rc = dense_chol_inv(c, S, n, X, okd, okd + 2);
   if (rc) return rc;
   std::vector<double> xr((size_t)n * n);
+  int okh[2] = {0, 0};
   QX_CHECK(hipMemcpyAsync(xr.data(), X, sizeof(double) * xr.size(), hipMemcpyDeviceToHost, c->stream));
-  QX_CHECK(hipMemcpyAsync(ok, okd, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  QX_CHECK(hipMemcpyAsync(okh, okd, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   QX_CHECK(hipStreamSynchronize(c->stream));
+  if (okh[1]) return fail(-20, "qx_icholesky: a grid barrier of the Cholesky gave up");
+  *ok = okh[0];
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < n; ++i) s[i + (size_t)n * j] = xr[(size_t)i * n + j];  // row-major -> col-major
   return 0;

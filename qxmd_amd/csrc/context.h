@@ -60,6 +60,9 @@ struct Ctx {
   int rho_groups = 0;
   double *d_rho = nullptr;                // density (N)
   double *d_scal = nullptr;               // scalar work (device)
+  int* d_sync = nullptr;                  // grid-barrier counter and error flags
+  double* d_dwork = nullptr;              // dense eigensolver work (8 nbmax^2 + ...)
+  size_t dwork_len = 0;
   hipEvent_t ev[8] = {};
 };
 
@@ -75,6 +78,8 @@ int local_apply_fast(Ctx* c, const double* psi, double* out, int nb, int prec, i
 int density_fast(Ctx* c, const double* psi, int nb, const double* w, int prec);
 int kb_apply(Ctx* c, const double* psi, double* out, int nb, int accumulate_into_out,
              double* slmir_out);
+int coop_launch(Ctx* c, const void* fn, int want, size_t lds, void** args);
+int dense_chol_inv(Ctx* c, double* s, int n, double* x, int* ok, int* flag);
 int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
             int ldb, double beta, double* C, int ldc, int corr);
 int gemm_nn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,

@@ -14,8 +14,10 @@
 //     back-transformation by the stored reflectors (parallel over columns).
 //     Eigenvalues come out ascending with their vectors, as RsEIGQR's sort.
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 #include "context.h"
+#include "coop.h"
 
 namespace qx {
 
@@ -218,11 +220,168 @@ __global__ void k_zero_upper_rm(double* x, int n) {
 __global__ void k_flag_clear(int* f) { f[0] = 0; }
 __global__ void k_flag_to_ok(const int* fail, int* ok) { ok[0] = fail[0] ? 0 : 1; }
 
-// s: n x n column-major (lower valid; modified), x: n x n row-major output
-// L^-1; ok[0] = lcholesky.  flag: device int scratch.
-int dense_chol_inv(Ctx* c, double* s, int n, double* x, int* ok, int* flag) {
+// ---- cooperative Cholesky + inverse: one persistent launch -----------------
+// Per block column b: (A) workgroup 0 factors the diagonal block and inverts
+// it in LDS; (B) the other workgroups form the panel below, L_rb = S_rb X_bb^T,
+// and block row b of the inverse, X_bj = -X_bb sum_{k=j}^{b-1} L_bk X_kj
+// (j < b), both as 64 x 64 MFMA tiles; (C) they update the trailing lower
+// triangle, S_rc -= L_rb L_cb^T.  Three grid barriers per block column.
+
+// diagonal block in LDS: P = S_bb -> L_bb, Q = I -> L_bb^-1 (column sweep:
+// row k of Q is final once divided by L_kk, then rows below subtract L_ik Q_k)
+__device__ void chol_diag_block(double* s, int n, int k0, int kb, double* x, int* fail,
+                                double (*P)[NBK + 1], double (*Q)[NBK + 1]) {
+  const int t = threadIdx.x;
+  for (int idx = t; idx < NBK * NBK; idx += blockDim.x) {
+    const int i = idx & (NBK - 1), j = idx / NBK;
+    P[i][j] = (i < kb && j < kb && i >= j) ? s[(size_t)(k0 + i) + (size_t)n * (k0 + j)] : 0.0;
+    Q[i][j] = (i == j && i < kb) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const int rr = t & 63, part = t >> 6;
+  for (int k = 0; k < kb; ++k) {
+    const double dkk = P[k][k];
+    const bool bad = !(dkk > 1e-30);
+    const double lkk = bad ? 0.0 : sqrt(dkk), il = bad ? 0.0 : 1.0 / lkk;
+    __syncthreads();
+    if (t == 0) {
+      P[k][k] = lkk;
+      if (bad) fail[0] = 1;
+    }
+    if (t > k && t < kb) P[t][k] *= il;
+    if (t <= k) Q[k][t] *= il;
+    __syncthreads();
+    const int i = k + 1 + rr;
+    if (i < kb) {
+      const double lik = P[i][k];
+      for (int c = k + 1 + part; c <= i; c += 4) P[i][c] -= lik * P[c][k];
+      for (int c = part; c <= k; c += 4) Q[i][c] -= lik * Q[k][c];
+    }
+    __syncthreads();
+  }
+  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
+    const int i = idx % kb, j = idx / kb;
+    if (i >= j) s[(size_t)(k0 + i) + (size_t)n * (k0 + j)] = P[i][j];
+    x[(size_t)(k0 + i) * n + (k0 + j)] = Q[i][j];
+  }
+  __syncthreads();
+}
+
+constexpr size_t kCholLds = sizeof(double) * (32 * kTileLd + 2 * NBK * (NBK + 1));
+
+__global__ void __launch_bounds__(256) k_chol_inv_coop(double* __restrict__ s, int n,
+                                                       double* __restrict__ x, int* fail, int* bar,
+                                                       int* err) {
+  extern __shared__ double smem[];
+  auto As = reinterpret_cast<double (*)[kTileLd]>(smem);
+  auto Bs = reinterpret_cast<double (*)[kTileLd]>(smem + 16 * kTileLd);
+  auto P = reinterpret_cast<double (*)[NBK + 1]>(smem + 32 * kTileLd);
+  auto Q = reinterpret_cast<double (*)[NBK + 1]>(smem + 32 * kTileLd + NBK * (NBK + 1));
+  const int G = gridDim.x, g = blockIdx.x, t = threadIdx.x;
+  const int nblk = (n + NBK - 1) / NBK;
+  int nbar = 0;
+  for (size_t i = (size_t)g * blockDim.x + t; i < (size_t)n * n; i += (size_t)G * blockDim.x)
+    x[i] = 0.0;
+  grid_barrier(bar, ++nbar * G, err);
+  for (int b = 0; b < nblk; ++b) {
+    const int k0 = b * NBK, kb = min(NBK, n - k0);
+    if (g == 0) chol_diag_block(s, n, k0, kb, x, fail, P, Q);
+    grid_barrier(bar, ++nbar * G, err);
+    const int nrb = nblk - b - 1;
+    for (int task = g; task < nrb + b; task += G) {
+      if (task < nrb) {
+        const int r0 = k0 + kb + task * NBK, nr = min(NBK, n - r0);
+        Tile64 T;
+        T.zero();
+        T.run(
+            kb,
+            [&](int m, int k) { return m < nr ? s[(size_t)(r0 + m) + (size_t)n * (k0 + k)] : 0.0; },
+            [&](int k, int c) { return c < kb ? x[(size_t)(k0 + c) * n + (k0 + k)] : 0.0; }, As, Bs);
+        T.each([&](int m, int c, double v) {
+          if (m < nr && c < kb) s[(size_t)(r0 + m) + (size_t)n * (k0 + c)] = v;
+        });
+      } else {
+        const int j0 = (task - nrb) * NBK;  // a full block column left of b
+        Tile64 T;
+        T.zero();
+        T.run(
+            k0 - j0,
+            [&](int m, int k) { return m < kb ? s[(size_t)(k0 + m) + (size_t)n * (j0 + k)] : 0.0; },
+            [&](int k, int c) { return x[(size_t)(j0 + k) * n + (j0 + c)]; }, As, Bs);
+        T.each([&](int m, int c, double v) { P[m][c] = v; });
+        __syncthreads();
+        Tile64 U;
+        U.zero();
+        U.run(
+            kb,
+            [&](int m, int k) { return m < kb ? x[(size_t)(k0 + m) * n + (k0 + k)] : 0.0; },
+            [&](int k, int c) { return P[k][c]; }, As, Bs);
+        U.each([&](int m, int c, double v) {
+          if (m < kb) x[(size_t)(k0 + m) * n + (j0 + c)] = -v;
+        });
+        __syncthreads();
+      }
+    }
+    grid_barrier(bar, ++nbar * G, err);
+    const int ntile = nrb * (nrb + 1) / 2;
+    for (int task = g; task < ntile; task += G) {
+      int ri = (int)((sqrt(8.0 * task + 1.0) - 1.0) * 0.5);
+      while ((ri + 1) * (ri + 2) / 2 <= task) ++ri;
+      while (ri * (ri + 1) / 2 > task) --ri;
+      const int ci = task - ri * (ri + 1) / 2;
+      const int r0 = k0 + kb + ri * NBK, c0 = k0 + kb + ci * NBK;
+      const int nr = min(NBK, n - r0), nc = min(NBK, n - c0);
+      Tile64 T;
+      T.zero();
+      T.run(
+          kb,
+          [&](int m, int k) { return m < nr ? s[(size_t)(r0 + m) + (size_t)n * (k0 + k)] : 0.0; },
+          [&](int k, int c) { return c < nc ? s[(size_t)(c0 + c) + (size_t)n * (k0 + k)] : 0.0; },
+          As, Bs);
+      T.each([&](int m, int c, double v) {
+        if (m < nr && c < nc) s[(size_t)(r0 + m) + (size_t)n * (c0 + c)] -= v;
+      });
+    }
+    grid_barrier(bar, ++nbar * G, err);
+  }
+}
+
+// sync[0] barrier counter, sync[1] barrier error of the current launch,
+// sync[2] sticky error of all launches since the host last looked
+__global__ void k_coop_reset(int* sync) {
+  sync[2] |= sync[1];
+  sync[0] = 0;
+  sync[1] = 0;
+}
+__global__ void k_chol_flags(const int* fail, const int* sync, int* ok) {
+  ok[0] = (fail[0] || sync[1]) ? 0 : 1;
+  ok[1] = sync[1];
+}
+
+int coop_launch(Ctx* c, const void* fn, int want, size_t lds, void** args) {
+  if (std::getenv("QX_DENSE_LEGACY")) return -1;
+  int dev = 0, ncu = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+  if (lds > 65536 &&
+      hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+    return -1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 256, lds) != hipSuccess || per < 1)
+    return -1;
+  const int G = std::max(1, std::min(want, per * ncu));
+  hipLaunchKernelGGL(k_coop_reset, dim3(1), dim3(1), 0, c->stream, c->d_sync);
+  if (hipLaunchCooperativeKernel(fn, dim3(G), dim3(256), args, (unsigned)lds, c->stream) !=
+      hipSuccess) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  return G;
+}
+
+// legacy multi-launch Cholesky + inverse (used when a cooperative launch is
+// not available)
+static int chol_inv_legacy(Ctx* c, double* s, int n, double* x, int* flag) {
   hipStream_t st = c->stream;
-  hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(1), 0, st, flag);
   const int nblk = (n + NBK - 1) / NBK;
   for (int b = 0; b < nblk; ++b) {
     const int k0 = b * NBK, kb = std::min(NBK, n - k0);
@@ -238,7 +397,29 @@ int dense_chol_inv(Ctx* c, double* s, int n, double* x, int* ok, int* flag) {
                      x, n);
   for (int bi = 1; bi < nblk; ++bi)
     hipLaunchKernelGGL(k_trtri_row, dim3(bi), dim3(256), 0, st, s, x, n, bi);
-  hipLaunchKernelGGL(k_flag_to_ok, dim3(1), dim3(1), 0, st, flag, ok);
+  return 0;
+}
+
+// s: n x n column-major (lower valid; modified), x: n x n row-major output
+// L^-1; ok[0] = lcholesky, ok[1] = 1 if a grid barrier gave up.  flag:
+// device int scratch.
+int dense_chol_inv(Ctx* c, double* s, int n, double* x, int* ok, int* flag) {
+  hipStream_t st = c->stream;
+  hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(1), 0, st, flag);
+  const int nblk = (n + NBK - 1) / NBK;
+  int want = 1;
+  for (int b = 0; b < nblk; ++b) {
+    const int nrb = nblk - b - 1;
+    want = std::max(want, std::max(nrb * (nrb + 1) / 2, nrb + b));
+  }
+  int* bar = c->d_sync;
+  int* err = c->d_sync + 1;
+  void* args[] = {&s, &n, &x, &flag, &bar, &err};
+  if (coop_launch(c, reinterpret_cast<const void*>(k_chol_inv_coop), want, kCholLds, args) < 0) {
+    hipLaunchKernelGGL(k_coop_reset, dim3(1), dim3(1), 0, st, c->d_sync);
+    chol_inv_legacy(c, s, n, x, flag);
+  }
+  hipLaunchKernelGGL(k_chol_flags, dim3(1), dim3(1), 0, st, flag, c->d_sync, ok);
   QX_CHECK(hipGetLastError());
   return 0;
 }
@@ -526,8 +707,7 @@ __global__ void __launch_bounds__(256) k_backtrans(const double* a, int n, doubl
 // host driver.  a: n x n column-major (symmetric, destroyed), z: eigenvectors
 // (columns), w: eigenvalues ascending; work >= 12n doubles; iwork >= n+1 ints
 // (device).  Cluster boundaries are found on the host from w.
-int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int* info) {
-  (void)info;
+int dense_eigh_legacy(Ctx* c, double* a, int n, double* z, double* w, double* work) {
   hipStream_t st = c->stream;
   double* d = work;
   double* e = work + n;

@@ -1,0 +1,503 @@
+// eigh.hip -- symmetric eigensolver of the subspace Hamiltonian on the GPU
+// (the role of RsEIGQR, Sources/funcs.f90:380-764: eigenvalues ascending with
+// their vectors), built for many workgroups:
+//
+//   1. Householder tridiagonalisation in one cooperative launch: the rows of
+//      A live in the LDS of the workgroups (row i in workgroup i mod G); each
+//      step publishes a column, every workgroup forms the reflector from it,
+//      the row owners form p = A v and publish it, and every workgroup applies
+//      the rank-2 update A -= v w^T + w v^T to its rows.  Two grid barriers
+//      per column, no pass over A in HBM.
+//   2. Eigenvalues by multisection on Sturm counts: one wavefront per
+//      eigenvalue, its 64 lanes probe 64 points of the bracket at once.
+//   3. Eigenvectors by inverse iteration, one thread per eigenvalue (LU with
+//      partial pivoting of T - lambda I), followed by a shifted CholeskyQR3
+//      re-orthonormalisation of all vectors (Gram matrix and Z L^-T on MFMA);
+//      two rounds.  Numerically equal eigenvalues get shifts pulled apart by
+//      10 eps |lambda| as in LAPACK's dstein.
+//   4. Back-transformation by the stored reflectors, 16 (8, 4 for large n)
+//      columns of Z per workgroup held in LDS, reflectors staged through LDS.
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+#include "context.h"
+#include "coop.h"
+
+namespace qx {
+
+int dense_eigh_legacy(Ctx* c, double* a, int n, double* z, double* w, double* work);
+
+__device__ inline double block_sum256(double v, double* red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  const double s = red[0] + red[1] + red[2] + red[3];
+  __syncthreads();
+  return s;
+}
+
+// ---- 1. tridiagonalisation --------------------------------------------------
+// a: n x n column-major symmetric (full storage).  On exit d, e (e[i] couples
+// i and i+1, e[n-1] = 0) and the unit reflector of step k in a(k+1:n, k).
+__global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, int n,
+                                                      double* __restrict__ d,
+                                                      double* __restrict__ e, double* xg,
+                                                      double* pg, int* bar, int* err) {
+  extern __shared__ double sm[];
+  const int G = gridDim.x, g = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int rpw = (n + G - 1) / G;
+  double* R = sm;                      // R[r * n + j] = A(g + G r, j)
+  double* v = R + (size_t)rpw * n;     // reflector, indexed by global column
+  double* w = v + n;                   // rank-2 partner
+  double* red = w + n;                 // reduction scratch
+  int nbar = 0;
+  for (int r = 0; r < rpw; ++r) {
+    const int i = g + G * r;
+    if (i >= n) break;
+    for (int j = t; j < n; j += 256) R[(size_t)r * n + j] = a[(size_t)i + (size_t)n * j];
+  }
+  __syncthreads();
+  for (int r = t; r < rpw; r += 256) {
+    const int i = g + G * r;
+    if (i >= n) continue;
+    if (i == 0) d[0] = R[0];
+    else st_agent(&xg[i], R[(size_t)r * n]);
+  }
+  grid_barrier(bar, ++nbar * G, err);
+  for (int k = 0; k + 2 < n; ++k) {
+    // reflector of column k, formed redundantly by every workgroup
+    double part = 0.0;
+    for (int j = k + 1 + t; j < n; j += 256) {
+      const double xv = ld_agent(&xg[j]);
+      v[j] = xv;
+      part += xv * xv;
+    }
+    const double nx2 = block_sum256(part, red);
+    const double x0 = v[k + 1];
+    const double alpha = nx2 > 0.0 ? (x0 >= 0.0 ? -sqrt(nx2) : sqrt(nx2)) : 0.0;
+    const double vn2 = nx2 - 2.0 * alpha * x0 + alpha * alpha;
+    const bool skip = !(vn2 > 1e-300);
+    const double inv = skip ? 0.0 : 1.0 / sqrt(vn2);
+    __syncthreads();  // every thread has read v[k+1] before it changes
+    for (int j = k + 1 + t; j < n; j += 256) v[j] = (j == k + 1 ? v[j] - alpha : v[j]) * inv;
+    __syncthreads();
+    if (g == 0) {
+      for (int j = k + 1 + t; j < n; j += 256) a[(size_t)j + (size_t)n * k] = v[j];
+      if (t == 0) e[k] = skip ? x0 : alpha;
+    }
+    if (!skip) {
+      // p_i = sum_j A(i, j) v_j for the owned rows of the trailing block
+      for (int r = wv; r < rpw; r += 4) {
+        const int i = g + G * r;
+        if (i <= k || i >= n) continue;
+        double acc = 0.0;
+        for (int j = k + 1 + lane; j < n; j += 64) acc += R[(size_t)r * n + j] * v[j];
+        acc = wave_sum(acc);
+        if (lane == 0) st_agent(&pg[i], acc);
+      }
+    }
+    grid_barrier(bar, ++nbar * G, err);
+    if (!skip) {
+      double part2 = 0.0;
+      for (int j = k + 1 + t; j < n; j += 256) {
+        const double p = ld_agent(&pg[j]);
+        w[j] = p;
+        part2 += v[j] * p;
+      }
+      const double K = block_sum256(part2, red);
+      for (int j = k + 1 + t; j < n; j += 256) w[j] = 2.0 * (w[j] - K * v[j]);
+      __syncthreads();
+      for (int r = 0; r < rpw; ++r) {
+        const int i = g + G * r;
+        if (i <= k || i >= n) continue;
+        const double vi = v[i], wi = w[i];
+        double* Rr = R + (size_t)r * n;
+        for (int j = k + 1 + t; j < n; j += 256) Rr[j] -= vi * w[j] + wi * v[j];
+      }
+      __syncthreads();
+    }
+    // publish column k+1 below the diagonal, and the final diagonal d[k+1]
+    for (int r = t; r < rpw; r += 256) {
+      const int i = g + G * r;
+      if (i <= k || i >= n) continue;
+      if (i == k + 1) d[k + 1] = R[(size_t)r * n + k + 1];
+      else st_agent(&xg[i], R[(size_t)r * n + k + 1]);
+    }
+    grid_barrier(bar, ++nbar * G, err);
+  }
+  for (int r = t; r < rpw; r += 256) {
+    const int i = g + G * r;
+    if (i == n - 1) {
+      e[n - 2] = R[(size_t)r * n + n - 2];
+      d[n - 1] = R[(size_t)r * n + n - 1];
+      e[n - 1] = 0.0;
+    }
+  }
+}
+
+// ---- 2. eigenvalues ----------------------------------------------------------
+// Gershgorin bracket and scales: bounds = {lo, hi, pivmin, ||T||_inf}
+__global__ void __launch_bounds__(256) k_gersh(const double* d, const double* e, int n,
+                                               double* bounds) {
+  __shared__ double rlo[256], rhi[256], rnm[256], re2[256];
+  double lo = 1e300, hi = -1e300, nm = 0.0, e2 = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < n - 1 ? fabs(e[i]) : 0.0);
+    lo = fmin(lo, d[i] - r);
+    hi = fmax(hi, d[i] + r);
+    nm = fmax(nm, fabs(d[i]) + r);
+    if (i < n - 1) e2 = fmax(e2, e[i] * e[i]);
+  }
+  rlo[threadIdx.x] = lo;
+  rhi[threadIdx.x] = hi;
+  rnm[threadIdx.x] = nm;
+  re2[threadIdx.x] = e2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      rlo[threadIdx.x] = fmin(rlo[threadIdx.x], rlo[threadIdx.x + o]);
+      rhi[threadIdx.x] = fmax(rhi[threadIdx.x], rhi[threadIdx.x + o]);
+      rnm[threadIdx.x] = fmax(rnm[threadIdx.x], rnm[threadIdx.x + o]);
+      re2[threadIdx.x] = fmax(re2[threadIdx.x], re2[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double eps = 2.220446049250313e-16, safmin = 2.2250738585072014e-308;
+    const double pivmin = fmax(safmin * fmax(1.0, re2[0]), safmin);
+    const double tn = rnm[0];
+    bounds[0] = rlo[0] - 2.0 * eps * tn - 4.0 * pivmin;
+    bounds[1] = rhi[0] + 2.0 * eps * tn + 4.0 * pivmin;
+    bounds[2] = pivmin;
+    bounds[3] = tn;
+  }
+}
+
+__device__ inline int sturm_count_ms(const double* d, const double* e, int n, double sigma,
+                                     double pivmin) {
+  int cnt = 0;
+  double q = d[0] - sigma;
+  if (fabs(q) < pivmin) q = -pivmin;
+  cnt += q < 0.0;
+  for (int i = 1; i < n; ++i) {
+    q = d[i] - sigma - e[i - 1] * e[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += q < 0.0;
+  }
+  return cnt;
+}
+
+// one wavefront per eigenvalue j (0-based, ascending): the bracket [lo, hi]
+// holds lambda_j; lane l tests lo + (l+1)(hi-lo)/65, the first point with more
+// than j eigenvalues below it closes the new bracket.
+__global__ void __launch_bounds__(256) k_bisect_ms(const double* __restrict__ d,
+                                                   const double* __restrict__ e, int n,
+                                                   const double* __restrict__ bounds,
+                                                   double* __restrict__ w) {
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= n) return;
+  const double eps = 2.220446049250313e-16;
+  double lo = bounds[0], hi = bounds[1];
+  const double pivmin = bounds[2];
+  for (int it = 0; it < 64; ++it) {
+    if (hi - lo <= 4.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+    const double step = (hi - lo) / 65.0;
+    const double x = lo + step * (double)(lane + 1);
+    const int cnt = sturm_count_ms(d, e, n, x, pivmin);
+    const unsigned long long m = __ballot(cnt > j);
+    const int l = m ? __ffsll((long long)m) - 1 : 64;
+    const double nlo = l > 0 ? lo + step * (double)l : lo;
+    const double nhi = l < 64 ? lo + step * (double)(l + 1) : hi;
+    if (!(nhi - nlo < hi - lo)) break;
+    lo = nlo;
+    hi = nhi;
+  }
+  if (lane == 0) w[j] = 0.5 * (lo + hi);
+}
+
+// ---- 3. eigenvectors ---------------------------------------------------------
+// One thread per eigenvalue j: factor T - lambda_j I (partial pivoting, rows i
+// and i+1 swapped when the subdiagonal dominates), then solve with the column
+// j of X (row-major, X[i*n + j]) as right-hand side, in place, and normalise.
+// first != 0: the right-hand side is a fixed pseudo-random vector instead.
+// lu: 5 n^2 doubles, entry (q, i) of thread j at lu[(q*n + i)*n + j].
+__global__ void __launch_bounds__(256) k_inviter_par(const double* __restrict__ d,
+                                                     const double* __restrict__ e, int n,
+                                                     const double* __restrict__ w,
+                                                     const double* __restrict__ bounds,
+                                                     double* __restrict__ lu,
+                                                     double* __restrict__ X, int first) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double eps = 2.220446049250313e-16;
+  const double pert = eps * bounds[3];
+  double lam = w[j];
+  {
+    int r = 0;
+    for (int q = j; q > 0 && w[q] - w[q - 1] < 10.0 * eps * fabs(w[q]); --q) ++r;
+    lam += (double)r * 10.0 * eps * fabs(lam);
+  }
+  const size_t nn = (size_t)n * n;
+  double* dd = lu + j;           // U diagonal
+  double* du = dd + nn;          // U superdiagonal
+  double* du2 = du + nn;         // U second superdiagonal
+  double* dl = du2 + nn;         // multipliers
+  double* pv = dl + nn;          // 1 when rows i, i+1 were swapped
+#define L_(arr, i) arr[(size_t)(i) * n]
+  for (int i = 0; i < n; ++i) {
+    L_(dd, i) = d[i] - lam;
+    L_(du, i) = i < n - 1 ? e[i] : 0.0;
+    L_(du2, i) = 0.0;
+  }
+  for (int i = 0; i < n - 1; ++i) {
+    const double sub = e[i];
+    const double di = L_(dd, i);
+    if (fabs(di) >= fabs(sub)) {
+      const double f = di != 0.0 ? sub / di : 0.0;
+      L_(pv, i) = 0.0;
+      L_(dl, i) = f;
+      L_(dd, i + 1) -= f * L_(du, i);
+    } else {
+      const double f = di / sub;
+      const double dn = L_(dd, i + 1), un = L_(du, i);
+      L_(pv, i) = 1.0;
+      L_(dl, i) = f;
+      L_(dd, i) = sub;
+      L_(du, i) = dn;
+      L_(dd, i + 1) = un - f * dn;
+      if (i < n - 2) {
+        const double u2 = L_(du, i + 1);
+        L_(du2, i) = u2;
+        L_(du, i + 1) = -f * u2;
+      }
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    const double v = L_(dd, i);
+    if (fabs(v) < pert) L_(dd, i) = v < 0.0 ? -pert : pert;
+  }
+  double* x = X + j;
+  if (first) {
+    for (int i = 0; i < n; ++i) {
+      unsigned h = (unsigned)(i * 2654435761u) ^ (unsigned)(j * 40503u + 12345u);
+      h ^= h >> 13;
+      h *= 0x5bd1e995u;
+      h ^= h >> 15;
+      L_(x, i) = 0.5 + (double)(h & 0xffff) / 65536.0;
+    }
+  }
+  // L U x = P b: forward sweep with the recorded swaps, then back substitution
+  for (int i = 0; i < n - 1; ++i) {
+    const double xi = L_(x, i), xn = L_(x, i + 1), f = L_(dl, i);
+    if (L_(pv, i) == 0.0) {
+      L_(x, i + 1) = xn - f * xi;
+    } else {
+      L_(x, i) = xn;
+      L_(x, i + 1) = xi - f * xn;
+    }
+  }
+  double big = 0.0, x1 = 0.0, x2 = 0.0;  // x1 = x[i+1], x2 = x[i+2]
+  for (int i = n - 1; i >= 0; --i) {
+    double v = L_(x, i);
+    if (i + 1 < n) v -= L_(du, i) * x1;
+    if (i + 2 < n) v -= L_(du2, i) * x2;
+    v /= L_(dd, i);
+    L_(x, i) = v;
+    x2 = x1;
+    x1 = v;
+    big = fmax(big, fabs(v));
+    if (big > 1e150) {  // rescale the solved part and the remaining right-hand side
+      for (int q = 0; q < n; ++q) L_(x, q) *= 1e-150;
+      x1 *= 1e-150;
+      x2 *= 1e-150;
+      big *= 1e-150;
+    }
+  }
+  double s2 = 0.0;
+  for (int i = 0; i < n; ++i) s2 += L_(x, i) * L_(x, i);
+  const double sc = s2 > 0.0 ? 1.0 / sqrt(s2) : 0.0;
+  for (int i = 0; i < n; ++i) L_(x, i) *= sc;
+#undef L_
+}
+
+// out[j*n + i] = in[i*n + j]
+__global__ void __launch_bounds__(256) k_transpose_sq(const double* __restrict__ in, int n,
+                                                      double* __restrict__ out) {
+  __shared__ double tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows per pass
+  for (int r = ty; r < 32; r += 8) {
+    const int i = by + r, j = bx + tx;
+    if (i < n && j < n) tile[r][tx] = in[(size_t)i * n + j];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int j = bx + r, i = by + tx;
+    if (i < n && j < n) out[(size_t)j * n + i] = tile[tx][r];
+  }
+}
+
+__global__ void k_add_diag(double* g, int n, double s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) g[(size_t)i * n + i] += s;
+}
+
+// ---- 4. back-transformation ----------------------------------------------------
+// COLS columns of Z per workgroup, 256 / COLS lanes per column, COLS / 2
+// reflectors staged at a time; LDS = 1.5 COLS n doubles.
+template <int COLS>
+__global__ void __launch_bounds__(256) k_backtrans_lds(const double* __restrict__ a, int n,
+                                                       double* __restrict__ z) {
+  constexpr int LANES = 256 / COLS, REFL = COLS / 2;
+  extern __shared__ double sm[];
+  double* zs = sm;                         // [COLS][n]
+  double* vs = sm + (size_t)COLS * n;      // [REFL][n]
+  const int c0 = blockIdx.x * COLS;
+  const int ncol = min(COLS, n - c0);
+  const int cl = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+  for (int idx = threadIdx.x; idx < COLS * n; idx += 256) {
+    const int cc = idx / n, i = idx - cc * n;
+    zs[idx] = cc < ncol ? z[(size_t)(c0 + cc) * n + i] : 0.0;
+  }
+  double* zc = zs + (size_t)cl * n;
+  for (int kt = n - 3; kt >= 0; kt -= REFL) {
+    const int kl = max(0, kt - REFL + 1);  // reflectors kl..kt
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < (kt - kl + 1) * n; idx += 256) {
+      const int q = idx / n, i = idx - q * n;
+      vs[idx] = i > kl + q ? a[(size_t)(kl + q) * n + i] : 0.0;
+    }
+    __syncthreads();
+    for (int k = kt; k >= kl; --k) {
+      const double* vk = vs + (size_t)(k - kl) * n;
+      // element i belongs to lane i % LANES for every k: no cross-lane LDS traffic
+      const int i0 = k + 1 + ((lane - (k + 1)) % LANES + LANES) % LANES;
+      double part = 0.0;
+      for (int i = i0; i < n; i += LANES) part += vk[i] * zc[i];
+      for (int o = LANES / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, LANES);
+      const double tv = 2.0 * part;
+      for (int i = i0; i < n; i += LANES) zc[i] -= tv * vk[i];
+    }
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < COLS * n; idx += 256) {
+    const int cc = idx / n, i = idx - cc * n;
+    if (cc < ncol) z[(size_t)(c0 + cc) * n + i] = zs[idx];
+  }
+}
+
+template <int COLS>
+static int backtrans(Ctx* c, const double* a, int n, double* z) {
+  const size_t lds = sizeof(double) * (size_t)(COLS + COLS / 2) * n;
+  if (lds > 160 * 1024) return -22;
+  if (lds > 65536)
+    QX_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_backtrans_lds<COLS>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k_backtrans_lds<COLS>, dim3((n + COLS - 1) / COLS), dim3(256), lds, c->stream,
+                     a, n, z);
+  return 0;
+}
+
+__global__ void k_sync_take(int* sync, int* out) {
+  out[0] = sync[1] | sync[2];
+  sync[2] = 0;
+}
+
+// ---- host driver ----------------------------------------------------------------
+static int tridiag_groups(int n) {
+  if (const char* s = std::getenv("QX_TRID_WG")) return std::max(1, std::atoi(s));
+  return std::max(1, (n + 7) / 8);
+}
+
+// z (n x n, column-major, vectors in columns) <- QR factor of z: shifted
+// CholeskyQR followed by two plain CholeskyQR passes
+static int chol_qr3(Ctx* c, double* z, int n, double* gram, double* linv, double* tmp, int* ok) {
+  const double u = 1.1102230246251565e-16;
+  const double shift = 11.0 * (2.0 * n * (double)n + n) * u * (double)n;
+  for (int pass = 0; pass < 3; ++pass) {
+    int rc = gemm_tn(c, n, n, n, 1.0, z, n, z, n, 0.0, gram, n, 0);
+    if (rc) return rc;
+    if (pass == 0)
+      hipLaunchKernelGGL(k_add_diag, dim3((n + 255) / 256), dim3(256), 0, c->stream, gram, n, shift);
+    if ((rc = dense_chol_inv(c, gram, n, linv, ok + 2 * pass, ok + 6))) return rc;
+    if ((rc = gemm_nn(c, n, n, n, 1.0, z, n, linv, n, 0.0, tmp, n))) return rc;
+    QX_CHECK(hipMemcpyAsync(z, tmp, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice,
+                            c->stream));
+  }
+  return 0;
+}
+
+int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int* info) {
+  (void)info;
+  if (n < 3 || std::getenv("QX_DENSE_LEGACY") ||
+      c->dwork_len < 8 * (size_t)n * n + 16 * (size_t)n + 64)
+    return dense_eigh_legacy(c, a, n, z, w, work);
+  hipStream_t st = c->stream;
+  double* dw = c->d_dwork;
+  double* d = dw;
+  double* e = d + n;
+  double* xg = e + n;
+  double* pg = xg + n;
+  double* bounds = pg + n;                                      // 8
+  int* ok = reinterpret_cast<int*>(bounds + 8);                 // 8 ints
+  int* sflag = ok + 8;                                          // 1 int
+  double* X = dw + 16 * (size_t)n + 64;                         // n^2, row-major vectors
+  double* gram = X + (size_t)n * n;                             // n^2
+  double* linv = gram + (size_t)n * n;                          // n^2
+  double* lu = linv + (size_t)n * n;                            // 5 n^2
+  // 1. tridiagonalisation
+  const int G = tridiag_groups(n);
+  const int rpw_min = (n + std::min(G, n) - 1) / std::min(G, n);
+  size_t lds = sizeof(double) * ((size_t)rpw_min * n + 2 * (size_t)n + 8);
+  int* bar = c->d_sync;
+  int* err = c->d_sync + 1;
+  void* args[] = {&a, &n, &d, &e, &xg, &pg, &bar, &err};
+  // the launch may grant fewer groups than asked; size LDS for the worst case
+  // the device can hold (rows per group grow as groups shrink)
+  int dev = 0, ncu = 0;
+  QX_CHECK(hipGetDevice(&dev));
+  QX_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int Gw = std::min(G, std::min(n, ncu));
+  lds = sizeof(double) * ((size_t)((n + Gw - 1) / Gw) * n + 2 * (size_t)n + 8);
+  if (lds > 160 * 1024 - 1024) return dense_eigh_legacy(c, a, n, z, w, work);
+  const int Gl = coop_launch(c, reinterpret_cast<const void*>(k_tridiag_coop), Gw, lds, args);
+  if (Gl < 0) return dense_eigh_legacy(c, a, n, z, w, work);
+  if (Gl != Gw) {
+    // fewer resident groups than sized for: the launch used Gl groups with
+    // more rows each than the LDS holds -- refuse rather than overrun
+    QX_CHECK(hipStreamSynchronize(st));
+    return -21;
+  }
+  // 2. eigenvalues
+  hipLaunchKernelGGL(k_gersh, dim3(1), dim3(256), 0, st, d, e, n, bounds);
+  hipLaunchKernelGGL(k_bisect_ms, dim3((n + 3) / 4), dim3(256), 0, st, d, e, n, bounds, w);
+  // 3. eigenvectors: inverse iteration + CholeskyQR3, twice
+  const dim3 tgrid((n + 31) / 32, (n + 31) / 32);
+  for (int round = 0; round < 2; ++round) {
+    if (round > 0) hipLaunchKernelGGL(k_transpose_sq, tgrid, dim3(256), 0, st, z, n, X);
+    hipLaunchKernelGGL(k_inviter_par, dim3((n + 63) / 64), dim3(64), 0, st, d, e, n, w, bounds, lu,
+                       X, round == 0 ? 1 : 0);
+    hipLaunchKernelGGL(k_transpose_sq, tgrid, dim3(256), 0, st, X, n, z);
+    int rc = chol_qr3(c, z, n, gram, linv, lu, ok);
+    if (rc) return rc;
+  }
+  // 4. back-transformation
+  {
+    int rc = n <= 850 ? backtrans<16>(c, a, n, z) : n <= 1700 ? backtrans<8>(c, a, n, z)
+                                                              : backtrans<4>(c, a, n, z);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_sync_take, dim3(1), dim3(1), 0, st, c->d_sync, sflag);
+  QX_CHECK(hipGetLastError());
+  int hflags[9];
+  QX_CHECK(hipMemcpyAsync(hflags, ok, sizeof(int) * 9, hipMemcpyDeviceToHost, st));
+  QX_CHECK(hipStreamSynchronize(st));
+  if (hflags[8]) return -20;                    // a grid barrier gave up
+  if (!hflags[0] || !hflags[2] || !hflags[4]) return -23;  // Gram matrix not positive definite
+  return 0;
+}
+
+}  // namespace qx

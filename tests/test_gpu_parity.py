@@ -209,8 +209,13 @@ def test_multi_batch_pencil(si64, monkeypatch):
     assert _relerr(rho, ref) < 1e-12
 
 
-@pytest.mark.parametrize("n", [1, 2, 7, 64, 65, 200])
-def test_icholesky(si8, n):
+@pytest.mark.parametrize("path", ["coop", "legacy"])
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 65, 129, 200, 540])
+def test_icholesky(si8, n, path, monkeypatch):
+    """Cholesky + inverse: the one-launch cooperative kernel and the
+    multi-launch fallback against rsdecmp/rsinvcd."""
+    if path == "legacy":
+        monkeypatch.setenv("QX_DENSE_LEGACY", "1")
     gs, ctx, H, v = _setup(si8, max(n, 2), kb=False)
     rng = np.random.default_rng(n)
     x = rng.standard_normal((n + 5, n))
@@ -231,8 +236,11 @@ def test_icholesky_not_positive_definite(si8):
     assert not ok
 
 
+@pytest.mark.parametrize("path", ["coop", "legacy"])
 @pytest.mark.parametrize("case", ["random", "degenerate", "clustered"])
-def test_rseigqr(si8, case):
+def test_rseigqr(si8, case, path, monkeypatch):
+    if path == "legacy":
+        monkeypatch.setenv("QX_DENSE_LEGACY", "1")
     n = 150
     gs, ctx, H, v = _setup(si8, n, kb=False)
     rng = np.random.default_rng(11)
@@ -251,6 +259,21 @@ def test_rseigqr(si8, case):
         a = 0.5 * (a + a.T)
     w, z = ctx.rseigqr(a)
     wr, zr = O.eigh(a)
+    np.testing.assert_allclose(w, wr, rtol=0, atol=1e-11 * np.abs(wr).max())
+    assert np.abs(z.T @ z - np.eye(n)).max() < 1e-10
+    assert np.abs(a @ z - z * w).max() < 1e-10 * np.abs(wr).max()
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 17, 64, 65, 300, 540])
+def test_rseigqr_sizes(si8, n):
+    """The cooperative eigensolver over the band counts the workloads use
+    (20, 160, 540) and odd sizes around the 64-row tiles."""
+    gs, ctx, H, v = _setup(si8, max(n, 4), kb=False)
+    rng = np.random.default_rng(100 + n)
+    x = rng.standard_normal((n, n))
+    a = x + x.T + np.diag(np.linspace(0, 5, n))
+    w, z = ctx.rseigqr(a)
+    wr = np.linalg.eigvalsh(a)
     np.testing.assert_allclose(w, wr, rtol=0, atol=1e-11 * np.abs(wr).max())
     assert np.abs(z.T @ z - np.eye(n)).max() < 1e-10
     assert np.abs(a @ z - z * w).max() < 1e-10 * np.abs(wr).max()
