@@ -96,6 +96,10 @@ int qx_scf_band_step(qx_ctx *ctx, int psi, int hpsi, int nb, const double *occ, 
 /* ---------------- timing on the library stream ---------------------------- */
 int qx_event_record(qx_ctx *ctx, int slot);
 int qx_event_elapsed(qx_ctx *ctx, int slot0, int slot1, float *ms);
+/* phase times of the last SCF band step (ms): energy pass, rotation, CG,
+ * Gram-Schmidt, density -- the role of the ltimecnt phase timers
+ * (Sources/pwlda.f90:1194-1196) */
+int qx_scf_phase_ms(qx_ctx *ctx, float *ms5);
 
 /* ---------------- Fortran-shaped drop-ins (host arrays) -------------------- */
 /* rfft3 (Sources/fft3.f90:113-232): key=1 r2c x1/N with full Hermitian grid

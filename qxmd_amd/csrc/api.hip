@@ -244,6 +244,7 @@ int qx_create(qx_ctx** out, int n1, int n2, int n3, int nplw, const int* nga, co
   c->dwork_len = 8 * (size_t)c->nbmax * c->nbmax + 16 * (size_t)c->nbmax + 64;
   QX_CHECK(hipMalloc(&c->d_dwork, sizeof(double) * c->dwork_len));
   for (auto& e : c->ev) QX_CHECK(hipEventCreate(&e));
+  for (auto& e : c->pev) QX_CHECK(hipEventCreate(&e));
   *out = h;
   return 0;
 }
@@ -260,6 +261,8 @@ int qx_destroy(qx_ctx* h) {
   for (auto* p : c->d_slot)
     if (p) (void)hipFree(p);
   for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->pev)
     if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->stream);
   delete h;
@@ -447,12 +450,16 @@ int qx_scf_band_step_ex(qx_ctx* h, int psi, int hpsi, int nb, const double* occ,
   double* HP = c->d_slot[hpsi];
   std::vector<double> ekib(nb), bz(nb);
   std::vector<int> nr(nb);
+  // phase timers on the context stream (the role of the ltimecnt timers)
+  QX_CHECK(hipEventRecord(c->pev[0], c->stream));
   // energy pass: H psi, ekib
   int rc = hpsi_batched(c, P, HP, nb, prec);
   if (rc) return rc;
   if ((rc = qx_ekib_slot(h, psi, nb, ekib.data()))) return rc;
+  QX_CHECK(hipEventRecord(c->pev[1], c->stream));
   // subspace rotation
   if ((rc = rotate_dev(c, P, HP, nb, eig))) return rc;
+  QX_CHECK(hipEventRecord(c->pev[2], c->stream));
   // CG
   if (batched)
     rc = bkl_batched(c, P, HP, nb, ekib.data(), eig, noband, 4, 4, 1e-10, 0.3, 0.3, bz.data(),
@@ -461,11 +468,19 @@ int qx_scf_band_step_ex(qx_ctx* h, int psi, int hpsi, int nb, const double* occ,
     rc = bkl_band_by_band(c, P, HP, nb, ekib.data(), eig, noband, 4, 4, 1e-10, 0.0, 0.9, 0.3, 0.3,
                           bz.data(), nr.data(), prec);
   if (rc) return rc;
+  QX_CHECK(hipEventRecord(c->pev[3], c->stream));
   if ((rc = schmidt_dev(c, P, nb, lcholesky))) return rc;
-  if (rho || true) {
-    if ((rc = qx_density_slot(h, psi, nb, occ, rvol, rho, prec))) return rc;
-  }
+  QX_CHECK(hipEventRecord(c->pev[4], c->stream));
+  if ((rc = qx_density_slot(h, psi, nb, occ, rvol, rho, prec))) return rc;
+  QX_CHECK(hipEventRecord(c->pev[5], c->stream));
   QX_CHECK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < 5; ++i) QX_CHECK(hipEventElapsedTime(&c->phase_ms[i], c->pev[i], c->pev[i + 1]));
+  return 0;
+}
+
+// phases of the last SCF band step, ms: energy pass, rotation, CG, schmidt, density
+int qx_scf_phase_ms(qx_ctx* h, float* ms) {
+  for (int i = 0; i < 5; ++i) ms[i] = h->c.phase_ms[i];
   return 0;
 }
 

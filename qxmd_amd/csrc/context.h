@@ -64,6 +64,8 @@ struct Ctx {
   double* d_dwork = nullptr;              // dense eigensolver work (8 nbmax^2 + ...)
   size_t dwork_len = 0;
   hipEvent_t ev[8] = {};
+  hipEvent_t pev[6] = {};                 // SCF band-step phase timers
+  float phase_ms[5] = {};
 };
 
 int pencil_build(Ctx* c, const int* nga, const int* ngb, const int* ngc, const double* recnrm);
@@ -78,7 +80,7 @@ int local_apply_fast(Ctx* c, const double* psi, double* out, int nb, int prec, i
 int density_fast(Ctx* c, const double* psi, int nb, const double* w, int prec);
 int kb_apply(Ctx* c, const double* psi, double* out, int nb, int accumulate_into_out,
              double* slmir_out);
-int coop_launch(Ctx* c, const void* fn, int want, size_t lds, void** args);
+int coop_launch(Ctx* c, const void* fn, int want, size_t lds, void** args, bool exact);
 int dense_chol_inv(Ctx* c, double* s, int n, double* x, int* ok, int* flag);
 int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
             int ldb, double beta, double* C, int ldc, int corr);

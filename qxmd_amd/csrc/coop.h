@@ -16,19 +16,23 @@ __device__ inline double ld_agent(const double* p) {
 }
 
 // Grid barrier for a kernel launched with hipLaunchCooperativeKernel (every
-// workgroup resident).  bar[0] counts arrivals and only grows: the b-th
-// barrier of a launch waits until it reaches b * gridDim.x (the counter is
-// cleared by a fill kernel before the launch).  The wait is bounded: after
-// about 2^20 polls the barrier raises err[0], and every barrier after that
-// returns at once, so a launch that cannot complete still ends.  Returns
-// false once err is raised.
+// workgroup resident).  Data that crosses workgroups must go through
+// st_agent / ld_agent (device-scope accesses, coherent across the XCDs'
+// L2s): the barrier then only has to wait until this workgroup's stores are
+// acknowledged before it arrives, with no L2 write-back or invalidation.
+// bar[0] counts arrivals and only grows: the b-th barrier of a launch waits
+// until it reaches b * gridDim.x (the counter is cleared by a fill kernel
+// before the launch).  The wait is bounded: after about 2^20 polls the
+// barrier raises err[0], and every barrier after that returns at once, so a
+// launch that cannot complete still ends.  Returns false once err is raised.
 __device__ inline bool grid_barrier(int* bar, int target, int* err) {
   __shared__ int s_ok;
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);  // this thread's stores acknowledged
   __syncthreads();
   if (threadIdx.x == 0) {
     int ok = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
     if (ok) {
-      __threadfence();
       __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       int spins = 0;
       while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
@@ -44,7 +48,6 @@ __device__ inline bool grid_barrier(int* bar, int target, int* err) {
           break;
         }
       }
-      __threadfence();
     }
     s_ok = ok;
   }
@@ -76,17 +79,25 @@ struct Tile64 {
   __device__ void run(int K, FA fa, FB fb, double (*As)[kTileLd], double (*Bs)[kTileLd]) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-    for (int k0 = 0; k0 < K; k0 += 16) {
-      {
-        const int m = tid >> 2, kk = (tid & 3) * 4;
+    const int m = tid >> 2, kk = (tid & 3) * 4;
+    double ra[4], rb[4];
+    auto fetch = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int k = k0 + kk + u;
-          As[kk + u][m] = k < K ? fa(m, k) : 0.0;
-          Bs[kk + u][m] = k < K ? fb(k, m) : 0.0;
-        }
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + kk + u;
+        ra[u] = k < K ? fa(m, k) : 0.0;
+        rb[u] = k < K ? fb(k, m) : 0.0;
+      }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        As[kk + u][m] = ra[u];
+        Bs[kk + u][m] = rb[u];
       }
       __syncthreads();
+      if (k0 + 16 < K) fetch(k0 + 16);  // next chunk in flight during the MFMAs
 #pragma unroll
       for (int k4 = 0; k4 < 16; k4 += 4) {
         const int kr = k4 + (lane >> 4), c = lane & 15;

@@ -221,20 +221,28 @@ __global__ void k_flag_clear(int* f) { f[0] = 0; }
 __global__ void k_flag_to_ok(const int* fail, int* ok) { ok[0] = fail[0] ? 0 : 1; }
 
 // ---- cooperative Cholesky + inverse: one persistent launch -----------------
-// Per block column b: (A) workgroup 0 factors the diagonal block and inverts
-// it in LDS; (B) the other workgroups form the panel below, L_rb = S_rb X_bb^T,
-// and block row b of the inverse, X_bj = -X_bb sum_{k=j}^{b-1} L_bk X_kj
-// (j < b), both as 64 x 64 MFMA tiles; (C) they update the trailing lower
-// triangle, S_rc -= L_rb L_cb^T.  Three grid barriers per block column.
+// With X = L^-1, X_ij = -X_ii T_ij and T_ij = sum_{k=j}^{i-1} L_ik X_kj.  T is
+// accumulated in place in the lower blocks of X as the factorisation moves
+// right, so every tile product has K = 64.  Block column b takes two phases
+// separated by grid barriers:
+//   (B) the panel, L_rb = S_rb X_bb^T (r > b), and block row b of the
+//       inverse, X_bj = -X_bb T_bj (j < b);
+//   (C) the trailing update S_rc -= L_rb L_cb^T (b < c <= r) and the
+//       accumulation T_ij += L_ib X_bj (i > b, j <= b); the workgroup that
+//       owns the next diagonal tile factors and inverts it in LDS right after
+//       its update (look-ahead), so block b+1 is ready at the barrier.
+// All matrix traffic goes through device-scope accesses (the tiles move
+// between workgroups on different XCDs).
 
-// diagonal block in LDS: P = S_bb -> L_bb, Q = I -> L_bb^-1 (column sweep:
-// row k of Q is final once divided by L_kk, then rows below subtract L_ik Q_k)
+// diagonal block in LDS: P (lower part filled) -> L_bb, Q -> L_bb^-1 (column
+// sweep: row k of Q is final once divided by L_kk, then the rows below
+// subtract L_ik Q_k).  Written to s (lower) and x (full block, zeros above).
 __device__ void chol_diag_block(double* s, int n, int k0, int kb, double* x, int* fail,
                                 double (*P)[NBK + 1], double (*Q)[NBK + 1]) {
   const int t = threadIdx.x;
   for (int idx = t; idx < NBK * NBK; idx += blockDim.x) {
     const int i = idx & (NBK - 1), j = idx / NBK;
-    P[i][j] = (i < kb && j < kb && i >= j) ? s[(size_t)(k0 + i) + (size_t)n * (k0 + j)] : 0.0;
+    if (!(i < kb && j < kb && i >= j)) P[i][j] = 0.0;
     Q[i][j] = (i == j && i < kb) ? 1.0 : 0.0;
   }
   __syncthreads();
@@ -261,8 +269,8 @@ __device__ void chol_diag_block(double* s, int n, int k0, int kb, double* x, int
   }
   for (int idx = t; idx < kb * kb; idx += blockDim.x) {
     const int i = idx % kb, j = idx / kb;
-    if (i >= j) s[(size_t)(k0 + i) + (size_t)n * (k0 + j)] = P[i][j];
-    x[(size_t)(k0 + i) * n + (k0 + j)] = Q[i][j];
+    if (i >= j) st_agent(&s[(size_t)(k0 + i) + (size_t)n * (k0 + j)], P[i][j]);
+    st_agent(&x[(size_t)(k0 + i) * n + (k0 + j)], Q[i][j]);
   }
   __syncthreads();
 }
@@ -280,67 +288,87 @@ __global__ void __launch_bounds__(256) k_chol_inv_coop(double* __restrict__ s, i
   const int G = gridDim.x, g = blockIdx.x, t = threadIdx.x;
   const int nblk = (n + NBK - 1) / NBK;
   int nbar = 0;
+  auto S_ = [&](int i, int j) -> double* { return &s[(size_t)i + (size_t)n * j]; };
+  auto X_ = [&](int i, int j) -> double* { return &x[(size_t)i * n + j]; };
   for (size_t i = (size_t)g * blockDim.x + t; i < (size_t)n * n; i += (size_t)G * blockDim.x)
-    x[i] = 0.0;
+    st_agent(&x[i], 0.0);
+  if (g == 0) {
+    const int kb = min(NBK, n);
+    for (int idx = t; idx < kb * kb; idx += blockDim.x) {
+      const int i = idx % kb, j = idx / kb;
+      if (i >= j) P[i][j] = *S_(i, j);
+    }
+    __syncthreads();
+  }
+  grid_barrier(bar, ++nbar * G, err);  // X zeroed before any block of it is written
+  if (g == 0) chol_diag_block(s, n, 0, min(NBK, n), x, fail, P, Q);
   grid_barrier(bar, ++nbar * G, err);
   for (int b = 0; b < nblk; ++b) {
     const int k0 = b * NBK, kb = min(NBK, n - k0);
-    if (g == 0) chol_diag_block(s, n, k0, kb, x, fail, P, Q);
-    grid_barrier(bar, ++nbar * G, err);
     const int nrb = nblk - b - 1;
+    // (B) panel and block row b of the inverse
     for (int task = g; task < nrb + b; task += G) {
+      Tile64 T;
+      T.zero();
       if (task < nrb) {
         const int r0 = k0 + kb + task * NBK, nr = min(NBK, n - r0);
-        Tile64 T;
-        T.zero();
         T.run(
-            kb,
-            [&](int m, int k) { return m < nr ? s[(size_t)(r0 + m) + (size_t)n * (k0 + k)] : 0.0; },
-            [&](int k, int c) { return c < kb ? x[(size_t)(k0 + c) * n + (k0 + k)] : 0.0; }, As, Bs);
+            kb, [&](int m, int k) { return m < nr ? ld_agent(S_(r0 + m, k0 + k)) : 0.0; },
+            [&](int k, int c) { return c < kb ? ld_agent(X_(k0 + c, k0 + k)) : 0.0; }, As, Bs);
         T.each([&](int m, int c, double v) {
-          if (m < nr && c < kb) s[(size_t)(r0 + m) + (size_t)n * (k0 + c)] = v;
+          if (m < nr && c < kb) st_agent(S_(r0 + m, k0 + c), v);
         });
       } else {
         const int j0 = (task - nrb) * NBK;  // a full block column left of b
-        Tile64 T;
-        T.zero();
         T.run(
-            k0 - j0,
-            [&](int m, int k) { return m < kb ? s[(size_t)(k0 + m) + (size_t)n * (j0 + k)] : 0.0; },
-            [&](int k, int c) { return x[(size_t)(j0 + k) * n + (j0 + c)]; }, As, Bs);
-        T.each([&](int m, int c, double v) { P[m][c] = v; });
-        __syncthreads();
-        Tile64 U;
-        U.zero();
-        U.run(
-            kb,
-            [&](int m, int k) { return m < kb ? x[(size_t)(k0 + m) * n + (k0 + k)] : 0.0; },
-            [&](int k, int c) { return P[k][c]; }, As, Bs);
-        U.each([&](int m, int c, double v) {
-          if (m < kb) x[(size_t)(k0 + m) * n + (j0 + c)] = -v;
+            kb, [&](int m, int k) { return m < kb ? ld_agent(X_(k0 + m, k0 + k)) : 0.0; },
+            [&](int k, int c) { return k < kb ? ld_agent(X_(k0 + k, j0 + c)) : 0.0; }, As, Bs);
+        T.each([&](int m, int c, double v) {
+          if (m < kb) st_agent(X_(k0 + m, j0 + c), -v);
         });
-        __syncthreads();
       }
     }
+    if (nrb == 0) break;
     grid_barrier(bar, ++nbar * G, err);
+    // (C) trailing update with the look-ahead factorisation of block b+1, and
+    // the accumulation of T for the block rows below b
     const int ntile = nrb * (nrb + 1) / 2;
-    for (int task = g; task < ntile; task += G) {
-      int ri = (int)((sqrt(8.0 * task + 1.0) - 1.0) * 0.5);
-      while ((ri + 1) * (ri + 2) / 2 <= task) ++ri;
-      while (ri * (ri + 1) / 2 > task) --ri;
-      const int ci = task - ri * (ri + 1) / 2;
-      const int r0 = k0 + kb + ri * NBK, c0 = k0 + kb + ci * NBK;
-      const int nr = min(NBK, n - r0), nc = min(NBK, n - c0);
+    for (int task = g; task < ntile + nrb * (b + 1); task += G) {
       Tile64 T;
       T.zero();
-      T.run(
-          kb,
-          [&](int m, int k) { return m < nr ? s[(size_t)(r0 + m) + (size_t)n * (k0 + k)] : 0.0; },
-          [&](int k, int c) { return c < nc ? s[(size_t)(c0 + c) + (size_t)n * (k0 + k)] : 0.0; },
-          As, Bs);
-      T.each([&](int m, int c, double v) {
-        if (m < nr && c < nc) s[(size_t)(r0 + m) + (size_t)n * (c0 + c)] -= v;
-      });
+      if (task < ntile) {
+        int ri = (int)((sqrt(8.0 * task + 1.0) - 1.0) * 0.5);
+        while ((ri + 1) * (ri + 2) / 2 <= task) ++ri;
+        while (ri * (ri + 1) / 2 > task) --ri;
+        const int ci = task - ri * (ri + 1) / 2;
+        const int r0 = k0 + kb + ri * NBK, c0 = k0 + kb + ci * NBK;
+        const int nr = min(NBK, n - r0), nc = min(NBK, n - c0);
+        T.run(
+            kb, [&](int m, int k) { return m < nr ? ld_agent(S_(r0 + m, k0 + k)) : 0.0; },
+            [&](int k, int c) { return c < nc ? ld_agent(S_(c0 + c, k0 + k)) : 0.0; }, As, Bs);
+        if (task == 0) {
+          // next diagonal block: updated values straight into LDS, then factor
+          T.each([&](int m, int c, double v) {
+            if (m < nr && c < nc && m >= c) P[m][c] = ld_agent(S_(r0 + m, c0 + c)) - v;
+          });
+          __syncthreads();
+          chol_diag_block(s, n, r0, nr, x, fail, P, Q);
+        } else {
+          T.each([&](int m, int c, double v) {
+            if (m < nr && c < nc) st_agent(S_(r0 + m, c0 + c), ld_agent(S_(r0 + m, c0 + c)) - v);
+          });
+        }
+      } else {
+        const int tt = task - ntile;
+        const int i0 = k0 + kb + (tt / (b + 1)) * NBK, j0 = (tt % (b + 1)) * NBK;
+        const int ni = min(NBK, n - i0), nj = j0 == k0 ? kb : NBK;
+        T.run(
+            kb, [&](int m, int k) { return m < ni ? ld_agent(S_(i0 + m, k0 + k)) : 0.0; },
+            [&](int k, int c) { return c < nj ? ld_agent(X_(k0 + k, j0 + c)) : 0.0; }, As, Bs);
+        T.each([&](int m, int c, double v) {
+          if (m < ni && c < nj) st_agent(X_(i0 + m, j0 + c), ld_agent(X_(i0 + m, j0 + c)) + v);
+        });
+      }
     }
     grid_barrier(bar, ++nbar * G, err);
   }
@@ -358,7 +386,9 @@ __global__ void k_chol_flags(const int* fail, const int* sync, int* ok) {
   ok[1] = sync[1];
 }
 
-int coop_launch(Ctx* c, const void* fn, int want, size_t lds, void** args) {
+// exact: launch exactly `want` groups or nothing (kernels whose data layout
+// depends on the group count); otherwise fewer groups are fine.
+int coop_launch(Ctx* c, const void* fn, int want, size_t lds, void** args, bool exact) {
   if (std::getenv("QX_DENSE_LEGACY")) return -1;
   int dev = 0, ncu = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess) return -1;
@@ -369,6 +399,7 @@ int coop_launch(Ctx* c, const void* fn, int want, size_t lds, void** args) {
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 256, lds) != hipSuccess || per < 1)
     return -1;
   const int G = std::max(1, std::min(want, per * ncu));
+  if (exact && G != want) return -1;
   hipLaunchKernelGGL(k_coop_reset, dim3(1), dim3(1), 0, c->stream, c->d_sync);
   if (hipLaunchCooperativeKernel(fn, dim3(G), dim3(256), args, (unsigned)lds, c->stream) !=
       hipSuccess) {
@@ -410,12 +441,12 @@ int dense_chol_inv(Ctx* c, double* s, int n, double* x, int* ok, int* flag) {
   int want = 1;
   for (int b = 0; b < nblk; ++b) {
     const int nrb = nblk - b - 1;
-    want = std::max(want, std::max(nrb * (nrb + 1) / 2, nrb + b));
+    want = std::max(want, std::max(nrb * (nrb + 1) / 2 + nrb * (b + 1), nrb + b));
   }
   int* bar = c->d_sync;
   int* err = c->d_sync + 1;
   void* args[] = {&s, &n, &x, &flag, &bar, &err};
-  if (coop_launch(c, reinterpret_cast<const void*>(k_chol_inv_coop), want, kCholLds, args) < 0) {
+  if (coop_launch(c, reinterpret_cast<const void*>(k_chol_inv_coop), want, kCholLds, args, false) < 0) {
     hipLaunchKernelGGL(k_coop_reset, dim3(1), dim3(1), 0, st, c->d_sync);
     chol_inv_legacy(c, s, n, x, flag);
   }
@@ -744,8 +775,9 @@ int dense_eigh_legacy(Ctx* c, double* a, int n, double* z, double* w, double* wo
   const int ncl = (int)cs.size() - 1;
   int* dcs = reinterpret_cast<int*>(work + 6 * n);  // n+1 ints
   QX_CHECK(hipMemcpyAsync(dcs, cs.data(), sizeof(int) * cs.size(), hipMemcpyHostToDevice, st));
-  double* lu = c->d_rho_part;  // >= 5 n * ncl doubles of scratch
-  if ((size_t)5 * n * ncl > (size_t)c->rho_groups * c->N) return -8;
+  // 5 n * ncl <= 5 n^2 doubles of scratch from the dense workspace
+  if (c->dwork_len < 5 * (size_t)n * n) return -8;
+  double* lu = c->d_dwork;
   hipLaunchKernelGGL(k_inviter, dim3(ncl), dim3(256), 0, st, d, e, n, w, dcs, ncl, z, lu, tnorm);
   hipLaunchKernelGGL(k_backtrans, dim3((n + 15) / 16), dim3(256), 0, st, a, n, z);
   QX_CHECK(hipGetLastError());

@@ -11,11 +11,13 @@
 //   2. Eigenvalues by multisection on Sturm counts: one wavefront per
 //      eigenvalue, its 64 lanes probe 64 points of the bracket at once.
 //   3. Eigenvectors by inverse iteration, one thread per eigenvalue (LU with
-//      partial pivoting of T - lambda I), followed by a shifted CholeskyQR3
-//      re-orthonormalisation of all vectors (Gram matrix and Z L^-T on MFMA);
-//      two rounds.  Numerically equal eigenvalues get shifts pulled apart by
-//      10 eps |lambda| as in LAPACK's dstein.
-//   4. Back-transformation by the stored reflectors, 16 (8, 4 for large n)
+//      partial pivoting of T - lambda I), followed by a CholeskyQR
+//      re-orthonormalisation of all vectors (Gram matrix and Z L^-T on MFMA):
+//      from random start vectors with a shifted first pass (shifted
+//      CholeskyQR3), then once more from the orthonormal vectors (CholeskyQR2).
+//      Numerically equal eigenvalues get shifts pulled apart by 10 eps |lambda|
+//      as in LAPACK's dstein.
+//   4. Back-transformation by the stored reflectors, 8 (4 for large n)
 //      columns of Z per workgroup held in LDS, reflectors staged through LDS.
 #include <cmath>
 #include <cstdlib>
@@ -41,17 +43,29 @@ __device__ inline double block_sum256(double v, double* red) {
 // ---- 1. tridiagonalisation --------------------------------------------------
 // a: n x n column-major symmetric (full storage).  On exit d, e (e[i] couples
 // i and i+1, e[n-1] = 0) and the unit reflector of step k in a(k+1:n, k).
+//
+// One exchange per step.  At the top of step k every workgroup holds the
+// current column k (x_j = A_k(j, k), j > k) and forms the reflector v from it.
+// Row owners publish p_i = (A_k v)_i for their rows, and the owner of row k+1
+// publishes that row, A_k(k+1, :).  After the barrier every workgroup forms
+// w = 2 (p - (v.p) v), updates its rows, A_{k+1} = A_k - v w^T - w v^T, and
+// forms the next column itself from the published row:
+// x_j = A_k(k+1, j) - v_{k+1} w_j - w_{k+1} v_j.  The published buffers are
+// double-buffered by step parity (a workgroup writes parity q again only
+// after every workgroup has passed the barrier that follows its reads).
 __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, int n,
                                                       double* __restrict__ d,
-                                                      double* __restrict__ e, double* xg,
-                                                      double* pg, int* bar, int* err) {
+                                                      double* __restrict__ e, double* rowbuf,
+                                                      double* pbuf, int* bar, int* err) {
   extern __shared__ double sm[];
   const int G = gridDim.x, g = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int rpw = (n + G - 1) / G;
   double* R = sm;                      // R[r * n + j] = A(g + G r, j)
   double* v = R + (size_t)rpw * n;     // reflector, indexed by global column
   double* w = v + n;                   // rank-2 partner
-  double* red = w + n;                 // reduction scratch
+  double* x = w + n;                   // current column
+  double* rr = x + n;                  // published row k+1
+  double* red = rr + n;                // reduction scratch
   int nbar = 0;
   for (int r = 0; r < rpw; ++r) {
     const int i = g + G * r;
@@ -59,50 +73,52 @@ __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, in
     for (int j = t; j < n; j += 256) R[(size_t)r * n + j] = a[(size_t)i + (size_t)n * j];
   }
   __syncthreads();
-  for (int r = t; r < rpw; r += 256) {
-    const int i = g + G * r;
-    if (i >= n) continue;
-    if (i == 0) d[0] = R[0];
-    else st_agent(&xg[i], R[(size_t)r * n]);
-  }
+  if (g == 0)  // row 0 is column 0: publish it (row 0 lives in workgroup 0)
+    for (int j = t; j < n; j += 256) st_agent(&rowbuf[j], R[j]);
   grid_barrier(bar, ++nbar * G, err);
+  for (int j = t; j < n; j += 256) x[j] = ld_agent(&rowbuf[j]);
+  if (g == 0 && t == 0) d[0] = ld_agent(&rowbuf[0]);
+  __syncthreads();
   for (int k = 0; k + 2 < n; ++k) {
+    const int q = (k + 1) & 1;  // buffer parity of this step
+    double* pq = pbuf + (size_t)q * n;
+    double* rq = rowbuf + (size_t)q * n;
     // reflector of column k, formed redundantly by every workgroup
     double part = 0.0;
-    for (int j = k + 1 + t; j < n; j += 256) {
-      const double xv = ld_agent(&xg[j]);
-      v[j] = xv;
-      part += xv * xv;
-    }
+    for (int j = k + 1 + t; j < n; j += 256) part += x[j] * x[j];
     const double nx2 = block_sum256(part, red);
-    const double x0 = v[k + 1];
+    const double x0 = x[k + 1];
     const double alpha = nx2 > 0.0 ? (x0 >= 0.0 ? -sqrt(nx2) : sqrt(nx2)) : 0.0;
     const double vn2 = nx2 - 2.0 * alpha * x0 + alpha * alpha;
     const bool skip = !(vn2 > 1e-300);
     const double inv = skip ? 0.0 : 1.0 / sqrt(vn2);
-    __syncthreads();  // every thread has read v[k+1] before it changes
-    for (int j = k + 1 + t; j < n; j += 256) v[j] = (j == k + 1 ? v[j] - alpha : v[j]) * inv;
+    for (int j = k + 1 + t; j < n; j += 256) v[j] = (j == k + 1 ? x[j] - alpha : x[j]) * inv;
     __syncthreads();
     if (g == 0) {
       for (int j = k + 1 + t; j < n; j += 256) a[(size_t)j + (size_t)n * k] = v[j];
       if (t == 0) e[k] = skip ? x0 : alpha;
     }
+    // publish p for the owned rows of the trailing block, and row k+1
     if (!skip) {
-      // p_i = sum_j A(i, j) v_j for the owned rows of the trailing block
       for (int r = wv; r < rpw; r += 4) {
         const int i = g + G * r;
         if (i <= k || i >= n) continue;
         double acc = 0.0;
         for (int j = k + 1 + lane; j < n; j += 64) acc += R[(size_t)r * n + j] * v[j];
         acc = wave_sum(acc);
-        if (lane == 0) st_agent(&pg[i], acc);
+        if (lane == 0) st_agent(&pq[i], acc);
       }
     }
+    if ((k + 1) % G == g) {
+      const double* Rr = R + (size_t)((k + 1) / G) * n;
+      for (int j = k + 1 + t; j < n; j += 256) st_agent(&rq[j], Rr[j]);
+    }
     grid_barrier(bar, ++nbar * G, err);
+    for (int j = k + 1 + t; j < n; j += 256) rr[j] = ld_agent(&rq[j]);
     if (!skip) {
       double part2 = 0.0;
       for (int j = k + 1 + t; j < n; j += 256) {
-        const double p = ld_agent(&pg[j]);
+        const double p = ld_agent(&pq[j]);
         w[j] = p;
         part2 += v[j] * p;
       }
@@ -111,29 +127,25 @@ __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, in
       __syncthreads();
       for (int r = 0; r < rpw; ++r) {
         const int i = g + G * r;
-        if (i <= k || i >= n) continue;
+        if (i <= k + 1 || i >= n) continue;  // row k+1 is finished
         const double vi = v[i], wi = w[i];
         double* Rr = R + (size_t)r * n;
-        for (int j = k + 1 + t; j < n; j += 256) Rr[j] -= vi * w[j] + wi * v[j];
+        for (int j = k + 2 + t; j < n; j += 256) Rr[j] -= vi * w[j] + wi * v[j];
       }
-      __syncthreads();
+    } else {
+      for (int j = k + 1 + t; j < n; j += 256) w[j] = 0.0;
     }
-    // publish column k+1 below the diagonal, and the final diagonal d[k+1]
-    for (int r = t; r < rpw; r += 256) {
-      const int i = g + G * r;
-      if (i <= k || i >= n) continue;
-      if (i == k + 1) d[k + 1] = R[(size_t)r * n + k + 1];
-      else st_agent(&xg[i], R[(size_t)r * n + k + 1]);
-    }
-    grid_barrier(bar, ++nbar * G, err);
+    __syncthreads();
+    // next column and the finished diagonal d[k+1]
+    const double v1 = v[k + 1], w1 = w[k + 1];
+    for (int j = k + 2 + t; j < n; j += 256) x[j] = rr[j] - v1 * w[j] - w1 * v[j];
+    if (g == 0 && t == 0) d[k + 1] = rr[k + 1] - 2.0 * v1 * w1;
+    __syncthreads();
   }
-  for (int r = t; r < rpw; r += 256) {
-    const int i = g + G * r;
-    if (i == n - 1) {
-      e[n - 2] = R[(size_t)r * n + n - 2];
-      d[n - 1] = R[(size_t)r * n + n - 1];
-      e[n - 1] = 0.0;
-    }
+  if (g == 0 && t == 0) e[n - 2] = x[n - 1];
+  if ((n - 1) % G == g && t == 0) {
+    d[n - 1] = R[(size_t)((n - 1) / G) * n + n - 1];
+    e[n - 1] = 0.0;
   }
 }
 
@@ -223,103 +235,153 @@ __global__ void __launch_bounds__(256) k_bisect_ms(const double* __restrict__ d,
 // and i+1 swapped when the subdiagonal dominates), then solve with the column
 // j of X (row-major, X[i*n + j]) as right-hand side, in place, and normalise.
 // first != 0: the right-hand side is a fixed pseudo-random vector instead.
+// The running row of the factorisation and of the forward sweep stay in
+// registers, so every global access is independent of the recurrence.
 // lu: 5 n^2 doubles, entry (q, i) of thread j at lu[(q*n + i)*n + j].
-__global__ void __launch_bounds__(256) k_inviter_par(const double* __restrict__ d,
-                                                     const double* __restrict__ e, int n,
-                                                     const double* __restrict__ w,
-                                                     const double* __restrict__ bounds,
-                                                     double* __restrict__ lu,
-                                                     double* __restrict__ X, int first) {
+__device__ inline double inviter_start(int i, int j) {
+  unsigned h = (unsigned)(i * 2654435761u) ^ (unsigned)(j * 40503u + 12345u);
+  h ^= h >> 13;
+  h *= 0x5bd1e995u;
+  h ^= h >> 15;
+  return 0.5 + (double)(h & 0xffff) / 65536.0;
+}
+
+// Loads are gathered 32 steps at a time into registers ahead of the
+// recurrence (the solution overwrites the right-hand side in place, so the
+// compiler cannot hoist them on its own).
+constexpr int kIvChunk = 32;
+
+__global__ void __launch_bounds__(64) k_inviter_par(const double* __restrict__ d,
+                                                    const double* __restrict__ e, int n,
+                                                    const double* __restrict__ w,
+                                                    const double* __restrict__ bounds,
+                                                    double* __restrict__ lu,
+                                                    double* __restrict__ X, int first) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  const double eps = 2.220446049250313e-16;
-  const double pert = eps * bounds[3];
-  double lam = w[j];
-  {
-    int r = 0;
-    for (int q = j; q > 0 && w[q] - w[q - 1] < 10.0 * eps * fabs(w[q]); --q) ++r;
-    lam += (double)r * 10.0 * eps * fabs(lam);
-  }
   const size_t nn = (size_t)n * n;
-  double* dd = lu + j;           // U diagonal
+  double* dd = lu + j;           // U diagonal (tiny pivots replaced by +-pert)
   double* du = dd + nn;          // U superdiagonal
   double* du2 = du + nn;         // U second superdiagonal
   double* dl = du2 + nn;         // multipliers
   double* pv = dl + nn;          // 1 when rows i, i+1 were swapped
 #define L_(arr, i) arr[(size_t)(i) * n]
-  for (int i = 0; i < n; ++i) {
-    L_(dd, i) = d[i] - lam;
-    L_(du, i) = i < n - 1 ? e[i] : 0.0;
-    L_(du2, i) = 0.0;
+  if (first) {
+    // factor T - lambda I once; the second round reuses it
+    const double eps = 2.220446049250313e-16;
+    const double pert = eps * bounds[3];
+    double lam = w[j];
+    int r = 0;
+    for (int q = j; q > 0 && w[q] - w[q - 1] < 10.0 * eps * fabs(w[q]); --q) ++r;
+    lam += (double)r * 10.0 * eps * fabs(lam);
+    auto guard = [&](double v) { return fabs(v) < pert ? (v < 0.0 ? -pert : pert) : v; };
+    double dc = d[0] - lam, uc = n > 1 ? e[0] : 0.0;  // current row: diagonal, superdiagonal
+    for (int i = 0; i < n - 1; ++i) {
+      const double sub = e[i];
+      const double dn = d[i + 1] - lam;
+      const double un = i + 2 < n ? e[i + 1] : 0.0;
+      if (fabs(dc) >= fabs(sub)) {
+        const double f = dc != 0.0 ? sub / dc : 0.0;
+        L_(dd, i) = guard(dc);
+        L_(du, i) = uc;
+        L_(du2, i) = 0.0;
+        L_(dl, i) = f;
+        L_(pv, i) = 0.0;
+        dc = dn - f * uc;
+        uc = un;
+      } else {
+        const double f = dc / sub;
+        L_(dd, i) = guard(sub);
+        L_(du, i) = dn;
+        L_(du2, i) = un;
+        L_(dl, i) = f;
+        L_(pv, i) = 1.0;
+        dc = uc - f * dn;
+        uc = -f * un;
+      }
+    }
+    L_(dd, n - 1) = guard(dc);
+    L_(du, n - 1) = 0.0;
+    L_(du2, n - 1) = 0.0;
   }
-  for (int i = 0; i < n - 1; ++i) {
-    const double sub = e[i];
-    const double di = L_(dd, i);
-    if (fabs(di) >= fabs(sub)) {
-      const double f = di != 0.0 ? sub / di : 0.0;
-      L_(pv, i) = 0.0;
-      L_(dl, i) = f;
-      L_(dd, i + 1) -= f * L_(du, i);
-    } else {
-      const double f = di / sub;
-      const double dn = L_(dd, i + 1), un = L_(du, i);
-      L_(pv, i) = 1.0;
-      L_(dl, i) = f;
-      L_(dd, i) = sub;
-      L_(du, i) = dn;
-      L_(dd, i + 1) = un - f * dn;
-      if (i < n - 2) {
-        const double u2 = L_(du, i + 1);
-        L_(du2, i) = u2;
-        L_(du, i + 1) = -f * u2;
+  double* x = X + j;
+  // forward sweep L y = P b, the row being eliminated carried in xc
+  double xc = first ? inviter_start(0, j) : L_(x, 0);
+  for (int i0 = 0; i0 < n - 1; i0 += kIvChunk) {
+    double xr[kIvChunk], fr[kIvChunk], pr[kIvChunk];
+#pragma unroll
+    for (int c = 0; c < kIvChunk; ++c) {
+      const int i = i0 + c;
+      if (i < n - 1) {
+        xr[c] = first ? inviter_start(i + 1, j) : L_(x, i + 1);
+        fr[c] = L_(dl, i);
+        pr[c] = L_(pv, i);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kIvChunk; ++c) {
+      const int i = i0 + c;
+      if (i < n - 1) {
+        if (pr[c] == 0.0) {
+          L_(x, i) = xc;
+          xc = xr[c] - fr[c] * xc;
+        } else {
+          L_(x, i) = xr[c];
+          xc = xc - fr[c] * xr[c];
+        }
       }
     }
   }
-  for (int i = 0; i < n; ++i) {
-    const double v = L_(dd, i);
-    if (fabs(v) < pert) L_(dd, i) = v < 0.0 ? -pert : pert;
-  }
-  double* x = X + j;
-  if (first) {
-    for (int i = 0; i < n; ++i) {
-      unsigned h = (unsigned)(i * 2654435761u) ^ (unsigned)(j * 40503u + 12345u);
-      h ^= h >> 13;
-      h *= 0x5bd1e995u;
-      h ^= h >> 15;
-      L_(x, i) = 0.5 + (double)(h & 0xffff) / 65536.0;
+  L_(x, n - 1) = xc;
+  // back substitution U x = y, with the squared norm accumulated on the way
+  double x1 = 0.0, x2 = 0.0, big = 0.0, s2 = 0.0;  // x1 = x[i+1], x2 = x[i+2]
+  for (int i1 = n - 1; i1 >= 0; i1 -= kIvChunk) {
+    double yr[kIvChunk], ur[kIvChunk], u2r[kIvChunk], dr[kIvChunk];
+#pragma unroll
+    for (int c = 0; c < kIvChunk; ++c) {
+      const int i = i1 - c;
+      if (i >= 0) {
+        yr[c] = L_(x, i);
+        ur[c] = L_(du, i);
+        u2r[c] = L_(du2, i);
+        dr[c] = L_(dd, i);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kIvChunk; ++c) {
+      const int i = i1 - c;
+      if (i >= 0) {
+        const double v = (yr[c] - ur[c] * x1 - u2r[c] * x2) / dr[c];
+        L_(x, i) = v;
+        x2 = x1;
+        x1 = v;
+        s2 += v * v;
+        big = fmax(big, fabs(v));
+        if (big > 1e150) {
+          // rescale the solved part, the right-hand side still in memory and
+          // the part of it already in registers
+          for (int q = 0; q < n; ++q) L_(x, q) *= 1e-150;
+#pragma unroll
+          for (int c2 = 0; c2 < kIvChunk; ++c2)
+            if (c2 > c) yr[c2] *= 1e-150;
+          x1 *= 1e-150;
+          x2 *= 1e-150;
+          s2 *= 1e-300;
+          big *= 1e-150;
+        }
+      }
     }
   }
-  // L U x = P b: forward sweep with the recorded swaps, then back substitution
-  for (int i = 0; i < n - 1; ++i) {
-    const double xi = L_(x, i), xn = L_(x, i + 1), f = L_(dl, i);
-    if (L_(pv, i) == 0.0) {
-      L_(x, i + 1) = xn - f * xi;
-    } else {
-      L_(x, i) = xn;
-      L_(x, i + 1) = xi - f * xn;
-    }
-  }
-  double big = 0.0, x1 = 0.0, x2 = 0.0;  // x1 = x[i+1], x2 = x[i+2]
-  for (int i = n - 1; i >= 0; --i) {
-    double v = L_(x, i);
-    if (i + 1 < n) v -= L_(du, i) * x1;
-    if (i + 2 < n) v -= L_(du2, i) * x2;
-    v /= L_(dd, i);
-    L_(x, i) = v;
-    x2 = x1;
-    x1 = v;
-    big = fmax(big, fabs(v));
-    if (big > 1e150) {  // rescale the solved part and the remaining right-hand side
-      for (int q = 0; q < n; ++q) L_(x, q) *= 1e-150;
-      x1 *= 1e-150;
-      x2 *= 1e-150;
-      big *= 1e-150;
-    }
-  }
-  double s2 = 0.0;
-  for (int i = 0; i < n; ++i) s2 += L_(x, i) * L_(x, i);
   const double sc = s2 > 0.0 ? 1.0 / sqrt(s2) : 0.0;
-  for (int i = 0; i < n; ++i) L_(x, i) *= sc;
+  for (int i0 = 0; i0 < n; i0 += kIvChunk) {
+    double xr[kIvChunk];
+#pragma unroll
+    for (int c = 0; c < kIvChunk; ++c)
+      if (i0 + c < n) xr[c] = L_(x, i0 + c);
+#pragma unroll
+    for (int c = 0; c < kIvChunk; ++c)
+      if (i0 + c < n) L_(x, i0 + c) = xr[c] * sc;
+  }
 #undef L_
 }
 
@@ -346,15 +408,15 @@ __global__ void k_add_diag(double* g, int n, double s) {
 }
 
 // ---- 4. back-transformation ----------------------------------------------------
-// COLS columns of Z per workgroup, 256 / COLS lanes per column, COLS / 2
-// reflectors staged at a time; LDS = 1.5 COLS n doubles.
+// COLS columns of Z per workgroup, 256 / COLS lanes per column; nref
+// reflectors staged in LDS at a time.  LDS = (COLS + nref) n doubles.
 template <int COLS>
 __global__ void __launch_bounds__(256) k_backtrans_lds(const double* __restrict__ a, int n,
-                                                       double* __restrict__ z) {
-  constexpr int LANES = 256 / COLS, REFL = COLS / 2;
+                                                       double* __restrict__ z, int nref) {
+  constexpr int LANES = 256 / COLS;
   extern __shared__ double sm[];
   double* zs = sm;                         // [COLS][n]
-  double* vs = sm + (size_t)COLS * n;      // [REFL][n]
+  double* vs = sm + (size_t)COLS * n;      // [nref][n]
   const int c0 = blockIdx.x * COLS;
   const int ncol = min(COLS, n - c0);
   const int cl = threadIdx.x / LANES, lane = threadIdx.x % LANES;
@@ -363,8 +425,8 @@ __global__ void __launch_bounds__(256) k_backtrans_lds(const double* __restrict_
     zs[idx] = cc < ncol ? z[(size_t)(c0 + cc) * n + i] : 0.0;
   }
   double* zc = zs + (size_t)cl * n;
-  for (int kt = n - 3; kt >= 0; kt -= REFL) {
-    const int kl = max(0, kt - REFL + 1);  // reflectors kl..kt
+  for (int kt = n - 3; kt >= 0; kt -= nref) {
+    const int kl = max(0, kt - nref + 1);  // reflectors kl..kt
     __syncthreads();
     for (int idx = threadIdx.x; idx < (kt - kl + 1) * n; idx += 256) {
       const int q = idx / n, i = idx - q * n;
@@ -391,13 +453,15 @@ __global__ void __launch_bounds__(256) k_backtrans_lds(const double* __restrict_
 
 template <int COLS>
 static int backtrans(Ctx* c, const double* a, int n, double* z) {
-  const size_t lds = sizeof(double) * (size_t)(COLS + COLS / 2) * n;
-  if (lds > 160 * 1024) return -22;
+  const size_t cap = 150 * 1024 / sizeof(double);
+  const long nref = std::min<long>(32, (long)(cap / (size_t)n) - COLS);
+  if (nref < 1) return -22;
+  const size_t lds = sizeof(double) * (size_t)(COLS + nref) * n;
   if (lds > 65536)
     QX_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_backtrans_lds<COLS>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(k_backtrans_lds<COLS>, dim3((n + COLS - 1) / COLS), dim3(256), lds, c->stream,
-                     a, n, z);
+                     a, n, z, (int)nref);
   return 0;
 }
 
@@ -412,17 +476,21 @@ static int tridiag_groups(int n) {
   return std::max(1, (n + 7) / 8);
 }
 
-// z (n x n, column-major, vectors in columns) <- QR factor of z: shifted
-// CholeskyQR followed by two plain CholeskyQR passes
-static int chol_qr3(Ctx* c, double* z, int n, double* gram, double* linv, double* tmp, int* ok) {
+// z (n x n, column-major, vectors in columns) <- QR factor of z by CholeskyQR
+// passes; with shifted != 0 the first pass factors Z^T Z + s I (shifted
+// CholeskyQR, for a Z that may be far from orthonormal), then two plain ones.
+// ok: 2 ints per pass + 1 scratch.
+static int chol_qr(Ctx* c, double* z, int n, double* gram, double* linv, double* tmp, int* ok,
+                   int shifted) {
   const double u = 1.1102230246251565e-16;
   const double shift = 11.0 * (2.0 * n * (double)n + n) * u * (double)n;
-  for (int pass = 0; pass < 3; ++pass) {
+  const int npass = shifted ? 3 : 2;
+  for (int pass = 0; pass < npass; ++pass) {
     int rc = gemm_tn(c, n, n, n, 1.0, z, n, z, n, 0.0, gram, n, 0);
     if (rc) return rc;
-    if (pass == 0)
+    if (shifted && pass == 0)
       hipLaunchKernelGGL(k_add_diag, dim3((n + 255) / 256), dim3(256), 0, c->stream, gram, n, shift);
-    if ((rc = dense_chol_inv(c, gram, n, linv, ok + 2 * pass, ok + 6))) return rc;
+    if ((rc = dense_chol_inv(c, gram, n, linv, ok + 2 * pass, ok + 7))) return rc;
     if ((rc = gemm_nn(c, n, n, n, 1.0, z, n, linv, n, 0.0, tmp, n))) return rc;
     QX_CHECK(hipMemcpyAsync(z, tmp, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice,
                             c->stream));
@@ -439,64 +507,57 @@ int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int
   double* dw = c->d_dwork;
   double* d = dw;
   double* e = d + n;
-  double* xg = e + n;
-  double* pg = xg + n;
-  double* bounds = pg + n;                                      // 8
-  int* ok = reinterpret_cast<int*>(bounds + 8);                 // 8 ints
-  int* sflag = ok + 8;                                          // 1 int
+  double* rowbuf = e + n;                                       // 2n
+  double* pbuf = rowbuf + 2 * (size_t)n;                        // 2n
+  double* bounds = pbuf + 2 * (size_t)n;                        // 8
+  int* ok = reinterpret_cast<int*>(bounds + 8);                 // 2 x 8 ints
+  int* sflag = ok + 16;                                         // 1 int
   double* X = dw + 16 * (size_t)n + 64;                         // n^2, row-major vectors
   double* gram = X + (size_t)n * n;                             // n^2
   double* linv = gram + (size_t)n * n;                          // n^2
   double* lu = linv + (size_t)n * n;                            // 5 n^2
-  // 1. tridiagonalisation
-  const int G = tridiag_groups(n);
-  const int rpw_min = (n + std::min(G, n) - 1) / std::min(G, n);
-  size_t lds = sizeof(double) * ((size_t)rpw_min * n + 2 * (size_t)n + 8);
-  int* bar = c->d_sync;
-  int* err = c->d_sync + 1;
-  void* args[] = {&a, &n, &d, &e, &xg, &pg, &bar, &err};
-  // the launch may grant fewer groups than asked; size LDS for the worst case
-  // the device can hold (rows per group grow as groups shrink)
+  // 1. tridiagonalisation: the row distribution follows the group count, so
+  // the launch must get exactly the groups its LDS is sized for
   int dev = 0, ncu = 0;
   QX_CHECK(hipGetDevice(&dev));
   QX_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  const int Gw = std::min(G, std::min(n, ncu));
-  lds = sizeof(double) * ((size_t)((n + Gw - 1) / Gw) * n + 2 * (size_t)n + 8);
+  const int G = std::min(tridiag_groups(n), std::min(n, ncu));
+  const size_t lds = sizeof(double) * ((size_t)((n + G - 1) / G) * n + 4 * (size_t)n + 8);
   if (lds > 160 * 1024 - 1024) return dense_eigh_legacy(c, a, n, z, w, work);
-  const int Gl = coop_launch(c, reinterpret_cast<const void*>(k_tridiag_coop), Gw, lds, args);
-  if (Gl < 0) return dense_eigh_legacy(c, a, n, z, w, work);
-  if (Gl != Gw) {
-    // fewer resident groups than sized for: the launch used Gl groups with
-    // more rows each than the LDS holds -- refuse rather than overrun
-    QX_CHECK(hipStreamSynchronize(st));
-    return -21;
-  }
+  int* bar = c->d_sync;
+  int* err = c->d_sync + 1;
+  void* args[] = {&a, &n, &d, &e, &rowbuf, &pbuf, &bar, &err};
+  if (coop_launch(c, reinterpret_cast<const void*>(k_tridiag_coop), G, lds, args, true) < 0)
+    return dense_eigh_legacy(c, a, n, z, w, work);
   // 2. eigenvalues
   hipLaunchKernelGGL(k_gersh, dim3(1), dim3(256), 0, st, d, e, n, bounds);
   hipLaunchKernelGGL(k_bisect_ms, dim3((n + 3) / 4), dim3(256), 0, st, d, e, n, bounds, w);
-  // 3. eigenvectors: inverse iteration + CholeskyQR3, twice
+  // 3. eigenvectors: inverse iteration + CholeskyQR, twice; the LU factors
+  // of round 0 are reused by round 1, so the CholeskyQR scratch is X
   const dim3 tgrid((n + 31) / 32, (n + 31) / 32);
   for (int round = 0; round < 2; ++round) {
     if (round > 0) hipLaunchKernelGGL(k_transpose_sq, tgrid, dim3(256), 0, st, z, n, X);
     hipLaunchKernelGGL(k_inviter_par, dim3((n + 63) / 64), dim3(64), 0, st, d, e, n, w, bounds, lu,
                        X, round == 0 ? 1 : 0);
     hipLaunchKernelGGL(k_transpose_sq, tgrid, dim3(256), 0, st, X, n, z);
-    int rc = chol_qr3(c, z, n, gram, linv, lu, ok);
+    // round 0 starts from random vectors (shifted first pass); round 1 from
+    // orthonormal ones, which the solve keeps near orthonormal
+    int rc = chol_qr(c, z, n, gram, linv, X, ok + 8 * round, round == 0);
     if (rc) return rc;
   }
   // 4. back-transformation
   {
-    int rc = n <= 850 ? backtrans<16>(c, a, n, z) : n <= 1700 ? backtrans<8>(c, a, n, z)
-                                                              : backtrans<4>(c, a, n, z);
+    int rc = n <= 2000 ? backtrans<8>(c, a, n, z) : backtrans<4>(c, a, n, z);
     if (rc) return rc;
   }
   hipLaunchKernelGGL(k_sync_take, dim3(1), dim3(1), 0, st, c->d_sync, sflag);
   QX_CHECK(hipGetLastError());
-  int hflags[9];
-  QX_CHECK(hipMemcpyAsync(hflags, ok, sizeof(int) * 9, hipMemcpyDeviceToHost, st));
+  int hflags[17];
+  QX_CHECK(hipMemcpyAsync(hflags, ok, sizeof(int) * 17, hipMemcpyDeviceToHost, st));
   QX_CHECK(hipStreamSynchronize(st));
-  if (hflags[8]) return -20;                    // a grid barrier gave up
-  if (!hflags[0] || !hflags[2] || !hflags[4]) return -23;  // Gram matrix not positive definite
+  if (hflags[16]) return -20;  // a grid barrier gave up
+  // a Gram matrix of the re-orthonormalisation was not positive definite
+  if (!hflags[0] || !hflags[2] || !hflags[4] || !hflags[8] || !hflags[10]) return -23;
   return 0;
 }
 

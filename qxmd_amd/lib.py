@@ -63,6 +63,7 @@ SIGNATURES = {
     "qx_scf_band_step": [_vp, _i, _i, _i, _dp, _i, _d, _dp, _vp, _pi, _i],
     "qx_event_record": [_vp, _i],
     "qx_event_elapsed": [_vp, _i, _i, C.POINTER(C.c_float)],
+    "qx_scf_phase_ms": [_vp, C.POINTER(C.c_float)],
     "qx_rfft3": [_i, _dp, _dp, _i, _i, _i, _pi],
     "qx_fft3": [_i, _dp, _dp, _i, _i, _i, _pi],
     "qx_hcnonloc": [_vp, _dp, _dp],

@@ -174,6 +174,13 @@ class PlaneWave:
         return eig, rho, bool(ok.value)
 
     # ---- timing
+    def scf_phase_ms(self):
+        """Phase times (ms) of the last SCF band step."""
+        ms = (C.c_float * 5)()
+        check(load().qx_scf_phase_ms(self.h, ms), "qx_scf_phase_ms")
+        return dict(zip(("energy", "rotation", "cg", "schmidt", "density"),
+                        (round(float(x), 3) for x in ms)))
+
     def record(self, slot):
         check(load().qx_event_record(self.h, slot), "qx_event_record")
 
