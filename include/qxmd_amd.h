@@ -88,10 +88,27 @@ int qx_eigen_bkl_slot(qx_ctx *ctx, int psi, int hpsi, int nb, const double *ekib
                       int noband, int itermx, int iteremx, double tolera, double threinn,
                       double wdinner, double roughfzan, double roughfeig, double *bzan,
                       int *nriter, int prec);
+/* the same CG update with all bands advanced together: every iteration is one
+ * batched H|psi> over the bands still active, and the projection against the
+ * other bands (subgs3, Sources/engrad.f90:1072-1142) is done against all bands
+ * as two GEMMs.  Same arguments as qx_eigen_bkl_slot without the inner-loop
+ * controls threinn / wdinner. */
+int qx_eigen_bkl_batched_slot(qx_ctx *ctx, int psi, int hpsi, int nb, const double *ekib,
+                              double *eig, int noband, int itermx, int iteremx, double tolera,
+                              double roughfzan, double roughfeig, double *bzan, int *nriter,
+                              int prec);
+/* per-band kinetic energy ekib = 2 <psi|T|psi> (ekinetic2, Sources/engrad.f90:66-127),
+ * the preconditioner scale of prpcg */
+int qx_ekib_slot(qx_ctx *ctx, int slot, int nb, double *ekib);
 /* one SCF band step with frozen potential (energy pass, rotation, CG,
  * Gram-Schmidt, density) -- see Sources/pwlda.f90:2087-3574 */
 int qx_scf_band_step(qx_ctx *ctx, int psi, int hpsi, int nb, const double *occ, int noband,
                      double rvol, double *eig, double *rho, int *lcholesky, int prec);
+/* qx_scf_band_step with the CG form chosen: batched != 0 uses
+ * qx_eigen_bkl_batched_slot, 0 the band-by-band qx_eigen_bkl_slot */
+int qx_scf_band_step_ex(qx_ctx *ctx, int psi, int hpsi, int nb, const double *occ, int noband,
+                        double rvol, double *eig, double *rho, int *lcholesky, int prec,
+                        int batched);
 
 /* ---------------- timing on the library stream ---------------------------- */
 int qx_event_record(qx_ctx *ctx, int slot);

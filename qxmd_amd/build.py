@@ -4,7 +4,8 @@ Run as ``python -m qxmd_amd.build`` or through ``__graft_entry__.build()``.
 Every HIP source is compiled for gfx950 only, with packed-FP32 instruction
 selection disabled; the plane-wave set-up (gsphere.cpp) is host code compiled
 without floating-point contraction so that its G-vector sort keys match the
-reference bit for bit.
+reference bit for bit.  The Fortran host layer (qxmd_amd/fortran) is built
+with flang into libqxmd_amd_f.so when flang is present.
 """
 from __future__ import annotations
 
@@ -22,6 +23,9 @@ ARCH = os.environ.get("QX_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["pencil.hip", "pencil_fast.hip", "gemm.hip", "dense.hip", "eigh.hip", "fullgrid.hip", "cg.hip", "api.hip"]
 HOST_SOURCES = ["gsphere.cpp"]
+FORTRAN_SOURCES = ["qxmd_amd_c.f90", "qxmd_dropin.f90"]
+FOUT = os.path.join(HERE, "libqxmd_amd_f.so")
+ROOT = os.path.dirname(HERE)
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -68,7 +72,42 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     tmp = OUT + ".tmp"
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs])
     os.replace(tmp, OUT)
+    build_fortran(verbose)
     return OUT
+
+
+def _flang():
+    for c in (os.environ.get("FC_FLANG"), "/opt/rocm/lib/llvm/bin/flang", shutil.which("flang")):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+def build_fortran(verbose: bool = False):
+    """The Fortran host layer: ISO_C_BINDING interfaces + drop-in externals
+    (libqxmd_amd_f.so next to libqxmd_amd.so) and the Fortran check driver
+    tests/fortran/dropin_check.  Skipped when no flang is installed."""
+    fc = _flang()
+    if fc is None:
+        if verbose:
+            print("flang not found: Fortran layer skipped")
+        return None
+    fdir = os.path.join(HERE, "fortran")
+    objs = []
+    for src in FORTRAN_SOURCES:
+        o = os.path.join(BLD, src.replace(".f90", ".o"))
+        _run([fc, "-O2", "-fPIC", "-module-dir", BLD, "-c", os.path.join(fdir, src), "-o", o])
+        objs.append(o)
+    tmp = FOUT + ".tmp"
+    _run([fc, "-shared", "-o", tmp, *objs, "-L" + HERE, "-lqxmd_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, FOUT)
+    drv_src = os.path.join(ROOT, "tests", "fortran", "dropin_check.f90")
+    if os.path.exists(drv_src):
+        drv = os.path.join(ROOT, "tests", "fortran", "dropin_check")
+        _run([fc, "-O2", "-I" + BLD, drv_src, "-o", drv + ".tmp", "-L" + HERE, "-lqxmd_amd_f",
+              "-lqxmd_amd", "-Wl,-rpath,$ORIGIN/../../qxmd_amd"])
+        os.replace(drv + ".tmp", drv)
+    return FOUT
 
 
 if __name__ == "__main__":

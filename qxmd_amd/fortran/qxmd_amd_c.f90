@@ -1,0 +1,304 @@
+!-----------------------------------------------------------------------
+! qxmd_amd_c -- ISO_C_BINDING interfaces of the qxmd_amd C-ABI
+! (include/qxmd_amd.h).  Every function returns 0 on success and a
+! negative code on failure; qx_last_error() describes the failure.
+! Arrays follow the reference's Fortran layouts (cgjr(nplwex, nb),
+! fft3x(kfft0), eqlmr_(nplwex, lmmax, natom), dmtrxr(n, n)).
+!-----------------------------------------------------------------------
+module qxmd_amd_c
+use, intrinsic :: iso_c_binding
+implicit none
+
+integer(c_int), parameter :: QX_FP64 = 0, QX_MIXED = 1
+
+interface
+
+type(c_ptr) function qx_last_error() bind(C, name="qx_last_error")
+  import :: c_ptr
+end function
+
+integer(c_int) function qx_version() bind(C, name="qx_version")
+  import :: c_int
+end function
+
+integer(c_int) function qx_get_mesh(hcell, ecutdens, nd) bind(C, name="qx_get_mesh")
+  import :: c_int, c_double
+  real(c_double), intent(in) :: hcell(3, 3)
+  real(c_double), value :: ecutdens
+  integer(c_int), intent(out) :: nd(3)
+end function
+
+integer(c_int) function qx_strvec(rba, ecutdens, ecut, cap, counts, nga, ngb, ngc, recnrm,  &
+&                                 gx, gy, gz) bind(C, name="qx_strvec")
+  import :: c_int, c_double, c_ptr
+  real(c_double), intent(in) :: rba(3, 3)
+  real(c_double), value :: ecutdens, ecut
+  integer(c_int), value :: cap
+  integer(c_int), intent(out) :: counts(6)
+  integer(c_int), intent(out) :: nga(*), ngb(*), ngc(*)
+  real(c_double), intent(out) :: recnrm(*)
+  type(c_ptr), value :: gx, gy, gz
+end function
+
+integer(c_int) function qx_ijkgd(nplw5, nga, ngb, ngc, nd, ijkgd) bind(C, name="qx_ijkgd")
+  import :: c_int
+  integer(c_int), value :: nplw5
+  integer(c_int), intent(in) :: nga(*), ngb(*), ngc(*), nd(3)
+  integer(c_int), intent(out) :: ijkgd(*)
+end function
+
+integer(c_int) function qx_create(ctx, n1, n2, n3, nplw, nga, ngb, ngc, recnrm, nbmax)  &
+&                                 bind(C, name="qx_create")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), intent(out) :: ctx
+  integer(c_int), value :: n1, n2, n3, nplw, nbmax
+  integer(c_int), intent(in) :: nga(*), ngb(*), ngc(*)
+  real(c_double), intent(in) :: recnrm(*)
+end function
+
+integer(c_int) function qx_destroy(ctx) bind(C, name="qx_destroy")
+  import :: c_int, c_ptr
+  type(c_ptr), value :: ctx
+end function
+
+integer(c_int) function qx_sync(ctx) bind(C, name="qx_sync")
+  import :: c_int, c_ptr
+  type(c_ptr), value :: ctx
+end function
+
+integer(c_int) function qx_set_glocal(ctx, glocal) bind(C, name="qx_set_glocal")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_double), intent(in) :: glocal(*)
+end function
+
+integer(c_int) function qx_set_kb(ctx, eqlmr, lmmax, natom, lmx_atom, vclalp_atom)  &
+&                                 bind(C, name="qx_set_kb")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_double), intent(in) :: eqlmr(*)
+  integer(c_int), value :: lmmax, natom
+  integer(c_int), intent(in) :: lmx_atom(*)
+  real(c_double), intent(in) :: vclalp_atom(*)
+end function
+
+integer(c_int) function qx_info(ctx, out) bind(C, name="qx_info")
+  import :: c_int, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), intent(out) :: out(5)
+end function
+
+integer(c_int) function qx_slot_put(ctx, slot, gdcr, nb) bind(C, name="qx_slot_put")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: slot, nb
+  real(c_double), intent(in) :: gdcr(*)
+end function
+
+integer(c_int) function qx_slot_get(ctx, slot, gdcr, nb) bind(C, name="qx_slot_get")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: slot, nb
+  real(c_double), intent(out) :: gdcr(*)
+end function
+
+integer(c_int) function qx_slot_copy(ctx, src, dst, nb) bind(C, name="qx_slot_copy")
+  import :: c_int, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: src, dst, nb
+end function
+
+integer(c_int) function qx_hpsi_slot(ctx, src, dst, nb, prec) bind(C, name="qx_hpsi_slot")
+  import :: c_int, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: src, dst, nb, prec
+end function
+
+integer(c_int) function qx_local_slot(ctx, src, dst, nb, prec, accumulate)  &
+&                                     bind(C, name="qx_local_slot")
+  import :: c_int, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: src, dst, nb, prec, accumulate
+end function
+
+integer(c_int) function qx_kb_slot(ctx, src, dst, nb, slmir) bind(C, name="qx_kb_slot")
+  import :: c_int, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: src, dst, nb
+  type(c_ptr), value :: slmir
+end function
+
+integer(c_int) function qx_schmidt_slot(ctx, slot, nb, lcholesky) bind(C, name="qx_schmidt_slot")
+  import :: c_int, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: slot, nb
+  integer(c_int), intent(out) :: lcholesky
+end function
+
+integer(c_int) function qx_rotate_slot(ctx, psi, hpsi, nb, eig) bind(C, name="qx_rotate_slot")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: psi, hpsi, nb
+  real(c_double), intent(out) :: eig(*)
+end function
+
+integer(c_int) function qx_density_slot(ctx, slot, nb, occ, rvol, rho, prec)  &
+&                                       bind(C, name="qx_density_slot")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: slot, nb, prec
+  real(c_double), intent(in) :: occ(*)
+  real(c_double), value :: rvol
+  type(c_ptr), value :: rho
+end function
+
+integer(c_int) function qx_eigen_bkl_slot(ctx, psi, hpsi, nb, ekib, eig, noband, itermx,  &
+&     iteremx, tolera, threinn, wdinner, roughfzan, roughfeig, bzan, nriter, prec)  &
+&     bind(C, name="qx_eigen_bkl_slot")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: psi, hpsi, nb, noband, itermx, iteremx, prec
+  real(c_double), intent(in) :: ekib(*)
+  real(c_double), intent(inout) :: eig(*)
+  real(c_double), value :: tolera, threinn, wdinner, roughfzan, roughfeig
+  real(c_double), intent(out) :: bzan(*)
+  integer(c_int), intent(out) :: nriter(*)
+end function
+
+integer(c_int) function qx_eigen_bkl_batched_slot(ctx, psi, hpsi, nb, ekib, eig, noband,  &
+&     itermx, iteremx, tolera, roughfzan, roughfeig, bzan, nriter, prec)  &
+&     bind(C, name="qx_eigen_bkl_batched_slot")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: psi, hpsi, nb, noband, itermx, iteremx, prec
+  real(c_double), intent(in) :: ekib(*)
+  real(c_double), intent(inout) :: eig(*)
+  real(c_double), value :: tolera, roughfzan, roughfeig
+  real(c_double), intent(out) :: bzan(*)
+  integer(c_int), intent(out) :: nriter(*)
+end function
+
+integer(c_int) function qx_ekib_slot(ctx, slot, nb, ekib) bind(C, name="qx_ekib_slot")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: slot, nb
+  real(c_double), intent(out) :: ekib(*)
+end function
+
+integer(c_int) function qx_scf_band_step(ctx, psi, hpsi, nb, occ, noband, rvol, eig, rho,  &
+&     lcholesky, prec) bind(C, name="qx_scf_band_step")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: psi, hpsi, nb, noband, prec
+  real(c_double), intent(in) :: occ(*)
+  real(c_double), value :: rvol
+  real(c_double), intent(out) :: eig(*)
+  type(c_ptr), value :: rho
+  integer(c_int), intent(out) :: lcholesky
+end function
+
+integer(c_int) function qx_scf_band_step_ex(ctx, psi, hpsi, nb, occ, noband, rvol, eig, rho,  &
+&     lcholesky, prec, batched) bind(C, name="qx_scf_band_step_ex")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: psi, hpsi, nb, noband, prec, batched
+  real(c_double), intent(in) :: occ(*)
+  real(c_double), value :: rvol
+  real(c_double), intent(out) :: eig(*)
+  type(c_ptr), value :: rho
+  integer(c_int), intent(out) :: lcholesky
+end function
+
+integer(c_int) function qx_event_record(ctx, slot) bind(C, name="qx_event_record")
+  import :: c_int, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: slot
+end function
+
+integer(c_int) function qx_event_elapsed(ctx, slot0, slot1, ms) bind(C, name="qx_event_elapsed")
+  import :: c_int, c_float, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: slot0, slot1
+  real(c_float), intent(out) :: ms
+end function
+
+integer(c_int) function qx_scf_phase_ms(ctx, ms) bind(C, name="qx_scf_phase_ms")
+  import :: c_int, c_float, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_float), intent(out) :: ms(5)
+end function
+
+integer(c_int) function qx_rfft3(key, fft3x, fft3y, k1, k2, k3, ierror) bind(C, name="qx_rfft3")
+  import :: c_int, c_double
+  integer(c_int), value :: key, k1, k2, k3
+  real(c_double), intent(inout) :: fft3x(*), fft3y(*)
+  integer(c_int), intent(out) :: ierror
+end function
+
+integer(c_int) function qx_fft3(key, fft3x, fft3y, k1, k2, k3, ierror) bind(C, name="qx_fft3")
+  import :: c_int, c_double
+  integer(c_int), value :: key, k1, k2, k3
+  real(c_double), intent(inout) :: fft3x(*), fft3y(*)
+  integer(c_int), intent(out) :: ierror
+end function
+
+integer(c_int) function qx_hcnonloc(ctx, cgjr, rhcr) bind(C, name="qx_hcnonloc")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_double), intent(in) :: cgjr(*)
+  real(c_double), intent(inout) :: rhcr(*)
+end function
+
+integer(c_int) function qx_hckbpp_g(ctx, cgjr, rhcr, nb, slmir) bind(C, name="qx_hckbpp_g")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_double), intent(in) :: cgjr(*)
+  real(c_double), intent(inout) :: rhcr(*)
+  integer(c_int), value :: nb
+  type(c_ptr), value :: slmir
+end function
+
+integer(c_int) function qx_hckbpp_ib_g(ctx, cgjr, rhcr) bind(C, name="qx_hckbpp_ib_g")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_double), intent(in) :: cgjr(*)
+  real(c_double), intent(inout) :: rhcr(*)
+end function
+
+integer(c_int) function qx_schmidt(ctx, cgjr, nband, lcholesky) bind(C, name="qx_schmidt")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_double), intent(inout) :: cgjr(*)
+  integer(c_int), value :: nband
+  integer(c_int), intent(out) :: lcholesky
+end function
+
+integer(c_int) function qx_rseigqr(ctx, a, n, w) bind(C, name="qx_rseigqr")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_double), intent(inout) :: a(*)
+  integer(c_int), value :: n
+  real(c_double), intent(out) :: w(*)
+end function
+
+integer(c_int) function qx_icholesky(ctx, s, n, ok) bind(C, name="qx_icholesky")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_double), intent(inout) :: s(*)
+  integer(c_int), value :: n
+  integer(c_int), intent(out) :: ok
+end function
+
+integer(c_int) function qx_softchg(ctx, cgjr, nb, occ, rvol, rho) bind(C, name="qx_softchg")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_double), intent(in) :: cgjr(*)
+  integer(c_int), value :: nb
+  real(c_double), intent(in) :: occ(*)
+  real(c_double), value :: rvol
+  real(c_double), intent(out) :: rho(*)
+end function
+
+end interface
+
+end module qxmd_amd_c

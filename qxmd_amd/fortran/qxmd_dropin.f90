@@ -1,0 +1,334 @@
+!-----------------------------------------------------------------------
+! qxmd_amd_gpu -- Fortran host side of the MI355X path.
+!
+! The module holds the one device context of the process (G sphere, FFT
+! grid, local potential, KB tables, band storage) and gives the QXMD
+! operations whose inputs live in QXMD modules an explicit-argument form.
+! After it, the external subroutines with the reference's names and
+! argument lists replace the reference's own:
+!   rfft3, fft3          Sources/fft3.f90:113, 237
+!   RsEIGQR              Sources/funcs.f90:380
+!   icholesky            Sources/schmidt.f90:383
+!   softchg              Sources/chgdns.f90:388
+!   hckbpp_g             Sources/nlkbpp.f90:1284
+!   hckbpp_ib_g          Sources/nlkbpp.f90:1623
+! A failing library call stops the program with the library's message,
+! the role of fstop in the reference.
+!-----------------------------------------------------------------------
+module qxmd_amd_gpu
+use, intrinsic :: iso_c_binding
+use qxmd_amd_c
+implicit none
+private :: c_msg
+
+type(c_ptr), save :: qxctx = c_null_ptr
+integer, save :: qx_nplw = -1, qx_nbmax = 0, qx_lmmax = 0, qx_natom = 0
+integer, save :: qx_nd(3) = 0
+! slmir(lmmax, natom, nband) of the last hckbpp_g / hckbpp_ib_g call (the
+! module output of nl_variables in the reference)
+real(c_double), allocatable, target, save :: qx_slmir(:,:,:)
+
+contains
+
+function c_msg() result(s)
+  character(len=256) :: s
+  character(kind=c_char), pointer :: p(:)
+  type(c_ptr) :: cp
+  integer :: i
+  s = ' '
+  cp = qx_last_error()
+  if (.not. c_associated(cp)) return
+  call c_f_pointer(cp, p, [256])
+  do i = 1, 256
+    if (p(i) == c_null_char) exit
+    s(i:i) = p(i)
+  end do
+end function
+
+subroutine qxmd_gpu_check(rc, what)
+  integer(c_int), intent(in) :: rc
+  character(*), intent(in) :: what
+  if (rc /= 0) then
+    write(*, '(a, a, i0, a, a)') what, ': qxmd_amd error ', rc, ': ', trim(c_msg())
+    error stop 1
+  end if
+end subroutine
+
+subroutine qxmd_gpu_need_ctx(what)
+  character(*), intent(in) :: what
+  if (.not. c_associated(qxctx)) then
+    write(*, '(a, a)') what, ': qxmd_gpu_init has not been called'
+    error stop 1
+  end if
+end subroutine
+
+! Context from the reference's own set_rciprl arrays for G index 0..nplw
+! (Sources/modules.f90:1402-1603) and FFT grid nd (Sources/input.f90:1082).
+subroutine qxmd_gpu_init(nd, nplw, nga, ngb, ngc, recnrm, nbmax)
+  integer, intent(in) :: nd(3), nplw, nbmax
+  integer, intent(in) :: nga(0:nplw), ngb(0:nplw), ngc(0:nplw)
+  real*8, intent(in) :: recnrm(0:nplw)
+  integer(c_int) :: ia(0:nplw), ib(0:nplw), ic(0:nplw)
+  call qxmd_gpu_finalize()
+  ia = nga
+  ib = ngb
+  ic = ngc
+  call qxmd_gpu_check(qx_create(qxctx, int(nd(1), c_int), int(nd(2), c_int),  &
+&      int(nd(3), c_int), int(nplw, c_int), ia, ib, ic, recnrm, int(nbmax, c_int)),  &
+&      'qxmd_gpu_init')
+  qx_nplw = nplw
+  qx_nbmax = nbmax
+  qx_nd = nd
+end subroutine
+
+! Context for a cell alone: get_mesh + strvec + set_rciprl in the library.
+! hcell(3,3): lattice vectors as columns (bohr); cutoffs in Ry.
+subroutine qxmd_gpu_init_cell(hcell, ecut, ecutdens, nbmax, nd, nplw)
+  real*8, intent(in) :: hcell(3, 3), ecut, ecutdens
+  integer, intent(in) :: nbmax
+  integer, intent(out) :: nd(3), nplw
+  integer(c_int) :: ndc(3), counts(6), cap, rc, dummy(1)
+  integer(c_int), allocatable :: nga(:), ngb(:), ngc(:)
+  real(c_double), allocatable :: rec(:)
+  real(c_double) :: dr(1)
+  call qxmd_gpu_check(qx_get_mesh(hcell, ecutdens, ndc), 'qx_get_mesh')
+  rc = qx_strvec(hcell, ecutdens, ecut, 1_c_int, counts, dummy, dummy, dummy, dr,  &
+&                c_null_ptr, c_null_ptr, c_null_ptr)
+  cap = -rc
+  allocate(nga(cap), ngb(cap), ngc(cap), rec(cap))
+  rc = qx_strvec(hcell, ecutdens, ecut, cap, counts, nga, ngb, ngc, rec,  &
+&                c_null_ptr, c_null_ptr, c_null_ptr)
+  if (rc <= 0) call qxmd_gpu_check(-1_c_int, 'qx_strvec')
+  nd = ndc
+  nplw = counts(3)
+  call qxmd_gpu_init(nd, nplw, nga(1:nplw + 1), ngb(1:nplw + 1), ngc(1:nplw + 1),  &
+&                    rec(1:nplw + 1), nbmax)
+end subroutine
+
+subroutine qxmd_gpu_finalize()
+  integer(c_int) :: rc
+  if (c_associated(qxctx)) rc = qx_destroy(qxctx)
+  qxctx = c_null_ptr
+  qx_nplw = -1
+end subroutine
+
+! glocal = V_H + V_xc + V_loc on the FFT grid, Ry (Sources/engrad.f90:487-495)
+subroutine qxmd_gpu_set_glocal(glocal)
+  real*8, intent(in) :: glocal(*)
+  call qxmd_gpu_need_ctx('qxmd_gpu_set_glocal')
+  call qxmd_gpu_check(qx_set_glocal(qxctx, glocal), 'qx_set_glocal')
+end subroutine
+
+! KB tables as setnlc_g leaves them (Sources/nlkbpp.f90:1236-1279)
+subroutine qxmd_gpu_set_kb(eqlmr_, nplwex, lmmax, natom, lmx, vclalp)
+  integer, intent(in) :: nplwex, lmmax, natom, lmx(natom)
+  real*8, intent(in) :: eqlmr_(nplwex, lmmax, natom), vclalp(lmmax, natom)
+  integer(c_int) :: lmxc(natom)
+  call qxmd_gpu_need_ctx('qxmd_gpu_set_kb')
+  lmxc = lmx
+  call qxmd_gpu_check(qx_set_kb(qxctx, eqlmr_, int(lmmax, c_int), int(natom, c_int), lmxc,  &
+&      vclalp), 'qx_set_kb')
+  qx_lmmax = lmmax
+  qx_natom = natom
+end subroutine
+
+! hcnonloc (Sources/engrad.f90:411-606): rhcr += V_loc cgjr for one band
+subroutine qxmd_hcnonloc(cgjr, rhcr)
+  real*8, intent(in) :: cgjr(*)
+  real*8, intent(inout) :: rhcr(*)
+  call qxmd_gpu_need_ctx('qxmd_hcnonloc')
+  call qxmd_gpu_check(qx_hcnonloc(qxctx, cgjr, rhcr), 'qx_hcnonloc')
+end subroutine
+
+! schmidt (Sources/schmidt.f90:71-378) on cgjr_(nplwex, nband)
+subroutine qxmd_schmidt(cgjr_, nplwex, nband, lcholesky)
+  integer, intent(in) :: nplwex, nband
+  real*8, intent(inout) :: cgjr_(nplwex, nband)
+  logical, intent(out) :: lcholesky
+  integer(c_int) :: ok
+  call qxmd_gpu_need_ctx('qxmd_schmidt')
+  call qxmd_gpu_check(qx_schmidt(qxctx, cgjr_, int(nband, c_int), ok), 'qx_schmidt')
+  lcholesky = ok /= 0
+end subroutine
+
+! One SCF band step with frozen potential on gdcr(nplwex, nband): energy
+! pass, rotation, CG (batched or band by band), Gram-Schmidt, density
+! (Sources/pwlda.f90:2087-3574).  rho on the FFT grid.
+subroutine qxmd_scf_band_step(gdcr, nplwex, nband, occ, noband, rvol, eig, rho, lcholesky,  &
+&                             mixed, batched)
+  integer, intent(in) :: nplwex, nband, noband
+  real*8, intent(inout) :: gdcr(nplwex, nband)
+  real*8, intent(in) :: occ(nband), rvol
+  real*8, intent(out) :: eig(nband)
+  real*8, intent(out), target :: rho(*)
+  logical, intent(out) :: lcholesky
+  logical, intent(in) :: mixed, batched
+  integer(c_int) :: ok, prec, bat
+  call qxmd_gpu_need_ctx('qxmd_scf_band_step')
+  prec = QX_FP64
+  if (mixed) prec = QX_MIXED
+  bat = 0
+  if (batched) bat = 1
+  call qxmd_gpu_check(qx_slot_put(qxctx, 0_c_int, gdcr, int(nband, c_int)), 'qx_slot_put')
+  call qxmd_gpu_check(qx_scf_band_step_ex(qxctx, 0_c_int, 1_c_int, int(nband, c_int), occ,  &
+&      int(noband, c_int), rvol, eig, c_loc(rho), ok, prec, bat), 'qx_scf_band_step_ex')
+  call qxmd_gpu_check(qx_slot_get(qxctx, 0_c_int, gdcr, int(nband, c_int)), 'qx_slot_get')
+  lcholesky = ok /= 0
+end subroutine
+
+end module qxmd_amd_gpu
+
+
+!-----------------------------------------------------------------------
+! Drop-in externals: the reference's names and argument lists.
+!-----------------------------------------------------------------------
+
+! rfft3 (Sources/fft3.f90:113-232): key = 1 real-to-complex x1/N with the full
+! Hermitian grid written back, otherwise complex-to-real from the half grid.
+! work is not used (the transform runs on the device).
+subroutine rfft3(key, fft3x, fft3y, work, kfft1, kfft2, kfft3, kfft0, ierror)
+use qxmd_amd_c
+implicit none
+integer :: key, kfft1, kfft2, kfft3, kfft0, ierror
+real*8 :: fft3x(kfft0), fft3y(kfft0)
+complex*16 :: work(*)
+integer(c_int) :: ie, rc
+rc = qx_rfft3(int(key, c_int), fft3x, fft3y, int(kfft1, c_int), int(kfft2, c_int),  &
+&             int(kfft3, c_int), ie)
+ierror = ie
+if (rc /= 0) ierror = 1
+end subroutine
+
+! fft3 (Sources/fft3.f90:237-333): complex 3-D transform, key = 1 forward x1/N.
+subroutine fft3(key, fft3x, fft3y, work, kfft1, kfft2, kfft3, kfft0, ierror)
+use qxmd_amd_c
+implicit none
+integer :: key, kfft1, kfft2, kfft3, kfft0, ierror
+real*8 :: fft3x(kfft0), fft3y(kfft0)
+complex*16 :: work(*)
+integer(c_int) :: ie, rc
+rc = qx_fft3(int(key, c_int), fft3x, fft3y, int(kfft1, c_int), int(kfft2, c_int),  &
+&            int(kfft3, c_int), ie)
+ierror = ie
+if (rc /= 0) ierror = 1
+end subroutine
+
+! RsEIGQR (Sources/funcs.f90:380-764): eigenvalues of the symmetric AR (lower
+! triangle used) ascending in EVR; with IFVEC = 1 the eigenvectors replace AR
+! column by column.  Sorting is always ascending (IFSRT = 1 returns the
+! identity sorting list in W1I).  Runs in FP64 on the device of the context
+! (N <= nbmax).  CHECK is left untouched.
+SUBROUTINE RsEIGQR(AR, EVR, N, MTRX, EPSI, IFVEC, IFSRT, CHECK, W1R, W1I, work)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: N, MTRX
+real*8 :: AR(MTRX, N), EVR(N)
+real*8 :: EPSI
+integer :: IFVEC, IFSRT
+character*5 :: CHECK
+real*8 :: W1R(N), W1I(N), work(N)
+real*8, allocatable :: a(:,:)
+integer :: j
+call qxmd_gpu_need_ctx('RsEIGQR')
+allocate(a(N, N))
+a = AR(1:N, 1:N)
+call qxmd_gpu_check(qx_rseigqr(qxctx, a, int(N, c_int), EVR), 'RsEIGQR')
+if (IFVEC == 1) AR(1:N, 1:N) = a
+if (IFSRT == 1) then
+  do j = 1, N
+    W1I(j) = dble(j)
+  end do
+end if
+deallocate(a)
+END SUBROUTINE
+
+! icholesky (Sources/schmidt.f90:383-423), one rank: dmtrxr(1:nband, 1:nband)
+! lower triangle -> L^-1 (lower), upper zeroed (rsdecmp + rsinvcd).
+subroutine icholesky(nfile, myid, nodes, dmtrxr, nband, nbnod1, nbnod2, nbxxxx, prod, prod2,  &
+& nbncnt, nbndsp, node_c, lbncnt, lbndsp, mbncnt, mbndsp, jdstnd, bijr, bm1r)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: nfile(*), myid, nodes, nband, nbnod1, nbnod2, nbxxxx, node_c
+integer :: nbncnt(*), nbndsp(*), lbncnt(*), lbndsp(*), mbncnt(*), mbndsp(*), jdstnd(*)
+real*8 :: dmtrxr(nbxxxx, *), prod(*), prod2(*), bijr(*), bm1r(*)
+real*8, allocatable :: s(:,:)
+integer(c_int) :: ok
+call qxmd_gpu_need_ctx('icholesky')
+allocate(s(nband, nband))
+s = dmtrxr(1:nband, 1:nband)
+call qxmd_gpu_check(qx_icholesky(qxctx, s, int(nband, c_int), ok), 'icholesky')
+dmtrxr(1:nband, 1:nband) = s
+deallocate(s)
+end subroutine
+
+! softchg (Sources/chgdns.f90:388-454), one rank, VECTOR layout: the density
+! sum occ/rvol psi(r)^2 on the FFT grid in fft3y, mapped to the FD mesh in
+! glocal(ifd) = fft3y(mfd2ft(ifd)).
+subroutine softchg(nfile, myid, nodes, glocal, occ, cgjr, nplwex, nplw, nband, nbnod1, nbnod2,  &
+& nbnod, iod, rvol, mfd2ft, ntotfd, nd1vks, kfft0, ijkgd, ngenh, fft3x, fft3y, fftwork)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: nfile(*), myid, nodes
+integer :: nplwex, nplw, nband, nbnod1, nbnod2, nbnod, iod(*)
+real*8 :: glocal(*), occ(*), cgjr(nplwex, *), rvol
+integer :: mfd2ft(*), ntotfd, nd1vks(*), kfft0, ngenh
+integer :: ijkgd(-ngenh:ngenh)
+real*8 :: fft3x(*), fft3y(*)
+complex*16 :: fftwork(*)
+real*8, allocatable :: occl(:)
+integer :: i, ifd
+call qxmd_gpu_need_ctx('softchg')
+allocate(occl(nbnod))
+do i = 1, nbnod
+  occl(i) = occ(iod(i))
+end do
+call qxmd_gpu_check(qx_softchg(qxctx, cgjr, int(nbnod, c_int), occl, rvol, fft3y), 'softchg')
+do ifd = 1, ntotfd
+  glocal(ifd) = fft3y(mfd2ft(ifd))
+end do
+deallocate(occl)
+end subroutine
+
+! hckbpp_g (Sources/nlkbpp.f90:1284-1393): rhcr(:, 1:nbnod) += V_NL cgjr with
+! the projector tables registered by qxmd_gpu_set_kb; slmir goes to
+! qx_slmir(lmmax, natom, nbnod).  ycos/ysin are already folded into eqlmr_.
+subroutine hckbpp_g(nfile, myid, nodes, cgjr, rhcr, nplwex, nplw, nband, nbnod1, nbnod2, nbnod,  &
+& ntype, nhk1, nhk2, natom, lkbppi, lking, ycos, ysin, nplwcs)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: nfile(*), myid, nodes, nplwex, nplw, nband, nbnod1, nbnod2, nbnod
+integer :: ntype, natom, nhk1(ntype), nhk2(ntype), nplwcs
+real*8 :: cgjr(nplwex, nbnod), rhcr(nplwex, nbnod)
+logical :: lkbppi(ntype), lking(ntype)
+real*8 :: ycos(0:nplwcs, natom), ysin(0:nplwcs, natom)
+call qxmd_gpu_need_ctx('hckbpp_g')
+if (allocated(qx_slmir)) then
+  if (size(qx_slmir, 3) < nbnod) deallocate(qx_slmir)
+end if
+if (.not. allocated(qx_slmir)) allocate(qx_slmir(max(qx_lmmax, 1), max(qx_natom, 1), nbnod))
+call qxmd_gpu_check(qx_hckbpp_g(qxctx, cgjr, rhcr, int(nbnod, c_int), c_loc(qx_slmir)),  &
+&    'hckbpp_g')
+end subroutine
+
+! hckbpp_ib_g (Sources/nlkbpp.f90:1623-1721): one band.
+subroutine hckbpp_ib_g(nfile, myid, nodes, cgjr, rhcr, nplwex, nplw, ntype, nhk1, nhk2, natom,  &
+& lkbppi, lking, ycos, ysin, nplwcs)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: nfile(*), myid, nodes, nplwex, nplw
+integer :: ntype, natom, nhk1(ntype), nhk2(ntype), nplwcs
+real*8 :: cgjr(nplwex), rhcr(nplwex)
+logical :: lkbppi(ntype), lking(ntype)
+real*8 :: ycos(0:nplwcs, natom), ysin(0:nplwcs, natom)
+call qxmd_gpu_need_ctx('hckbpp_ib_g')
+if (allocated(qx_slmir)) then
+  if (size(qx_slmir, 3) < 1) deallocate(qx_slmir)
+end if
+if (.not. allocated(qx_slmir)) allocate(qx_slmir(max(qx_lmmax, 1), max(qx_natom, 1), 1))
+call qxmd_gpu_check(qx_hckbpp_g(qxctx, cgjr, rhcr, 1_c_int, c_loc(qx_slmir)), 'hckbpp_ib_g')
+end subroutine

@@ -1,0 +1,208 @@
+!-----------------------------------------------------------------------
+! Fortran host check of the drop-in layer: every replaced QXMD routine is
+! called by its reference name and argument list through the ISO_C_BINDING
+! layer, on an 8-atom Si cell, and checked against closed forms.  Prints one
+! line per check and "DROPIN OK" at the end; stops with status 1 on failure.
+!-----------------------------------------------------------------------
+program dropin_check
+use qxmd_amd_gpu
+implicit none
+integer, parameter :: nb = 8, nmat = 12
+real*8 :: hcell(3, 3), pi, rvol, err, worst
+integer :: nd(3), nplw, nplwex, n1, n2, n3, ntot, ierror, i, j, k, l, ib, ia
+real*8, allocatable :: f(:), fx(:), fy(:), f0(:), cg(:,:), rh(:,:), rho(:), occ(:)
+real*8, allocatable :: a(:,:), a0(:,:), w(:), s(:,:), s0(:,:), t(:,:), eq(:,:,:), vcl(:,:)
+real*8, allocatable :: slm(:,:,:), ref(:,:), glocal(:), ycos(:,:), ysin(:,:)
+complex*16 :: work(1), fftwork(1), acc
+integer, allocatable :: lmx(:), mfd2ft(:), iod(:), nfile(:), ijk(:), nhk(:)
+logical, allocatable :: lk(:)
+logical :: lchol
+character*5 :: chk
+real*8 :: w1r(nmat), w1i(nmat), wk(nmat)
+integer :: seed(64)
+
+pi = 4.d0 * atan(1.d0)
+worst = 0.d0
+seed = 20261202
+call random_seed(put=seed(1:size(seed)))
+hcell = 0.d0
+hcell(1, 1) = 10.26d0
+hcell(2, 2) = 10.26d0
+hcell(3, 3) = 10.26d0
+rvol = 10.26d0**3
+call qxmd_gpu_init_cell(hcell, 20.d0, 80.d0, 16, nd, nplw)
+nplwex = 2 * (nplw + 1)
+n1 = nd(1); n2 = nd(2); n3 = nd(3); ntot = n1 * n2 * n3
+write(*, '(a, 3i4, a, i6)') 'grid', nd, '  nplw', nplw
+
+! ---- rfft3: one coefficient against a direct sum, then the round trip
+allocate(f(ntot), fx(ntot), fy(ntot), f0(ntot))
+call random_number(f0)
+f0 = f0 - 0.5d0
+fx = f0
+fy = 0.d0
+call rfft3(1, fx, fy, work, n1, n2, n3, ntot, ierror)
+call expect(ierror == 0, 'rfft3 forward ierror')
+! F(1, 2, 3) = (1/N) sum_r f(r) exp(-2 pi i (x/n1 + 2 y/n2 + 3 z/n3))
+acc = (0.d0, 0.d0)
+do k = 0, n3 - 1
+  do j = 0, n2 - 1
+    do i = 0, n1 - 1
+      acc = acc + f0(1 + i + n1 * (j + n2 * k)) *  &
+&           exp(cmplx(0.d0, -2.d0 * pi * (dble(i) / n1 + 2.d0 * j / n2 + 3.d0 * k / n3), kind=8))
+    end do
+  end do
+end do
+acc = acc / ntot
+l = 1 + 1 + n1 * (2 + n2 * 3)
+err = abs(cmplx(fx(l), fy(l), kind=8) - acc) / abs(acc)
+call report('rfft3 coefficient vs direct sum', err, 1.d-10)
+call rfft3(-1, fx, fy, work, n1, n2, n3, ntot, ierror)
+call expect(ierror == 0, 'rfft3 backward ierror')
+err = maxval(abs(fx - f0)) / maxval(abs(f0))
+call report('rfft3 round trip', err, 1.d-12)
+
+! ---- fft3: complex round trip
+call random_number(fx)
+call random_number(fy)
+f = fx
+f0 = fy
+call fft3(1, fx, fy, work, n1, n2, n3, ntot, ierror)
+call fft3(-1, fx, fy, work, n1, n2, n3, ntot, ierror)
+err = max(maxval(abs(fx - f)), maxval(abs(fy - f0)))
+call report('fft3 round trip', err, 1.d-12)
+
+! ---- RsEIGQR: residual, orthonormality, ascending order
+allocate(a(nmat, nmat), a0(nmat, nmat), w(nmat))
+call random_number(a0)
+a0 = a0 + transpose(a0)
+a = a0
+call RsEIGQR(a, w, nmat, nmat, 1.d-15, 1, 2, chk, w1r, w1i, wk)
+err = 0.d0
+do j = 1, nmat
+  err = max(err, maxval(abs(matmul(a0, a(:, j)) - w(j) * a(:, j))))
+  if (j > 1) call expect(w(j) >= w(j - 1), 'RsEIGQR ascending')
+end do
+err = max(err, maxval(abs(matmul(transpose(a), a) - eye(nmat))))
+call report('RsEIGQR residual + orthonormality', err, 1.d-12)
+
+! ---- icholesky: L^-1 S L^-T = I, upper triangle zero
+allocate(s(nmat, nmat), s0(nmat, nmat), t(nmat, nmat))
+call random_number(t)
+s0 = matmul(transpose(t), t) + 0.1d0 * eye(nmat)
+s = s0
+allocate(nfile(1), iod(1))
+call icholesky(nfile, 0, 1, s, nmat, 1, nmat, nmat, t, t, iod, iod, 1, iod, iod, iod, iod, iod,  &
+&              t, t)
+err = maxval(abs(matmul(matmul(s, s0), transpose(s)) - eye(nmat)))
+do j = 2, nmat
+  err = max(err, maxval(abs(s(1:j - 1, j))))
+end do
+call report('icholesky L^-1 S L^-T = I', err, 1.d-12)
+
+! ---- hcnonloc with a constant potential: rhcr = v0 * cgjr exactly
+allocate(cg(nplwex, nb), rh(nplwex, nb), glocal(ntot))
+call random_number(cg)
+cg = cg - 0.5d0
+cg(2, :) = 0.d0  ! Im psi(G = 0) = 0
+glocal = 0.7d0
+call qxmd_gpu_set_glocal(glocal)
+rh = 0.d0
+call qxmd_hcnonloc(cg(:, 1), rh(:, 1))
+err = maxval(abs(rh(:, 1) - 0.7d0 * cg(:, 1))) / maxval(abs(cg(:, 1)))
+call report('hcnonloc constant potential', err, 1.d-12)
+
+! ---- schmidt: packed overlap 2 sum - x0 y0 = identity afterwards
+call qxmd_schmidt(cg, nplwex, nb, lchol)
+call expect(lchol, 'schmidt lcholesky')
+err = 0.d0
+do j = 1, nb
+  do i = 1, nb
+    err = max(err, abs(2.d0 * dot_product(cg(:, i), cg(:, j)) - cg(1, i) * cg(1, j) -  &
+&                      merge(1.d0, 0.d0, i == j)))
+  end do
+end do
+call report('schmidt orthonormality', err, 1.d-12)
+
+! ---- softchg: integral of rho = sum of occupations
+allocate(rho(ntot), occ(nb), mfd2ft(ntot), ijk(1), nhk(1), lk(1))
+deallocate(iod)
+allocate(iod(nb))
+do i = 1, nb
+  occ(i) = 2.d0 - 0.1d0 * i
+  iod(i) = i
+end do
+do i = 1, ntot
+  mfd2ft(i) = i
+end do
+call softchg(nfile, 0, 1, rho, occ, cg, nplwex, nplw, nb, 1, nb, nb, iod, rvol, mfd2ft, ntot,  &
+&            nd, ntot, ijk, 0, fx, fy, fftwork)
+err = abs(sum(rho) * rvol / ntot - sum(occ)) / sum(occ)
+call report('softchg integral = sum occ', err, 1.d-12)
+
+! ---- hckbpp_g: the KB sum 2 sum_{k>=3} + x0 y0 and its back-projection
+ia = 2
+allocate(eq(nplwex, 4, ia), vcl(4, ia), lmx(ia), slm(4, ia, nb), ref(nplwex, nb))
+allocate(ycos(0:1, ia), ysin(0:1, ia))
+call random_number(eq)
+eq = eq - 0.5d0
+call random_number(vcl)
+lmx = 4
+call qxmd_gpu_set_kb(eq, nplwex, 4, ia, lmx, vcl)
+rh = 0.d0
+call hckbpp_g(nfile, 0, 1, cg, rh, nplwex, nplw, nb, 1, nb, nb, 1, nhk, nhk, ia, lk, lk,  &
+&             ycos, ysin, 1)
+ref = 0.d0
+do ib = 1, nb
+  do k = 1, ia
+    do l = 1, 4
+      slm(l, k, ib) = 2.d0 * dot_product(eq(3:, l, k), cg(3:, ib)) + eq(1, l, k) * cg(1, ib)
+      ref(:, ib) = ref(:, ib) + vcl(l, k) * slm(l, k, ib) * eq(:, l, k)
+    end do
+  end do
+end do
+err = maxval(abs(rh - ref)) / maxval(abs(ref))
+err = max(err, maxval(abs(qx_slmir(:, :, 1:nb) - slm)) / maxval(abs(slm)))
+call report('hckbpp_g projection + back-projection', err, 1.d-12)
+rh(:, 1) = 0.d0
+call hckbpp_ib_g(nfile, 0, 1, cg(:, 3), rh(:, 1), nplwex, nplw, 1, nhk, nhk, ia, lk, lk,  &
+&                ycos, ysin, 1)
+err = maxval(abs(rh(:, 1) - ref(:, 3))) / maxval(abs(ref(:, 3)))
+call report('hckbpp_ib_g one band', err, 1.d-12)
+
+call qxmd_gpu_finalize()
+write(*, '(a, es10.2)') 'DROPIN OK worst', worst
+
+contains
+
+function eye(n) result(m)
+  integer, intent(in) :: n
+  real*8 :: m(n, n)
+  integer :: q
+  m = 0.d0
+  do q = 1, n
+    m(q, q) = 1.d0
+  end do
+end function
+
+subroutine report(what, e, tol)
+  character(*), intent(in) :: what
+  real*8, intent(in) :: e, tol
+  write(*, '(a, a, es10.2)') what, ': ', e
+  worst = max(worst, e)
+  if (.not. (e <= tol)) then
+    write(*, '(a, a)') 'FAILED ', what
+    error stop 1
+  end if
+end subroutine
+
+subroutine expect(cond, what)
+  logical, intent(in) :: cond
+  character(*), intent(in) :: what
+  if (.not. cond) then
+    write(*, '(a, a)') 'FAILED ', what
+    error stop 1
+  end if
+end subroutine
+
+end program dropin_check
