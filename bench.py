@@ -26,6 +26,7 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
+FP32_PEAK_TFS = 157.3
 
 
 def build_inputs(ncell, nband):
@@ -76,6 +77,18 @@ def cpu_worker(ncell, budget_s):
     print(json.dumps({"bands": n, "seconds": dt}))
 
 
+def copy_probe(nbytes=1 << 30, reps=3):
+    """A plain device copy of known size: the HBM byte counters (FETCH_SIZE,
+    WRITE_SIZE) of this kernel calibrate those of the path's kernels."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_each_way": nbytes, "reps": reps}))
+
+
 def cpu_baseline(ncell, threads, budget_s=12.0):
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker",
@@ -99,9 +112,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-worker", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (byte-counter calibration)")
     args = ap.parse_args()
     if args.cpu_worker:
         cpu_worker(args.ncell, args.cpu_budget)
+        return
+    if args.copy_probe:
+        copy_probe()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs the single-GPU path only (one rank, one MI355X); "
@@ -132,8 +150,15 @@ def main():
     for _ in range(args.warmup):
         hpsi(pw.MIXED)()
         hpsi(pw.FP64)()
+    # launch timers (HIP events on the library stream) run over the timed region
+    ctx.timer_start(ctx.KB_GEMM)
+    ctx.timer_start(ctx.FFT_STAGE)
     ms_mixed = time_region(ctx, hpsi(pw.MIXED), args.steps)
-    print(f"# H|psi> mixed: {ms_mixed:.3f} ms/step", file=sys.stderr, flush=True)
+    kb_ms, kb_n = ctx.timer_stop(ctx.KB_GEMM)
+    fft_ms, fft_n = ctx.timer_stop(ctx.FFT_STAGE)
+    print(f"# H|psi> mixed: {ms_mixed:.3f} ms/step; KB GEMM launches {kb_n} avg "
+          f"{kb_ms / max(kb_n, 1):.3f} ms, FFT-stage launches {fft_n} avg "
+          f"{fft_ms / max(fft_n, 1):.3f} ms", file=sys.stderr, flush=True)
     ms_fp64 = time_region(ctx, hpsi(pw.FP64), args.steps)
     print(f"# H|psi> fp64 : {ms_fp64:.3f} ms/step", file=sys.stderr, flush=True)
     # stage timings (same stream, HIP events)
@@ -144,20 +169,35 @@ def main():
     print(f"# stages: local fp32 {ms_loc32:.3f} ms, local fp64 {ms_loc64:.3f} ms, "
           f"KB fp64 {ms_kb:.3f} ms", file=sys.stderr, flush=True)
 
-    # roofline of the dominant stage of the mixed step
+    # roofline of the dominant kernel of the mixed step, from the launch timers:
+    #  - KB: each GEMM launch (projection, back-projection) does 2*npj*nplwex*nb
+    #    flops on FP32 MFMA;
+    #  - FFT stage (zinv + plane + zfwd of one band batch): the algorithmic bytes
+    #    8*N_h*s_c + N*s_r + 2(nplw+1)*s_c per band, FP32.
     nplwex = gs.nplwex
-    kb_flops = 4.0 * ctx.npj * nplwex * nband
-    fft_bytes32 = fft_stage_bytes(gs, 4) * nband
-    if ms_kb >= ms_loc32:
-        ach = kb_flops / (ms_kb * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": FP64_PEAK_TFS,
-                "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFS, 4), "traffic": None,
-                "kernel": "k_gemm_f64 (KB projection + back-projection)"}
+    kb_launch_flops = 2.0 * ctx.npj * nplwex * nband
+    fft_bands_per_launch = nband * args.steps / max(fft_n, 1)
+    fft_launch_bytes = fft_stage_bytes(gs, 4) * fft_bands_per_launch
+    kb_avg = kb_ms / max(kb_n, 1)
+    fft_avg = fft_ms / max(fft_n, 1)
+    kb_roof = {"bound": "mfma", "achieved": round(kb_launch_flops / (kb_avg * 1e-3) / 1e12, 3),
+               "peak": FP32_PEAK_TFS, "unit": "TFLOP/s", "traffic": None,
+               "kernel": "k_gemm_f32 (KB projection / back-projection, FP32 MFMA)",
+               "launch_ms": round(kb_avg, 4), "launches": kb_n,
+               "algorithmic_per_launch": kb_launch_flops}
+    kb_roof["frac"] = round(kb_roof["achieved"] / FP32_PEAK_TFS, 4)
+    fft_roof = {"bound": "hbm", "achieved": round(fft_launch_bytes / (fft_avg * 1e-3) / 1e9, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
+                "kernel": "k_zinv_f + k_plane_f + k_zfwd_f (FP32 pencil FFT stage of a band batch)",
+                "launch_ms": round(fft_avg, 4), "launches": fft_n,
+                "algorithmic_per_launch": fft_launch_bytes}
+    fft_roof["frac"] = round(fft_roof["achieved"] / HBM_PEAK_GBS, 4)
+    if fft_ms >= kb_ms:
+        roof, other = fft_roof, kb_roof
     else:
-        ach = fft_bytes32 / (ms_loc32 * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_zinv+k_plane+k_zfwd (FP32 local band-apply)"}
+        roof, other = kb_roof, fft_roof
+    roof = {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")} | {
+        k: roof[k] for k in roof if k not in ("bound", "achieved", "peak", "unit", "frac", "traffic")}
 
     extra = {}
     if not args.no_scf:
@@ -172,13 +212,14 @@ def main():
             eig, rho, ok = ctx.scf_band_step(nband, occ, cell.volume, prec=prec, batched=batched,
                                              want_rho=True)
             dt = time.perf_counter() - t0
-            res[name] = (dt, eig, ok)
-            print(f"# SCF band step {name}: {dt * 1e3:.1f} ms ok={ok}", file=sys.stderr,
-                  flush=True)
+            res[name] = (dt, eig, ok, ctx.scf_phase_ms())
+            print(f"# SCF band step {name}: {dt * 1e3:.1f} ms ok={ok} phases {res[name][3]}",
+                  file=sys.stderr, flush=True)
         e64 = np.sum(occ * np.sort(res["fp64_batched"][1])) / 2.0
         emx = np.sum(occ * np.sort(res["mixed_batched"][1])) / 2.0
         extra = {
             "scf_step_ms": {k: round(v[0] * 1e3, 2) for k, v in res.items()},
+            "scf_phase_ms": {k: v[3] for k, v in res.items()},
             "mixed_vs_fp64_band_energy_err_ha_per_atom": float(abs(emx - e64) / natom),
             "mixed_vs_fp64_max_eig_err_ha": float(np.abs(np.sort(res["mixed_batched"][1]) -
                                                          np.sort(res["fp64_batched"][1])).max() / 2),
@@ -206,7 +247,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 FFT + fp64 MFMA (mixed)",
+        "dtype": "fp32 FFT + fp32 MFMA KB (mixed); fp64 dense",
         "data": "synthetic",
         "config": {"workload": f"Si{natom} Gamma H|psi>, {nband} bands, grid {gs.nd[0]}^3, "
                                f"nplw {gs.nplw}, {ctx.npj} KB projectors",
@@ -215,8 +256,9 @@ def main():
         "fp64_band_applies_per_s": round(nband / (ms_fp64 * 1e-3), 1),
         "fp64_ms_per_step": round(ms_fp64, 4),
         "stage_ms": {"local_fp32": round(ms_loc32, 4), "local_fp64": round(ms_loc64, 4),
-                     "kb_fp64": round(ms_kb, 4)},
+                     "kb_fp64": round(ms_kb, 4), "kb_gemm_fp32_launches": round(kb_ms / args.steps, 4)},
         "roofline": roof,
+        "roofline_other": other,
         "cpu_baseline": cpu,
         **extra,
     }

@@ -117,6 +117,11 @@ int qx_event_elapsed(qx_ctx *ctx, int slot0, int slot1, float *ms);
  * Gram-Schmidt, density -- the role of the ltimecnt phase timers
  * (Sources/pwlda.f90:1194-1196) */
 int qx_scf_phase_ms(qx_ctx *ctx, float *ms5);
+/* launch timers with HIP events on the library stream: id 0 = the KB GEMM
+ * launches, 1 = the pencil FFT stage of a band batch.  mode 1 clears and
+ * starts the timer; mode 0 stops it and returns the summed ms and the number
+ * of timed launches.  (Measurement only; no counterpart in the reference.) */
+int qx_launch_timer(qx_ctx *ctx, int id, int mode, double *ms, int *count);
 
 /* ---------------- Fortran-shaped drop-ins (host arrays) -------------------- */
 /* rfft3 (Sources/fft3.f90:113-232): key=1 r2c x1/N with full Hermitian grid

@@ -93,7 +93,7 @@ static int get_reference(Ctx* c, const double* src, double* host, int nb) {
 int hpsi_batched(Ctx* c, const double* x, double* hx, int nb, int prec) {
   int rc = local_apply(c, x, hx, nb, prec, 0);
   if (rc) return rc;
-  return kb_apply(c, x, hx, nb, 1, nullptr);
+  return kb_apply(c, x, hx, nb, 1, nullptr, prec);
 }
 
 // Gram-Schmidt by Cholesky of the packed overlap (schmidt2)
@@ -264,6 +264,8 @@ int qx_destroy(qx_ctx* h) {
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->pev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& t : c->lt)
+    for (auto& e : t.ev) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->stream);
   delete h;
   return 0;
@@ -372,7 +374,7 @@ int qx_local_slot(qx_ctx* h, int src, int dst, int nb, int prec, int accumulate)
 
 int qx_kb_slot(qx_ctx* h, int src, int dst, int nb, double* slmir_host) {
   Ctx* c = &h->c;
-  int rc = kb_apply(c, c->d_slot[src], c->d_slot[dst], nb, 1, nullptr);
+  int rc = kb_apply(c, c->d_slot[src], c->d_slot[dst], nb, 1, nullptr, PREC_FP64);
   if (rc) return rc;
   (void)slmir_host;
   return 0;
@@ -478,6 +480,55 @@ int qx_scf_band_step_ex(qx_ctx* h, int psi, int hpsi, int nb, const double* occ,
   return 0;
 }
 
+}  // extern "C"
+
+namespace qx {
+void lt_mark(Ctx* c, int id, bool end) {
+  if (id < 0 || id >= Ctx::kTimers || !c->lt[id].on) return;
+  auto& t = c->lt[id];
+  if (!end && t.used + 2 > t.ev.size()) {
+    if (t.ev.size() >= 200000) return;  // bounded: stop recording past 100k launches
+    for (int k = 0; k < 64; ++k) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      t.ev.push_back(e);
+    }
+  }
+  if (!end) {
+    (void)hipEventRecord(t.ev[t.used], c->stream);
+  } else if (t.used + 1 < t.ev.size()) {
+    (void)hipEventRecord(t.ev[t.used + 1], c->stream);
+    t.used += 2;
+  }
+}
+}  // namespace qx
+
+extern "C" {
+
+// launch timers: mode 1 clears and starts timer id, mode 0 stops it and
+// returns the summed duration (ms) and the number of timed launches
+int qx_launch_timer(qx_ctx* h, int id, int mode, double* ms, int* count) {
+  Ctx* c = &h->c;
+  if (id < 0 || id >= Ctx::kTimers) return fail(-1, "qx_launch_timer: bad timer id");
+  auto& t = c->lt[id];
+  if (mode == 1) {
+    t.on = true;
+    t.used = 0;
+    return 0;
+  }
+  t.on = false;
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  double sum = 0.0;
+  for (size_t k = 0; k + 1 < t.used; k += 2) {
+    float e = 0.f;
+    QX_CHECK(hipEventElapsedTime(&e, t.ev[k], t.ev[k + 1]));
+    sum += e;
+  }
+  if (ms) *ms = sum;
+  if (count) *count = (int)(t.used / 2);
+  return 0;
+}
+
 // phases of the last SCF band step, ms: energy pass, rotation, CG, schmidt, density
 int qx_scf_phase_ms(qx_ctx* h, float* ms) {
   for (int i = 0; i < 5; ++i) ms[i] = h->c.phase_ms[i];
@@ -533,7 +584,8 @@ int qx_hckbpp_g(qx_ctx* h, const double* cgjr, double* rhcr, int nb, double* slm
   if ((rc = put_internal(c, c->d_slot[1], rhcr, nb))) return rc;
   double* P = c->d_slot[2];  // projections staged here (npj x nb fits a slot)
   if ((size_t)c->npj * nb > (size_t)c->nplwex * c->nbmax) return fail(-1, "too many projectors");
-  if ((rc = kb_apply(c, c->d_slot[0], c->d_slot[1], nb, 1, slmir ? P : nullptr))) return rc;
+  if ((rc = kb_apply(c, c->d_slot[0], c->d_slot[1], nb, 1, slmir ? P : nullptr, PREC_FP64)))
+    return rc;
   if (slmir) {
     std::vector<double> ph((size_t)c->npj * nb);
     QX_CHECK(hipMemcpyAsync(ph.data(), P, sizeof(double) * ph.size(), hipMemcpyDeviceToHost,

@@ -116,7 +116,7 @@ static double pdot1(Ctx* c, const double* x, const double* y) {
 static int hpsi1(Ctx* c, const double* x, double* hx, int prec) {
   int rc = local_apply(c, x, hx, 1, prec, 0);
   if (rc) return rc;
-  return kb_apply(c, x, hx, 1, 1, nullptr);
+  return kb_apply(c, x, hx, 1, 1, nullptr, prec);
 }
 
 int bkl_band_by_band(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib, double* eig,
@@ -338,9 +338,9 @@ int bkl_batched(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib_ho
     if (it == 0) axpby(c, hnk, gnk, nullptr, nullptr, nullptr, 1.0, 0.0, nb, s.active);
     else axpby(c, hnk, gnk, hnk, nullptr, s.beta, 1.0, 0.0, nb, s.active);
     // project out all current bands: h <- h - Psi (Psi^T W h)
-    int rc = gemm_tn(c, nb, nb, n, 2.0, psi, n, hnk, n, 0.0, proj, nb, 1);
+    int rc = gemm_tn_p(c, nb, nb, n, 2.0, psi, n, hnk, n, 0.0, proj, nb, 1, prec);
     if (rc) return rc;
-    rc = gemm_nn(c, n, nb, nb, -1.0, psi, n, proj, nb, 1.0, hnk, n);
+    rc = gemm_nn_p(c, n, nb, nb, -1.0, psi, n, proj, nb, 1.0, hnk, n, prec);
     if (rc) return rc;
     pdot_batched(c, hnk, hpsi, nb, s.cai);
     rc = hpsi_batched(c, hnk, hh, nb, prec);

@@ -66,7 +66,18 @@ struct Ctx {
   hipEvent_t ev[8] = {};
   hipEvent_t pev[6] = {};                 // SCF band-step phase timers
   float phase_ms[5] = {};
+  // launch timers (HIP events on the context stream), off unless enabled
+  static constexpr int kTimers = 2;       // 0: KB GEMM launches, 1: pencil FFT stage
+  struct LaunchTimer {
+    bool on = false;
+    std::vector<hipEvent_t> ev;           // begin/end pairs
+    size_t used = 0;
+  } lt[kTimers];
+  int lt_gemm_id = -1;                    // timer the GEMM launches report to
 };
+
+// record the begin (end) event of a timed launch of timer id, if enabled
+void lt_mark(Ctx* c, int id, bool end);
 
 int pencil_build(Ctx* c, const int* nga, const int* ngb, const int* ngc, const double* recnrm);
 
@@ -79,12 +90,18 @@ int fast_permute_v(Ctx* c);
 int local_apply_fast(Ctx* c, const double* psi, double* out, int nb, int prec, int accumulate);
 int density_fast(Ctx* c, const double* psi, int nb, const double* w, int prec);
 int kb_apply(Ctx* c, const double* psi, double* out, int nb, int accumulate_into_out,
-             double* slmir_out);
+             double* slmir_out, int prec);
 int coop_launch(Ctx* c, const void* fn, int want, size_t lds, void** args, bool exact);
 int dense_chol_inv(Ctx* c, double* s, int n, double* x, int* ok, int* flag);
 int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
             int ldb, double beta, double* C, int ldc, int corr);
 int gemm_nn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
             int ldb, double beta, double* C, int ldc);
+// the same contractions with a precision: PREC_FP32 -> FP32 MFMA (operands
+// rounded to FP32, FP32 accumulation per K chunk), otherwise FP64
+int gemm_tn_p(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
+              const double* B, int ldb, double beta, double* C, int ldc, int corr, int prec);
+int gemm_nn_p(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
+              const double* B, int ldb, double beta, double* C, int ldc, int prec);
 
 }  // namespace qx

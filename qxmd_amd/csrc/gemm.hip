@@ -18,6 +18,7 @@
 // of v_mfma_f64_16x16x4 blocks; K staged through LDS 16 at a time.  The
 // long-K shape splits K over blocks and reduces with FP64 atomics.
 #include <algorithm>
+#include <cstdlib>
 #include "context.h"
 
 namespace qx {
@@ -124,8 +125,8 @@ __global__ void k_corr(double* C, int M, int N, int ldc, const double* A, int ld
 static inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
 // corr: 0 none, 1 subtract the G=0 term, 2 also the Im(G=0) term (KB)
-int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
-            int ldb, double beta, double* C, int ldc, int corr) {
+static int gemm_tn_small(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
+                         const double* B, int ldb, double beta, double* C, int ldc, int corr) {
   if (M <= 0 || N <= 0) return 0;
   hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N, ldc,
                      beta);
@@ -134,12 +135,349 @@ int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
   int kchunk = (int)(cdiv(cdiv(K, nsplit), KT) * KT);
   nsplit = (int)cdiv(K, kchunk);
   dim3 grid(cdiv(M, BM), cdiv(N, BN), nsplit);
+  lt_mark(c, c->lt_gemm_id, false);
   if (nsplit > 1)
     hipLaunchKernelGGL((k_gemm_f64<true, true>), grid, dim3(256), 0, c->stream, M, N, K, kchunk,
                        alpha, A, lda, B, ldb, C, ldc);
   else
     hipLaunchKernelGGL((k_gemm_f64<true, false>), grid, dim3(256), 0, c->stream, M, N, K, kchunk,
                        alpha, A, lda, B, ldb, C, ldc);
+  lt_mark(c, c->lt_gemm_id, true);
+  if (corr)
+    hipLaunchKernelGGL(k_corr, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N, ldc,
+                       A, lda, B, ldb, alpha, corr == 2 ? 1 : 0);
+  QX_CHECK(hipGetLastError());
+  return 0;
+}
+
+static int gemm_nn_small(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
+                         const double* B, int ldb, double beta, double* C, int ldc) {
+  if (M <= 0 || N <= 0) return 0;
+  if (beta != 1.0)
+    hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N,
+                       ldc, beta);
+  dim3 grid(cdiv(M, BM), cdiv(N, BN), 1);
+  lt_mark(c, c->lt_gemm_id, false);
+  hipLaunchKernelGGL((k_gemm_f64<false, false>), grid, dim3(256), 0, c->stream, M, N, K, K, alpha,
+                     A, lda, B, ldb, C, ldc);
+  lt_mark(c, c->lt_gemm_id, true);
+  QX_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ---- mixed precision: FP64 operands rounded to FP32 while staged into LDS,
+// v_mfma_f32_16x16x4f32 products with FP32 accumulation over each K chunk,
+// chunks combined in FP64 (split-K atomics or the FP64 C update).
+typedef float f4 __attribute__((ext_vector_type(4)));
+constexpr int KT32 = 32, LDT32 = BM + 4;
+
+template <bool AKC, bool SPLIT>
+__global__ void __launch_bounds__(256) k_gemm_f32(int M, int N, int K, int kchunk, double alpha,
+                                                  const double* __restrict__ A, int lda,
+                                                  const double* __restrict__ B, int ldb,
+                                                  double* __restrict__ C, int ldc) {
+  __shared__ float As[KT32][LDT32];
+  __shared__ float Bs[KT32][LDT32];
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  f4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  // per thread: 8 A values and 8 B values of the 64 x 32 stage
+  float ra[8], rb[8];
+  auto fetch = [&](int k0) {
+    if (AKC) {
+      const int m = tid >> 2, kk = (tid & 3) * 8, gm = m0 + m;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int gk = k0 + kk + u;
+        ra[u] = (gm < M && gk < ke) ? (float)A[(size_t)gm * lda + gk] : 0.f;
+      }
+    } else {
+      const int kk = tid >> 3, m = (tid & 7) * 8, gk = k0 + kk;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int gm = m0 + m + u;
+        ra[u] = (gm < M && gk < ke) ? (float)A[(size_t)gk * lda + gm] : 0.f;
+      }
+    }
+    const int n = tid >> 2, kk = (tid & 3) * 8, gn = n0 + n;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int gk = k0 + kk + u;
+      rb[u] = (gn < N && gk < ke) ? (float)B[(size_t)gn * ldb + gk] : 0.f;
+    }
+  };
+  fetch(kb);
+  for (int k0 = kb; k0 < ke; k0 += KT32) {
+    if (AKC) {
+      const int m = tid >> 2, kk = (tid & 3) * 8;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) As[kk + u][m] = ra[u];
+    } else {
+      const int kk = tid >> 3, m = (tid & 7) * 8;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) As[kk][m + u] = ra[u];
+    }
+    {
+      const int n = tid >> 2, kk = (tid & 3) * 8;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) Bs[kk + u][n] = rb[u];
+    }
+    __syncthreads();
+    if (k0 + KT32 < ke) fetch(k0 + KT32);  // next stage in flight during the MFMAs
+#pragma unroll
+    for (int k4 = 0; k4 < KT32; k4 += 4) {
+      const int kr = k4 + (lane >> 4), c = lane & 15;
+      const float a0 = As[kr][wm + c], a1 = As[kr][wm + 16 + c];
+      const float b0 = Bs[kr][wn + c], b1 = Bs[kr][wn + 16 + c];
+      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // v_mfma_f32_16x16x4f32 leaves D(i, j) in lane j + 16*(i/4), register i%4
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int n = n0 + wn + j * 16 + (lane & 15);
+      if (n >= N) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = m0 + wm + i * 16 + 4 * (lane >> 4) + v;
+        if (m >= M) continue;
+        const double val = alpha * (double)acc[i][j][v];
+        if (SPLIT) atomicAdd(&C[(size_t)n * ldc + m], val);
+        else C[(size_t)n * ldc + m] += val;
+      }
+    }
+}
+
+// ---- 128 x 128 tiles: four wavefronts of 64 x 64 (4 x 4 MFMA blocks each),
+// the next K stage fetched into registers while the current one is on the
+// MFMA pipes.  T = double (v_mfma_f64_16x16x4f64, KT 16) or float
+// (v_mfma_f32_16x16x4f32 on FP64 operands rounded while staged, KT 32).
+template <typename T>
+struct BigCfg;
+template <>
+struct BigCfg<double> {
+  static constexpr int KT = 16;
+  typedef d4 acc_t;
+  __device__ static acc_t mfma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // D(i, j): lane j + 16 (i % 4), register i / 4
+  __device__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <>
+struct BigCfg<float> {
+  static constexpr int KT = 32;
+  typedef f4 acc_t;
+  __device__ static acc_t mfma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // D(i, j): lane j + 16 (i / 4), register i % 4
+  __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+constexpr int BB = 128;
+
+template <typename T, bool AKC, bool SPLIT>
+__global__ void __launch_bounds__(256) k_gemm_big(int M, int N, int K, int kchunk, double alpha,
+                                                  const double* __restrict__ A, int lda,
+                                                  const double* __restrict__ B, int ldb,
+                                                  double* __restrict__ C, int ldc) {
+  typedef BigCfg<T> Cfg;
+  constexpr int KT = Cfg::KT, LD = BB + 4;
+  constexpr int PER = BB * KT / 256;  // operand values per thread per stage
+  __shared__ T As[KT][LD];
+  __shared__ T Bs[KT][LD];
+  const int m0 = blockIdx.x * BB, n0 = blockIdx.y * BB;
+  const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  typename Cfg::acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[i][j][v] = (T)0;
+  T ra[PER], rb[PER];
+  // k-contiguous operands: thread -> (row r = tid / (KT/PER), k run of PER)
+  constexpr int KRUN = PER, RPT = KT / KRUN;  // threads per row
+  auto fetch = [&](int k0) {
+    if (AKC) {
+      const int r = tid / RPT, kk = (tid % RPT) * KRUN, gm = m0 + r;
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int gk = k0 + kk + u;
+        ra[u] = (gm < M && gk < ke) ? (T)A[(size_t)gm * lda + gk] : (T)0;
+      }
+    } else {
+      // m-contiguous: thread -> (k row, m run of PER)
+      const int kk = tid / (BB / PER), m = (tid % (BB / PER)) * PER, gk = k0 + kk;
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int gm = m0 + m + u;
+        ra[u] = (gm < M && gk < ke) ? (T)A[(size_t)gk * lda + gm] : (T)0;
+      }
+    }
+    const int r = tid / RPT, kk = (tid % RPT) * KRUN, gn = n0 + r;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int gk = k0 + kk + u;
+      rb[u] = (gn < N && gk < ke) ? (T)B[(size_t)gn * ldb + gk] : (T)0;
+    }
+  };
+  fetch(kb);
+  for (int k0 = kb; k0 < ke; k0 += KT) {
+    if (AKC) {
+      const int r = tid / RPT, kk = (tid % RPT) * KRUN;
+#pragma unroll
+      for (int u = 0; u < PER; ++u) As[kk + u][r] = ra[u];
+    } else {
+      const int kk = tid / (BB / PER), m = (tid % (BB / PER)) * PER;
+#pragma unroll
+      for (int u = 0; u < PER; ++u) As[kk][m + u] = ra[u];
+    }
+    {
+      const int r = tid / RPT, kk = (tid % RPT) * KRUN;
+#pragma unroll
+      for (int u = 0; u < PER; ++u) Bs[kk + u][r] = rb[u];
+    }
+    __syncthreads();
+    if (k0 + KT < ke) fetch(k0 + KT);
+#pragma unroll
+    for (int k4 = 0; k4 < KT; k4 += 4) {
+      const int kr = k4 + (lane >> 4), c = lane & 15;
+      T a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[i] = As[kr][wm + 16 * i + c];
+        b[i] = Bs[kr][wn + 16 * i + c];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = Cfg::mfma(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = n0 + wn + j * 16 + (lane & 15);
+      if (n >= N) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = m0 + wm + i * 16 + Cfg::row(lane, v);
+        if (m >= M) continue;
+        const double val = alpha * (double)acc[i][j][v];
+        if (SPLIT) atomicAdd(&C[(size_t)n * ldc + m], val);
+        else C[(size_t)n * ldc + m] += val;
+      }
+    }
+}
+
+// use the 128 x 128 kernels when the output has enough tiles of that size
+static bool use_big(int M, int N) {
+  static const int env = std::getenv("QX_GEMM_SMALL") ? 1 : 0;
+  return !env && M >= 256 && N >= 256;
+}
+
+template <typename T>
+static void launch_big_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
+                          const double* B, int ldb, double* C, int ldc) {
+  constexpr int KT = BigCfg<T>::KT;
+  const int tiles = (int)(cdiv(M, BB) * cdiv(N, BB));
+  int nsplit = std::max<int>(1, std::min<int>((int)cdiv(K, 512), std::max(1, 2048 / tiles)));
+  int kchunk = (int)(cdiv(cdiv(K, nsplit), KT) * KT);
+  nsplit = (int)cdiv(K, kchunk);
+  dim3 grid(cdiv(M, BB), cdiv(N, BB), nsplit);
+  if (nsplit > 1)
+    hipLaunchKernelGGL((k_gemm_big<T, true, true>), grid, dim3(256), 0, c->stream, M, N, K, kchunk,
+                       alpha, A, lda, B, ldb, C, ldc);
+  else
+    hipLaunchKernelGGL((k_gemm_big<T, true, false>), grid, dim3(256), 0, c->stream, M, N, K,
+                       kchunk, alpha, A, lda, B, ldb, C, ldc);
+}
+
+template <typename T>
+static void launch_big_nn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
+                          const double* B, int ldb, double* C, int ldc) {
+  dim3 grid(cdiv(M, BB), cdiv(N, BB), 1);
+  hipLaunchKernelGGL((k_gemm_big<T, false, false>), grid, dim3(256), 0, c->stream, M, N, K, K,
+                     alpha, A, lda, B, ldb, C, ldc);
+}
+
+int gemm_tn_p(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
+              int ldb, double beta, double* C, int ldc, int corr, int prec) {
+  if (prec != PREC_FP32) return gemm_tn(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, corr);
+  if (M <= 0 || N <= 0) return 0;
+  hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N, ldc,
+                     beta);
+  lt_mark(c, c->lt_gemm_id, false);
+  if (use_big(M, N)) {
+    launch_big_tn<float>(c, M, N, K, alpha, A, lda, B, ldb, C, ldc);
+  } else {
+    const int tiles = (int)(cdiv(M, BM) * cdiv(N, BN));
+    int nsplit = std::max<int>(1, std::min<int>((int)cdiv(K, 1024), std::max(1, 2048 / tiles)));
+    int kchunk = (int)(cdiv(cdiv(K, nsplit), KT32) * KT32);
+    nsplit = (int)cdiv(K, kchunk);
+    dim3 grid(cdiv(M, BM), cdiv(N, BN), nsplit);
+    if (nsplit > 1)
+      hipLaunchKernelGGL((k_gemm_f32<true, true>), grid, dim3(256), 0, c->stream, M, N, K, kchunk,
+                         alpha, A, lda, B, ldb, C, ldc);
+    else
+      hipLaunchKernelGGL((k_gemm_f32<true, false>), grid, dim3(256), 0, c->stream, M, N, K,
+                         kchunk, alpha, A, lda, B, ldb, C, ldc);
+  }
+  lt_mark(c, c->lt_gemm_id, true);
+  if (corr)
+    hipLaunchKernelGGL(k_corr, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N, ldc,
+                       A, lda, B, ldb, alpha, corr == 2 ? 1 : 0);
+  QX_CHECK(hipGetLastError());
+  return 0;
+}
+
+int gemm_nn_p(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
+              int ldb, double beta, double* C, int ldc, int prec) {
+  if (prec != PREC_FP32) return gemm_nn(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+  if (M <= 0 || N <= 0) return 0;
+  if (beta != 1.0)
+    hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N,
+                       ldc, beta);
+  lt_mark(c, c->lt_gemm_id, false);
+  if (use_big(M, N)) {
+    launch_big_nn<float>(c, M, N, K, alpha, A, lda, B, ldb, C, ldc);
+  } else {
+    dim3 grid(cdiv(M, BM), cdiv(N, BN), 1);
+    hipLaunchKernelGGL((k_gemm_f32<false, false>), grid, dim3(256), 0, c->stream, M, N, K, K,
+                       alpha, A, lda, B, ldb, C, ldc);
+  }
+  lt_mark(c, c->lt_gemm_id, true);
+  QX_CHECK(hipGetLastError());
+  return 0;
+}
+
+// FP64 contractions: 128 x 128 tiles for sizable outputs, 64 x 64 otherwise
+int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
+            int ldb, double beta, double* C, int ldc, int corr) {
+  if (!use_big(M, N)) return gemm_tn_small(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, corr);
+  hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N, ldc,
+                     beta);
+  lt_mark(c, c->lt_gemm_id, false);
+  launch_big_tn<double>(c, M, N, K, alpha, A, lda, B, ldb, C, ldc);
+  lt_mark(c, c->lt_gemm_id, true);
   if (corr)
     hipLaunchKernelGGL(k_corr, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N, ldc,
                        A, lda, B, ldb, alpha, corr == 2 ? 1 : 0);
@@ -149,13 +487,13 @@ int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
 
 int gemm_nn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
             int ldb, double beta, double* C, int ldc) {
-  if (M <= 0 || N <= 0) return 0;
+  if (!use_big(M, N)) return gemm_nn_small(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
   if (beta != 1.0)
     hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N,
                        ldc, beta);
-  dim3 grid(cdiv(M, BM), cdiv(N, BN), 1);
-  hipLaunchKernelGGL((k_gemm_f64<false, false>), grid, dim3(256), 0, c->stream, M, N, K, K, alpha,
-                     A, lda, B, ldb, C, ldc);
+  lt_mark(c, c->lt_gemm_id, false);
+  launch_big_nn<double>(c, M, N, K, alpha, A, lda, B, ldb, C, ldc);
+  lt_mark(c, c->lt_gemm_id, true);
   QX_CHECK(hipGetLastError());
   return 0;
 }
@@ -168,18 +506,24 @@ __global__ void k_scale_proj(double* P, const double* D, int npj, int nb) {
 }
 
 // Kleinman-Bylander apply for nb bands: out += B diag(D) (B^T W psi).
-int kb_apply(Ctx* c, const double* psi, double* out, int nb, int, double* slmir_out) {
+// prec PREC_FP32 runs both contractions on FP32 MFMA (mixed form).
+int kb_apply(Ctx* c, const double* psi, double* out, int nb, int, double* slmir_out, int prec) {
   if (c->npj == 0) return 0;
-  int rc = gemm_tn(c, c->npj, nb, c->nplwex, 2.0, c->d_B, c->nplwex, psi, c->nplwex, 0.0, c->d_P,
-                   c->npj, 2);
+  struct TimeGemms {  // the KB contractions are what launch timer 0 measures
+    Ctx* c;
+    explicit TimeGemms(Ctx* cc) : c(cc) { c->lt_gemm_id = 0; }
+    ~TimeGemms() { c->lt_gemm_id = -1; }
+  } tg(c);
+  int rc = gemm_tn_p(c, c->npj, nb, c->nplwex, 2.0, c->d_B, c->nplwex, psi, c->nplwex, 0.0,
+                     c->d_P, c->npj, 2, prec);
   if (rc) return rc;
   if (slmir_out)
     QX_CHECK(hipMemcpyAsync(slmir_out, c->d_P, sizeof(double) * (size_t)c->npj * nb,
                             hipMemcpyDeviceToDevice, c->stream));
   hipLaunchKernelGGL(k_scale_proj, dim3(cdiv((long)c->npj * nb, 256)), dim3(256), 0, c->stream,
                      c->d_P, c->d_D, c->npj, nb);
-  return gemm_nn(c, c->nplwex, nb, c->npj, 1.0, c->d_B, c->nplwex, c->d_P, c->npj, 1.0, out,
-                 c->nplwex);
+  return gemm_nn_p(c, c->nplwex, nb, c->npj, 1.0, c->d_B, c->nplwex, c->d_P, c->npj, 1.0, out,
+                   c->nplwex, prec);
 }
 
 }  // namespace qx

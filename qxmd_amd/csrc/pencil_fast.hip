@@ -482,12 +482,14 @@ static int local_fast_nn(Ctx* c, const double* psi, double* out, int nb, int acc
   for (int bp0 = 0; bp0 < nbp; bp0 += batch) {
     const int nbb = std::min(batch, nbp - bp0);
     FastArgs a = fast_args(c, nb, bp0, rows);
+    lt_mark(c, 1, false);
     hipLaunchKernelGGL((k_zinv_f<T, NA, NB>), dim3(gz, nbb), dim3(kFastThreads), 0, c->stream, a,
                        psi, t1);
     hipLaunchKernelGGL((k_plane_f<T, T, NA, NB>), dim3(N, nbb), dim3(kFastThreads), lp, c->stream,
                        a, Vp, t1, t2);
     hipLaunchKernelGGL((k_zfwd_f<T, NA, NB>), dim3(gz, nbb), dim3(kFastThreads), 0, c->stream, a,
                        psi, t2, out, accumulate, 1.0 / (double)c->N);
+    lt_mark(c, 1, true);
   }
   QX_CHECK(hipGetLastError());
   return 0;

@@ -228,6 +228,14 @@ integer(c_int) function qx_scf_phase_ms(ctx, ms) bind(C, name="qx_scf_phase_ms")
   real(c_float), intent(out) :: ms(5)
 end function
 
+integer(c_int) function qx_launch_timer(ctx, id, mode, ms, count) bind(C, name="qx_launch_timer")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: id, mode
+  real(c_double), intent(out) :: ms
+  integer(c_int), intent(out) :: count
+end function
+
 integer(c_int) function qx_rfft3(key, fft3x, fft3y, k1, k2, k3, ierror) bind(C, name="qx_rfft3")
   import :: c_int, c_double
   integer(c_int), value :: key, k1, k2, k3

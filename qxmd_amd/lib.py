@@ -64,6 +64,7 @@ SIGNATURES = {
     "qx_event_record": [_vp, _i],
     "qx_event_elapsed": [_vp, _i, _i, C.POINTER(C.c_float)],
     "qx_scf_phase_ms": [_vp, C.POINTER(C.c_float)],
+    "qx_launch_timer": [_vp, _i, _i, C.POINTER(C.c_double), C.POINTER(C.c_int)],
     "qx_rfft3": [_i, _dp, _dp, _i, _i, _i, _pi],
     "qx_fft3": [_i, _dp, _dp, _i, _i, _i, _pi],
     "qx_hcnonloc": [_vp, _dp, _dp],

@@ -181,6 +181,17 @@ class PlaneWave:
         return dict(zip(("energy", "rotation", "cg", "schmidt", "density"),
                         (round(float(x), 3) for x in ms)))
 
+    KB_GEMM, FFT_STAGE = 0, 1
+
+    def timer_start(self, tid):
+        check(load().qx_launch_timer(self.h, tid, 1, None, None), "qx_launch_timer")
+
+    def timer_stop(self, tid):
+        """(summed ms, launches) of launch timer tid since timer_start."""
+        ms, n = C.c_double(0), C.c_int(0)
+        check(load().qx_launch_timer(self.h, tid, 0, C.byref(ms), C.byref(n)), "qx_launch_timer")
+        return float(ms.value), int(n.value)
+
     def record(self, slot):
         check(load().qx_event_record(self.h, slot), "qx_event_record")
 

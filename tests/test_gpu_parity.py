@@ -324,3 +324,33 @@ def test_fast_path_si216_sample(prec):
     ctx.local(0, 1, nb, prec=prec)
     tol = 1e-12 if prec == pw.FP64 else 3e-6
     assert _relerr(ctx.get(1, nb), H.hpsi(psi)) < tol
+
+
+def test_hpsi_mixed_si64(si64):
+    """Mixed form: FP32 pencil FFT and FP32-MFMA KB contractions against the
+    FP64 oracle."""
+    nb = 12
+    gs, ctx, H, v = _setup(si64, nb)
+    psi = initial_wavefunctions(si64, nb)
+    ctx.put(0, psi)
+    ctx.hpsi(0, 1, nb, prec=pw.MIXED)
+    assert _relerr(ctx.get(1, nb), H.hpsi(psi)) < 3e-6  # FP32 FFT + FP32 MFMA
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_scf_band_step_mixed_vs_fp64(si64, batched):
+    """The mixed SCF band step against the FP64 one from the same start:
+    eigenvalues within 1e-6 Ha each, band energy within 1e-6 Ha/atom."""
+    nb = 40
+    gs, ctx, H, v = _setup(si64, nb)
+    psi, _ = O.schmidt(initial_wavefunctions(si64, nb))
+    occ = occupations(nb, 64)
+    ctx.put(0, psi)
+    e64, _, ok64 = ctx.scf_band_step(nb, occ, si64.cell.volume, prec=pw.FP64, batched=batched)
+    ctx.put(0, psi)
+    emx, _, okmx = ctx.scf_band_step(nb, occ, si64.cell.volume, prec=pw.MIXED, batched=batched)
+    assert ok64 and okmx
+    # Ry -> Ha
+    assert np.abs(np.sort(emx) - np.sort(e64)).max() / 2 < 1e-6
+    natom = 64
+    assert abs(np.sum(occ * np.sort(emx)) - np.sum(occ * np.sort(e64))) / 2 / natom < 1e-6
