@@ -62,8 +62,11 @@ __device__ inline double wave_sum(double v) {
 
 // ---- 64 x 64 FP64 MFMA tile: acc += A (64 x K) * B (K x 64) ---------------
 // 256 threads = 4 wavefronts, each a 32 x 32 quadrant as 2 x 2 blocks of
-// v_mfma_f64_16x16x4f64.  fa(m, k) and fb(k, n) return the operands (0 out of
-// range); K is walked 16 at a time through the LDS tiles As/Bs.
+// v_mfma_f64_16x16x4f64.  fa(m, k) and fb(k, n) return the operands, 0 out of
+// range, and must read a valid address for every m, n < 64 and k < K (they
+// load unconditionally from a clamped index and select, so that the loads of
+// a stage are all in flight together); K is walked 16 at a time through the
+// LDS tiles As/Bs.
 typedef double dv4 __attribute__((ext_vector_type(4)));
 constexpr int kTileLd = 66;
 
@@ -84,9 +87,10 @@ struct Tile64 {
     auto fetch = [&](int k0) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int k = k0 + kk + u;
-        ra[u] = k < K ? fa(m, k) : 0.0;
-        rb[u] = k < K ? fb(k, m) : 0.0;
+        const int k = k0 + kk + u, kc = min(k, K - 1);
+        const double va = fa(m, kc), vb = fb(kc, m);
+        ra[u] = k < K ? va : 0.0;
+        rb[u] = k < K ? vb : 0.0;
       }
     };
     fetch(0);

@@ -313,16 +313,25 @@ __global__ void __launch_bounds__(256) k_chol_inv_coop(double* __restrict__ s, i
       if (task < nrb) {
         const int r0 = k0 + kb + task * NBK, nr = min(NBK, n - r0);
         T.run(
-            kb, [&](int m, int k) { return m < nr ? ld_agent(S_(r0 + m, k0 + k)) : 0.0; },
-            [&](int k, int c) { return c < kb ? ld_agent(X_(k0 + c, k0 + k)) : 0.0; }, As, Bs);
+            kb, [&](int m, int k) {
+              const double v = ld_agent(S_(r0 + min(m, nr - 1), k0 + k));
+              return m < nr ? v : 0.0;
+            },
+            [&](int k, int c) {
+              const double v = ld_agent(X_(k0 + min(c, kb - 1), k0 + k));
+              return c < kb ? v : 0.0;
+            }, As, Bs);
         T.each([&](int m, int c, double v) {
           if (m < nr && c < kb) st_agent(S_(r0 + m, k0 + c), v);
         });
       } else {
         const int j0 = (task - nrb) * NBK;  // a full block column left of b
         T.run(
-            kb, [&](int m, int k) { return m < kb ? ld_agent(X_(k0 + m, k0 + k)) : 0.0; },
-            [&](int k, int c) { return k < kb ? ld_agent(X_(k0 + k, j0 + c)) : 0.0; }, As, Bs);
+            kb, [&](int m, int k) {
+              const double v = ld_agent(X_(k0 + min(m, kb - 1), k0 + k));
+              return m < kb ? v : 0.0;
+            },
+            [&](int k, int c) { return ld_agent(X_(k0 + k, j0 + c)); }, As, Bs);
         T.each([&](int m, int c, double v) {
           if (m < kb) st_agent(X_(k0 + m, j0 + c), -v);
         });
@@ -344,8 +353,14 @@ __global__ void __launch_bounds__(256) k_chol_inv_coop(double* __restrict__ s, i
         const int r0 = k0 + kb + ri * NBK, c0 = k0 + kb + ci * NBK;
         const int nr = min(NBK, n - r0), nc = min(NBK, n - c0);
         T.run(
-            kb, [&](int m, int k) { return m < nr ? ld_agent(S_(r0 + m, k0 + k)) : 0.0; },
-            [&](int k, int c) { return c < nc ? ld_agent(S_(c0 + c, k0 + k)) : 0.0; }, As, Bs);
+            kb, [&](int m, int k) {
+              const double v = ld_agent(S_(r0 + min(m, nr - 1), k0 + k));
+              return m < nr ? v : 0.0;
+            },
+            [&](int k, int c) {
+              const double v = ld_agent(S_(c0 + min(c, nc - 1), k0 + k));
+              return c < nc ? v : 0.0;
+            }, As, Bs);
         if (task == 0) {
           // next diagonal block: updated values straight into LDS, then factor
           T.each([&](int m, int c, double v) {
@@ -363,8 +378,14 @@ __global__ void __launch_bounds__(256) k_chol_inv_coop(double* __restrict__ s, i
         const int i0 = k0 + kb + (tt / (b + 1)) * NBK, j0 = (tt % (b + 1)) * NBK;
         const int ni = min(NBK, n - i0), nj = j0 == k0 ? kb : NBK;
         T.run(
-            kb, [&](int m, int k) { return m < ni ? ld_agent(S_(i0 + m, k0 + k)) : 0.0; },
-            [&](int k, int c) { return c < nj ? ld_agent(X_(k0 + k, j0 + c)) : 0.0; }, As, Bs);
+            kb, [&](int m, int k) {
+              const double v = ld_agent(S_(i0 + min(m, ni - 1), k0 + k));
+              return m < ni ? v : 0.0;
+            },
+            [&](int k, int c) {
+              const double v = ld_agent(X_(k0 + k, j0 + min(c, nj - 1)));
+              return c < nj ? v : 0.0;
+            }, As, Bs);
         T.each([&](int m, int c, double v) {
           if (m < ni && c < nj) st_agent(X_(i0 + m, j0 + c), ld_agent(X_(i0 + m, j0 + c)) + v);
         });

@@ -44,34 +44,54 @@ __global__ void __launch_bounds__(256) k_gemm_f64(int M, int N, int K, int kchun
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-  for (int k0 = kb; k0 < ke; k0 += KT) {
-    // ---- stage A tile (64 x 16) ----
+  // operands of a stage: unconditional loads from clamped (valid) addresses,
+  // then a select, so all of them are in flight together; the next stage is
+  // fetched into registers while the current one is on the MFMA pipes
+  double ra[4], rb[4];
+  auto fetch = [&](int k0) {
     if (AKC) {
-      int m = tid >> 2, kk = (tid & 3) * 4;
+      const int m = tid >> 2, kk = (tid & 3) * 4, gm = m0 + m, gmc = min(gm, M - 1);
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        int gk = k0 + kk + u, gm = m0 + m;
-        As[kk + u][m] = (gm < M && gk < ke) ? A[(size_t)gm * lda + gk] : 0.0;
+        const int gk = k0 + kk + u;
+        const double v = A[(size_t)gmc * lda + min(gk, ke - 1)];
+        ra[u] = (gm < M && gk < ke) ? v : 0.0;
       }
     } else {
-      int kk = tid >> 4, m = (tid & 15) * 4;
-      int gk = k0 + kk;
+      const int kk = tid >> 4, m = (tid & 15) * 4, gk = k0 + kk, gkc = min(gk, ke - 1);
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        int gm = m0 + m + u;
-        As[kk][m + u] = (gm < M && gk < ke) ? A[(size_t)gk * lda + gm] : 0.0;
+        const int gm = m0 + m + u;
+        const double v = A[(size_t)gkc * lda + min(gm, M - 1)];
+        ra[u] = (gm < M && gk < ke) ? v : 0.0;
       }
     }
-    // ---- stage B tile (64 x 16), B[n][k] ----
-    {
-      int n = tid >> 2, kk = (tid & 3) * 4;
+    const int n = tid >> 2, kk = (tid & 3) * 4, gn = n0 + n, gnc = min(gn, N - 1);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        int gk = k0 + kk + u, gn = n0 + n;
-        Bs[kk + u][n] = (gn < N && gk < ke) ? B[(size_t)gn * ldb + gk] : 0.0;
-      }
+    for (int u = 0; u < 4; ++u) {
+      const int gk = k0 + kk + u;
+      const double v = B[(size_t)gnc * ldb + min(gk, ke - 1)];
+      rb[u] = (gn < N && gk < ke) ? v : 0.0;
+    }
+  };
+  fetch(kb);
+  for (int k0 = kb; k0 < ke; k0 += KT) {
+    if (AKC) {
+      const int m = tid >> 2, kk = (tid & 3) * 4;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) As[kk + u][m] = ra[u];
+    } else {
+      const int kk = tid >> 4, m = (tid & 15) * 4;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) As[kk][m + u] = ra[u];
+    }
+    {
+      const int n = tid >> 2, kk = (tid & 3) * 4;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) Bs[kk + u][n] = rb[u];
     }
     __syncthreads();
+    if (k0 + KT < ke) fetch(k0 + KT);
 #pragma unroll
     for (int k4 = 0; k4 < KT; k4 += 4) {
       const int kr = k4 + (lane >> 4), c = lane & 15;
@@ -191,25 +211,28 @@ __global__ void __launch_bounds__(256) k_gemm_f32(int M, int N, int K, int kchun
   float ra[8], rb[8];
   auto fetch = [&](int k0) {
     if (AKC) {
-      const int m = tid >> 2, kk = (tid & 3) * 8, gm = m0 + m;
+      const int m = tid >> 2, kk = (tid & 3) * 8, gm = m0 + m, gmc = min(gm, M - 1);
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int gk = k0 + kk + u;
-        ra[u] = (gm < M && gk < ke) ? (float)A[(size_t)gm * lda + gk] : 0.f;
+        const double v = A[(size_t)gmc * lda + min(gk, ke - 1)];
+        ra[u] = (gm < M && gk < ke) ? (float)v : 0.f;
       }
     } else {
-      const int kk = tid >> 3, m = (tid & 7) * 8, gk = k0 + kk;
+      const int kk = tid >> 3, m = (tid & 7) * 8, gk = k0 + kk, gkc = min(gk, ke - 1);
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int gm = m0 + m + u;
-        ra[u] = (gm < M && gk < ke) ? (float)A[(size_t)gk * lda + gm] : 0.f;
+        const double v = A[(size_t)gkc * lda + min(gm, M - 1)];
+        ra[u] = (gm < M && gk < ke) ? (float)v : 0.f;
       }
     }
-    const int n = tid >> 2, kk = (tid & 3) * 8, gn = n0 + n;
+    const int n = tid >> 2, kk = (tid & 3) * 8, gn = n0 + n, gnc = min(gn, N - 1);
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int gk = k0 + kk + u;
-      rb[u] = (gn < N && gk < ke) ? (float)B[(size_t)gn * ldb + gk] : 0.f;
+      const double v = B[(size_t)gnc * ldb + min(gk, ke - 1)];
+      rb[u] = (gn < N && gk < ke) ? (float)v : 0.f;
     }
   };
   fetch(kb);
@@ -315,26 +338,30 @@ __global__ void __launch_bounds__(256) k_gemm_big(int M, int N, int K, int kchun
   constexpr int KRUN = PER, RPT = KT / KRUN;  // threads per row
   auto fetch = [&](int k0) {
     if (AKC) {
-      const int r = tid / RPT, kk = (tid % RPT) * KRUN, gm = m0 + r;
+      const int r = tid / RPT, kk = (tid % RPT) * KRUN, gm = m0 + r, gmc = min(gm, M - 1);
 #pragma unroll
       for (int u = 0; u < PER; ++u) {
         const int gk = k0 + kk + u;
-        ra[u] = (gm < M && gk < ke) ? (T)A[(size_t)gm * lda + gk] : (T)0;
+        const double v = A[(size_t)gmc * lda + min(gk, ke - 1)];
+        ra[u] = (gm < M && gk < ke) ? (T)v : (T)0;
       }
     } else {
       // m-contiguous: thread -> (k row, m run of PER)
       const int kk = tid / (BB / PER), m = (tid % (BB / PER)) * PER, gk = k0 + kk;
+      const int gkc = min(gk, ke - 1);
 #pragma unroll
       for (int u = 0; u < PER; ++u) {
         const int gm = m0 + m + u;
-        ra[u] = (gm < M && gk < ke) ? (T)A[(size_t)gk * lda + gm] : (T)0;
+        const double v = A[(size_t)gkc * lda + min(gm, M - 1)];
+        ra[u] = (gm < M && gk < ke) ? (T)v : (T)0;
       }
     }
-    const int r = tid / RPT, kk = (tid % RPT) * KRUN, gn = n0 + r;
+    const int r = tid / RPT, kk = (tid % RPT) * KRUN, gn = n0 + r, gnc = min(gn, N - 1);
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int gk = k0 + kk + u;
-      rb[u] = (gn < N && gk < ke) ? (T)B[(size_t)gn * ldb + gk] : (T)0;
+      const double v = B[(size_t)gnc * ldb + min(gk, ke - 1)];
+      rb[u] = (gn < N && gk < ke) ? (T)v : (T)0;
     }
   };
   fetch(kb);
@@ -423,6 +450,7 @@ int gemm_tn_p(Ctx* c, int M, int N, int K, double alpha, const double* A, int ld
               int ldb, double beta, double* C, int ldc, int corr, int prec) {
   if (prec != PREC_FP32) return gemm_tn(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, corr);
   if (M <= 0 || N <= 0) return 0;
+
   hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N, ldc,
                      beta);
   lt_mark(c, c->lt_gemm_id, false);
@@ -453,6 +481,7 @@ int gemm_nn_p(Ctx* c, int M, int N, int K, double alpha, const double* A, int ld
               int ldb, double beta, double* C, int ldc, int prec) {
   if (prec != PREC_FP32) return gemm_nn(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
   if (M <= 0 || N <= 0) return 0;
+
   if (beta != 1.0)
     hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N,
                        ldc, beta);
