@@ -21,7 +21,8 @@ OUT = os.path.join(HERE, "libqxmd_amd.so")
 BLD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("QX_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["pencil.hip", "pencil_fast.hip", "gemm.hip", "dense.hip", "eigh.hip", "fullgrid.hip", "cg.hip", "api.hip"]
+HIP_SOURCES = ["pencil.hip", "pencil_fast.hip", "gemm.hip", "blaslt.hip", "dense.hip", "eigh.hip",
+               "fullgrid.hip", "cg.hip", "api.hip"]
 HOST_SOURCES = ["gsphere.cpp"]
 FORTRAN_SOURCES = ["qxmd_amd_c.f90", "qxmd_dropin.f90"]
 FOUT = os.path.join(HERE, "libqxmd_amd_f.so")
@@ -70,7 +71,8 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
             if o.strip():
                 print(o)
     tmp = OUT + ".tmp"
-    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs])
+    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs,
+          "-L/opt/rocm/lib", "-lhipblaslt", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, OUT)
     build_fortran(verbose)
     return OUT

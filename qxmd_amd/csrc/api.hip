@@ -266,6 +266,7 @@ int qx_destroy(qx_ctx* h) {
     if (e) (void)hipEventDestroy(e);
   for (auto& t : c->lt)
     for (auto& e : t.ev) (void)hipEventDestroy(e);
+  lt_destroy(c);
   (void)hipStreamDestroy(c->stream);
   delete h;
   return 0;
@@ -311,6 +312,7 @@ int qx_set_kb(qx_ctx* h, const double* eqlmr, int lmmax, int natom, const int* l
     }
   const int npj = (int)pa.size();
   const size_t n = (size_t)c->nplwex;
+  lt_forget_kb(c);
   if (c->d_B) { (void)hipFree(c->d_B); c->d_B = nullptr; }
   if (c->d_D) { (void)hipFree(c->d_D); c->d_D = nullptr; }
   if (c->d_P) { (void)hipFree(c->d_P); c->d_P = nullptr; }

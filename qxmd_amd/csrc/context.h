@@ -74,6 +74,7 @@ struct Ctx {
     size_t used = 0;
   } lt[kTimers];
   int lt_gemm_id = -1;                    // timer the GEMM launches report to
+  void* blaslt = nullptr;                 // hipBLASLt state (handle, plans, FP32 copies)
 };
 
 // record the begin (end) event of a timed launch of timer id, if enabled
@@ -91,6 +92,12 @@ int local_apply_fast(Ctx* c, const double* psi, double* out, int nb, int prec, i
 int density_fast(Ctx* c, const double* psi, int nb, const double* w, int prec);
 int kb_apply(Ctx* c, const double* psi, double* out, int nb, int accumulate_into_out,
              double* slmir_out, int prec);
+int lt_dgemm(Ctx* c, bool ta, int M, int N, int K, double alpha, const double* A, int lda,
+             const double* B, int ldb, double beta, double* C, int ldc);
+int lt_sgemm_mixed(Ctx* c, bool ta, int M, int N, int K, double alpha, const double* A, int lda,
+                   const double* B, int ldb, double beta, double* C, int ldc);
+void lt_forget_kb(Ctx* c);
+void lt_destroy(Ctx* c);
 int coop_launch(Ctx* c, const void* fn, int want, size_t lds, void** args, bool exact);
 int dense_chol_inv(Ctx* c, double* s, int n, double* x, int* ok, int* flag);
 int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,

@@ -450,6 +450,13 @@ int gemm_tn_p(Ctx* c, int M, int N, int K, double alpha, const double* A, int ld
               int ldb, double beta, double* C, int ldc, int corr, int prec) {
   if (prec != PREC_FP32) return gemm_tn(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, corr);
   if (M <= 0 || N <= 0) return 0;
+  if (use_big(M, N) && lt_sgemm_mixed(c, true, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc) == 0) {
+    if (corr)
+      hipLaunchKernelGGL(k_corr, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N,
+                         ldc, A, lda, B, ldb, alpha, corr == 2 ? 1 : 0);
+    QX_CHECK(hipGetLastError());
+    return 0;
+  }
 
   hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N, ldc,
                      beta);
@@ -481,6 +488,8 @@ int gemm_nn_p(Ctx* c, int M, int N, int K, double alpha, const double* A, int ld
               int ldb, double beta, double* C, int ldc, int prec) {
   if (prec != PREC_FP32) return gemm_nn(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
   if (M <= 0 || N <= 0) return 0;
+  if (use_big(M, N) && lt_sgemm_mixed(c, false, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc) == 0)
+    return 0;
 
   if (beta != 1.0)
     hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N,
@@ -498,15 +507,13 @@ int gemm_nn_p(Ctx* c, int M, int N, int K, double alpha, const double* A, int ld
   return 0;
 }
 
-// FP64 contractions: 128 x 128 tiles for sizable outputs, 64 x 64 otherwise
+// FP64 contractions: hipBLASLt for sizable outputs (64 x 64 MFMA kernels of
+// ours when it cannot take a shape), 64 x 64 kernels for the skinny ones
 int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
             int ldb, double beta, double* C, int ldc, int corr) {
   if (!use_big(M, N)) return gemm_tn_small(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, corr);
-  hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N, ldc,
-                     beta);
-  lt_mark(c, c->lt_gemm_id, false);
-  launch_big_tn<double>(c, M, N, K, alpha, A, lda, B, ldb, C, ldc);
-  lt_mark(c, c->lt_gemm_id, true);
+  if (lt_dgemm(c, true, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc) != 0)
+    return gemm_tn_small(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, corr);
   if (corr)
     hipLaunchKernelGGL(k_corr, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N, ldc,
                        A, lda, B, ldb, alpha, corr == 2 ? 1 : 0);
@@ -517,12 +524,8 @@ int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
 int gemm_nn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
             int ldb, double beta, double* C, int ldc) {
   if (!use_big(M, N)) return gemm_nn_small(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
-  if (beta != 1.0)
-    hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N,
-                       ldc, beta);
-  lt_mark(c, c->lt_gemm_id, false);
-  launch_big_nn<double>(c, M, N, K, alpha, A, lda, B, ldb, C, ldc);
-  lt_mark(c, c->lt_gemm_id, true);
+  if (lt_dgemm(c, false, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc) != 0)
+    return gemm_nn_small(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
   QX_CHECK(hipGetLastError());
   return 0;
 }
