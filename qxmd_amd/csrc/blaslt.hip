@@ -203,10 +203,10 @@ int lt_sgemm_mixed(Ctx* c, bool ta, int M, int N, int K, double alpha, const dou
   if (!lt_enabled()) return -1;
   LtState* s = state(c);
   if (!s) return -1;
-  static const int fast = [] {
-    const char* e = std::getenv("QX_MIXED_GEMM");
-    return e && std::strcmp(e, "xf32") == 0 ? 1 : 0;
-  }();
+  // QX_MIXED_GEMM=f32 keeps plain FP32 products; the default is the
+  // library's fast FP32 mode (split products on the matrix cores)
+  const char* mode_env = std::getenv("QX_MIXED_GEMM");
+  const int fast = mode_env && std::strcmp(mode_env, "f32") == 0 ? 0 : 1;
   const int arows = ta ? K : M, acols = ta ? M : K;
   const float* A32;
   int lda32 = arows;

@@ -354,3 +354,25 @@ def test_scf_band_step_mixed_vs_fp64(si64, batched):
     assert np.abs(np.sort(emx) - np.sort(e64)).max() / 2 < 1e-6
     natom = 64
     assert abs(np.sum(occ * np.sort(emx)) - np.sum(occ * np.sort(e64))) / 2 / natom < 1e-6
+
+
+@pytest.mark.parametrize("mode", ["f32", "xf32"])
+def test_kb_mixed_gemm_accuracy(si64, mode, monkeypatch):
+    """The mixed KB contractions (FP32 copies on hipBLASLt, plain FP32 or the
+    fast split-product mode) against the FP64 oracle: the KB part of H|psi>
+    alone.  Tolerances: FP32 products 5e-6; split products (about 16
+    significant bits per factor) 3e-5."""
+    monkeypatch.setenv("QX_MIXED_GEMM", mode)
+    nb = 256  # large enough for the library path
+    gs, ctx, H, v = _setup(si64, nb)
+    psi = initial_wavefunctions(si64, nb)
+    ctx.put(0, psi)
+    ctx.hpsi(0, 1, nb, prec=pw.MIXED)
+    got = ctx.get(1, nb)
+    ctx.local(0, 1, nb, prec=pw.MIXED)
+    kb_got = got - ctx.get(1, nb)
+    eq, lmx, vcl, cl = model_kb_projectors(si64, si64.cell)
+    kb_ref, _ = O.hckbpp_g(eq, lmx, vcl, psi, np.zeros_like(psi))
+    err = _relerr(kb_got, kb_ref)
+    print(mode, "KB rel err", err)
+    assert err < (5e-6 if mode == "f32" else 3e-5)
