@@ -77,16 +77,30 @@ def cpu_worker(ncell, budget_s):
     print(json.dumps({"bands": n, "seconds": dt}))
 
 
+def measured_traffic(key):
+    """HBM bytes per launch measured by the calibrated rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs summarized in profiles/ (None when not measured)."""
+    import glob
+    for path in glob.glob(os.path.join(ROOT, "profiles", "pmc_*.md")):
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == key:
+                    return float(parts[2])
+    return None
+
+
 def copy_probe(nbytes=1 << 30, reps=3):
-    """A plain device copy of known size: the HBM byte counters (FETCH_SIZE,
-    WRITE_SIZE) of this kernel calibrate those of the path's kernels."""
+    """Streaming kernels of known size whose HBM byte counters (FETCH_SIZE,
+    WRITE_SIZE) calibrate those of the path's kernels: a fill (writes nbytes)
+    and an elementwise scale y = 2x (reads nbytes, writes nbytes)."""
     import torch
     x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
     for _ in range(reps):
-        y.copy_(x)
+        torch.mul(x, 2.0, out=y)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes_each_way": nbytes, "reps": reps}))
+    print(json.dumps({"copy_probe_bytes_each_way": nbytes, "reps": reps}), file=sys.stderr)
 
 
 def cpu_baseline(ncell, threads, budget_s=12.0):
@@ -113,14 +127,14 @@ def main():
     ap.add_argument("--cpu-worker", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--copy-probe", action="store_true",
-                    help="only run a plain 1 GiB device copy (byte-counter calibration)")
+                    help="first run a plain 1 GiB device copy (calibrates the HBM byte counters "
+                         "of a rocprofv3 --pmc run over the same process)")
     args = ap.parse_args()
     if args.cpu_worker:
         cpu_worker(args.ncell, args.cpu_budget)
         return
     if args.copy_probe:
         copy_probe()
-        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs the single-GPU path only (one rank, one MI355X); "
               "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
@@ -192,6 +206,8 @@ def main():
                 "launch_ms": round(fft_avg, 4), "launches": fft_n,
                 "algorithmic_per_launch": fft_launch_bytes}
     fft_roof["frac"] = round(fft_roof["achieved"] / HBM_PEAK_GBS, 4)
+    if natom == 216 and round(fft_bands_per_launch) == 540:
+        fft_roof["traffic"] = measured_traffic("fft_stage_fp32_si216_540")
     if fft_ms >= kb_ms:
         roof, other = fft_roof, kb_roof
     else:
@@ -263,6 +279,7 @@ def main():
         **extra,
     }
     print(json.dumps(line))
+    ctx.close()
 
 
 if __name__ == "__main__":
