@@ -27,6 +27,7 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 FP32_PEAK_TFS = 157.3
+BF16_PEAK_TFS = 2500.0
 
 
 def build_inputs(ncell, nband):
@@ -194,12 +195,19 @@ def main():
     fft_launch_bytes = fft_stage_bytes(gs, 4) * fft_bands_per_launch
     kb_avg = kb_ms / max(kb_n, 1)
     fft_avg = fft_ms / max(fft_n, 1)
+    # FP32 KB products run on hipBLASLt; its default FAST_TF32 compute type splits
+    # the FP32 operands over the bf16 MFMA pipe (the only way past the 157.3 TF/s
+    # FP32 MFMA peak), so that pipe's dense peak is the ceiling quoted.
+    split = os.environ.get("QX_MIXED_GEMM", "") != "f32"
+    kb_peak = BF16_PEAK_TFS if split else FP32_PEAK_TFS
     kb_roof = {"bound": "mfma", "achieved": round(kb_launch_flops / (kb_avg * 1e-3) / 1e12, 3),
-               "peak": FP32_PEAK_TFS, "unit": "TFLOP/s", "traffic": None,
-               "kernel": "k_gemm_f32 (KB projection / back-projection, FP32 MFMA)",
+               "peak": kb_peak, "unit": "TFLOP/s", "traffic": None,
+               "kernel": ("hipBLASLt FP32 GEMM, FAST_TF32 split products on bf16 MFMA"
+                          if split else "hipBLASLt FP32 GEMM on FP32 MFMA")
+               + " (KB projection / back-projection of a band batch)",
                "launch_ms": round(kb_avg, 4), "launches": kb_n,
                "algorithmic_per_launch": kb_launch_flops}
-    kb_roof["frac"] = round(kb_roof["achieved"] / FP32_PEAK_TFS, 4)
+    kb_roof["frac"] = round(kb_roof["achieved"] / kb_peak, 4)
     fft_roof = {"bound": "hbm", "achieved": round(fft_launch_bytes / (fft_avg * 1e-3) / 1e9, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
                 "kernel": "k_zinv_f + k_plane_f + k_zfwd_f (FP32 pencil FFT stage of a band batch)",
@@ -208,6 +216,13 @@ def main():
     fft_roof["frac"] = round(fft_roof["achieved"] / HBM_PEAK_GBS, 4)
     if natom == 216 and round(fft_bands_per_launch) == 540:
         fft_roof["traffic"] = measured_traffic("fft_stage_fp32_si216_540")
+    fft_roof["note"] = ("algorithmic bytes model two full-grid HBM passes per 3-D transform; "
+                        "the sphere-pruned pencil kernels keep xy planes in LDS and move far "
+                        "less, so achieved/peak can exceed 1; traffic_gbs is the measured "
+                        "HBM rate")
+    if fft_roof["traffic"]:
+        fft_roof["traffic_gbs"] = round(fft_roof["traffic"] / (fft_avg * 1e-3) / 1e9, 1)
+        fft_roof["traffic_frac"] = round(fft_roof["traffic_gbs"] / HBM_PEAK_GBS, 4)
     if fft_ms >= kb_ms:
         roof, other = fft_roof, kb_roof
     else:
@@ -263,7 +278,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 FFT + fp32 MFMA KB (mixed); fp64 dense",
+        "dtype": "fp32 FFT + fp32 KB (hipBLASLt FAST_TF32 split products) (mixed); fp64 dense",
         "data": "synthetic",
         "config": {"workload": f"Si{natom} Gamma H|psi>, {nband} bands, grid {gs.nd[0]}^3, "
                                f"nplw {gs.nplw}, {ctx.npj} KB projectors",

@@ -64,7 +64,9 @@ int qx_slot_get(qx_ctx *ctx, int slot, double *gdcr, int nb);
 int qx_slot_copy(qx_ctx *ctx, int src, int dst, int nb);
 
 /* dst = H src for nb bands: kinetic |G|^2 + local (two pruned FFTs) + KB;
- * prec QX_FP64 or QX_MIXED (FP32 FFT stage).  Replaces ekinetic2 +
+ * prec QX_FP64, or QX_MIXED: FP32 FFT stage and FP32 KB products (split
+ * bf16 products by default, plain FP32 with QX_MIXED_GEMM=f32), FP64
+ * accumulation of the result.  Replaces ekinetic2 +
  * hcnonloc + hckbpp_g (Sources/engrad.f90:66-127, 411-606;
  * Sources/nlkbpp.f90:1284-1393). */
 int qx_hpsi_slot(qx_ctx *ctx, int src, int dst, int nb, int prec);
