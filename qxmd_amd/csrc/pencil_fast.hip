@@ -182,60 +182,114 @@ __device__ inline void x_stepAp(cplx<T>* A, const cplx<T>* R, int ys, int b, int
   }
 }
 
-template <typename T, typename VT, int NA, int NB>
-__global__ void __launch_bounds__(kFastThreads) k_plane_f(FastArgs a, const VT* __restrict__ Vp,
-                                                          const cplx<T>* __restrict__ t1,
-                                                          cplx<T>* __restrict__ t2) {
+// LDS accesses of one wavefront are performed in issue order, so lanes of a
+// wave can exchange data through LDS without a workgroup barrier; the fences
+// only keep the compiler from moving LDS accesses across this point.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kPlaneMaxThreads = 512;
+constexpr int kPlaneLoads = 4;  // sphere columns held in registers per thread
+
+// x rows a wavefront transforms at once: step A runs NB lanes per row, step B
+// NA lanes per row
+__host__ __device__ constexpr int plane_rows_per_wave(int NA, int NB) {
+  return 64 / (NA > NB ? NA : NB);
+}
+
+// One z plane of one band pair: y inverse over the nH sphere rows, then for
+// every y slot the x line (inverse, V(r) multiply, forward) and y forward.
+// The x lines are owned by wavefronts (a wave takes plane_rows_per_wave rows
+// at a time through its own LDS strip), so the x pass needs no workgroup
+// barrier; V(r) of a wave's next rows is loaded while the current rows are
+// transformed.
+template <typename T, int NA, int NB>
+__global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const T* __restrict__ Vp,
+                                                            const cplx<T>* __restrict__ t1,
+                                                            cplx<T>* __restrict__ t2) {
   constexpr int N = NA * NB;
+  constexpr int RW = plane_rows_per_wave(NA, NB);
+  constexpr int NG = (N + RW - 1) / RW;
   extern __shared__ unsigned char smem_raw[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* A = tw + N;           // [nH][N]
-  cplx<T>* R = A + a.nH * N;     // [rows][N]
+  cplx<T>* A = tw + N;                          // [nH][N], y in slot order
+  cplx<T>* Rw = A + a.nH * N + wave * RW * N;   // this wave's [RW][N] strip
   const int zs = blockIdx.x, bp = blockIdx.y;
+
+  // lane roles of the x pass
+  const int rA = lane / NB, bA = lane % NB, rB = lane / NA, kaB = lane % NA;
+  const bool actA = lane < RW * NB, actB = lane < RW * NA;
+  // V(r) row pieces of step B; rows past the grid and idle lanes load a valid
+  // row that is not used
+  const T* Vz = Vp + (size_t)zs * N * N + NB * kaB;
+  T vnext[NB];
+  {
+    const T* row = Vz + (size_t)min(wave * RW + rB, N - 1) * N;
+#pragma unroll
+    for (int kb = 0; kb < NB; ++kb) vnext[kb] = row[kb];
+  }
+
+  // sphere columns of this plane: loads issued before the LDS fill
+  const cplx<T>* in = t1 + ((size_t)bp * N + zs) * a.ncol;
+  cplx<T> cv[kPlaneLoads];
+  int ci[kPlaneLoads];
+#pragma unroll
+  for (int j = 0; j < kPlaneLoads; ++j) {
+    const int cl = threadIdx.x + j * blockDim.x;
+    const int cc = cl < a.ncol ? cl : 0;
+    cv[j] = in[cc];
+    ci[j] = cl < a.ncol ? a.col_hi[cc] * N + a.col_kmod[cc] : -1;
+  }
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = mk<T>((T)a.tw[2 * k], (T)a.tw[2 * k + 1]);
   for (int i = threadIdx.x; i < a.nH * N; i += blockDim.x) A[i] = mk<T>(0, 0);
   __syncthreads();
-  const cplx<T>* in = t1 + ((size_t)bp * N + zs) * a.ncol;
-  for (int cl = threadIdx.x; cl < a.ncol; cl += blockDim.x)
+#pragma unroll
+  for (int j = 0; j < kPlaneLoads; ++j)
+    if (ci[j] >= 0) A[ci[j]] = cv[j];
+  for (int cl = threadIdx.x + kPlaneLoads * blockDim.x; cl < a.ncol; cl += blockDim.x)
     A[a.col_hi[cl] * N + a.col_kmod[cl]] = in[cl];
   __syncthreads();
   plane_y_inverse<T, NA, NB>(A, a.nH, tw);
-  const VT* Vz = Vp + (size_t)zs * N * N;
-  for (int y0 = 0; y0 < N; y0 += a.rows) {
-    const int rows = min(a.rows, N - y0);
-    for (int it = threadIdx.x; it < rows * NB; it += blockDim.x) {
-      int r = it / NB;
-      x_stepA<T, NA, NB>(A, R + r * N, y0 + r, it % NB, a.hmax, tw);
+
+  for (int g = wave; g < NG; g += nwave) {
+    const int y0 = g * RW;
+    T vcur[NB];
+#pragma unroll
+    for (int kb = 0; kb < NB; ++kb) vcur[kb] = vnext[kb];
+    if (g + nwave < NG) {
+      const T* row = Vz + (size_t)min((g + nwave) * RW + rB, N - 1) * N;
+#pragma unroll
+      for (int kb = 0; kb < NB; ++kb) vnext[kb] = row[kb];
     }
-    __syncthreads();
-    for (int it = threadIdx.x; it < rows * NA; it += blockDim.x) {
-      const int r = it / NA, ka = it % NA;
-      cplx<T>* L = R + r * N + NB * ka;
-      const VT* vrow = Vz + (size_t)(y0 + r) * N + NB * ka;
+    if (actA && y0 + rA < N) x_stepA<T, NA, NB>(A, Rw + rA * N, y0 + rA, bA, a.hmax, tw);
+    wave_lds_sync();
+    if (actB && y0 + rB < N) {
+      cplx<T>* L = Rw + rB * N + NB * kaB;
       cplx<T> v[NB];
 #pragma unroll
       for (int b = 0; b < NB; ++b) v[b] = L[b];
       Dft<T, NB>::run(v, +1);
 #pragma unroll
       for (int kb = 0; kb < NB; ++kb) {
-        T s = (T)vrow[kb];
-        v[kb].x *= s;
-        v[kb].y *= s;
+        v[kb].x *= vcur[kb];
+        v[kb].y *= vcur[kb];
       }
       Dft<T, NB>::run(v, -1);
 #pragma unroll
       for (int b = 1; b < NB; ++b)
-        if (ka) v[b] = cmul(v[b], twid(tw, b * ka, -1));
+        if (kaB) v[b] = cmul(v[b], twid(tw, b * kaB, -1));
 #pragma unroll
       for (int b = 0; b < NB; ++b) L[b] = v[b];
     }
-    __syncthreads();
-    for (int it = threadIdx.x; it < rows * NB; it += blockDim.x) {
-      int r = it / NB;
-      x_stepAp<T, NA, NB>(A, R + r * N, y0 + r, it % NB, a.hmax);
-    }
-    __syncthreads();
+    wave_lds_sync();
+    if (actA && y0 + rA < N) x_stepAp<T, NA, NB>(A, Rw + rA * N, y0 + rA, bA, a.hmax);
+    wave_lds_sync();
   }
+  __syncthreads();
   for (int it = threadIdx.x; it < a.nH * NA; it += blockDim.x)
     pn_stepB<T, NA, NB>(A + (it / NA) * N, it % NA, tw, -1);
   __syncthreads();
@@ -452,8 +506,8 @@ static size_t fast_plane_lds(int N, int nH, int rows) {
 
 template <typename T>
 static int fast_rows(int N, int nH) {
-  // LDS budget of the plane kernel: the sphere tile A is fixed, the rows of
-  // the x pass are chunked to fit (two blocks per CU in FP32)
+  // LDS budget of the density plane kernel: the sphere tile A is fixed, the
+  // rows of the x pass are chunked to fit (two blocks per CU in FP32)
   size_t budget = sizeof(T) == 4 ? 76 * 1024 : 120 * 1024;
   if (const char* e = std::getenv("QX_FAST_LDS_KB")) budget = (size_t)std::atoi(e) * 1024;
   int rows = N;
@@ -462,31 +516,47 @@ static int fast_rows(int N, int nH) {
   return rows;
 }
 
+// LDS of the band-apply plane kernel with nwave wavefronts
+template <typename T>
+static size_t plane_lds(int N, int nH, int rw, int nwave) {
+  return sizeof(cplx<T>) * ((size_t)N + (size_t)nH * N + (size_t)nwave * rw * N);
+}
+
+// wavefronts per plane block: 8 unless the LDS strips would not fit
+template <typename T>
+static int plane_waves(int N, int nH, int rw) {
+  int w = kPlaneMaxThreads / 64;
+  if (const char* e = std::getenv("QX_PLANE_WAVES")) w = std::max(1, std::min(w, std::atoi(e)));
+  while (w > 1 && plane_lds<T>(N, nH, rw, w) > 160 * 1024) w /= 2;
+  return plane_lds<T>(N, nH, rw, w) <= 160 * 1024 ? w : -1;
+}
+
 template <typename T, int NA, int NB>
 static int local_fast_nn(Ctx* c, const double* psi, double* out, int nb, int accumulate) {
   constexpr int N = NA * NB;
+  constexpr int RW = plane_rows_per_wave(NA, NB);
   const int nbp = (nb + 1) / 2;
-  const int rows = fast_rows<T>(N, c->pc.nH);
-  if (rows < 1) return -3;
+  const int nwave = plane_waves<T>(N, c->pc.nH, RW);
+  if (nwave < 1) return -3;
   const size_t per_bp = sizeof(cplx<T>) * (size_t)N * c->pc.ncol;
   const int batch = (int)std::max<size_t>(1, std::min<size_t>(c->t_bytes / per_bp, (size_t)nbp));
   constexpr int PB = zpairs(N, (int)sizeof(cplx<T>));
   const unsigned gz = (unsigned)((c->pc.npair + PB - 1) / PB);
-  const size_t lp = fast_plane_lds<T>(N, c->pc.nH, rows);
+  const size_t lp = plane_lds<T>(N, c->pc.nH, RW, nwave);
   if (lp > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_plane_f<T, T, NA, NB>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_plane_f<T, NA, NB>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lp);
   cplx<T>* t1 = reinterpret_cast<cplx<T>*>(c->d_t1);
   cplx<T>* t2 = reinterpret_cast<cplx<T>*>(c->d_t2);
   const T* Vp = sizeof(T) == 8 ? reinterpret_cast<const T*>(c->d_Vp) : reinterpret_cast<const T*>(c->d_Vpf);
   for (int bp0 = 0; bp0 < nbp; bp0 += batch) {
     const int nbb = std::min(batch, nbp - bp0);
-    FastArgs a = fast_args(c, nb, bp0, rows);
+    FastArgs a = fast_args(c, nb, bp0, 0);
     lt_mark(c, 1, false);
     hipLaunchKernelGGL((k_zinv_f<T, NA, NB>), dim3(gz, nbb), dim3(kFastThreads), 0, c->stream, a,
                        psi, t1);
-    hipLaunchKernelGGL((k_plane_f<T, T, NA, NB>), dim3(N, nbb), dim3(kFastThreads), lp, c->stream,
-                       a, Vp, t1, t2);
+    hipLaunchKernelGGL((k_plane_f<T, NA, NB>), dim3(N, nbb), dim3(64 * nwave), lp, c->stream, a,
+                       Vp, t1, t2);
     hipLaunchKernelGGL((k_zfwd_f<T, NA, NB>), dim3(gz, nbb), dim3(kFastThreads), 0, c->stream, a,
                        psi, t2, out, accumulate, 1.0 / (double)c->N);
     lt_mark(c, 1, true);
