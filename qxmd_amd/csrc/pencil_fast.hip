@@ -14,7 +14,10 @@
 // V(r) is stored pre-permuted.  In x the inverse step B, the V(r) multiply and
 // the forward step B' act on the same registers.
 #include <algorithm>
+#include <array>
 #include <cstdlib>
+#include <map>
+#include <mutex>
 #include "context.h"
 
 namespace qx {
@@ -30,6 +33,7 @@ __host__ __device__ inline int slot_phys(int s, int NA, int NB) { return s / NB 
 
 struct FastArgs {
   int N, nH, hmax, npair, ncol, nplwex, nb, bp0, rows;
+  int sa, sr;  // row strides (elements) of the plane kernel's sphere tile and x strips
   const int *pt_col, *pt_z, *pair_p0, *col_hi, *col_kmod;
   const double* kin;
   const double* tw;  // exp(+2 pi i k / N), k < N
@@ -148,7 +152,7 @@ __device__ inline void plane_y_inverse(cplx<T>* A, int nH, const cplx<T>* tw) {
 
 // x step A for row slot ys: gather the natural-x inputs (|h| <= hmax) from A
 template <typename T, int NA, int NB>
-__device__ inline void x_stepA(const cplx<T>* A, cplx<T>* R, int ys, int b, int hmax,
+__device__ inline void x_stepA(const cplx<T>* A, int sa, cplx<T>* R, int ys, int b, int hmax,
                                const cplx<T>* tw) {
   constexpr int N = NA * NB;
   cplx<T> v[NA];
@@ -156,7 +160,7 @@ __device__ inline void x_stepA(const cplx<T>* A, cplx<T>* R, int ys, int b, int 
   for (int a = 0; a < NA; ++a) {
     int n = b + NB * a;
     int h = n <= hmax ? n : (n >= N - hmax ? n - N : N);
-    v[a] = h < N ? A[(h + hmax) * N + ys] : mk<T>(0, 0);
+    v[a] = h < N ? A[(h + hmax) * sa + ys] : mk<T>(0, 0);
   }
   Dft<T, NA>::run(v, +1);
 #pragma unroll
@@ -168,7 +172,7 @@ __device__ inline void x_stepA(const cplx<T>* A, cplx<T>* R, int ys, int b, int 
 
 // x step A': back to natural x, keep |h| <= hmax into A
 template <typename T, int NA, int NB>
-__device__ inline void x_stepAp(cplx<T>* A, const cplx<T>* R, int ys, int b, int hmax) {
+__device__ inline void x_stepAp(cplx<T>* A, int sa, const cplx<T>* R, int ys, int b, int hmax) {
   constexpr int N = NA * NB;
   cplx<T> v[NA];
 #pragma unroll
@@ -178,7 +182,7 @@ __device__ inline void x_stepAp(cplx<T>* A, const cplx<T>* R, int ys, int b, int
   for (int a = 0; a < NA; ++a) {
     int n = b + NB * a;
     int h = n <= hmax ? n : (n >= N - hmax ? n - N : N);
-    if (h < N) A[(h + hmax) * N + ys] = v[a];
+    if (h < N) A[(h + hmax) * sa + ys] = v[a];
   }
 }
 
@@ -205,7 +209,9 @@ __host__ __device__ constexpr int plane_rows_per_wave(int NA, int NB) {
 // The x lines are owned by wavefronts (a wave takes plane_rows_per_wave rows
 // at a time through its own LDS strip), so the x pass needs no workgroup
 // barrier; V(r) of a wave's next rows is loaded while the current rows are
-// transformed.
+// transformed.  LDS bank conflicts are kept down by the row strides a.sa (sphere
+// tile) and a.sr (x strips), chosen on the host, and by lane maps that put the
+// line (y pass) or row (x pass) index fastest across lanes.
 template <typename T, int NA, int NB>
 __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const T* __restrict__ Vp,
                                                             const cplx<T>* __restrict__ t1,
@@ -215,20 +221,21 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
   constexpr int NG = (N + RW - 1) / RW;
   extern __shared__ unsigned char smem_raw[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int sa = a.sa, sr = a.sr, nH = a.nH;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* A = tw + N;                          // [nH][N], y in slot order
-  cplx<T>* Rw = A + a.nH * N + wave * RW * N;   // this wave's [RW][N] strip
+  cplx<T>* A = tw + N;                           // [nH][sa], y in slot order
+  cplx<T>* Rw = A + nH * sa + wave * RW * sr;    // this wave's [RW][sr] strip
   const int zs = blockIdx.x, bp = blockIdx.y;
 
-  // lane roles of the x pass
-  const int rA = lane / NB, bA = lane % NB, rB = lane / NA, kaB = lane % NA;
-  const bool actA = lane < RW * NB, actB = lane < RW * NA;
+  // lane roles of the x pass (row fastest)
+  const int rx = lane % RW, jx = lane / RW;
+  const bool actA = jx < NB, actB = jx < NA;
   // V(r) row pieces of step B; rows past the grid and idle lanes load a valid
   // row that is not used
-  const T* Vz = Vp + (size_t)zs * N * N + NB * kaB;
+  const T* Vz = Vp + (size_t)zs * N * N + NB * min(jx, NA - 1);
   T vnext[NB];
   {
-    const T* row = Vz + (size_t)min(wave * RW + rB, N - 1) * N;
+    const T* row = Vz + (size_t)min(wave * RW + rx, N - 1) * N;
 #pragma unroll
     for (int kb = 0; kb < NB; ++kb) vnext[kb] = row[kb];
   }
@@ -242,33 +249,42 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
     const int cl = threadIdx.x + j * blockDim.x;
     const int cc = cl < a.ncol ? cl : 0;
     cv[j] = in[cc];
-    ci[j] = cl < a.ncol ? a.col_hi[cc] * N + a.col_kmod[cc] : -1;
+    ci[j] = cl < a.ncol ? a.col_hi[cc] * sa + a.col_kmod[cc] : -1;
   }
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = mk<T>((T)a.tw[2 * k], (T)a.tw[2 * k + 1]);
-  for (int i = threadIdx.x; i < a.nH * N; i += blockDim.x) A[i] = mk<T>(0, 0);
+  for (int i = threadIdx.x; i < nH * sa; i += blockDim.x) A[i] = mk<T>(0, 0);
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < kPlaneLoads; ++j)
     if (ci[j] >= 0) A[ci[j]] = cv[j];
   for (int cl = threadIdx.x + kPlaneLoads * blockDim.x; cl < a.ncol; cl += blockDim.x)
-    A[a.col_hi[cl] * N + a.col_kmod[cl]] = in[cl];
+    A[a.col_hi[cl] * sa + a.col_kmod[cl]] = in[cl];
   __syncthreads();
-  plane_y_inverse<T, NA, NB>(A, a.nH, tw);
+
+  // y inverse (line fastest across lanes)
+  for (int it = threadIdx.x; it < nH * NB; it += blockDim.x)
+    np_stepA<T, NA, NB>(A + (it % nH) * sa, it / nH, tw, +1);
+  __syncthreads();
+  for (int it = threadIdx.x; it < nH * NA; it += blockDim.x)
+    np_stepB<T, NA, NB>(A + (it % nH) * sa, it / nH, +1);
+  __syncthreads();
 
   for (int g = wave; g < NG; g += nwave) {
     const int y0 = g * RW;
+    const bool row_ok = y0 + rx < N;
     T vcur[NB];
 #pragma unroll
     for (int kb = 0; kb < NB; ++kb) vcur[kb] = vnext[kb];
     if (g + nwave < NG) {
-      const T* row = Vz + (size_t)min((g + nwave) * RW + rB, N - 1) * N;
+      const T* row = Vz + (size_t)min((g + nwave) * RW + rx, N - 1) * N;
 #pragma unroll
       for (int kb = 0; kb < NB; ++kb) vnext[kb] = row[kb];
     }
-    if (actA && y0 + rA < N) x_stepA<T, NA, NB>(A, Rw + rA * N, y0 + rA, bA, a.hmax, tw);
+    cplx<T>* Rr = Rw + rx * sr;
+    if (actA && row_ok) x_stepA<T, NA, NB>(A, sa, Rr, y0 + rx, jx, a.hmax, tw);
     wave_lds_sync();
-    if (actB && y0 + rB < N) {
-      cplx<T>* L = Rw + rB * N + NB * kaB;
+    if (actB && row_ok) {
+      cplx<T>* L = Rr + NB * jx;
       cplx<T> v[NB];
 #pragma unroll
       for (int b = 0; b < NB; ++b) v[b] = L[b];
@@ -281,24 +297,25 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
       Dft<T, NB>::run(v, -1);
 #pragma unroll
       for (int b = 1; b < NB; ++b)
-        if (kaB) v[b] = cmul(v[b], twid(tw, b * kaB, -1));
+        if (jx) v[b] = cmul(v[b], twid(tw, b * jx, -1));
 #pragma unroll
       for (int b = 0; b < NB; ++b) L[b] = v[b];
     }
     wave_lds_sync();
-    if (actA && y0 + rA < N) x_stepAp<T, NA, NB>(A, Rw + rA * N, y0 + rA, bA, a.hmax);
+    if (actA && row_ok) x_stepAp<T, NA, NB>(A, sa, Rr, y0 + rx, jx, a.hmax);
     wave_lds_sync();
   }
   __syncthreads();
-  for (int it = threadIdx.x; it < a.nH * NA; it += blockDim.x)
-    pn_stepB<T, NA, NB>(A + (it / NA) * N, it % NA, tw, -1);
+  // y forward (line fastest across lanes)
+  for (int it = threadIdx.x; it < nH * NA; it += blockDim.x)
+    pn_stepB<T, NA, NB>(A + (it % nH) * sa, it / nH, tw, -1);
   __syncthreads();
-  for (int it = threadIdx.x; it < a.nH * NB; it += blockDim.x)
-    pn_stepA<T, NA, NB>(A + (it / NB) * N, it % NB, -1);
+  for (int it = threadIdx.x; it < nH * NB; it += blockDim.x)
+    pn_stepA<T, NA, NB>(A + (it % nH) * sa, it / nH, -1);
   __syncthreads();
   cplx<T>* out = t2 + ((size_t)bp * N + zs) * a.ncol;
   for (int cl = threadIdx.x; cl < a.ncol; cl += blockDim.x)
-    out[cl] = A[a.col_hi[cl] * N + a.col_kmod[cl]];
+    out[cl] = A[a.col_hi[cl] * sa + a.col_kmod[cl]];
 }
 
 // ---- z forward: T2 -> split bands, x1/N, + |G|^2 psi --------------------
@@ -388,7 +405,7 @@ __global__ void __launch_bounds__(kFastThreads) k_plane_rho_f(FastArgs a,
       const int rows = min(a.rows, N - y0);
       for (int it = threadIdx.x; it < rows * NB; it += blockDim.x) {
         int r = it / NB;
-        x_stepA<T, NA, NB>(A, R + r * N, y0 + r, it % NB, a.hmax, tw);
+        x_stepA<T, NA, NB>(A, N, R + r * N, y0 + r, it % NB, a.hmax, tw);
       }
       __syncthreads();
       for (int it = threadIdx.x; it < rows * NA; it += blockDim.x) {
@@ -516,19 +533,133 @@ static int fast_rows(int N, int nH) {
   return rows;
 }
 
-// LDS of the band-apply plane kernel with nwave wavefronts
+// LDS of the band-apply plane kernel with nwave wavefronts and row strides sa, sr
 template <typename T>
-static size_t plane_lds(int N, int nH, int rw, int nwave) {
-  return sizeof(cplx<T>) * ((size_t)N + (size_t)nH * N + (size_t)nwave * rw * N);
+static size_t plane_lds(int N, int nH, int sa, int sr, int rw, int nwave) {
+  return sizeof(cplx<T>) * ((size_t)N + (size_t)nH * sa + (size_t)nwave * rw * sr);
 }
+
+constexpr size_t kCuLds = 160 * 1024;
 
 // wavefronts per plane block: 8 unless the LDS strips would not fit
 template <typename T>
 static int plane_waves(int N, int nH, int rw) {
   int w = kPlaneMaxThreads / 64;
   if (const char* e = std::getenv("QX_PLANE_WAVES")) w = std::max(1, std::min(w, std::atoi(e)));
-  while (w > 1 && plane_lds<T>(N, nH, rw, w) > 160 * 1024) w /= 2;
-  return plane_lds<T>(N, nH, rw, w) <= 160 * 1024 ? w : -1;
+  while (w > 1 && plane_lds<T>(N, nH, N, N, rw, w) > kCuLds) w /= 2;
+  return plane_lds<T>(N, nH, N, N, rw, w) <= kCuLds ? w : -1;
+}
+
+// Passes one wavefront LDS access takes when each lane moves el dwords: 64
+// banks of 4 bytes serve 64/el lanes per pass, and a pass lasts as many
+// cycles as the most distinct addresses one bank is asked for (lanes reading
+// the same address share it).  e[l] < 0 marks an idle lane.
+static int lds_passes(const int* e, int el) {
+  int tot = 0;
+  const int lpp = 64 / el;
+  for (int g = 0; g < 64; g += lpp) {
+    int d[64], nd = 0;
+    for (int l = g; l < g + lpp; ++l)
+      if (e[l] >= 0)
+        for (int k = 0; k < el; ++k) d[nd++] = el * e[l] + k;
+    if (!nd) continue;
+    std::sort(d, d + nd);
+    nd = (int)(std::unique(d, d + nd) - d);
+    int cnt[64] = {0}, mx = 0;
+    for (int i = 0; i < nd; ++i) mx = std::max(mx, ++cnt[d[i] & 63]);
+    tot += mx;
+  }
+  return tot;
+}
+
+// modelled LDS passes of the accesses that depend on the sphere tile stride
+// sa (y passes and the x pass's column reads/writes) ...
+static long plane_cost_sa(int NA, int NB, int nH, int sa, int el) {
+  const int N = NA * NB, hmax = (nH - 1) / 2, RW = plane_rows_per_wave(NA, NB);
+  long c = 0;
+  int e[64];
+  for (int step = 0; step < 2; ++step) {
+    const int K = step == 0 ? NB : NA, Q = step == 0 ? NA : NB, n = nH * K;
+    for (int q = 0; q < Q; ++q)
+      for (int w0 = 0; w0 < n; w0 += 64) {
+        for (int l = 0; l < 64; ++l) {
+          const int it = w0 + l, line = it % nH, j = it / nH;
+          e[l] = it < n ? line * sa + (step == 0 ? j + NB * q : NB * j + q) : -1;
+        }
+        c += 4 * lds_passes(e, el);  // inverse and forward, read and write
+      }
+  }
+  for (int y0 = 0; y0 < N; y0 += RW)
+    for (int q = 0; q < NA; ++q) {
+      for (int l = 0; l < 64; ++l) {
+        const int rx = l % RW, jx = l / RW, n = jx + NB * q;
+        const int h = n <= hmax ? n : (n >= N - hmax ? n - N : N);
+        e[l] = (jx < NB && y0 + rx < N && h < N) ? (h + hmax) * sa + y0 + rx : -1;
+      }
+      c += 2 * lds_passes(e, el);
+    }
+  return c;
+}
+
+// ... and on the x strip stride sr
+static long plane_cost_sr(int NA, int NB, int sr, int el) {
+  const int N = NA * NB, RW = plane_rows_per_wave(NA, NB);
+  long c = 0;
+  int e[64];
+  for (int y0 = 0; y0 < N; y0 += RW) {
+    for (int q = 0; q < NA; ++q) {
+      for (int l = 0; l < 64; ++l) {
+        const int rx = l % RW, jx = l / RW;
+        e[l] = (jx < NB && y0 + rx < N) ? rx * sr + jx + NB * q : -1;
+      }
+      c += 2 * lds_passes(e, el);
+    }
+    for (int q = 0; q < NB; ++q) {
+      for (int l = 0; l < 64; ++l) {
+        const int rx = l % RW, jx = l / RW;
+        e[l] = (jx < NA && y0 + rx < N) ? rx * sr + NB * jx + q : -1;
+      }
+      c += 2 * lds_passes(e, el);
+    }
+  }
+  return c;
+}
+
+// Row strides of the plane kernel: the pair in [N, N+32]^2 with the fewest
+// modelled LDS passes that keeps as many blocks per CU as unpadded rows do
+// (QX_PLANE_SA / QX_PLANE_SR override).  Cached per shape.
+template <typename T>
+static void plane_strides(int NA, int NB, int nH, int nwave, int* psa, int* psr) {
+  static std::mutex mu;
+  static std::map<std::array<int, 4>, std::pair<int, int>> cache;
+  const std::array<int, 4> key{NA * 100 + NB, nH, (int)sizeof(T), nwave};
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(key);
+  if (it == cache.end()) {
+    const int N = NA * NB, RW = plane_rows_per_wave(NA, NB), el = (int)sizeof(cplx<T>) / 4;
+    const size_t occ0 = kCuLds / plane_lds<T>(N, nH, N, N, RW, nwave);
+    long csr[33];
+    for (int d = 0; d <= 32; ++d) csr[d] = plane_cost_sr(NA, NB, N + d, el);
+    long best = -1;
+    int bsa = N, bsr = N;
+    for (int da = 0; da <= 32; ++da) {
+      const long ca = plane_cost_sa(NA, NB, nH, N + da, el);
+      for (int dr = 0; dr <= 32; ++dr) {
+        const size_t l = plane_lds<T>(N, nH, N + da, N + dr, RW, nwave);
+        if (l > kCuLds || kCuLds / l < occ0) continue;
+        if (best < 0 || ca + csr[dr] < best) {
+          best = ca + csr[dr];
+          bsa = N + da;
+          bsr = N + dr;
+        }
+      }
+    }
+    it = cache.emplace(key, std::make_pair(bsa, bsr)).first;
+  }
+  *psa = it->second.first;
+  *psr = it->second.second;
+  if (const char* e = std::getenv("QX_PLANE_SA")) *psa = std::max(NA * NB, std::atoi(e));
+  if (const char* e = std::getenv("QX_PLANE_SR")) *psr = std::max(NA * NB, std::atoi(e));
 }
 
 template <typename T, int NA, int NB>
@@ -538,11 +669,14 @@ static int local_fast_nn(Ctx* c, const double* psi, double* out, int nb, int acc
   const int nbp = (nb + 1) / 2;
   const int nwave = plane_waves<T>(N, c->pc.nH, RW);
   if (nwave < 1) return -3;
+  int sa, sr;
+  plane_strides<T>(NA, NB, c->pc.nH, nwave, &sa, &sr);
   const size_t per_bp = sizeof(cplx<T>) * (size_t)N * c->pc.ncol;
   const int batch = (int)std::max<size_t>(1, std::min<size_t>(c->t_bytes / per_bp, (size_t)nbp));
   constexpr int PB = zpairs(N, (int)sizeof(cplx<T>));
   const unsigned gz = (unsigned)((c->pc.npair + PB - 1) / PB);
-  const size_t lp = plane_lds<T>(N, c->pc.nH, RW, nwave);
+  const size_t lp = plane_lds<T>(N, c->pc.nH, sa, sr, RW, nwave);
+  if (lp > kCuLds) return -3;
   if (lp > 65536)
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_plane_f<T, NA, NB>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lp);
@@ -552,6 +686,8 @@ static int local_fast_nn(Ctx* c, const double* psi, double* out, int nb, int acc
   for (int bp0 = 0; bp0 < nbp; bp0 += batch) {
     const int nbb = std::min(batch, nbp - bp0);
     FastArgs a = fast_args(c, nb, bp0, 0);
+    a.sa = sa;
+    a.sr = sr;
     lt_mark(c, 1, false);
     hipLaunchKernelGGL((k_zinv_f<T, NA, NB>), dim3(gz, nbb), dim3(kFastThreads), 0, c->stream, a,
                        psi, t1);
