@@ -385,6 +385,74 @@ __global__ void __launch_bounds__(64) k_inviter_par(const double* __restrict__ d
 #undef L_
 }
 
+// ---- 3b. re-orthonormalisation within clusters ----------------------------------
+// Eigenvalues within kClusterTol * ||T|| of their neighbour form a cluster (the
+// grouping of LAPACK dstein).  Inverse-iteration vectors of different clusters
+// are orthogonal to O(eps ||T|| / gap) already, so only the vectors of one
+// cluster are orthonormalised against each other.
+constexpr double kClusterTol = 1e-3;
+constexpr int kClusterMax = 64;  // larger clusters take the CholeskyQR path
+
+// cl[j] = first index of the cluster holding eigenvalue j; cl[n] = size of the
+// largest cluster.  One thread: n is at most a few thousand.
+__global__ void k_cluster_scan(const double* __restrict__ w, int n,
+                               const double* __restrict__ bounds, int* __restrict__ cl) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double tol = kClusterTol * bounds[3];
+  int s = 0, mx = 1;
+  cl[0] = 0;
+  for (int j = 1; j < n; ++j) {
+    if (w[j] - w[j - 1] > tol) s = j;
+    cl[j] = s;
+    mx = max(mx, j - s + 1);
+  }
+  cl[n] = mx;
+}
+
+// One workgroup per cluster start: classical Gram-Schmidt applied twice to
+// each vector against the earlier ones of its cluster, then normalised.
+// z: n x n column-major, vectors in columns.  Clusters hold <= kClusterMax.
+__global__ void __launch_bounds__(256) k_cluster_orth(double* __restrict__ z, int n,
+                                                      const int* __restrict__ cl) {
+  const int j0 = blockIdx.x;
+  if (cl[j0] != j0) return;
+  int k = 1;
+  while (j0 + k < n && cl[j0 + k] == j0) ++k;
+  if (k == 1 || k > kClusterMax) return;
+  __shared__ double dots[kClusterMax];
+  __shared__ double red[4];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  double* zj = z + (size_t)j0 * n;
+  for (int c = 0; c < k; ++c) {
+    double* zc = zj + (size_t)c * n;
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int p = wv; p < c; p += 4) {  // one wavefront per inner product
+        const double* zp = zj + (size_t)p * n;
+        double acc = 0.0;
+        for (int i = lane; i < n; i += 64) acc += zp[i] * zc[i];
+        acc = wave_sum(acc);
+        if (lane == 0) dots[p] = acc;
+      }
+      __syncthreads();
+      for (int i = t; i < n; i += 256) {
+        double v = zc[i];
+        for (int p = 0; p < c; ++p) v -= dots[p] * zj[(size_t)p * n + i];
+        zc[i] = v;
+      }
+      __syncthreads();
+    }
+    double acc = 0.0;
+    for (int i = t; i < n; i += 256) acc += zc[i] * zc[i];
+    acc = wave_sum(acc);
+    if (lane == 0) red[wv] = acc;
+    __syncthreads();
+    const double nrm = red[0] + red[1] + red[2] + red[3];
+    const double sc = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
+    for (int i = t; i < n; i += 256) zc[i] *= sc;
+    __syncthreads();
+  }
+}
+
 // out[j*n + i] = in[i*n + j]
 __global__ void __launch_bounds__(256) k_transpose_sq(const double* __restrict__ in, int n,
                                                       double* __restrict__ out) {
@@ -512,6 +580,7 @@ int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int
   double* bounds = pbuf + 2 * (size_t)n;                        // 8
   int* ok = reinterpret_cast<int*>(bounds + 8);                 // 2 x 8 ints
   int* sflag = ok + 16;                                         // 1 int
+  int* cl = reinterpret_cast<int*>(dw + 8 * (size_t)n + 64);    // n + 1 ints
   double* X = dw + 16 * (size_t)n + 64;                         // n^2, row-major vectors
   double* gram = X + (size_t)n * n;                             // n^2
   double* linv = gram + (size_t)n * n;                          // n^2
@@ -532,14 +601,25 @@ int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int
   // 2. eigenvalues
   hipLaunchKernelGGL(k_gersh, dim3(1), dim3(256), 0, st, d, e, n, bounds);
   hipLaunchKernelGGL(k_bisect_ms, dim3((n + 3) / 4), dim3(256), 0, st, d, e, n, bounds, w);
-  // 3. eigenvectors: inverse iteration + CholeskyQR, twice; the LU factors
-  // of round 0 are reused by round 1, so the CholeskyQR scratch is X
+  // 3. eigenvectors: inverse iteration + re-orthonormalisation, twice; the LU
+  // factors of round 0 are reused by round 1.  Vectors are orthonormalised
+  // within eigenvalue clusters, or, when a cluster is too large for that, all
+  // together by CholeskyQR (scratch X).
+  hipLaunchKernelGGL(k_cluster_scan, dim3(1), dim3(64), 0, st, w, n, bounds, cl);
+  int maxcl = 0;
+  QX_CHECK(hipMemcpyAsync(&maxcl, cl + n, sizeof(int), hipMemcpyDeviceToHost, st));
+  QX_CHECK(hipStreamSynchronize(st));
+  const bool local = maxcl <= kClusterMax && std::getenv("QX_EIGH_CHOLQR") == nullptr;
   const dim3 tgrid((n + 31) / 32, (n + 31) / 32);
   for (int round = 0; round < 2; ++round) {
     if (round > 0) hipLaunchKernelGGL(k_transpose_sq, tgrid, dim3(256), 0, st, z, n, X);
     hipLaunchKernelGGL(k_inviter_par, dim3((n + 63) / 64), dim3(64), 0, st, d, e, n, w, bounds, lu,
                        X, round == 0 ? 1 : 0);
     hipLaunchKernelGGL(k_transpose_sq, tgrid, dim3(256), 0, st, X, n, z);
+    if (local) {
+      hipLaunchKernelGGL(k_cluster_orth, dim3(n), dim3(256), 0, st, z, n, cl);
+      continue;
+    }
     // round 0 starts from random vectors (shifted first pass); round 1 from
     // orthonormal ones, which the solve keeps near orthonormal
     int rc = chol_qr(c, z, n, gram, linv, X, ok + 8 * round, round == 0);
@@ -557,7 +637,7 @@ int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int
   QX_CHECK(hipStreamSynchronize(st));
   if (hflags[16]) return -20;  // a grid barrier gave up
   // a Gram matrix of the re-orthonormalisation was not positive definite
-  if (!hflags[0] || !hflags[2] || !hflags[4] || !hflags[8] || !hflags[10]) return -23;
+  if (!local && (!hflags[0] || !hflags[2] || !hflags[4] || !hflags[8] || !hflags[10])) return -23;
   return 0;
 }
 

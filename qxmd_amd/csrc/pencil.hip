@@ -437,7 +437,10 @@ static int local_apply_t(Ctx* c, const double* psi, double* out, int nb, int acc
 }
 
 int local_apply(Ctx* c, const double* psi, double* out, int nb, int prec, int accumulate) {
-  if (fast_ok(c)) return local_apply_fast(c, psi, out, nb, prec, accumulate);
+  if (fast_ok(c)) {
+    const int rc = local_apply_fast(c, psi, out, nb, prec, accumulate);
+    if (rc != -3) return rc;  // -3: the fast plane kernel does not fit this grid
+  }
   if (prec == PREC_FP32) return local_apply_t<float>(c, psi, out, nb, accumulate);
   return local_apply_t<double>(c, psi, out, nb, accumulate);
 }
@@ -477,7 +480,10 @@ static int density_t(Ctx* c, const double* psi, int nb, const double* w) {
 }
 
 int density(Ctx* c, const double* psi, int nb, const double* w, double, int prec) {
-  if (fast_ok(c)) return density_fast(c, psi, nb, w, prec);
+  if (fast_ok(c)) {
+    const int rc = density_fast(c, psi, nb, w, prec);
+    if (rc != -3) return rc;  // -3: the fast plane kernel does not fit this grid
+  }
   if (prec == PREC_FP32) return density_t<float>(c, psi, nb, w);
   return density_t<double>(c, psi, nb, w);
 }

@@ -32,7 +32,7 @@ __host__ __device__ constexpr int zpairs(int N, int elem_bytes) {
 __host__ __device__ inline int slot_phys(int s, int NA, int NB) { return s / NB + NA * (s % NB); }
 
 struct FastArgs {
-  int N, nH, hmax, npair, ncol, nplwex, nb, bp0, rows;
+  int N, nH, hmax, npair, ncol, nplwex, nb, bp0;
   int sa, sr;  // row strides (elements) of the plane kernel's sphere tile and x strips
   const int *pt_col, *pt_z, *pair_p0, *col_hi, *col_kmod;
   const double* kin;
@@ -139,17 +139,6 @@ __global__ void __launch_bounds__(kFastThreads) k_zinv_f(FastArgs a, const doubl
 }
 
 // ---- plane: y inverse, x inverse, x V, x forward, y forward ----------------
-template <typename T, int NA, int NB>
-__device__ inline void plane_y_inverse(cplx<T>* A, int nH, const cplx<T>* tw) {
-  constexpr int N = NA * NB;
-  for (int it = threadIdx.x; it < nH * NB; it += blockDim.x)
-    np_stepA<T, NA, NB>(A + (it / NB) * N, it % NB, tw, +1);
-  __syncthreads();
-  for (int it = threadIdx.x; it < nH * NA; it += blockDim.x)
-    np_stepB<T, NA, NB>(A + (it / NA) * N, it % NA, +1);
-  __syncthreads();
-}
-
 // x step A for row slot ys: gather the natural-x inputs (|h| <= hmax) from A
 template <typename T, int NA, int NB>
 __device__ inline void x_stepA(const cplx<T>* A, int sa, cplx<T>* R, int ys, int b, int hmax,
@@ -374,56 +363,85 @@ __global__ void __launch_bounds__(kFastThreads) k_zfwd_f(FastArgs a, const doubl
 }
 
 // ---- density: y inverse, x inverse, accumulate |psi|^2 in slot order --------
-template <typename T, int NA, int NB>
-__global__ void __launch_bounds__(kFastThreads) k_plane_rho_f(FastArgs a,
-                                                              const cplx<T>* __restrict__ t1,
-                                                              int nbp_batch, int bp_per_group,
-                                                              const double* __restrict__ w,
-                                                              double* __restrict__ rho_part) {
+// Same plane layout and lane maps as k_plane_f.  A block (z plane, group of
+// band pairs) keeps its share of the plane's density in registers over all its
+// band pairs (RG row groups per wavefront at most) and writes it once.
+constexpr int kRhoGroups = 4;  // largest RG instantiated
+
+template <typename T, int NA, int NB, int RG>
+__global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_rho_f(FastArgs a,
+                                                                const cplx<T>* __restrict__ t1,
+                                                                int nbp_batch, int bp_per_group,
+                                                                const double* __restrict__ w,
+                                                                double* __restrict__ rho_part) {
   constexpr int N = NA * NB;
+  constexpr int RW = plane_rows_per_wave(NA, NB);
+  constexpr int NG = (N + RW - 1) / RW;
   extern __shared__ unsigned char smem_raw[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int sa = a.sa, sr = a.sr, nH = a.nH;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* A = tw + N;
-  cplx<T>* R = A + a.nH * N;
+  cplx<T>* Rw = A + nH * sa + wave * RW * sr;
   const int zs = blockIdx.x, grp = blockIdx.y;
+  const int rx = lane % RW, jx = lane / RW;
+  const bool actA = jx < NB, actB = jx < NA;
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = mk<T>((T)a.tw[2 * k], (T)a.tw[2 * k + 1]);
-  double* rp = rho_part + ((size_t)grp * N + zs) * N * N;
+  double acc[RG][NB];
+#pragma unroll
+  for (int gi = 0; gi < RG; ++gi)
+#pragma unroll
+    for (int kb = 0; kb < NB; ++kb) acc[gi][kb] = 0.0;
   const int bpe = min(nbp_batch, (grp + 1) * bp_per_group);
   for (int bp = grp * bp_per_group; bp < bpe; ++bp) {
     const int ba = 2 * (a.bp0 + bp), bb = ba + 1;
     const double wa = w[ba], wb = bb < a.nb ? w[bb] : 0.0;
-    const bool first = bp == grp * bp_per_group;
     __syncthreads();
-    for (int i = threadIdx.x; i < a.nH * N; i += blockDim.x) A[i] = mk<T>(0, 0);
+    for (int i = threadIdx.x; i < nH * sa; i += blockDim.x) A[i] = mk<T>(0, 0);
     __syncthreads();
     const cplx<T>* in = t1 + ((size_t)bp * N + zs) * a.ncol;
     for (int cl = threadIdx.x; cl < a.ncol; cl += blockDim.x)
-      A[a.col_hi[cl] * N + a.col_kmod[cl]] = in[cl];
+      A[a.col_hi[cl] * sa + a.col_kmod[cl]] = in[cl];
     __syncthreads();
-    plane_y_inverse<T, NA, NB>(A, a.nH, tw);
-    for (int y0 = 0; y0 < N; y0 += a.rows) {
-      const int rows = min(a.rows, N - y0);
-      for (int it = threadIdx.x; it < rows * NB; it += blockDim.x) {
-        int r = it / NB;
-        x_stepA<T, NA, NB>(A, N, R + r * N, y0 + r, it % NB, a.hmax, tw);
-      }
-      __syncthreads();
-      for (int it = threadIdx.x; it < rows * NA; it += blockDim.x) {
-        const int r = it / NA, ka = it % NA;
-        const cplx<T>* L = R + r * N + NB * ka;
-        cplx<T> v[NB];
+    for (int it = threadIdx.x; it < nH * NB; it += blockDim.x)
+      np_stepA<T, NA, NB>(A + (it % nH) * sa, it / nH, tw, +1);
+    __syncthreads();
+    for (int it = threadIdx.x; it < nH * NA; it += blockDim.x)
+      np_stepB<T, NA, NB>(A + (it % nH) * sa, it / nH, +1);
+    __syncthreads();
 #pragma unroll
-        for (int b = 0; b < NB; ++b) v[b] = L[b];
-        Dft<T, NB>::run(v, +1);
-        double* dst = rp + (size_t)(y0 + r) * N + NB * ka;
+    for (int gi = 0; gi < RG; ++gi) {
+      const int g = wave + gi * nwave;
+      if (g < NG) {
+        const int y0 = g * RW;
+        const bool row_ok = y0 + rx < N;
+        cplx<T>* Rr = Rw + rx * sr;
+        if (actA && row_ok) x_stepA<T, NA, NB>(A, sa, Rr, y0 + rx, jx, a.hmax, tw);
+        wave_lds_sync();
+        if (actB && row_ok) {
+          const cplx<T>* L = Rr + NB * jx;
+          cplx<T> v[NB];
 #pragma unroll
-        for (int kb = 0; kb < NB; ++kb) {
-          double re = (double)v[kb].x, im = (double)v[kb].y;
-          double acc = first ? 0.0 : dst[kb];
-          dst[kb] = acc + re * re * wa + im * im * wb;
+          for (int b = 0; b < NB; ++b) v[b] = L[b];
+          Dft<T, NB>::run(v, +1);
+#pragma unroll
+          for (int kb = 0; kb < NB; ++kb) {
+            const double re = (double)v[kb].x, im = (double)v[kb].y;
+            acc[gi][kb] += re * re * wa + im * im * wb;
+          }
         }
+        wave_lds_sync();
       }
-      __syncthreads();
+    }
+  }
+  double* rp = rho_part + ((size_t)grp * N + zs) * N * N;
+#pragma unroll
+  for (int gi = 0; gi < RG; ++gi) {
+    const int g = wave + gi * nwave, y = g * RW + rx;
+    if (g < NG && y < N && actB) {
+      double* dst = rp + (size_t)y * N + NB * jx;
+#pragma unroll
+      for (int kb = 0; kb < NB; ++kb) dst[kb] = acc[gi][kb];
     }
   }
 }
@@ -495,7 +513,7 @@ int fast_permute_v(Ctx* c) {
   return 0;
 }
 
-static FastArgs fast_args(Ctx* c, int nb, int bp0, int rows) {
+static FastArgs fast_args(Ctx* c, int nb, int bp0) {
   FastArgs a{};
   a.N = c->n1;
   a.nH = c->pc.nH;
@@ -505,7 +523,6 @@ static FastArgs fast_args(Ctx* c, int nb, int bp0, int rows) {
   a.nplwex = c->nplwex;
   a.nb = nb;
   a.bp0 = bp0;
-  a.rows = rows;
   a.pt_col = c->d_pt_col;
   a.pt_z = c->d_pt_z;
   a.pair_p0 = c->d_pair_p0;
@@ -514,23 +531,6 @@ static FastArgs fast_args(Ctx* c, int nb, int bp0, int rows) {
   a.kin = c->d_kin;
   a.tw = c->d_tw;  // first n1 entries: exp(+2 pi i k/N)
   return a;
-}
-
-template <typename T>
-static size_t fast_plane_lds(int N, int nH, int rows) {
-  return sizeof(cplx<T>) * ((size_t)N + (size_t)nH * N + (size_t)rows * N);
-}
-
-template <typename T>
-static int fast_rows(int N, int nH) {
-  // LDS budget of the density plane kernel: the sphere tile A is fixed, the
-  // rows of the x pass are chunked to fit (two blocks per CU in FP32)
-  size_t budget = sizeof(T) == 4 ? 76 * 1024 : 120 * 1024;
-  if (const char* e = std::getenv("QX_FAST_LDS_KB")) budget = (size_t)std::atoi(e) * 1024;
-  int rows = N;
-  while (rows > 2 && fast_plane_lds<T>(N, nH, rows) > budget) rows = (rows + 1) / 2;
-  if (fast_plane_lds<T>(N, nH, rows) > 150 * 1024) return -1;
-  return rows;
 }
 
 // LDS of the band-apply plane kernel with nwave wavefronts and row strides sa, sr
@@ -685,7 +685,7 @@ static int local_fast_nn(Ctx* c, const double* psi, double* out, int nb, int acc
   const T* Vp = sizeof(T) == 8 ? reinterpret_cast<const T*>(c->d_Vp) : reinterpret_cast<const T*>(c->d_Vpf);
   for (int bp0 = 0; bp0 < nbp; bp0 += batch) {
     const int nbb = std::min(batch, nbp - bp0);
-    FastArgs a = fast_args(c, nb, bp0, 0);
+    FastArgs a = fast_args(c, nb, bp0);
     a.sa = sa;
     a.sr = sr;
     lt_mark(c, 1, false);
@@ -704,17 +704,24 @@ static int local_fast_nn(Ctx* c, const double* psi, double* out, int nb, int acc
 template <typename T, int NA, int NB>
 static int density_fast_nn(Ctx* c, const double* psi, int nb, const double* w) {
   constexpr int N = NA * NB;
+  constexpr int RW = plane_rows_per_wave(NA, NB);
   const int nbp = (nb + 1) / 2;
-  const int rows = fast_rows<T>(N, c->pc.nH);
-  if (rows < 1) return -3;
+  const int nwave = plane_waves<T>(N, c->pc.nH, RW);
+  // every wavefront holds at most kRhoGroups row groups in registers
+  if (nwave < 1 || nwave * kRhoGroups < (N + RW - 1) / RW) return -3;
+  int sa, sr;
+  plane_strides<T>(NA, NB, c->pc.nH, nwave, &sa, &sr);
   const size_t per_bp = sizeof(cplx<T>) * (size_t)N * c->pc.ncol;
   const int batch = (int)std::max<size_t>(1, std::min<size_t>(c->t_bytes / per_bp, (size_t)nbp));
   constexpr int PB = zpairs(N, (int)sizeof(cplx<T>));
   const unsigned gz = (unsigned)((c->pc.npair + PB - 1) / PB);
-  const size_t lp = fast_plane_lds<T>(N, c->pc.nH, rows);
+  const size_t lp = plane_lds<T>(N, c->pc.nH, sa, sr, RW, nwave);
+  if (lp > kCuLds) return -3;
+  const bool rg2 = nwave * 2 >= (N + RW - 1) / RW;
+  const void* kfn = rg2 ? reinterpret_cast<const void*>(k_plane_rho_f<T, NA, NB, 2>)
+                        : reinterpret_cast<const void*>(k_plane_rho_f<T, NA, NB, kRhoGroups>);
   if (lp > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_plane_rho_f<T, NA, NB>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lp);
+    (void)hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lp);
   cplx<T>* t1 = reinterpret_cast<cplx<T>*>(c->d_t1);
   const int nbatch = (nbp + batch - 1) / batch;
   const int groups_per_batch = std::max(1, c->rho_groups / nbatch);
@@ -722,14 +729,20 @@ static int density_fast_nn(Ctx* c, const double* psi, int nb, const double* w) {
   int nparts = 0;
   for (int bp0 = 0; bp0 < nbp; bp0 += batch) {
     const int nbb = std::min(batch, nbp - bp0);
-    FastArgs a = fast_args(c, nb, bp0, rows);
+    FastArgs a = fast_args(c, nb, bp0);
+    a.sa = sa;
+    a.sr = sr;
     hipLaunchKernelGGL((k_zinv_f<T, NA, NB>), dim3(gz, nbb), dim3(kFastThreads), 0, c->stream, a,
                        psi, t1);
     int ng = std::min(groups_per_batch, nbb);
     int per = (nbb + ng - 1) / ng;
     ng = (nbb + per - 1) / per;
-    hipLaunchKernelGGL((k_plane_rho_f<T, NA, NB>), dim3(N, ng), dim3(kFastThreads), lp, c->stream,
-                       a, t1, nbb, per, w, c->d_rho_part + (size_t)nparts * c->N);
+    if (rg2)
+      hipLaunchKernelGGL((k_plane_rho_f<T, NA, NB, 2>), dim3(N, ng), dim3(64 * nwave), lp,
+                         c->stream, a, t1, nbb, per, w, c->d_rho_part + (size_t)nparts * c->N);
+    else
+      hipLaunchKernelGGL((k_plane_rho_f<T, NA, NB, kRhoGroups>), dim3(N, ng), dim3(64 * nwave), lp,
+                         c->stream, a, t1, nbb, per, w, c->d_rho_part + (size_t)nparts * c->N);
     nparts += ng;
   }
   hipLaunchKernelGGL(k_rho_reduce_f, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream,
