@@ -15,6 +15,28 @@ __device__ inline double ld_agent(const double* p) {
   return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A double published with a tag, for readers that wait on the data itself
+// instead of a barrier: two 64-bit words, each 32 bits of the value and the
+// 32-bit tag.  Each word is stored and loaded single-copy atomically, so a
+// reader that finds the tag in both words has the whole value of that
+// publication (the buffer must not hold the tag from earlier use).
+__device__ inline void st_tagged(unsigned long long* p, double v, unsigned tag) {
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+  __hip_atomic_store(p, (bits & 0xffffffff00000000ull) | tag, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 1, (bits << 32) | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned long long ld_u64_agent(const unsigned long long* p) {
+  return __hip_atomic_load(const_cast<unsigned long long*>(p), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline bool tag_ok(unsigned long long w0, unsigned long long w1, unsigned tag) {
+  return (unsigned)w0 == tag && (unsigned)w1 == tag;
+}
+__device__ inline double tag_value(unsigned long long w0, unsigned long long w1) {
+  return __longlong_as_double((long long)((w0 & 0xffffffff00000000ull) | (w1 >> 32)));
+}
+
 // Grid barrier for a kernel launched with hipLaunchCooperativeKernel (every
 // workgroup resident).  Data that crosses workgroups must go through
 // st_agent / ld_agent (device-scope accesses, coherent across the XCDs'

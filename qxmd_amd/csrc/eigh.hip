@@ -46,27 +46,42 @@ __device__ inline double block_sum256(double v, double* red) {
 //
 // One exchange per step.  At the top of step k every workgroup holds the
 // current column k (x_j = A_k(j, k), j > k) and forms the reflector v from it.
-// Row owners publish p_i = (A_k v)_i for their rows, and the owner of row k+1
-// publishes that row, A_k(k+1, :).  After the barrier every workgroup forms
-// w = 2 (p - (v.p) v), updates its rows, A_{k+1} = A_k - v w^T - w v^T, and
-// forms the next column itself from the published row:
-// x_j = A_k(k+1, j) - v_{k+1} w_j - w_{k+1} v_j.  The published buffers are
-// double-buffered by step parity (a workgroup writes parity q again only
-// after every workgroup has passed the barrier that follows its reads).
+// Row owners publish p_i = (A_k v)_i for their rows with their workgroup's
+// share of K = v.p, and the owner of row k+1 publishes that row, A_k(k+1, :).
+// Every thread then forms w_j = 2 (p_j - K v_j) for its columns, updates them
+// in the owned rows, A_{k+1} = A_k - v w^T - w v^T, and forms the next column
+// from the published row: x_j = A_k(k+1, j) - v_{k+1} w_j - w_{k+1} v_j.
+// Column j is owned by thread j mod 256 throughout (x, v and w of it stay in
+// that thread's registers).
+//
+// The published values carry the step as a tag (st_tagged), and readers wait
+// for the tags of exactly the values they need instead of a grid barrier, so
+// a step costs one store-to-load trip through the device-coherent level.  The
+// buffers are double-buffered by step parity: a workgroup writes parity q
+// again two steps later, which needs the step in between from every
+// workgroup, and each workgroup publishes that step only after it has read all
+// of its inputs of the step before.  Waits are bounded (err raised, as in
+// grid_barrier), so a launch that cannot complete still ends.
+constexpr int kTrdM = 8;   // columns per thread: n <= 256 * kTrdM
+constexpr int kTrdKq = 4;  // K shares read per lane: groups <= 64 * kTrdKq
+
 __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, int n,
                                                       double* __restrict__ d,
                                                       double* __restrict__ e, double* rowbuf,
-                                                      double* pbuf, int* bar, int* err) {
+                                                      unsigned long long* pub, int* bar,
+                                                      int* err) {
   extern __shared__ double sm[];
   const int G = gridDim.x, g = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int rpw = (n + G - 1) / G;
   double* R = sm;                      // R[r * n + j] = A(g + G r, j)
   double* v = R + (size_t)rpw * n;     // reflector, indexed by global column
-  double* w = v + n;                   // rank-2 partner
-  double* x = w + n;                   // current column
-  double* rr = x + n;                  // published row k+1
-  double* red = rr + n;                // reduction scratch
-  int nbar = 0;
+  double* prow = v + n;                // p of the owned rows
+  double* red = prow + rpw;            // 2 x 8 reduction slots (step parity)
+  double* kw = red + 16;               // K shares of the 4 wavefronts
+  // tagged buffers, 2 words per value: p [2][n], row k+1 [2][n], K shares [2][G]
+  unsigned long long* pb = pub;
+  unsigned long long* rb = pb + 4 * (size_t)n;
+  unsigned long long* kb = rb + 4 * (size_t)n;
   for (int r = 0; r < rpw; ++r) {
     const int i = g + G * r;
     if (i >= n) break;
@@ -75,78 +90,163 @@ __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, in
   __syncthreads();
   if (g == 0)  // row 0 is column 0: publish it (row 0 lives in workgroup 0)
     for (int j = t; j < n; j += 256) st_agent(&rowbuf[j], R[j]);
-  grid_barrier(bar, ++nbar * G, err);
-  for (int j = t; j < n; j += 256) x[j] = ld_agent(&rowbuf[j]);
+  grid_barrier(bar, G, err);
+  double x[kTrdM], vr[kTrdM], w[kTrdM];
+#pragma unroll
+  for (int m = 0; m < kTrdM; ++m) {
+    const int j = t + 256 * m;
+    x[m] = j < n ? ld_agent(&rowbuf[j]) : 0.0;
+    vr[m] = 0.0;
+    w[m] = 0.0;
+  }
   if (g == 0 && t == 0) d[0] = ld_agent(&rowbuf[0]);
-  __syncthreads();
+  bool failed = false;
   for (int k = 0; k + 2 < n; ++k) {
-    const int q = (k + 1) & 1;  // buffer parity of this step
-    double* pq = pbuf + (size_t)q * n;
-    double* rq = rowbuf + (size_t)q * n;
+    const int q = k & 1;
+    const unsigned tag = (unsigned)k + 1u;
+    unsigned long long* pq = pb + 2 * (size_t)q * n;
+    unsigned long long* rq = rb + 2 * (size_t)q * n;
+    unsigned long long* kq = kb + 2 * (size_t)q * G;
+    double* rd = red + 8 * q;
     // reflector of column k, formed redundantly by every workgroup
     double part = 0.0;
-    for (int j = k + 1 + t; j < n; j += 256) part += x[j] * x[j];
-    const double nx2 = block_sum256(part, red);
-    const double x0 = x[k + 1];
+#pragma unroll
+    for (int m = 0; m < kTrdM; ++m) {
+      const int j = t + 256 * m;
+      if (j > k && j < n) part += x[m] * x[m];
+      if (j == k + 1) rd[4] = x[m];
+    }
+    part = wave_sum(part);
+    if (lane == 0) rd[wv] = part;
+    __syncthreads();
+    const double nx2 = rd[0] + rd[1] + rd[2] + rd[3];
+    const double x0 = rd[4];
     const double alpha = nx2 > 0.0 ? (x0 >= 0.0 ? -sqrt(nx2) : sqrt(nx2)) : 0.0;
     const double vn2 = nx2 - 2.0 * alpha * x0 + alpha * alpha;
     const bool skip = !(vn2 > 1e-300);
     const double inv = skip ? 0.0 : 1.0 / sqrt(vn2);
-    for (int j = k + 1 + t; j < n; j += 256) v[j] = (j == k + 1 ? x[j] - alpha : x[j]) * inv;
-    __syncthreads();
-    if (g == 0) {
-      for (int j = k + 1 + t; j < n; j += 256) a[(size_t)j + (size_t)n * k] = v[j];
-      if (t == 0) e[k] = skip ? x0 : alpha;
-    }
-    // publish p for the owned rows of the trailing block, and row k+1
-    if (!skip) {
-      for (int r = wv; r < rpw; r += 4) {
-        const int i = g + G * r;
-        if (i <= k || i >= n) continue;
-        double acc = 0.0;
-        for (int j = k + 1 + lane; j < n; j += 64) acc += R[(size_t)r * n + j] * v[j];
-        acc = wave_sum(acc);
-        if (lane == 0) st_agent(&pq[i], acc);
+#pragma unroll
+    for (int m = 0; m < kTrdM; ++m) {
+      const int j = t + 256 * m;
+      if (j > k && j < n) {
+        vr[m] = (j == k + 1 ? x[m] - alpha : x[m]) * inv;
+        v[j] = vr[m];
+        if (g == 0) a[(size_t)j + (size_t)n * k] = vr[m];
       }
     }
+    if (g == 0 && t == 0) e[k] = skip ? x0 : alpha;
+    __syncthreads();
+    // publish p and the K share for the owned rows of the trailing block, and row k+1
+    double kp = 0.0;
+    for (int r = wv; r < rpw; r += 4) {
+      const int i = g + G * r;
+      if (i <= k || i >= n) continue;
+      double acc = 0.0;
+      if (!skip)
+        for (int j = k + 1 + lane; j < n; j += 64) acc += R[(size_t)r * n + j] * v[j];
+      acc = wave_sum(acc);
+      if (lane == 0) {
+        st_tagged(pq + 2 * i, acc, tag);
+        prow[r] = acc;
+      }
+      kp += v[i] * acc;
+    }
+    if (lane == 0) kw[wv] = kp;
     if ((k + 1) % G == g) {
       const double* Rr = R + (size_t)((k + 1) / G) * n;
-      for (int j = k + 1 + t; j < n; j += 256) st_agent(&rq[j], Rr[j]);
+      for (int j = k + 1 + t; j < n; j += 256) st_tagged(rq + 2 * j, Rr[j], tag);
     }
-    grid_barrier(bar, ++nbar * G, err);
-    for (int j = k + 1 + t; j < n; j += 256) rr[j] = ld_agent(&rq[j]);
-    if (!skip) {
-      double part2 = 0.0;
-      for (int j = k + 1 + t; j < n; j += 256) {
-        const double p = ld_agent(&pq[j]);
-        w[j] = p;
-        part2 += v[j] * p;
+    __syncthreads();
+    if (t == 0) st_tagged(kq + 2 * g, kw[0] + kw[1] + kw[2] + kw[3], tag);
+    // wait for this step's values: every load is issued before any tag is
+    // tested (entries clamped to published ones), so their latencies overlap
+    unsigned long long kw0[kTrdKq], kw1[kTrdKq], pw0[kTrdM], pw1[kTrdM], rw0[kTrdM], rw1[kTrdM];
+    unsigned long long p10, p11, r10, r11;
+    for (int spins = 0;; ++spins) {
+#pragma unroll
+      for (int u = 0; u < kTrdKq; ++u) {
+        const unsigned long long* s = kq + 2 * min(lane + 64 * u, G - 1);
+        kw0[u] = ld_u64_agent(s);
+        kw1[u] = ld_u64_agent(s + 1);
       }
-      const double K = block_sum256(part2, red);
-      for (int j = k + 1 + t; j < n; j += 256) w[j] = 2.0 * (w[j] - K * v[j]);
-      __syncthreads();
+      p10 = ld_u64_agent(pq + 2 * (k + 1));
+      p11 = ld_u64_agent(pq + 2 * (k + 1) + 1);
+      r10 = ld_u64_agent(rq + 2 * (k + 1));
+      r11 = ld_u64_agent(rq + 2 * (k + 1) + 1);
+#pragma unroll
+      for (int m = 0; m < kTrdM; ++m) {
+        const int j = min(max(t + 256 * m, k + 1), n - 1);
+        pw0[m] = ld_u64_agent(pq + 2 * j);
+        pw1[m] = ld_u64_agent(pq + 2 * j + 1);
+        rw0[m] = ld_u64_agent(rq + 2 * j);
+        rw1[m] = ld_u64_agent(rq + 2 * j + 1);
+      }
+      bool all = tag_ok(p10, p11, tag) && tag_ok(r10, r11, tag);
+#pragma unroll
+      for (int u = 0; u < kTrdKq; ++u) all = all && tag_ok(kw0[u], kw1[u], tag);
+#pragma unroll
+      for (int m = 0; m < kTrdM; ++m)
+        all = all && tag_ok(pw0[m], pw1[m], tag) && tag_ok(rw0[m], rw1[m], tag);
+      if (all || failed) break;
+      if ((spins & 1023) == 1023 &&
+          (spins > (1 << 20) ||
+           __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        failed = true;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    const double p1 = tag_value(p10, p11), r1 = tag_value(r10, r11);
+    double ks = 0.0;
+#pragma unroll
+    for (int u = 0; u < kTrdKq; ++u) ks += lane + 64 * u < G ? tag_value(kw0[u], kw1[u]) : 0.0;
+    const double K = wave_sum(ks);
+    if (!skip) {
+      const double v1 = v[k + 1];
+      const double w1 = 2.0 * (p1 - K * v1);
+#pragma unroll
+      for (int m = 0; m < kTrdM; ++m) {
+        const int j = t + 256 * m;
+        if (j > k && j < n) w[m] = 2.0 * (tag_value(pw0[m], pw1[m]) - K * vr[m]);
+        if (j > k + 1 && j < n) x[m] = tag_value(rw0[m], rw1[m]) - v1 * w[m] - w1 * vr[m];
+      }
+      if (g == 0 && t == 0) d[k + 1] = r1 - 2.0 * v1 * w1;
       for (int r = 0; r < rpw; ++r) {
         const int i = g + G * r;
         if (i <= k + 1 || i >= n) continue;  // row k+1 is finished
-        const double vi = v[i], wi = w[i];
+        const double vi = v[i], wi = 2.0 * (prow[r] - K * vi);
         double* Rr = R + (size_t)r * n;
-        for (int j = k + 2 + t; j < n; j += 256) Rr[j] -= vi * w[j] + wi * v[j];
+#pragma unroll
+        for (int m = 0; m < kTrdM; ++m) {
+          const int j = t + 256 * m;
+          if (j > k + 1 && j < n) Rr[j] -= vi * w[m] + wi * vr[m];
+        }
       }
     } else {
-      for (int j = k + 1 + t; j < n; j += 256) w[j] = 0.0;
+#pragma unroll
+      for (int m = 0; m < kTrdM; ++m) {
+        const int j = t + 256 * m;
+        if (j > k + 1 && j < n) x[m] = tag_value(rw0[m], rw1[m]);
+      }
+      if (g == 0 && t == 0) d[k + 1] = r1;
     }
-    __syncthreads();
-    // next column and the finished diagonal d[k+1]
-    const double v1 = v[k + 1], w1 = w[k + 1];
-    for (int j = k + 2 + t; j < n; j += 256) x[j] = rr[j] - v1 * w[j] - w1 * v[j];
-    if (g == 0 && t == 0) d[k + 1] = rr[k + 1] - 2.0 * v1 * w1;
-    __syncthreads();
   }
-  if (g == 0 && t == 0) e[n - 2] = x[n - 1];
-  if ((n - 1) % G == g && t == 0) {
+  if (g == 0) {
+#pragma unroll
+    for (int m = 0; m < kTrdM; ++m)
+      if (t + 256 * m == n - 1) e[n - 2] = x[m];
+  }
+  // R(n-1, n-1) was last updated by the thread owning column n-1: it reads it
+  if ((n - 1) % G == g && t == (n - 1) % 256) {
     d[n - 1] = R[(size_t)((n - 1) / G) * n + n - 1];
     e[n - 1] = 0.0;
   }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, long cnt) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cnt) p[i] = 0ull;
 }
 
 // ---- 2. eigenvalues ----------------------------------------------------------
@@ -576,7 +676,7 @@ int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int
   double* d = dw;
   double* e = d + n;
   double* rowbuf = e + n;                                       // 2n
-  double* pbuf = rowbuf + 2 * (size_t)n;                        // 2n
+  double* pbuf = rowbuf + 2 * (size_t)n;                        // 2n (spare)
   double* bounds = pbuf + 2 * (size_t)n;                        // 8
   int* ok = reinterpret_cast<int*>(bounds + 8);                 // 2 x 8 ints
   int* sflag = ok + 16;                                         // 1 int
@@ -591,11 +691,18 @@ int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int
   QX_CHECK(hipGetDevice(&dev));
   QX_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   const int G = std::min(tridiag_groups(n), std::min(n, ncu));
-  const size_t lds = sizeof(double) * ((size_t)((n + G - 1) / G) * n + 4 * (size_t)n + 8);
-  if (lds > 160 * 1024 - 1024) return dense_eigh_legacy(c, a, n, z, w, work);
+  const size_t rpw = (size_t)((n + G - 1) / G);
+  const size_t lds = sizeof(double) * (rpw * n + (size_t)n + rpw + 24);
+  if (lds > 160 * 1024 - 1024 || n > 256 * kTrdM || G > 64 * kTrdKq)
+    return dense_eigh_legacy(c, a, n, z, w, work);
   int* bar = c->d_sync;
   int* err = c->d_sync + 1;
-  void* args[] = {&a, &n, &d, &e, &rowbuf, &pbuf, &bar, &err};
+  // the tagged exchange buffers (8 n + 4 G words) use the eigenvector scratch,
+  // idle until step 3; tags must start cleared
+  unsigned long long* pub = reinterpret_cast<unsigned long long*>(X);
+  const long npub = 8 * (long)n + 4 * (long)G;
+  hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((npub + 255) / 256)), dim3(256), 0, st, pub, npub);
+  void* args[] = {&a, &n, &d, &e, &rowbuf, &pub, &bar, &err};
   if (coop_launch(c, reinterpret_cast<const void*>(k_tridiag_coop), G, lds, args, true) < 0)
     return dense_eigh_legacy(c, a, n, z, w, work);
   // 2. eigenvalues
