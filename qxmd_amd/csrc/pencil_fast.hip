@@ -54,8 +54,7 @@ __device__ inline void np_stepA(cplx<T>* L, int b, const cplx<T>* tw, int sign) 
   for (int a = 0; a < NA; ++a) v[a] = L[b + NB * a];
   Dft<T, NA>::run(v, sign);
 #pragma unroll
-  for (int ka = 1; ka < NA; ++ka)
-    if (b) v[ka] = cmul(v[ka], twid(tw, b * ka, sign));
+  for (int ka = 1; ka < NA; ++ka) v[ka] = cmul(v[ka], twid(tw, b * ka, sign));
 #pragma unroll
   for (int ka = 0; ka < NA; ++ka) L[b + NB * ka] = v[ka];
 }
@@ -75,8 +74,7 @@ __device__ inline void pn_stepB(cplx<T>* L, int ka, const cplx<T>* tw, int sign)
   for (int kb = 0; kb < NB; ++kb) v[kb] = L[NB * ka + kb];
   Dft<T, NB>::run(v, sign);
 #pragma unroll
-  for (int b = 1; b < NB; ++b)
-    if (ka) v[b] = cmul(v[b], twid(tw, b * ka, sign));
+  for (int b = 1; b < NB; ++b) v[b] = cmul(v[b], twid(tw, b * ka, sign));
 #pragma unroll
   for (int b = 0; b < NB; ++b) L[NB * ka + b] = v[b];
 }
@@ -139,7 +137,9 @@ __global__ void __launch_bounds__(kFastThreads) k_zinv_f(FastArgs a, const doubl
 }
 
 // ---- plane: y inverse, x inverse, x V, x forward, y forward ----------------
-// x step A for row slot ys: gather the natural-x inputs (|h| <= hmax) from A
+// x step A for row slot ys: gather the natural-x inputs (|h| <= hmax) from A;
+// row 2 hmax + 1 of A is kept zero and stands in for the x outside the sphere,
+// so every lane loads unconditionally
 template <typename T, int NA, int NB>
 __device__ inline void x_stepA(const cplx<T>* A, int sa, cplx<T>* R, int ys, int b, int hmax,
                                const cplx<T>* tw) {
@@ -147,14 +147,13 @@ __device__ inline void x_stepA(const cplx<T>* A, int sa, cplx<T>* R, int ys, int
   cplx<T> v[NA];
 #pragma unroll
   for (int a = 0; a < NA; ++a) {
-    int n = b + NB * a;
-    int h = n <= hmax ? n : (n >= N - hmax ? n - N : N);
-    v[a] = h < N ? A[(h + hmax) * sa + ys] : mk<T>(0, 0);
+    const int n = b + NB * a;
+    const int row = n <= hmax ? n + hmax : (n >= N - hmax ? n - N + hmax : 2 * hmax + 1);
+    v[a] = A[row * sa + ys];
   }
   Dft<T, NA>::run(v, +1);
 #pragma unroll
-  for (int ka = 1; ka < NA; ++ka)
-    if (b) v[ka] = cmul(v[ka], twid(tw, b * ka, +1));
+  for (int ka = 1; ka < NA; ++ka) v[ka] = cmul(v[ka], twid(tw, b * ka, +1));
 #pragma unroll
   for (int ka = 0; ka < NA; ++ka) R[b + NB * ka] = v[ka];
 }
@@ -212,8 +211,8 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int sa = a.sa, sr = a.sr, nH = a.nH;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* A = tw + N;                           // [nH][sa], y in slot order
-  cplx<T>* Rw = A + nH * sa + wave * RW * sr;    // this wave's [RW][sr] strip
+  cplx<T>* A = tw + N;                               // [nH + 1][sa], y in slot order
+  cplx<T>* Rw = A + (nH + 1) * sa + wave * RW * sr;  // this wave's [RW][sr] strip
   const int zs = blockIdx.x, bp = blockIdx.y;
 
   // lane roles of the x pass (row fastest)
@@ -222,11 +221,11 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
   // V(r) row pieces of step B; rows past the grid and idle lanes load a valid
   // row that is not used
   const T* Vz = Vp + (size_t)zs * N * N + NB * min(jx, NA - 1);
-  T vnext[NB];
+  T vv[NB];  // V(r) of the wave's current rows; the next rows' are loaded once it is used
   {
     const T* row = Vz + (size_t)min(wave * RW + rx, N - 1) * N;
 #pragma unroll
-    for (int kb = 0; kb < NB; ++kb) vnext[kb] = row[kb];
+    for (int kb = 0; kb < NB; ++kb) vv[kb] = row[kb];
   }
 
   // sphere columns of this plane: loads issued before the LDS fill
@@ -241,7 +240,7 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
     ci[j] = cl < a.ncol ? a.col_hi[cc] * sa + a.col_kmod[cc] : -1;
   }
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = mk<T>((T)a.tw[2 * k], (T)a.tw[2 * k + 1]);
-  for (int i = threadIdx.x; i < nH * sa; i += blockDim.x) A[i] = mk<T>(0, 0);
+  for (int i = threadIdx.x; i < (nH + 1) * sa; i += blockDim.x) A[i] = mk<T>(0, 0);
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < kPlaneLoads; ++j)
@@ -261,14 +260,6 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
   for (int g = wave; g < NG; g += nwave) {
     const int y0 = g * RW;
     const bool row_ok = y0 + rx < N;
-    T vcur[NB];
-#pragma unroll
-    for (int kb = 0; kb < NB; ++kb) vcur[kb] = vnext[kb];
-    if (g + nwave < NG) {
-      const T* row = Vz + (size_t)min((g + nwave) * RW + rx, N - 1) * N;
-#pragma unroll
-      for (int kb = 0; kb < NB; ++kb) vnext[kb] = row[kb];
-    }
     cplx<T>* Rr = Rw + rx * sr;
     if (actA && row_ok) x_stepA<T, NA, NB>(A, sa, Rr, y0 + rx, jx, a.hmax, tw);
     wave_lds_sync();
@@ -280,15 +271,19 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
       Dft<T, NB>::run(v, +1);
 #pragma unroll
       for (int kb = 0; kb < NB; ++kb) {
-        v[kb].x *= vcur[kb];
-        v[kb].y *= vcur[kb];
+        v[kb].x *= vv[kb];
+        v[kb].y *= vv[kb];
       }
       Dft<T, NB>::run(v, -1);
 #pragma unroll
-      for (int b = 1; b < NB; ++b)
-        if (jx) v[b] = cmul(v[b], twid(tw, b * jx, -1));
+      for (int b = 1; b < NB; ++b) v[b] = cmul(v[b], twid(tw, b * jx, -1));
 #pragma unroll
       for (int b = 0; b < NB; ++b) L[b] = v[b];
+    }
+    if (g + nwave < NG) {
+      const T* row = Vz + (size_t)min((g + nwave) * RW + rx, N - 1) * N;
+#pragma unroll
+      for (int kb = 0; kb < NB; ++kb) vv[kb] = row[kb];
     }
     wave_lds_sync();
     if (actA && row_ok) x_stepAp<T, NA, NB>(A, sa, Rr, y0 + rx, jx, a.hmax);
@@ -381,8 +376,8 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_rho_f(FastArgs a,
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int sa = a.sa, sr = a.sr, nH = a.nH;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* A = tw + N;
-  cplx<T>* Rw = A + nH * sa + wave * RW * sr;
+  cplx<T>* A = tw + N;                               // [nH + 1][sa], last row zero
+  cplx<T>* Rw = A + (nH + 1) * sa + wave * RW * sr;
   const int zs = blockIdx.x, grp = blockIdx.y;
   const int rx = lane % RW, jx = lane / RW;
   const bool actA = jx < NB, actB = jx < NA;
@@ -397,7 +392,7 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_rho_f(FastArgs a,
     const int ba = 2 * (a.bp0 + bp), bb = ba + 1;
     const double wa = w[ba], wb = bb < a.nb ? w[bb] : 0.0;
     __syncthreads();
-    for (int i = threadIdx.x; i < nH * sa; i += blockDim.x) A[i] = mk<T>(0, 0);
+    for (int i = threadIdx.x; i < (nH + 1) * sa; i += blockDim.x) A[i] = mk<T>(0, 0);
     __syncthreads();
     const cplx<T>* in = t1 + ((size_t)bp * N + zs) * a.ncol;
     for (int cl = threadIdx.x; cl < a.ncol; cl += blockDim.x)
@@ -536,7 +531,7 @@ static FastArgs fast_args(Ctx* c, int nb, int bp0) {
 // LDS of the band-apply plane kernel with nwave wavefronts and row strides sa, sr
 template <typename T>
 static size_t plane_lds(int N, int nH, int sa, int sr, int rw, int nwave) {
-  return sizeof(cplx<T>) * ((size_t)N + (size_t)nH * sa + (size_t)nwave * rw * sr);
+  return sizeof(cplx<T>) * ((size_t)N + (size_t)(nH + 1) * sa + (size_t)nwave * rw * sr);
 }
 
 constexpr size_t kCuLds = 160 * 1024;
@@ -591,12 +586,17 @@ static long plane_cost_sa(int NA, int NB, int nH, int sa, int el) {
   }
   for (int y0 = 0; y0 < N; y0 += RW)
     for (int q = 0; q < NA; ++q) {
+      int e2[64];
       for (int l = 0; l < 64; ++l) {
         const int rx = l % RW, jx = l / RW, n = jx + NB * q;
         const int h = n <= hmax ? n : (n >= N - hmax ? n - N : N);
-        e[l] = (jx < NB && y0 + rx < N && h < N) ? (h + hmax) * sa + y0 + rx : -1;
+        const bool ok = jx < NB && y0 + rx < N;
+        // step A reads every slot (the zero row nH outside the sphere), step A'
+        // writes the sphere's slots only
+        e[l] = ok ? (h < N ? h + hmax : nH) * sa + y0 + rx : -1;
+        e2[l] = ok && h < N ? (h + hmax) * sa + y0 + rx : -1;
       }
-      c += 2 * lds_passes(e, el);
+      c += lds_passes(e, el) + lds_passes(e2, el);
     }
   return c;
 }
