@@ -53,8 +53,12 @@ __device__ inline void np_stepA(cplx<T>* L, int b, const cplx<T>* tw, int sign) 
 #pragma unroll
   for (int a = 0; a < NA; ++a) v[a] = L[b + NB * a];
   Dft<T, NA>::run(v, sign);
+  int k = 0;
 #pragma unroll
-  for (int ka = 1; ka < NA; ++ka) v[ka] = cmul(v[ka], twid(tw, b * ka, sign));
+  for (int ka = 1; ka < NA; ++ka) {
+    k += b;  // b * ka
+    v[ka] = cmul(v[ka], twid(tw, k, sign));
+  }
 #pragma unroll
   for (int ka = 0; ka < NA; ++ka) L[b + NB * ka] = v[ka];
 }
@@ -73,8 +77,12 @@ __device__ inline void pn_stepB(cplx<T>* L, int ka, const cplx<T>* tw, int sign)
 #pragma unroll
   for (int kb = 0; kb < NB; ++kb) v[kb] = L[NB * ka + kb];
   Dft<T, NB>::run(v, sign);
+  int k = 0;
 #pragma unroll
-  for (int b = 1; b < NB; ++b) v[b] = cmul(v[b], twid(tw, b * ka, sign));
+  for (int b = 1; b < NB; ++b) {
+    k += ka;  // b * ka
+    v[b] = cmul(v[b], twid(tw, k, sign));
+  }
 #pragma unroll
   for (int b = 0; b < NB; ++b) L[NB * ka + b] = v[b];
 }
@@ -137,41 +145,53 @@ __global__ void __launch_bounds__(kFastThreads) k_zinv_f(FastArgs a, const doubl
 }
 
 // ---- plane: y inverse, x inverse, x V, x forward, y forward ----------------
-// x step A for row slot ys: gather the natural-x inputs (|h| <= hmax) from A;
-// row 2 hmax + 1 of A is kept zero and stands in for the x outside the sphere,
-// so every lane loads unconditionally
-template <typename T, int NA, int NB>
-__device__ inline void x_stepA(const cplx<T>* A, int sa, cplx<T>* R, int ys, int b, int hmax,
-                               const cplx<T>* tw) {
+// Row offsets in the sphere tile of the natural-x inputs b + NB a of x step A
+// (lane b): row h + hmax for |h| <= hmax, else row 2 hmax + 1, which is kept
+// zero, so every lane loads unconditionally; bit a of *inside is set for the
+// slots inside the sphere, the ones step A' writes back.
+template <int NA, int NB>
+__device__ inline void x_offsets(int b, int hmax, int sa, int (&off)[NA], unsigned* inside) {
   constexpr int N = NA * NB;
-  cplx<T> v[NA];
+  unsigned m = 0;
 #pragma unroll
   for (int a = 0; a < NA; ++a) {
     const int n = b + NB * a;
-    const int row = n <= hmax ? n + hmax : (n >= N - hmax ? n - N + hmax : 2 * hmax + 1);
-    v[a] = A[row * sa + ys];
+    const bool lo = n <= hmax, hi = n >= N - hmax;
+    off[a] = (lo ? n + hmax : (hi ? n - N + hmax : 2 * hmax + 1)) * sa;
+    m |= (lo || hi) ? 1u << a : 0u;
   }
-  Dft<T, NA>::run(v, +1);
+  *inside = m;
+}
+
+// x step A for row slot ys, inputs gathered from the sphere tile
+template <typename T, int NA, int NB>
+__device__ inline void x_stepA(const cplx<T>* A, const int (&off)[NA], cplx<T>* R, int ys, int b,
+                               const cplx<T>* tw) {
+  cplx<T> v[NA];
 #pragma unroll
-  for (int ka = 1; ka < NA; ++ka) v[ka] = cmul(v[ka], twid(tw, b * ka, +1));
+  for (int a = 0; a < NA; ++a) v[a] = A[off[a] + ys];
+  Dft<T, NA>::run(v, +1);
+  int k = 0;
+#pragma unroll
+  for (int ka = 1; ka < NA; ++ka) {
+    k += b;  // b * ka
+    v[ka] = cmul(v[ka], twid(tw, k, +1));
+  }
 #pragma unroll
   for (int ka = 0; ka < NA; ++ka) R[b + NB * ka] = v[ka];
 }
 
-// x step A': back to natural x, keep |h| <= hmax into A
+// x step A': back to natural x, the slots inside the sphere into the tile
 template <typename T, int NA, int NB>
-__device__ inline void x_stepAp(cplx<T>* A, int sa, const cplx<T>* R, int ys, int b, int hmax) {
-  constexpr int N = NA * NB;
+__device__ inline void x_stepAp(cplx<T>* A, const int (&off)[NA], unsigned inside,
+                                const cplx<T>* R, int ys, int b) {
   cplx<T> v[NA];
 #pragma unroll
   for (int ka = 0; ka < NA; ++ka) v[ka] = R[b + NB * ka];
   Dft<T, NA>::run(v, -1);
 #pragma unroll
-  for (int a = 0; a < NA; ++a) {
-    int n = b + NB * a;
-    int h = n <= hmax ? n : (n >= N - hmax ? n - N : N);
-    if (h < N) A[(h + hmax) * sa + ys] = v[a];
-  }
+  for (int a = 0; a < NA; ++a)
+    if (inside & (1u << a)) A[off[a] + ys] = v[a];
 }
 
 // LDS accesses of one wavefront are performed in issue order, so lanes of a
@@ -218,6 +238,9 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
   // lane roles of the x pass (row fastest)
   const int rx = lane % RW, jx = lane / RW;
   const bool actA = jx < NB, actB = jx < NA;
+  int xoff[NA];
+  unsigned xin;
+  x_offsets<NA, NB>(min(jx, NB - 1), a.hmax, sa, xoff, &xin);
   // V(r) row pieces of step B; rows past the grid and idle lanes load a valid
   // row that is not used
   const T* Vz = Vp + (size_t)zs * N * N + NB * min(jx, NA - 1);
@@ -261,7 +284,7 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
     const int y0 = g * RW;
     const bool row_ok = y0 + rx < N;
     cplx<T>* Rr = Rw + rx * sr;
-    if (actA && row_ok) x_stepA<T, NA, NB>(A, sa, Rr, y0 + rx, jx, a.hmax, tw);
+    if (actA && row_ok) x_stepA<T, NA, NB>(A, xoff, Rr, y0 + rx, jx, tw);
     wave_lds_sync();
     if (actB && row_ok) {
       cplx<T>* L = Rr + NB * jx;
@@ -275,8 +298,12 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
         v[kb].y *= vv[kb];
       }
       Dft<T, NB>::run(v, -1);
+      int k = 0;
 #pragma unroll
-      for (int b = 1; b < NB; ++b) v[b] = cmul(v[b], twid(tw, b * jx, -1));
+      for (int b = 1; b < NB; ++b) {
+        k += jx;  // b * jx
+        v[b] = cmul(v[b], twid(tw, k, -1));
+      }
 #pragma unroll
       for (int b = 0; b < NB; ++b) L[b] = v[b];
     }
@@ -286,7 +313,7 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
       for (int kb = 0; kb < NB; ++kb) vv[kb] = row[kb];
     }
     wave_lds_sync();
-    if (actA && row_ok) x_stepAp<T, NA, NB>(A, sa, Rr, y0 + rx, jx, a.hmax);
+    if (actA && row_ok) x_stepAp<T, NA, NB>(A, xoff, xin, Rr, y0 + rx, jx);
     wave_lds_sync();
   }
   __syncthreads();
@@ -381,6 +408,9 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_rho_f(FastArgs a,
   const int zs = blockIdx.x, grp = blockIdx.y;
   const int rx = lane % RW, jx = lane / RW;
   const bool actA = jx < NB, actB = jx < NA;
+  int xoff[NA];
+  unsigned xin;
+  x_offsets<NA, NB>(min(jx, NB - 1), a.hmax, sa, xoff, &xin);
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = mk<T>((T)a.tw[2 * k], (T)a.tw[2 * k + 1]);
   double acc[RG][NB];
 #pragma unroll
@@ -411,7 +441,7 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_rho_f(FastArgs a,
         const int y0 = g * RW;
         const bool row_ok = y0 + rx < N;
         cplx<T>* Rr = Rw + rx * sr;
-        if (actA && row_ok) x_stepA<T, NA, NB>(A, sa, Rr, y0 + rx, jx, a.hmax, tw);
+        if (actA && row_ok) x_stepA<T, NA, NB>(A, xoff, Rr, y0 + rx, jx, tw);
         wave_lds_sync();
         if (actB && row_ok) {
           const cplx<T>* L = Rr + NB * jx;
