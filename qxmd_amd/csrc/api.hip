@@ -91,6 +91,10 @@ static int get_reference(Ctx* c, const double* src, double* host, int nb) {
 }
 
 int hpsi_batched(Ctx* c, const double* x, double* hx, int nb, int prec) {
+  if (prec == PREC_FP32) {
+    const int rf = hpsi_mixed_fused(c, x, hx, nb);
+    if (rf != -3) return rf;  // -3: shape not covered by the fused order
+  }
   int rc = local_apply(c, x, hx, nb, prec, 0);
   if (rc) return rc;
   return kb_apply(c, x, hx, nb, 1, nullptr, prec);
@@ -485,7 +489,7 @@ int qx_scf_band_step_ex(qx_ctx* h, int psi, int hpsi, int nb, const double* occ,
 }  // extern "C"
 
 namespace qx {
-void lt_mark(Ctx* c, int id, bool end) {
+void lt_mark(Ctx* c, int id, bool end, bool cont) {
   if (id < 0 || id >= Ctx::kTimers || !c->lt[id].on) return;
   auto& t = c->lt[id];
   if (!end && t.used + 2 > t.ev.size()) {
@@ -498,8 +502,11 @@ void lt_mark(Ctx* c, int id, bool end) {
   }
   if (!end) {
     (void)hipEventRecord(t.ev[t.used], c->stream);
+    t.next_cont = cont;
   } else if (t.used + 1 < t.ev.size()) {
     (void)hipEventRecord(t.ev[t.used + 1], c->stream);
+    t.cont.resize(t.used / 2 + 1);
+    t.cont[t.used / 2] = t.next_cont ? 1 : 0;
     t.used += 2;
   }
 }
@@ -521,13 +528,15 @@ int qx_launch_timer(qx_ctx* h, int id, int mode, double* ms, int* count) {
   t.on = false;
   QX_CHECK(hipStreamSynchronize(c->stream));
   double sum = 0.0;
+  int launches = 0;
   for (size_t k = 0; k + 1 < t.used; k += 2) {
     float e = 0.f;
     QX_CHECK(hipEventElapsedTime(&e, t.ev[k], t.ev[k + 1]));
     sum += e;
+    if (k / 2 >= t.cont.size() || !t.cont[k / 2]) ++launches;
   }
   if (ms) *ms = sum;
-  if (count) *count = (int)(t.used / 2);
+  if (count) *count = launches;
   return 0;
 }
 

@@ -29,13 +29,15 @@ struct LtPlan {
 
 typedef std::tuple<int, int, int, int, int, int, int, int> PlanKey;  // ta M N K lda ldb ldc mode
 
+constexpr int kScratch = 8;
+
 struct LtState {
   hipblasLtHandle_t h = nullptr;
   void* ws = nullptr;
   size_t ws_bytes = 0;
   std::map<PlanKey, LtPlan> plans;
-  float* f32[3] = {};
-  size_t f32_len[3] = {};
+  float* f32[kScratch] = {};  // 0-2: operand copies of lt_sgemm_mixed; the rest lt_f32_scratch
+  size_t f32_len[kScratch] = {};
   float* B32 = nullptr;          // FP32 copy of the KB table
   const double* B32_src = nullptr;
   size_t B32_len = 0;
@@ -174,12 +176,12 @@ __global__ void k_f32_axpby(const float* __restrict__ R, int M, long N, double a
   *p = beta == 0.0 ? v : beta * *p + v;
 }
 
+}  // namespace
+
 bool lt_enabled() {
   static const int own = std::getenv("QX_GEMM_OWN") ? 1 : 0;
   return !own;
 }
-
-}  // namespace
 
 // FP64: C = alpha op(A) B + beta C.  Returns < 0 when hipBLASLt cannot take
 // the shape (the caller falls back to its own kernels).
@@ -197,31 +199,66 @@ int lt_dgemm(Ctx* c, bool ta, int M, int N, int K, double alpha, const double* A
   return st == HIPBLAS_STATUS_SUCCESS ? 0 : -2;
 }
 
+// FP32 copy of the KB table (nplwex x npj), made once per table
+const float* lt_kb_b32(Ctx* c) {
+  LtState* s = state(c);
+  if (!s || !c->d_B) return nullptr;
+  const size_t len = (size_t)c->nplwex * c->npj;
+  if (s->B32_src != c->d_B || s->B32_len != len || !s->B32) {
+    if (s->B32) (void)hipFree(s->B32);
+    s->B32 = nullptr;
+    if (hipMalloc(&s->B32, sizeof(float) * len) != hipSuccess) return nullptr;
+    hipLaunchKernelGGL(k_to_f32, dim3(cdivu((long)len, 256)), dim3(256), 0, c->stream, c->d_B,
+                       c->nplwex, s->B32, c->nplwex, c->nplwex, (long)c->npj);
+    s->B32_src = c->d_B;
+    s->B32_len = len;
+  }
+  return s->B32;
+}
+
+// FP32 work buffer i (0 <= i < 4) of at least n floats, kept by the context
+float* lt_f32_scratch(Ctx* c, int i, size_t n) {
+  LtState* s = state(c);
+  if (!s || i < 0 || i + 3 >= kScratch) return nullptr;
+  return scratch(s, i + 3, n);
+}
+
+// the FP32 compute mode of the mixed products: QX_MIXED_GEMM=f32 keeps plain
+// FP32 products; the default is the library's fast FP32 mode (split products
+// on the matrix cores)
+static int mixed_mode() {
+  const char* mode_env = std::getenv("QX_MIXED_GEMM");
+  return mode_env && std::strcmp(mode_env, "f32") == 0 ? 1 : 2;
+}
+
+// FP32 in and out: C = op(A) B
+int lt_sgemm_f32(Ctx* c, bool ta, int M, int N, int K, const float* A, int lda, const float* B,
+                 int ldb, float* C, int ldc) {
+  if (!lt_enabled()) return -1;
+  LtState* s = state(c);
+  if (!s) return -1;
+  LtPlan* p = plan(c, s, ta, M, N, K, lda, ldb, ldc, mixed_mode(), A, B, C);
+  if (!p) return -1;
+  const float one = 1.0f, zero = 0.0f;
+  lt_mark(c, c->lt_gemm_id, false);
+  hipblasStatus_t st = hipblasLtMatmul(s->h, p->desc, &one, A, p->la, B, p->lb, &zero, C, p->lc, C,
+                                       p->lc, &p->algo, s->ws, s->ws_bytes, c->stream);
+  lt_mark(c, c->lt_gemm_id, true);
+  return st == HIPBLAS_STATUS_SUCCESS ? 0 : -2;
+}
+
 // mixed: FP32 copies of A and B, FP32 product R, then C = beta C + alpha R
 int lt_sgemm_mixed(Ctx* c, bool ta, int M, int N, int K, double alpha, const double* A, int lda,
                    const double* B, int ldb, double beta, double* C, int ldc) {
   if (!lt_enabled()) return -1;
   LtState* s = state(c);
   if (!s) return -1;
-  // QX_MIXED_GEMM=f32 keeps plain FP32 products; the default is the
-  // library's fast FP32 mode (split products on the matrix cores)
-  const char* mode_env = std::getenv("QX_MIXED_GEMM");
-  const int fast = mode_env && std::strcmp(mode_env, "f32") == 0 ? 0 : 1;
   const int arows = ta ? K : M, acols = ta ? M : K;
   const float* A32;
   int lda32 = arows;
   if (A == c->d_B && lda == c->nplwex && c->d_B) {
-    const size_t len = (size_t)c->nplwex * c->npj;
-    if (s->B32_src != c->d_B || s->B32_len != len || !s->B32) {
-      if (s->B32) (void)hipFree(s->B32);
-      s->B32 = nullptr;
-      if (hipMalloc(&s->B32, sizeof(float) * len) != hipSuccess) return -3;
-      hipLaunchKernelGGL(k_to_f32, dim3(cdivu((long)len, 256)), dim3(256), 0, c->stream, c->d_B,
-                         c->nplwex, s->B32, c->nplwex, c->nplwex, (long)c->npj);
-      s->B32_src = c->d_B;
-      s->B32_len = len;
-    }
-    A32 = s->B32;
+    A32 = lt_kb_b32(c);
+    if (!A32) return -3;
     lda32 = c->nplwex;
   } else {
     float* a = scratch(s, 0, (size_t)arows * acols);
@@ -235,7 +272,7 @@ int lt_sgemm_mixed(Ctx* c, bool ta, int M, int N, int K, double alpha, const dou
   if (!b || !r) return -3;
   hipLaunchKernelGGL(k_to_f32, dim3(cdivu((long)K * N, 256)), dim3(256), 0, c->stream, B, ldb, b,
                      K, K, (long)N);
-  LtPlan* p = plan(c, s, ta, M, N, K, lda32, K, M, 1 + fast, A32, b, r);
+  LtPlan* p = plan(c, s, ta, M, N, K, lda32, K, M, mixed_mode(), A32, b, r);
   if (!p) return -1;
   const float one = 1.0f, zero = 0.0f;
   lt_mark(c, c->lt_gemm_id, false);

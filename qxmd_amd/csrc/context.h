@@ -71,14 +71,17 @@ struct Ctx {
   struct LaunchTimer {
     bool on = false;
     std::vector<hipEvent_t> ev;           // begin/end pairs
+    std::vector<unsigned char> cont;      // per pair: 1 = continues the previous launch
+    bool next_cont = false;
     size_t used = 0;
   } lt[kTimers];
   int lt_gemm_id = -1;                    // timer the GEMM launches report to
   void* blaslt = nullptr;                 // hipBLASLt state (handle, plans, FP32 copies)
 };
 
-// record the begin (end) event of a timed launch of timer id, if enabled
-void lt_mark(Ctx* c, int id, bool end);
+// record the begin (end) event of a timed launch of timer id, if enabled; a
+// begin with cont = true opens a further segment of the same launch
+void lt_mark(Ctx* c, int id, bool end, bool cont = false);
 
 int pencil_build(Ctx* c, const int* nga, const int* ngb, const int* ngc, const double* recnrm);
 
@@ -98,6 +101,20 @@ int lt_sgemm_mixed(Ctx* c, bool ta, int M, int N, int K, double alpha, const dou
                    const double* B, int ldb, double beta, double* C, int ldc);
 void lt_forget_kb(Ctx* c);
 void lt_destroy(Ctx* c);
+// FP32 pieces of the fused mixed band-apply
+bool lt_enabled();
+const float* lt_kb_b32(Ctx* c);
+float* lt_f32_scratch(Ctx* c, int i, size_t n);
+int lt_sgemm_f32(Ctx* c, bool ta, int M, int N, int K, const float* A, int lda, const float* B,
+                 int ldb, float* C, int ldc);
+bool use_big(int M, int N);
+// KB term of the mixed band-apply from the FP32 copy psi32 of psi: kb32 =
+// B diag(D) (B^T W psi) in FP32 (nplwex x nb)
+int kb_mixed_f32(Ctx* c, const double* psi, const float* psi32, int nb, float* kb32);
+// mixed band-apply with the KB contractions between the pencil FFT passes:
+// the FP32 copy of psi comes out of the z inverse pass and the KB term goes
+// into the z forward pass.  Returns -3 when the shape is not covered.
+int hpsi_mixed_fused(Ctx* c, const double* psi, double* out, int nb);
 int coop_launch(Ctx* c, const void* fn, int want, size_t lds, void** args, bool exact);
 int dense_chol_inv(Ctx* c, double* s, int n, double* x, int* ok, int* flag);
 int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,

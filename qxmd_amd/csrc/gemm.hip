@@ -416,7 +416,7 @@ __global__ void __launch_bounds__(256) k_gemm_big(int M, int N, int K, int kchun
 }
 
 // use the 128 x 128 kernels when the output has enough tiles of that size
-static bool use_big(int M, int N) {
+bool use_big(int M, int N) {
   static const int env = std::getenv("QX_GEMM_SMALL") ? 1 : 0;
   return !env && M >= 256 && N >= 256;
 }
@@ -556,6 +556,38 @@ int kb_apply(Ctx* c, const double* psi, double* out, int nb, int, double* slmir_
                      c->d_P, c->d_D, c->npj, nb);
   return gemm_nn_p(c, c->nplwex, nb, c->npj, 1.0, c->d_B, c->nplwex, c->d_P, c->npj, 1.0, out,
                    c->nplwex, prec);
+}
+
+// Pd32 = (float) (D (2 P32 - c)) with c the G = 0 correction of the packed
+// KB sum (as k_corr with the Im term, alpha 2): the projection scaled for the
+// back-projection, from the FP32 product P32 = B^T psi
+__global__ void k_proj_fix(const float* __restrict__ P32, const double* __restrict__ B, int ldb,
+                           const double* __restrict__ psi, int ldp, const double* __restrict__ D,
+                           int npj, int nb, float* __restrict__ Pd32) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)npj * nb) return;
+  const int m = (int)(i % npj), n = (int)(i / npj);
+  const double c = B[(size_t)m * ldb] * psi[(size_t)n * ldp] +
+                   2.0 * B[(size_t)m * ldb + 1] * psi[(size_t)n * ldp + 1];
+  Pd32[i] = (float)((2.0 * (double)P32[i] - c) * D[m]);
+}
+
+int kb_mixed_f32(Ctx* c, const double* psi, const float* psi32, int nb, float* kb32) {
+  const int n = c->nplwex, npj = c->npj;
+  const float* B32 = lt_kb_b32(c);
+  float* P32 = lt_f32_scratch(c, 2, (size_t)npj * nb);
+  float* Pd32 = lt_f32_scratch(c, 3, (size_t)npj * nb);
+  if (!B32 || !P32 || !Pd32) return -3;
+  struct TimeGemms {  // the KB contractions are what launch timer 0 measures
+    Ctx* c;
+    explicit TimeGemms(Ctx* cc) : c(cc) { c->lt_gemm_id = 0; }
+    ~TimeGemms() { c->lt_gemm_id = -1; }
+  } tg(c);
+  int rc = lt_sgemm_f32(c, true, npj, nb, n, B32, n, psi32, n, P32, npj);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_proj_fix, dim3(cdiv((long)npj * nb, 256)), dim3(256), 0, c->stream, P32,
+                     c->d_B, n, psi, n, c->d_D, npj, nb, Pd32);
+  return lt_sgemm_f32(c, false, n, nb, npj, B32, n, Pd32, npj, kb32, n);
 }
 
 }  // namespace qx

@@ -101,9 +101,11 @@ __device__ inline int mirror_col_f(int col) {
 }
 
 // ---- z inverse: sphere -> T1[bp][z slot][col] ------------------------------
+// psi32 (optional): receives the FP32 copy of the coefficients read
 template <typename T, int NA, int NB>
 __global__ void __launch_bounds__(kFastThreads) k_zinv_f(FastArgs a, const double* __restrict__ psi,
-                                                         cplx<T>* __restrict__ t1) {
+                                                         cplx<T>* __restrict__ t1,
+                                                         float* __restrict__ psi32) {
   constexpr int N = NA * NB;
   constexpr int PB = zpairs(N, (int)sizeof(cplx<T>));
   __shared__ cplx<T> tw[N];
@@ -122,6 +124,15 @@ __global__ void __launch_bounds__(kFastThreads) k_zinv_f(FastArgs a, const doubl
   for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
     double ar = pa[2 * p], ai = pa[2 * p + 1];
     double br = hasb ? pb[2 * p] : 0.0, bi = hasb ? pb[2 * p + 1] : 0.0;
+    if (psi32) {
+      float* qa = psi32 + (size_t)ba * a.nplwex;
+      qa[2 * p] = (float)ar;
+      qa[2 * p + 1] = (float)ai;
+      if (hasb) {
+        qa[a.nplwex + 2 * p] = (float)br;
+        qa[a.nplwex + 2 * p + 1] = (float)bi;
+      }
+    }
     int col = a.pt_col[p] - c0, z = a.pt_z[p];
     if (p == 0) {
       lines[col][0] = mk<T>((T)ar, (T)br);
@@ -330,11 +341,13 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
 }
 
 // ---- z forward: T2 -> split bands, x1/N, + |G|^2 psi --------------------
+// kb32 (optional): an FP32 term added to the result (the KB part of H psi)
 template <typename T, int NA, int NB>
 __global__ void __launch_bounds__(kFastThreads) k_zfwd_f(FastArgs a, const double* __restrict__ psi,
                                                          const cplx<T>* __restrict__ t2,
                                                          double* __restrict__ out, int accumulate,
-                                                         double invN) {
+                                                         double invN,
+                                                         const float* __restrict__ kb32) {
   constexpr int N = NA * NB;
   constexpr int PB = zpairs(N, (int)sizeof(cplx<T>));
   __shared__ cplx<T> tw[N];
@@ -373,11 +386,19 @@ __global__ void __launch_bounds__(kFastThreads) k_zfwd_f(FastArgs a, const doubl
     double kin = a.kin[p];
     double r0 = kin * pa[2 * p] + Ar, r1 = kin * pa[2 * p + 1] + Ai;
     if (accumulate) { r0 += oa[2 * p]; r1 += oa[2 * p + 1]; }
+    if (kb32) {
+      r0 += (double)kb32[(size_t)ba * a.nplwex + 2 * p];
+      r1 += (double)kb32[(size_t)ba * a.nplwex + 2 * p + 1];
+    }
     oa[2 * p] = r0;
     oa[2 * p + 1] = r1;
     if (hasb) {
       double s0 = kin * pb[2 * p] + Br, s1 = kin * pb[2 * p + 1] + Bi;
       if (accumulate) { s0 += ob[2 * p]; s1 += ob[2 * p + 1]; }
+      if (kb32) {
+        s0 += (double)kb32[(size_t)bb * a.nplwex + 2 * p];
+        s1 += (double)kb32[(size_t)bb * a.nplwex + 2 * p + 1];
+      }
       ob[2 * p] = s0;
       ob[2 * p + 1] = s1;
     }
@@ -692,8 +713,17 @@ static void plane_strides(int NA, int NB, int nH, int nwave, int* psa, int* psr)
   if (const char* e = std::getenv("QX_PLANE_SR")) *psr = std::max(NA * NB, std::atoi(e));
 }
 
+// mid (optional): work launched between the plane pass and the z forward pass,
+// e.g. the KB contractions of the fused mixed band-apply (needs one batch)
+struct FastMid {
+  float* psi32;        // FP32 copy of psi, written by the z inverse pass
+  const float* kb32;   // FP32 term the z forward pass adds
+  int (*run)(Ctx*, const double*, int, const FastMid&);
+};
+
 template <typename T, int NA, int NB>
-static int local_fast_nn(Ctx* c, const double* psi, double* out, int nb, int accumulate) {
+static int local_fast_nn(Ctx* c, const double* psi, double* out, int nb, int accumulate,
+                         const FastMid* mid = nullptr) {
   constexpr int N = NA * NB;
   constexpr int RW = plane_rows_per_wave(NA, NB);
   const int nbp = (nb + 1) / 2;
@@ -713,6 +743,7 @@ static int local_fast_nn(Ctx* c, const double* psi, double* out, int nb, int acc
   cplx<T>* t1 = reinterpret_cast<cplx<T>*>(c->d_t1);
   cplx<T>* t2 = reinterpret_cast<cplx<T>*>(c->d_t2);
   const T* Vp = sizeof(T) == 8 ? reinterpret_cast<const T*>(c->d_Vp) : reinterpret_cast<const T*>(c->d_Vpf);
+  if (mid && batch < nbp) return -3;
   for (int bp0 = 0; bp0 < nbp; bp0 += batch) {
     const int nbb = std::min(batch, nbp - bp0);
     FastArgs a = fast_args(c, nb, bp0);
@@ -720,11 +751,17 @@ static int local_fast_nn(Ctx* c, const double* psi, double* out, int nb, int acc
     a.sr = sr;
     lt_mark(c, 1, false);
     hipLaunchKernelGGL((k_zinv_f<T, NA, NB>), dim3(gz, nbb), dim3(kFastThreads), 0, c->stream, a,
-                       psi, t1);
+                       psi, t1, mid ? mid->psi32 : nullptr);
     hipLaunchKernelGGL((k_plane_f<T, NA, NB>), dim3(N, nbb), dim3(64 * nwave), lp, c->stream, a,
                        Vp, t1, t2);
+    if (mid) {  // the mid work is not part of the FFT stage's launch time
+      lt_mark(c, 1, true);
+      const int rc = mid->run(c, psi, nb, *mid);
+      if (rc) return rc;
+      lt_mark(c, 1, false, true);
+    }
     hipLaunchKernelGGL((k_zfwd_f<T, NA, NB>), dim3(gz, nbb), dim3(kFastThreads), 0, c->stream, a,
-                       psi, t2, out, accumulate, 1.0 / (double)c->N);
+                       psi, t2, out, accumulate, 1.0 / (double)c->N, mid ? mid->kb32 : nullptr);
     lt_mark(c, 1, true);
   }
   QX_CHECK(hipGetLastError());
@@ -763,7 +800,7 @@ static int density_fast_nn(Ctx* c, const double* psi, int nb, const double* w) {
     a.sa = sa;
     a.sr = sr;
     hipLaunchKernelGGL((k_zinv_f<T, NA, NB>), dim3(gz, nbb), dim3(kFastThreads), 0, c->stream, a,
-                       psi, t1);
+                       psi, t1, nullptr);
     int ng = std::min(groups_per_batch, nbb);
     int per = (nbb + ng - 1) / ng;
     ng = (nbb + per - 1) / per;
@@ -782,17 +819,34 @@ static int density_fast_nn(Ctx* c, const double* psi, int nb, const double* w) {
 }
 
 template <typename T>
-static int local_fast_t(Ctx* c, const double* psi, double* out, int nb, int accumulate) {
+static int local_fast_t(Ctx* c, const double* psi, double* out, int nb, int accumulate,
+                        const FastMid* mid = nullptr) {
   switch (c->n1) {
-    case 30: return local_fast_nn<T, 6, 5>(c, psi, out, nb, accumulate);
-    case 60: return local_fast_nn<T, 6, 10>(c, psi, out, nb, accumulate);
-    case 64: return local_fast_nn<T, 8, 8>(c, psi, out, nb, accumulate);
-    case 90: return local_fast_nn<T, 9, 10>(c, psi, out, nb, accumulate);
-    case 96: return local_fast_nn<T, 12, 8>(c, psi, out, nb, accumulate);
-    case 120: return local_fast_nn<T, 12, 10>(c, psi, out, nb, accumulate);
-    case 128: return local_fast_nn<T, 16, 8>(c, psi, out, nb, accumulate);
+    case 30: return local_fast_nn<T, 6, 5>(c, psi, out, nb, accumulate, mid);
+    case 60: return local_fast_nn<T, 6, 10>(c, psi, out, nb, accumulate, mid);
+    case 64: return local_fast_nn<T, 8, 8>(c, psi, out, nb, accumulate, mid);
+    case 90: return local_fast_nn<T, 9, 10>(c, psi, out, nb, accumulate, mid);
+    case 96: return local_fast_nn<T, 12, 8>(c, psi, out, nb, accumulate, mid);
+    case 120: return local_fast_nn<T, 12, 10>(c, psi, out, nb, accumulate, mid);
+    case 128: return local_fast_nn<T, 16, 8>(c, psi, out, nb, accumulate, mid);
     default: return -10;
   }
+}
+
+static int fused_kb_mid(Ctx* c, const double* psi, int nb, const FastMid& m) {
+  return kb_mixed_f32(c, psi, m.psi32, nb, const_cast<float*>(m.kb32));
+}
+
+int hpsi_mixed_fused(Ctx* c, const double* psi, double* out, int nb) {
+  if (!fast_ok(c) || c->npj == 0 || !lt_enabled() || std::getenv("QX_HPSI_UNFUSED") ||
+      !use_big(c->npj, nb) || !use_big(c->nplwex, nb))
+    return -3;
+  FastMid m;
+  m.psi32 = lt_f32_scratch(c, 0, (size_t)c->nplwex * nb);
+  m.kb32 = lt_f32_scratch(c, 1, (size_t)c->nplwex * nb);
+  m.run = fused_kb_mid;
+  if (!m.psi32 || !m.kb32) return -3;
+  return local_fast_t<float>(c, psi, out, nb, 0, &m);
 }
 
 template <typename T>

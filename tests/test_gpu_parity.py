@@ -337,6 +337,26 @@ def test_hpsi_mixed_si64(si64):
     assert _relerr(ctx.get(1, nb), H.hpsi(psi)) < 3e-6  # FP32 FFT + FP32 MFMA
 
 
+def test_hpsi_mixed_fused_si64(si64, monkeypatch):
+    """Mixed band-apply at a size where the KB contractions run between the
+    pencil passes (FP32 copy of psi from the z inverse pass, KB term added in
+    the z forward pass): same result as the plain order, and within the FP32
+    split-product tolerance of the FP64 oracle."""
+    nb = 256
+    gs, ctx, H, v = _setup(si64, nb)
+    assert ctx.npj >= 256
+    psi = initial_wavefunctions(si64, nb)
+    ctx.put(0, psi)
+    ctx.hpsi(0, 1, nb, prec=pw.MIXED)
+    fused = ctx.get(1, nb)
+    monkeypatch.setenv("QX_HPSI_UNFUSED", "1")
+    ctx.hpsi(0, 1, nb, prec=pw.MIXED)
+    plain = ctx.get(1, nb)
+    assert _relerr(fused, plain) < 1e-12
+    sub = slice(0, 16)
+    assert _relerr(fused[sub], H.hpsi(psi[sub])) < 3e-5
+
+
 @pytest.mark.parametrize("batched", [False, True])
 def test_scf_band_step_mixed_vs_fp64(si64, batched):
     """The mixed SCF band step against the FP64 one from the same start:
