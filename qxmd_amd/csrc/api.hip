@@ -237,7 +237,8 @@ int qx_create(qx_ctx** out, int n1, int n2, int n3, int nplw, const int* nga, co
   QX_CHECK(hipMemset(c->d_Vf, 0, sizeof(float) * c->N));
   const size_t nsmall = 4 * (size_t)(c->nbmax + 1) * c->nbmax + 16 * (size_t)c->nbmax;
   QX_CHECK(hipMalloc(&c->d_small, sizeof(double) * nsmall));
-  const size_t nsc = 64 * (size_t)c->nbmax + (size_t)c->nbmax * (c->nplwex / 4096 + 2);
+  // 64 scalar arrays, then partial sums of up to 3 inner products per band and 4096-chunk
+  const size_t nsc = 64 * (size_t)c->nbmax + 3 * (size_t)c->nbmax * (c->nplwex / 4096 + 2);
   QX_CHECK(hipMalloc(&c->d_scal, sizeof(double) * nsc));
   QX_CHECK(hipMemset(c->d_scal, 0, sizeof(double) * nsc));
   c->rho_groups = std::max(1, std::min(nbp, 32));

@@ -86,6 +86,122 @@ __global__ void k_precond_mul(double* g, const double* kin2, const double* ekib,
 
 static inline unsigned cdivu(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
+// ---- fused passes of the batched schedule: each reads its vectors once and
+// leaves per-chunk partial sums in the layout of k_dot_partial (NQ sums per
+// band and chunk: part[(b * nch + ch) * NQ + q]) --------------------------------
+template <int NQ>
+__device__ inline void dot_store(double (&s)[NQ], double* part, int b, int ch, int nch) {
+  __shared__ double red[NQ][kDotThreads / 64];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double v = s[q];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NQ) {
+    double t = 0.0;
+    for (int w = 0; w < kDotThreads / 64; ++w) t += red[threadIdx.x][w];
+    part[((size_t)b * nch + ch) * NQ + threadIdx.x] = t;
+  }
+}
+
+// g <- preconditioned g for the active bands (as k_precond_mul); sums of g.g
+__global__ void __launch_bounds__(kDotThreads) k_precond_dot(double* __restrict__ g,
+                                                             const double* __restrict__ kin2,
+                                                             const double* __restrict__ ekib,
+                                                             int n, const int* __restrict__ mask,
+                                                             int nch, double* __restrict__ part) {
+  const int b = blockIdx.y, ch = blockIdx.x;
+  const bool act = !mask || mask[b];
+  const double e15 = 1.5 * ekib[b];
+  double* gb = g + (size_t)b * n;
+  const int i0 = ch * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  double s[1] = {0.0};
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    double v = gb[i];
+    if (act) {
+      const double x = kin2[i >> 1] / e15;
+      double tesu = 27.0 + x * (18.0 + x * (12.0 + 8.0 * x));
+      const double x2 = x * x, x4 = x2 * x2;
+      tesu = tesu / (tesu + 16.0 * x4);
+      v *= tesu * 2.0 / e15;
+      gb[i] = v;
+    }
+    s[0] += v * v;
+  }
+  dot_store<1>(s, part, b, ch, nch);
+}
+
+// sums of h.hpsi, h.hh, h.h
+__global__ void __launch_bounds__(kDotThreads) k_dot3(const double* __restrict__ h,
+                                                      const double* __restrict__ hp,
+                                                      const double* __restrict__ hh, int n,
+                                                      int nch, double* __restrict__ part) {
+  const int b = blockIdx.y, ch = blockIdx.x;
+  const size_t o = (size_t)b * n;
+  const int i0 = ch * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const double hv = h[o + i];
+    s[0] += hv * hp[o + i];
+    s[1] += hv * hh[o + i];
+    s[2] += hv * hv;
+  }
+  dot_store<3>(s, part, b, ch, nch);
+}
+
+// active bands: x <- a x + t h, hx <- a hx + t hh, g <- eig x - hx (t = a dt,
+// as the three k_axpby passes it replaces); every band: sums of g.g
+__global__ void __launch_bounds__(kDotThreads) k_cg_update(double* __restrict__ x,
+                                                           double* __restrict__ hx,
+                                                           double* __restrict__ g,
+                                                           const double* __restrict__ h,
+                                                           const double* __restrict__ hh,
+                                                           const double* __restrict__ a,
+                                                           const double* __restrict__ t,
+                                                           const double* __restrict__ eig, int n,
+                                                           const int* __restrict__ mask, int nch,
+                                                           double* __restrict__ part) {
+  const int b = blockIdx.y, ch = blockIdx.x;
+  const bool act = !mask || mask[b];
+  const double av = a[b], tv = t[b], ev = eig[b];
+  const size_t o = (size_t)b * n;
+  const int i0 = ch * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  double s[1] = {0.0};
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    double gv;
+    if (act) {
+      const double xv = av * x[o + i] + tv * h[o + i];
+      const double hv = av * hx[o + i] + tv * hh[o + i];
+      x[o + i] = xv;
+      hx[o + i] = hv;
+      gv = ev * xv + -1.0 * hv;
+      g[o + i] = gv;
+    } else {
+      gv = g[o + i];
+    }
+    s[0] += gv * gv;
+  }
+  dot_store<1>(s, part, b, ch, nch);
+}
+
+// out_q[b] = 2 sum_q - x_b[0] y_qb[0] for the NQ sums of the fused passes
+__global__ void k_dotq_final(const double* __restrict__ part, int nch, int nb, int nq,
+                             const double* __restrict__ x, const double* y0,
+                             const double* y1, const double* y2, int ld,
+                             double* out0, double* out1, double* out2) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const double* ys[3] = {y0, y1, y2};
+  double* outs[3] = {out0, out1, out2};
+  for (int q = 0; q < nq; ++q) {
+    double s = 0.0;
+    for (int c = 0; c < nch; ++c) s += part[((size_t)b * nch + c) * nq + q];
+    outs[q][b] = 2.0 * s - x[(size_t)b * ld] * ys[q][(size_t)b * ld];
+  }
+}
+
 int pdot_batched(Ctx* c, const double* x, const double* y, int nb, double* out_dev) {
   const int n = c->nplwex, nch = (n + kDotChunk - 1) / kDotChunk;
   double* part = c->d_scal + 64 * (size_t)c->nbmax;  // scratch after the scalar arrays
@@ -329,11 +445,15 @@ int bkl_batched(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib_ho
   pdot_batched(c, gnk, gnk, nb, s.zansa);
   hipLaunchKernelGGL(k_bkl_init, dim3(gb), dim3(256), 0, c->stream, s, nb);
   const int maxit = std::max(itermx, iteremx);
+  const int nch = (n + kDotChunk - 1) / kDotChunk;
+  double* part = c->d_scal + 64 * (size_t)c->nbmax;  // 3 x nch x nb partial sums
+  const dim3 dgrid(nch, nb);
   for (int it = 0; it < maxit; ++it) {
     // precondition, CG direction
-    hipLaunchKernelGGL(k_precond_mul, dim3(cdivu((long)n * nb, 256)), dim3(256), 0, c->stream, gnk,
-                       c->d_kin, ek, n, n, nb, s.active);
-    pdot_batched(c, gnk, gnk, nb, s.zansap);
+    hipLaunchKernelGGL(k_precond_dot, dgrid, dim3(kDotThreads), 0, c->stream, gnk, c->d_kin, ek, n,
+                       s.active, nch, part);
+    hipLaunchKernelGGL(k_dotq_final, dim3(gb), dim3(256), 0, c->stream, part, nch, nb, 1, gnk, gnk,
+                       nullptr, nullptr, n, s.zansap, nullptr, nullptr);
     hipLaunchKernelGGL(k_bkl_beta, dim3(gb), dim3(256), 0, c->stream, s, nb);
     if (it == 0) axpby(c, hnk, gnk, nullptr, nullptr, nullptr, 1.0, 0.0, nb, s.active);
     else axpby(c, hnk, gnk, hnk, nullptr, s.beta, 1.0, 0.0, nb, s.active);
@@ -342,18 +462,18 @@ int bkl_batched(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib_ho
     if (rc) return rc;
     rc = gemm_nn_p(c, n, nb, nb, -1.0, psi, n, proj, nb, 1.0, hnk, n, prec);
     if (rc) return rc;
-    pdot_batched(c, hnk, hpsi, nb, s.cai);
     rc = hpsi_batched(c, hnk, hh, nb, prec);
     if (rc) return rc;
-    pdot_batched(c, hnk, hh, nb, s.cbi);
-    pdot_batched(c, hnk, hnk, nb, s.hhi);
+    hipLaunchKernelGGL(k_dot3, dgrid, dim3(kDotThreads), 0, c->stream, hnk, hpsi, hh, n, nch, part);
+    hipLaunchKernelGGL(k_dotq_final, dim3(gb), dim3(256), 0, c->stream, part, nch, nb, 3, hnk, hpsi,
+                       hh, hnk, n, s.cai, s.cbi, s.hhi);
     hipLaunchKernelGGL(k_bkl_step, dim3(gb), dim3(256), 0, c->stream, s, nb);
     // x = a (x + dt h); Hx = a (Hx + dt Hh); g = eig x - Hx
     hipLaunchKernelGGL(k_mul, dim3(gb), dim3(256), 0, c->stream, tmp, s.alpha, s.dt, nb);
-    axpby(c, psi, psi, hnk, s.alpha, tmp, 0.0, 0.0, nb, s.active);
-    axpby(c, hpsi, hpsi, hh, s.alpha, tmp, 0.0, 0.0, nb, s.active);
-    axpby(c, gnk, psi, hpsi, s.eig, nullptr, 0.0, -1.0, nb, s.active);
-    pdot_batched(c, gnk, gnk, nb, tmp2);
+    hipLaunchKernelGGL(k_cg_update, dgrid, dim3(kDotThreads), 0, c->stream, psi, hpsi, gnk, hnk, hh,
+                       s.alpha, tmp, s.eig, n, s.active, nch, part);
+    hipLaunchKernelGGL(k_dotq_final, dim3(gb), dim3(256), 0, c->stream, part, nch, nb, 1, gnk, gnk,
+                       nullptr, nullptr, n, tmp2, nullptr, nullptr);
     QX_CHECK(hipMemcpyAsync(s.zansa, tmp2, sizeof(double) * nb, hipMemcpyDeviceToDevice, c->stream));
     hipLaunchKernelGGL(k_bkl_conv, dim3(gb), dim3(256), 0, c->stream, s, nb, noband, itermx,
                        iteremx, tolera, roughfzan, roughfeig);
