@@ -145,9 +145,28 @@ __global__ void k_corr(double* C, int M, int N, int ldc, const double* A, int ld
 static inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
 // corr: 0 none, 1 subtract the G=0 term, 2 also the Im(G=0) term (KB)
+static int gemv_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
+                   const double* B, int ldb, double beta, double* C, int ldc);
+static int gemv_nn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
+                   const double* B, int ldb, double beta, double* C, int ldc);
+// products with at most kGemvCols columns and more than kGemvMinRows rows take
+// the GEMV kernels (tiny products keep the tile kernels' summation order)
+constexpr int kGemvCols = 4, kGemvMinRows = 16;
+static bool gemv_shape(int M, int N) {
+  return N <= kGemvCols && M > kGemvMinRows && !std::getenv("QX_NO_GEMV");
+}
+
 static int gemm_tn_small(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
                          const double* B, int ldb, double beta, double* C, int ldc, int corr) {
   if (M <= 0 || N <= 0) return 0;
+  if (gemv_shape(M, N)) {
+    gemv_tn(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (corr)
+      hipLaunchKernelGGL(k_corr, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N,
+                         ldc, A, lda, B, ldb, alpha, corr == 2 ? 1 : 0);
+    QX_CHECK(hipGetLastError());
+    return 0;
+  }
   hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N, ldc,
                      beta);
   const int tiles = (int)(cdiv(M, BM) * cdiv(N, BN));
@@ -173,6 +192,11 @@ static int gemm_tn_small(Ctx* c, int M, int N, int K, double alpha, const double
 static int gemm_nn_small(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
                          const double* B, int ldb, double beta, double* C, int ldc) {
   if (M <= 0 || N <= 0) return 0;
+  if (gemv_shape(M, N)) {
+    gemv_nn(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    QX_CHECK(hipGetLastError());
+    return 0;
+  }
   if (beta != 1.0)
     hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N,
                        ldc, beta);
@@ -182,6 +206,109 @@ static int gemm_nn_small(Ctx* c, int M, int N, int K, double alpha, const double
                      A, lda, B, ldb, C, ldc);
   lt_mark(c, c->lt_gemm_id, true);
   QX_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ---- skinny shapes (N <= 4 columns): matrix-vector products at HBM speed ---
+// The single-band projections of the band-by-band schedule read all bands once
+// per product; a 64 x 64 tile kernel would waste 63/64 of its work there.
+// C(m, n) = alpha sum_k A(k, m) B(k, n) + beta C(m, n), n < NV: one workgroup
+// per row m of C (A column m is contiguous), K spread over its threads
+template <int NV>
+__global__ void __launch_bounds__(256) k_gemv_t(int M, int K, double alpha,
+                                                const double* __restrict__ A, int lda,
+                                                const double* __restrict__ B, int ldb, double beta,
+                                                double* __restrict__ C, int ldc) {
+  __shared__ double red[NV][4];
+  const int m = blockIdx.x;
+  const double* a = A + (size_t)m * lda;
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  for (int k = threadIdx.x; k < K; k += 256) {
+    const double av = a[k];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] += av * B[(size_t)v * ldb + k];
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    double x = acc[v];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    if (lane == 0) red[v][wv] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    const int v = threadIdx.x;
+    const double sum = red[v][0] + red[v][1] + red[v][2] + red[v][3];
+    double* cp = C + (size_t)v * ldc + m;
+    *cp = beta == 0.0 ? alpha * sum : beta * *cp + alpha * sum;
+  }
+}
+
+// C(m, n) += alpha sum_k A(m, k) B(k, n), n < NV (C already scaled by beta):
+// threads over m (coalesced along the columns of A), K split over blockIdx.y
+// and combined with FP64 atomics
+template <int NV>
+__global__ void __launch_bounds__(256) k_gemv_n(int M, int K, int kchunk, double alpha,
+                                                const double* __restrict__ A, int lda,
+                                                const double* __restrict__ B, int ldb,
+                                                double* __restrict__ C, int ldc) {
+  const int m = blockIdx.x * 256 + threadIdx.x;
+  const int k0 = blockIdx.y * kchunk, k1 = min(K, k0 + kchunk);
+  if (m >= M) return;
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  for (int k = k0; k < k1; ++k) {
+    const double av = A[(size_t)k * lda + m];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] += av * B[(size_t)v * ldb + k];
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) atomicAdd(&C[(size_t)v * ldc + m], alpha * acc[v]);
+}
+
+template <int NV>
+static void launch_gemv_t(Ctx* c, int M, int K, double alpha, const double* A, int lda,
+                          const double* B, int ldb, double beta, double* C, int ldc) {
+  hipLaunchKernelGGL(k_gemv_t<NV>, dim3(M), dim3(256), 0, c->stream, M, K, alpha, A, lda, B, ldb,
+                     beta, C, ldc);
+}
+
+template <int NV>
+static void launch_gemv_n(Ctx* c, int M, int K, double alpha, const double* A, int lda,
+                          const double* B, int ldb, double* C, int ldc) {
+  // about 4 workgroups per CU in total
+  const int mb = (int)cdiv(M, 256);
+  const int ks = std::max(1, std::min(K, 1024 / std::max(1, mb)));
+  const int kchunk = (int)cdiv(K, ks);
+  hipLaunchKernelGGL(k_gemv_n<NV>, dim3(mb, cdiv(K, kchunk)), dim3(256), 0, c->stream, M, K, kchunk,
+                     alpha, A, lda, B, ldb, C, ldc);
+}
+
+static int gemv_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
+                   const double* B, int ldb, double beta, double* C, int ldc) {
+  switch (N) {
+    case 1: launch_gemv_t<1>(c, M, K, alpha, A, lda, B, ldb, beta, C, ldc); break;
+    case 2: launch_gemv_t<2>(c, M, K, alpha, A, lda, B, ldb, beta, C, ldc); break;
+    case 3: launch_gemv_t<3>(c, M, K, alpha, A, lda, B, ldb, beta, C, ldc); break;
+    default: launch_gemv_t<4>(c, M, K, alpha, A, lda, B, ldb, beta, C, ldc); break;
+  }
+  return 0;
+}
+
+static int gemv_nn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
+                   const double* B, int ldb, double beta, double* C, int ldc) {
+  if (beta != 1.0)
+    hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N,
+                       ldc, beta);
+  switch (N) {
+    case 1: launch_gemv_n<1>(c, M, K, alpha, A, lda, B, ldb, C, ldc); break;
+    case 2: launch_gemv_n<2>(c, M, K, alpha, A, lda, B, ldb, C, ldc); break;
+    case 3: launch_gemv_n<3>(c, M, K, alpha, A, lda, B, ldb, C, ldc); break;
+    default: launch_gemv_n<4>(c, M, K, alpha, A, lda, B, ldb, C, ldc); break;
+  }
   return 0;
 }
 
