@@ -58,6 +58,40 @@ int qx_set_kb(qx_ctx *ctx, const double *eqlmr, int lmmax, int natom, const int 
               const double *vclalp_atom);
 int qx_info(qx_ctx *ctx, int *out /* {ncol, npair, nH, npj, nplwex} */);
 
+/* ---------------- KB inputs built on the device (once per MD step) --------- */
+/* The nlpp_variables tables setqlm reads (Sources/ppkb.f90:108-191; allocated
+ * at 636-644 and 736-762), column-major with the reference's shapes:
+ * lmx(ntype), clalp(lmmax, ntype), tbqlm/tbqlma(0:mxqlm, lnref, ntype),
+ * dlqlm(ntype), itola/itolm/itolmf/litoll(lmmax, ntype) (litoll as 0/1),
+ * qlmr/qlmi(0:lplh, nqlmmx) for k point 1.  Registered once per cell and
+ * pseudopotential set; fails (-2) when the radial table ends below the
+ * sphere's largest |G|. */
+int qx_set_nlpp_tables(qx_ctx *ctx, int ntype, int lmmax, int lnref, int mxqlm, const int *lmx,
+                       const double *clalp, const double *tbqlm, const double *tbqlma,
+                       const double *dlqlm, const int *itola, const int *itolm,
+                       const int *itolmf, const int *litoll, int nqlmmx, int lplh,
+                       const double *qlmr, const double *qlmi);
+/* stscfn_pw (Sources/pwlda.f90:5711-5854): structure factors ycos/ysin(0:nplwcs,
+ * natom) from ratm(3, natom) (scaled coordinates; real ones when lclust != 0),
+ * hcell(3,3), nd1v/nd1vks(3), kmax1..3 and nga/ngb/ngc(0:nplw5).  The result
+ * stays on the device for qx_setnlc_g; ycos/ysin (host, may be NULL) receive a
+ * copy. */
+int qx_stscfn_pw(qx_ctx *ctx, const double *ratm, int natom, const double *hcell,
+                 const int *nd1v, const int *nd1vks, int lclust, int kmax1, int kmax2,
+                 int kmax3, int nplwcs, int nplw5, const int *nga, const int *ngb,
+                 const int *ngc, double *ycos, double *ysin);
+/* setnlc_g (Sources/nlkbpp.f90:1236-1279) with setbgq / setqlm (2135-2247):
+ * builds eqlmr_ and vclalp = clalp / rvol for the KB types (lkbppi and not
+ * lking, as 0/1 ints) on the device, in place of qx_set_kb's upload.  ycos and
+ * ysin NULL take the device copy of the last qx_stscfn_pw; vclalp(lmmax, ntype)
+ * (may be NULL) receives the scaled energies. */
+int qx_setnlc_g(qx_ctx *ctx, int nplw, int ntype, const int *nhk1, const int *nhk2, int natom,
+                const int *lkbppi, const int *lking, const double *ycos, const double *ysin,
+                int nplwcs, double rvol, double *vclalp);
+/* the context's KB table in the reference layout: eqlmr_(nplwex, lmmax, natom)
+ * and vclalp per atom (lmmax, natom); zero where no projector is held. */
+int qx_get_kb(qx_ctx *ctx, double *eqlmr, double *vclalp_atom);
+
 /* ---------------- device-resident band storage ----------------------------- */
 int qx_slot_put(qx_ctx *ctx, int slot, const double *gdcr, int nb);
 int qx_slot_get(qx_ctx *ctx, int slot, double *gdcr, int nb);

@@ -602,6 +602,139 @@ void qxo_hckbpp_g(const double *eqlmr, int nplwex, int lmmax, int natom, const i
   }
 }
 
+/* stscfn_pw (Sources/pwlda.f90:5711-5854): structure factors
+ *   ycos/ysin(ig, ia) = cos/sin(G_ig . tau_ia)
+ * by the reference's 1-D recurrences.  Per atom and axis a (5754-5834):
+ *   phase = ratm(a, ia) * fact_a, fact_a = 2 pi nd1v(a)/nd1vks(a) (periodic)
+ *   or 2 pi (1/hcell(a,a)) nd1v(a)/nd1vks(a) (cluster);
+ *   c_0 = 1, s_0 = 0, c_1 + i s_1 = cos + i sin(phase),
+ *   c_k = c_1 c_{k-1} - s_1 s_{k-1}, s_k = s_1 c_{k-1} + c_1 s_{k-1},
+ *   c_{-k} = c_k, s_{-k} = -s_k;
+ * then per G (5836-5848): (c1 + i s1)(c2 + i s2) first, times (c3 + i s3).
+ * ycos, ysin: (nplwcs + 1) x natom, column-major; entries past
+ * min(nplwcs, nplw5) are left as they are, like the reference. */
+void qxo_stscfn_pw(const double *ratm, const double *hcell, const int *nd1v, const int *nd1vks,
+                   int lclust, int natom, int kmax1, int kmax2, int kmax3, int nplwcs, int nplw5,
+                   const int *nga, const int *ngb, const int *ngc, double *ycos, double *ysin)
+{
+  const double dpi = 3.141592653589793 * 2.0;
+  const int kmax[3] = {kmax1, kmax2, kmax3};
+  double fact[3];
+  int a, ia, k, ig;
+  for (a = 0; a < 3; ++a) {
+    if (lclust)
+      fact[a] = 1.0 / hcell[4 * a] * (double)nd1v[a] / (double)nd1vks[a];
+    else
+      fact[a] = (double)nd1v[a] / (double)nd1vks[a];
+    fact[a] = fact[a] * dpi;
+  }
+  const int ngmax = nplwcs < nplw5 ? nplwcs : nplw5;
+  for (ia = 0; ia < natom; ++ia) {
+    double *dc[3], *ds[3];
+    for (a = 0; a < 3; ++a) {
+      const int km = kmax[a];
+      double *c = (double *)malloc(sizeof(double) * (2 * km + 1)) + km;
+      double *sn = (double *)malloc(sizeof(double) * (2 * km + 1)) + km;
+      const double ph = ratm[a + 3 * ia] * fact[a];
+      c[0] = 1.0;
+      sn[0] = 0.0;
+      if (km >= 1) {
+        c[1] = cos(ph);
+        sn[1] = sin(ph);
+        c[-1] = c[1];
+        sn[-1] = -sn[1];
+      }
+      for (k = 2; k <= km; ++k) {
+        c[k] = c[1] * c[k - 1] - sn[1] * sn[k - 1];
+        sn[k] = sn[1] * c[k - 1] + c[1] * sn[k - 1];
+        c[-k] = c[k];
+        sn[-k] = -sn[k];
+      }
+      dc[a] = c;
+      ds[a] = sn;
+    }
+    double *yc = ycos + (size_t)(nplwcs + 1) * ia, *ys = ysin + (size_t)(nplwcs + 1) * ia;
+    yc[0] = 1.0;
+    ys[0] = 0.0;
+    for (ig = 1; ig <= ngmax; ++ig) {
+      const int k1 = nga[ig], k2 = ngb[ig], k3 = ngc[ig];
+      const double c12 = dc[0][k1] * dc[1][k2] - ds[0][k1] * ds[1][k2];
+      const double s12 = ds[0][k1] * dc[1][k2] + dc[0][k1] * ds[1][k2];
+      yc[ig] = c12 * dc[2][k3] - s12 * ds[2][k3];
+      ys[ig] = s12 * dc[2][k3] + c12 * ds[2][k3];
+    }
+    for (a = 0; a < 3; ++a) {
+      free(dc[a] - kmax[a]);
+      free(ds[a] - kmax[a]);
+    }
+  }
+}
+
+/* setnlc_g (Sources/nlkbpp.f90:1236-1279) with setbgq (2135-2192) and setqlm
+ * (2197-2247), Gamma point (nk = ikng = 1):
+ *   vclalp(l, it) = clalp(l, it) / rvol for the KB types (lkbppi, not lking);
+ *   per type it, l = 1..lmx(it), G index ig = 0..nplw:
+ *     g = sqrt(recnrm(ig)) (abgvec, Sources/ppkb.f90:785-787),
+ *     m = 2 int(g / (2 dlqlm(it))), d = (g / dlqlm(it) - m) / 2,
+ *     z = d ((d - 1) tbqlma(m, L, it) + tbqlm(m+2, L, it) - tbqlm(m, L, it))
+ *         + tbqlm(m, L, it)                    with L = itola(l, it),
+ *     z *= qlmr(ig, lm) if itolmf(l, it) == 1, else qlmi(ig, lm), lm = itolm(l, it);
+ *   and for the atoms ia = nhk1(it)..nhk2(it):
+ *     litoll(l, it): eqlmr_(2ig+1, l, ia) = ycos(ig, ia) z, eqlmr_(2ig+2, l, ia) = -ysin(ig, ia) z
+ *     otherwise:     eqlmr_(2ig+1, l, ia) = -ysin(ig, ia) z, eqlmr_(2ig+2, l, ia) = -ycos(ig, ia) z.
+ * Arrays column-major with the reference's shapes: tbqlm/tbqlma(0:mxqlm, lnref,
+ * ntype), itola/itolm/itolmf/litoll(lmmax, ntype), clalp/vclalp(lmmax, ntype),
+ * qlmr/qlmi(0:nplw, nqlmmx), ycos/ysin(0:nplwcs, natom), eqlmr(2(nplw+1),
+ * lmmax, natom).  nhk1/nhk2 are 1-based atom indices. */
+void qxo_setnlc_g(int nplw, int ntype, const int *nhk1, const int *nhk2, int natom,
+                  const int *lkbppi, const int *lking, const double *ycos, const double *ysin,
+                  int nplwcs, double rvol, int lmmax, int lnref, int mxqlm, const int *lmx,
+                  const double *clalp, const double *tbqlm, const double *tbqlma,
+                  const double *dlqlm, const int *itola, const int *itolm, const int *itolmf,
+                  const int *litoll, int nqlmmx, const double *qlmr, const double *qlmi,
+                  const double *recnrm, double *eqlmr, double *vclalp)
+{
+  const size_t nex = 2 * (size_t)(nplw + 1), ntb = (size_t)mxqlm + 1;
+  double *z = (double *)malloc(sizeof(double) * (nplw + 1));
+  int it, l, ia, ig;
+  (void)natom;
+  for (it = 0; it < ntype; ++it) {
+    if (!lkbppi[it] || lking[it]) continue;
+    for (l = 0; l < lmx[it]; ++l) vclalp[l + (size_t)lmmax * it] = clalp[l + (size_t)lmmax * it] / rvol;
+    for (l = 0; l < lmx[it]; ++l) {
+      const int L = itola[l + lmmax * it], lm = itolm[l + lmmax * it];
+      const int lmf = itolmf[l + lmmax * it], par = litoll[l + lmmax * it];
+      const double *tb = tbqlm + ntb * ((size_t)(L - 1) + (size_t)lnref * it);
+      const double *ta = tbqlma + ntb * ((size_t)(L - 1) + (size_t)lnref * it);
+      const double *q = (lmf == 1 ? qlmr : qlmi) + (size_t)(nplw + 1) * (lm - 1);
+      const double dl = dlqlm[it];
+      for (ig = 0; ig <= nplw; ++ig) {
+        const double g = sqrt(recnrm[ig]);
+        int m = (int)(g / (2.0 * dl));
+        m = 2 * m;
+        const double d = 0.5 * (g / dl - (double)m);
+        const double zz = d * ((d - 1.0) * ta[m] + tb[m + 2] - tb[m]) + tb[m];
+        z[ig] = zz * q[ig];
+      }
+      (void)nqlmmx;
+      for (ia = nhk1[it] - 1; ia < nhk2[it]; ++ia) {
+        double *e = eqlmr + nex * ((size_t)l + (size_t)lmmax * ia);
+        const double *yc = ycos + (size_t)(nplwcs + 1) * ia, *ys = ysin + (size_t)(nplwcs + 1) * ia;
+        for (ig = 0; ig <= nplw; ++ig) {
+          if (par) {
+            e[2 * ig] = yc[ig] * z[ig];
+            e[2 * ig + 1] = -ys[ig] * z[ig];
+          } else {
+            e[2 * ig] = -ys[ig] * z[ig];
+            e[2 * ig + 1] = -yc[ig] * z[ig];
+          }
+        }
+      }
+    }
+  }
+  free(z);
+}
+
 /* enkbpp_g2 (Sources/nlkbpp.f90:1763-1824), Gamma point. */
 double qxo_enkbpp_g(const double *slmir, int lmmax, int natom, const int *lmx_atom,
                     const double *clalp_atom, const double *occ, int nb, double rvol)

@@ -52,6 +52,16 @@ void qxo_ekinetic(const double *dkgnrm, const double *cgjr, double *rhcr, int np
 void qxo_hckbpp_g(const double *eqlmr, int nplwex, int lmmax, int natom, const int *lmx_atom,
                   const double *vclalp_atom, const double *cgjr, double *rhcr, int nb,
                   double *slmir);
+void qxo_stscfn_pw(const double *ratm, const double *hcell, const int *nd1v, const int *nd1vks,
+                   int lclust, int natom, int kmax1, int kmax2, int kmax3, int nplwcs, int nplw5,
+                   const int *nga, const int *ngb, const int *ngc, double *ycos, double *ysin);
+void qxo_setnlc_g(int nplw, int ntype, const int *nhk1, const int *nhk2, int natom,
+                  const int *lkbppi, const int *lking, const double *ycos, const double *ysin,
+                  int nplwcs, double rvol, int lmmax, int lnref, int mxqlm, const int *lmx,
+                  const double *clalp, const double *tbqlm, const double *tbqlma,
+                  const double *dlqlm, const int *itola, const int *itolm, const int *itolmf,
+                  const int *litoll, int nqlmmx, const double *qlmr, const double *qlmi,
+                  const double *recnrm, double *eqlmr, double *vclalp);
 double qxo_enkbpp_g(const double *slmir, int lmmax, int natom, const int *lmx_atom,
                     const double *clalp_atom, const double *occ, int nb, double rvol);
 

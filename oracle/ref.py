@@ -180,3 +180,46 @@ def enkbpp_g2(nband, natom, occ, rvol):
     L.qxref_enkbpp_g2.argtypes = [C.c_int, C.c_int, _dp, C.c_double]
     L.qxref_enkbpp_g2.restype = C.c_double
     return L.qxref_enkbpp_g2(int(nband), int(natom), _F(occ), float(rvol))
+
+
+def stscfn_pw(ratm, hcell, nd1v, nd1vks, nga, ngb, ngc, nplwcs, kmax, lclust=False):
+    """Reference stscfn_pw; ratm (natom, 3).  Returns ycos, ysin (natom, nplwcs + 1)."""
+    L = lib()
+    r = _F(np.asarray(ratm).T)
+    natom = r.shape[1]
+    ga = _F(nga, np.int32)
+    yc = np.zeros((nplwcs + 1, natom), order="F")
+    ys = np.zeros((nplwcs + 1, natom), order="F")
+    L.qxref_stscfn_pw.argtypes = [_dp, _dp, _ip, _ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  C.c_int, C.c_int, _ip, _ip, _ip, _dp, _dp]
+    L.qxref_stscfn_pw(r, _F(hcell), _F(nd1v, np.int32), _F(nd1vks, np.int32), int(bool(lclust)),
+                      natom, int(kmax[0]), int(kmax[1]), int(kmax[2]), int(nplwcs), len(ga) - 1,
+                      ga, _F(ngb, np.int32), _F(ngc, np.int32), yc, ys)
+    return np.ascontiguousarray(yc.T), np.ascontiguousarray(ys.T)
+
+
+def setnlc_g(t, nhk1, nhk2, natom, ycos, ysin, recnrm, rvol, lkbppi=None, lking=None):
+    """Reference setnlc_g with the nlpp tables t (qxmd_amd.nlpp.NlppTables).
+    Returns eqlmr (natom, lmmax, nplwex), vclalp (ntype, lmmax)."""
+    L = lib()
+    nplw = len(recnrm) - 1
+    ntype, lmmax = t.itola.shape
+    yc = _F(np.asarray(ycos).T)
+    nplwcs = yc.shape[0] - 1
+    lk = np.ones(ntype, np.int32) if lkbppi is None else np.asarray(lkbppi, np.int32)
+    lg = np.zeros(ntype, np.int32) if lking is None else np.asarray(lking, np.int32)
+    eq = np.zeros((2 * (nplw + 1), lmmax, natom), order="F")
+    vcl = np.zeros((lmmax, ntype), order="F")
+    L.qxref_setnlc_g.argtypes = [C.c_int, C.c_int, _ip, _ip, C.c_int, _ip, _ip, _dp, _dp, C.c_int,
+                                 C.c_double, C.c_int, C.c_int, _ip, _dp, _dp, _dp, _dp, _ip, _ip,
+                                 _ip, _ip, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.qxref_setnlc_g(nplw, ntype, _F(nhk1, np.int32), _F(nhk2, np.int32), natom, _F(lk, np.int32),
+                     _F(lg, np.int32), yc, _F(np.asarray(ysin).T), nplwcs, float(rvol), lmmax,
+                     t.tbqlm.shape[1], _F(t.lmx, np.int32), _F(np.asarray(t.clalp).T),
+                     _F(np.transpose(t.tbqlm, (2, 1, 0))), _F(np.transpose(t.tbqlma, (2, 1, 0))),
+                     _F(t.dlqlm), _F(np.asarray(t.itola).T, np.int32),
+                     _F(np.asarray(t.itolm).T, np.int32), _F(np.asarray(t.itolmf).T, np.int32),
+                     _F(np.asarray(t.litoll).T, np.int32), t.qlmr.shape[0],
+                     _F(np.asarray(t.qlmr)[:, :nplw + 1].T), _F(np.asarray(t.qlmi)[:, :nplw + 1].T),
+                     _F(recnrm), eq, vcl)
+    return np.ascontiguousarray(np.transpose(eq, (2, 1, 0))), np.ascontiguousarray(vcl.T)

@@ -333,3 +333,109 @@ function qxref_enkbpp_g2(nband, natom, occ, rvol) bind(C, name="qxref_enkbpp_g2"
 &     lkb, lkg, rv)
   qxref_enkbpp_g2 = e
 end function
+
+! stscfn_pw (Sources/pwlda.f90:5711): structure factors for natom atoms, with
+! the recurrence work arrays sized here; lnlpp_g = .true. so ycos/ysin are set.
+subroutine qxref_stscfn_pw(ratm, hcell, nd1v, nd1vks, lclust_i, natom, kmax1, kmax2, kmax3, &
+&   nplwcs, nplw5, nga, ngb, ngc, ycos, ysin) bind(C, name="qxref_stscfn_pw")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: lclust_i, natom, kmax1, kmax2, kmax3, nplwcs, nplw5
+  real(c_double) :: ratm(3, natom), hcell(3, 3)
+  integer(c_int) :: nd1v(3), nd1vks(3), nga(0:nplw5), ngb(0:nplw5), ngc(0:nplw5)
+  real(c_double) :: ycos(0:nplwcs, natom), ysin(0:nplwcs, natom)
+  real*8, allocatable :: dcq1(:,:), dcq2(:,:), dcq3(:,:), dsq1(:,:), dsq2(:,:), dsq3(:,:)
+  integer :: nfile(2), nd1(3), nd2(3)
+  logical :: lcl
+  nfile = 6
+  nd1 = nd1v
+  nd2 = nd1vks
+  lcl = lclust_i /= 0
+  allocate(dcq1(-kmax1:kmax1, natom), dsq1(-kmax1:kmax1, natom))
+  allocate(dcq2(-kmax2:kmax2, natom), dsq2(-kmax2:kmax2, natom))
+  allocate(dcq3(-kmax3:kmax3, natom), dsq3(-kmax3:kmax3, natom))
+  call stscfn_pw(nfile, 0, 1, ycos, ysin, dcq1, dcq2, dcq3, dsq1, dsq2, dsq3, &
+&     kmax1, kmax2, kmax3, natom, nplwcs, nplw5, nga, ngb, ngc, &
+&     lcl, ratm, hcell, nd1, nd2, .true., .false., .false., .false.)
+  deallocate(dcq1, dsq1, dcq2, dsq2, dcq3, dsq3)
+end subroutine
+
+! setnlc_g (Sources/nlkbpp.f90:1236) for ntype types: the nlpp_variables /
+! nl_variables state it reads (tables of the nonlocal term, angular factors,
+! abgvec = sqrt(recnrm)) is set from the arguments, then the reference routine
+! runs unchanged; eqlmr_ and vclalp are copied out.
+subroutine qxref_setnlc_g(nplw, ntype, nhk1, nhk2, natom, lkbppi_i, lking_i, ycos, ysin, &
+&   nplwcs, rvol, lmmax_in, lnref_in, lmx_t, clalp_t, tbqlm_t, tbqlma_t, dlqlm_t, itola_t, &
+&   itolm_t, itolmf_t, litoll_i, nqlmmx_in, qlmr_t, qlmi_t, recnrm, eq_out, vcl_out) &
+&   bind(C, name="qxref_setnlc_g")
+  use iso_c_binding
+  use nlpp_parameters, only: lmmax, lnref, nqlmmx
+  use nlpp_variables, only: lmx, clalp, tbqlm, tbqlma, dlqlm, itola, itolm, itolmf, litoll, &
+&     qlmr, qlmi, abgvec, mxqlm
+  use nl_variables, only: eqlmr_, vclalp, zzzr, zzzi, lbigmm2
+  implicit none
+  integer(c_int), value :: nplw, ntype, natom, nplwcs, lmmax_in, lnref_in, nqlmmx_in
+  integer(c_int) :: nhk1(ntype), nhk2(ntype), lkbppi_i(ntype), lking_i(ntype), lmx_t(ntype)
+  real(c_double) :: ycos(0:nplwcs, natom), ysin(0:nplwcs, natom)
+  real(c_double), value :: rvol
+  real(c_double) :: clalp_t(lmmax_in, ntype), dlqlm_t(ntype)
+  real(c_double) :: tbqlm_t(0:550, lnref_in, ntype), tbqlma_t(0:550, lnref_in, ntype)
+  integer(c_int) :: itola_t(lmmax_in, ntype), itolm_t(lmmax_in, ntype), itolmf_t(lmmax_in, ntype)
+  integer(c_int) :: litoll_i(lmmax_in, ntype)
+  real(c_double) :: qlmr_t(0:nplw, nqlmmx_in), qlmi_t(0:nplw, nqlmmx_in), recnrm(0:nplw)
+  real(c_double) :: eq_out(2 * (nplw + 1), lmmax_in, natom), vcl_out(lmmax_in, ntype)
+  integer :: nfile(2), n1(ntype), n2(ntype), i
+  logical :: lkb(ntype), lkg(ntype)
+  real*8 :: rv
+  nfile = 6
+  lmmax = lmmax_in
+  lnref = lnref_in
+  nqlmmx = nqlmmx_in
+  if (allocated(lmx)) deallocate(lmx)
+  if (allocated(clalp)) deallocate(clalp)
+  if (allocated(tbqlm)) deallocate(tbqlm)
+  if (allocated(tbqlma)) deallocate(tbqlma)
+  if (allocated(dlqlm)) deallocate(dlqlm)
+  if (allocated(itola)) deallocate(itola)
+  if (allocated(itolm)) deallocate(itolm)
+  if (allocated(itolmf)) deallocate(itolmf)
+  if (allocated(litoll)) deallocate(litoll)
+  if (allocated(qlmr)) deallocate(qlmr)
+  if (allocated(qlmi)) deallocate(qlmi)
+  if (allocated(abgvec)) deallocate(abgvec)
+  if (allocated(eqlmr_)) deallocate(eqlmr_)
+  if (allocated(vclalp)) deallocate(vclalp)
+  if (allocated(zzzr)) deallocate(zzzr)
+  if (allocated(zzzi)) deallocate(zzzi)
+  allocate(lmx(ntype), clalp(lmmax, ntype), dlqlm(ntype))
+  allocate(tbqlm(0:mxqlm, lnref, ntype), tbqlma(0:mxqlm, lnref, ntype))
+  allocate(itola(lmmax, ntype), itolm(lmmax, ntype), itolmf(lmmax, ntype), litoll(lmmax, ntype))
+  allocate(qlmr(0:nplw, nqlmmx, 1, 1), qlmi(0:nplw, nqlmmx, 1, 1), abgvec(0:nplw, 1))
+  allocate(eqlmr_(2 * (nplw + 1), lmmax, natom), vclalp(lmmax, ntype))
+  allocate(zzzr(0:nplw), zzzi(0:nplw))
+  lmx = lmx_t
+  clalp = clalp_t
+  dlqlm = dlqlm_t
+  tbqlm = tbqlm_t
+  tbqlma = tbqlma_t
+  itola = itola_t
+  itolm = itolm_t
+  itolmf = itolmf_t
+  litoll = litoll_i /= 0
+  qlmr(:, :, 1, 1) = qlmr_t
+  qlmi(:, :, 1, 1) = qlmi_t
+  do i = 0, nplw
+    abgvec(i, 1) = sqrt(recnrm(i))
+  end do
+  eqlmr_ = 0.d0
+  vclalp = 0.d0
+  lbigmm2 = .true.
+  n1 = nhk1
+  n2 = nhk2
+  lkb = lkbppi_i /= 0
+  lkg = lking_i /= 0
+  rv = rvol
+  call setnlc_g(nfile, 0, 1, nplw, ntype, n1, n2, natom, lkb, lkg, ycos, ysin, nplwcs, rv)
+  eq_out = eqlmr_
+  vcl_out = vclalp
+end subroutine

@@ -272,6 +272,7 @@ int qx_destroy(qx_ctx* h) {
   for (auto& t : c->lt)
     for (auto& e : t.ev) (void)hipEventDestroy(e);
   lt_destroy(c);
+  nl_destroy(c);
   (void)hipStreamDestroy(c->stream);
   delete h;
   return 0;
@@ -344,6 +345,67 @@ int qx_set_kb(qx_ctx* h, const double* eqlmr, int lmmax, int natom, const int* l
   QX_CHECK(hipMemcpyAsync(c->d_D, D.data(), sizeof(double) * npj, hipMemcpyHostToDevice, c->stream));
   QX_CHECK(hipStreamSynchronize(c->stream));
   QX_CHECK(hipFree(stage));
+  return 0;
+}
+
+int qx_set_nlpp_tables(qx_ctx* h, int ntype, int lmmax, int lnref, int mxqlm, const int* lmx,
+                       const double* clalp, const double* tbqlm, const double* tbqlma,
+                       const double* dlqlm, const int* itola, const int* itolm, const int* itolmf,
+                       const int* litoll, int nqlmmx, int lplh, const double* qlmr,
+                       const double* qlmi) {
+  const int rc = nl_set_tables(&h->c, ntype, lmmax, lnref, mxqlm, lmx, clalp, tbqlm, tbqlma,
+                               dlqlm, itola, itolm, itolmf, litoll, nqlmmx, lplh, qlmr, qlmi);
+  if (rc == -2) return fail(rc, "qx_set_nlpp_tables: the radial table ends below the sphere's |G|");
+  if (rc) return fail(rc, "qx_set_nlpp_tables: bad table shapes or indices");
+  return 0;
+}
+
+int qx_stscfn_pw(qx_ctx* h, const double* ratm, int natom, const double* hcell, const int* nd1v,
+                 const int* nd1vks, int lclust, int kmax1, int kmax2, int kmax3, int nplwcs,
+                 int nplw5, const int* nga, const int* ngb, const int* ngc, double* ycos,
+                 double* ysin) {
+  const int rc = nl_stscfn(&h->c, ratm, natom, hcell, nd1v, nd1vks, lclust, kmax1, kmax2, kmax3,
+                           nplwcs, nplw5, nga, ngb, ngc, ycos, ysin);
+  if (rc == -2) return fail(rc, "qx_stscfn_pw: a G index exceeds kmax");
+  if (rc) return fail(rc, "qx_stscfn_pw: bad arguments");
+  return 0;
+}
+
+int qx_setnlc_g(qx_ctx* h, int nplw, int ntype, const int* nhk1, const int* nhk2, int natom,
+                const int* lkbppi, const int* lking, const double* ycos, const double* ysin,
+                int nplwcs, double rvol, double* vclalp) {
+  const int rc = nl_setnlc_g(&h->c, nplw, ntype, nhk1, nhk2, natom, lkbppi, lking, ycos, ysin,
+                             nplwcs, rvol, vclalp);
+  if (rc == -4) return fail(rc, "qx_setnlc_g: nonlocal tables not registered for ntype types");
+  if (rc == -5) return fail(rc, "qx_setnlc_g: no device structure factors of this shape");
+  if (rc) return fail(rc, "qx_setnlc_g: bad arguments");
+  return 0;
+}
+
+// the KB table in the reference layout: eqlmr(nplwex, lmmax, natom) (zero for
+// projectors the context does not hold) and vclalp per atom (lmmax, natom)
+int qx_get_kb(qx_ctx* h, double* eqlmr, double* vclalp_atom) {
+  Ctx* c = &h->c;
+  const size_t n = (size_t)c->nplwex;
+  std::memset(eqlmr, 0, sizeof(double) * n * c->lmmax * c->natom);
+  if (vclalp_atom) std::memset(vclalp_atom, 0, sizeof(double) * c->lmmax * c->natom);
+  if (c->npj == 0) return 0;
+  double* stage = nullptr;
+  QX_CHECK(hipMalloc(&stage, sizeof(double) * n * c->npj));
+  hipLaunchKernelGGL(k_to_reference, dim3(cdiv((long)(c->nplw + 1) * c->npj, 256)), dim3(256), 0,
+                     c->stream, c->d_B, stage, c->d_perm, c->nplw + 1, c->npj);
+  std::vector<double> B(n * c->npj), D(c->npj);
+  QX_CHECK(hipMemcpyAsync(B.data(), stage, sizeof(double) * B.size(), hipMemcpyDeviceToHost,
+                          c->stream));
+  QX_CHECK(hipMemcpyAsync(D.data(), c->d_D, sizeof(double) * D.size(), hipMemcpyDeviceToHost,
+                          c->stream));
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  QX_CHECK(hipFree(stage));
+  for (int p = 0; p < c->npj; ++p) {
+    const size_t slot = (size_t)c->proj_l[p] + (size_t)c->lmmax * c->proj_atom[p];
+    std::memcpy(eqlmr + n * slot, &B[n * p], sizeof(double) * n);
+    if (vclalp_atom) vclalp_atom[slot] = D[p];
+  }
   return 0;
 }
 

@@ -75,6 +75,18 @@ struct Ctx {
     bool next_cont = false;
     size_t used = 0;
   } lt[kTimers];
+  // nonlocal-pseudopotential tables of setqlm (qx_set_nlpp_tables) and the
+  // structure factors of the last qx_stscfn_pw, on the device
+  struct Nlpp {
+    int ntype = 0, lmmax = 0, lnref = 0, mxqlm = 0, nqlmmx = 0, lplh = 0;
+    std::vector<int> lmx, itola, itolm, itolmf, litoll;  // (lmmax x ntype) column-major
+    std::vector<double> clalp, dlqlm;
+    double *d_tbqlm = nullptr, *d_tbqlma = nullptr;      // (mxqlm+1) x lnref x ntype
+    double *d_qlm = nullptr;                             // qlmr then qlmi, (lplh+1) x nqlmmx each
+  } nl;
+  double *d_ycos = nullptr, *d_ysin = nullptr;  // (nplwcs+1) x natom
+  int sf_natom = 0, sf_nplwcs = 0;
+  double gmax2 = 0.0;                           // largest |G|^2 of the sphere
   int lt_gemm_id = -1;                    // timer the GEMM launches report to
   void* blaslt = nullptr;                 // hipBLASLt state (handle, plans, FP32 copies)
 };
@@ -115,6 +127,20 @@ int kb_mixed_f32(Ctx* c, const double* psi, const float* psi32, int nb, float* k
 // the FP32 copy of psi comes out of the z inverse pass and the KB term goes
 // into the z forward pass.  Returns -3 when the shape is not covered.
 int hpsi_mixed_fused(Ctx* c, const double* psi, double* out, int nb);
+// per-MD-step KB inputs on the device (nlsetup.hip)
+int nl_set_tables(Ctx* c, int ntype, int lmmax, int lnref, int mxqlm, const int* lmx,
+                  const double* clalp, const double* tbqlm, const double* tbqlma,
+                  const double* dlqlm, const int* itola, const int* itolm, const int* itolmf,
+                  const int* litoll, int nqlmmx, int lplh, const double* qlmr,
+                  const double* qlmi);
+int nl_stscfn(Ctx* c, const double* ratm, int natom, const double* hcell, const int* nd1v,
+              const int* nd1vks, int lclust, int kmax1, int kmax2, int kmax3, int nplwcs,
+              int nplw5, const int* nga, const int* ngb, const int* ngc, double* ycos_out,
+              double* ysin_out);
+int nl_setnlc_g(Ctx* c, int nplw, int ntype, const int* nhk1, const int* nhk2, int natom,
+                const int* lkbppi, const int* lking, const double* ycos, const double* ysin,
+                int nplwcs, double rvol, double* vclalp_out);
+void nl_destroy(Ctx* c);
 int coop_launch(Ctx* c, const void* fn, int want, size_t lds, void** args, bool exact);
 int dense_chol_inv(Ctx* c, double* s, int n, double* x, int* ok, int* flag);
 int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,

@@ -112,6 +112,8 @@ int pencil_build(Ctx* c, const int* nga, const int* ngb, const int* ngc, const d
   if ((rc = up(&c->d_col_kmod, P.col_kmod))) return rc;
   std::vector<double> kin(nplw + 1);
   for (int p = 0; p <= nplw; ++p) kin[p] = recnrm[P.perm[p]];
+  c->gmax2 = 0.0;
+  for (int p = 0; p <= nplw; ++p) c->gmax2 = std::max(c->gmax2, recnrm[p]);
   QX_CHECK(hipMalloc(&c->d_kin, sizeof(double) * kin.size()));
   QX_CHECK(hipMemcpy(c->d_kin, kin.data(), sizeof(double) * kin.size(), hipMemcpyHostToDevice));
   std::vector<double> tw(2 * (c->n1 + c->n2 + c->n3));

@@ -50,6 +50,12 @@ SIGNATURES = {
     "qx_set_glocal": [_vp, _dp],
     "qx_set_kb": [_vp, _dp, _i, _i, _ip, _dp],
     "qx_info": [_vp, _ip],
+    "qx_set_nlpp_tables": [_vp, _i, _i, _i, _i, _ip, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _ip, _i,
+                           _i, _dp, _dp],
+    "qx_stscfn_pw": [_vp, _dp, _i, _dp, _ip, _ip, _i, _i, _i, _i, _i, _i, _ip, _ip, _ip, _vp,
+                     _vp],
+    "qx_setnlc_g": [_vp, _i, _i, _ip, _ip, _i, _ip, _ip, _vp, _vp, _i, _d, _vp],
+    "qx_get_kb": [_vp, _dp, _vp],
     "qx_slot_put": [_vp, _i, _dp, _i],
     "qx_slot_get": [_vp, _i, _dp, _i],
     "qx_slot_copy": [_vp, _i, _i, _i],

@@ -98,6 +98,63 @@ class PlaneWave:
         load().qx_info(self.h, info)
         self.npj = int(info[3])
 
+    # ---- KB inputs built on the device (stscfn_pw, setnlc_g)
+    def set_nlpp_tables(self, t):
+        """Register the nonlocal tables (qxmd_amd.nlpp.NlppTables) setqlm reads."""
+        ntype, lmmax = t.itola.shape
+        self._nl_lmmax = lmmax
+        qr, qi = f64(t.qlmr), f64(t.qlmi)
+        check(load().qx_set_nlpp_tables(
+            self.h, ntype, lmmax, t.tbqlm.shape[1], t.tbqlm.shape[2] - 1, i32(t.lmx), f64(t.clalp),
+            f64(t.tbqlm), f64(t.tbqlma), f64(t.dlqlm), i32(t.itola), i32(t.itolm), i32(t.itolmf),
+            i32(t.litoll), qr.shape[0], qr.shape[1] - 1, qr, qi), "qx_set_nlpp_tables")
+
+    def stscfn_pw(self, ratm, hcell, nd1v, nd1vks, nga, ngb, ngc, nplwcs, kmax, lclust=False,
+                  copy_out=True):
+        """Structure factors on the device (kept for setnlc_g); ratm (natom, 3).
+        Returns ycos, ysin (natom, nplwcs + 1) when copy_out."""
+        r = f64(ratm)
+        natom = r.shape[0]
+        ga = i32(nga)
+        yc = np.zeros((natom, nplwcs + 1)) if copy_out else None
+        ys = np.zeros((natom, nplwcs + 1)) if copy_out else None
+        check(load().qx_stscfn_pw(
+            self.h, r, natom, f64(np.asarray(hcell).T), i32(nd1v), i32(nd1vks), int(bool(lclust)),
+            int(kmax[0]), int(kmax[1]), int(kmax[2]), int(nplwcs), len(ga) - 1, ga, i32(ngb),
+            i32(ngc), yc.ctypes.data_as(C.c_void_p) if copy_out else None,
+            ys.ctypes.data_as(C.c_void_p) if copy_out else None), "qx_stscfn_pw")
+        return (yc, ys) if copy_out else None
+
+    def setnlc_g(self, ntype, nhk1, nhk2, natom, rvol, ycos=None, ysin=None, lkbppi=None,
+                 lking=None, nplwcs=None):
+        """KB table on the device from the registered tables and the structure
+        factors (the device copy of the last stscfn_pw when ycos is None).
+        Returns vclalp (ntype, lmmax)."""
+        lk = np.ones(ntype, np.int32) if lkbppi is None else i32(lkbppi)
+        lg = np.zeros(ntype, np.int32) if lking is None else i32(lking)
+        if ycos is not None:
+            yc, ys = f64(ycos), f64(ysin)
+            nplwcs = yc.shape[1] - 1
+            pc, ps = yc.ctypes.data_as(C.c_void_p), ys.ctypes.data_as(C.c_void_p)
+        else:
+            pc = ps = None
+            nplwcs = self.gs.nplw if nplwcs is None else nplwcs
+        vcl = np.zeros((ntype, getattr(self, "_nl_lmmax", 0) or 1))
+        check(load().qx_setnlc_g(self.h, self.gs.nplw, ntype, i32(nhk1), i32(nhk2), natom, lk, lg,
+                                 pc, ps, int(nplwcs), float(rvol),
+                                 vcl.ctypes.data_as(C.c_void_p)), "qx_setnlc_g")
+        info = np.zeros(5, np.int32)
+        load().qx_info(self.h, info)
+        self.npj = int(info[3])
+        return vcl
+
+    def get_kb(self, natom, lmmax):
+        """KB table in the reference layout: eqlmr (natom, lmmax, nplwex), vclalp (natom, lmmax)."""
+        eq = np.zeros((natom, lmmax, self.gs.nplwex))
+        vcl = np.zeros((natom, lmmax))
+        check(load().qx_get_kb(self.h, eq, vcl.ctypes.data_as(C.c_void_p)), "qx_get_kb")
+        return eq, vcl
+
     def info(self):
         info = np.zeros(5, np.int32)
         load().qx_info(self.h, info)

@@ -29,6 +29,27 @@ class Sphere:
         self.N = int(np.prod(self.nd))
 
 
+def nlpp_case(sp, ntype=2):
+    """Inputs of stscfn_pw / setnlc_g for a Sphere: fractional atom positions
+    (periodic cell, FD mesh = FFT mesh), kmax per axis over the nplw5 G list,
+    s + p projector tables for ntype types splitting the atoms in equal runs."""
+    from qxmd_amd import nlpp
+    cell = sp.cell
+    frac = np.linalg.solve(cell.hcell, cell.tau.T).T
+    n5 = sp.nplw5 + 1
+    g = [np.asarray(a[:n5], np.int32) for a in (sp.nga, sp.ngb, sp.ngc)]
+    kmax = [int(np.abs(a).max()) for a in g]
+    nd = np.array(sp.nd, np.int32)
+    nw = sp.nplw + 1
+    t = nlpp.model_tables(sp.gx[:nw], sp.gy[:nw], sp.gz[:nw], ntype=ntype)
+    natom = len(frac)
+    per = natom // ntype
+    nhk1 = np.array([1 + per * it for it in range(ntype)], np.int32)
+    nhk2 = np.array([per * (it + 1) if it < ntype - 1 else natom for it in range(ntype)], np.int32)
+    return dict(frac=frac, nd=nd, g=g, kmax=kmax, tables=t, nhk1=nhk1, nhk2=nhk2, natom=natom,
+                recnrm=np.asarray(sp.recnrm[:nw]), nplwcs=sp.nplw)
+
+
 _SPHERES = {}
 
 

@@ -101,6 +101,15 @@ def main():
         v = np.asarray(v_ref)
         out[f"rseigqr_{n}_v"] = v * np.sign(v[np.argmax(np.abs(v), axis=0), np.arange(n)])
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_vectors.npz")
+    # stscfn_pw and setnlc_g (two projector types) on Si8
+    c = conftest.nlpp_case(si8)
+    yc, ys = R.stscfn_pw(c["frac"], si8.cell.hcell, c["nd"], c["nd"], *c["g"], c["nplwcs"], c["kmax"])
+    out["stscfn_ycos"] = np.asarray(yc)[:, ::SUB]
+    out["stscfn_ysin"] = np.asarray(ys)[:, ::SUB]
+    eqr, vclr = R.setnlc_g(c["tables"], c["nhk1"], c["nhk2"], c["natom"], yc, ys, c["recnrm"],
+                           si8.cell.volume)
+    out["setnlc_eqlmr"] = np.asarray(eqr)[:, :, ::SUB]
+    out["setnlc_vclalp"] = np.asarray(vclr)
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes,", len(out), "arrays")
 

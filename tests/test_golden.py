@@ -101,3 +101,23 @@ def test_golden_rseigqr():
         np.testing.assert_allclose(w, G[f"rseigqr_{n}_w"], rtol=1e-11, atol=1e-12)
         v = v * np.sign(v[np.argmax(np.abs(v), axis=0), np.arange(n)])
         np.testing.assert_allclose(v, G[f"rseigqr_{n}_v"], atol=1e-9)
+
+
+def test_golden_stscfn_setnlc(si8):
+    """Structure factors and the KB table: the oracle against the reference's
+    stscfn_pw / setnlc_g (bit-equal here; 1e-14 allowed for a libm with other
+    cos/sin roundings), and the structure factors against cos/sin(G . tau)."""
+    import conftest
+    c = conftest.nlpp_case(si8)
+    yc, ys = O.stscfn_pw(c["frac"], si8.cell.hcell, c["nd"], c["nd"], *c["g"], c["nplwcs"],
+                         c["kmax"])
+    np.testing.assert_allclose(yc[:, ::SUB], G["stscfn_ycos"], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(ys[:, ::SUB], G["stscfn_ysin"], rtol=0, atol=1e-14)
+    gv = 2 * np.pi * np.stack([a[: si8.nplw + 1] for a in c["g"]], 1)
+    ph = c["frac"] @ gv.T
+    assert np.abs(yc - np.cos(ph)).max() < 1e-13
+    assert np.abs(ys - np.sin(ph)).max() < 1e-13
+    eq, vcl = O.setnlc_g(c["tables"], c["nhk1"], c["nhk2"], c["natom"], yc, ys, c["recnrm"],
+                         si8.cell.volume)
+    np.testing.assert_allclose(eq[:, :, ::SUB], G["setnlc_eqlmr"], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(vcl, G["setnlc_vclalp"], rtol=0, atol=0)

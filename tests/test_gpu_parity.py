@@ -396,3 +396,47 @@ def test_kb_mixed_gemm_accuracy(si64, mode, monkeypatch):
     err = _relerr(kb_got, kb_ref)
     print(mode, "KB rel err", err)
     assert err < (5e-6 if mode == "f32" else 3e-5)
+
+
+@pytest.mark.parametrize("host_sf", [False, True])
+def test_stscfn_setnlc_device(si64, host_sf):
+    """Structure factors and the KB table built on the device (stscfn_pw,
+    setnlc_g) equal the oracle's, which equals the reference's; a band-apply
+    with that table matches the oracle Hamiltonian built from the oracle's."""
+    import conftest
+    c = conftest.nlpp_case(si64, ntype=2)
+    t = c["tables"]
+    nb = 6
+    gs = pw.GSphere(si64.cell.hcell)
+    ctx = pw.PlaneWave(gs, nb)
+    v = model_local_potential(si64, si64.cell)
+    ctx.set_glocal(v)
+    ctx.set_nlpp_tables(t)
+    yo, so = O.stscfn_pw(c["frac"], si64.cell.hcell, c["nd"], c["nd"], *c["g"], c["nplwcs"],
+                         c["kmax"])
+    yg, sg = ctx.stscfn_pw(c["frac"], si64.cell.hcell, c["nd"], c["nd"], *c["g"], c["nplwcs"],
+                           c["kmax"])
+    np.testing.assert_array_equal(yg, yo)
+    np.testing.assert_array_equal(sg, so)
+    lking = np.array([0, 1], np.int32) if host_sf else None
+    eo, vo = O.setnlc_g(t, c["nhk1"], c["nhk2"], c["natom"], yo, so, c["recnrm"],
+                        si64.cell.volume, lking=lking)
+    if host_sf:
+        vg = ctx.setnlc_g(2, c["nhk1"], c["nhk2"], c["natom"], si64.cell.volume, ycos=yo, ysin=so,
+                          lking=lking)
+    else:
+        vg = ctx.setnlc_g(2, c["nhk1"], c["nhk2"], c["natom"], si64.cell.volume)
+    np.testing.assert_array_equal(vg, vo)
+    lmmax = t.lmmax
+    eg, vclg = ctx.get_kb(c["natom"], lmmax)
+    np.testing.assert_array_equal(eg, eo)
+    types = np.repeat(np.arange(2), c["natom"] // 2)
+    lmx_atom = np.array([0 if (host_sf and types[ia] == 1) else t.lmx[types[ia]]
+                         for ia in range(c["natom"])], np.int32)
+    vcl_atom = vo[types]
+    np.testing.assert_array_equal(vclg, vcl_atom * (lmx_atom[:, None] > 0))
+    H = O.Hamiltonian(si64.nplw, si64.nd, si64.ijk, v, si64.dkg, eo, lmx_atom, vcl_atom)
+    psi = initial_wavefunctions(si64, nb)
+    ctx.put(0, psi)
+    ctx.hpsi(0, 1, nb)
+    assert _relerr(ctx.get(1, nb), H.hpsi(psi)) < 1e-12

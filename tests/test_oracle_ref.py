@@ -132,3 +132,27 @@ def test_eigh_vs_rseigqr():
         # eigenvectors agree up to sign (eigenvalues are simple here)
         dots = np.abs(np.sum(v * v_ref, axis=0))
         np.testing.assert_allclose(dots, 1.0, atol=1e-9)
+
+
+@pytest.mark.parametrize("lclust", [False, True])
+def test_stscfn_setnlc_vs_reference(si64, lclust):
+    """stscfn_pw (periodic and cluster coordinates) and setnlc_g with one KB
+    type, one type skipped (lking) and an imaginary angular row: the oracle
+    against the reference routines."""
+    import conftest
+    c = conftest.nlpp_case(si64, ntype=3)
+    ratm = c["frac"] * (si64.cell.hcell.diagonal() if lclust else 1.0)
+    args = (ratm, si64.cell.hcell, c["nd"], c["nd"], *c["g"], c["nplwcs"], c["kmax"])
+    yo, so = O.stscfn_pw(*args, lclust=lclust)
+    yr, sr = R.stscfn_pw(*args, lclust=lclust)
+    assert np.abs(yo - yr).max() < 1e-14 and np.abs(so - sr).max() < 1e-14
+    t = c["tables"]
+    t.itolmf[1, 2] = 2  # an angular factor from qlmi
+    t.qlmi[2] = 0.5 * t.qlmr[2]
+    lking = np.array([0, 1, 0], np.int32)
+    eo, vo = O.setnlc_g(t, c["nhk1"], c["nhk2"], c["natom"], yo, so, c["recnrm"],
+                        si64.cell.volume, lking=lking)
+    er, vr = R.setnlc_g(t, c["nhk1"], c["nhk2"], c["natom"], yo, so, c["recnrm"],
+                        si64.cell.volume, lking=lking)
+    assert np.abs(eo - er).max() < 1e-14 and np.abs(vo - vr).max() == 0.0
+    assert np.abs(er).max() > 0.1
