@@ -82,6 +82,43 @@ integer(c_int) function qx_set_kb(ctx, eqlmr, lmmax, natom, lmx_atom, vclalp_ato
   real(c_double), intent(in) :: vclalp_atom(*)
 end function
 
+integer(c_int) function qx_set_nlpp_tables(ctx, ntype, lmmax, lnref, mxqlm, lmx, clalp, tbqlm,  &
+&     tbqlma, dlqlm, itola, itolm, itolmf, litoll, nqlmmx, lplh, qlmr, qlmi)  &
+&     bind(C, name="qx_set_nlpp_tables")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: ntype, lmmax, lnref, mxqlm, nqlmmx, lplh
+  integer(c_int), intent(in) :: lmx(*), itola(*), itolm(*), itolmf(*), litoll(*)
+  real(c_double), intent(in) :: clalp(*), tbqlm(*), tbqlma(*), dlqlm(*), qlmr(*), qlmi(*)
+end function
+
+integer(c_int) function qx_stscfn_pw(ctx, ratm, natom, hcell, nd1v, nd1vks, lclust, kmax1,  &
+&     kmax2, kmax3, nplwcs, nplw5, nga, ngb, ngc, ycos, ysin) bind(C, name="qx_stscfn_pw")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_double), intent(in) :: ratm(*), hcell(*)
+  integer(c_int), value :: natom, lclust, kmax1, kmax2, kmax3, nplwcs, nplw5
+  integer(c_int), intent(in) :: nd1v(3), nd1vks(3), nga(*), ngb(*), ngc(*)
+  type(c_ptr), value :: ycos, ysin
+end function
+
+integer(c_int) function qx_setnlc_g(ctx, nplw, ntype, nhk1, nhk2, natom, lkbppi, lking, ycos,  &
+&     ysin, nplwcs, rvol, vclalp) bind(C, name="qx_setnlc_g")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: nplw, ntype, natom, nplwcs
+  integer(c_int), intent(in) :: nhk1(*), nhk2(*), lkbppi(*), lking(*)
+  type(c_ptr), value :: ycos, ysin, vclalp
+  real(c_double), value :: rvol
+end function
+
+integer(c_int) function qx_get_kb(ctx, eqlmr, vclalp_atom) bind(C, name="qx_get_kb")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_double), intent(out) :: eqlmr(*)
+  type(c_ptr), value :: vclalp_atom
+end function
+
 integer(c_int) function qx_info(ctx, out) bind(C, name="qx_info")
   import :: c_int, c_ptr
   type(c_ptr), value :: ctx

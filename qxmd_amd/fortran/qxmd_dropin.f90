@@ -132,6 +132,51 @@ subroutine qxmd_gpu_set_kb(eqlmr_, nplwex, lmmax, natom, lmx, vclalp)
   qx_natom = natom
 end subroutine
 
+! Tables of the nonlocal term (the nlpp_variables arrays setqlm reads,
+! Sources/ppkb.f90:108-191) registered once per cell / pseudopotential set.
+subroutine qxmd_gpu_set_nlpp_tables(ntype, lmmax, lnref, mxqlm, lmx, clalp, tbqlm, tbqlma,  &
+&   dlqlm, itola, itolm, itolmf, litoll, nqlmmx, lplh, qlmr, qlmi)
+  integer, intent(in) :: ntype, lmmax, lnref, mxqlm, nqlmmx, lplh, lmx(ntype)
+  real*8, intent(in) :: clalp(lmmax, ntype), dlqlm(ntype)
+  real*8, intent(in) :: tbqlm(0:mxqlm, lnref, ntype), tbqlma(0:mxqlm, lnref, ntype)
+  integer, intent(in) :: itola(lmmax, ntype), itolm(lmmax, ntype), itolmf(lmmax, ntype)
+  logical, intent(in) :: litoll(lmmax, ntype)
+  real*8, intent(in) :: qlmr(0:lplh, nqlmmx), qlmi(0:lplh, nqlmmx)
+  integer(c_int) :: lmxc(ntype), ia(lmmax, ntype), im(lmmax, ntype), imf(lmmax, ntype)
+  integer(c_int) :: il(lmmax, ntype)
+  call qxmd_gpu_need_ctx('qxmd_gpu_set_nlpp_tables')
+  lmxc = lmx
+  ia = itola
+  im = itolm
+  imf = itolmf
+  il = merge(1, 0, litoll)
+  call qxmd_gpu_check(qx_set_nlpp_tables(qxctx, int(ntype, c_int), int(lmmax, c_int),  &
+&      int(lnref, c_int), int(mxqlm, c_int), lmxc, clalp, tbqlm, tbqlma, dlqlm, ia, im, imf, il, &
+&      int(nqlmmx, c_int), int(lplh, c_int), qlmr, qlmi), 'qx_set_nlpp_tables')
+end subroutine
+
+! setnlc_g (Sources/nlkbpp.f90:1236-1279): eqlmr_ and vclalp built on the
+! device from the registered tables and the device copy of the structure
+! factors of the last stscfn_pw (the ycos/ysin arguments are not read).
+subroutine qxmd_gpu_setnlc_g(nplw, ntype, nhk1, nhk2, natom, lkbppi, lking, ycos, ysin, nplwcs,  &
+&   rvol, lmmax, vclalp)
+  integer, intent(in) :: nplw, ntype, natom, nplwcs, lmmax, nhk1(ntype), nhk2(ntype)
+  logical, intent(in) :: lkbppi(ntype), lking(ntype)
+  real*8, intent(in) :: ycos(0:nplwcs, natom), ysin(0:nplwcs, natom), rvol
+  real*8, target, intent(out) :: vclalp(lmmax, ntype)
+  integer(c_int) :: n1(ntype), n2(ntype), lk(ntype), lg(ntype)
+  call qxmd_gpu_need_ctx('qxmd_gpu_setnlc_g')
+  n1 = nhk1
+  n2 = nhk2
+  lk = merge(1, 0, lkbppi)
+  lg = merge(1, 0, lking)
+  call qxmd_gpu_check(qx_setnlc_g(qxctx, int(nplw, c_int), int(ntype, c_int), n1, n2,  &
+&      int(natom, c_int), lk, lg, c_null_ptr, c_null_ptr, int(nplwcs, c_int), rvol,  &
+&      c_loc(vclalp)), 'qx_setnlc_g')
+  qx_lmmax = lmmax
+  qx_natom = natom
+end subroutine
+
 ! hcnonloc (Sources/engrad.f90:411-606): rhcr += V_loc cgjr for one band
 subroutine qxmd_hcnonloc(cgjr, rhcr)
   real*8, intent(in) :: cgjr(*)
@@ -312,6 +357,74 @@ end if
 if (.not. allocated(qx_slmir)) allocate(qx_slmir(max(qx_lmmax, 1), max(qx_natom, 1), nbnod))
 call qxmd_gpu_check(qx_hckbpp_g(qxctx, cgjr, rhcr, int(nbnod, c_int), c_loc(qx_slmir)),  &
 &    'hckbpp_g')
+end subroutine
+
+! stscfn_pw (Sources/pwlda.f90:5711-5854) with the reference's argument list:
+! the 1-D tables DCQ*/DSQ* are filled by the reference's recurrence here, and
+! ycos/ysin come from the device (qx_stscfn_pw), which keeps them for setnlc_g.
+subroutine stscfn_pw(nfile, myid, nodes, ycos, ysin, DCQ1, DCQ2, DCQ3, DSQ1, DSQ2, DSQ3,  &
+& kmax1, kmax2, kmax3, natom, nplwcs, nplw5, nga, ngb, ngc, lclust, ratm, hcell, nd1v, nd1vks,  &
+& lnlpp_g, llclpp_g, lpcc_g, lsphexp)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: nfile(*), myid, nodes, natom, nplwcs, nplw5, kmax1, kmax2, kmax3
+real*8, target :: ycos(0:nplwcs, natom), ysin(0:nplwcs, natom)
+real*8 :: DCQ1(-kmax1:kmax1, natom), DSQ1(-kmax1:kmax1, natom)
+real*8 :: DCQ2(-kmax2:kmax2, natom), DSQ2(-kmax2:kmax2, natom)
+real*8 :: DCQ3(-kmax3:kmax3, natom), DSQ3(-kmax3:kmax3, natom)
+integer :: nga(0:nplw5), ngb(0:nplw5), ngc(0:nplw5)
+logical :: lclust, lnlpp_g, llclpp_g, lpcc_g, lsphexp
+real*8 :: ratm(3, natom), hcell(3, 3)
+integer :: nd1v(3), nd1vks(3)
+real*8, parameter :: dpi = 3.141592653589793d0 * 2.d0
+real*8 :: fact(3)
+integer(c_int) :: d1(3), d2(3), ga(0:nplw5), gb(0:nplw5), gc(0:nplw5)
+integer :: k, a
+do a = 1, 3
+  if (lclust) then
+    fact(a) = 1.d0 / hcell(a, a) * dble(nd1v(a)) / dble(nd1vks(a))
+  else
+    fact(a) = dble(nd1v(a)) / dble(nd1vks(a))
+  end if
+  fact(a) = fact(a) * dpi
+end do
+do k = 1, natom
+  call axis_table(DCQ1(:, k), DSQ1(:, k), kmax1, ratm(1, k) * fact(1))
+  call axis_table(DCQ2(:, k), DSQ2(:, k), kmax2, ratm(2, k) * fact(2))
+  call axis_table(DCQ3(:, k), DSQ3(:, k), kmax3, ratm(3, k) * fact(3))
+end do
+if (lnlpp_g .or. lpcc_g .or. lsphexp) then
+  call qxmd_gpu_need_ctx('stscfn_pw')
+  d1 = nd1v
+  d2 = nd1vks
+  ga = nga
+  gb = ngb
+  gc = ngc
+  call qxmd_gpu_check(qx_stscfn_pw(qxctx, ratm, int(natom, c_int), hcell, d1, d2,  &
+&      merge(1_c_int, 0_c_int, lclust), int(kmax1, c_int), int(kmax2, c_int), int(kmax3, c_int), &
+&      int(nplwcs, c_int), int(nplw5, c_int), ga, gb, gc, c_loc(ycos), c_loc(ysin)), 'stscfn_pw')
+end if
+contains
+  ! c + i s over k = -km..km: powers of exp(i ph) by the recurrence of 5806-5834
+  subroutine axis_table(c, s, km, ph)
+    integer, intent(in) :: km
+    real*8, intent(out) :: c(-km:km), s(-km:km)
+    real*8, intent(in) :: ph
+    integer :: m
+    c(0) = 1.d0
+    s(0) = 0.d0
+    c(1) = cos(ph)
+    s(1) = sin(ph)
+    c(-1) = c(1)
+    s(-1) = -s(1)
+    do m = 2, km
+      c(m) = c(1) * c(m - 1) - s(1) * s(m - 1)
+      s(m) = s(1) * c(m - 1) + c(1) * s(m - 1)
+      c(-m) = c(m)
+      s(-m) = -s(m)
+    end do
+  end subroutine
 end subroutine
 
 ! hckbpp_ib_g (Sources/nlkbpp.f90:1623-1721): one band.

@@ -170,6 +170,9 @@ call hckbpp_ib_g(nfile, 0, 1, cg(:, 3), rh(:, 1), nplwex, nplw, 1, nhk, nhk, ia,
 err = maxval(abs(rh(:, 1) - ref(:, 3))) / maxval(abs(ref(:, 3)))
 call report('hckbpp_ib_g one band', err, 1.d-12)
 
+! ---- stscfn_pw: 3 atoms, 40 G indices in -3..3, against cos/sin(2 pi G . x)
+call check_stscfn()
+
 call qxmd_gpu_finalize()
 write(*, '(a, es10.2)') 'DROPIN OK worst', worst
 
@@ -194,6 +197,35 @@ subroutine report(what, e, tol)
     write(*, '(a, a)') 'FAILED ', what
     error stop 1
   end if
+end subroutine
+
+subroutine check_stscfn()
+  integer, parameter :: na = 3, ng = 40, km = 3
+  real*8 :: ra(3, na), yc(0:ng, na), ys(0:ng, na), ph, e2
+  real*8 :: c1(-km:km, na), c2(-km:km, na), c3(-km:km, na)
+  real*8 :: s1(-km:km, na), s2(-km:km, na), s3(-km:km, na), h(3, 3)
+  integer :: g1(0:ng), g2(0:ng), g3(0:ng), d(3), ig, ja, nf(2)
+  call random_number(ra)
+  g1(0) = 0; g2(0) = 0; g3(0) = 0
+  do ig = 1, ng
+    g1(ig) = mod(ig, 7) - 3
+    g2(ig) = mod(3 * ig, 7) - 3
+    g3(ig) = mod(5 * ig + 1, 7) - 3
+  end do
+  d = 30
+  h = 0.d0
+  nf = 6
+  call stscfn_pw(nf, 0, 1, yc, ys, c1, c2, c3, s1, s2, s3, km, km, km, na, ng, ng, g1, g2, g3,  &
+&     .false., ra, h, d, d, .true., .false., .false., .false.)
+  e2 = 0.d0
+  do ja = 1, na
+    do ig = 0, ng
+      ph = 2.d0 * pi * (g1(ig) * ra(1, ja) + g2(ig) * ra(2, ja) + g3(ig) * ra(3, ja))
+      e2 = max(e2, abs(yc(ig, ja) - cos(ph)), abs(ys(ig, ja) - sin(ph)))
+    end do
+    e2 = max(e2, abs(c2(-2, ja) - cos(4.d0 * pi * ra(2, ja))), abs(s3(3, ja) - sin(6.d0 * pi * ra(3, ja))))
+  end do
+  call report('stscfn_pw structure factors vs cos/sin(G.x)', e2, 1.d-13)
 end subroutine
 
 subroutine expect(cond, what)
