@@ -42,6 +42,42 @@ def build_inputs(ncell, nband):
     return cell, gs, v, eq, lmx, vcl, cl, psi
 
 
+def md_kb_setup(ctx, cell, gs, eq, lmx, vcl, reps=3):
+    """Time the per-MD-step KB inputs: stscfn_pw + setnlc_g on the device
+    (model s + p tables) against qx_set_kb's upload of the host-built table;
+    leaves the host-built table registered again."""
+    from qxmd_amd import nlpp
+    nw = gs.nplw + 1
+    t = nlpp.model_tables(gs.gx[:nw], gs.gy[:nw], gs.gz[:nw])
+    ctx.set_nlpp_tables(t)
+    frac = np.linalg.solve(cell.hcell, cell.tau.T).T
+    n5 = gs.nplw5 + 1
+    g = [np.asarray(a[:n5], np.int32) for a in (gs.nga, gs.ngb, gs.ngc)]
+    kmax = [int(np.abs(a).max()) for a in g]
+    nd = np.array(gs.nd, np.int32)
+    natom = len(frac)
+    nhk = np.array([1], np.int32), np.array([natom], np.int32)
+
+    def device():
+        ctx.stscfn_pw(frac, cell.hcell, nd, nd, *g, gs.nplw, kmax, copy_out=False)
+        ctx.setnlc_g(1, nhk[0], nhk[1], natom, cell.volume)
+
+    def host():
+        ctx.set_kb(eq, lmx, vcl)
+
+    out = {}
+    for name, fn in (("device_ms", device), ("host_upload_ms", host)):
+        fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.sync()
+        out[name] = round((time.perf_counter() - t0) / reps * 1e3, 3)
+    out["table_mb"] = round(8.0 * gs.nplwex * ctx.npj / 1e6, 1)
+    return out
+
+
 def fft_stage_bytes(gs, s):
     """Algorithmic FFT-stage bytes per band-apply: 8*N_h*s_c + N*s_r + 2*(nplw+1)*s_c
     (two real 3-D transforms as two HBM passes each, V(r) read, sphere I/O)."""
@@ -258,6 +294,13 @@ def main():
             "mixed_vs_fp64_max_eig_err_ha": float(np.abs(np.sort(res["mixed_batched"][1]) -
                                                          np.sort(res["fp64_batched"][1])).max() / 2),
         }
+
+    # per-MD-step KB inputs: structure factors + eqlmr_ built on the device
+    # (stscfn_pw, setnlc_g) against uploading a host-built table (qx_set_kb)
+    kb_setup = md_kb_setup(ctx, cell, gs, eq, lmx, vcl)
+    print(f"# KB set-up per MD step: device {kb_setup['device_ms']:.2f} ms, host table upload "
+          f"{kb_setup['host_upload_ms']:.2f} ms", file=sys.stderr, flush=True)
+    extra["md_kb_setup_ms"] = kb_setup
 
     cpu = None
     if not args.no_cpu:
