@@ -301,6 +301,16 @@ def main():
     print(f"# KB set-up per MD step: device {kb_setup['device_ms']:.2f} ms, host table upload "
           f"{kb_setup['host_upload_ms']:.2f} ms", file=sys.stderr, flush=True)
     extra["md_kb_setup_ms"] = kb_setup
+    # KB Hellmann-Feynman forces of all bands (kbnlpp_frc2) on the device
+    occ_f = occupations(nband, nel)
+    ctx.put(0, psi_orth)
+    ctx.kb_forces(0, nband, occ_f)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.kb_forces(0, nband, occ_f)
+    extra["kb_forces_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+    print(f"# KB forces ({natom} atoms, {nband} bands): {extra['kb_forces_ms']:.2f} ms",
+          file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:

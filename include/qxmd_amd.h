@@ -169,6 +169,15 @@ int qx_fft3(int key, double *fft3x, double *fft3y, int kfft1, int kfft2, int kff
 /* hcnonloc for one band with the context's glocal (Sources/engrad.f90:411-606):
  * rhcr += local part of H cgjr (no kinetic). */
 int qx_hcnonloc(qx_ctx *ctx, const double *cgjr, double *rhcr);
+/* kbnlpp_frc2 without stress (Sources/nlkbpp.f90:1878-2060): the KB
+ * Hellmann-Feynman forces of nb bands with occupations occ(nb) are added to
+ * fnlc(3, natom) (Ry/bohr); gx/gy/gz(0:nplw) are the Cartesian G of the
+ * sphere in the reference's order.  The slot form reads device-resident bands
+ * (slot 0, 1 or 5; slots 2-4 are used as work), the host form cgjr(nplwex, nb). */
+int qx_kbnlpp_frc2_slot(qx_ctx *ctx, int slot, int nb, const double *occ, const double *gx,
+                        const double *gy, const double *gz, double *fnlc);
+int qx_kbnlpp_frc2(qx_ctx *ctx, const double *cgjr, int nb, const double *occ, const double *gx,
+                   const double *gy, const double *gz, double *fnlc);
 /* hckbpp_g (Sources/nlkbpp.f90:1284-1393): rhcr(:, 1:nb) += KB cgjr; slmir
  * (lmmax, natom, nb) written if non-null. */
 int qx_hckbpp_g(qx_ctx *ctx, const double *cgjr, double *rhcr, int nb, double *slmir);

@@ -96,6 +96,8 @@ def _declare(L):
     L.qxo_hcnonloc.argtypes = [_dp, _dp, _dp, _i, _vp, _i, _i, _i, _dp, _dp]
     L.qxo_ekinetic.argtypes = [_dp, _dp, _dp, _i, _i, _vp, _vp, _vp]
     L.qxo_hckbpp_g.argtypes = [_dp, _i, _i, _i, _ip, _dp, _dp, _dp, _i, _vp]
+    L.qxo_kbnlpp_frc2.argtypes = [_dp, _i, _i, _i, _ip, _dp, _dp, _i, _dp, _dp, _dp, _dp, _dp,
+                                  _dp]
     L.qxo_stscfn_pw.argtypes = [_dp, _dp, _ip, _ip, _i, _i, _i, _i, _i, _i, _i, _ip, _ip, _ip,
                                 _dp, _dp]
     L.qxo_setnlc_g.argtypes = [_i, _i, _ip, _ip, _i, _ip, _ip, _dp, _dp, _i, _d, _i, _i, _i, _ip,
@@ -245,6 +247,21 @@ def hckbpp_g(eqlmr, lmx_atom, vclalp_atom, cgjr, rhcr):
     lib().qxo_hckbpp_g(e, n, lmmax, natom, _f(lmx_atom, np.int32), _f(vclalp_atom), c, r, nb,
                        _cptr(sl))
     return r, sl
+
+
+def kbnlpp_frc2(eqlmr, lmx_atom, vclalp_atom, cgjr, occ, slmir, gx, gy, gz):
+    """KB Hellmann-Feynman forces (kbnlpp_frc2, Sources/nlkbpp.f90:1878-2060):
+    eqlmr (natom, lmmax, nplwex), cgjr (nb, nplwex), slmir (nb, natom, lmmax) from
+    hckbpp_g, gx/gy/gz over G index 0..nplw.  Returns fnlc (natom, 3)."""
+    e = _f(eqlmr)
+    natom, lmmax, n = e.shape
+    c = _f(cgjr)
+    f = np.zeros((natom, 3))
+    nw = n // 2
+    lib().qxo_kbnlpp_frc2(e, n, lmmax, natom, _f(lmx_atom, np.int32), _f(vclalp_atom), c,
+                          c.shape[0], _f(occ), _f(slmir), _f(np.asarray(gx)[:nw]),
+                          _f(np.asarray(gy)[:nw]), _f(np.asarray(gz)[:nw]), f)
+    return f
 
 
 def stscfn_pw(ratm, hcell, nd1v, nd1vks, nga, ngb, ngc, nplwcs, kmax, lclust=False):

@@ -735,6 +735,45 @@ void qxo_setnlc_g(int nplw, int ntype, const int *nhk1, const int *nhk2, int nat
   free(z);
 }
 
+/* kbnlpp_frc2 with lbigmm2, no stress (Sources/nlkbpp.f90:1878-2060, VECTOR
+ * branch 1980-2030):
+ *   fsl(l, ia, ib) = sum_{ig=1..nplw} (e(2ig+1) psi(2ig+2) - e(2ig+2) psi(2ig+1)) G_ig
+ *   fnlc(:, ia)   += 2 sum_{l, ib} fsl(l, ia, ib) slmir(l, ia, ib) 2 occ(ib) vclalp(l)
+ * for the three Cartesian components of G (gx, gy, gz over G index 0..nplw);
+ * slmir as qxo_hckbpp_g leaves it.  fnlc (natom x 3, row-major) is added to. */
+void qxo_kbnlpp_frc2(const double *eqlmr, int nplwex, int lmmax, int natom, const int *lmx_atom,
+                     const double *vclalp_atom, const double *cgjr, int nb, const double *occ,
+                     const double *slmir, const double *gx, const double *gy, const double *gz,
+                     double *fnlc)
+{
+  const int nplw = nplwex / 2 - 1;
+  int ia, l, ib, ig;
+  for (ia = 0; ia < natom; ++ia) {
+    double f1 = 0.0, f2 = 0.0, f3 = 0.0;
+    for (l = 0; l < lmx_atom[ia]; ++l) {
+      const double *e = eqlmr + (size_t)nplwex * (l + (size_t)lmmax * ia);
+      for (ib = 0; ib < nb; ++ib) {
+        const double *c = cgjr + (size_t)nplwex * ib;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (ig = 1; ig <= nplw; ++ig) {
+          const double z = e[2 * ig] * c[2 * ig + 1] - e[2 * ig + 1] * c[2 * ig];
+          sx += z * gx[ig];
+          sy += z * gy[ig];
+          sz += z * gz[ig];
+        }
+        const double w = slmir[l + (size_t)lmmax * (ia + (size_t)natom * ib)] *
+                         (vclalp_atom[l + lmmax * ia] * occ[ib] * 2.0);
+        f1 += sx * w;
+        f2 += sy * w;
+        f3 += sz * w;
+      }
+    }
+    fnlc[3 * ia] += 2.0 * f1;
+    fnlc[3 * ia + 1] += 2.0 * f2;
+    fnlc[3 * ia + 2] += 2.0 * f3;
+  }
+}
+
 /* enkbpp_g2 (Sources/nlkbpp.f90:1763-1824), Gamma point. */
 double qxo_enkbpp_g(const double *slmir, int lmmax, int natom, const int *lmx_atom,
                     const double *clalp_atom, const double *occ, int nb, double rvol)

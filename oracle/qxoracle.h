@@ -62,6 +62,10 @@ void qxo_setnlc_g(int nplw, int ntype, const int *nhk1, const int *nhk2, int nat
                   const double *dlqlm, const int *itola, const int *itolm, const int *itolmf,
                   const int *litoll, int nqlmmx, const double *qlmr, const double *qlmi,
                   const double *recnrm, double *eqlmr, double *vclalp);
+void qxo_kbnlpp_frc2(const double *eqlmr, int nplwex, int lmmax, int natom, const int *lmx_atom,
+                     const double *vclalp_atom, const double *cgjr, int nb, const double *occ,
+                     const double *slmir, const double *gx, const double *gy, const double *gz,
+                     double *fnlc);
 double qxo_enkbpp_g(const double *slmir, int lmmax, int natom, const int *lmx_atom,
                     const double *clalp_atom, const double *occ, int nb, double rvol);
 

@@ -223,3 +223,15 @@ def setnlc_g(t, nhk1, nhk2, natom, ycos, ysin, recnrm, rvol, lkbppi=None, lking=
                      _F(np.asarray(t.qlmr)[:, :nplw + 1].T), _F(np.asarray(t.qlmi)[:, :nplw + 1].T),
                      _F(recnrm), eq, vcl)
     return np.ascontiguousarray(np.transpose(eq, (2, 1, 0))), np.ascontiguousarray(vcl.T)
+
+
+def kbnlpp_frc2(cgjr, occ, gx, gy, gz, natom):
+    """Reference kbnlpp_frc2 after hckbpp_g on the same cgjr (nb, nplwex); gx/gy/gz
+    for G index 0..nplw.  Returns fnlc (natom, 3)."""
+    L = lib()
+    c = _F(np.asarray(cgjr).T)
+    n, nb = c.shape
+    f = np.zeros((3, natom), order="F")
+    L.qxref_kbnlpp_frc2.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.qxref_kbnlpp_frc2(c, n, nb, natom, _F(occ), _F(gx), _F(gy), _F(gz), f)
+    return np.ascontiguousarray(f.T)

@@ -439,3 +439,36 @@ subroutine qxref_setnlc_g(nplw, ntype, nhk1, nhk2, natom, lkbppi_i, lking_i, yco
   eq_out = eqlmr_
   vcl_out = vclalp
 end subroutine
+
+! kbnlpp_frc2 (Sources/nlkbpp.f90:1878), no stress, one type per atom as in
+! qxref_kb_setup; slmir is the module state the last qxref_hckbpp_g left for
+! the same cgjr.  fnlc is accumulated into, as in the reference.
+subroutine qxref_kbnlpp_frc2(cgjr, nplwex, nband, natom, occ, gx, gy, gz, fnlc) &
+&   bind(C, name="qxref_kbnlpp_frc2")
+  use iso_c_binding
+  use nlpp_parameters, only: lmmax
+  use nl_variables, only: fslxr, fslyr, fslzr, dslmir, vclweg
+  implicit none
+  integer(c_int), value :: nplwex, nband, natom
+  real(c_double) :: cgjr(nplwex, nband), occ(nband), fnlc(3, natom)
+  real(c_double) :: gx(0:nplwex/2 - 1), gy(0:nplwex/2 - 1), gz(0:nplwex/2 - 1)
+  integer :: nfile(2), nhk(natom), i, iod(nband)
+  logical :: lkb(natom), lkg(natom)
+  real*8 :: yc(1), ys(1), str(6)
+  nfile = 6
+  do i = 1, natom
+    nhk(i) = i
+  end do
+  do i = 1, nband
+    iod(i) = i
+  end do
+  lkb = .true.
+  lkg = .false.
+  str = 0.d0
+  ! work arrays of the VECTOR branch, lmx(it) * nbnod long
+  if (allocated(fslxr)) deallocate(fslxr, fslyr, fslzr, dslmir, vclweg)
+  allocate(fslxr(lmmax * nband), fslyr(lmmax * nband), fslzr(lmmax * nband))
+  allocate(dslmir(lmmax * nband), vclweg(lmmax * nband))
+  call kbnlpp_frc2(nfile, 0, 1, fnlc, str, .false., cgjr, nplwex, nplwex/2 - 1, gx, gy, gz, &
+&     nband, 1, nband, nband, occ, iod, natom, nhk, nhk, natom, lkb, lkg, yc, ys, 0)
+end subroutine

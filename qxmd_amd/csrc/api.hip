@@ -674,6 +674,26 @@ int qx_hckbpp_g(qx_ctx* h, const double* cgjr, double* rhcr, int nb, double* slm
   return get_reference(c, c->d_slot[1], rhcr, nb);
 }
 
+int qx_kbnlpp_frc2_slot(qx_ctx* h, int slot, int nb, const double* occ, const double* gx,
+                        const double* gy, const double* gz, double* fnlc) {
+  Ctx* c = &h->c;
+  if (slot < 0 || slot >= Ctx::kSlots || (slot >= 2 && slot <= 4))
+    return fail(-1, "qx_kbnlpp_frc2_slot: slot 0, 1 or 5 (2-4 hold the G-weighted bands)");
+  if (nb < 1 || nb > c->nbmax) return fail(-1, "qx_kbnlpp_frc2_slot: 1 <= nb <= nbmax");
+  const int rc = kb_forces(c, c->d_slot[slot], nb, occ, gx, gy, gz, fnlc);
+  if (rc) return fail(rc, "qx_kbnlpp_frc2_slot failed");
+  return 0;
+}
+
+int qx_kbnlpp_frc2(qx_ctx* h, const double* cgjr, int nb, const double* occ, const double* gx,
+                   const double* gy, const double* gz, double* fnlc) {
+  Ctx* c = &h->c;
+  if (nb < 1 || nb > c->nbmax) return fail(-1, "qx_kbnlpp_frc2: 1 <= nb <= nbmax");
+  const int rc = put_internal(c, c->d_slot[0], cgjr, nb);
+  if (rc) return rc;
+  return qx_kbnlpp_frc2_slot(h, 0, nb, occ, gx, gy, gz, fnlc);
+}
+
 int qx_hckbpp_ib_g(qx_ctx* h, const double* cgjr, double* rhcr) {
   return qx_hckbpp_g(h, cgjr, rhcr, 1, nullptr);
 }

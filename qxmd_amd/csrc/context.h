@@ -141,6 +141,9 @@ int nl_setnlc_g(Ctx* c, int nplw, int ntype, const int* nhk1, const int* nhk2, i
                 const int* lkbppi, const int* lking, const double* ycos, const double* ysin,
                 int nplwcs, double rvol, double* vclalp_out);
 void nl_destroy(Ctx* c);
+// KB Hellmann-Feynman forces (forces.hip): fnlc(3, natom) += kbnlpp_frc2
+int kb_forces(Ctx* c, const double* psi, int nb, const double* occ_host, const double* gx,
+              const double* gy, const double* gz, double* fnlc_host);
 int coop_launch(Ctx* c, const void* fn, int want, size_t lds, void** args, bool exact);
 int dense_chol_inv(Ctx* c, double* s, int n, double* x, int* ok, int* flag);
 int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,

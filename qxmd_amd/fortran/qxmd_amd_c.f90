@@ -303,6 +303,15 @@ integer(c_int) function qx_hckbpp_g(ctx, cgjr, rhcr, nb, slmir) bind(C, name="qx
   type(c_ptr), value :: slmir
 end function
 
+integer(c_int) function qx_kbnlpp_frc2(ctx, cgjr, nb, occ, gx, gy, gz, fnlc)  &
+&     bind(C, name="qx_kbnlpp_frc2")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_double), intent(in) :: cgjr(*), occ(*), gx(*), gy(*), gz(*)
+  integer(c_int), value :: nb
+  real(c_double), intent(inout) :: fnlc(*)
+end function
+
 integer(c_int) function qx_hckbpp_ib_g(ctx, cgjr, rhcr) bind(C, name="qx_hckbpp_ib_g")
   import :: c_int, c_double, c_ptr
   type(c_ptr), value :: ctx

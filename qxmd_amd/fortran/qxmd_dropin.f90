@@ -359,6 +359,32 @@ call qxmd_gpu_check(qx_hckbpp_g(qxctx, cgjr, rhcr, int(nbnod, c_int), c_loc(qx_s
 &    'hckbpp_g')
 end subroutine
 
+! kbnlpp_frc2 (Sources/nlkbpp.f90:1878-2060) with the reference's argument
+! list: fnlc(3, natom) += KB forces of the nbnod local bands (occupation
+! occ(iod(ibb))).  The stress term (lcalstr) is not computed on the device.
+subroutine kbnlpp_frc2(nfile, myid, nodes, fnlc, str, lcalstr, cgjr, nplwex, nplw, gx, gy, gz,  &
+& nband, nbnod1, nbnod2, nbnod, occ, iod, ntype, nhk1, nhk2, natom, lkbppi, lking, ycos, ysin,  &
+& nplwcs)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: nfile(*), myid, nodes, nplwex, nplw, nband, nbnod1, nbnod2, nbnod
+integer :: ntype, natom, nhk1(ntype), nhk2(ntype), nplwcs, iod(nbnod)
+real*8 :: fnlc(3, natom), str(6), cgjr(nplwex, nbnod), occ(nband)
+real*8 :: gx(0:nplw), gy(0:nplw), gz(0:nplw)
+logical :: lcalstr, lkbppi(ntype), lking(ntype)
+real*8 :: ycos(0:nplwcs, natom), ysin(0:nplwcs, natom)
+real*8 :: occb(nbnod)
+integer :: ibb
+call qxmd_gpu_need_ctx('kbnlpp_frc2')
+if (lcalstr) call qxmd_gpu_check(-1_c_int, 'kbnlpp_frc2: stress (lcalstr) is not on the device')
+do ibb = 1, nbnod
+  occb(ibb) = occ(iod(ibb))
+end do
+call qxmd_gpu_check(qx_kbnlpp_frc2(qxctx, cgjr, int(nbnod, c_int), occb, gx, gy, gz, fnlc),  &
+&    'kbnlpp_frc2')
+end subroutine
+
 ! stscfn_pw (Sources/pwlda.f90:5711-5854) with the reference's argument list:
 ! the 1-D tables DCQ*/DSQ* are filled by the reference's recurrence here, and
 ! ycos/ysin come from the device (qx_stscfn_pw), which keeps them for setnlc_g.

@@ -56,6 +56,8 @@ SIGNATURES = {
                      _vp],
     "qx_setnlc_g": [_vp, _i, _i, _ip, _ip, _i, _ip, _ip, _vp, _vp, _i, _d, _vp],
     "qx_get_kb": [_vp, _dp, _vp],
+    "qx_kbnlpp_frc2_slot": [_vp, _i, _i, _dp, _dp, _dp, _dp, _dp],
+    "qx_kbnlpp_frc2": [_vp, _dp, _i, _dp, _dp, _dp, _dp, _dp],
     "qx_slot_put": [_vp, _i, _dp, _i],
     "qx_slot_get": [_vp, _i, _dp, _i],
     "qx_slot_copy": [_vp, _i, _i, _i],

@@ -94,6 +94,7 @@ class PlaneWave:
         assert n == self.gs.nplwex
         check(load().qx_set_kb(self.h, e, lmmax, natom, i32(lmx_atom), f64(vclalp_atom)),
               "qx_set_kb")
+        self.natom_kb = natom
         info = np.zeros(5, np.int32)
         load().qx_info(self.h, info)
         self.npj = int(info[3])
@@ -143,10 +144,22 @@ class PlaneWave:
         check(load().qx_setnlc_g(self.h, self.gs.nplw, ntype, i32(nhk1), i32(nhk2), natom, lk, lg,
                                  pc, ps, int(nplwcs), float(rvol),
                                  vcl.ctypes.data_as(C.c_void_p)), "qx_setnlc_g")
+        self.natom_kb = natom
         info = np.zeros(5, np.int32)
         load().qx_info(self.h, info)
         self.npj = int(info[3])
         return vcl
+
+    def kb_forces(self, slot, nb, occ, fnlc=None):
+        """KB Hellmann-Feynman forces of the bands in slot (kbnlpp_frc2): added to
+        fnlc (natom, 3), zeros if None; returns it."""
+        natom = self.natom_kb
+        f = np.zeros((natom, 3)) if fnlc is None else f64(fnlc).copy()
+        nw = self.gs.nplw + 1
+        check(load().qx_kbnlpp_frc2_slot(self.h, slot, nb, f64(occ), f64(self.gs.gx[:nw]),
+                                         f64(self.gs.gy[:nw]), f64(self.gs.gz[:nw]), f),
+              "qx_kbnlpp_frc2_slot")
+        return f
 
     def get_kb(self, natom, lmmax):
         """KB table in the reference layout: eqlmr (natom, lmmax, nplwex), vclalp (natom, lmmax)."""

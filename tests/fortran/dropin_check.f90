@@ -169,6 +169,7 @@ call hckbpp_ib_g(nfile, 0, 1, cg(:, 3), rh(:, 1), nplwex, nplw, 1, nhk, nhk, ia,
 &                ycos, ysin, 1)
 err = maxval(abs(rh(:, 1) - ref(:, 3))) / maxval(abs(ref(:, 3)))
 call report('hckbpp_ib_g one band', err, 1.d-12)
+call check_forces()
 
 ! ---- stscfn_pw: 3 atoms, 40 G indices in -3..3, against cos/sin(2 pi G . x)
 call check_stscfn()
@@ -197,6 +198,43 @@ subroutine report(what, e, tol)
     write(*, '(a, a)') 'FAILED ', what
     error stop 1
   end if
+end subroutine
+
+! kbnlpp_frc2 against its sums written out, with G = (1, 2, 3) scaled per index
+subroutine check_forces()
+  real*8 :: gxv(0:nplw), gyv(0:nplw), gzv(0:nplw), fn(3, ia), fr(3, ia), z, sx, sy, sz, wgt
+  real*8 :: oc(nb), st(6)
+  integer :: io(nb), ig, jb, ja, ll
+  do ig = 0, nplw
+    gxv(ig) = 0.01d0 * ig
+    gyv(ig) = 0.02d0 * ig
+    gzv(ig) = -0.03d0 * ig
+  end do
+  do jb = 1, nb
+    oc(jb) = 2.d0 - 0.25d0 * jb
+    io(jb) = jb
+  end do
+  fr = 0.d0
+  do ja = 1, ia
+    do ll = 1, 4
+      do jb = 1, nb
+        sx = 0.d0; sy = 0.d0; sz = 0.d0
+        do ig = 1, nplw
+          z = eq(2 * ig + 1, ll, ja) * cg(2 * ig + 2, jb) - eq(2 * ig + 2, ll, ja) * cg(2 * ig + 1, jb)
+          sx = sx + z * gxv(ig); sy = sy + z * gyv(ig); sz = sz + z * gzv(ig)
+        end do
+        wgt = 2.d0 * slm(ll, ja, jb) * vcl(ll, ja) * oc(jb) * 2.d0
+        fr(1, ja) = fr(1, ja) + sx * wgt
+        fr(2, ja) = fr(2, ja) + sy * wgt
+        fr(3, ja) = fr(3, ja) + sz * wgt
+      end do
+    end do
+  end do
+  fn = 1.d0
+  st = 0.d0
+  call kbnlpp_frc2(nfile, 0, 1, fn, st, .false., cg, nplwex, nplw, gxv, gyv, gzv, nb, 1, nb, nb,  &
+&     oc, io, ia, nhk, nhk, ia, lk, lk, ycos, ysin, 1)
+  call report('kbnlpp_frc2 forces', maxval(abs(fn - 1.d0 - fr)) / maxval(abs(fr)), 1.d-11)
 end subroutine
 
 subroutine check_stscfn()

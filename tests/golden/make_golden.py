@@ -75,6 +75,8 @@ def main():
     out["hckbpp_slmir"] = sl
     occ = np.array([2.0, 2.0, 2.0, 0.0, 1.0])
     out["enkbpp"] = np.array(R.enkbpp_g2(nb, natom, occ, si8.cell.volume))
+    nw = si8.nplw + 1
+    out["kbnlpp_frc2"] = R.kbnlpp_frc2(psi, occ, si8.gx[:nw], si8.gy[:nw], si8.gz[:nw], natom)
     out["hckbpp_ib"] = np.asarray(R.hckbpp_ib_g(psi[2], rh0[2], natom))[::SUB]
     # schmidt2
     b, okb = R.schmidt2(initial_wavefunctions(si8, 12, 25.0))

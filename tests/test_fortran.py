@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FLIB = os.path.join(ROOT, "qxmd_amd", "libqxmd_amd_f.so")
 DRIVER = os.path.join(ROOT, "tests", "fortran", "dropin_check")
 DROPINS = ["rfft3_", "fft3_", "rseigqr_", "icholesky_", "softchg_", "hckbpp_g_", "hckbpp_ib_g_",
-           "stscfn_pw_"]
+           "stscfn_pw_", "kbnlpp_frc2_"]
 
 needs_fortran = pytest.mark.skipif(not os.path.exists(FLIB),
                                    reason="Fortran layer not built (no flang)")

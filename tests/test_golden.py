@@ -69,6 +69,10 @@ def test_golden_hckbpp(si8):
     occ = np.array([2.0, 2.0, 2.0, 0.0, 1.0])
     e_or = O.enkbpp_g(sl_or, lmx, cl, occ, si8.cell.volume)
     assert abs(e_or - float(G["enkbpp"])) <= 1e-12 * abs(float(G["enkbpp"]))
+    nw = si8.nplw + 1
+    f_or = O.kbnlpp_frc2(eq, lmx, vcl, psi, occ, sl_or, si8.gx[:nw], si8.gy[:nw], si8.gz[:nw])
+    np.testing.assert_allclose(f_or, G["kbnlpp_frc2"], rtol=1e-11,
+                               atol=1e-13 * np.abs(G["kbnlpp_frc2"]).max())
     np.testing.assert_allclose(r_or[2, ::SUB], G["hckbpp_ib"], rtol=1e-12, atol=1e-14)
 
 
@@ -121,3 +125,32 @@ def test_golden_stscfn_setnlc(si8):
                          si8.cell.volume)
     np.testing.assert_allclose(eq[:, :, ::SUB], G["setnlc_eqlmr"], rtol=0, atol=1e-14)
     np.testing.assert_allclose(vcl, G["setnlc_vclalp"], rtol=0, atol=0)
+
+
+def test_kb_force_is_energy_gradient(si8):
+    """kbnlpp_frc2 of the oracle is minus the derivative of the KB energy
+    (enkbpp_g2) with respect to the atom position, by central differences of
+    the projector table built at displaced positions."""
+    from qxmd_amd.system import SiCell
+    nb = 5
+    psi = initial_wavefunctions(si8, nb, 25.0)
+    occ = np.array([2.0, 2.0, 2.0, 0.0, 1.0])
+    nw = si8.nplw + 1
+
+    def energy(cell):
+        eq, lmx, vcl, cl = model_kb_projectors(si8, cell)
+        _, sl = O.hckbpp_g(eq, lmx, vcl, psi, np.zeros_like(psi))
+        return O.enkbpp_g(sl, lmx, cl, occ, cell.volume), (eq, lmx, vcl, sl)
+
+    _, (eq, lmx, vcl, sl) = energy(si8.cell)
+    f = O.kbnlpp_frc2(eq, lmx, vcl, psi, occ, sl, si8.gx[:nw], si8.gy[:nw], si8.gz[:nw])
+    h = 1e-4
+    for ia in (0, 5):
+        for a in range(3):
+            tp, tm = si8.cell.tau.copy(), si8.cell.tau.copy()
+            tp[ia, a] += h
+            tm[ia, a] -= h
+            ep = energy(SiCell(ncell=1, hcell=si8.cell.hcell, tau=tp))[0]
+            em = energy(SiCell(ncell=1, hcell=si8.cell.hcell, tau=tm))[0]
+            fd = -(ep - em) / (2 * h)
+            assert abs(f[ia, a] - fd) < 1e-6 * np.abs(f).max() + 1e-12, (ia, a, f[ia, a], fd)

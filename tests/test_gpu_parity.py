@@ -440,3 +440,19 @@ def test_stscfn_setnlc_device(si64, host_sf):
     ctx.put(0, psi)
     ctx.hpsi(0, 1, nb)
     assert _relerr(ctx.get(1, nb), H.hpsi(psi)) < 1e-12
+
+
+def test_kb_forces_device(si64):
+    """KB Hellmann-Feynman forces (kbnlpp_frc2) on the device against the oracle."""
+    nb = 20
+    gs, ctx, H, v = _setup(si64, nb)
+    eq, lmx, vcl, cl = model_kb_projectors(si64, si64.cell)
+    psi, _ = O.schmidt(initial_wavefunctions(si64, nb))
+    occ = np.where(np.arange(nb) < 16, 2.0, 0.0)
+    ctx.put(0, psi)
+    f0 = np.full((len(lmx), 3), 0.25)
+    f = ctx.kb_forces(0, nb, occ, f0)
+    nw = si64.nplw + 1
+    _, sl = O.hckbpp_g(eq, lmx, vcl, psi, np.zeros_like(psi))
+    fo = O.kbnlpp_frc2(eq, lmx, vcl, psi, occ, sl, si64.gx[:nw], si64.gy[:nw], si64.gz[:nw])
+    assert np.abs((f - 0.25) - fo).max() < 1e-10 * np.abs(fo).max()

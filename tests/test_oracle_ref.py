@@ -156,3 +156,17 @@ def test_stscfn_setnlc_vs_reference(si64, lclust):
                         si64.cell.volume, lking=lking)
     assert np.abs(eo - er).max() < 1e-14 and np.abs(vo - vr).max() == 0.0
     assert np.abs(er).max() > 0.1
+
+
+def test_kbnlpp_frc2_vs_reference(si8):
+    nb = 7
+    psi = initial_wavefunctions(si8, nb, 31.0)
+    eq, lmx, vcl, cl = model_kb_projectors(si8, si8.cell)
+    occ = np.linspace(2.0, 0.0, nb)
+    natom, lmmax = R.kb_setup(eq, lmx, vcl, cl, nb)
+    R.hckbpp_g(psi, np.zeros_like(psi), natom, lmmax)
+    nw = si8.nplw + 1
+    fr = R.kbnlpp_frc2(psi, occ, si8.gx[:nw], si8.gy[:nw], si8.gz[:nw], natom)
+    _, sl = O.hckbpp_g(eq, lmx, vcl, psi, np.zeros_like(psi))
+    fo = O.kbnlpp_frc2(eq, lmx, vcl, psi, occ, sl, si8.gx[:nw], si8.gy[:nw], si8.gz[:nw])
+    assert np.abs(fo - fr).max() < 1e-11 * np.abs(fr).max()
