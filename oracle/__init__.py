@@ -96,6 +96,16 @@ def _declare(L):
     L.qxo_hcnonloc.argtypes = [_dp, _dp, _dp, _i, _vp, _i, _i, _i, _dp, _dp]
     L.qxo_ekinetic.argtypes = [_dp, _dp, _dp, _i, _i, _vp, _vp, _vp]
     L.qxo_hckbpp_g.argtypes = [_dp, _i, _i, _i, _ip, _dp, _dp, _dp, _i, _vp]
+    L.qxo_xclda.argtypes = [_dp, _vp, _i, _d, _dp, _dp, _dp]
+    L.qxo_harting.argtypes = [_dp, _i, _dp, _vp, _i, _i, _i, _dp]
+    L.qxo_chgr2g.argtypes = [_dp, _i, _vp, _i, _i, _i, _dp]
+    L.qxo_chgg2r.argtypes = [_dp, _i, _vp, _i, _i, _i, _dp]
+    L.qxo_invers.argtypes = [_dp, _dp, _i, _i]
+    L.qxo_invers.restype = _i
+    L.qxo_mxchgg_pulay.argtypes = [_dp, _dp, _i, _i, _dp, _d, _d, _d, _d, _d, _i, _i, _i,
+                                   C.POINTER(_i), _dp, _dp, _dp, _dp]
+    L.qxo_mxchgg_pulay.restype = _i
+    L.qxo_mxchgg_simple.argtypes = [_dp, _dp, _i, _d, _i]
     L.qxo_kbnlpp_frc2.argtypes = [_dp, _i, _i, _i, _ip, _dp, _dp, _i, _dp, _dp, _dp, _dp, _dp,
                                   _dp]
     L.qxo_stscfn_pw.argtypes = [_dp, _dp, _ip, _ip, _i, _i, _i, _i, _i, _i, _i, _ip, _ip, _ip,
@@ -247,6 +257,105 @@ def hckbpp_g(eqlmr, lmx_atom, vclalp_atom, cgjr, rhcr):
     lib().qxo_hckbpp_g(e, n, lmmax, natom, _f(lmx_atom, np.int32), _f(vclalp_atom), c, r, nb,
                        _cptr(sl))
     return r, sl
+
+
+def xclda(rho, rhocore=None, rdelv=1.0):
+    """LDA exchange-correlation (xclda, Sources/vxc.f90:961-1017).  Returns
+    vexc, eeexc (Ry) and (eexc, evexc, ecor, evcor)."""
+    r = _f(rho)
+    n = r.size
+    core = None if rhocore is None else _f(rhocore)
+    v = np.zeros(n)
+    e = np.zeros(n)
+    e4 = np.zeros(4)
+    lib().qxo_xclda(r, _cptr(core) if core is not None else None, n, float(rdelv), v, e, e4)
+    return v, e, e4
+
+
+def harting(rhgr, recnrmex, ijk, nd):
+    """Hartree potential (harting, Sources/engrad.f90:1673-1743): rhgr over G index
+    0..nplw5 (real-packed), recnrmex over 0..nplw5, ijk = ijkgd(-nplw5:nplw5).
+    Returns vhar on the FFT grid (FFT order, x fastest)."""
+    rh = _f(rhgr)
+    nplw5 = rh.size // 2 - 1
+    ij = _f(ijk, np.int32)
+    out = np.zeros(int(np.prod(nd)))
+    lib().qxo_harting(rh, nplw5, _f(recnrmex), ij.ctypes.data + 4 * nplw5, int(nd[0]), int(nd[1]),
+                      int(nd[2]), out)
+    return out
+
+
+def chgr2g(rho, nplw5, ijk, nd):
+    """chgr2g (Sources/engrad.f90:1253-1297): rho (FFT order) -> rhgr over G index
+    0..nplw5 (real-packed); ijk = ijkgd(-m:m) with m >= nplw5."""
+    ij = _f(ijk, np.int32)
+    m = (ij.size - 1) // 2
+    out = np.zeros(2 * (int(nplw5) + 1))
+    lib().qxo_chgr2g(_f(rho), int(nplw5), ij.ctypes.data + 4 * m, int(nd[0]), int(nd[1]),
+                     int(nd[2]), out)
+    return out
+
+
+def chgg2r(rhgr, ijk, nd):
+    """chgg2r (Sources/engrad.f90:1302-1353): rhgr over G index 0..nplw5 -> rho
+    (FFT order); ijk = ijkgd(-m:m) with m >= nplw5."""
+    rh = _f(rhgr)
+    ij = _f(ijk, np.int32)
+    m = (ij.size - 1) // 2
+    out = np.zeros(int(np.prod(nd)))
+    lib().qxo_chgg2r(rh, rh.size // 2 - 1, ij.ctypes.data + 4 * m, int(nd[0]), int(nd[1]),
+                     int(nd[2]), out)
+    return out
+
+
+def invers(w):
+    """INVERS (Sources/funcs.f90:5-61); returns (inverse, ierr)."""
+    a = np.asfortranarray(np.asarray(w, np.float64))
+    n = a.shape[0]
+    flat = np.ascontiguousarray(a.T).ravel()  # column-major as a flat C array
+    wi = np.zeros(n * n)
+    ierr = lib().qxo_invers(flat, wi, n, n)
+    return wi.reshape(n, n).T.copy(), ierr
+
+
+def factq1(cmetric, g2min, g2max):
+    """Metric parameter of the Pulay mixer (Sources/chgdns.f90:661-667)."""
+    bbb = g2max - cmetric * g2min
+    return (cmetric - 1.0) * g2max * g2min / bbb if bbb > 1.0e-5 else 0.0
+
+
+class PulayMixer:
+    """State of module mxchgg_variables (Sources/chgdns.f90:494-515) for the
+    Pulay mixer (imxchg = 1), advanced by mxchgg_pulay's mixing part."""
+
+    def __init__(self, nplwcex, itratn, dkgnrm, rvol, factq1, wkerker=1.0):
+        self.nplwcex, self.itratn = int(nplwcex), int(itratn)
+        self.dkgnrm = _f(np.asarray(dkgnrm)[: self.nplwcex])
+        self.rvol, self.factq1, self.wkerker = float(rvol), float(factq1), float(wkerker)
+        self.kntchg = 0
+        self.unr = np.zeros(self.itratn * self.nplwcex)
+        self.dfmr = np.zeros(self.itratn * self.nplwcex)
+        self.aij = np.zeros(self.itratn * self.itratn)
+        self.cmk = np.zeros(self.itratn)
+        self.nreset = 0
+
+    def mix(self, rhgr, rinr, aslh, bslh, lreset=False, lmixhold=False):
+        """Returns (rhgr mixed, rinr) as new arrays; advances the history."""
+        rh, ri = _f(rhgr).copy(), _f(rinr).copy()
+        k = _i(self.kntchg)
+        self.nreset = lib().qxo_mxchgg_pulay(
+            rh, ri, self.nplwcex, rh.size, self.dkgnrm, self.rvol, float(aslh), float(bslh),
+            self.factq1, self.wkerker, int(bool(lreset)), int(bool(lmixhold)), self.itratn,
+            C.byref(k), self.unr, self.dfmr, self.aij, self.cmk)
+        self.kntchg = k.value
+        return rh, ri
+
+
+def mxchgg_simple(rhgr, rinr, aslh, none=False):
+    """mxchgg_simple / mxchgg_none mixing part (Sources/chgdns.f90:907-1210)."""
+    rh, ri = _f(rhgr).copy(), _f(rinr).copy()
+    lib().qxo_mxchgg_simple(rh, ri, rh.size, float(aslh), int(bool(none)))
+    return rh, ri
 
 
 def kbnlpp_frc2(eqlmr, lmx_atom, vclalp_atom, cgjr, occ, slmir, gx, gy, gz):

@@ -774,6 +774,357 @@ void qxo_kbnlpp_frc2(const double *eqlmr, int nplwex, int lmmax, int natom, cons
   }
 }
 
+/* ======================================================================= */
+/* SCF glue: LDA exchange-correlation and the Hartree potential              */
+/* ======================================================================= */
+
+/* VXC (Sources/vxc.f90:2063-2104): Ceperley-Alder as parametrised by Perdew
+ * and Zunger, Hartree units.  B2, B3, C2..C4 are default-real (single
+ * precision) constants in the reference's DATA statement, so they carry the
+ * float rounding of their decimal values. */
+static void pz_vxc(double rs, double *ex, double *vx, double *ec, double *vc)
+{
+  const double A1 = -0.4582, B1 = -0.1423, B2 = (double)1.0529f, B3 = (double)0.3334f;
+  const double C1 = -0.0480, C2 = (double)0.0311f, C3 = (double)-0.0116f, C4 = (double)0.0020f;
+  *ex = A1 / rs;
+  *vx = 4.0 * *ex / 3.0;
+  if (rs >= 1.0) {
+    const double bunb = 1.0 + B2 * sqrt(rs) + B3 * rs;
+    *ec = B1 / bunb;
+    *vc = *ec * (1.0 + (B2 * sqrt(rs) / 6.0 + B3 * rs / 3.0) / bunb);
+  } else {
+    *ec = C1 + C2 * log(rs) + C3 * rs + C4 * rs * log(rs);
+    *vc = *ec - (C2 + (C3 + C4) * rs + C4 * rs * log(rs)) / 3.0;
+  }
+}
+
+/* xclda (Sources/vxc.f90:961-1017): per mesh point rhot = rho + rhocore
+ * (rhocore < 0 or NULL: rho only); rhot < 1e-30 gives 0; else
+ * rs = (3 / (pi4 rhot))^(1/3) with pi4 = 12.56637061436 as in the reference,
+ * vexc = 2 (vx + vc), eeexc = 2 (ex + ec) (Ry); e4 = {eexc, evexc, ecor, evcor}
+ * = 2 rdelv * {sum ex rhot, sum vx rho, sum ec rhot, sum vc rho}. */
+void qxo_xclda(const double *rho, const double *rhocore, int n, double rdelv, double *vexc,
+               double *eeexc, double *e4)
+{
+  const double pi4 = 12.56637061436;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int m;
+  for (m = 0; m < n; ++m) {
+    const double rhot = (rhocore && rhocore[m] >= 0.0) ? rho[m] + rhocore[m] : rho[m];
+    if (rhot < 1e-30) {
+      vexc[m] = 0.0;
+      eeexc[m] = 0.0;
+      continue;
+    }
+    const double rs = pow(3.0 / (pi4 * rhot), 1.0 / 3.0);
+    double ex, vx, ec, vc;
+    pz_vxc(rs, &ex, &vx, &ec, &vc);
+    vexc[m] = 2.0 * (vx + vc);
+    eeexc[m] = 2.0 * (ex + ec);
+    s0 += ex * rhot;
+    s1 += vx * rho[m];
+    s2 += ec * rhot;
+    s3 += vc * rho[m];
+  }
+  e4[0] = s0 * 2.0 * rdelv;
+  e4[1] = s1 * 2.0 * rdelv;
+  e4[2] = s2 * 2.0 * rdelv;
+  e4[3] = s3 * 2.0 * rdelv;
+}
+
+/* harting (Sources/engrad.f90:1673-1743): the density rhgr on the G sphere
+ * (index 0..nplw5, real-packed) times 2 recnrmex(ig) onto the full grid at
+ * ijkgd(ig) and its conjugate at ijkgd(-ig) (G = 0: real part only), then the
+ * c2r transform of rfft3 (key = inverse).  ijkgd_c points at ijkgd(0) of the
+ * array ijkgd(-nplw5:nplw5) (1-based grid indices); vhar (n1 n2 n3) in FFT
+ * order. */
+void qxo_harting(const double *rhgr, int nplw5, const double *recnrmex, const int *ijkgd_c,
+                 int n1, int n2, int n3, double *vhar)
+{
+  const size_t N = (size_t)n1 * n2 * n3;
+  double *fx = (double *)calloc(N, sizeof(double)), *fy = (double *)calloc(N, sizeof(double));
+  int ig;
+  fx[ijkgd_c[0] - 1] = 2.0 * recnrmex[0] * rhgr[0];
+  /* the VECTOR order (1720-1732): all +G first, then all -G */
+  for (ig = 1; ig <= nplw5; ++ig) {
+    const double p4 = 2.0 * recnrmex[ig];
+    const int ijk = ijkgd_c[ig] - 1;
+    fx[ijk] = p4 * rhgr[2 * ig];
+    fy[ijk] = p4 * rhgr[2 * ig + 1];
+  }
+  for (ig = 1; ig <= nplw5; ++ig) {
+    const double p4 = 2.0 * recnrmex[ig];
+    const int ijkm = ijkgd_c[-ig] - 1;
+    fx[ijkm] = p4 * rhgr[2 * ig];
+    fy[ijkm] = -(p4 * rhgr[2 * ig + 1]);
+  }
+  qxo_rfft3(-1, fx, fy, n1, n2, n3);
+  memcpy(vhar, fx, sizeof(double) * N);
+  free(fx);
+  free(fy);
+}
+
+/* chgr2g (Sources/engrad.f90:1253-1297): rho on the grid (FFT order) -> the
+ * forward rfft3 (x 1/N) -> rhgr(2ig..2ig+1) = value at ijkgd(ig), ig = 0..nplw5. */
+void qxo_chgr2g(const double *rho, int nplw5, const int *ijkgd_c, int n1, int n2, int n3,
+                double *rhgr)
+{
+  const size_t N = (size_t)n1 * n2 * n3;
+  double *fx = (double *)malloc(N * sizeof(double)), *fy = (double *)calloc(N, sizeof(double));
+  int ig;
+  memcpy(fx, rho, N * sizeof(double));
+  qxo_rfft3(1, fx, fy, n1, n2, n3);
+  for (ig = 0; ig <= nplw5; ++ig) {
+    const int ijk = ijkgd_c[ig] - 1;
+    rhgr[2 * ig] = fx[ijk];
+    rhgr[2 * ig + 1] = fy[ijk];
+  }
+  free(fx);
+  free(fy);
+}
+
+/* chgg2r (Sources/engrad.f90:1302-1353): rhgr at ijkgd(ig), ig = 0..nplw5, then
+ * its conjugate at ijkgd(-ig), ig = 1..nplw5, and the inverse rfft3. */
+void qxo_chgg2r(const double *rhgr, int nplw5, const int *ijkgd_c, int n1, int n2, int n3,
+                double *rho)
+{
+  const size_t N = (size_t)n1 * n2 * n3;
+  double *fx = (double *)calloc(N, sizeof(double)), *fy = (double *)calloc(N, sizeof(double));
+  int ig;
+  for (ig = 0; ig <= nplw5; ++ig) {
+    const int ijk = ijkgd_c[ig] - 1;
+    fx[ijk] = rhgr[2 * ig];
+    fy[ijk] = rhgr[2 * ig + 1];
+  }
+  for (ig = 1; ig <= nplw5; ++ig) {
+    const int ijk = ijkgd_c[-ig] - 1;
+    fx[ijk] = rhgr[2 * ig];
+    fy[ijk] = -rhgr[2 * ig + 1];
+  }
+  qxo_rfft3(-1, fx, fy, n1, n2, n3);
+  memcpy(rho, fx, N * sizeof(double));
+  free(fx);
+  free(fy);
+}
+
+/* DETMAT (Sources/funcs.f90:66-128): determinant by row scaling and Gaussian
+ * elimination with a row-permutation vector; w (column-major, leading
+ * dimension mx) is overwritten. */
+static double detmat(double *w, int n, int *ip, int mx)
+{
+  const double dzero = 1e-30;
+  double fugo = 1.0, det = 1.0;
+  int i, j, j2, jpv = 0;
+#define W_(r, c) w[(r) + (size_t)mx * (c)]
+  for (i = 0; i < n; ++i) ip[i] = i;
+  for (i = 0; i < n; ++i) {
+    double pivot = 0.0;
+    for (j = 0; j < n; ++j) pivot = fmax(pivot, fabs(W_(i, j)));
+    if (pivot < dzero) return 0.0;
+    for (j = 0; j < n; ++j) W_(i, j) = W_(i, j) / pivot;
+    det = det * pivot;
+  }
+  for (i = 0; i < n - 1; ++i) {
+    double pivot = 0.0;
+    int ipv;
+    for (j = i; j < n; ++j) {
+      const int jj = ip[j];
+      if (fabs(W_(jj, i)) > pivot) {
+        pivot = fabs(W_(jj, i));
+        jpv = j;
+      }
+    }
+    if (pivot < dzero) return 0.0;
+    if (jpv != i) {
+      const int t = ip[jpv];
+      ip[jpv] = ip[i];
+      ip[i] = t;
+      fugo = fugo * (-1.0);
+    }
+    ipv = ip[i];
+    for (j = i + 1; j < n; ++j) {
+      const int jj = ip[j];
+      const double dmj = W_(jj, i) / W_(ipv, i);
+      W_(jj, i) = 0.0;
+      for (j2 = i + 1; j2 < n; ++j2) W_(jj, j2) = W_(jj, j2) - dmj * W_(ipv, j2);
+    }
+  }
+  for (i = 0; i < n; ++i) det = det * W_(ip[i], i);
+#undef W_
+  return det * fugo;
+}
+
+/* INVERS (Sources/funcs.f90:5-61): inverse by cofactors over determinants of
+ * the matrix scaled by its largest element.  w, wi: n x n in arrays of
+ * leading dimension mx.  Returns ierr (1 = singular). */
+int qxo_invers(const double *w, double *wi, int n, int mx)
+{
+  double *work = (double *)malloc(sizeof(double) * (size_t)mx * mx);
+  int *ip = (int *)malloc(sizeof(int) * (size_t)mx);
+  double elmax = 0.0, det;
+  int i, j, ii, jj;
+  for (i = 0; i < n; ++i)
+    for (j = 0; j < n; ++j) elmax = fmax(elmax, fabs(w[i + (size_t)mx * j]));
+  for (i = 0; i < n; ++i)
+    for (j = 0; j < n; ++j) work[i + (size_t)mx * j] = w[i + (size_t)mx * j] / elmax;
+  det = detmat(work, n, ip, mx);
+  if (fabs(det) < 1e-30) {
+    free(work);
+    free(ip);
+    return 1;
+  }
+  for (i = 0; i < n; ++i)
+    for (j = 0; j < n; ++j) {
+      int isum = 0;
+      double detij;
+      for (ii = 0; ii < n; ++ii) {
+        int jsum = 0;
+        if (ii == j) continue;
+        for (jj = 0; jj < n; ++jj) {
+          if (jj == i) continue;
+          work[isum + (size_t)mx * jsum] = w[ii + (size_t)mx * jj] / elmax;
+          ++jsum;
+        }
+        ++isum;
+      }
+      detij = detmat(work, n - 1, ip, mx);
+      /* (-1)**(I+J) with 1-based I, J */
+      wi[i + (size_t)mx * j] = (((i + j) % 2) ? -1.0 : 1.0) * detij / det;
+    }
+  for (i = 0; i < n; ++i)
+    for (j = 0; j < n; ++j) wi[i + (size_t)mx * j] = wi[i + (size_t)mx * j] / elmax;
+  free(work);
+  free(ip);
+  return 0;
+}
+
+/* mxchgg_pulay (Sources/chgdns.f90:954-1150), the mixing part on node 0 (the
+ * chgg2r at its end is qxo_chgg2r).  State of module mxchgg_variables passed
+ * explicitly: kntchg, unr/dfmr (nplwcex x itratn, column-major), aij (itratn x
+ * itratn).  rhgr (in: output density, out: mixed density in its first nplwcex
+ * entries, the rest untouched) and rinr (in: input density; out: = rhgr over
+ * nplw5ex).  cmk_out (itratn) receives the Pulay coefficients of the last
+ * successful combination (zeros on a Kerker step).  Returns the number of
+ * history resets taken (INVERS failures and the small-amax trap). */
+int qxo_mxchgg_pulay(double *rhgr, double *rinr, int nplwcex, int nplw5ex, const double *dkgnrm,
+                     double rvol, double aslh, double bslh, double factq1, double wkerker,
+                     int lreset, int lmixhold, int itratn, int *kntchg_io, double *unr,
+                     double *dfmr, double *aij, double *cmk_out)
+{
+  const double small = 1e-15;
+  const size_t ld = (size_t)nplwcex;
+  int kntchg = *kntchg_io + 1, ierr, nreset = 0, ig, i1, ii, jj, i, j;
+  double *betakl = (double *)calloc((size_t)itratn * itratn, sizeof(double));
+  double *cmk = (double *)calloc((size_t)itratn, sizeof(double));
+#define A_(r, c) aij[(r) + (size_t)itratn * (c)]
+#define B_(r, c) betakl[(r) + (size_t)itratn * (c)]
+  for (i = 0; i < itratn; ++i) cmk_out[i] = 0.0;
+  if (kntchg > itratn || lreset) kntchg = 1;
+  for (;;) {
+    const int k = kntchg - 1;
+    for (ig = 0; ig < nplwcex; ++ig) {
+      unr[ig + ld * k] = rinr[ig];
+      dfmr[ig + ld * k] = rhgr[ig] - rinr[ig];
+    }
+    for (i1 = 0; i1 < kntchg; ++i1) {
+      double dfmsq = 0.0, factq;
+      for (ig = 2; ig < nplwcex; ++ig) {
+        factq = (dkgnrm[ig] + factq1) / dkgnrm[ig];
+        dfmsq = dfmsq + dfmr[ig + ld * i1] * dfmr[ig + ld * k] * factq;
+      }
+      factq = (dkgnrm[2] + factq1) / dkgnrm[2];
+      dfmsq = 2.0 * dfmsq + dfmr[ld * i1] * dfmr[ld * k] * factq;
+      dfmsq = dfmsq * rvol;
+      A_(i1, k) = dfmsq;
+      A_(k, i1) = dfmsq;
+    }
+    ierr = 0;
+    if (kntchg == 1) {
+      for (ig = 0; ig < nplwcex; ++ig) {
+        double f = dkgnrm[ig] / (dkgnrm[ig] + bslh);
+        f = aslh * (1.0 - wkerker + wkerker * f);
+        rhgr[ig] = rinr[ig] + f * dfmr[ig + ld * k];
+      }
+    } else {
+      const int n = kntchg, icntr = n / 2 + n % 2;
+      const double amax = fabs(A_(icntr - 1, icntr - 1));
+      if (amax < small) {
+        kntchg = 1;
+        ++nreset;
+        continue;
+      }
+      for (j = 0; j < n; ++j)
+        for (i = 0; i < n; ++i) A_(i, j) = A_(i, j) / amax;
+      ierr = qxo_invers(aij, betakl, n, itratn);
+      if (ierr == 0) {
+        double bbnb = 0.0;
+        for (j = 0; j < n; ++j)
+          for (i = 0; i < n; ++i) A_(i, j) = A_(i, j) * amax;
+        for (j = 0; j < n; ++j)
+          for (i = 0; i < n; ++i) B_(i, j) = B_(i, j) / amax;
+        for (ii = 0; ii < n; ++ii) {
+          cmk[ii] = 0.0;
+          for (jj = 0; jj < n; ++jj) {
+            bbnb = bbnb + B_(ii, jj);
+            cmk[ii] = cmk[ii] + B_(ii, jj);
+          }
+        }
+        for (ii = 0; ii < n; ++ii) cmk[ii] = cmk[ii] / bbnb;
+        for (ig = 0; ig < nplwcex; ++ig) rhgr[ig] = 0.0;
+        for (ii = 0; ii < n; ++ii)
+          for (ig = 0; ig < nplwcex; ++ig) {
+            double f = dkgnrm[ig] / (dkgnrm[ig] + bslh);
+            f = aslh * (1.0 - wkerker + wkerker * f);
+            rhgr[ig] = rhgr[ig] + cmk[ii] * (unr[ig + ld * ii] + f * dfmr[ig + ld * ii]);
+          }
+        for (ii = 0; ii < n; ++ii) cmk_out[ii] = cmk[ii];
+      } else {
+        const int nhi = 1;
+        ++nreset;
+        kntchg = (kntchg - nhi > 1) ? kntchg - nhi : 1;
+        for (ii = 0; ii < kntchg - 1; ++ii)
+          for (ig = 0; ig < nplwcex; ++ig) {
+            unr[ig + ld * ii] = unr[ig + ld * (ii + nhi)];
+            dfmr[ig + ld * ii] = dfmr[ig + ld * (ii + nhi)];
+          }
+        for (j = 0; j < kntchg - 1; ++j)
+          for (i = 0; i < kntchg - 1; ++i) A_(i, j) = A_(i + nhi, j + nhi) * amax;
+      }
+    }
+    if (ierr == 0) break;
+  }
+  if (kntchg == itratn) {
+    for (i = 0; i < kntchg - 1; ++i)
+      for (ig = 0; ig < nplwcex; ++ig) {
+        unr[ig + ld * i] = unr[ig + ld * (i + 1)];
+        dfmr[ig + ld * i] = dfmr[ig + ld * (i + 1)];
+      }
+    for (j = 0; j < kntchg - 1; ++j)
+      for (i = 0; i < kntchg - 1; ++i) A_(i, j) = A_(i + 1, j + 1);
+    if (lmixhold) kntchg = kntchg - 1;
+  }
+#undef A_
+#undef B_
+  for (ig = 0; ig < nplw5ex; ++ig) rinr[ig] = rhgr[ig];
+  *kntchg_io = kntchg;
+  free(betakl);
+  free(cmk);
+  return nreset;
+}
+
+/* mxchgg_simple (Sources/chgdns.f90:1155-1210) and mxchgg_none (907-949),
+ * mixing part: ph = aslh (simple) or 1 (none), rhgr = (1-ph) rinr + ph rhgr
+ * over nplw5ex; then rinr = rhgr. */
+void qxo_mxchgg_simple(double *rhgr, double *rinr, int nplw5ex, double aslh, int none)
+{
+  int ig;
+  if (!none) {
+    const double ph = aslh;
+    for (ig = 0; ig < nplw5ex; ++ig) rhgr[ig] = (1.0 - ph) * rinr[ig] + ph * rhgr[ig];
+  }
+  for (ig = 0; ig < nplw5ex; ++ig) rinr[ig] = rhgr[ig];
+}
+
 /* enkbpp_g2 (Sources/nlkbpp.f90:1763-1824), Gamma point. */
 double qxo_enkbpp_g(const double *slmir, int lmmax, int natom, const int *lmx_atom,
                     const double *clalp_atom, const double *occ, int nb, double rvol)

@@ -66,6 +66,20 @@ void qxo_kbnlpp_frc2(const double *eqlmr, int nplwex, int lmmax, int natom, cons
                      const double *vclalp_atom, const double *cgjr, int nb, const double *occ,
                      const double *slmir, const double *gx, const double *gy, const double *gz,
                      double *fnlc);
+void qxo_xclda(const double *rho, const double *rhocore, int n, double rdelv, double *vexc,
+               double *eeexc, double *e4);
+void qxo_harting(const double *rhgr, int nplw5, const double *recnrmex, const int *ijkgd_c,
+                 int n1, int n2, int n3, double *vhar);
+void qxo_chgr2g(const double *rho, int nplw5, const int *ijkgd_c, int n1, int n2, int n3,
+                double *rhgr);
+void qxo_chgg2r(const double *rhgr, int nplw5, const int *ijkgd_c, int n1, int n2, int n3,
+                double *rho);
+int qxo_invers(const double *w, double *wi, int n, int mx);
+int qxo_mxchgg_pulay(double *rhgr, double *rinr, int nplwcex, int nplw5ex, const double *dkgnrm,
+                     double rvol, double aslh, double bslh, double factq1, double wkerker,
+                     int lreset, int lmixhold, int itratn, int *kntchg_io, double *unr,
+                     double *dfmr, double *aij, double *cmk_out);
+void qxo_mxchgg_simple(double *rhgr, double *rinr, int nplw5ex, double aslh, int none);
 double qxo_enkbpp_g(const double *slmir, int lmmax, int natom, const int *lmx_atom,
                     const double *clalp_atom, const double *occ, int nb, double rvol);
 

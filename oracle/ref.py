@@ -235,3 +235,29 @@ def kbnlpp_frc2(cgjr, occ, gx, gy, gz, natom):
     L.qxref_kbnlpp_frc2.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
     L.qxref_kbnlpp_frc2(c, n, nb, natom, _F(occ), _F(gx), _F(gy), _F(gz), f)
     return np.ascontiguousarray(f.T)
+
+
+def xclda(rho, rhocore, rdelv):
+    """Reference xclda; rhocore with negative entries means no core charge there."""
+    L = lib()
+    r = _F(rho)
+    n = r.size
+    v = np.zeros(n)
+    e = np.zeros(n)
+    e4 = np.zeros(4)
+    L.qxref_xclda.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, _dp, _dp]
+    L.qxref_xclda(r, _F(rhocore), n, float(rdelv), v, e, e4)
+    return v, e, e4
+
+
+def invers(w):
+    """Reference INVERS; returns (inverse, ierr)."""
+    L = lib()
+    a = np.asfortranarray(np.asarray(w, np.float64))
+    n = a.shape[0]
+    flat = np.ascontiguousarray(a.T).ravel()
+    wi = np.zeros(n * n)
+    ierr = C.c_int(0)
+    L.qxref_invers.argtypes = [_dp, _dp, C.c_int, C.POINTER(C.c_int)]
+    L.qxref_invers(flat, wi, n, C.byref(ierr))
+    return wi.reshape(n, n).T.copy(), ierr.value

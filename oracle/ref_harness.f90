@@ -472,3 +472,35 @@ subroutine qxref_kbnlpp_frc2(cgjr, nplwex, nband, natom, occ, gx, gy, gz, fnlc) 
   call kbnlpp_frc2(nfile, 0, 1, fnlc, str, .false., cgjr, nplwex, nplwex/2 - 1, gx, gy, gz, &
 &     nband, 1, nband, nband, occ, iod, natom, nhk, nhk, natom, lkb, lkg, yc, ys, 0)
 end subroutine
+
+! xclda (Sources/vxc.f90:961): LDA exchange-correlation on n mesh points
+subroutine qxref_xclda(rho, rhocore, n, rdelv, vexc, eeexc, e4) bind(C, name="qxref_xclda")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: n
+  real(c_double) :: rho(n), rhocore(n), vexc(n), eeexc(n), e4(4)
+  real(c_double), value :: rdelv
+  real*8 :: a, b, c, d, rd
+  rd = rdelv
+  call xclda(rho, rhocore, n, rd, vexc, eeexc, a, b, c, d)
+  e4(1) = a
+  e4(2) = b
+  e4(3) = c
+  e4(4) = d
+end subroutine
+
+! INVERS (Sources/funcs.f90:5): cofactor inverse of an n x n matrix
+subroutine qxref_invers(w, wi, n, ierr) bind(C, name="qxref_invers")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: n
+  real(c_double) :: w(n, n), wi(n, n)
+  integer(c_int) :: ierr
+  real*8, allocatable :: work(:,:)
+  integer, allocatable :: ip(:)
+  integer :: ie
+  allocate(work(n, n), ip(n))
+  call INVERS(w, wi, n, work, ip, n, ie)
+  ierr = ie
+  deallocate(work, ip)
+end subroutine

@@ -112,6 +112,19 @@ def main():
                            si8.cell.volume)
     out["setnlc_eqlmr"] = np.asarray(eqr)[:, :, ::SUB]
     out["setnlc_vclalp"] = np.asarray(vclr)
+    # xclda (both Perdew-Zunger branches, core charge on part of the mesh) and INVERS
+    from test_scf_glue import xc_inputs
+    rho, core = xc_inputs()
+    v, e, e4 = R.xclda(rho, core, 0.37)
+    out["xclda_vexc"] = np.asarray(v)[::7]
+    out["xclda_eeexc"] = np.asarray(e)[::7]
+    out["xclda_e4"] = np.asarray(e4)
+    rng = np.random.default_rng(21)
+    for n in (2, 5, 9):
+        x = rng.standard_normal((n, n))
+        wi, ierr = R.invers(x @ x.T + 0.05 * np.eye(n))
+        assert ierr == 0
+        out[f"invers_{n}"] = np.asarray(wi)
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes,", len(out), "arrays")
 
