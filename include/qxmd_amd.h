@@ -196,6 +196,61 @@ int qx_icholesky(qx_ctx *ctx, double *s, int n, int *ok);
 int qx_softchg(qx_ctx *ctx, const double *cgjr, int nb, const double *occ, double rvol,
                double *rho);
 
+/* ---------------- SCF glue between band updates ----------------------------
+ * Sphere arrays: rhgr(0:2*nplw5+1) real-packed (Re, Im per G, G index
+ * 0..nplw5); ijkgd points at ijkgd(-nplw5) of the array ijkgd(-nplw5:nplw5)
+ * (1-based grid offsets, as set_rciprl builds it); recnrmex(0:nplw5).
+ * Grid arrays in FD order through mfd2ft(ntotfd) (1-based FFT offsets; null
+ * means FFT order, ntotfd = kfft1 kfft2 kfft3). */
+/* xclda (Sources/vxc.f90:961-1017): LDA (Perdew-Zunger) vexc(mshnod) and
+ * eeexc(mshnod) in Ry; e4 = {eexc, evexc, ecor, evcor}.  rhocore may be null;
+ * negative rhocore entries mean no core charge there. */
+int qx_xclda(const double *rho, const double *rhocore, int mshnod, double rdelv, double *vexc,
+             double *eeexc, double *e4);
+/* harting (Sources/engrad.f90:1673-1743): vhar = c2r of 2 recnrmex rhgr. */
+int qx_harting(double *vhar, const double *rhgr, int nplw5, const double *recnrmex,
+               const int *mfd2ft, int ntotfd, const int *nd1vks, const int *ijkgd);
+/* chgr2g (Sources/engrad.f90:1253-1297): rho -> rhgr (forward x1/N). */
+int qx_chgr2g(const double *rho, double *rhgr, int nplw5, const int *mfd2ft, int ntotfd,
+              const int *nd1vks, const int *ijkgd);
+/* chgg2r (Sources/engrad.f90:1302-1353): rhgr -> rho. */
+int qx_chgg2r(double *rho, const double *rhgr, int nplw5, const int *mfd2ft, int ntotfd,
+              const int *nd1vks, const int *ijkgd);
+/* mxchgg_alloc + set_g2ming2max_in_chgdns (Sources/chgdns.f90:529-690):
+ * imxchg 0 (none), 1 (Pulay, itratn <= 32) or 3 (simple); dkgnrm(nplwcex);
+ * rvol is the cell volume (the reference's rvol).
+ * Resets the mixing history of the context. */
+int qx_mxchgg_alloc(qx_ctx *ctx, int nplwcex, int imxchg, int itratn, double cmetric,
+                    double wkerker, double g2min, double g2max, const double *dkgnrm,
+                    double rvol);
+/* mxchgg (Sources/chgdns.f90:796-1210) on host arrays: rhgr (output density
+ * in, mixed out), rinr (input density in, = rhgr out), glocal = chgg2r(rhgr);
+ * *lreset cleared.  The history stays on the device between calls. */
+int qx_mxchgg(qx_ctx *ctx, int *lreset, double *rhgr, double *rinr, int nplw5, double aslh,
+              double bslh, double *glocal, const int *mfd2ft, int ntotfd, const int *ijkgd,
+              int lmixhold);
+/* Device-resident SCF glue.  qx_scf_set_grid fixes the density sphere, the
+ * Hartree kernel recnrmex, the external (local pseudopotential) potential
+ * vext and the core charge (null: none), grid arrays in FFT order. */
+int qx_scf_set_grid(qx_ctx *ctx, int nplw5, const int *ijkgd, const double *recnrmex,
+                    const double *vext, const double *rhocore, double rdelv);
+/* softchg + outchg (Sources/chgdns.f90:29-383): density occ/rvol psi^2 of nb
+ * bands of a slot (rvol = cell volume), then its rhgr, kept on the device. */
+int qx_pwpchg_slot(qx_ctx *ctx, int slot, int nb, const double *occ, double rvol, int prec);
+/* pwpmxchgg2 (Sources/chgdns.f90:724-790): mix the device rhgr with rinr,
+ * then rho = chgg2r(rhgr); info = {kntchg, history resets} if non-null. */
+int qx_pwpmxchgg(qx_ctx *ctx, int lreset, double aslh, double bslh, int lmixhold, int *info);
+/* harting + xclda + pwpunifylc2 (Sources/pwlda.f90:1640-2030): the local
+ * potential of the next band update, V = vhar + vext + vexc, written into the
+ * context (as qx_set_glocal does); ene = {ehar, elcl, eexc, evexc, ecor,
+ * evcor} (Sources/pwlda.f90:1868-1873, 1963-1968). */
+int qx_scf_potential(qx_ctx *ctx, double *ene);
+/* read (write) the device SCF arrays: 0 rho, 1 rhgr, 2 rinr, 3 vhar, 4 vexc,
+ * 5 eeexc, 6 glocal (FFT order), 7 Pulay coefficients (itratn, read only);
+ * only 0-2 can be written. */
+int qx_scf_get(qx_ctx *ctx, int what, double *host);
+int qx_scf_put(qx_ctx *ctx, int what, const double *host);
+
 #ifdef __cplusplus
 }
 #endif

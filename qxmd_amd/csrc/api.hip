@@ -273,6 +273,7 @@ int qx_destroy(qx_ctx* h) {
     for (auto& e : t.ev) (void)hipEventDestroy(e);
   lt_destroy(c);
   nl_destroy(c);
+  scf_destroy(c);
   (void)hipStreamDestroy(c->stream);
   delete h;
   return 0;
@@ -764,6 +765,88 @@ int qx_icholesky(qx_ctx* h, double* s, int n, int* ok) {
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < n; ++i) s[i + (size_t)n * j] = xr[(size_t)i * n + j];  // row-major -> col-major
   return 0;
+}
+
+// ---------------- SCF glue --------------------------------------------------
+// xclda (Sources/vxc.f90:961-1017) on host arrays; rhocore may be null
+int qx_xclda(const double* rho, const double* rhocore, int mshnod, double rdelv, double* vexc,
+             double* eeexc, double* e4) {
+  int rc = dropin_xclda(rho, rhocore, mshnod, rdelv, vexc, eeexc, e4);
+  return rc ? fail(rc, "qx_xclda: failed (%d)", rc) : 0;
+}
+
+int qx_harting(double* vhar, const double* rhgr, int nplw5, const double* recnrmex,
+               const int* mfd2ft, int ntotfd, const int* nd1vks, const int* ijkgd) {
+  int rc = dropin_sphere(1, vhar, const_cast<double*>(rhgr), nplw5, recnrmex, mfd2ft, ntotfd,
+                         nd1vks, ijkgd);
+  return rc ? fail(rc, "qx_harting: bad sizes or index maps (%d)", rc) : 0;
+}
+
+int qx_chgr2g(const double* rho, double* rhgr, int nplw5, const int* mfd2ft, int ntotfd,
+              const int* nd1vks, const int* ijkgd) {
+  int rc = dropin_sphere(2, const_cast<double*>(rho), rhgr, nplw5, nullptr, mfd2ft, ntotfd,
+                         nd1vks, ijkgd);
+  return rc ? fail(rc, "qx_chgr2g: bad sizes or index maps (%d)", rc) : 0;
+}
+
+int qx_chgg2r(double* rho, const double* rhgr, int nplw5, const int* mfd2ft, int ntotfd,
+              const int* nd1vks, const int* ijkgd) {
+  int rc = dropin_sphere(0, rho, const_cast<double*>(rhgr), nplw5, nullptr, mfd2ft, ntotfd,
+                         nd1vks, ijkgd);
+  return rc ? fail(rc, "qx_chgg2r: bad sizes or index maps (%d)", rc) : 0;
+}
+
+int qx_mxchgg_alloc(qx_ctx* h, int nplwcex, int imxchg, int itratn, double cmetric,
+                    double wkerker, double g2min, double g2max, const double* dkgnrm,
+                    double rvol) {
+  int rc = scf_mixer_alloc(&h->c, nplwcex, imxchg, itratn, cmetric, wkerker, g2min, g2max, dkgnrm,
+                           rvol);
+  return rc ? fail(rc, "qx_mxchgg_alloc: imxchg must be 0, 1 or 3 and 1 <= itratn <= 32 (%d)", rc)
+            : 0;
+}
+
+int qx_mxchgg(qx_ctx* h, int* lreset, double* rhgr, double* rinr, int nplw5, double aslh,
+              double bslh, double* glocal, const int* mfd2ft, int ntotfd, const int* ijkgd,
+              int lmixhold) {
+  int rc = dropin_mxchgg(&h->c, lreset, rhgr, rinr, nplw5, aslh, bslh, glocal, mfd2ft, ntotfd,
+                         ijkgd, lmixhold);
+  return rc ? fail(rc, "qx_mxchgg: mixer not set up or bad sizes (%d)", rc) : 0;
+}
+
+int qx_scf_set_grid(qx_ctx* h, int nplw5, const int* ijkgd, const double* recnrmex,
+                    const double* vext, const double* rhocore, double rdelv) {
+  int rc = scf_set_grid(&h->c, nplw5, ijkgd, recnrmex, vext, rhocore, rdelv);
+  return rc ? fail(rc, "qx_scf_set_grid: bad sphere or index map (%d)", rc) : 0;
+}
+
+int qx_pwpchg_slot(qx_ctx* h, int slot, int nb, const double* occ, double rvol, int prec) {
+  Ctx* c = &h->c;
+  if (slot < 0 || slot >= Ctx::kSlots || nb < 1 || nb > c->nbmax)
+    return fail(-1, "qx_pwpchg_slot: bad slot or band count");
+  int rc = scf_pwpchg(c, c->d_slot[slot], nb, occ, rvol, prec);
+  return rc ? fail(rc, "qx_pwpchg_slot: call qx_scf_set_grid first (%d)", rc) : 0;
+}
+
+int qx_pwpmxchgg(qx_ctx* h, int lreset, double aslh, double bslh, int lmixhold, int* info) {
+  int rc = scf_mix(&h->c, lreset, aslh, bslh, lmixhold, info);
+  return rc ? fail(rc, "qx_pwpmxchgg: call qx_scf_set_grid and qx_mxchgg_alloc first (%d)", rc)
+            : 0;
+}
+
+int qx_scf_potential(qx_ctx* h, double* ene) {
+  int rc = scf_potential(&h->c, ene);
+  return rc ? fail(rc, "qx_scf_potential: call qx_scf_set_grid first (%d)", rc) : 0;
+}
+
+int qx_scf_get(qx_ctx* h, int what, double* host) {
+  int rc = scf_io(&h->c, what, host, false);
+  return rc ? fail(rc, "qx_scf_get: unknown or unset array %d", what) : 0;
+}
+
+int qx_scf_put(qx_ctx* h, int what, const double* host) {
+  int rc = scf_io(&h->c, what, const_cast<double*>(host), true);
+  return rc ? fail(rc, "qx_scf_put: only rho (0), rhgr (1) and rinr (2) can be set (%d)", what)
+            : 0;
 }
 
 }  // extern "C"

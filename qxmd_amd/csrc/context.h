@@ -23,6 +23,23 @@ struct Pencil {
   std::vector<int> col_kmod;    // k mod n2 of column c
 };
 
+// dense complex 3-D FFT over the whole grid (fullgrid.hip): lines along x,
+// y and z transformed in LDS; d_g holds the N complex values, x fastest
+struct FullGrid {
+  int n1 = 0, n2 = 0, n3 = 0;
+  FftPlan p[3] = {};
+  double* d_tw[3] = {nullptr, nullptr, nullptr};
+  double2* d_g = nullptr;
+  double *d_x = nullptr, *d_y = nullptr;  // host-array staging of the drop-ins
+};
+int fg_prepare(FullGrid& f, int n1, int n2, int n3, bool staging);
+void fg_release(FullGrid& f);
+// sign -1: forward (exp(-i)), +1: backward; scale applied on the last pass
+int fg_transform(FullGrid& f, hipStream_t st, int sign, double scale);
+// the grid and stream of the context-free drop-ins (rfft3, fft3, harting, ...)
+int dropin_grid(int n1, int n2, int n3, FullGrid** f, hipStream_t* st);
+int dropin_stream(hipStream_t* st);
+
 struct Ctx {
   int dev = 0;
   int n1 = 0, n2 = 0, n3 = 0;
@@ -87,6 +104,30 @@ struct Ctx {
   double *d_ycos = nullptr, *d_ysin = nullptr;  // (nplwcs+1) x natom
   int sf_natom = 0, sf_nplwcs = 0;
   double gmax2 = 0.0;                           // largest |G|^2 of the sphere
+  // SCF glue on the dense grid (scfglue.hip): the density sphere, Hartree
+  // kernel, fixed potentials, the charge-density vectors and the mixer history
+  FullGrid fg;
+  struct Scf {
+    int nplw5 = 0;
+    int* d_ijk = nullptr;         // ijkgd(-nplw5:nplw5) as 0-based grid offsets
+    double* d_p4 = nullptr;       // 2 recnrmex(ig), ig = 0..nplw5
+    double *d_vext = nullptr, *d_core = nullptr;  // FFT order; d_core null: no core
+    double rdelv = 0.0;
+    double *d_rhgr = nullptr, *d_rinr = nullptr;  // 2 (nplw5+1)
+    double *d_vhar = nullptr, *d_vexc = nullptr, *d_eeexc = nullptr;
+    double* d_red = nullptr;      // reduction partials and results
+    bool z_vhar = false;          // fg.d_g holds rho + i vhar of the current rhgr
+    // mixer (module mxchgg_variables)
+    int imxchg = -1, itratn = 0, nplwcex = 0, kntchg = 0;
+    double rvol = 0.0, factq1 = 0.0, wkerker = 1.0;
+    double *d_unr = nullptr, *d_dfmr = nullptr;   // nplwcex x itratn physical columns
+    double *d_dkg = nullptr, *d_wq = nullptr;     // dkgnrm and metric weights (nplwcex)
+    double* d_cmk = nullptr;
+    std::vector<int> col;         // logical history column -> physical column
+    std::vector<double> aij;      // itratn x itratn, column-major
+    std::vector<double> cmk;
+    int nreset = 0;
+  } scf;
   int lt_gemm_id = -1;                    // timer the GEMM launches report to
   void* blaslt = nullptr;                 // hipBLASLt state (handle, plans, FP32 copies)
 };
@@ -141,6 +182,23 @@ int nl_setnlc_g(Ctx* c, int nplw, int ntype, const int* nhk1, const int* nhk2, i
                 const int* lkbppi, const int* lking, const double* ycos, const double* ysin,
                 int nplwcs, double rvol, double* vclalp_out);
 void nl_destroy(Ctx* c);
+void scf_destroy(Ctx* c);
+// SCF glue (scfglue.hip)
+int scf_set_grid(Ctx* c, int nplw5, const int* ijkgd, const double* recnrmex, const double* vext,
+                 const double* rhocore, double rdelv);
+int scf_mixer_alloc(Ctx* c, int nplwcex, int imxchg, int itratn, double cmetric, double wkerker,
+                    double g2min, double g2max, const double* dkgnrm, double rvol);
+int scf_pwpchg(Ctx* c, const double* psi, int nb, const double* occ, double rvol, int prec);
+int scf_mix(Ctx* c, int lreset, double aslh, double bslh, int lmixhold, int* info);
+int scf_potential(Ctx* c, double* ene);
+int scf_io(Ctx* c, int what, double* host, bool put);
+int dropin_xclda(const double* rho, const double* rhocore, int n, double rdelv, double* vexc,
+                 double* eeexc, double* e4);
+int dropin_sphere(int mode, double* grid, double* rhgr, int nplw5, const double* recnrmex,
+                  const int* mfd2ft, int ntotfd, const int* nd, const int* ijkgd);
+int dropin_mxchgg(Ctx* c, int* lreset, double* rhgr, double* rinr, int nplw5, double aslh,
+                  double bslh, double* glocal, const int* mfd2ft, int ntotfd, const int* ijkgd,
+                  int lmixhold);
 // KB Hellmann-Feynman forces (forces.hip): fnlc(3, natom) += kbnlpp_frc2
 int kb_forces(Ctx* c, const double* psi, int nb, const double* occ_host, const double* gx,
               const double* gy, const double* gz, double* fnlc_host);

@@ -97,51 +97,45 @@ __global__ void k_fg_unpack(const double2* __restrict__ g, double* x, double* y,
   y[i] = real_only ? 0.0 : v.y;
 }
 
-struct FullGrid {
-  int n1 = 0, n2 = 0, n3 = 0;
-  FftPlan p[3];
-  double* d_tw[3] = {nullptr, nullptr, nullptr};
-  double2* d_g = nullptr;
-  double *d_x = nullptr, *d_y = nullptr;
-  hipStream_t stream = nullptr;
-};
-
-static FullGrid g_fg;
-
-static int fg_prepare(int n1, int n2, int n3) {
-  if (g_fg.n1 == n1 && g_fg.n2 == n2 && g_fg.n3 == n3) return 0;
-  if (g_fg.d_g) {
-    (void)hipFree(g_fg.d_g);
-    (void)hipFree(g_fg.d_x);
-    (void)hipFree(g_fg.d_y);
-    for (auto& t : g_fg.d_tw) (void)hipFree(t);
-  }
-  if (!g_fg.stream) QX_CHECK(hipStreamCreateWithFlags(&g_fg.stream, hipStreamNonBlocking));
+int fg_prepare(FullGrid& f, int n1, int n2, int n3, bool staging) {
+  if (f.n1 == n1 && f.n2 == n2 && f.n3 == n3 && (!staging || f.d_x)) return 0;
+  fg_release(f);
   const int dims[3] = {n1, n2, n3};
   for (int a = 0; a < 3; ++a) {
-    g_fg.p[a] = make_plan(dims[a]);
-    if (g_fg.p[a].npass < 0 || dims[a] > 2 * 4 * kFgThreads) return -6;
+    f.p[a] = make_plan(dims[a]);
+    if (f.p[a].npass < 0 || dims[a] > 2 * 4 * kFgThreads) return -6;
     std::vector<double> tw(2 * dims[a]);
     for (int k = 0; k < dims[a]; ++k) {
       double ang = 2.0 * M_PI * (double)k / (double)dims[a];
       tw[2 * k] = std::cos(ang);
       tw[2 * k + 1] = std::sin(ang);
     }
-    QX_CHECK(hipMalloc(&g_fg.d_tw[a], sizeof(double) * tw.size()));
-    QX_CHECK(hipMemcpy(g_fg.d_tw[a], tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+    QX_CHECK(hipMalloc(&f.d_tw[a], sizeof(double) * tw.size()));
+    QX_CHECK(hipMemcpy(f.d_tw[a], tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
   }
   long N = (long)n1 * n2 * n3;
-  QX_CHECK(hipMalloc(&g_fg.d_g, sizeof(double2) * N));
-  QX_CHECK(hipMalloc(&g_fg.d_x, sizeof(double) * N));
-  QX_CHECK(hipMalloc(&g_fg.d_y, sizeof(double) * N));
-  g_fg.n1 = n1;
-  g_fg.n2 = n2;
-  g_fg.n3 = n3;
+  QX_CHECK(hipMalloc(&f.d_g, sizeof(double2) * N));
+  if (staging) {
+    QX_CHECK(hipMalloc(&f.d_x, sizeof(double) * N));
+    QX_CHECK(hipMalloc(&f.d_y, sizeof(double) * N));
+  }
+  f.n1 = n1;
+  f.n2 = n2;
+  f.n3 = n3;
   return 0;
 }
 
-static int fg_transform(int sign, double scale) {
-  const int n1 = g_fg.n1, n2 = g_fg.n2, n3 = g_fg.n3;
+void fg_release(FullGrid& f) {
+  if (f.d_g) (void)hipFree(f.d_g);
+  if (f.d_x) (void)hipFree(f.d_x);
+  if (f.d_y) (void)hipFree(f.d_y);
+  for (auto& t : f.d_tw)
+    if (t) (void)hipFree(t);
+  f = FullGrid{};
+}
+
+int fg_transform(FullGrid& f, hipStream_t st, int sign, double scale) {
+  const int n1 = f.n1, n2 = f.n2, n3 = f.n3;
   const int dims[3] = {n1, n2, n3};
   for (int a = 0; a < 3; ++a) {
     const int n = dims[a];
@@ -154,52 +148,75 @@ static int fg_transform(int sign, double scale) {
     if (lds > 65536)
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_fg_lines),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_fg_lines, dim3(blocks), dim3(kFgThreads), lds, g_fg.stream, g_fg.d_g, n1,
-                       n2, n3, a, nl, g_fg.p[a], g_fg.d_tw[a], sign, s);
+    hipLaunchKernelGGL(k_fg_lines, dim3(blocks), dim3(kFgThreads), lds, st, f.d_g, n1, n2, n3, a,
+                       nl, f.p[a], f.d_tw[a], sign, s);
   }
   QX_CHECK(hipGetLastError());
   return 0;
 }
 
+// the drop-ins' own grid and stream (no context involved)
+static FullGrid g_fg;
+static hipStream_t g_fg_stream = nullptr;
+
+int dropin_stream(hipStream_t* st) {
+  if (!g_fg_stream) QX_CHECK(hipStreamCreateWithFlags(&g_fg_stream, hipStreamNonBlocking));
+  *st = g_fg_stream;
+  return 0;
+}
+
+int dropin_grid(int n1, int n2, int n3, FullGrid** f, hipStream_t* st) {
+  int rc = dropin_stream(st);
+  if (rc) return rc;
+  rc = fg_prepare(g_fg, n1, n2, n3, true);
+  if (rc) return rc;
+  *f = &g_fg;
+  return 0;
+}
+
 int fullgrid_rfft3(int key, double* fx, double* fy, int n1, int n2, int n3) {
-  int rc = fg_prepare(n1, n2, n3);
+  FullGrid* f = nullptr;
+  hipStream_t st = nullptr;
+  int rc = dropin_grid(n1, n2, n3, &f, &st);
   if (rc) return rc;
   const long N = (long)n1 * n2 * n3;
   const unsigned nb = (unsigned)((N + 255) / 256);
-  QX_CHECK(hipMemcpyAsync(g_fg.d_x, fx, sizeof(double) * N, hipMemcpyHostToDevice, g_fg.stream));
+  QX_CHECK(hipMemcpyAsync(f->d_x, fx, sizeof(double) * N, hipMemcpyHostToDevice, st));
   if (key == 1) {
-    hipLaunchKernelGGL(k_fg_pack, dim3(nb), dim3(256), 0, g_fg.stream, g_fg.d_x, nullptr, g_fg.d_g, N);
-    if ((rc = fg_transform(-1, 1.0 / (double)N))) return rc;
-    hipLaunchKernelGGL(k_fg_unpack, dim3(nb), dim3(256), 0, g_fg.stream, g_fg.d_g, g_fg.d_x,
-                       g_fg.d_y, N, 0);
+    hipLaunchKernelGGL(k_fg_pack, dim3(nb), dim3(256), 0, st, f->d_x, nullptr, f->d_g, N);
+    if ((rc = fg_transform(*f, st, -1, 1.0 / (double)N))) return rc;
+    hipLaunchKernelGGL(k_fg_unpack, dim3(nb), dim3(256), 0, st, f->d_g, f->d_x,
+                       f->d_y, N, 0);
   } else {
-    QX_CHECK(hipMemcpyAsync(g_fg.d_y, fy, sizeof(double) * N, hipMemcpyHostToDevice, g_fg.stream));
-    hipLaunchKernelGGL(k_fg_hermitian, dim3(nb), dim3(256), 0, g_fg.stream, g_fg.d_x, g_fg.d_y,
-                       g_fg.d_g, n1, n2, n3);
-    if ((rc = fg_transform(+1, 1.0))) return rc;
-    hipLaunchKernelGGL(k_fg_unpack, dim3(nb), dim3(256), 0, g_fg.stream, g_fg.d_g, g_fg.d_x,
-                       g_fg.d_y, N, 1);
+    QX_CHECK(hipMemcpyAsync(f->d_y, fy, sizeof(double) * N, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_fg_hermitian, dim3(nb), dim3(256), 0, st, f->d_x, f->d_y,
+                       f->d_g, n1, n2, n3);
+    if ((rc = fg_transform(*f, st, +1, 1.0))) return rc;
+    hipLaunchKernelGGL(k_fg_unpack, dim3(nb), dim3(256), 0, st, f->d_g, f->d_x,
+                       f->d_y, N, 1);
   }
-  QX_CHECK(hipMemcpyAsync(fx, g_fg.d_x, sizeof(double) * N, hipMemcpyDeviceToHost, g_fg.stream));
-  QX_CHECK(hipMemcpyAsync(fy, g_fg.d_y, sizeof(double) * N, hipMemcpyDeviceToHost, g_fg.stream));
-  QX_CHECK(hipStreamSynchronize(g_fg.stream));
+  QX_CHECK(hipMemcpyAsync(fx, f->d_x, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+  QX_CHECK(hipMemcpyAsync(fy, f->d_y, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+  QX_CHECK(hipStreamSynchronize(st));
   return 0;
 }
 
 int fullgrid_fft3(int key, double* fx, double* fy, int n1, int n2, int n3) {
-  int rc = fg_prepare(n1, n2, n3);
+  FullGrid* f = nullptr;
+  hipStream_t st = nullptr;
+  int rc = dropin_grid(n1, n2, n3, &f, &st);
   if (rc) return rc;
   const long N = (long)n1 * n2 * n3;
   const unsigned nb = (unsigned)((N + 255) / 256);
-  QX_CHECK(hipMemcpyAsync(g_fg.d_x, fx, sizeof(double) * N, hipMemcpyHostToDevice, g_fg.stream));
-  QX_CHECK(hipMemcpyAsync(g_fg.d_y, fy, sizeof(double) * N, hipMemcpyHostToDevice, g_fg.stream));
-  hipLaunchKernelGGL(k_fg_pack, dim3(nb), dim3(256), 0, g_fg.stream, g_fg.d_x, g_fg.d_y, g_fg.d_g, N);
-  if ((rc = fg_transform(key == 1 ? -1 : +1, key == 1 ? 1.0 / (double)N : 1.0))) return rc;
-  hipLaunchKernelGGL(k_fg_unpack, dim3(nb), dim3(256), 0, g_fg.stream, g_fg.d_g, g_fg.d_x, g_fg.d_y,
+  QX_CHECK(hipMemcpyAsync(f->d_x, fx, sizeof(double) * N, hipMemcpyHostToDevice, st));
+  QX_CHECK(hipMemcpyAsync(f->d_y, fy, sizeof(double) * N, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_fg_pack, dim3(nb), dim3(256), 0, st, f->d_x, f->d_y, f->d_g, N);
+  if ((rc = fg_transform(*f, st, key == 1 ? -1 : +1, key == 1 ? 1.0 / (double)N : 1.0))) return rc;
+  hipLaunchKernelGGL(k_fg_unpack, dim3(nb), dim3(256), 0, st, f->d_g, f->d_x, f->d_y,
                      N, 0);
-  QX_CHECK(hipMemcpyAsync(fx, g_fg.d_x, sizeof(double) * N, hipMemcpyDeviceToHost, g_fg.stream));
-  QX_CHECK(hipMemcpyAsync(fy, g_fg.d_y, sizeof(double) * N, hipMemcpyDeviceToHost, g_fg.stream));
-  QX_CHECK(hipStreamSynchronize(g_fg.stream));
+  QX_CHECK(hipMemcpyAsync(fx, f->d_x, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+  QX_CHECK(hipMemcpyAsync(fy, f->d_y, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+  QX_CHECK(hipStreamSynchronize(st));
   return 0;
 }
 

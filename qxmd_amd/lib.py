@@ -82,6 +82,18 @@ SIGNATURES = {
     "qx_softchg": [_vp, _dp, _i, _dp, _d, _dp],
     "qx_rseigqr": [_vp, _vp, _i, _dp],
     "qx_icholesky": [_vp, _vp, _i, _pi],
+    "qx_xclda": [_dp, _vp, _i, _d, _dp, _dp, _dp],
+    "qx_harting": [_dp, _dp, _i, _dp, _vp, _i, _ip, _ip],
+    "qx_chgr2g": [_dp, _dp, _i, _vp, _i, _ip, _ip],
+    "qx_chgg2r": [_dp, _dp, _i, _vp, _i, _ip, _ip],
+    "qx_mxchgg_alloc": [_vp, _i, _i, _i, _d, _d, _d, _d, _dp, _d],
+    "qx_mxchgg": [_vp, _pi, _dp, _dp, _i, _d, _d, _dp, _vp, _i, _ip, _i],
+    "qx_scf_set_grid": [_vp, _i, _ip, _dp, _vp, _vp, _d],
+    "qx_pwpchg_slot": [_vp, _i, _i, _dp, _d, _i],
+    "qx_pwpmxchgg": [_vp, _i, _d, _d, _i, _vp],
+    "qx_scf_potential": [_vp, _dp],
+    "qx_scf_get": [_vp, _i, _dp],
+    "qx_scf_put": [_vp, _i, _dp],
 }
 # entry points beyond the header's core set (batched CG, kinetic energies)
 EXTRA = {

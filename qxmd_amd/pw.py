@@ -308,6 +308,73 @@ class PlaneWave:
               "qx_icholesky")
         return np.asarray(m), bool(ok.value)
 
+    # ---- SCF glue (harting, xclda, potential assembly, charge mixing)
+    def scf_set_grid(self, recnrmex, vext=None, rhocore=None, rdelv=None):
+        """Density sphere of the context's G list, Hartree kernel recnrmex
+        (0..nplw5), external potential and core charge on the grid (FFT order)."""
+        gs = self.gs
+        self._ijk = gs.ijkgd()
+        rd = gs.volume / gs.N if rdelv is None else float(rdelv)
+        vx = None if vext is None else f64(vext)
+        rc_ = None if rhocore is None else f64(rhocore)
+        check(load().qx_scf_set_grid(
+            self.h, gs.nplw5, self._ijk, f64(recnrmex),
+            vx.ctypes.data_as(C.c_void_p) if vx is not None else None,
+            rc_.ctypes.data_as(C.c_void_p) if rc_ is not None else None, rd), "qx_scf_set_grid")
+
+    def mxchgg_alloc(self, imxchg=1, itratn=10, cmetric=20.0, wkerker=1.0, nplwc=None):
+        """Charge mixer over the nplwc sphere (default: the wave-function sphere)."""
+        gs = self.gs
+        nplwc = gs.nplwc if nplwc is None else int(nplwc)
+        dkg = f64(np.repeat(gs.recnrm[: nplwc + 1], 2))
+        check(load().qx_mxchgg_alloc(self.h, 2 * (nplwc + 1), int(imxchg), int(itratn),
+                                     float(cmetric), float(wkerker), float(gs.recnrm[1]),
+                                     float(gs.recnrm[nplwc]), dkg, gs.volume),
+              "qx_mxchgg_alloc")
+
+    def pwpchg_slot(self, slot, nb, occ, prec=FP64):
+        check(load().qx_pwpchg_slot(self.h, slot, nb, f64(occ), self.gs.volume, prec),
+              "qx_pwpchg_slot")
+
+    def pwpmxchgg(self, aslh, bslh, lreset=False, lmixhold=False):
+        """Returns (kntchg, history resets)."""
+        info = np.zeros(2, np.int32)
+        check(load().qx_pwpmxchgg(self.h, int(bool(lreset)), float(aslh), float(bslh),
+                                  int(bool(lmixhold)), info.ctypes.data_as(C.c_void_p)),
+              "qx_pwpmxchgg")
+        return int(info[0]), int(info[1])
+
+    def scf_potential(self):
+        """V = vhar + vext + vexc into the context; returns {ehar, elcl, eexc,
+        evexc, ecor, evcor}."""
+        e = np.zeros(6)
+        check(load().qx_scf_potential(self.h, e), "qx_scf_potential")
+        return dict(zip(("ehar", "elcl", "eexc", "evexc", "ecor", "evcor"), e))
+
+    _SCF = {"rho": 0, "rhgr": 1, "rinr": 2, "vhar": 3, "vexc": 4, "eeexc": 5, "glocal": 6,
+            "cmk": 7}
+
+    def scf_get(self, what, n=None):
+        k = self._SCF[what]
+        size = 2 * (self.gs.nplw5 + 1) if k in (1, 2) else (n if k == 7 else self.gs.N)
+        out = np.zeros(size)
+        check(load().qx_scf_get(self.h, k, out), "qx_scf_get")
+        return out
+
+    def scf_put(self, what, a):
+        check(load().qx_scf_put(self.h, self._SCF[what], f64(a)), "qx_scf_put")
+
+    def mxchgg(self, rhgr, rinr, aslh, bslh, lreset=False, lmixhold=False):
+        """Host-array mxchgg: returns (rhgr mixed, rinr, glocal = chgg2r(rhgr))."""
+        gs = self.gs
+        ijk = gs.ijkgd()
+        rh, ri = f64(rhgr).copy(), f64(rinr).copy()
+        g = np.zeros(gs.N)
+        lr = C.c_int(int(bool(lreset)))
+        check(load().qx_mxchgg(self.h, C.byref(lr), rh, ri, gs.nplw5, float(aslh), float(bslh), g,
+                               None, gs.N, ijk, int(bool(lmixhold))), "qx_mxchgg")
+        return rh, ri, g
+
     def softchg(self, cgjr, occ, rvol):
         c = f64(cgjr)
         rho = np.zeros(self.gs.N)
@@ -327,3 +394,39 @@ def fft3(key, fft3x, fft3y, nd):
     ierr = C.c_int(0)
     check(load().qx_fft3(int(key), x, y, *[int(v) for v in nd], C.byref(ierr)), "qx_fft3")
     return x, y
+
+
+def xclda(rho, rhocore=None, rdelv=1.0):
+    """xclda on the device: (vexc, eeexc, [eexc, evexc, ecor, evcor])."""
+    r = f64(rho)
+    core = None if rhocore is None else f64(rhocore)
+    v, e, e4 = np.zeros(r.size), np.zeros(r.size), np.zeros(4)
+    check(load().qx_xclda(r, core.ctypes.data_as(C.c_void_p) if core is not None else None,
+                          r.size, float(rdelv), v, e, e4), "qx_xclda")
+    return v, e, e4
+
+
+def _sphere_call(fn, grid, rhgr, nplw5, ijk, nd, *extra):
+    return fn(grid, rhgr, int(nplw5), *extra, None, int(np.prod(nd)), i32(nd), i32(ijk))
+
+
+def harting(rhgr, recnrmex, ijk, nd):
+    """Hartree potential on the grid (FFT order); ijk = ijkgd(-nplw5:nplw5)."""
+    rh = f64(rhgr)
+    v = np.zeros(int(np.prod(nd)))
+    n5 = rh.size // 2 - 1
+    check(_sphere_call(load().qx_harting, v, rh, n5, ijk, nd, f64(recnrmex)), "qx_harting")
+    return v
+
+
+def chgr2g(rho, nplw5, ijk, nd):
+    rh = np.zeros(2 * (int(nplw5) + 1))
+    check(_sphere_call(load().qx_chgr2g, f64(rho), rh, nplw5, ijk, nd), "qx_chgr2g")
+    return rh
+
+
+def chgg2r(rhgr, ijk, nd):
+    rh = f64(rhgr)
+    rho = np.zeros(int(np.prod(nd)))
+    check(_sphere_call(load().qx_chgg2r, rho, rh, rh.size // 2 - 1, ijk, nd), "qx_chgg2r")
+    return rho

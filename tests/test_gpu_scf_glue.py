@@ -1,0 +1,172 @@
+"""GPU parity of the SCF glue (xclda, harting, chgr2g/chgg2r, Pulay / simple
+mixing, potential assembly) through the C-ABI against the oracle on the same
+inputs.  Tolerances: pointwise XC 1e-13 relative (device pow/log/sqrt differ
+from the host libm by an ulp or two); transforms 1e-12 of the largest value
+(FFT round-off); mixed densities 1e-10 (the Pulay coefficients solve a small
+normal equation whose inputs are sums taken in another order)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from qxmd_amd import pw
+from qxmd_amd.system import initial_wavefunctions, model_local_potential, occupations
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _recnrmex(sp):
+    r = np.zeros(sp.nplw5 + 1)
+    r[1:] = 4 * np.pi / np.asarray(sp.recnrm[1: sp.nplw5 + 1])
+    return r
+
+
+def _xc_inputs(n=20000, seed=5):
+    rng = np.random.default_rng(seed)
+    rho = np.concatenate([10 ** rng.uniform(-6, 1.5, n), [0.0, 1e-31, 2.0]])
+    core = np.where(rng.uniform(size=rho.size) < 0.3, 10 ** rng.uniform(-4, 0, rho.size), -1.0)
+    return rho, core
+
+
+@pytest.mark.parametrize("with_core", [False, True])
+def test_xclda_device(with_core):
+    rho, core = _xc_inputs()
+    c = core if with_core else None
+    v, e, e4 = pw.xclda(rho, c, 0.37)
+    vo, eo, e4o = O.xclda(rho, c, 0.37)
+    assert _rel(v, vo) < 1e-13 and _rel(e, eo) < 1e-13
+    np.testing.assert_allclose(e4, e4o, rtol=1e-12)
+    assert v[-3] == vo[-3] and e[-2] == eo[-2]
+
+
+@pytest.mark.parametrize("ncell", [1, 2])
+def test_sphere_transforms_device(ncell, si8, si64):
+    sp = si8 if ncell == 1 else si64
+    rng = np.random.default_rng(3 + ncell)
+    rho = rng.uniform(0.0, 0.05, sp.N)
+    rh = pw.chgr2g(rho, sp.nplw5, sp.ijk, sp.nd)
+    rh_o = O.chgr2g(rho, sp.nplw5, sp.ijk, sp.nd)
+    assert _rel(rh, rh_o) < 1e-12
+    back = pw.chgg2r(rh_o, sp.ijk, sp.nd)
+    assert _rel(back, O.chgg2r(rh_o, sp.ijk, sp.nd)) < 1e-12
+    rx = _recnrmex(sp)
+    vh = pw.harting(rh_o, rx, sp.ijk, sp.nd)
+    assert _rel(vh, O.harting(rh_o, rx, sp.ijk, sp.nd)) < 1e-12
+
+
+def _scf_case(sp, nb=8, itratn=4, imxchg=1, core=False):
+    gs = pw.GSphere(sp.cell.hcell)
+    ctx = pw.PlaneWave(gs, nb)
+    vext = model_local_potential(sp, sp.cell) - 0.4
+    rng = np.random.default_rng(17)
+    rc = rng.uniform(0.0, 0.01, sp.N) if core else None
+    rx = _recnrmex(sp)
+    ctx.scf_set_grid(rx, vext, rc)
+    ctx.mxchgg_alloc(imxchg=imxchg, itratn=itratn)
+    return gs, ctx, vext, rc, rx
+
+
+@pytest.mark.parametrize("lmixhold", [False, True])
+def test_scf_glue_pipeline_device(si8, lmixhold):
+    """Seven rounds of pwpchg -> Pulay mix -> chgg2r -> harting + xclda +
+    unify, device-resident, against the oracle run on the same bands; the
+    history restarts (itratn = 4) or slides (lmixhold) inside the run."""
+    nb = 8
+    gs, ctx, vext, rc, rx = _scf_case(si8, nb, core=True)
+    occ = occupations(nb, 16)
+    rvol = si8.cell.volume
+    rdelv = si8.cell.volume / si8.N
+    ncx = 2 * (si8.nplwc + 1)
+    q1 = O.factq1(20.0, si8.recnrm[1], si8.recnrm[si8.nplwc])
+    mix_o = O.PulayMixer(ncx, 4, O.dkgnrm(si8.recnrm, si8.nplwc)[:ncx], rvol, q1)
+    psi0, _ = O.schmidt(initial_wavefunctions(si8, nb))
+    rng = np.random.default_rng(5)
+    rin_o = np.zeros(2 * (si8.nplw5 + 1))
+    rho_in = O.softchg(psi0, si8.nplw, occ, si8.cell.volume, si8.ijk, si8.nd)
+    rin_o = O.chgr2g(rho_in, si8.nplw5, si8.ijk, si8.nd)
+    ctx.scf_put("rinr", rin_o)
+    for it in range(7):
+        psi, _ = O.schmidt(psi0 + 0.05 / (1 + it) * rng.standard_normal(psi0.shape))
+        ctx.put(0, psi)
+        ctx.pwpchg_slot(0, nb, occ)
+        rho_o = O.softchg(psi, si8.nplw, occ, si8.cell.volume, si8.ijk, si8.nd)
+        rh_o = O.chgr2g(rho_o, si8.nplw5, si8.ijk, si8.nd)
+        assert _rel(ctx.scf_get("rhgr"), rh_o) < 1e-11, it
+        kn, nres = ctx.pwpmxchgg(0.8, 0.6, lmixhold=lmixhold)
+        rh_o, rin_o = mix_o.mix(rh_o, rin_o, 0.8, 0.6, lmixhold=lmixhold)
+        assert kn == mix_o.kntchg and nres == mix_o.nreset, (it, kn, mix_o.kntchg)
+        assert _rel(ctx.scf_get("rhgr"), rh_o) < 1e-10, it
+        assert np.array_equal(ctx.scf_get("rinr"), ctx.scf_get("rhgr"))
+        np.testing.assert_allclose(ctx.scf_get("cmk", 4), mix_o.cmk, rtol=0, atol=1e-9)
+        rho_m = O.chgg2r(rh_o, si8.ijk, si8.nd)
+        assert _rel(ctx.scf_get("rho"), rho_m) < 1e-10, it
+        ene = ctx.scf_potential()
+        vh_o = O.harting(rh_o, rx, si8.ijk, si8.nd)
+        vx_o, ee_o, e4_o = O.xclda(rho_m, rc, rdelv)
+        assert _rel(ctx.scf_get("vhar"), vh_o) < 1e-10
+        assert _rel(ctx.scf_get("vexc"), vx_o) < 1e-10
+        assert _rel(ctx.scf_get("eeexc"), ee_o) < 1e-10
+        assert _rel(ctx.scf_get("glocal"), vh_o + vext + vx_o) < 1e-10
+        np.testing.assert_allclose(
+            [ene[k] for k in ("eexc", "evexc", "ecor", "evcor")], e4_o, rtol=1e-10)
+        np.testing.assert_allclose(ene["ehar"], 0.5 * np.sum(vh_o * rho_m) * rdelv, rtol=1e-10)
+        np.testing.assert_allclose(ene["elcl"], np.sum(vext * rho_m) * rdelv, rtol=1e-10)
+
+
+def test_scf_potential_feeds_band_apply(si8):
+    """The potential the glue assembles is the one the next H|psi> applies:
+    H psi after qx_scf_potential equals H psi after qx_set_glocal of the same V."""
+    nb = 6
+    gs, ctx, vext, rc, rx = _scf_case(si8, nb)
+    psi, _ = O.schmidt(initial_wavefunctions(si8, nb))
+    occ = occupations(nb, 12)
+    ctx.put(0, psi)
+    ctx.pwpchg_slot(0, nb, occ)
+    ctx.scf_put("rinr", ctx.scf_get("rhgr"))
+    ctx.pwpmxchgg(0.8, 0.6)
+    ctx.scf_potential()
+    for prec in (pw.FP64, pw.MIXED):
+        ctx.put(0, psi)
+        ctx.local(0, 1, nb, prec=prec)
+        a = ctx.get(1, nb)
+        ctx.set_glocal(ctx.scf_get("glocal"))
+        ctx.local(0, 1, nb, prec=prec)
+        b = ctx.get(1, nb)
+        assert np.array_equal(a, b), prec
+
+
+@pytest.mark.parametrize("imxchg", [0, 3])
+def test_simple_and_no_mixing_device(si8, imxchg):
+    gs, ctx, vext, rc, rx = _scf_case(si8, 4, imxchg=imxchg)
+    rng = np.random.default_rng(2)
+    n5x = 2 * (si8.nplw5 + 1)
+    a, b = rng.standard_normal(n5x) * 0.01, rng.standard_normal(n5x) * 0.01
+    a[1] = b[1] = 0.0
+    rh, ri, g = ctx.mxchgg(a, b, 0.3, 0.6)
+    rh_o, ri_o = O.mxchgg_simple(a, b, 0.3, none=(imxchg == 0))
+    assert _rel(rh, rh_o) < 1e-15 and np.array_equal(ri, rh)
+    assert _rel(g, O.chgg2r(rh_o, si8.ijk, si8.nd)) < 1e-12
+
+
+def test_mxchgg_dropin_pulay(si8):
+    """The host-array mxchgg keeps its history on the device between calls."""
+    gs, ctx, vext, rc, rx = _scf_case(si8, 4, itratn=5)
+    ncx = 2 * (si8.nplwc + 1)
+    q1 = O.factq1(20.0, si8.recnrm[1], si8.recnrm[si8.nplwc])
+    m = O.PulayMixer(ncx, 5, O.dkgnrm(si8.recnrm, si8.nplwc)[:ncx], si8.cell.volume, q1)
+    rng = np.random.default_rng(9)
+    n5x = 2 * (si8.nplw5 + 1)
+    rin = rng.standard_normal(n5x) * 0.01
+    rin[1] = 0.0
+    rin_d = rin.copy()
+    for it in range(6):
+        out = rin + rng.standard_normal(n5x) * 0.01 / (1 + it)
+        out[1] = 0.0
+        rh, ri, g = ctx.mxchgg(out, rin_d, 0.7, 0.5)
+        rh_o, rin = m.mix(out, rin, 0.7, 0.5)
+        assert _rel(rh, rh_o) < 1e-10, it
+        assert _rel(g, O.chgg2r(rh_o, si8.ijk, si8.nd)) < 1e-10
+        rin_d = ri

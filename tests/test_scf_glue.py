@@ -168,7 +168,7 @@ def test_pulay_mixer_properties(si8, lmixhold):
     dkg = O.dkgnrm(si8.recnrm, si8.nplwc)[:ncx]
     q1 = O.factq1(20.0, si8.recnrm[1], si8.recnrm[si8.nplwc])
     assert q1 > 0
-    rvol = 1 / si8.cell.volume
+    rvol = si8.cell.volume
     itr = 5
     m = O.PulayMixer(ncx, itr, dkg, rvol, q1)
     w = full_metric(si8, ncx, q1, rvol)
