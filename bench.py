@@ -78,6 +78,47 @@ def md_kb_setup(ctx, cell, gs, eq, lmx, vcl, reps=3):
     return out
 
 
+def scf_glue(ctx, cell, gs, v, nband, occ, reps=6, itratn=10):
+    """Time the SCF glue between two band updates, device-resident: pwpchg
+    (density of all bands + chgr2g), pwpmxchgg (Pulay over a filled, sliding
+    history + chgg2r and harting in one inverse transform) and scf_potential
+    (xclda + potential assembly + energies).  The occupations are perturbed at
+    random from call to call so that every mixing step sees a new output
+    density.  Runs after
+    everything else: it replaces the context's local potential."""
+    from qxmd_amd import pw
+    rx = np.zeros(gs.nplw5 + 1)
+    rx[1:] = 4 * np.pi / gs.recnrm[1: gs.nplw5 + 1]
+    ctx.scf_set_grid(rx, v)
+    ctx.mxchgg_alloc(imxchg=1, itratn=itratn)
+    rng = np.random.default_rng(7)
+
+    def occ_k():
+        # independent perturbations keep successive residuals linearly independent
+        return occ * (1.0 + 0.02 * rng.standard_normal(len(occ)))
+
+    def glue():
+        ctx.pwpchg_slot(0, nband, occ_k())
+        ctx.pwpmxchgg(0.8, 0.6, lmixhold=True)
+        ctx.scf_potential()
+
+    ctx.pwpchg_slot(0, nband, occ)
+    ctx.scf_put("rinr", ctx.scf_get("rhgr"))
+    for _ in range(itratn + 2):  # fill the history
+        glue()
+    ctx.pwpchg_slot(0, nband, occ, pw.MIXED)  # first FP32 density call sets up its buffers
+    out = {"glue_total": time_region(ctx, glue, reps),
+           "pwpchg": time_region(ctx, lambda: ctx.pwpchg_slot(0, nband, occ_k()), reps),
+           "scf_potential": time_region(ctx, ctx.scf_potential, reps),
+           "pwpchg_fp32": time_region(ctx, lambda: ctx.pwpchg_slot(0, nband, occ_k(), pw.MIXED),
+                                      reps)}
+    out["pwpmxchgg"] = out["glue_total"] - out["pwpchg"] - out["scf_potential"]
+    out = {k: round(x, 4) for k, x in out.items()}
+    ctx.pwpchg_slot(0, nband, occ_k())
+    out["pulay_history"], out["pulay_resets"] = ctx.pwpmxchgg(0.8, 0.6, lmixhold=True)
+    return out
+
+
 def fft_stage_bytes(gs, s):
     """Algorithmic FFT-stage bytes per band-apply: 8*N_h*s_c + N*s_r + 2*(nplw+1)*s_c
     (two real 3-D transforms as two HBM passes each, V(r) read, sphere I/O)."""
@@ -311,6 +352,10 @@ def main():
     extra["kb_forces_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
     print(f"# KB forces ({natom} atoms, {nband} bands): {extra['kb_forces_ms']:.2f} ms",
           file=sys.stderr, flush=True)
+
+    # SCF glue between band updates (density -> mixing -> Hartree + XC -> V)
+    extra["scf_glue_ms"] = scf_glue(ctx, cell, gs, v, nband, occ_f)
+    print(f"# SCF glue: {extra['scf_glue_ms']}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:

@@ -5,6 +5,7 @@
 // the pencil path; blocks take runs of lines adjacent in x so that strided
 // (y, z) lines are still read in coalesced rows.
 #include <algorithm>
+#include <cstdlib>
 #include "context.h"
 
 namespace qx {
@@ -134,12 +135,23 @@ void fg_release(FullGrid& f) {
   f = FullGrid{};
 }
 
+// lines per block: small enough that a 90^3 grid gives ~500 blocks per pass
+// on 256 CUs, large enough that the strided passes read rows of nl x 16 bytes
+static int fg_lines_per_block(int n) {
+  static const int cap = [] {
+    const char* e = std::getenv("QX_FG_NL");
+    const int v = e ? std::atoi(e) : 16;
+    return v >= 1 && v <= 64 ? v : 16;
+  }();
+  return std::max(1, std::min(cap, 4096 / n));
+}
+
 int fg_transform(FullGrid& f, hipStream_t st, int sign, double scale) {
   const int n1 = f.n1, n2 = f.n2, n3 = f.n3;
   const int dims[3] = {n1, n2, n3};
   for (int a = 0; a < 3; ++a) {
     const int n = dims[a];
-    int nl = std::max(1, std::min(64, 4096 / n));
+    const int nl = fg_lines_per_block(n);
     unsigned blocks;
     if (a == 0) blocks = (unsigned)(((long)n2 * n3 + nl - 1) / nl);
     else blocks = (unsigned)(((n1 + nl - 1) / nl) * (a == 1 ? n3 : n2));

@@ -166,15 +166,32 @@ __global__ void __launch_bounds__(kRedThreads)
   block_sum<kNe>(s, part + (size_t)blockIdx.x * 8);
 }
 
-// energies: ehar = sum vhar rho rdelv / 2 (Sources/pwlda.f90:1868-1873),
-// elcl = sum vext rho rdelv (1960-1965), XC sums x 2 rdelv (Sources/vxc.f90:1009-1012)
-__global__ void k_sg_energies(const double* __restrict__ part, int nblk, double rdelv,
-                              double* __restrict__ out) {
-  const int k = threadIdx.x;
-  if (k >= kNe) return;
-  double v = 0.0;
-  for (int b = 0; b < nblk; ++b) v += part[(size_t)b * 8 + k];
-  out[k] = k == 0 ? 0.5 * v * rdelv : (k == 1 ? v * rdelv : v * 2.0 * rdelv);
+// out[k] = scale(k) * sum_b part[b * stride + k] for k < K, one block of
+// kRedThreads: thread t takes b = t, t + kRedThreads, ... then a fixed wave
+// and block tree, so the order of the sum is the same on every run.
+// mode 0: scale 1 x s; mode 1: the energies (ehar = s0 rdelv / 2, elcl =
+// s1 rdelv (Sources/pwlda.f90:1868-1873, 1963-1968), XC sums x 2 rdelv
+// (Sources/vxc.f90:1009-1012)).
+__global__ void __launch_bounds__(kRedThreads)
+    k_red_final(const double* __restrict__ part, int nblk, int stride, int K, int mode, double s,
+                double* __restrict__ out) {
+  __shared__ double red[kRedThreads / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int k = 0; k < K; ++k) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kRedThreads) v += part[(size_t)b * stride + k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < kRedThreads / 64; ++w) t += red[w];
+      if (mode == 1) t = k == 0 ? 0.5 * t * s : (k == 1 ? t * s : t * 2.0 * s);
+      else t = t * s;
+      out[k] = t;
+    }
+    __syncthreads();
+  }
 }
 
 struct Cols {
@@ -209,15 +226,6 @@ __global__ void __launch_bounds__(kRedThreads)
       if (j < k) s[j] += dfmr[(size_t)cols.c[j] * ld + i] * x;
   }
   block_sum<kPlMax>(s, part + (size_t)blockIdx.x * kPlMax);
-}
-
-__global__ void k_pl_dots_final(const double* __restrict__ part, int nblk, int k, double rvol,
-                                double* __restrict__ out) {
-  const int j = threadIdx.x;
-  if (j >= k) return;
-  double v = 0.0;
-  for (int b = 0; b < nblk; ++b) v += part[(size_t)b * kPlMax + j];
-  out[j] = v * rvol;
 }
 
 __device__ inline double kerker(double g2, double aslh, double bslh, double wk) {
@@ -442,8 +450,8 @@ int mix_dev(Ctx* c, int lreset, double aslh, double bslh, int lmixhold) {
       hipLaunchKernelGGL(k_pl_dots, dim3(nblk), dim3(kRedThreads), 0, c->stream, S.d_dfmr, n,
                          cols, kntchg, pk, S.d_wq, n, S.d_red);
       double* d_out = S.d_red + (size_t)kRedBlocks * kPlMax;
-      hipLaunchKernelGGL(k_pl_dots_final, dim3(1), dim3(64), 0, c->stream, S.d_red, nblk,
-                         kntchg, S.rvol, d_out);
+      hipLaunchKernelGGL(k_red_final, dim3(1), dim3(kRedThreads), 0, c->stream, S.d_red, nblk,
+                         kPlMax, kntchg, 0, S.rvol, d_out);
       QX_CHECK(hipGetLastError());
       QX_CHECK(hipMemcpyAsync(a.data(), d_out, sizeof(double) * kntchg, hipMemcpyDeviceToHost,
                               c->stream));
@@ -531,8 +539,8 @@ int potential_dev(Ctx* c, double* ene) {
                      c->fg.d_g, S.d_vext, S.d_core, c->N, c->d_V, c->d_Vf, S.d_vhar, S.d_vexc,
                      S.d_eeexc, S.d_red);
   double* d_out = S.d_red + (size_t)kRedBlocks * kPlMax;
-  hipLaunchKernelGGL(k_sg_energies, dim3(1), dim3(64), 0, c->stream, S.d_red, kRedBlocks, S.rdelv,
-                     d_out);
+  hipLaunchKernelGGL(k_red_final, dim3(1), dim3(kRedThreads), 0, c->stream, S.d_red, kRedBlocks, 8,
+                     kNe, 1, S.rdelv, d_out);
   QX_CHECK(hipGetLastError());
   if ((rc = fast_permute_v(c))) return rc;
   double e[kNe];
@@ -743,8 +751,8 @@ int dropin_xclda(const double* rho, const double* rhocore, int n, double rdelv, 
                      nullptr, nullptr, rhocore ? g_db.d_b : nullptr, (long)n, nullptr, nullptr,
                      nullptr, g_db.d_c, g_db.d_d, g_db.d_red);
   double* d_out = g_db.d_red + (size_t)kRedBlocks * 8;
-  hipLaunchKernelGGL(k_sg_energies, dim3(1), dim3(64), 0, st, g_db.d_red, kRedBlocks, rdelv,
-                     d_out);
+  hipLaunchKernelGGL(k_red_final, dim3(1), dim3(kRedThreads), 0, st, g_db.d_red, kRedBlocks, 8,
+                     kNe, 1, rdelv, d_out);
   QX_CHECK(hipGetLastError());
   double e[kNe];
   QX_CHECK(hipMemcpyAsync(vexc, g_db.d_c, sizeof(double) * n, hipMemcpyDeviceToHost, st));

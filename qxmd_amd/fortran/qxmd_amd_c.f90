@@ -353,6 +353,113 @@ integer(c_int) function qx_softchg(ctx, cgjr, nb, occ, rvol, rho) bind(C, name="
   real(c_double), intent(out) :: rho(*)
 end function
 
+! ---- SCF glue
+integer(c_int) function qx_xclda(rho, rhocore, mshnod, rdelv, vexc, eeexc, e4)  &
+&     bind(C, name="qx_xclda")
+  import :: c_int, c_double
+  real(c_double), intent(in) :: rho(*), rhocore(*)
+  integer(c_int), value :: mshnod
+  real(c_double), value :: rdelv
+  real(c_double), intent(out) :: vexc(*), eeexc(*), e4(4)
+end function
+
+integer(c_int) function qx_harting(vhar, rhgr, nplw5, recnrmex, mfd2ft, ntotfd, nd1vks, ijkgd) &
+&     bind(C, name="qx_harting")
+  import :: c_int, c_double
+  real(c_double), intent(out) :: vhar(*)
+  real(c_double), intent(in) :: rhgr(*), recnrmex(*)
+  integer(c_int), value :: nplw5, ntotfd
+  integer(c_int), intent(in) :: mfd2ft(*), nd1vks(3), ijkgd(*)
+end function
+
+integer(c_int) function qx_chgr2g(rho, rhgr, nplw5, mfd2ft, ntotfd, nd1vks, ijkgd)  &
+&     bind(C, name="qx_chgr2g")
+  import :: c_int, c_double
+  real(c_double), intent(in) :: rho(*)
+  real(c_double), intent(out) :: rhgr(*)
+  integer(c_int), value :: nplw5, ntotfd
+  integer(c_int), intent(in) :: mfd2ft(*), nd1vks(3), ijkgd(*)
+end function
+
+integer(c_int) function qx_chgg2r(rho, rhgr, nplw5, mfd2ft, ntotfd, nd1vks, ijkgd)  &
+&     bind(C, name="qx_chgg2r")
+  import :: c_int, c_double
+  real(c_double), intent(out) :: rho(*)
+  real(c_double), intent(in) :: rhgr(*)
+  integer(c_int), value :: nplw5, ntotfd
+  integer(c_int), intent(in) :: mfd2ft(*), nd1vks(3), ijkgd(*)
+end function
+
+integer(c_int) function qx_mxchgg_alloc(ctx, nplwcex, imxchg, itratn, cmetric, wkerker, g2min,  &
+&     g2max, dkgnrm, rvol) bind(C, name="qx_mxchgg_alloc")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: nplwcex, imxchg, itratn
+  real(c_double), value :: cmetric, wkerker, g2min, g2max, rvol
+  real(c_double), intent(in) :: dkgnrm(*)
+end function
+
+integer(c_int) function qx_mxchgg(ctx, lreset, rhgr, rinr, nplw5, aslh, bslh, glocal, mfd2ft,  &
+&     ntotfd, ijkgd, lmixhold) bind(C, name="qx_mxchgg")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), intent(inout) :: lreset
+  real(c_double), intent(inout) :: rhgr(*), rinr(*)
+  integer(c_int), value :: nplw5, ntotfd, lmixhold
+  real(c_double), value :: aslh, bslh
+  real(c_double), intent(out) :: glocal(*)
+  integer(c_int), intent(in) :: mfd2ft(*), ijkgd(*)
+end function
+
+integer(c_int) function qx_scf_set_grid(ctx, nplw5, ijkgd, recnrmex, vext, rhocore, rdelv)  &
+&     bind(C, name="qx_scf_set_grid")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: nplw5
+  integer(c_int), intent(in) :: ijkgd(*)
+  real(c_double), intent(in) :: recnrmex(*)
+  type(c_ptr), value :: vext, rhocore
+  real(c_double), value :: rdelv
+end function
+
+integer(c_int) function qx_pwpchg_slot(ctx, slot, nb, occ, rvol, prec)  &
+&     bind(C, name="qx_pwpchg_slot")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: slot, nb, prec
+  real(c_double), intent(in) :: occ(*)
+  real(c_double), value :: rvol
+end function
+
+integer(c_int) function qx_pwpmxchgg(ctx, lreset, aslh, bslh, lmixhold, info)  &
+&     bind(C, name="qx_pwpmxchgg")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: lreset, lmixhold
+  real(c_double), value :: aslh, bslh
+  integer(c_int), intent(out) :: info(2)
+end function
+
+integer(c_int) function qx_scf_potential(ctx, ene) bind(C, name="qx_scf_potential")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_double), intent(out) :: ene(6)
+end function
+
+integer(c_int) function qx_scf_get(ctx, what, host) bind(C, name="qx_scf_get")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: what
+  real(c_double), intent(out) :: host(*)
+end function
+
+integer(c_int) function qx_scf_put(ctx, what, host) bind(C, name="qx_scf_put")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: what
+  real(c_double), intent(in) :: host(*)
+end function
+
 end interface
 
 end module qxmd_amd_c

@@ -27,6 +27,11 @@ integer, save :: qx_nd(3) = 0
 ! slmir(lmmax, natom, nband) of the last hckbpp_g / hckbpp_ib_g call (the
 ! module output of nl_variables in the reference)
 real(c_double), allocatable, target, save :: qx_slmir(:,:,:)
+! mixer settings of mxchgg_alloc / set_g2ming2max_in_chgdns, applied to the
+! device mixer at the next mxchgg (which brings dkgnrm and rvol)
+integer, save :: mx_imxchg = -1, mx_itratn = 0, mx_nplwcex = 0
+real*8, save :: mx_cmetric = 1.d0, mx_wkerker = 1.d0, mx_g2min = 0.d0, mx_g2max = 0.d0
+logical, save :: mx_pending = .false.
 
 contains
 
@@ -470,4 +475,135 @@ if (allocated(qx_slmir)) then
 end if
 if (.not. allocated(qx_slmir)) allocate(qx_slmir(max(qx_lmmax, 1), max(qx_natom, 1), 1))
 call qxmd_gpu_check(qx_hckbpp_g(qxctx, cgjr, rhcr, 1_c_int, c_loc(qx_slmir)), 'hckbpp_ib_g')
+end subroutine
+
+
+! xclda (Sources/vxc.f90:961-1017): LDA exchange-correlation on the device.
+subroutine xclda(rho, rhocore, mshnod, rdelv, vexc, eeexc, eexc, evexc, ecor, evcor)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: mshnod
+real*8 :: rho(mshnod), rhocore(mshnod), vexc(mshnod), eeexc(mshnod)
+real*8 :: rdelv, eexc, evexc, ecor, evcor
+real(c_double) :: e4(4)
+call qxmd_gpu_check(qx_xclda(rho, rhocore, int(mshnod, c_int), rdelv, vexc, eeexc, e4), 'xclda')
+eexc = e4(1)
+evexc = e4(2)
+ecor = e4(3)
+evcor = e4(4)
+end subroutine
+
+! harting (Sources/engrad.f90:1673-1743): vhar(ntotfd) in FD order.
+subroutine harting(nfile, myid, nodes, vhar, rhgr, nplw5ex, nplw5, recnrmex, mfd2ft, ntotfd,  &
+& nd1vks, kfft0, fft3x, fft3y, fftwork, ijkgd, ngenh)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: nfile(*), myid, nodes, nplw5ex, nplw5, ntotfd, kfft0, ngenh
+real*8 :: vhar(*), rhgr(*), recnrmex(0:*), fft3x(*), fft3y(*)
+integer :: mfd2ft(*), nd1vks(3), ijkgd(-ngenh:ngenh)
+complex*16 :: fftwork(*)
+integer(c_int) :: d(3), ij(-nplw5:nplw5), mf(ntotfd)
+d = nd1vks
+ij = ijkgd(-nplw5:nplw5)
+mf = mfd2ft(1:ntotfd)
+call qxmd_gpu_check(qx_harting(vhar, rhgr, int(nplw5, c_int), recnrmex, mf,  &
+&    int(ntotfd, c_int), d, ij), 'harting')
+end subroutine
+
+! chgr2g (Sources/engrad.f90:1253-1297)
+subroutine chgr2g(nfile, myid, nodes, rho, rhgr, nplw5ex, nplw5, mfd2ft, ntotfd, nd1vks, kfft0,  &
+& fft3x, fft3y, fftwork, ijkgd, ngenh)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: nfile(*), myid, nodes, nplw5ex, nplw5, ntotfd, kfft0, ngenh
+real*8 :: rho(*), rhgr(*), fft3x(*), fft3y(*)
+integer :: mfd2ft(*), nd1vks(3), ijkgd(-ngenh:ngenh)
+complex*16 :: fftwork(*)
+integer(c_int) :: d(3), ij(-nplw5:nplw5), mf(ntotfd)
+d = nd1vks
+ij = ijkgd(-nplw5:nplw5)
+mf = mfd2ft(1:ntotfd)
+call qxmd_gpu_check(qx_chgr2g(rho, rhgr, int(nplw5, c_int), mf, int(ntotfd, c_int), d, ij),  &
+&    'chgr2g')
+end subroutine
+
+! chgg2r (Sources/engrad.f90:1302-1353)
+subroutine chgg2r(nfile, myid, nodes, rho, rhgr, nplw5ex, nplw5, mfd2ft, ntotfd, nd1vks, kfft0,  &
+& fft3x, fft3y, fftwork, ijkgd, ngenh)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: nfile(*), myid, nodes, nplw5ex, nplw5, ntotfd, kfft0, ngenh
+real*8 :: rho(*), rhgr(*), fft3x(*), fft3y(*)
+integer :: mfd2ft(*), nd1vks(3), ijkgd(-ngenh:ngenh)
+complex*16 :: fftwork(*)
+integer(c_int) :: d(3), ij(-nplw5:nplw5), mf(ntotfd)
+d = nd1vks
+ij = ijkgd(-nplw5:nplw5)
+mf = mfd2ft(1:ntotfd)
+call qxmd_gpu_check(qx_chgg2r(rho, rhgr, int(nplw5, c_int), mf, int(ntotfd, c_int), d, ij),  &
+&    'chgg2r')
+end subroutine
+
+! set_g2ming2max_in_chgdns (Sources/chgdns.f90:529-539)
+subroutine set_g2ming2max_in_chgdns(g2min_, g2max_)
+use qxmd_amd_gpu
+implicit none
+real*8 :: g2min_, g2max_
+mx_g2min = g2min_
+mx_g2max = g2max_
+mx_pending = mx_imxchg >= 0
+end subroutine
+
+! mxchgg_alloc (Sources/chgdns.f90:559-690): imxchg 0 (none), 1 (Pulay) or 3
+! (simple); spin and noncollinear mixing are not on the device.
+subroutine mxchgg_alloc(nfile, myid, nodes, alloc_mem, lspin, nplwcex_, imxchg_, itratn_,  &
+& imxspin_, nmxspin_, cmetric, spinmetric, wkerker, pwscale, lnoncollinear)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: nfile(*), myid, nodes, nplwcex_, imxchg_, itratn_, imxspin_, nmxspin_
+real*8 :: alloc_mem, cmetric, spinmetric, wkerker, pwscale
+logical :: lspin, lnoncollinear
+if (lspin .or. lnoncollinear) call qxmd_gpu_check(-1_c_int,  &
+&    'mxchgg_alloc: spin-polarised mixing is not on the device')
+mx_imxchg = imxchg_
+mx_itratn = itratn_
+mx_nplwcex = nplwcex_
+mx_cmetric = cmetric
+mx_wkerker = wkerker
+mx_pending = .true.
+end subroutine
+
+! mxchgg (Sources/chgdns.f90:796-902): mixing of rhgr with rinr on the
+! device (history kept there), glocal = chgg2r(mixed rhgr) in FD order.
+subroutine mxchgg(nfile, myid, nodes, lreset, rhgr, rinr, nplw5ex, nplw5, dkgnrm, rvol, aslh,  &
+& bslh, glocal, apk, eigr, mfd2ft, ntotfd, nd1vks, kfft0, fft3x, fft3y, fftwork, ijkgd, ngenh,  &
+& rdelv, nel, lmixhold)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: nfile(*), myid, nodes, nplw5ex, nplw5, ntotfd, kfft0, ngenh, nel
+logical :: lreset, lmixhold
+real*8 :: rhgr(nplw5ex), rinr(nplw5ex), dkgnrm(*), rvol, aslh, bslh, rdelv
+real*8 :: glocal(ntotfd), apk(*), eigr(*), fft3x(*), fft3y(*)
+integer :: mfd2ft(ntotfd), nd1vks(3), ijkgd(-ngenh:ngenh)
+complex*16 :: fftwork(*)
+integer(c_int) :: ij(-nplw5:nplw5), mf(ntotfd), lr
+call qxmd_gpu_need_ctx('mxchgg')
+if (mx_pending) then
+  call qxmd_gpu_check(qx_mxchgg_alloc(qxctx, int(mx_nplwcex, c_int), int(mx_imxchg, c_int),  &
+&      int(mx_itratn, c_int), mx_cmetric, mx_wkerker, mx_g2min, mx_g2max, dkgnrm, rvol),  &
+&      'mxchgg_alloc')
+  mx_pending = .false.
+end if
+ij = ijkgd(-nplw5:nplw5)
+mf = mfd2ft
+lr = merge(1_c_int, 0_c_int, lreset)
+call qxmd_gpu_check(qx_mxchgg(qxctx, lr, rhgr, rinr, int(nplw5, c_int), aslh, bslh, glocal, mf,  &
+&    int(ntotfd, c_int), ij, merge(1_c_int, 0_c_int, lmixhold)), 'mxchgg')
+lreset = .false.
 end subroutine

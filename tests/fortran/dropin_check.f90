@@ -174,6 +174,10 @@ call check_forces()
 ! ---- stscfn_pw: 3 atoms, 40 G indices in -3..3, against cos/sin(2 pi G . x)
 call check_stscfn()
 
+! ---- SCF glue: chgg2r / harting / chgr2g on a three-vector density, xclda,
+! ---- simple and Pulay mixing through mxchgg_alloc / mxchgg
+call check_scf_glue()
+
 call qxmd_gpu_finalize()
 write(*, '(a, es10.2)') 'DROPIN OK worst', worst
 
@@ -264,6 +268,125 @@ subroutine check_stscfn()
     e2 = max(e2, abs(c2(-2, ja) - cos(4.d0 * pi * ra(2, ja))), abs(s3(3, ja) - sin(6.d0 * pi * ra(3, ja))))
   end do
   call report('stscfn_pw structure factors vs cos/sin(G.x)', e2, 1.d-13)
+end subroutine
+
+! A density of G = 0 and three G vectors (and mirrors): chgg2r, harting and
+! chgr2g against the Fourier sums written out; xclda's potential against the
+! derivative of its energy density; simple mixing and the first two Pulay
+! steps (Kerker, then the 2 x 2 normal equation) against their closed forms.
+subroutine check_scf_glue()
+  integer, parameter :: n5 = 3, n5x = 2 * (n5 + 1)
+  integer :: gv(3, 0:n5), ijg(-n5:n5), ig, ix, iy, iz, q, nf(1)
+  real*8 :: rx(0:n5), g2(0:n5), rg(0:n5x - 1), rg2(0:n5x - 1), r0(ntot), vh(ntot), gl(ntot)
+  real*8 :: rref, vref, ph, e1, e2, tpl, vx(ntot), ee(ntot), vxp(ntot), eep(ntot), vxm(ntot)
+  real*8 :: eem(ntot), eexc, evexc, ecor, evcor, ex2, ev2, ec2, evc2, hh(ntot), dkg(n5x)
+  real*8 :: rin(n5x), out(n5x), in1(n5x), f1(n5x), f2(n5x), wq(n5x), fk(n5x), a11, a12, a22
+  real*8 :: c1, c2, q1, mixed(n5x), fx_(ntot), fy_(ntot), amem
+  logical :: lres
+  complex*16 :: fw(1)
+  tpl = 2.d0 * pi / 10.26d0
+  gv(:, 0) = (/0, 0, 0/)
+  gv(:, 1) = (/1, 0, 0/)
+  gv(:, 2) = (/0, 2, 1/)
+  gv(:, 3) = (/1, -1, 2/)
+  do ig = 0, n5
+    ijg(ig) = 1 + modulo(gv(1, ig), n1) + n1 * (modulo(gv(2, ig), n2) + n2 * modulo(gv(3, ig), n3))
+    ijg(-ig) = 1 + modulo(-gv(1, ig), n1) + n1 * (modulo(-gv(2, ig), n2) +  &
+&              n2 * modulo(-gv(3, ig), n3))
+    g2(ig) = tpl**2 * sum(gv(:, ig)**2)
+    rx(ig) = 0.d0
+    if (ig > 0) rx(ig) = 4.d0 * pi / g2(ig)
+  end do
+  rg = (/0.03d0, 0.d0, 0.004d0, -0.002d0, -0.003d0, 0.001d0, 0.0015d0, 0.0025d0/)
+  nf = 6
+  call chgg2r(nf, 0, 1, r0, rg, n5x, n5, mfd2ft, ntot, nd, ntot, fx_, fy_, fw, ijg, n5)
+  call harting(nf, 0, 1, vh, rg, n5x, n5, rx, mfd2ft, ntot, nd, ntot, fx_, fy_, fw, ijg, n5)
+  e1 = 0.d0
+  e2 = 0.d0
+  do iz = 0, n3 - 1
+    do iy = 0, n2 - 1
+      do ix = 0, n1 - 1
+        rref = rg(0)
+        vref = 0.d0
+        do ig = 1, n5
+          ph = 2.d0 * pi * (gv(1, ig) * dble(ix) / n1 + gv(2, ig) * dble(iy) / n2 +  &
+&              gv(3, ig) * dble(iz) / n3)
+          rref = rref + 2.d0 * (rg(2 * ig) * cos(ph) - rg(2 * ig + 1) * sin(ph))
+          vref = vref + 2.d0 * rx(ig) * 2.d0 * (rg(2 * ig) * cos(ph) - rg(2 * ig + 1) * sin(ph))
+        end do
+        q = 1 + ix + n1 * (iy + n2 * iz)
+        e1 = max(e1, abs(r0(q) - rref))
+        e2 = max(e2, abs(vh(q) - vref))
+      end do
+    end do
+  end do
+  call report('chgg2r vs Fourier sum', e1 / 0.03d0, 1.d-12)
+  call report('harting vs 8 pi/G^2 Fourier sum', e2 / maxval(abs(vh)), 1.d-12)
+  call chgr2g(nf, 0, 1, r0, rg2, n5x, n5, mfd2ft, ntot, nd, ntot, fx_, fy_, fw, ijg, n5)
+  call report('chgr2g(chgg2r(rhgr)) = rhgr', maxval(abs(rg2 - rg)) / 0.03d0, 1.d-13)
+  ! xclda: vexc = d(rho eeexc)/d rho by central differences
+  call xclda(r0, -r0, ntot, rvol / ntot, vx, ee, eexc, evexc, ecor, evcor)
+  hh = 1.d-6 * r0
+  call xclda(r0 + hh, -r0, ntot, rvol / ntot, vxp, eep, ex2, ev2, ec2, evc2)
+  call xclda(r0 - hh, -r0, ntot, rvol / ntot, vxm, eem, ex2, ev2, ec2, evc2)
+  e1 = maxval(abs(vx - ((r0 + hh) * eep - (r0 - hh) * eem) / (2.d0 * hh))) / maxval(abs(vx))
+  call report('xclda vexc = d(rho eeexc)/d rho', e1, 1.d-7)
+  e2 = abs(sum(ee * r0) * rvol / ntot - (eexc + ecor)) / abs(eexc + ecor)
+  call report('xclda energy sums', e2, 1.d-12)
+  ! simple mixing
+  do ig = 0, n5
+    dkg(2 * ig + 1) = g2(ig)
+    dkg(2 * ig + 2) = g2(ig)
+  end do
+  rin = rg
+  out = rg * 1.3d0
+  out(2) = 0.d0
+  call mxchgg_alloc(nf, 0, 1, amem, .false., n5x, 3, 4, 0, 0, 2.d0, 1.d0, 1.d0, 1.d0, .false.)
+  call set_g2ming2max_in_chgdns(g2(1), g2(n5))
+  lres = .true.
+  mixed = out
+  call mxchgg(nf, 0, 1, lres, mixed, rin, n5x, n5, dkg, rvol, 0.3d0, 0.5d0, gl, fx_, fx_,  &
+&     mfd2ft, ntot, nd, ntot, fx_, fy_, fw, ijg, n5, rvol / ntot, 8, .false.)
+  e1 = maxval(abs(mixed - (0.7d0 * rg + 0.3d0 * out)))
+  e1 = max(e1, maxval(abs(rin - mixed)))
+  call chgg2r(nf, 0, 1, r0, mixed, n5x, n5, mfd2ft, ntot, nd, ntot, fx_, fy_, fw, ijg, n5)
+  e1 = max(e1, maxval(abs(gl - r0)))
+  call report('mxchgg simple mixing', e1 / 0.03d0, 1.d-13)
+  ! Pulay (cmetric = 2): Kerker first, then the two-residual combination
+  call mxchgg_alloc(nf, 0, 1, amem, .false., n5x, 1, 4, 0, 0, 2.d0, 1.d0, 1.d0, 1.d0, .false.)
+  call set_g2ming2max_in_chgdns(g2(1), g2(n5))
+  q1 = (2.d0 - 1.d0) * g2(n5) * g2(1) / (g2(n5) - 2.d0 * g2(1))
+  do q = 1, n5x
+    fk(q) = 0.8d0 * dkg(q) / (dkg(q) + 0.6d0)
+    wq(q) = 0.d0
+    if (q >= 3) wq(q) = 2.d0 * (dkg(q) + q1) / dkg(q)
+  end do
+  wq(1) = (dkg(3) + q1) / dkg(3)
+  in1 = rg
+  out = rg
+  out(3:) = rg(2:) * 1.2d0 + 0.0005d0
+  f1 = out - in1
+  rin = in1
+  mixed = out
+  lres = .false.
+  call mxchgg(nf, 0, 1, lres, mixed, rin, n5x, n5, dkg, rvol, 0.8d0, 0.6d0, gl, fx_, fx_,  &
+&     mfd2ft, ntot, nd, ntot, fx_, fy_, fw, ijg, n5, rvol / ntot, 8, .false.)
+  e1 = maxval(abs(mixed - (in1 + fk * f1)))
+  out = mixed
+  out(3:) = out(3:) * 0.9d0 - 0.0003d0
+  f2 = out - mixed
+  a11 = rvol * sum(wq * f1 * f1)
+  a12 = rvol * sum(wq * f1 * f2)
+  a22 = rvol * sum(wq * f2 * f2)
+  c1 = (a22 - a12) / (a11 + a22 - 2.d0 * a12)
+  c2 = (a11 - a12) / (a11 + a22 - 2.d0 * a12)
+  in1 = in1 + 0.d0
+  rg2 = mixed
+  call mxchgg(nf, 0, 1, lres, out, rin, n5x, n5, dkg, rvol, 0.8d0, 0.6d0, gl, fx_, fx_,  &
+&     mfd2ft, ntot, nd, ntot, fx_, fy_, fw, ijg, n5, rvol / ntot, 8, .false.)
+  e2 = maxval(abs(out - (c1 * (in1 + fk * f1) + c2 * (rg2 + fk * f2))))
+  call report('mxchgg Pulay: Kerker step, then 2 x 2 normal equation', max(e1, e2) / 0.03d0,  &
+&     1.d-11)
 end subroutine
 
 subroutine expect(cond, what)
