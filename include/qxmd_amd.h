@@ -237,6 +237,9 @@ int qx_scf_set_grid(qx_ctx *ctx, int nplw5, const int *ijkgd, const double *recn
 /* softchg + outchg (Sources/chgdns.f90:29-383): density occ/rvol psi^2 of nb
  * bands of a slot (rvol = cell volume), then its rhgr, kept on the device. */
 int qx_pwpchg_slot(qx_ctx *ctx, int slot, int nb, const double *occ, double rvol, int prec);
+/* outchg (Sources/chgdns.f90:248-383): rhgr of the density the context holds
+ * (left by qx_scf_band_step, qx_density_slot or qx_pwpchg_slot). */
+int qx_outchg(qx_ctx *ctx);
 /* pwpmxchgg2 (Sources/chgdns.f90:724-790): mix the device rhgr with rinr,
  * then rho = chgg2r(rhgr); info = {kntchg, history resets} if non-null. */
 int qx_pwpmxchgg(qx_ctx *ctx, int lreset, double aslh, double bslh, int lmixhold, int *info);

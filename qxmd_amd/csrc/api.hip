@@ -827,6 +827,11 @@ int qx_pwpchg_slot(qx_ctx* h, int slot, int nb, const double* occ, double rvol, 
   return rc ? fail(rc, "qx_pwpchg_slot: call qx_scf_set_grid first (%d)", rc) : 0;
 }
 
+int qx_outchg(qx_ctx* h) {
+  int rc = scf_outchg(&h->c);
+  return rc ? fail(rc, "qx_outchg: call qx_scf_set_grid first (%d)", rc) : 0;
+}
+
 int qx_pwpmxchgg(qx_ctx* h, int lreset, double aslh, double bslh, int lmixhold, int* info) {
   int rc = scf_mix(&h->c, lreset, aslh, bslh, lmixhold, info);
   return rc ? fail(rc, "qx_pwpmxchgg: call qx_scf_set_grid and qx_mxchgg_alloc first (%d)", rc)

@@ -189,6 +189,7 @@ int scf_set_grid(Ctx* c, int nplw5, const int* ijkgd, const double* recnrmex, co
 int scf_mixer_alloc(Ctx* c, int nplwcex, int imxchg, int itratn, double cmetric, double wkerker,
                     double g2min, double g2max, const double* dkgnrm, double rvol);
 int scf_pwpchg(Ctx* c, const double* psi, int nb, const double* occ, double rvol, int prec);
+int scf_outchg(Ctx* c);
 int scf_mix(Ctx* c, int lreset, double aslh, double bslh, int lmixhold, int* info);
 int scf_potential(Ctx* c, double* ene);
 int scf_io(Ctx* c, int what, double* host, bool put);

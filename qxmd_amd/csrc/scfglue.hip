@@ -642,6 +642,14 @@ int scf_pwpchg(Ctx* c, const double* psi, int nb, const double* occ, double rvol
   return chgr2g_dev(c, c->d_rho, S.d_rhgr);
 }
 
+// outchg (Sources/chgdns.f90:248-383): rhgr of the density already in the
+// context (the one the last band step or density call left there)
+int scf_outchg(Ctx* c) {
+  auto& S = c->scf;
+  if (!S.d_ijk) return -2;
+  return chgr2g_dev(c, c->d_rho, S.d_rhgr);
+}
+
 // pwpmxchgg2 (Sources/chgdns.f90:724-790): mix, then rho = chgg2r(rhgr)
 int scf_mix(Ctx* c, int lreset, double aslh, double bslh, int lmixhold, int* info) {
   auto& S = c->scf;

@@ -431,6 +431,11 @@ integer(c_int) function qx_pwpchg_slot(ctx, slot, nb, occ, rvol, prec)  &
   real(c_double), value :: rvol
 end function
 
+integer(c_int) function qx_outchg(ctx) bind(C, name="qx_outchg")
+  import :: c_int, c_ptr
+  type(c_ptr), value :: ctx
+end function
+
 integer(c_int) function qx_pwpmxchgg(ctx, lreset, aslh, bslh, lmixhold, info)  &
 &     bind(C, name="qx_pwpmxchgg")
   import :: c_int, c_double, c_ptr

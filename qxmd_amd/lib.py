@@ -90,6 +90,7 @@ SIGNATURES = {
     "qx_mxchgg": [_vp, _pi, _dp, _dp, _i, _d, _d, _dp, _vp, _i, _ip, _i],
     "qx_scf_set_grid": [_vp, _i, _ip, _dp, _vp, _vp, _d],
     "qx_pwpchg_slot": [_vp, _i, _i, _dp, _d, _i],
+    "qx_outchg": [_vp],
     "qx_pwpmxchgg": [_vp, _i, _d, _d, _i, _vp],
     "qx_scf_potential": [_vp, _dp],
     "qx_scf_get": [_vp, _i, _dp],

@@ -336,6 +336,10 @@ class PlaneWave:
         check(load().qx_pwpchg_slot(self.h, slot, nb, f64(occ), self.gs.volume, prec),
               "qx_pwpchg_slot")
 
+    def outchg(self):
+        """rhgr of the density the context holds (after a band step)."""
+        check(load().qx_outchg(self.h), "qx_outchg")
+
     def pwpmxchgg(self, aslh, bslh, lreset=False, lmixhold=False):
         """Returns (kntchg, history resets)."""
         info = np.zeros(2, np.int32)

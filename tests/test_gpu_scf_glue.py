@@ -170,3 +170,19 @@ def test_mxchgg_dropin_pulay(si8):
         assert _rel(rh, rh_o) < 1e-10, it
         assert _rel(g, O.chgg2r(rh_o, si8.ijk, si8.nd)) < 1e-10
         rin_d = ri
+
+
+def test_outchg_uses_the_context_density(si8):
+    """qx_outchg transforms the density a band step or density call left in
+    the context: the same rhgr as qx_pwpchg_slot on the same bands."""
+    nb = 6
+    gs, ctx, vext, rc, rx = _scf_case(si8, nb)
+    psi, _ = O.schmidt(initial_wavefunctions(si8, nb))
+    occ = occupations(nb, 12)
+    ctx.put(0, psi)
+    ctx.pwpchg_slot(0, nb, occ)
+    a = ctx.scf_get("rhgr")
+    ctx.scf_put("rhgr", np.zeros_like(a))
+    ctx.density(0, nb, occ, si8.cell.volume)
+    ctx.outchg()
+    assert np.array_equal(ctx.scf_get("rhgr"), a)
