@@ -119,6 +119,30 @@ def scf_glue(ctx, cell, gs, v, nband, occ, reps=6, itratn=10):
     return out
 
 
+def flocal(ctx, cell, gs, reps=5):
+    """Time flocal_g on the device: axis tables from the device stscfn_pw over
+    the density sphere, a model local form factor (Gaussian-screened 1/G^2),
+    the density the SCF glue left in the context."""
+    n5 = gs.nplw5
+    g2 = gs.recnrm[: n5 + 1].copy()
+    g2[0] = 1.0
+    svco = (-8 * np.pi * 4.0 / gs.volume * np.exp(-0.25 * g2) / g2)[None, :]
+    svco[0, 0] = 0.0
+    ctx.set_lopp(svco)
+    frac = np.linalg.solve(cell.hcell, cell.tau.T).T
+    g = [np.asarray(a[: n5 + 1], np.int32) for a in (gs.nga, gs.ngb, gs.ngc)]
+    kmax = [int(np.abs(a).max()) for a in g]
+    ctx.stscfn_pw(frac, cell.hcell, gs.nd, gs.nd, *g, gs.nplw, kmax, copy_out=False)
+    natom = len(frac)
+    args = (1, [1], [natom], np.arange(1, natom + 1), [0], natom)
+    ctx.flocal_g_dev(*args)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.flocal_g_dev(*args)
+    return round((time.perf_counter() - t0) * 1e3 / reps, 4)
+
+
 def fft_stage_bytes(gs, s):
     """Algorithmic FFT-stage bytes per band-apply: 8*N_h*s_c + N*s_r + 2*(nplw+1)*s_c
     (two real 3-D transforms as two HBM passes each, V(r) read, sphere I/O)."""
@@ -356,6 +380,10 @@ def main():
     # SCF glue between band updates (density -> mixing -> Hartree + XC -> V)
     extra["scf_glue_ms"] = scf_glue(ctx, cell, gs, v, nband, occ_f)
     print(f"# SCF glue: {extra['scf_glue_ms']}", file=sys.stderr, flush=True)
+    # local-pseudopotential forces on the mixed density the glue left
+    extra["flocal_ms"] = flocal(ctx, cell, gs)
+    print(f"# flocal_g ({natom} atoms, {gs.nplw5} G): {extra['flocal_ms']:.3f} ms",
+          file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:

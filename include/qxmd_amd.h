@@ -196,6 +196,30 @@ int qx_icholesky(qx_ctx *ctx, double *s, int n, int *ok);
 int qx_softchg(qx_ctx *ctx, const double *cgjr, int nb, const double *occ, double rvol,
                double *rho);
 
+/* ---------------- local-pseudopotential forces ----------------------------- */
+/* svco(ig, it) of lopp_variables (Sources/ppkb.f90:66-79): local form
+ * factors over ig = 0..nplw5 of the density sphere, (nplw5+1) x ntype
+ * column-major, and its Cartesian G gx/gy/gz(0:nplw5). */
+int qx_set_lopp(qx_ctx *ctx, int ntype, int nplw5, const double *svco, const double *gx,
+                const double *gy, const double *gz);
+/* flocal_g without stress (Sources/nlkbpp.f90:1037-1125): for the ions
+ * k = nhk1(it)..nhk2(it) of the types with llking(it) = 0, ia = iatmpt(k)
+ * (1-based), floc(3, ia) = 2 (floc(3, ia) + sum over G != 0 of svco (Re rhgr
+ * sin + Im rhgr cos)(G.tau) G).  Host form: the caller's axis tables
+ * DCQ1..3 and DSQ1..3 (-kmax:kmax, natom) of stscfn_pw, its G list
+ * nga/ngb/ngc(0:nplw5)
+ * and rhgr; gx/gy/gz and svco from qx_set_lopp. */
+int qx_flocal_g(qx_ctx *ctx, double *floc, int ntype, const int *nhk1, const int *nhk2,
+                const int *iatmpt, int natom, const int *llking, int nplw5, const int *nga,
+                const int *ngb, const int *ngc, const double *rhgr, int kmax1, int kmax2,
+                int kmax3, const double *dcq1, const double *dcq2, const double *dcq3,
+                const double *dsq1, const double *dsq2, const double *dsq3);
+/* device-resident form: the density sphere of qx_scf_set_grid (the mixed
+ * rhgr of the last qx_pwpmxchgg, or as put), the axis tables of the last
+ * qx_stscfn_pw (called with the nplw5 G list), svco of qx_set_lopp. */
+int qx_flocal_g_dev(qx_ctx *ctx, int ntype, const int *nhk1, const int *nhk2,
+                    const int *iatmpt, const int *llking, int natom, double *floc);
+
 /* ---------------- SCF glue between band updates ----------------------------
  * Sphere arrays: rhgr(0:2*nplw5+1) real-packed (Re, Im per G, G index
  * 0..nplw5); ijkgd points at ijkgd(-nplw5) of the array ijkgd(-nplw5:nplw5)

@@ -106,6 +106,8 @@ def _declare(L):
                                    C.POINTER(_i), _dp, _dp, _dp, _dp]
     L.qxo_mxchgg_pulay.restype = _i
     L.qxo_mxchgg_simple.argtypes = [_dp, _dp, _i, _d, _i]
+    L.qxo_flocal_g.argtypes = [_i, _ip, _ip, _ip, _ip, _i, _i, _ip, _ip, _ip, _dp, _dp, _dp, _dp,
+                               _dp, _ip, C.POINTER(_vp), C.POINTER(_vp), _dp]
     L.qxo_kbnlpp_frc2.argtypes = [_dp, _i, _i, _i, _ip, _dp, _dp, _i, _dp, _dp, _dp, _dp, _dp,
                                   _dp]
     L.qxo_stscfn_pw.argtypes = [_dp, _dp, _ip, _ip, _i, _i, _i, _i, _i, _i, _i, _ip, _ip, _ip,
@@ -356,6 +358,36 @@ def mxchgg_simple(rhgr, rinr, aslh, none=False):
     rh, ri = _f(rhgr).copy(), _f(rinr).copy()
     lib().qxo_mxchgg_simple(rh, ri, rh.size, float(aslh), int(bool(none)))
     return rh, ri
+
+
+def axis_tables(frac, kmax):
+    """DCQ/DSQ of stscfn_pw for periodic cells: cos/sin(2 pi k s_a) for
+    k = -kmax_a..kmax_a, one (natom, 2 kmax_a + 1) pair per axis a."""
+    fr = np.asarray(frac, np.float64)
+    out = []
+    for a in range(3):
+        k = np.arange(-kmax[a], kmax[a] + 1)
+        ph = 2 * np.pi * np.outer(fr[:, a], k)
+        out.append((np.ascontiguousarray(np.cos(ph)), np.ascontiguousarray(np.sin(ph))))
+    return out
+
+
+def flocal_g(ntype, nhk1, nhk2, iatmpt, llking, natom, nga, ngb, ngc, gx, gy, gz, rhgr, svco,
+             kmax, tables, floc=None):
+    """flocal_g without stress (Sources/nlkbpp.f90:1037-1125).  svco (ntype,
+    nplw5 + 1) over ig = 0..nplw5; tables from axis_tables; returns floc (natom, 3)."""
+    rh = _f(rhgr)
+    n5 = rh.size // 2 - 1
+    f = np.zeros((natom, 3)) if floc is None else _f(floc).copy()
+    sv = _f(svco)
+    dc = (_vp * 3)(*[t[0].ctypes.data for t in tables])
+    ds = (_vp * 3)(*[t[1].ctypes.data for t in tables])
+    lib().qxo_flocal_g(int(ntype), _f(nhk1, np.int32), _f(nhk2, np.int32), _f(iatmpt, np.int32),
+                       _f(llking, np.int32), int(natom), n5, _f(np.asarray(nga)[: n5 + 1], np.int32),
+                       _f(np.asarray(ngb)[: n5 + 1], np.int32), _f(np.asarray(ngc)[: n5 + 1], np.int32),
+                       _f(np.asarray(gx)[: n5 + 1]), _f(np.asarray(gy)[: n5 + 1]),
+                       _f(np.asarray(gz)[: n5 + 1]), rh, sv, _f(kmax, np.int32), dc, ds, f)
+    return f
 
 
 def kbnlpp_frc2(eqlmr, lmx_atom, vclalp_atom, cgjr, occ, slmir, gx, gy, gz):

@@ -1125,6 +1125,56 @@ void qxo_mxchgg_simple(double *rhgr, double *rinr, int nplw5ex, double aslh, int
   for (ig = 0; ig < nplw5ex; ++ig) rinr[ig] = rhgr[ig];
 }
 
+/* flocal_g without stress (Sources/nlkbpp.f90:1037-1125), one plane-wave
+ * rank: for the ions k = nhk1(it)..nhk2(it) of each type with llking(it)
+ * false, atom ia = iatmpt(k), floc(:, ia) = 2 (floc(:, ia) + sum_{ig=1..nplw5}
+ * svco(ig+1, it) (Re rhgr_G sin(G.tau) + Im rhgr_G cos(G.tau)) G), the phase
+ * from the axis tables DCQ/DSQ (k = -kmax..kmax per atom) as 1091-1098 does.
+ * nhk1/nhk2/iatmpt 1-based; svco (nplw5+1) x ntype column-major; dcq[a] and
+ * dsq[a] (2 kmax_a + 1) x natom; floc 3 x natom. */
+void qxo_flocal_g(int ntype, const int *nhk1, const int *nhk2, const int *iatmpt,
+                  const int *llking, int natom, int nplw5, const int *nga, const int *ngb,
+                  const int *ngc, const double *gx, const double *gy, const double *gz,
+                  const double *rhgr, const double *svco, const int *kmax,
+                  const double *const *dcq, const double *const *dsq, double *floc)
+{
+  int it, k, ig;
+  for (it = 0; it < ntype; ++it) {
+    if (llking[it]) continue;
+    for (k = nhk1[it]; k <= nhk2[it]; ++k) {
+      const int ia = iatmpt[k - 1] - 1;
+      const double *c1 = dcq[0] + (size_t)ia * (2 * kmax[0] + 1) + kmax[0];
+      const double *s1 = dsq[0] + (size_t)ia * (2 * kmax[0] + 1) + kmax[0];
+      const double *c2 = dcq[1] + (size_t)ia * (2 * kmax[1] + 1) + kmax[1];
+      const double *s2 = dsq[1] + (size_t)ia * (2 * kmax[1] + 1) + kmax[1];
+      const double *c3 = dcq[2] + (size_t)ia * (2 * kmax[2] + 1) + kmax[2];
+      const double *s3 = dsq[2] + (size_t)ia * (2 * kmax[2] + 1) + kmax[2];
+      double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+      for (ig = 1; ig <= nplw5; ++ig) {
+        const int k1 = nga[ig], k2 = ngb[ig], k3 = ngc[ig];
+        const double dcc12 = c1[k1] * c2[k2] - s1[k1] * s2[k2];
+        const double dsc12 = s1[k1] * c2[k2] + c1[k1] * s2[k2];
+        const double zcosk = dcc12 * c3[k3] - dsc12 * s3[k3];
+        const double zsink = dsc12 * c3[k3] + dcc12 * s3[k3];
+        const double ftmp = svco[ig + (size_t)(nplw5 + 1) * it] *
+                            (rhgr[2 * ig] * zsink + rhgr[2 * ig + 1] * zcosk);
+        b0 = b0 + ftmp * gx[ig];
+        b1 = b1 + ftmp * gy[ig];
+        b2 = b2 + ftmp * gz[ig];
+      }
+      floc[3 * ia] = floc[3 * ia] + b0;
+      floc[3 * ia + 1] = floc[3 * ia + 1] + b1;
+      floc[3 * ia + 2] = floc[3 * ia + 2] + b2;
+    }
+    for (k = nhk1[it]; k <= nhk2[it]; ++k) {
+      const int ia = iatmpt[k - 1] - 1;
+      floc[3 * ia] = floc[3 * ia] * 2.0;
+      floc[3 * ia + 1] = floc[3 * ia + 1] * 2.0;
+      floc[3 * ia + 2] = floc[3 * ia + 2] * 2.0;
+    }
+  }
+}
+
 /* enkbpp_g2 (Sources/nlkbpp.f90:1763-1824), Gamma point. */
 double qxo_enkbpp_g(const double *slmir, int lmmax, int natom, const int *lmx_atom,
                     const double *clalp_atom, const double *occ, int nb, double rvol)

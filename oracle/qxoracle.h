@@ -80,6 +80,11 @@ int qxo_mxchgg_pulay(double *rhgr, double *rinr, int nplwcex, int nplw5ex, const
                      int lreset, int lmixhold, int itratn, int *kntchg_io, double *unr,
                      double *dfmr, double *aij, double *cmk_out);
 void qxo_mxchgg_simple(double *rhgr, double *rinr, int nplw5ex, double aslh, int none);
+void qxo_flocal_g(int ntype, const int *nhk1, const int *nhk2, const int *iatmpt,
+                  const int *llking, int natom, int nplw5, const int *nga, const int *ngb,
+                  const int *ngc, const double *gx, const double *gy, const double *gz,
+                  const double *rhgr, const double *svco, const int *kmax,
+                  const double *const *dcq, const double *const *dsq, double *floc);
 double qxo_enkbpp_g(const double *slmir, int lmmax, int natom, const int *lmx_atom,
                     const double *clalp_atom, const double *occ, int nb, double rvol);
 

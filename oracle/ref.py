@@ -261,3 +261,27 @@ def invers(w):
     L.qxref_invers.argtypes = [_dp, _dp, C.c_int, C.POINTER(C.c_int)]
     L.qxref_invers(flat, wi, n, C.byref(ierr))
     return wi.reshape(n, n).T.copy(), ierr.value
+
+
+def flocal_g(ntype, nhk1, nhk2, iatmpt, llking, natom, nga, ngb, ngc, gx, gy, gz, rhgr, svco,
+             kmax, tables, floc=None):
+    """Reference flocal_g (no stress); same arguments as oracle.flocal_g."""
+    L = lib()
+    rh = _F(rhgr)
+    n5 = rh.size // 2 - 1
+    f = np.zeros((3, natom), order="F") if floc is None else _F(np.asarray(floc).T).copy(order="F")
+    ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    L.qxref_flocal_g.argtypes = ([C.c_int, ip, ip, ip, ip, C.c_int, C.c_int, ip, ip, ip, _dp,
+                                  _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int]
+                                 + [_dp] * 7)
+    L.qxref_flocal_g(int(ntype), np.ascontiguousarray(nhk1, np.int32),
+                     np.ascontiguousarray(nhk2, np.int32), np.ascontiguousarray(iatmpt, np.int32),
+                     np.ascontiguousarray(llking, np.int32), int(natom), n5,
+                     np.ascontiguousarray(np.asarray(nga)[: n5 + 1], np.int32),
+                     np.ascontiguousarray(np.asarray(ngb)[: n5 + 1], np.int32),
+                     np.ascontiguousarray(np.asarray(ngc)[: n5 + 1], np.int32),
+                     _F(np.asarray(gx)[: n5 + 1]), _F(np.asarray(gy)[: n5 + 1]),
+                     _F(np.asarray(gz)[: n5 + 1]), rh, _F(np.asarray(svco).T), int(kmax[0]),
+                     int(kmax[1]), int(kmax[2]), *[_F(t[0].T) for t in tables],
+                     *[_F(t[1].T) for t in tables], f)
+    return np.ascontiguousarray(f.T)

@@ -504,3 +504,61 @@ subroutine qxref_invers(w, wi, n, ierr) bind(C, name="qxref_invers")
   ierr = ie
   deallocate(work, ip)
 end subroutine
+
+! flocal_g (Sources/nlkbpp.f90:1037) without stress on one plane-wave rank:
+! svco of lopp_variables and the rank layout of planewave_decomp_variables set
+! here; DCQ*/DSQ* (2 kmax + 1, natom) per axis
+subroutine qxref_flocal_g(ntype, nhk1, nhk2, iatmpt, llk, natom, nplw5, nga, ngb, ngc, gx, gy,  &
+&    gz, rhgr, svco_in, kmax1, kmax2, kmax3, dcq1, dcq2, dcq3, dsq1, dsq2, dsq3, floc)  &
+&    bind(C, name="qxref_flocal_g")
+  use iso_c_binding
+  use lopp_variables
+  use planewave_decomp_variables
+  implicit none
+  integer(c_int), value :: ntype, natom, nplw5, kmax1, kmax2, kmax3
+  integer(c_int) :: nhk1(ntype), nhk2(ntype), iatmpt(natom), llk(ntype)
+  integer(c_int) :: nga(0:nplw5), ngb(0:nplw5), ngc(0:nplw5)
+  real(c_double) :: gx(0:nplw5), gy(0:nplw5), gz(0:nplw5), rhgr(2 * (nplw5 + 1))
+  real(c_double) :: svco_in(nplw5 + 1, ntype)
+  real(c_double) :: dcq1(-kmax1:kmax1, natom), dcq2(-kmax2:kmax2, natom), dcq3(-kmax3:kmax3, natom)
+  real(c_double) :: dsq1(-kmax1:kmax1, natom), dsq2(-kmax2:kmax2, natom), dsq3(-kmax3:kmax3, natom)
+  real(c_double) :: floc(3, natom)
+  integer :: nf(2), na, ni, n5, ni5, nh1(ntype), nh2(ntype), iat(natom), it1(natom)
+  integer :: k1, k2, k3, it
+  integer, allocatable :: g1(:), g2(:), g3(:)
+  logical :: llking(ntype)
+  real*8 :: st(3, 3), elclv
+  if (allocated(svco)) deallocate(svco)
+  allocate(svco(nplw5 + 1, ntype))
+  svco = svco_in
+  myid_pw = 0
+  nodes_pw = 1
+  nplw5nod1 = 1
+  nplw5nod = nplw5 + 1
+  nplw5nod2 = nplw5 + 1
+  nf = 6
+  na = natom
+  ni = natom
+  n5 = nplw5
+  ni5 = 2 * (nplw5 + 1)
+  nh1 = nhk1
+  nh2 = nhk2
+  iat = iatmpt
+  it1 = 1
+  allocate(g1(0:nplw5), g2(0:nplw5), g3(0:nplw5))
+  g1 = nga
+  g2 = ngb
+  g3 = ngc
+  do it = 1, ntype
+    llking(it) = llk(it) /= 0
+  end do
+  k1 = kmax1
+  k2 = kmax2
+  k3 = kmax3
+  st = 0.d0
+  elclv = 0.d0
+  call flocal_g(nf, 0, 1, floc, st, .false., ntype, nh1, nh2, it1, iat, na, ni, llking, n5, ni5,  &
+&    n5, g1, g2, g3, gx, gy, gz, rhgr, elclv, k1, k2, k3, dcq1, dcq2, dcq3, dsq1, dsq2, dsq3,  &
+&    .false.)
+  deallocate(g1, g2, g3)
+end subroutine

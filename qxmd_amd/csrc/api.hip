@@ -274,6 +274,7 @@ int qx_destroy(qx_ctx* h) {
   lt_destroy(c);
   nl_destroy(c);
   scf_destroy(c);
+  lopp_destroy(c);
   (void)hipStreamDestroy(c->stream);
   delete h;
   return 0;
@@ -765,6 +766,41 @@ int qx_icholesky(qx_ctx* h, double* s, int n, int* ok) {
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < n; ++i) s[i + (size_t)n * j] = xr[(size_t)i * n + j];  // row-major -> col-major
   return 0;
+}
+
+// ---------------- local-pseudopotential forces ------------------------------
+int qx_set_lopp(qx_ctx* h, int ntype, int nplw5, const double* svco, const double* gx,
+                const double* gy, const double* gz) {
+  int rc = lopp_set(&h->c, ntype, nplw5, svco, gx, gy, gz);
+  return rc ? fail(rc, "qx_set_lopp: bad sizes (%d)", rc) : 0;
+}
+
+int qx_flocal_g(qx_ctx* h, double* floc, int ntype, const int* nhk1, const int* nhk2,
+                const int* iatmpt, int natom, const int* llking, int nplw5, const int* nga,
+                const int* ngb, const int* ngc, const double* rhgr, int kmax1, int kmax2,
+                int kmax3, const double* dcq1, const double* dcq2, const double* dcq3,
+                const double* dsq1, const double* dsq2, const double* dsq3) {
+  const int km[3] = {kmax1, kmax2, kmax3};
+  const double* dc[3] = {dcq1, dcq2, dcq3};
+  const double* ds[3] = {dsq1, dsq2, dsq3};
+  int rc = flocal_host(&h->c, floc, ntype, nhk1, nhk2, iatmpt, natom, llking, nplw5, nga, ngb,
+                       ngc, rhgr, km, dc, ds);
+  if (rc == -4) return fail(rc, "qx_flocal_g: call qx_set_lopp with this ntype and nplw5 first");
+  return rc ? fail(rc, "qx_flocal_g: bad ion ranges or G beyond kmax (%d)", rc) : 0;
+}
+
+int qx_flocal_g_dev(qx_ctx* h, int ntype, const int* nhk1, const int* nhk2, const int* iatmpt,
+                    const int* llking, int natom, double* floc) {
+  Ctx* c = &h->c;
+  if (!c->scf.d_rhgr || c->scf.nplw5 != c->lopp.nplw5)
+    return fail(-2, "qx_flocal_g_dev: the density sphere (qx_scf_set_grid) and qx_set_lopp "
+                    "must have the same nplw5");
+  int rc = flocal_dev(c, ntype, nhk1, nhk2, iatmpt, llking, natom, c->scf.d_rhgr, c->scf.nplw5,
+                      floc);
+  if (rc == -5) return fail(rc, "qx_flocal_g_dev: call qx_stscfn_pw over the density sphere "
+                                "(nplw5) for these atoms first");
+  if (rc == -4) return fail(rc, "qx_flocal_g_dev: call qx_set_lopp first");
+  return rc ? fail(rc, "qx_flocal_g_dev: bad ion ranges (%d)", rc) : 0;
 }
 
 // ---------------- SCF glue --------------------------------------------------

@@ -104,6 +104,20 @@ struct Ctx {
   double *d_ycos = nullptr, *d_ysin = nullptr;  // (nplwcs+1) x natom
   int sf_natom = 0, sf_nplwcs = 0;
   double gmax2 = 0.0;                           // largest |G|^2 of the sphere
+  // axis tables c_k + i s_k of the last qx_stscfn_pw (per atom and axis,
+  // k = -km..km at (ia * 3 + a) * ld + km_a + k; sines after 3 natom ld
+  // cosines) and its G list nga | ngb | ngc over 0..ngmax
+  struct AxisTables {
+    double* d_tab = nullptr;
+    int* d_g = nullptr;
+    int ld = 0, km[3] = {0, 0, 0}, ngmax = -1, natom = 0;
+  } sf;
+  // local pseudopotential form factors svco(ig, it), ig = 0..nplw5, and the
+  // Cartesian G of the density sphere, for flocal_g
+  struct Lopp {
+    int ntype = 0, nplw5 = -1;
+    double *d_svco = nullptr, *d_g = nullptr;  // d_g: gx | gy | gz
+  } lopp;
   // SCF glue on the dense grid (scfglue.hip): the density sphere, Hartree
   // kernel, fixed potentials, the charge-density vectors and the mixer history
   FullGrid fg;
@@ -203,6 +217,16 @@ int dropin_mxchgg(Ctx* c, int* lreset, double* rhgr, double* rinr, int nplw5, do
 // KB Hellmann-Feynman forces (forces.hip): fnlc(3, natom) += kbnlpp_frc2
 int kb_forces(Ctx* c, const double* psi, int nb, const double* occ_host, const double* gx,
               const double* gy, const double* gz, double* fnlc_host);
+// local-pseudopotential forces (flocal.hip)
+int lopp_set(Ctx* c, int ntype, int nplw5, const double* svco, const double* gx,
+             const double* gy, const double* gz);
+void lopp_destroy(Ctx* c);
+int flocal_dev(Ctx* c, int ntype, const int* nhk1, const int* nhk2, const int* iatmpt,
+               const int* llking, int natom, const double* d_rhgr, int nplw5, double* floc);
+int flocal_host(Ctx* c, double* floc, int ntype, const int* nhk1, const int* nhk2,
+                const int* iatmpt, int natom, const int* llking, int nplw5, const int* nga,
+                const int* ngb, const int* ngc, const double* rhgr, const int* kmax,
+                const double* const* dcq, const double* const* dsq);
 int coop_launch(Ctx* c, const void* fn, int want, size_t lds, void** args, bool exact);
 int dense_chol_inv(Ctx* c, double* s, int n, double* x, int* ok, int* flag);
 int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,

@@ -140,6 +140,9 @@ void free_nl(Ctx* c) {
 
 void nl_destroy(Ctx* c) {
   free_nl(c);
+  if (c->sf.d_tab) (void)hipFree(c->sf.d_tab);
+  if (c->sf.d_g) (void)hipFree(c->sf.d_g);
+  c->sf = Ctx::AxisTables{};
   if (c->d_ycos) (void)hipFree(c->d_ycos);
   if (c->d_ysin) (void)hipFree(c->d_ysin);
   c->d_ycos = c->d_ysin = nullptr;
@@ -230,22 +233,28 @@ int nl_stscfn(Ctx* c, const double* ratm, int natom, const double* hcell, const 
   QX_CHECK(hipMemsetAsync(c->d_ycos, 0, sizeof(double) * ny, c->stream));
   QX_CHECK(hipMemsetAsync(c->d_ysin, 0, sizeof(double) * ny, c->stream));
   double* d_cs1 = upload(cs1.data(), cs1.size(), c->stream);
+  // the G list kept with the axis tables covers the whole density sphere
+  // when kmax does (what flocal_g needs), else the structure-factor range
+  int nkeep = nplw5;
+  for (int ig = ngmax + 1; ig <= nplw5 && nkeep == nplw5; ++ig)
+    if (std::abs(nga[ig]) > kmax1 || std::abs(ngb[ig]) > kmax2 || std::abs(ngc[ig]) > kmax3)
+      nkeep = ngmax;
+  const size_t ng1 = (size_t)nkeep + 1;
   int* d_g = nullptr;
   double* d_tab = nullptr;
-  QX_CHECK(hipMalloc(&d_g, sizeof(int) * 3 * (size_t)(ngmax + 1)));
+  QX_CHECK(hipMalloc(&d_g, sizeof(int) * 3 * ng1));
   QX_CHECK(hipMalloc(&d_tab, sizeof(double) * 2 * 3 * (size_t)natom * ld));
   if (!d_cs1) return -3;
-  QX_CHECK(hipMemcpyAsync(d_g, nga, sizeof(int) * (ngmax + 1), hipMemcpyHostToDevice, c->stream));
-  QX_CHECK(hipMemcpyAsync(d_g + ngmax + 1, ngb, sizeof(int) * (ngmax + 1), hipMemcpyHostToDevice,
+  QX_CHECK(hipMemcpyAsync(d_g, nga, sizeof(int) * ng1, hipMemcpyHostToDevice, c->stream));
+  QX_CHECK(hipMemcpyAsync(d_g + ng1, ngb, sizeof(int) * ng1, hipMemcpyHostToDevice, c->stream));
+  QX_CHECK(hipMemcpyAsync(d_g + 2 * ng1, ngc, sizeof(int) * ng1, hipMemcpyHostToDevice,
                           c->stream));
-  QX_CHECK(hipMemcpyAsync(d_g + 2 * (ngmax + 1), ngc, sizeof(int) * (ngmax + 1),
-                          hipMemcpyHostToDevice, c->stream));
   double* dc = d_tab;
   double* ds = d_tab + 3 * (size_t)natom * ld;
   hipLaunchKernelGGL(k_stscfn_axis, dim3(cdivu(3L * natom, 64)), dim3(64), 0, c->stream, d_cs1,
                      natom, kmax1, kmax2, kmax3, ld, dc, ds);
   hipLaunchKernelGGL(k_stscfn, dim3(cdivu(ngmax + 1, 256), natom), dim3(256), 0, c->stream, dc, ds,
-                     ld, kmax1, kmax2, kmax3, d_g, d_g + ngmax + 1, d_g + 2 * (ngmax + 1), ngmax,
+                     ld, kmax1, kmax2, kmax3, d_g, d_g + ng1, d_g + 2 * ng1, ngmax,
                      natom, nplwcs + 1, c->d_ycos, c->d_ysin);
   QX_CHECK(hipGetLastError());
   if (ycos_out)
@@ -256,8 +265,17 @@ int nl_stscfn(Ctx* c, const double* ratm, int natom, const double* hcell, const 
                             c->stream));
   QX_CHECK(hipStreamSynchronize(c->stream));
   (void)hipFree(d_cs1);
-  (void)hipFree(d_g);
-  (void)hipFree(d_tab);
+  // the axis tables and the G list stay for flocal_g (local-pseudopotential forces)
+  if (c->sf.d_tab) (void)hipFree(c->sf.d_tab);
+  if (c->sf.d_g) (void)hipFree(c->sf.d_g);
+  c->sf.d_tab = d_tab;
+  c->sf.d_g = d_g;
+  c->sf.ld = ld;
+  c->sf.km[0] = kmax1;
+  c->sf.km[1] = kmax2;
+  c->sf.km[2] = kmax3;
+  c->sf.ngmax = nkeep;
+  c->sf.natom = natom;
   return 0;
 }
 

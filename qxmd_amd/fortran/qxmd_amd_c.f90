@@ -353,6 +353,35 @@ integer(c_int) function qx_softchg(ctx, cgjr, nb, occ, rvol, rho) bind(C, name="
   real(c_double), intent(out) :: rho(*)
 end function
 
+! ---- local-pseudopotential forces
+integer(c_int) function qx_set_lopp(ctx, ntype, nplw5, svco, gx, gy, gz)  &
+&     bind(C, name="qx_set_lopp")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: ntype, nplw5
+  real(c_double), intent(in) :: svco(*), gx(*), gy(*), gz(*)
+end function
+
+integer(c_int) function qx_flocal_g(ctx, floc, ntype, nhk1, nhk2, iatmpt, natom, llking, nplw5,  &
+&     nga, ngb, ngc, rhgr, kmax1, kmax2, kmax3, dcq1, dcq2, dcq3, dsq1, dsq2, dsq3)  &
+&     bind(C, name="qx_flocal_g")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_double), intent(inout) :: floc(*)
+  integer(c_int), value :: ntype, natom, nplw5, kmax1, kmax2, kmax3
+  integer(c_int), intent(in) :: nhk1(*), nhk2(*), iatmpt(*), llking(*), nga(*), ngb(*), ngc(*)
+  real(c_double), intent(in) :: rhgr(*), dcq1(*), dcq2(*), dcq3(*), dsq1(*), dsq2(*), dsq3(*)
+end function
+
+integer(c_int) function qx_flocal_g_dev(ctx, ntype, nhk1, nhk2, iatmpt, llking, natom, floc)  &
+&     bind(C, name="qx_flocal_g_dev")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: ntype, natom
+  integer(c_int), intent(in) :: nhk1(*), nhk2(*), iatmpt(*), llking(*)
+  real(c_double), intent(inout) :: floc(*)
+end function
+
 ! ---- SCF glue
 integer(c_int) function qx_xclda(rho, rhocore, mshnod, rdelv, vexc, eeexc, e4)  &
 &     bind(C, name="qx_xclda")

@@ -226,6 +226,16 @@ subroutine qxmd_scf_band_step(gdcr, nplwex, nband, occ, noband, rvol, eig, rho, 
   lcholesky = ok /= 0
 end subroutine
 
+! svco(nplw5+1, ntype) of lopp_variables (Sources/ppkb.f90:66-79) and the
+! Cartesian G of the density sphere, once per cell, for flocal_g
+subroutine qxmd_gpu_set_lopp(ntype, nplw5, svco, gx, gy, gz)
+  integer, intent(in) :: ntype, nplw5
+  real*8, intent(in) :: svco(nplw5 + 1, ntype), gx(0:nplw5), gy(0:nplw5), gz(0:nplw5)
+  call qxmd_gpu_need_ctx('qxmd_gpu_set_lopp')
+  call qxmd_gpu_check(qx_set_lopp(qxctx, int(ntype, c_int), int(nplw5, c_int), svco, gx, gy,  &
+&      gz), 'qxmd_gpu_set_lopp')
+end subroutine
+
 end module qxmd_amd_gpu
 
 
@@ -606,4 +616,43 @@ lr = merge(1_c_int, 0_c_int, lreset)
 call qxmd_gpu_check(qx_mxchgg(qxctx, lr, rhgr, rinr, int(nplw5, c_int), aslh, bslh, glocal, mf,  &
 &    int(ntotfd, c_int), ij, merge(1_c_int, 0_c_int, lmixhold)), 'mxchgg')
 lreset = .false.
+end subroutine
+
+
+! flocal_g (Sources/nlkbpp.f90:1037-1231) with the reference's argument list:
+! local-pseudopotential forces added to floc (and doubled with it, as there);
+! svco and gx/gy/gz from qxmd_gpu_set_lopp.  Stress (lcstress) stops with an
+! error.
+subroutine flocal_g(nfile, myid, nodes, floc, strloc, lcstress, ntype, nhk1, nhk2, iatoit,  &
+& iatmpt, natom, nion, llking, nplw5, nplw5ex, nplw, nga, ngb, ngc, gx, gy, gz, rhgr, elclv,  &
+& kmax1, kmax2, kmax3, DCQ1, DCQ2, DCQ3, DSQ1, DSQ2, DSQ3, ltimecnt)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: nfile(*), myid, nodes, ntype, natom, nion, nplw5, nplw5ex, nplw
+integer :: kmax1, kmax2, kmax3
+real*8 :: floc(3, natom), strloc(3, 3), elclv
+logical :: lcstress, ltimecnt, llking(ntype)
+integer :: nhk1(ntype), nhk2(ntype), iatoit(natom), iatmpt(nion)
+integer :: nga(0:nplw5), ngb(0:nplw5), ngc(0:nplw5)
+real*8 :: gx(0:nplw5), gy(0:nplw5), gz(0:nplw5), rhgr(nplw5ex)
+real*8 :: DCQ1(-kmax1:kmax1, natom), DSQ1(-kmax1:kmax1, natom)
+real*8 :: DCQ2(-kmax2:kmax2, natom), DSQ2(-kmax2:kmax2, natom)
+real*8 :: DCQ3(-kmax3:kmax3, natom), DSQ3(-kmax3:kmax3, natom)
+integer(c_int) :: h1(ntype), h2(ntype), ia(nion), lk(ntype), g1(0:nplw5), g2(0:nplw5), g3(0:nplw5)
+integer :: it
+call qxmd_gpu_need_ctx('flocal_g')
+if (lcstress) call qxmd_gpu_check(-1_c_int, 'flocal_g: stress (lcstress) is not on the device')
+h1 = nhk1
+h2 = nhk2
+ia = iatmpt
+do it = 1, ntype
+  lk(it) = merge(1_c_int, 0_c_int, llking(it))
+end do
+g1 = nga
+g2 = ngb
+g3 = ngc
+call qxmd_gpu_check(qx_flocal_g(qxctx, floc, int(ntype, c_int), h1, h2, ia, int(natom, c_int),  &
+&    lk, int(nplw5, c_int), g1, g2, g3, rhgr, int(kmax1, c_int), int(kmax2, c_int),  &
+&    int(kmax3, c_int), DCQ1, DCQ2, DCQ3, DSQ1, DSQ2, DSQ3), 'flocal_g')
 end subroutine

@@ -82,6 +82,10 @@ SIGNATURES = {
     "qx_softchg": [_vp, _dp, _i, _dp, _d, _dp],
     "qx_rseigqr": [_vp, _vp, _i, _dp],
     "qx_icholesky": [_vp, _vp, _i, _pi],
+    "qx_set_lopp": [_vp, _i, _i, _dp, _dp, _dp, _dp],
+    "qx_flocal_g": [_vp, _dp, _i, _ip, _ip, _ip, _i, _ip, _i, _ip, _ip, _ip, _dp, _i, _i, _i,
+                    _dp, _dp, _dp, _dp, _dp, _dp],
+    "qx_flocal_g_dev": [_vp, _i, _ip, _ip, _ip, _ip, _i, _dp],
     "qx_xclda": [_dp, _vp, _i, _d, _dp, _dp, _dp],
     "qx_harting": [_dp, _dp, _i, _dp, _vp, _i, _ip, _ip],
     "qx_chgr2g": [_dp, _dp, _i, _vp, _i, _ip, _ip],

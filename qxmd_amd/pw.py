@@ -308,6 +308,36 @@ class PlaneWave:
               "qx_icholesky")
         return np.asarray(m), bool(ok.value)
 
+    # ---- local-pseudopotential forces (flocal_g)
+    def set_lopp(self, svco):
+        """svco (ntype, nplw5 + 1): local form factors over the density sphere."""
+        gs = self.gs
+        sv = f64(svco)
+        n5 = gs.nplw5
+        assert sv.shape[1] == n5 + 1
+        check(load().qx_set_lopp(self.h, sv.shape[0], n5, sv, f64(gs.gx[: n5 + 1]),
+                                 f64(gs.gy[: n5 + 1]), f64(gs.gz[: n5 + 1])), "qx_set_lopp")
+
+    def flocal_g(self, ntype, nhk1, nhk2, iatmpt, llking, natom, rhgr, kmax, tables, floc=None):
+        """Host-array flocal_g: tables = [(DCQ_a, DSQ_a)] with (natom, 2 kmax_a + 1)."""
+        gs = self.gs
+        n5 = gs.nplw5
+        f = np.zeros((natom, 3)) if floc is None else f64(floc).copy()
+        t = [(f64(c), f64(s)) for c, s in tables]
+        check(load().qx_flocal_g(self.h, f, int(ntype), i32(nhk1), i32(nhk2), i32(iatmpt),
+                                 int(natom), i32(llking), n5, i32(gs.nga[: n5 + 1]),
+                                 i32(gs.ngb[: n5 + 1]), i32(gs.ngc[: n5 + 1]), f64(rhgr),
+                                 int(kmax[0]), int(kmax[1]), int(kmax[2]), t[0][0], t[1][0],
+                                 t[2][0], t[0][1], t[1][1], t[2][1]), "qx_flocal_g")
+        return f
+
+    def flocal_g_dev(self, ntype, nhk1, nhk2, iatmpt, llking, natom, floc=None):
+        """Device-resident flocal_g on the context's density sphere."""
+        f = np.zeros((natom, 3)) if floc is None else f64(floc).copy()
+        check(load().qx_flocal_g_dev(self.h, int(ntype), i32(nhk1), i32(nhk2), i32(iatmpt),
+                                     i32(llking), int(natom), f), "qx_flocal_g_dev")
+        return f
+
     # ---- SCF glue (harting, xclda, potential assembly, charge mixing)
     def scf_set_grid(self, recnrmex, vext=None, rhocore=None, rdelv=None):
         """Density sphere of the context's G list, Hartree kernel recnrmex

@@ -50,6 +50,24 @@ def nlpp_case(sp, ntype=2):
                 recnrm=np.asarray(sp.recnrm[:nw]), nplwcs=sp.nplw)
 
 
+def flocal_case(sp, seed=3):
+    """Inputs of flocal_g for a Sphere: two types splitting the atoms, iatmpt a
+    fixed permutation, a random density on the density sphere and random
+    local form factors svco (ntype, nplw5 + 1), axis tables of the atoms."""
+    import oracle as O
+    c = nlpp_case(sp)
+    natom = c["natom"]
+    n5 = sp.nplw5
+    kmax = [int(np.abs(np.asarray(a[: n5 + 1])).max()) for a in (sp.nga, sp.ngb, sp.ngc)]
+    rng = np.random.default_rng(seed)
+    rhgr = rng.standard_normal(2 * (n5 + 1)) * 0.01
+    rhgr[1] = 0.0
+    svco = rng.standard_normal((2, n5 + 1))
+    iatmpt = np.roll(np.arange(1, natom + 1), 3)
+    return dict(ntype=2, nhk1=c["nhk1"], nhk2=c["nhk2"], iatmpt=iatmpt, natom=natom, kmax=kmax,
+                tables=O.axis_tables(c["frac"], kmax), rhgr=rhgr, svco=svco, frac=c["frac"])
+
+
 _SPHERES = {}
 
 

@@ -178,6 +178,9 @@ call check_stscfn()
 ! ---- simple and Pulay mixing through mxchgg_alloc / mxchgg
 call check_scf_glue()
 
+! ---- flocal_g: two atoms, three G vectors, against the sums written out
+call check_flocal()
+
 call qxmd_gpu_finalize()
 write(*, '(a, es10.2)') 'DROPIN OK worst', worst
 
@@ -387,6 +390,61 @@ subroutine check_scf_glue()
   e2 = maxval(abs(out - (c1 * (in1 + fk * f1) + c2 * (rg2 + fk * f2))))
   call report('mxchgg Pulay: Kerker step, then 2 x 2 normal equation', max(e1, e2) / 0.03d0,  &
 &     1.d-11)
+end subroutine
+
+! flocal_g on the density sphere of check_scf_glue's three G vectors, two
+! atoms of two types (iatmpt swapped), axis tables cos/sin(2 pi k s)
+subroutine check_flocal()
+  integer, parameter :: n5 = 3, na = 2, km = 2
+  integer :: g1(0:n5), g2(0:n5), g3(0:n5), nh1(2), nh2(2), iat(na), it_(na), nf(1), ig, ja, q
+  real*8 :: sv(n5 + 1, 2), gxv(0:n5), gyv(0:n5), gzv(0:n5), rg(2 * (n5 + 1)), s(3, na)
+  real*8 :: c1(-km:km, na), c2(-km:km, na), c3(-km:km, na), d1(-km:km, na), d2(-km:km, na)
+  real*8 :: d3(-km:km, na), fl(3, na), fr(3, na), st(3, 3), ph, ftmp, tpl
+  logical :: llk(2)
+  g1 = (/0, 1, 0, 1/)
+  g2 = (/0, 0, 2, -1/)
+  g3 = (/0, 0, 1, 2/)
+  tpl = 2.d0 * pi / 10.26d0
+  gxv = tpl * g1
+  gyv = tpl * g2
+  gzv = tpl * g3
+  do ig = 0, n5
+    sv(ig + 1, 1) = 1.d0 / (1.d0 + ig)
+    sv(ig + 1, 2) = -0.5d0 * ig
+  end do
+  rg = (/0.03d0, 0.d0, 0.004d0, -0.002d0, -0.003d0, 0.001d0, 0.0015d0, 0.0025d0/)
+  s(:, 1) = (/0.1d0, 0.25d0, 0.7d0/)
+  s(:, 2) = (/0.55d0, 0.8d0, 0.05d0/)
+  do ja = 1, na
+    do q = -km, km
+      c1(q, ja) = cos(2.d0 * pi * q * s(1, ja)); d1(q, ja) = sin(2.d0 * pi * q * s(1, ja))
+      c2(q, ja) = cos(2.d0 * pi * q * s(2, ja)); d2(q, ja) = sin(2.d0 * pi * q * s(2, ja))
+      c3(q, ja) = cos(2.d0 * pi * q * s(3, ja)); d3(q, ja) = sin(2.d0 * pi * q * s(3, ja))
+    end do
+  end do
+  ! ion 1 -> atom 2 (type 1), ion 2 -> atom 1 (type 2)
+  nh1 = (/1, 2/)
+  nh2 = (/1, 2/)
+  iat = (/2, 1/)
+  it_ = (/2, 1/)
+  fr = 0.d0
+  do ja = 1, na
+    do ig = 1, n5
+      ph = 2.d0 * pi * (g1(ig) * s(1, ja) + g2(ig) * s(2, ja) + g3(ig) * s(3, ja))
+      ftmp = sv(ig + 1, it_(ja)) * (rg(2 * ig + 1) * sin(ph) + rg(2 * ig + 2) * cos(ph))
+      fr(1, ja) = fr(1, ja) + 2.d0 * ftmp * gxv(ig)
+      fr(2, ja) = fr(2, ja) + 2.d0 * ftmp * gyv(ig)
+      fr(3, ja) = fr(3, ja) + 2.d0 * ftmp * gzv(ig)
+    end do
+  end do
+  call qxmd_gpu_set_lopp(2, n5, sv, gxv, gyv, gzv)
+  llk = .false.
+  fl = 0.d0
+  st = 0.d0
+  nf = 6
+  call flocal_g(nf, 0, 1, fl, st, .false., 2, nh1, nh2, it_, iat, na, na, llk, n5, 2 * (n5 + 1),  &
+&     n5, g1, g2, g3, gxv, gyv, gzv, rg, 0.d0, km, km, km, c1, c2, c3, d1, d2, d3, .false.)
+  call report('flocal_g vs its sums written out', maxval(abs(fl - fr)) / maxval(abs(fr)), 1.d-13)
 end subroutine
 
 subroutine expect(cond, what)

@@ -125,6 +125,13 @@ def main():
         wi, ierr = R.invers(x @ x.T + 0.05 * np.eye(n))
         assert ierr == 0
         out[f"invers_{n}"] = np.asarray(wi)
+    # flocal_g (two types, a permuted iatmpt, llking on the second type once)
+    f = conftest.flocal_case(si8)
+    for tag, llk in (("", [0, 0]), ("_lking", [0, 1])):
+        out["flocal_g" + tag] = R.flocal_g(f["ntype"], f["nhk1"], f["nhk2"], f["iatmpt"], llk,
+                                           f["natom"], si8.nga, si8.ngb, si8.ngc, si8.gx, si8.gy,
+                                           si8.gz, f["rhgr"], f["svco"], f["kmax"], f["tables"],
+                                           np.full((f["natom"], 3), 0.25))
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes,", len(out), "arrays")
 

@@ -170,3 +170,28 @@ def test_kbnlpp_frc2_vs_reference(si8):
     _, sl = O.hckbpp_g(eq, lmx, vcl, psi, np.zeros_like(psi))
     fo = O.kbnlpp_frc2(eq, lmx, vcl, psi, occ, sl, si8.gx[:nw], si8.gy[:nw], si8.gz[:nw])
     assert np.abs(fo - fr).max() < 1e-11 * np.abs(fr).max()
+
+
+@pytest.mark.parametrize("llk", [[0, 0], [1, 0]])
+def test_flocal_g_vs_reference(si64, llk):
+    import conftest
+    f = conftest.flocal_case(si64, seed=11)
+    args = (f["ntype"], f["nhk1"], f["nhk2"], f["iatmpt"], llk, f["natom"], si64.nga, si64.ngb,
+            si64.ngc, si64.gx, si64.gy, si64.gz, f["rhgr"], f["svco"], f["kmax"], f["tables"])
+    fo, fr = O.flocal_g(*args), R.flocal_g(*args)
+    assert np.array_equal(fo, fr)
+
+
+def test_xclda_invers_vs_reference():
+    rng = np.random.default_rng(2)
+    rho = 10 ** rng.uniform(-7, 2, 3000)
+    core = np.where(rng.uniform(size=rho.size) < 0.5, 10 ** rng.uniform(-5, 0, rho.size), -1.0)
+    for c in (core, -np.ones_like(rho)):
+        vo, eo, e4o = O.xclda(rho, c, 0.11)
+        vr, er, e4r = R.xclda(rho, c, 0.11)
+        assert np.array_equal(vo, vr) and np.array_equal(eo, er) and np.array_equal(e4o, e4r)
+    for n in (1, 4, 11):
+        x = rng.standard_normal((n, n))
+        a = x @ x.T + 0.01 * np.eye(n)
+        (wo, io), (wr, ir) = O.invers(a), R.invers(a)
+        assert io == ir == 0 and np.array_equal(wo, wr)

@@ -219,3 +219,51 @@ def test_simple_mixers():
     rh, ri = O.mxchgg_simple(a, b, 0.3, none=True)
     np.testing.assert_array_equal(rh, a)
     np.testing.assert_array_equal(ri, a)
+
+
+@pytest.mark.parametrize("tag,llk", [("", [0, 0]), ("_lking", [0, 1])])
+def test_flocal_g_golden(si8, tag, llk):
+    """flocal_g against the reference build's output (bit-equal there): a
+    permuted iatmpt, a skipped (llking) type, and the incoming floc doubled
+    with the sum as the reference does."""
+    import conftest
+    f = conftest.flocal_case(si8)
+    out = O.flocal_g(f["ntype"], f["nhk1"], f["nhk2"], f["iatmpt"], llk, f["natom"], si8.nga,
+                     si8.ngb, si8.ngc, si8.gx, si8.gy, si8.gz, f["rhgr"], f["svco"], f["kmax"],
+                     f["tables"], np.full((f["natom"], 3), 0.25))
+    np.testing.assert_allclose(out, G["flocal_g" + tag], rtol=0, atol=1e-13)
+    if llk[1]:
+        skipped = np.asarray(f["iatmpt"])[f["nhk1"][1] - 1: f["nhk2"][1]] - 1
+        assert np.all(out[skipped] == 0.25)
+
+
+def test_flocal_g_is_energy_gradient(si8):
+    """floc = -dE/dtau for E = 2 sum_{G != 0} svco(G) Re(rho(G) exp(i G.tau))
+    (the local energy over the +-G sphere), phases from numpy cos/sin of
+    2 pi n.s, central differences in the Cartesian position (cubic cell)."""
+    import conftest
+    f = conftest.flocal_case(si8)
+    n5 = si8.nplw5
+    L = si8.cell.hcell[0, 0]
+    ng = np.stack([np.asarray(a[: n5 + 1]) for a in (si8.nga, si8.ngb, si8.ngc)], 1)
+    a, b = f["rhgr"][0::2], f["rhgr"][1::2]
+    types = np.zeros(f["natom"], int)
+    for it in range(2):
+        types[np.asarray(f["iatmpt"])[f["nhk1"][it] - 1: f["nhk2"][it]] - 1] = it
+
+    def energy(ia, s):
+        ph = 2 * np.pi * (ng[1:] @ s)
+        sv = f["svco"][types[ia], 1:]
+        return 2 * np.sum(sv * (a[1:] * np.cos(ph) - b[1:] * np.sin(ph)))
+
+    fl = O.flocal_g(f["ntype"], f["nhk1"], f["nhk2"], f["iatmpt"], [0, 0], f["natom"], si8.nga,
+                    si8.ngb, si8.ngc, si8.gx, si8.gy, si8.gz, f["rhgr"], f["svco"], f["kmax"],
+                    f["tables"])
+    h = 1e-5
+    for ia in (0, 5):
+        for ax in range(3):
+            sp_, sm = f["frac"][ia].copy(), f["frac"][ia].copy()
+            sp_[ax] += h / L
+            sm[ax] -= h / L
+            fd = -(energy(ia, sp_) - energy(ia, sm)) / (2 * h)
+            assert abs(fl[ia, ax] - fd) < 1e-6 * np.abs(fl).max(), (ia, ax, fl[ia, ax], fd)
