@@ -143,6 +143,29 @@ def flocal(ctx, cell, gs, reps=5):
     return round((time.perf_counter() - t0) * 1e3 / reps, 4)
 
 
+def ewald(ctx, cell, gs, gamma=0.35, reps=5):
+    """Time frcrec on the device: the Ewald G list = the half-sphere G with
+    (pi g / gamma)^2 < 37 (units without 2 pi, as clmrcp builds it), one type
+    with zv = 4, the per-atom axis tables of the last device stscfn_pw."""
+    tp = 2 * np.pi
+    g2 = gs.recnrm[1: gs.nplw5 + 1] / tp ** 2
+    n = int(np.searchsorted(g2, 37.0 * gamma ** 2 / np.pi ** 2))
+    m = slice(1, n + 1)
+    ew = dict(nga=gs.nga[m], ngb=gs.ngb[m], ngc=gs.ngc[m], gx=gs.gx[m] / tp, gy=gs.gy[m] / tp,
+              gz=gs.gz[m] / tp, recnrm=g2[:n], recnrmex=np.zeros(n + 1), gamma=gamma,
+              volume=gs.volume)
+    ctx.set_ewald(ew)
+    natom = len(cell.tau)
+    args = (1, [1], [natom], [4.0], 4 * natom, natom, np.ones(natom, np.int32),
+            np.arange(1, natom + 1))
+    ctx.frcrec(*args)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.frcrec(*args)
+    return round((time.perf_counter() - t0) * 1e3 / reps, 4), n
+
+
 def fft_stage_bytes(gs, s):
     """Algorithmic FFT-stage bytes per band-apply: 8*N_h*s_c + N*s_r + 2*(nplw+1)*s_c
     (two real 3-D transforms as two HBM passes each, V(r) read, sphere I/O)."""
@@ -383,6 +406,10 @@ def main():
     # local-pseudopotential forces on the mixed density the glue left
     extra["flocal_ms"] = flocal(ctx, cell, gs)
     print(f"# flocal_g ({natom} atoms, {gs.nplw5} G): {extra['flocal_ms']:.3f} ms",
+          file=sys.stderr, flush=True)
+    # Ewald reciprocal-space forces on the device axis tables flocal left
+    extra["ewald_frcrec_ms"], numk2 = ewald(ctx, cell, gs)
+    print(f"# frcrec ({natom} ions, {numk2} Ewald G): {extra['ewald_frcrec_ms']:.3f} ms",
           file=sys.stderr, flush=True)
 
     cpu = None

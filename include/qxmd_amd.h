@@ -220,6 +220,30 @@ int qx_flocal_g(qx_ctx *ctx, double *floc, int ntype, const int *nhk1, const int
 int qx_flocal_g_dev(qx_ctx *ctx, int ntype, const int *nhk1, const int *nhk2,
                     const int *iatmpt, const int *llking, int natom, double *floc);
 
+/* ---------------- Ewald reciprocal-space forces ----------------------------- */
+/* The Ewald G list of clmrcp (Sources/bulk.f90:556-1068) and para_ewald:
+ * nga/ngb/ngc, gx/gy/gz, recnrm over m = 1..numk2 (arrays from index 0),
+ * recnrmex(0:numk2), gamma, volume.  Once per cell. */
+int qx_set_ewald(qx_ctx *ctx, int numk2, const int *nga, const int *ngb, const int *ngc,
+                 const double *gx, const double *gy, const double *gz, const double *recnrm,
+                 const double *recnrmex, double gamma, double volume);
+/* frcrec without stress (Sources/bulk.f90:1880-2026): fclm(3, natom),
+ * *ecolmb and atom_ecolmb(natom) accumulate the reciprocal-space Ewald
+ * forces and energies of the ions k = 1..nion (atom iatmpt(k), type
+ * iatoit(atom), all 1-based).  Host form: the per-ion axis tables
+ * DCQ1..3 and DSQ1..3 (-kcsma:kcsma, nion) of the Ewald stscfn
+ * (Sources/bulk.f90:2086). */
+int qx_frcrec(qx_ctx *ctx, double *fclm, int ntype, const int *nhk1, const int *nhk2,
+              const double *zv, int nel, double *ecolmb, double *atom_ecolmb, const int *iatoit,
+              const int *iatmpt, int natom, int nion, int kcsma1, int kcsma2, int kcsma3,
+              const double *dcq1, const double *dcq2, const double *dcq3, const double *dsq1,
+              const double *dsq2, const double *dsq3);
+/* device-resident form: the per-atom axis tables of the last qx_stscfn_pw
+ * (its kmax must cover the Ewald G list). */
+int qx_frcrec_dev(qx_ctx *ctx, double *fclm, int ntype, const int *nhk1, const int *nhk2,
+                  const double *zv, int nel, double *ecolmb, double *atom_ecolmb,
+                  const int *iatoit, const int *iatmpt, int natom, int nion);
+
 /* ---------------- SCF glue between band updates ----------------------------
  * Sphere arrays: rhgr(0:2*nplw5+1) real-packed (Re, Im per G, G index
  * 0..nplw5); ijkgd points at ijkgd(-nplw5) of the array ijkgd(-nplw5:nplw5)

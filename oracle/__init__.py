@@ -106,6 +106,9 @@ def _declare(L):
                                    C.POINTER(_i), _dp, _dp, _dp, _dp]
     L.qxo_mxchgg_pulay.restype = _i
     L.qxo_mxchgg_simple.argtypes = [_dp, _dp, _i, _d, _i]
+    L.qxo_frcrec.argtypes = [_i, _ip, _ip, _dp, _i, _i, _i, _ip, _ip, _i, _ip, _ip, _ip, _dp, _dp,
+                             _dp, _dp, _dp, _d, _d, _ip, C.POINTER(_vp), C.POINTER(_vp), _dp,
+                             C.POINTER(_d), _dp]
     L.qxo_flocal_g.argtypes = [_i, _ip, _ip, _ip, _ip, _i, _i, _ip, _ip, _ip, _dp, _dp, _dp, _dp,
                                _dp, _ip, C.POINTER(_vp), C.POINTER(_vp), _dp]
     L.qxo_kbnlpp_frc2.argtypes = [_dp, _i, _i, _i, _ip, _dp, _dp, _i, _dp, _dp, _dp, _dp, _dp,
@@ -388,6 +391,30 @@ def flocal_g(ntype, nhk1, nhk2, iatmpt, llking, natom, nga, ngb, ngc, gx, gy, gz
                        _f(np.asarray(gx)[: n5 + 1]), _f(np.asarray(gy)[: n5 + 1]),
                        _f(np.asarray(gz)[: n5 + 1]), rh, sv, _f(kmax, np.int32), dc, ds, f)
     return f
+
+
+def frcrec(ntype, nhk1, nhk2, zv, nel, natom, iatoit, iatmpt, ew, tables, fclm=None,
+           ecolmb=0.0, atom_ecolmb=None):
+    """frcrec without stress (Sources/bulk.f90:1880-2026).  ew: the Ewald G
+    list dict (nga, ngb, ngc, gx, gy, gz, recnrm over m = 1..numk2 at [m-1],
+    recnrmex over 0..numk2, gamma, volume); tables: per-ion axis tables
+    [(DCQ_a, DSQ_a)] (nion, 2 kcsma_a + 1).  Returns (fclm (natom, 3),
+    ecolmb, atom_ecolmb)."""
+    nion = len(iatmpt)
+    f = np.zeros((natom, 3)) if fclm is None else _f(fclm).copy()
+    ae = np.zeros(natom) if atom_ecolmb is None else _f(atom_ecolmb).copy()
+    e = _d(ecolmb)
+    km = np.array([(t[0].shape[1] - 1) // 2 for t in tables], np.int32)
+    dc = (_vp * 3)(*[t[0].ctypes.data for t in tables])
+    ds = (_vp * 3)(*[t[1].ctypes.data for t in tables])
+    n = len(ew["nga"])
+    lib().qxo_frcrec(int(ntype), _f(nhk1, np.int32), _f(nhk2, np.int32), _f(zv), int(nel),
+                     int(natom), nion, _f(iatoit, np.int32), _f(iatmpt, np.int32), n,
+                     _f(ew["nga"], np.int32), _f(ew["ngb"], np.int32), _f(ew["ngc"], np.int32),
+                     _f(ew["gx"]), _f(ew["gy"]), _f(ew["gz"]), _f(ew["recnrm"]),
+                     _f(ew["recnrmex"]), float(ew["gamma"]), float(ew["volume"]), km, dc, ds, f,
+                     C.byref(e), ae)
+    return f, e.value, ae
 
 
 def kbnlpp_frc2(eqlmr, lmx_atom, vclalp_atom, cgjr, occ, slmir, gx, gy, gz):

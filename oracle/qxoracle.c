@@ -1175,6 +1175,108 @@ void qxo_flocal_g(int ntype, const int *nhk1, const int *nhk2, const int *iatmpt
   }
 }
 
+/* frcrec without stress, VECTOR form (Sources/bulk.f90:1880-2026): Ewald
+ * reciprocal-space forces, energy and per-atom energies.  Per G (m =
+ * 1..numk2, arrays at [m-1]) the ion sum S = sum_k zv (cos, sin)(G.tau_k)
+ * from the per-ion axis tables (k = -kcsma..kcsma, dcq[a] + k_ion * (2
+ * kcsma_a + 1)), hep = (exp(-(pi/gamma)^2 |g|^2) - recnrmex) / |g|^2;
+ * forces += 8/V hep G (Re S sin_k - Im S cos_k) zv, energies as there.
+ * iatoit, iatmpt, nhk1/nhk2 1-based. */
+void qxo_frcrec(int ntype, const int *nhk1, const int *nhk2, const double *zv, int nel,
+                int natom, int nion, const int *iatoit, const int *iatmpt, int numk2,
+                const int *nga, const int *ngb, const int *ngc, const double *gx, const double *gy,
+                const double *gz, const double *recnrm, const double *recnrmex, double gamma,
+                double volume, const int *kcsma, const double *const *dcq,
+                const double *const *dsq, double *fclm, double *ecolmb, double *atom_ecolmb)
+{
+  const double pi = acos(-1.0);
+  const double vol8 = 8.0 / volume, const2 = -((pi / gamma) * (pi / gamma));
+  double cst = 2.0 / (volume * pi), recolmb = 0.0, ztot2 = 0.0;
+  double *sc = (double *)calloc((size_t)numk2, sizeof(double));
+  double *ss = (double *)calloc((size_t)numk2, sizeof(double));
+  double *hp = (double *)malloc(sizeof(double) * (size_t)numk2);
+  double *acc = (double *)calloc(4 * (size_t)nion, sizeof(double));
+  double *dyc = (double *)malloc(sizeof(double) * (size_t)nion);
+  double *dys = (double *)malloc(sizeof(double) * (size_t)nion);
+  int k, m, it;
+#define TAB(t, a, kk, q) ((t)[a][(size_t)(kk) * (2 * kcsma[a] + 1) + kcsma[a] + (q)])
+  for (k = 0; k < nion; ++k) {
+    const int ia = iatmpt[k] - 1, ty = iatoit[ia] - 1;
+    for (m = 0; m < numk2; ++m) {
+      const int k1 = nga[m], k2 = ngb[m], k3 = ngc[m];
+      const double b1 = TAB(dcq, 0, k, k1) * TAB(dcq, 1, k, k2) - TAB(dsq, 0, k, k1) * TAB(dsq, 1, k, k2);
+      const double b2 = TAB(dsq, 0, k, k1) * TAB(dcq, 1, k, k2) + TAB(dcq, 0, k, k1) * TAB(dsq, 1, k, k2);
+      double yc = b1 * TAB(dcq, 2, k, k3) - b2 * TAB(dsq, 2, k, k3);
+      double ys = b2 * TAB(dcq, 2, k, k3) + b1 * TAB(dsq, 2, k, k3);
+      yc = zv[ty] * yc;
+      ys = zv[ty] * ys;
+      sc[m] = sc[m] + yc;
+      ss[m] = ss[m] + ys;
+    }
+  }
+  for (m = 0; m < numk2; ++m) {
+    const double recr = 1.0 / recnrm[m];
+    hp[m] = (exp(const2 * recnrm[m]) - recnrmex[m + 1]) * recr;
+  }
+  for (m = 0; m < numk2; ++m) {
+    const int k1 = nga[m], k2 = ngb[m], k3 = ngc[m];
+    const double dckm = sc[m], dskm = ss[m], hep = hp[m];
+    double hep8, hckm, hskm, dckmx, dckmy, dckmz, dskmx, dskmy, dskmz, hepc;
+    for (k = 0; k < nion; ++k) {
+      const int ia = iatmpt[k] - 1, ty = iatoit[ia] - 1;
+      const double d12 = TAB(dcq, 0, k, k1) * TAB(dcq, 1, k, k2) - TAB(dsq, 0, k, k1) * TAB(dsq, 1, k, k2);
+      const double s12 = TAB(dsq, 0, k, k1) * TAB(dcq, 1, k, k2) + TAB(dcq, 0, k, k1) * TAB(dsq, 1, k, k2);
+      double yc = d12 * TAB(dcq, 2, k, k3) - s12 * TAB(dsq, 2, k, k3);
+      double ys = s12 * TAB(dcq, 2, k, k3) + d12 * TAB(dsq, 2, k, k3);
+      dyc[k] = zv[ty] * yc;
+      dys[k] = zv[ty] * ys;
+    }
+    recolmb = recolmb + hep * (dckm * dckm + dskm * dskm);
+    hep8 = hep * vol8;
+    hckm = hep8 * dckm;
+    hskm = hep8 * dskm;
+    dckmx = hckm * gx[m];
+    dckmy = hckm * gy[m];
+    dckmz = hckm * gz[m];
+    dskmx = hskm * gx[m];
+    dskmy = hskm * gy[m];
+    dskmz = hskm * gz[m];
+    hepc = hep * cst;
+    hckm = hepc * dckm;
+    hskm = hepc * dskm;
+    for (k = 0; k < nion; ++k) {
+      acc[4 * k] = acc[4 * k] + dckmx * dys[k] - dskmx * dyc[k];
+      acc[4 * k + 1] = acc[4 * k + 1] + dckmy * dys[k] - dskmy * dyc[k];
+      acc[4 * k + 2] = acc[4 * k + 2] + dckmz * dys[k] - dskmz * dyc[k];
+      acc[4 * k + 3] = acc[4 * k + 3] + dyc[k] * hckm + dys[k] * hskm;
+    }
+  }
+#undef TAB
+  for (k = 0; k < nion; ++k) {
+    const int ia = iatmpt[k] - 1;
+    fclm[3 * ia] = fclm[3 * ia] + acc[4 * k];
+    fclm[3 * ia + 1] = fclm[3 * ia + 1] + acc[4 * k + 1];
+    fclm[3 * ia + 2] = fclm[3 * ia + 2] + acc[4 * k + 2];
+    atom_ecolmb[ia] = atom_ecolmb[ia] + acc[4 * k + 3];
+  }
+  *ecolmb = *ecolmb + cst * recolmb;
+  for (it = 0; it < ntype; ++it) ztot2 = ztot2 + (double)(nhk2[it] - nhk1[it] + 1) * zv[it] * zv[it];
+  *ecolmb = *ecolmb - (double)(nel * nel) / volume * (pi / (gamma * gamma) - recnrmex[0]) -
+            2.0 * gamma * ztot2 / sqrt(pi);
+  for (k = 0; k < nion; ++k) {
+    const int ia = iatmpt[k] - 1, ty = iatoit[ia] - 1;
+    atom_ecolmb[ia] = atom_ecolmb[ia] -
+                      zv[ty] * (double)nel / volume * (pi / (gamma * gamma) - recnrmex[0]) -
+                      2.0 * gamma * zv[ty] * zv[ty] / sqrt(pi);
+  }
+  free(sc);
+  free(ss);
+  free(hp);
+  free(acc);
+  free(dyc);
+  free(dys);
+}
+
 /* enkbpp_g2 (Sources/nlkbpp.f90:1763-1824), Gamma point. */
 double qxo_enkbpp_g(const double *slmir, int lmmax, int natom, const int *lmx_atom,
                     const double *clalp_atom, const double *occ, int nb, double rvol)

@@ -85,6 +85,12 @@ void qxo_flocal_g(int ntype, const int *nhk1, const int *nhk2, const int *iatmpt
                   const int *ngc, const double *gx, const double *gy, const double *gz,
                   const double *rhgr, const double *svco, const int *kmax,
                   const double *const *dcq, const double *const *dsq, double *floc);
+void qxo_frcrec(int ntype, const int *nhk1, const int *nhk2, const double *zv, int nel,
+                int natom, int nion, const int *iatoit, const int *iatmpt, int numk2,
+                const int *nga, const int *ngb, const int *ngc, const double *gx, const double *gy,
+                const double *gz, const double *recnrm, const double *recnrmex, double gamma,
+                double volume, const int *kcsma, const double *const *dcq,
+                const double *const *dsq, double *fclm, double *ecolmb, double *atom_ecolmb);
 double qxo_enkbpp_g(const double *slmir, int lmmax, int natom, const int *lmx_atom,
                     const double *clalp_atom, const double *occ, int nb, double rvol);
 

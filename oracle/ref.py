@@ -285,3 +285,28 @@ def flocal_g(ntype, nhk1, nhk2, iatmpt, llking, natom, nga, ngb, ngc, gx, gy, gz
                      int(kmax[1]), int(kmax[2]), *[_F(t[0].T) for t in tables],
                      *[_F(t[1].T) for t in tables], f)
     return np.ascontiguousarray(f.T)
+
+
+def frcrec(ntype, nhk1, nhk2, zv, nel, natom, iatoit, iatmpt, ew, tables, fclm=None,
+           ecolmb=0.0, atom_ecolmb=None):
+    """Reference frcrec (VECTOR form, no stress); arguments as oracle.frcrec."""
+    L = lib()
+    nion = len(iatmpt)
+    f = (np.zeros((3, natom), order="F") if fclm is None
+         else _F(np.asarray(fclm).T).copy(order="F"))
+    ae = np.zeros(natom) if atom_ecolmb is None else _F(atom_ecolmb).copy()
+    e = C.c_double(ecolmb)
+    km = [(t[0].shape[1] - 1) // 2 for t in tables]
+    ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    n = len(ew["nga"])
+    L.qxref_frcrec.argtypes = ([C.c_int, ip, ip, _dp, C.c_int, C.c_int, C.c_int, ip, ip, C.c_int,
+                                ip, ip, ip, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                                C.c_int, C.c_int, C.c_int] + [_dp] * 7
+                               + [C.POINTER(C.c_double), _dp])
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    L.qxref_frcrec(int(ntype), i32(nhk1), i32(nhk2), _F(zv), int(nel), int(natom), nion,
+                   i32(iatoit), i32(iatmpt), n, i32(ew["nga"]), i32(ew["ngb"]), i32(ew["ngc"]),
+                   _F(ew["gx"]), _F(ew["gy"]), _F(ew["gz"]), _F(ew["recnrm"]), _F(ew["recnrmex"]),
+                   float(ew["gamma"]), float(ew["volume"]), *km, *[_F(t[0].T) for t in tables],
+                   *[_F(t[1].T) for t in tables], f, C.byref(e), ae)
+    return np.ascontiguousarray(f.T), e.value, ae

@@ -562,3 +562,67 @@ subroutine qxref_flocal_g(ntype, nhk1, nhk2, iatmpt, llk, natom, nplw5, nga, ngb
 &    .false.)
   deallocate(g1, g2, g3)
 end subroutine
+
+! frcrec (Sources/bulk.f90:1880, VECTOR form) without stress: the Ewald G list,
+! its axis tables per ion and the parameters of para_ewald set here
+subroutine qxref_frcrec(ntype, nhk1, nhk2, zv, nel, natom, nion, iatoit, iatmpt, numk2_, nga_,  &
+&    ngb_, ngc_, gx_, gy_, gz_, recnrm_, recnrmex_, gamma_, volume_, km1, km2, km3, c1, c2, c3,  &
+&    s1, s2, s3, fclm, ecolmb, atom_ecolmb) bind(C, name="qxref_frcrec")
+  use iso_c_binding
+  use para_ewald
+  implicit none
+  integer(c_int), value :: ntype, nel, natom, nion, numk2_, km1, km2, km3
+  integer(c_int) :: nhk1(ntype), nhk2(ntype), iatoit(natom), iatmpt(nion)
+  integer(c_int) :: nga_(numk2_), ngb_(numk2_), ngc_(numk2_)
+  real(c_double) :: zv(ntype), gx_(numk2_), gy_(numk2_), gz_(numk2_), recnrm_(numk2_)
+  real(c_double) :: recnrmex_(0:numk2_)
+  real(c_double), value :: gamma_, volume_
+  real(c_double) :: c1(-km1:km1, nion), c2(-km2:km2, nion), c3(-km3:km3, nion)
+  real(c_double) :: s1(-km1:km1, nion), s2(-km2:km2, nion), s3(-km3:km3, nion)
+  real(c_double) :: fclm(3, natom), ecolmb, atom_ecolmb(natom)
+  integer :: nt, nl, na, ni, h1(ntype), h2(ntype), iat(natom), imp(nion)
+  real*8 :: z(ntype), st(3, 3), e
+  real*8, allocatable :: dyc(:), dys(:)
+  if (allocated(nga)) deallocate(nga, ngb, ngc, gx, gy, gz, recnrm, recnrmex, sumtmp, buff)
+  if (allocated(DCQ1)) deallocate(DCQ1, DCQ2, DCQ3, DSQ1, DSQ2, DSQ3)
+  numk2 = numk2_
+  allocate(nga(numk2), ngb(numk2), ngc(numk2), gx(numk2), gy(numk2), gz(numk2), recnrm(numk2),  &
+&    recnrmex(0:numk2), sumtmp(4 * numk2), buff(4 * numk2))
+  nga = nga_
+  ngb = ngb_
+  ngc = ngc_
+  gx = gx_
+  gy = gy_
+  gz = gz_
+  recnrm = recnrm_
+  recnrmex = recnrmex_
+  KCSMA1 = km1
+  KCSMA2 = km2
+  KCSMA3 = km3
+  allocate(DCQ1(-km1:km1, nion), DCQ2(-km2:km2, nion), DCQ3(-km3:km3, nion),  &
+&    DSQ1(-km1:km1, nion), DSQ2(-km2:km2, nion), DSQ3(-km3:km3, nion))
+  DCQ1 = c1
+  DCQ2 = c2
+  DCQ3 = c3
+  DSQ1 = s1
+  DSQ2 = s2
+  DSQ3 = s3
+  gamma = gamma_
+  volume = volume_
+  pi = acos(-1.d0)
+  allocate(dyc(3 * natom), dys(3 * natom))
+  nt = ntype
+  nl = nel
+  na = natom
+  ni = nion
+  h1 = nhk1
+  h2 = nhk2
+  iat = iatoit
+  imp = iatmpt
+  z = zv
+  st = 0.d0
+  e = ecolmb
+  call frcrec(fclm, st, nt, h1, h2, z, nl, e, atom_ecolmb, iat, imp, na, ni, dyc, dys, .false.)
+  ecolmb = e
+  deallocate(dyc, dys)
+end subroutine

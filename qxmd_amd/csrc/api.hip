@@ -275,6 +275,7 @@ int qx_destroy(qx_ctx* h) {
   nl_destroy(c);
   scf_destroy(c);
   lopp_destroy(c);
+  ewald_destroy(c);
   (void)hipStreamDestroy(c->stream);
   delete h;
   return 0;
@@ -801,6 +802,39 @@ int qx_flocal_g_dev(qx_ctx* h, int ntype, const int* nhk1, const int* nhk2, cons
                                 "(nplw5) for these atoms first");
   if (rc == -4) return fail(rc, "qx_flocal_g_dev: call qx_set_lopp first");
   return rc ? fail(rc, "qx_flocal_g_dev: bad ion ranges (%d)", rc) : 0;
+}
+
+// ---------------- Ewald reciprocal-space forces ------------------------------
+int qx_set_ewald(qx_ctx* h, int numk2, const int* nga, const int* ngb, const int* ngc,
+                 const double* gx, const double* gy, const double* gz, const double* recnrm,
+                 const double* recnrmex, double gamma, double volume) {
+  int rc = ewald_set(&h->c, numk2, nga, ngb, ngc, gx, gy, gz, recnrm, recnrmex, gamma, volume);
+  return rc ? fail(rc, "qx_set_ewald: bad sizes or parameters (%d)", rc) : 0;
+}
+
+int qx_frcrec(qx_ctx* h, double* fclm, int ntype, const int* nhk1, const int* nhk2,
+              const double* zv, int nel, double* ecolmb, double* atom_ecolmb, const int* iatoit,
+              const int* iatmpt, int natom, int nion, int kcsma1, int kcsma2, int kcsma3,
+              const double* dcq1, const double* dcq2, const double* dcq3, const double* dsq1,
+              const double* dsq2, const double* dsq3) {
+  const int km[3] = {kcsma1, kcsma2, kcsma3};
+  const double* dc[3] = {dcq1, dcq2, dcq3};
+  const double* ds[3] = {dsq1, dsq2, dsq3};
+  int rc = ewald_frcrec_host(&h->c, fclm, ntype, nhk1, nhk2, zv, nel, ecolmb, atom_ecolmb, iatoit,
+                             iatmpt, natom, nion, km, dc, ds);
+  if (rc == -4) return fail(rc, "qx_frcrec: call qx_set_ewald first");
+  return rc ? fail(rc, "qx_frcrec: bad ion map or tables narrower than the G list (%d)", rc) : 0;
+}
+
+int qx_frcrec_dev(qx_ctx* h, double* fclm, int ntype, const int* nhk1, const int* nhk2,
+                  const double* zv, int nel, double* ecolmb, double* atom_ecolmb,
+                  const int* iatoit, const int* iatmpt, int natom, int nion) {
+  int rc = ewald_frcrec_dev(&h->c, fclm, ntype, nhk1, nhk2, zv, nel, ecolmb, atom_ecolmb, iatoit,
+                            iatmpt, natom, nion);
+  if (rc == -4) return fail(rc, "qx_frcrec_dev: call qx_set_ewald first");
+  if (rc == -5) return fail(rc, "qx_frcrec_dev: call qx_stscfn_pw for these atoms with kmax "
+                                "covering the Ewald G list first");
+  return rc ? fail(rc, "qx_frcrec_dev: bad ion map (%d)", rc) : 0;
 }
 
 // ---------------- SCF glue --------------------------------------------------

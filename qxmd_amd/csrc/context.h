@@ -118,6 +118,14 @@ struct Ctx {
     int ntype = 0, nplw5 = -1;
     double *d_svco = nullptr, *d_g = nullptr;  // d_g: gx | gy | gz
   } lopp;
+  // Ewald reciprocal-space G list (clmrcp) with hep = (exp(-(pi/gamma)^2 g^2)
+  // - recnrmex) / g^2 per G, for frcrec
+  struct Ewald {
+    int numk2 = -1, km[3] = {0, 0, 0};
+    int* d_g = nullptr;                         // nga | ngb | ngc
+    double *d_gv = nullptr, *d_hep = nullptr;   // gx | gy | gz, hep
+    double gamma = 0.0, volume = 0.0, recnrmex0 = 0.0;
+  } ew;
   // SCF glue on the dense grid (scfglue.hip): the density sphere, Hartree
   // kernel, fixed potentials, the charge-density vectors and the mixer history
   FullGrid fg;
@@ -221,6 +229,18 @@ int kb_forces(Ctx* c, const double* psi, int nb, const double* occ_host, const d
 int lopp_set(Ctx* c, int ntype, int nplw5, const double* svco, const double* gx,
              const double* gy, const double* gz);
 void lopp_destroy(Ctx* c);
+// Ewald reciprocal-space forces (ewald.hip)
+void ewald_destroy(Ctx* c);
+int ewald_set(Ctx* c, int numk2, const int* nga, const int* ngb, const int* ngc, const double* gx,
+              const double* gy, const double* gz, const double* recnrm, const double* recnrmex,
+              double gamma, double volume);
+int ewald_frcrec_host(Ctx* c, double* fclm, int ntype, const int* nhk1, const int* nhk2,
+                      const double* zv, int nel, double* ecolmb, double* atom_ecolmb,
+                      const int* iatoit, const int* iatmpt, int natom, int nion, const int* kcsma,
+                      const double* const* dcq, const double* const* dsq);
+int ewald_frcrec_dev(Ctx* c, double* fclm, int ntype, const int* nhk1, const int* nhk2,
+                     const double* zv, int nel, double* ecolmb, double* atom_ecolmb,
+                     const int* iatoit, const int* iatmpt, int natom, int nion);
 int flocal_dev(Ctx* c, int ntype, const int* nhk1, const int* nhk2, const int* iatmpt,
                const int* llking, int natom, const double* d_rhgr, int nplw5, double* floc);
 int flocal_host(Ctx* c, double* floc, int ntype, const int* nhk1, const int* nhk2,

@@ -382,6 +382,38 @@ integer(c_int) function qx_flocal_g_dev(ctx, ntype, nhk1, nhk2, iatmpt, llking, 
   real(c_double), intent(inout) :: floc(*)
 end function
 
+! ---- Ewald reciprocal-space forces
+integer(c_int) function qx_set_ewald(ctx, numk2, nga, ngb, ngc, gx, gy, gz, recnrm, recnrmex,  &
+&     gamma, volume) bind(C, name="qx_set_ewald")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: numk2
+  integer(c_int), intent(in) :: nga(*), ngb(*), ngc(*)
+  real(c_double), intent(in) :: gx(*), gy(*), gz(*), recnrm(*), recnrmex(*)
+  real(c_double), value :: gamma, volume
+end function
+
+integer(c_int) function qx_frcrec(ctx, fclm, ntype, nhk1, nhk2, zv, nel, ecolmb, atom_ecolmb,  &
+&     iatoit, iatmpt, natom, nion, kcsma1, kcsma2, kcsma3, dcq1, dcq2, dcq3, dsq1, dsq2, dsq3)  &
+&     bind(C, name="qx_frcrec")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_double), intent(inout) :: fclm(*), ecolmb, atom_ecolmb(*)
+  integer(c_int), value :: ntype, nel, natom, nion, kcsma1, kcsma2, kcsma3
+  integer(c_int), intent(in) :: nhk1(*), nhk2(*), iatoit(*), iatmpt(*)
+  real(c_double), intent(in) :: zv(*), dcq1(*), dcq2(*), dcq3(*), dsq1(*), dsq2(*), dsq3(*)
+end function
+
+integer(c_int) function qx_frcrec_dev(ctx, fclm, ntype, nhk1, nhk2, zv, nel, ecolmb,  &
+&     atom_ecolmb, iatoit, iatmpt, natom, nion) bind(C, name="qx_frcrec_dev")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  real(c_double), intent(inout) :: fclm(*), ecolmb, atom_ecolmb(*)
+  integer(c_int), value :: ntype, nel, natom, nion
+  integer(c_int), intent(in) :: nhk1(*), nhk2(*), iatoit(*), iatmpt(*)
+  real(c_double), intent(in) :: zv(*)
+end function
+
 ! ---- SCF glue
 integer(c_int) function qx_xclda(rho, rhocore, mshnod, rdelv, vexc, eeexc, e4)  &
 &     bind(C, name="qx_xclda")

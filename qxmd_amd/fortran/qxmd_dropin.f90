@@ -236,6 +236,44 @@ subroutine qxmd_gpu_set_lopp(ntype, nplw5, svco, gx, gy, gz)
 &      gz), 'qxmd_gpu_set_lopp')
 end subroutine
 
+! the Ewald G list of clmrcp and para_ewald's parameters, once per cell
+! (Sources/bulk.f90:556-1068): arrays over m = 1..numk2, recnrmex(0:numk2)
+subroutine qxmd_gpu_set_ewald(numk2, nga, ngb, ngc, gx, gy, gz, recnrm, recnrmex, gamma, volume)
+  integer, intent(in) :: numk2, nga(numk2), ngb(numk2), ngc(numk2)
+  real*8, intent(in) :: gx(numk2), gy(numk2), gz(numk2), recnrm(numk2), recnrmex(0:numk2)
+  real*8, intent(in) :: gamma, volume
+  integer(c_int) :: g1(max(numk2, 1)), g2(max(numk2, 1)), g3(max(numk2, 1))
+  call qxmd_gpu_need_ctx('qxmd_gpu_set_ewald')
+  g1(1:numk2) = nga
+  g2(1:numk2) = ngb
+  g3(1:numk2) = ngc
+  call qxmd_gpu_check(qx_set_ewald(qxctx, int(numk2, c_int), g1, g2, g3, gx, gy, gz, recnrm,  &
+&      recnrmex, gamma, volume), 'qxmd_gpu_set_ewald')
+end subroutine
+
+! frcrec (Sources/bulk.f90:1880-2026) without stress, explicit-argument form
+! (the reference reads the G list and the tables from module para_ewald):
+! call it in frcrec's body with para_ewald's KCSMA1..3 and DCQ1..DSQ3.
+subroutine qxmd_frcrec(fclm, ntype, nhk1, nhk2, zv, nel, ecolmb, atom_ecolmb, iatoit, iatmpt,  &
+&                      natom, nion, kcsma1, kcsma2, kcsma3, dcq1, dcq2, dcq3, dsq1, dsq2, dsq3)
+  integer, intent(in) :: ntype, nel, natom, nion, kcsma1, kcsma2, kcsma3
+  integer, intent(in) :: nhk1(ntype), nhk2(ntype), iatoit(natom), iatmpt(nion)
+  real*8, intent(inout) :: fclm(3, natom), ecolmb, atom_ecolmb(natom)
+  real*8, intent(in) :: zv(ntype)
+  real*8, intent(in) :: dcq1(-kcsma1:kcsma1, nion), dsq1(-kcsma1:kcsma1, nion)
+  real*8, intent(in) :: dcq2(-kcsma2:kcsma2, nion), dsq2(-kcsma2:kcsma2, nion)
+  real*8, intent(in) :: dcq3(-kcsma3:kcsma3, nion), dsq3(-kcsma3:kcsma3, nion)
+  integer(c_int) :: h1(ntype), h2(ntype), it(natom), ia(nion)
+  call qxmd_gpu_need_ctx('qxmd_frcrec')
+  h1 = nhk1
+  h2 = nhk2
+  it = iatoit
+  ia = iatmpt
+  call qxmd_gpu_check(qx_frcrec(qxctx, fclm, int(ntype, c_int), h1, h2, zv, int(nel, c_int),  &
+&      ecolmb, atom_ecolmb, it, ia, int(natom, c_int), int(nion, c_int), int(kcsma1, c_int),  &
+&      int(kcsma2, c_int), int(kcsma3, c_int), dcq1, dcq2, dcq3, dsq1, dsq2, dsq3), 'qxmd_frcrec')
+end subroutine
+
 end module qxmd_amd_gpu
 
 

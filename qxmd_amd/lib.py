@@ -86,6 +86,10 @@ SIGNATURES = {
     "qx_flocal_g": [_vp, _dp, _i, _ip, _ip, _ip, _i, _ip, _i, _ip, _ip, _ip, _dp, _i, _i, _i,
                     _dp, _dp, _dp, _dp, _dp, _dp],
     "qx_flocal_g_dev": [_vp, _i, _ip, _ip, _ip, _ip, _i, _dp],
+    "qx_set_ewald": [_vp, _i, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _d, _d],
+    "qx_frcrec": [_vp, _dp, _i, _ip, _ip, _dp, _i, _pd, _dp, _ip, _ip, _i, _i, _i, _i, _i, _dp,
+                  _dp, _dp, _dp, _dp, _dp],
+    "qx_frcrec_dev": [_vp, _dp, _i, _ip, _ip, _dp, _i, _pd, _dp, _ip, _ip, _i, _i],
     "qx_xclda": [_dp, _vp, _i, _d, _dp, _dp, _dp],
     "qx_harting": [_dp, _dp, _i, _dp, _vp, _i, _ip, _ip],
     "qx_chgr2g": [_dp, _dp, _i, _vp, _i, _ip, _ip],

@@ -338,6 +338,37 @@ class PlaneWave:
                                      i32(llking), int(natom), f), "qx_flocal_g_dev")
         return f
 
+    # ---- Ewald reciprocal-space forces (frcrec)
+    def set_ewald(self, ew):
+        """ew: nga, ngb, ngc, gx, gy, gz, recnrm (m = 1..numk2), recnrmex
+        (0..numk2), gamma, volume."""
+        n = len(ew["nga"])
+        check(load().qx_set_ewald(self.h, n, i32(ew["nga"]), i32(ew["ngb"]), i32(ew["ngc"]),
+                                  f64(ew["gx"]), f64(ew["gy"]), f64(ew["gz"]), f64(ew["recnrm"]),
+                                  f64(ew["recnrmex"]), float(ew["gamma"]), float(ew["volume"])),
+              "qx_set_ewald")
+
+    def frcrec(self, ntype, nhk1, nhk2, zv, nel, natom, iatoit, iatmpt, tables=None, fclm=None,
+               ecolmb=0.0, atom_ecolmb=None):
+        """Returns (fclm (natom, 3), ecolmb, atom_ecolmb); tables = per-ion
+        [(DCQ_a, DSQ_a)] (nion, 2 kcsma_a + 1), or None for the device tables."""
+        f = np.zeros((natom, 3)) if fclm is None else f64(fclm).copy()
+        ae = np.zeros(natom) if atom_ecolmb is None else f64(atom_ecolmb).copy()
+        e = C.c_double(ecolmb)
+        nion = len(iatmpt)
+        L = load()
+        if tables is None:
+            check(L.qx_frcrec_dev(self.h, f, int(ntype), i32(nhk1), i32(nhk2), f64(zv), int(nel),
+                                  C.byref(e), ae, i32(iatoit), i32(iatmpt), int(natom), nion),
+                  "qx_frcrec_dev")
+        else:
+            t = [(f64(c), f64(s)) for c, s in tables]
+            km = [(c.shape[1] - 1) // 2 for c, _ in t]
+            check(L.qx_frcrec(self.h, f, int(ntype), i32(nhk1), i32(nhk2), f64(zv), int(nel),
+                              C.byref(e), ae, i32(iatoit), i32(iatmpt), int(natom), nion, *km,
+                              t[0][0], t[1][0], t[2][0], t[0][1], t[1][1], t[2][1]), "qx_frcrec")
+        return f, e.value, ae
+
     # ---- SCF glue (harting, xclda, potential assembly, charge mixing)
     def scf_set_grid(self, recnrmex, vext=None, rhocore=None, rdelv=None):
         """Density sphere of the context's G list, Hartree kernel recnrmex

@@ -68,6 +68,35 @@ def flocal_case(sp, seed=3):
                 tables=O.axis_tables(c["frac"], kmax), rhgr=rhgr, svco=svco, frac=c["frac"])
 
 
+def ewald_case(sp, numk2=400, gamma=0.35, seed=4):
+    """Inputs of frcrec for a Sphere: the Ewald G list = the first numk2
+    half-sphere G of strvec in units without 2 pi (as clmrcp builds them),
+    recnrmex = 0 (no double grid), two types (zv 4 and 3) over a permuted
+    iatmpt, per-ion axis tables cos/sin(2 pi k s) up to the list's kcsma."""
+    import oracle as O
+    c = nlpp_case(sp)
+    natom = c["natom"]
+    tp = 2 * np.pi
+    m = slice(1, numk2 + 1)
+    ew = dict(nga=np.asarray(sp.nga[m], np.int32), ngb=np.asarray(sp.ngb[m], np.int32),
+              ngc=np.asarray(sp.ngc[m], np.int32), gx=np.asarray(sp.gx[m]) / tp,
+              gy=np.asarray(sp.gy[m]) / tp, gz=np.asarray(sp.gz[m]) / tp,
+              recnrm=np.asarray(sp.recnrm[m]) / tp ** 2, recnrmex=np.zeros(numk2 + 1),
+              gamma=gamma, volume=sp.cell.volume)
+    kcsma = [int(np.abs(ew[k]).max()) for k in ("nga", "ngb", "ngc")]
+    rng = np.random.default_rng(seed)
+    iatmpt = rng.permutation(natom) + 1
+    iatoit = np.where(np.arange(natom) < natom // 2, 1, 2).astype(np.int32)
+    zv = np.array([4.0, 3.0])
+    nhk1 = np.array([1, natom // 2 + 1], np.int32)
+    nhk2 = np.array([natom // 2, natom], np.int32)
+    nel = int(sum((nhk2[i] - nhk1[i] + 1) * zv[i] for i in range(2)))
+    frac = c["frac"]
+    tables = O.axis_tables(frac[iatmpt - 1], kcsma)
+    return dict(ntype=2, nhk1=nhk1, nhk2=nhk2, zv=zv, nel=nel, natom=natom, iatoit=iatoit,
+                iatmpt=iatmpt, ew=ew, tables=tables, kcsma=kcsma, frac=frac)
+
+
 _SPHERES = {}
 
 

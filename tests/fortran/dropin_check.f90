@@ -181,6 +181,9 @@ call check_scf_glue()
 ! ---- flocal_g: two atoms, three G vectors, against the sums written out
 call check_flocal()
 
+! ---- frcrec (Ewald reciprocal part): two ions, four G, against the sums
+call check_frcrec()
+
 call qxmd_gpu_finalize()
 write(*, '(a, es10.2)') 'DROPIN OK worst', worst
 
@@ -445,6 +448,81 @@ subroutine check_flocal()
   call flocal_g(nf, 0, 1, fl, st, .false., 2, nh1, nh2, it_, iat, na, na, llk, n5, 2 * (n5 + 1),  &
 &     n5, g1, g2, g3, gxv, gyv, gzv, rg, 0.d0, km, km, km, c1, c2, c3, d1, d2, d3, .false.)
   call report('flocal_g vs its sums written out', maxval(abs(fl - fr)) / maxval(abs(fr)), 1.d-13)
+end subroutine
+
+! frcrec on four Ewald G vectors (units without 2 pi) and two ions of two
+! types, iatmpt swapped: forces, total and per-atom energies written out
+subroutine check_frcrec()
+  integer, parameter :: ng = 4, na = 2, km = 2
+  integer :: g1(ng), g2(ng), g3(ng), nh1(2), nh2(2), iat(na), imp(na), m, k, ja, q
+  real*8 :: gxv(ng), gyv(ng), gzv(ng), rn(ng), rx(0:ng), zv2(2), s(3, na), gam, vol
+  real*8 :: c1(-km:km, na), c2(-km:km, na), c3(-km:km, na), d1(-km:km, na), d2(-km:km, na)
+  real*8 :: d3(-km:km, na), fl(3, na), fr(3, na), ae(na), aer(na), ec, er, hep, sc, ss, ph
+  real*8 :: yc(na), ys(na), cst
+  g1 = (/1, 0, 1, 2/)
+  g2 = (/0, 1, -1, 1/)
+  g3 = (/0, 1, 2, -2/)
+  do m = 1, ng
+    gxv(m) = g1(m) / 10.26d0
+    gyv(m) = g2(m) / 10.26d0
+    gzv(m) = g3(m) / 10.26d0
+    rn(m) = gxv(m)**2 + gyv(m)**2 + gzv(m)**2
+  end do
+  rx = 0.d0
+  gam = 0.4d0
+  vol = 10.26d0**3
+  zv2 = (/4.d0, 3.d0/)
+  s(:, 1) = (/0.1d0, 0.25d0, 0.7d0/)
+  s(:, 2) = (/0.55d0, 0.8d0, 0.05d0/)
+  nh1 = (/1, 2/)
+  nh2 = (/1, 2/)
+  iat = (/2, 1/)   ! atom 1 is type 2, atom 2 type 1
+  imp = (/2, 1/)   ! ion 1 is atom 2, ion 2 atom 1
+  do k = 1, na
+    ja = imp(k)
+    do q = -km, km
+      c1(q, k) = cos(2.d0 * pi * q * s(1, ja)); d1(q, k) = sin(2.d0 * pi * q * s(1, ja))
+      c2(q, k) = cos(2.d0 * pi * q * s(2, ja)); d2(q, k) = sin(2.d0 * pi * q * s(2, ja))
+      c3(q, k) = cos(2.d0 * pi * q * s(3, ja)); d3(q, k) = sin(2.d0 * pi * q * s(3, ja))
+    end do
+  end do
+  fr = 0.d0
+  aer = 0.d0
+  er = 0.d0
+  cst = 2.d0 / (vol * pi)
+  do m = 1, ng
+    sc = 0.d0
+    ss = 0.d0
+    do ja = 1, na
+      ph = 2.d0 * pi * (g1(m) * s(1, ja) + g2(m) * s(2, ja) + g3(m) * s(3, ja))
+      yc(ja) = zv2(iat(ja)) * cos(ph)
+      ys(ja) = zv2(iat(ja)) * sin(ph)
+      sc = sc + yc(ja)
+      ss = ss + ys(ja)
+    end do
+    hep = exp(-(pi / gam)**2 * rn(m)) / rn(m)
+    er = er + cst * hep * (sc * sc + ss * ss)
+    do ja = 1, na
+      fr(1, ja) = fr(1, ja) + 8.d0 / vol * hep * gxv(m) * (sc * ys(ja) - ss * yc(ja))
+      fr(2, ja) = fr(2, ja) + 8.d0 / vol * hep * gyv(m) * (sc * ys(ja) - ss * yc(ja))
+      fr(3, ja) = fr(3, ja) + 8.d0 / vol * hep * gzv(m) * (sc * ys(ja) - ss * yc(ja))
+      aer(ja) = aer(ja) + cst * hep * (yc(ja) * sc + ys(ja) * ss)
+    end do
+  end do
+  er = er - 49.d0 / vol * pi / gam**2 - 2.d0 * gam * 25.d0 / sqrt(pi)
+  do ja = 1, na
+    aer(ja) = aer(ja) - zv2(iat(ja)) * 7.d0 / vol * pi / gam**2  &
+&             - 2.d0 * gam * zv2(iat(ja))**2 / sqrt(pi)
+  end do
+  call qxmd_gpu_set_ewald(ng, g1, g2, g3, gxv, gyv, gzv, rn, rx, gam, vol)
+  fl = 0.d0
+  ae = 0.d0
+  ec = 0.d0
+  call qxmd_frcrec(fl, 2, nh1, nh2, zv2, 7, ec, ae, iat, imp, na, na, km, km, km, c1, c2, c3,  &
+&     d1, d2, d3)
+  call report('frcrec forces vs sums written out', maxval(abs(fl - fr)) / maxval(abs(fr)), 1.d-12)
+  call report('frcrec energies vs sums written out',  &
+&     max(abs(ec - er) / abs(er), maxval(abs(ae - aer)) / maxval(abs(aer))), 1.d-12)
 end subroutine
 
 subroutine expect(cond, what)

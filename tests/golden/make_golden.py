@@ -132,6 +132,14 @@ def main():
                                            f["natom"], si8.nga, si8.ngb, si8.ngc, si8.gx, si8.gy,
                                            si8.gz, f["rhgr"], f["svco"], f["kmax"], f["tables"],
                                            np.full((f["natom"], 3), 0.25))
+    # frcrec (Ewald reciprocal part, VECTOR form)
+    e = conftest.ewald_case(si8)
+    fr, er, aer = R.frcrec(e["ntype"], e["nhk1"], e["nhk2"], e["zv"], e["nel"], e["natom"],
+                           e["iatoit"], e["iatmpt"], e["ew"], e["tables"],
+                           np.full((e["natom"], 3), 0.5), 1.25, np.full(e["natom"], -0.5))
+    out["frcrec_fclm"] = fr
+    out["frcrec_ecolmb"] = np.array(er)
+    out["frcrec_atom_ecolmb"] = aer
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes,", len(out), "arrays")
 

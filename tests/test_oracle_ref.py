@@ -195,3 +195,13 @@ def test_xclda_invers_vs_reference():
         a = x @ x.T + 0.01 * np.eye(n)
         (wo, io), (wr, ir) = O.invers(a), R.invers(a)
         assert io == ir == 0 and np.array_equal(wo, wr)
+
+
+def test_frcrec_vs_reference(si64):
+    import conftest
+    e = conftest.ewald_case(si64, numk2=1500, gamma=0.3)
+    args = (e["ntype"], e["nhk1"], e["nhk2"], e["zv"], e["nel"], e["natom"], e["iatoit"],
+            e["iatmpt"], e["ew"], e["tables"])
+    fo, eo, ao = O.frcrec(*args)
+    fr, er, ar = R.frcrec(*args)
+    assert np.array_equal(fo, fr) and eo == er and np.array_equal(ao, ar)
