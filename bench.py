@@ -166,6 +166,26 @@ def ewald(ctx, cell, gs, gamma=0.35, reps=5):
     return round((time.perf_counter() - t0) * 1e3 / reps, 4), n
 
 
+def extrapolation(ctx, psi, nband, reps=3):
+    """Time sbalinaspc (ihest = 2, three previous sets): the bands of each
+    step are psi turned slightly out of its span, so the overlap, its
+    eigensolution, the rotation of the history and Gram-Schmidt all do real
+    work."""
+    rng = np.random.default_rng(3)
+    ctx.aspc_init(3, 2)
+    noise = 1e-3 * rng.standard_normal(psi.shape)
+    times = []
+    for step in range(1, 4 + reps):
+        ctx.put(0, psi + step * noise)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ia, ok = ctx.sbalinaspc(0, nband, step)
+        dt = time.perf_counter() - t0
+        if step > 3:
+            times.append(dt)
+    return {"step_ms": round(1e3 * float(np.mean(times)), 3), "iaspc": ia, "lcholesky": ok}
+
+
 def fft_stage_bytes(gs, s):
     """Algorithmic FFT-stage bytes per band-apply: 8*N_h*s_c + N*s_r + 2*(nplw+1)*s_c
     (two real 3-D transforms as two HBM passes each, V(r) read, sphere I/O)."""
@@ -410,6 +430,11 @@ def main():
     # Ewald reciprocal-space forces on the device axis tables flocal left
     extra["ewald_frcrec_ms"], numk2 = ewald(ctx, cell, gs)
     print(f"# frcrec ({natom} ions, {numk2} Ewald G): {extra['ewald_frcrec_ms']:.3f} ms",
+          file=sys.stderr, flush=True)
+
+    # wave-function extrapolation of an MD step (roteig + ASPC + Gram-Schmidt)
+    extra["sbalinaspc_ms"] = extrapolation(ctx, psi_orth, nband)
+    print(f"# sbalinaspc ({nband} bands, 3 previous steps): {extra['sbalinaspc_ms']}",
           file=sys.stderr, flush=True)
 
     cpu = None

@@ -220,6 +220,24 @@ int qx_flocal_g(qx_ctx *ctx, double *floc, int ntype, const int *nhk1, const int
 int qx_flocal_g_dev(qx_ctx *ctx, int ntype, const int *nhk1, const int *nhk2,
                     const int *iatmpt, const int *llking, int natom, double *floc);
 
+/* ---------------- wave-function extrapolation between MD steps ------------- */
+/* set_coef_ASPC with ichest = 0 (Sources/sbalin.f90:77-170): predictor of
+ * order ihest (kaspc = ihest + 1), history of nprvmx sets (<= 10) in HBM,
+ * keypwv = 0.  Resets the history. */
+int qx_aspc_init(qx_ctx *ctx, int nprvmx, int ihest);
+/* sbalinaspc for one spin without ultrasoft terms (Sources/sbalin.f90:
+ * 369-712) on the bands of slot 0 or 1: roteig2, the ASPC combination, the
+ * history shift, and Gram-Schmidt when ihest > 0.  info = {iaspc (0: the
+ * history was only started), lcholesky} if non-null. */
+int qx_sbalinaspc_slot(qx_ctx *ctx, int slot, int nb, int nstep, int nstep_ini, int *info);
+/* roteig2 (Sources/sbalin.f90:772-1373): history sets 1..min(nprvmx, kpwvx)
+ * turned to the bands of the slot. */
+int qx_roteig_slot(qx_ctx *ctx, int slot, int nb, int kpwvx);
+/* read / write history set k (0-based) in the reference's G order; put sets
+ * keypwv. */
+int qx_aspc_get(qx_ctx *ctx, int k, int nb, double *gdcr);
+int qx_aspc_put(qx_ctx *ctx, int k, int nb, const double *gdcr, int keypwv);
+
 /* ---------------- Ewald reciprocal-space forces ----------------------------- */
 /* The Ewald G list of clmrcp (Sources/bulk.f90:556-1068) and para_ewald:
  * nga/ngb/ngc, gx/gy/gz, recnrm over m = 1..numk2 (arrays from index 0),

@@ -106,6 +106,11 @@ def _declare(L):
                                    C.POINTER(_i), _dp, _dp, _dp, _dp]
     L.qxo_mxchgg_pulay.restype = _i
     L.qxo_mxchgg_simple.argtypes = [_dp, _dp, _i, _d, _i]
+    L.qxo_roteig.argtypes = [_dp, _dp, _i, _i, _i, _i, _vp]
+    L.qxo_roteig.restype = _i
+    L.qxo_coef_aspc.argtypes = [_i, _dp]
+    L.qxo_sbalinaspc.argtypes = [_dp, _i, _i, _dp, _i, _dp, _i, _i, _i, _i, C.POINTER(_i)]
+    L.qxo_sbalinaspc.restype = _i
     L.qxo_frcrec.argtypes = [_i, _ip, _ip, _dp, _i, _i, _i, _ip, _ip, _i, _ip, _ip, _ip, _dp, _dp,
                              _dp, _dp, _dp, _d, _d, _ip, C.POINTER(_vp), C.POINTER(_vp), _dp,
                              C.POINTER(_d), _dp]
@@ -415,6 +420,45 @@ def frcrec(ntype, nhk1, nhk2, zv, nel, natom, iatoit, iatmpt, ew, tables, fclm=N
                      _f(ew["recnrmex"]), float(ew["gamma"]), float(ew["volume"]), km, dc, ds, f,
                      C.byref(e), ae)
     return f, e.value, ae
+
+
+def roteig(cur, prev, kpwvx):
+    """roteig2 (Sources/sbalin.f90:772-1373): prev (nprvmx, nb, nplwex) is
+    rotated in place (sets 1..min(nprvmx, kpwvx)); returns (prev, U, rc)."""
+    c = _f(cur)
+    p = _f(prev).copy()
+    nprv, nb, n = p.shape
+    u = np.zeros(nb * nb)
+    rc = lib().qxo_roteig(c, p, n, nb, nprv, int(kpwvx), u.ctypes.data)
+    return p, u.reshape(nb, nb).T.copy(), rc
+
+
+def coef_aspc(kaspc):
+    """set_coef_ASPC's coefaspc (kaspc, kaspc) as coef[j-1, k-1]."""
+    c = np.zeros(kaspc * kaspc)
+    lib().qxo_coef_aspc(int(kaspc), c)
+    return c.reshape(kaspc, kaspc).T.copy()
+
+
+class Aspc:
+    """sbalinaspc state (Sources/sbalin.f90:369-712): the history prveig of
+    nprvmx sets and keypwv; step() advances one MD step."""
+
+    def __init__(self, nb, nplwex, nprvmx, ihest):
+        self.nprvmx, self.ihest = int(nprvmx), int(ihest)
+        self.kaspc = max(self.ihest, 0) + 1
+        self.coef = _f(np.ascontiguousarray(coef_aspc(self.kaspc).T).ravel())
+        self.prveig = np.zeros((self.nprvmx, nb, nplwex))
+        self.keypwv = 0
+
+    def step(self, gdcr, nstep, nstep_ini=0):
+        g = _f(gdcr).copy()
+        nb, n = g.shape
+        k = _i(self.keypwv)
+        ia = lib().qxo_sbalinaspc(g, n, nb, self.prveig, self.nprvmx, self.coef, self.kaspc,
+                                  self.ihest, int(nstep), int(nstep_ini), C.byref(k))
+        self.keypwv = k.value
+        return g, ia
 
 
 def kbnlpp_frc2(eqlmr, lmx_atom, vclalp_atom, cgjr, occ, slmir, gx, gy, gz):

@@ -1277,6 +1277,153 @@ void qxo_frcrec(int ntype, const int *nhk1, const int *nhk2, const double *zv, i
   free(dys);
 }
 
+/* roteig2, single rank, PCHOLESKY = 0 (Sources/sbalin.f90:772-1373): the
+ * previous wave functions are turned to the current ones by the orthogonal
+ * factor of their overlap.  A(i,j) = <prev1_i|cur_j> (packed product), B =
+ * A^T A = V diag(ew) V^T (eigenvectors by qxo_eigh), U = V diag(ew)^-1/2 V^T
+ * A^T, then every history set k = 1..min(nprvmx, kpwvx) becomes
+ * prev_k(:, ib) = sum_jb U(ib, jb) prev_k(:, jb).  cur, prev: band-major
+ * (band ib at ib * nplwex), prev sets nband * nplwex apart.  u_out (nband^2,
+ * column-major U) if non-null.  Returns 0, or 1 if B has a non-positive
+ * eigenvalue (then nothing is changed). */
+int qxo_roteig(const double *cur, double *prev, int nplwex, int nband, int nprvmx, int kpwvx,
+               double *u_out)
+{
+  const size_t nb = (size_t)nband, n = (size_t)nplwex;
+  double *A = (double *)malloc(sizeof(double) * nb * nb);
+  double *B = (double *)malloc(sizeof(double) * nb * nb);
+  double *ew = (double *)malloc(sizeof(double) * nb);
+  double *X = (double *)malloc(sizeof(double) * nb * nb);
+  double *U = (double *)malloc(sizeof(double) * nb * nb);
+  double *tmp = (double *)malloc(sizeof(double) * nb * n);
+  size_t i, j, k, m;
+  int key, rc = 0;
+#define M_(a, r, c) (a)[(r) + nb * (c)]
+  for (i = 0; i < nb; ++i)
+    for (j = 0; j < nb; ++j) {
+      const double *x = cur + j * n, *y = prev + i * n;
+      double sum = 0.0;
+      for (m = 0; m < n; ++m) sum = sum + x[m] * y[m];
+      M_(A, i, j) = sum * 2.0 - x[0] * y[0];
+    }
+  for (i = 0; i < nb; ++i)
+    for (j = 0; j <= i; ++j) {
+      double sum = 0.0;
+      for (k = 0; k < nb; ++k) sum = sum + M_(A, k, i) * M_(A, k, j);
+      M_(B, i, j) = sum;
+      M_(B, j, i) = sum;
+    }
+  qxo_eigh(B, nband, nband, ew);
+  for (i = 0; i < nb; ++i)
+    if (!(ew[i] > 0.0)) rc = 1;
+  if (!rc) {
+    /* X = diag(ew)^-1/2 V^T A^T, U = V X */
+    for (i = 0; i < nb; ++i) {
+      const double cosi = 1.0 / sqrt(ew[i]);
+      for (j = 0; j < nb; ++j) {
+        double sum = 0.0;
+        for (k = 0; k < nb; ++k) sum = sum + M_(B, k, i) * M_(A, j, k);
+        M_(X, i, j) = sum * cosi;
+      }
+    }
+    for (i = 0; i < nb; ++i)
+      for (j = 0; j < nb; ++j) {
+        double sum = 0.0;
+        for (k = 0; k < nb; ++k) sum = sum + M_(B, i, k) * M_(X, k, j);
+        M_(U, i, j) = sum;
+      }
+    for (key = 0; key < nprvmx && key < kpwvx; ++key) {
+      double *p = prev + (size_t)key * nb * n;
+      for (i = 0; i < nb; ++i)
+        for (m = 0; m < n; ++m) {
+          double sum = 0.0;
+          for (j = 0; j < nb; ++j) sum = sum + M_(U, i, j) * p[j * n + m];
+          tmp[i * n + m] = sum;
+        }
+      memcpy(p, tmp, sizeof(double) * nb * n);
+    }
+    if (u_out) memcpy(u_out, U, sizeof(double) * nb * nb);
+  }
+#undef M_
+  free(A);
+  free(B);
+  free(ew);
+  free(X);
+  free(U);
+  free(tmp);
+  return rc;
+}
+
+/* set_coef_ASPC (Sources/sbalin.f90:77-170): the predictor coefficients
+ * coefaspc(j, k), j <= k = 2..kaspc, kaspc x kaspc column-major, zero
+ * elsewhere (always-stable predictor-corrector of Kolafa). */
+void qxo_coef_aspc(int kaspc, double *coef)
+{
+  int k, j, i;
+  for (i = 0; i < kaspc * kaspc; ++i) coef[i] = 0.0;
+  for (k = 2; k <= kaspc; ++k)
+    for (j = 1; j <= k; ++j) {
+      double a = 1.0, b = 1.0;
+      for (i = k - j + 1; i <= k - 1; ++i) a = a * (double)i;
+      for (i = k; i <= k + j; ++i) b = b * (double)i;
+      coef[(j - 1) + (size_t)kaspc * (k - 1)] =
+          (double)(((j + 1) % 2) ? -1 : 1) * (double)(j * (2 * k) * (2 * k - 1)) * a / b;
+    }
+}
+
+/* sbalinaspc for one spin, lvand false (Sources/sbalin.f90:369-712): state
+ * keypwv in and out; gdcr (band-major) is replaced by the ASPC prediction
+ * from the current and the aligned previous wave functions, the history
+ * prveig (nprvmx sets) shifted; with ihest > 0 Gram-Schmidt (qxo_schmidt)
+ * follows.  Returns the iaspc used (0 when only the history was started). */
+int qxo_sbalinaspc(double *gdcr, int nplwex, int nband, double *prveig, int nprvmx,
+                   const double *coefaspc, int kaspc, int ihest, int nstep, int nstep_ini,
+                   int *keypwv)
+{
+  const size_t set = (size_t)nplwex * nband;
+  int i, iaspc = 0;
+  if (nstep == nstep_ini + 1 || *keypwv == 0) {
+    *keypwv = 1;
+    memcpy(prveig, gdcr, sizeof(double) * set);
+    return 0;
+  }
+  for (i = 0; i <= nprvmx; ++i)
+    if (ihest == i || nstep == nstep_ini + 1 + i || *keypwv == i) {
+      iaspc = i + 1;
+      break;
+    }
+  *keypwv = (*keypwv + 1 < nprvmx) ? *keypwv + 1 : nprvmx;
+  qxo_roteig(gdcr, prveig, nplwex, nband, nprvmx, (iaspc - 1 > 1) ? iaspc - 1 : 1, NULL);
+  {
+    double *prcd = (double *)malloc(sizeof(double) * (size_t)nplwex);
+    int ib, m, j;
+    for (ib = 0; ib < nband; ++ib) {
+      double *g = gdcr + (size_t)ib * nplwex;
+      if (iaspc > 1) {
+        const double c1 = coefaspc[(size_t)kaspc * (iaspc - 1)];
+        for (m = 0; m < nplwex; ++m) prcd[m] = c1 * g[m];
+        for (j = 2; j <= iaspc; ++j) {
+          const double cj = coefaspc[(j - 1) + (size_t)kaspc * (iaspc - 1)];
+          const double *p = prveig + (size_t)(j - 2) * set + (size_t)ib * nplwex;
+          for (m = 0; m < nplwex; ++m) prcd[m] = prcd[m] + cj * p[m];
+        }
+      }
+      for (j = nprvmx - 1; j >= 1; --j)
+        memcpy(prveig + (size_t)j * set + (size_t)ib * nplwex,
+               prveig + (size_t)(j - 1) * set + (size_t)ib * nplwex, sizeof(double) * nplwex);
+      memcpy(prveig + (size_t)ib * nplwex, g, sizeof(double) * nplwex);
+      if (iaspc > 1) memcpy(g, prcd, sizeof(double) * nplwex);
+    }
+    free(prcd);
+  }
+  if (ihest != 0) {
+    double *s = (double *)malloc(sizeof(double) * (size_t)nband * nband);
+    qxo_schmidt(gdcr, nplwex, nband, s);
+    free(s);
+  }
+  return iaspc;
+}
+
 /* enkbpp_g2 (Sources/nlkbpp.f90:1763-1824), Gamma point. */
 double qxo_enkbpp_g(const double *slmir, int lmmax, int natom, const int *lmx_atom,
                     const double *clalp_atom, const double *occ, int nb, double rvol)

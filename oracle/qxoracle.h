@@ -91,6 +91,12 @@ void qxo_frcrec(int ntype, const int *nhk1, const int *nhk2, const double *zv, i
                 const double *gz, const double *recnrm, const double *recnrmex, double gamma,
                 double volume, const int *kcsma, const double *const *dcq,
                 const double *const *dsq, double *fclm, double *ecolmb, double *atom_ecolmb);
+int qxo_roteig(const double *cur, double *prev, int nplwex, int nband, int nprvmx, int kpwvx,
+               double *u_out);
+void qxo_coef_aspc(int kaspc, double *coef);
+int qxo_sbalinaspc(double *gdcr, int nplwex, int nband, double *prveig, int nprvmx,
+                   const double *coefaspc, int kaspc, int ihest, int nstep, int nstep_ini,
+                   int *keypwv);
 double qxo_enkbpp_g(const double *slmir, int lmmax, int natom, const int *lmx_atom,
                     const double *clalp_atom, const double *occ, int nb, double rvol);
 

@@ -310,3 +310,25 @@ def frcrec(ntype, nhk1, nhk2, zv, nel, natom, iatoit, iatmpt, ew, tables, fclm=N
                    float(ew["gamma"]), float(ew["volume"]), *km, *[_F(t[0].T) for t in tables],
                    *[_F(t[1].T) for t in tables], f, C.byref(e), ae)
     return np.ascontiguousarray(f.T), e.value, ae
+
+
+def roteig(cur, prev, kpwvx):
+    """Reference roteig2; cur (nb, nplwex), prev (nprvmx, nb, nplwex)."""
+    L = lib()
+    c = np.ascontiguousarray(cur, np.float64)
+    p = np.ascontiguousarray(prev, np.float64).copy()
+    nprv, nb, n = p.shape
+    cp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    L.qxref_roteig.argtypes = [cp, cp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.qxref_roteig(c, p, n // 2, nb, nprv, int(kpwvx))
+    return p
+
+
+def coef_aspc(ihest):
+    L = lib()
+    k = max(ihest, 0) + 1
+    out = np.zeros(k * k)
+    cp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    L.qxref_coef_aspc.argtypes = [C.c_int, cp]
+    L.qxref_coef_aspc(int(ihest), out)
+    return out.reshape(k, k).T.copy()

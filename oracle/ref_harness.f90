@@ -626,3 +626,58 @@ subroutine qxref_frcrec(ntype, nhk1, nhk2, zv, nel, natom, nion, iatoit, iatmpt,
   ecolmb = e
   deallocate(dyc, dys)
 end subroutine
+
+! roteig2 (Sources/sbalin.f90:772) on one rank: cur(2 npnod, nband), prev(2
+! npnod, nband, nprvmx) rotated in place
+subroutine qxref_roteig(cur, prev, npnod, nband, nprvmx, kpwvx) bind(C, name="qxref_roteig")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: npnod, nband, nprvmx, kpwvx
+  real(c_double) :: cur(2 * npnod, nband), prev(2 * npnod, nband, nprvmx)
+  integer :: nf(2), nb, nn, np, nbx, ib, nprv, kpw, cnt(1), dsp(1), iw(1)
+  real*8, allocatable :: prod(:), prodr(:), aij(:,:), bij(:,:), bm1(:,:), bx0(:,:)
+  real*8, allocatable :: w1r(:), w1i(:), ew1(:), tmpr(:,:)
+  real*8 :: rv, tc
+  nf = 6
+  nb = nband
+  nn = npnod
+  np = npnod
+  nbx = nband
+  ib = 64
+  nprv = nprvmx
+  kpw = kpwvx
+  cnt(1) = nband
+  dsp(1) = 0
+  iw(1) = 0
+  rv = 1.d0
+  tc = 0.d0
+  allocate(prod(2 * nband + 2 * npnod), prodr(2 * nband + 2 * npnod), aij(nband, nband),  &
+&    bij(nband, nband), bm1(nband, nband), bx0(nband, nband), w1r(2 * nband), w1i(2 * nband),  &
+&    ew1(nband), tmpr(ib, nband))
+  call roteig2(nf, 0, 1, cur, prev, nb, 1, nb, nb, cnt, dsp, 1, np, nn, prod, prodr, .false.,  &
+&    rv, 1, nprv, kpw, aij, bij, bm1, bx0, nbx, w1r, w1i, ew1, tmpr, ib, 1, iw, iw, iw, iw, iw,  &
+&    tc, .false.)
+  deallocate(prod, prodr, aij, bij, bm1, bx0, w1r, w1i, ew1, tmpr)
+end subroutine
+
+! set_coef_ASPC (Sources/sbalin.f90:77): coefaspc of module aspc, kaspc^2
+subroutine qxref_coef_aspc(ihest, coef) bind(C, name="qxref_coef_aspc")
+  use iso_c_binding
+  use aspc
+  implicit none
+  integer(c_int), value :: ihest
+  real(c_double) :: coef(*)
+  integer :: nf(2), ich, ihe, nb, kx
+  real*8 :: corr
+  nf = 6
+  ich = 0
+  ihe = ihest
+  nb = 1
+  kx = 1
+  corr = 0.d0
+  if (allocated(coefaspc)) deallocate(coefaspc)
+  if (allocated(pspmat)) deallocate(pspmat)
+  if (allocated(omegaaspc)) deallocate(omegaaspc)
+  call set_coef_ASPC(nf, 0, 1, ich, ihe, .false., .true., nb, corr, .false., kx)
+  coef(1:kaspc * kaspc) = reshape(coefaspc, (/kaspc * kaspc/))
+end subroutine

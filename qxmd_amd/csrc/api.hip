@@ -101,7 +101,7 @@ int hpsi_batched(Ctx* c, const double* x, double* hx, int nb, int prec) {
 }
 
 // Gram-Schmidt by Cholesky of the packed overlap (schmidt2)
-static int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host) {
+int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host) {
   const int n = c->nplwex;
   double* S = c->d_small;                         // nb x nb overlap, then L
   double* Linv = c->d_small + (size_t)(nb + 1) * nb;  // row-major L^-1
@@ -276,6 +276,7 @@ int qx_destroy(qx_ctx* h) {
   scf_destroy(c);
   lopp_destroy(c);
   ewald_destroy(c);
+  aspc_destroy(c);
   (void)hipStreamDestroy(c->stream);
   delete h;
   return 0;
@@ -802,6 +803,56 @@ int qx_flocal_g_dev(qx_ctx* h, int ntype, const int* nhk1, const int* nhk2, cons
                                 "(nplw5) for these atoms first");
   if (rc == -4) return fail(rc, "qx_flocal_g_dev: call qx_set_lopp first");
   return rc ? fail(rc, "qx_flocal_g_dev: bad ion ranges (%d)", rc) : 0;
+}
+
+// ---------------- wave-function extrapolation -------------------------------
+int qx_aspc_init(qx_ctx* h, int nprvmx, int ihest) {
+  int rc = aspc_init(&h->c, nprvmx, ihest);
+  return rc ? fail(rc, "qx_aspc_init: need 1 <= nprvmx <= 10 and 0 <= ihest <= nprvmx (%d)", rc)
+            : 0;
+}
+
+int qx_sbalinaspc_slot(qx_ctx* h, int slot, int nb, int nstep, int nstep_ini, int* info) {
+  Ctx* c = &h->c;
+  if (slot < 0 || slot > 1) return fail(-1, "qx_sbalinaspc_slot: slot 0 or 1");
+  int iaspc = 0, lchol = 1;
+  int rc = aspc_step(c, c->d_slot[slot], nb, nstep, nstep_ini, &iaspc, &lchol);
+  if (rc == -21)
+    return fail(rc, "qx_sbalinaspc_slot: the previous bands do not overlap the current ones");
+  if (rc) return fail(rc, "qx_sbalinaspc_slot: call qx_aspc_init first, nb <= nbmax (%d)", rc);
+  if (info) {
+    info[0] = iaspc;
+    info[1] = lchol;
+  }
+  return 0;
+}
+
+int qx_roteig_slot(qx_ctx* h, int slot, int nb, int kpwvx) {
+  Ctx* c = &h->c;
+  if (slot < 0 || slot > 1) return fail(-1, "qx_roteig_slot: slot 0 or 1");
+  int rc = aspc_roteig(c, c->d_slot[slot], nb, kpwvx);
+  if (rc == -21) return fail(rc, "qx_roteig_slot: A^T A is not positive definite");
+  if (rc) return fail(rc, "qx_roteig_slot: call qx_aspc_init first (%d)", rc);
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int qx_aspc_get(qx_ctx* h, int k, int nb, double* gdcr) {
+  Ctx* c = &h->c;
+  if (k < 0 || k >= c->aspc.nprvmx || nb < 1 || nb > c->nbmax)
+    return fail(-1, "qx_aspc_get: bad set or band count");
+  return get_reference(c, c->aspc.d_prv[k], gdcr, nb);
+}
+
+int qx_aspc_put(qx_ctx* h, int k, int nb, const double* gdcr, int keypwv) {
+  Ctx* c = &h->c;
+  if (k < 0 || k >= c->aspc.nprvmx || nb < 1 || nb > c->nbmax || keypwv < 0)
+    return fail(-1, "qx_aspc_put: bad set, band count or keypwv");
+  int rc = put_internal(c, c->aspc.d_prv[k], gdcr, nb);
+  if (rc) return rc;
+  c->aspc.keypwv = keypwv;
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 // ---------------- Ewald reciprocal-space forces ------------------------------

@@ -126,6 +126,15 @@ struct Ctx {
     double *d_gv = nullptr, *d_hep = nullptr;   // gx | gy | gz, hep
     double gamma = 0.0, volume = 0.0, recnrmex0 = 0.0;
   } ew;
+  // wave-function extrapolation (sbalinaspc): nprvmx history sets of
+  // nplwex x nbmax in internal order, logical set i at d_prv[i]
+  struct Aspc {
+    static constexpr int kMax = 10;
+    int nprvmx = 0, kaspc = 0, ihest = 0, keypwv = 0;
+    std::vector<double> coef;                 // coefaspc, kaspc x kaspc column-major
+    double* d_prv[kMax] = {};
+    double* d_work = nullptr;                 // 8 nbmax^2 + nbmax + 64
+  } aspc;
   // SCF glue on the dense grid (scfglue.hip): the density sphere, Hartree
   // kernel, fixed potentials, the charge-density vectors and the mixer history
   FullGrid fg;
@@ -229,6 +238,12 @@ int kb_forces(Ctx* c, const double* psi, int nb, const double* occ_host, const d
 int lopp_set(Ctx* c, int ntype, int nplw5, const double* svco, const double* gx,
              const double* gy, const double* gz);
 void lopp_destroy(Ctx* c);
+int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host);
+// wave-function extrapolation between MD steps (aspc.hip)
+void aspc_destroy(Ctx* c);
+int aspc_init(Ctx* c, int nprvmx, int ihest);
+int aspc_roteig(Ctx* c, const double* cur, int nb, int kpwvx);
+int aspc_step(Ctx* c, double* psi, int nb, int nstep, int nstep_ini, int* iaspc, int* lchol);
 // Ewald reciprocal-space forces (ewald.hip)
 void ewald_destroy(Ctx* c);
 int ewald_set(Ctx* c, int numk2, const int* nga, const int* ngb, const int* ngc, const double* gx,

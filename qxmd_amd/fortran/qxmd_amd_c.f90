@@ -382,6 +382,21 @@ integer(c_int) function qx_flocal_g_dev(ctx, ntype, nhk1, nhk2, iatmpt, llking, 
   real(c_double), intent(inout) :: floc(*)
 end function
 
+! ---- wave-function extrapolation
+integer(c_int) function qx_aspc_init(ctx, nprvmx, ihest) bind(C, name="qx_aspc_init")
+  import :: c_int, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: nprvmx, ihest
+end function
+
+integer(c_int) function qx_sbalinaspc_slot(ctx, slot, nb, nstep, nstep_ini, info)  &
+&     bind(C, name="qx_sbalinaspc_slot")
+  import :: c_int, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: slot, nb, nstep, nstep_ini
+  integer(c_int), intent(out) :: info(2)
+end function
+
 ! ---- Ewald reciprocal-space forces
 integer(c_int) function qx_set_ewald(ctx, numk2, nga, ngb, ngc, gx, gy, gz, recnrm, recnrmex,  &
 &     gamma, volume) bind(C, name="qx_set_ewald")

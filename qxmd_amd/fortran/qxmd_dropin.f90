@@ -274,6 +274,33 @@ subroutine qxmd_frcrec(fclm, ntype, nhk1, nhk2, zv, nel, ecolmb, atom_ecolmb, ia
 &      int(kcsma2, c_int), int(kcsma3, c_int), dcq1, dcq2, dcq3, dsq1, dsq2, dsq3), 'qxmd_frcrec')
 end subroutine
 
+! set_coef_ASPC (Sources/sbalin.f90:77) with ichest = 0 and the device
+! history of nprvmx sets; once per run
+subroutine qxmd_aspc_init(nprvmx, ihest)
+  integer, intent(in) :: nprvmx, ihest
+  call qxmd_gpu_need_ctx('qxmd_aspc_init')
+  call qxmd_gpu_check(qx_aspc_init(qxctx, int(nprvmx, c_int), int(ihest, c_int)),  &
+&      'qxmd_aspc_init')
+end subroutine
+
+! sbalinaspc (Sources/sbalin.f90:369-712), one spin, lvand false: gdcr in
+! (current bands), out (the extrapolated bands of the next step); the
+! history stays on the device between calls
+subroutine qxmd_sbalinaspc(gdcr, nplwex, nband, nstep, nstep_ini, iaspc, lcholesky)
+  integer, intent(in) :: nplwex, nband, nstep, nstep_ini
+  real*8, intent(inout) :: gdcr(nplwex, nband)
+  integer, intent(out) :: iaspc
+  logical, intent(out) :: lcholesky
+  integer(c_int) :: info(2)
+  call qxmd_gpu_need_ctx('qxmd_sbalinaspc')
+  call qxmd_gpu_check(qx_slot_put(qxctx, 0_c_int, gdcr, int(nband, c_int)), 'qxmd_sbalinaspc')
+  call qxmd_gpu_check(qx_sbalinaspc_slot(qxctx, 0_c_int, int(nband, c_int), int(nstep, c_int),  &
+&      int(nstep_ini, c_int), info), 'qxmd_sbalinaspc')
+  call qxmd_gpu_check(qx_slot_get(qxctx, 0_c_int, gdcr, int(nband, c_int)), 'qxmd_sbalinaspc')
+  iaspc = info(1)
+  lcholesky = info(2) /= 0
+end subroutine
+
 end module qxmd_amd_gpu
 
 

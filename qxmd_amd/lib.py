@@ -86,6 +86,11 @@ SIGNATURES = {
     "qx_flocal_g": [_vp, _dp, _i, _ip, _ip, _ip, _i, _ip, _i, _ip, _ip, _ip, _dp, _i, _i, _i,
                     _dp, _dp, _dp, _dp, _dp, _dp],
     "qx_flocal_g_dev": [_vp, _i, _ip, _ip, _ip, _ip, _i, _dp],
+    "qx_aspc_init": [_vp, _i, _i],
+    "qx_sbalinaspc_slot": [_vp, _i, _i, _i, _i, _vp],
+    "qx_roteig_slot": [_vp, _i, _i, _i],
+    "qx_aspc_get": [_vp, _i, _i, _dp],
+    "qx_aspc_put": [_vp, _i, _i, _dp, _i],
     "qx_set_ewald": [_vp, _i, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _d, _d],
     "qx_frcrec": [_vp, _dp, _i, _ip, _ip, _dp, _i, _pd, _dp, _ip, _ip, _i, _i, _i, _i, _i, _dp,
                   _dp, _dp, _dp, _dp, _dp],

@@ -338,6 +338,29 @@ class PlaneWave:
                                      i32(llking), int(natom), f), "qx_flocal_g_dev")
         return f
 
+    # ---- wave-function extrapolation (sbalinaspc)
+    def aspc_init(self, nprvmx, ihest):
+        check(load().qx_aspc_init(self.h, int(nprvmx), int(ihest)), "qx_aspc_init")
+
+    def sbalinaspc(self, slot, nb, nstep, nstep_ini=0):
+        """Returns (iaspc, lcholesky)."""
+        info = np.zeros(2, np.int32)
+        check(load().qx_sbalinaspc_slot(self.h, slot, nb, int(nstep), int(nstep_ini),
+                                        info.ctypes.data_as(C.c_void_p)), "qx_sbalinaspc_slot")
+        return int(info[0]), bool(info[1])
+
+    def roteig(self, slot, nb, kpwvx):
+        check(load().qx_roteig_slot(self.h, slot, nb, int(kpwvx)), "qx_roteig_slot")
+
+    def aspc_get(self, k, nb):
+        out = np.zeros((nb, self.gs.nplwex))
+        check(load().qx_aspc_get(self.h, int(k), nb, out), "qx_aspc_get")
+        return out
+
+    def aspc_put(self, k, psi, keypwv):
+        p = f64(psi)
+        check(load().qx_aspc_put(self.h, int(k), p.shape[0], p, int(keypwv)), "qx_aspc_put")
+
     # ---- Ewald reciprocal-space forces (frcrec)
     def set_ewald(self, ew):
         """ew: nga, ngb, ngc, gx, gy, gz, recnrm (m = 1..numk2), recnrmex

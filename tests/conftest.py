@@ -97,6 +97,17 @@ def ewald_case(sp, numk2=400, gamma=0.35, seed=4):
                 iatmpt=iatmpt, ew=ew, tables=tables, kcsma=kcsma, frac=frac)
 
 
+def roteig_case(sp, nb=8, nprv=3, seed=1):
+    """Current orthonormal bands and nprv previous sets near them."""
+    import oracle as O
+    from qxmd_amd.system import initial_wavefunctions
+    cur, _ = O.schmidt(initial_wavefunctions(sp, nb, 25.0))
+    rng = np.random.default_rng(seed)
+    prev = np.stack([O.schmidt(cur + 0.1 * rng.standard_normal(cur.shape))[0]
+                     for _ in range(nprv)])
+    return cur, prev
+
+
 _SPHERES = {}
 
 

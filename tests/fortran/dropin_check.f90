@@ -184,6 +184,9 @@ call check_flocal()
 ! ---- frcrec (Ewald reciprocal part): two ions, four G, against the sums
 call check_frcrec()
 
+! ---- sbalinaspc: bands turning inside one subspace are predicted as they are
+call check_aspc()
+
 call qxmd_gpu_finalize()
 write(*, '(a, es10.2)') 'DROPIN OK worst', worst
 
@@ -523,6 +526,30 @@ subroutine check_frcrec()
   call report('frcrec forces vs sums written out', maxval(abs(fl - fr)) / maxval(abs(fr)), 1.d-12)
   call report('frcrec energies vs sums written out',  &
 &     max(abs(ec - er) / abs(er), maxval(abs(ae - aer)) / maxval(abs(aer))), 1.d-12)
+end subroutine
+
+! five MD steps of the orthonormal cg (from the schmidt check) turned by
+! Givens rotations inside their span: once aligned, every history set equals
+! the current bands and the ASPC prediction returns them
+subroutine check_aspc()
+  real*8 :: g(nplwex, nb), g0(nplwex, nb), th, e2
+  integer :: step, ia_, jb
+  logical :: lc
+  call qxmd_aspc_init(3, 2)
+  e2 = 0.d0
+  do step = 1, 5
+    th = 0.3d0 * step
+    g0 = cg
+    do jb = 1, nb - 1, 2
+      g0(:, jb) = cos(th) * cg(:, jb) - sin(th) * cg(:, jb + 1)
+      g0(:, jb + 1) = sin(th) * cg(:, jb) + cos(th) * cg(:, jb + 1)
+    end do
+    g = g0
+    call qxmd_sbalinaspc(g, nplwex, nb, step, 0, ia_, lc)
+    call expect(lc, 'sbalinaspc lcholesky')
+    if (step > 1) e2 = max(e2, maxval(abs(g - g0)))
+  end do
+  call report('sbalinaspc keeps bands turning in their span', e2, 1.d-12)
 end subroutine
 
 subroutine expect(cond, what)

@@ -140,6 +140,11 @@ def main():
     out["frcrec_fclm"] = fr
     out["frcrec_ecolmb"] = np.array(er)
     out["frcrec_atom_ecolmb"] = aer
+    # roteig2 and set_coef_ASPC (wave-function extrapolation)
+    cur, prev = conftest.roteig_case(si8)
+    out["roteig_prev"] = np.asarray(R.roteig(cur, prev, 2))[:, :, ::SUB]
+    for ih in (1, 2, 3, 4):
+        out[f"coef_aspc_{ih}"] = R.coef_aspc(ih)
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes,", len(out), "arrays")
 

@@ -205,3 +205,14 @@ def test_frcrec_vs_reference(si64):
     fo, eo, ao = O.frcrec(*args)
     fr, er, ar = R.frcrec(*args)
     assert np.array_equal(fo, fr) and eo == er and np.array_equal(ao, ar)
+
+
+def test_roteig_coef_aspc_vs_reference(si8):
+    import conftest
+    cur, prev = conftest.roteig_case(si8, nb=10, nprv=3, seed=3)
+    for kp in (1, 3):
+        out, _, rc = O.roteig(cur, prev, kp)
+        assert rc == 0
+        np.testing.assert_allclose(out, R.roteig(cur, prev, kp), rtol=0, atol=1e-13)
+    for ih in (1, 2, 5):
+        assert np.array_equal(O.coef_aspc(ih + 1), R.coef_aspc(ih))
