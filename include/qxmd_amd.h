@@ -220,6 +220,33 @@ int qx_flocal_g(qx_ctx *ctx, double *floc, int ntype, const int *nhk1, const int
 int qx_flocal_g_dev(qx_ctx *ctx, int ntype, const int *nhk1, const int *nhk2,
                     const int *iatmpt, const int *llking, int natom, double *floc);
 
+/* ---------------- restart files (the reference's own layout) ---------------- */
+/* svwfns (Sources/savedd.f90:2441-2810), one spin, ncscale = 1: QM_eig with
+ * gdcr(nplwex, nband) in the reference's G order, eig, occ; lsreal8 = 0 stores
+ * 16-bit integers with a per-band scale, as the reference does.  Byte-equal to
+ * the reference's file. */
+int qx_svwfns(const char *fname, const double *gdcr, int nplwex, int nplw, int nband,
+              const double *eig, const double *occ, int lsreal8);
+/* rdwfns / rdwfn22 (Sources/savedd.f90:268-1172): the file's nplw must match;
+ * min(nband, file's nband) bands read; *nband_file = the file's band count. */
+int qx_rdwfns(const char *fname, double *gdcr, int nplwex, int nplw, int nband, double *eig,
+              double *occ, int *nband_file);
+/* the same from / into a device slot (restart without host band arrays) */
+int qx_svwfns_slot(qx_ctx *ctx, const char *fname, int slot, int nb, const double *eig,
+                   const double *occ, int lsreal8);
+int qx_rdwfns_slot(qx_ctx *ctx, const char *fname, int slot, int nb, double *eig, double *occ,
+                   int *nband_file);
+/* svacon / rdacon (Sources/savedd.f90:3058-3143, 40-265), file version -2:
+ * QM_ion with nhk1/nhk2(ntype), ratm and frc (3, natom), prevr (3, natom, 3),
+ * hcell (3, 3), column-major; natom = nhk2(ntype).  rdacon checks ntype and
+ * nhk1/nhk2 against the file as the reference does. */
+int qx_svacon(const char *fname, int ntype, const int *nhk1, const int *nhk2, int nstepCG,
+              int nstepMD, int lclust, const double *ratm, int ifmd, const double *frc,
+              const double *prevr, const double *hcell);
+int qx_rdacon(const char *fname, int ntype, const int *nhk1, const int *nhk2, int *nstepCG,
+              int *nstepMD, int *lclust, double *ratm, int *ifmd, double *frc, double *prevr,
+              double *hcell);
+
 /* ---------------- wave-function extrapolation between MD steps ------------- */
 /* set_coef_ASPC with ichest = 0 (Sources/sbalin.f90:77-170): predictor of
  * order ihest (kaspc = ihest + 1), history of nprvmx sets (<= 10) in HBM,

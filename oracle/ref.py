@@ -332,3 +332,34 @@ def coef_aspc(ihest):
     L.qxref_coef_aspc.argtypes = [C.c_int, cp]
     L.qxref_coef_aspc(int(ihest), out)
     return out.reshape(k, k).T.copy()
+
+
+def svwfns(fname, gdcr, eig, occ, lsreal8):
+    """Write QM_eig with the reference's svwfns; gdcr (nband, nplwex)."""
+    L = lib()
+    g = np.ascontiguousarray(gdcr, np.float64)
+    nb, n = g.shape
+    fb = fname.encode()
+    cp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    L.qxref_svwfns.argtypes = [C.c_char_p, C.c_int, cp, C.c_int, C.c_int, C.c_int, cp, cp, C.c_int]
+    L.qxref_svwfns(fb, len(fb), g, n, n // 2 - 1, nb, np.ascontiguousarray(eig, np.float64),
+                   np.ascontiguousarray(occ, np.float64), int(bool(lsreal8)))
+
+
+def svacon(fname, nhk1, nhk2, nstepcg, nstepmd, lclust, ratm, ifmd, frc, prevr, hcell):
+    """Write QM_ion with the reference's svacon; ratm/frc (natom, 3), prevr
+    (3, natom, 3) as numpy (j, i, x), hcell (3, 3) with columns the lattice
+    vectors."""
+    L = lib()
+    fb = fname.encode()
+    ntype = len(nhk1)
+    natom = int(nhk2[-1])
+    ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    cp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    L.qxref_svacon.argtypes = [C.c_char_p, C.c_int, C.c_int, ip, ip, C.c_int, C.c_int, C.c_int,
+                               cp, C.c_int, cp, cp, cp, C.c_int]
+    L.qxref_svacon(fb, len(fb), ntype, np.ascontiguousarray(nhk1, np.int32),
+                   np.ascontiguousarray(nhk2, np.int32), int(nstepcg), int(nstepmd),
+                   int(bool(lclust)), np.ascontiguousarray(ratm, np.float64), int(ifmd),
+                   np.ascontiguousarray(frc, np.float64), np.ascontiguousarray(prevr, np.float64),
+                   np.ascontiguousarray(np.asarray(hcell).T, np.float64), natom)

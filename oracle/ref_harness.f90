@@ -681,3 +681,85 @@ subroutine qxref_coef_aspc(ihest, coef) bind(C, name="qxref_coef_aspc")
   call set_coef_ASPC(nf, 0, 1, ich, ihe, .false., .true., nb, corr, .false., kx)
   coef(1:kaspc * kaspc) = reshape(coefaspc, (/kaspc * kaspc/))
 end subroutine
+
+! one-rank communicator state for the reference's file writers
+subroutine qxref_serial_world()
+  use communicator
+  implicit none
+  myid = 0
+  nodes = 1
+  myid_qm = 0
+  nodes_qm = 1
+  myid_qm_un = 0
+  nodes_qm_un = 1
+  myid_kd = 0
+  nodes_kd = 1
+  nkd = 0
+  myid_lr = 0
+  nodes_lr = 1
+  myid_pw = 0
+  nodes_pw = 1
+end subroutine
+
+! svwfns (Sources/savedd.f90:2441) for one spin, ncscale = 1: writes gdcr
+subroutine qxref_svwfns(fname, nlen, gdcr, nplwex, nplw, nband, eig, occ, lsr)  &
+&    bind(C, name="qxref_svwfns")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: nlen, nplwex, nplw, nband, lsr
+  character(kind=c_char) :: fname(nlen)
+  real(c_double) :: gdcr(nplwex, nband), eig(nband), occ(nband)
+  character(len=256) :: fn
+  integer :: nf(2), i, nb, nex, np, cnt(1), dsp(1), nsp, nspn
+  real*8, allocatable :: work(:,:), dummy(:)
+  logical :: l8
+  real*8 :: wv(2)
+  fn = ' '
+  do i = 1, nlen
+    fn(i:i) = fname(i)
+  end do
+  call qxref_serial_world()
+  l8 = lsr /= 0
+  nf = 6
+  call set_lsreal8(nf, 0, 1, l8)
+  nb = nband
+  nex = nplwex
+  np = nplw
+  cnt(1) = nband
+  dsp(1) = 0
+  nsp = 1
+  nspn = 1
+  allocate(work(nplwex, nband), dummy(1))
+  call svwfns(nf, 1, trim(fn), work, nex, np, nb, 1, nb, nb, cnt, dsp, eig, dummy, occ, dummy,  &
+&    gdcr, l8, .false., nspn, dummy, nsp, .false., 1, wv)
+  deallocate(work, dummy)
+end subroutine
+
+! svacon (Sources/savedd.f90:3058), file version -2
+subroutine qxref_svacon(fname, nlen, ntype, nhk1, nhk2, nstepcg, nstepmd, lclust, ratm, ifmd,  &
+&    frc, prevr, hcell, natom) bind(C, name="qxref_svacon")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: nlen, ntype, nstepcg, nstepmd, lclust, ifmd, natom
+  character(kind=c_char) :: fname(nlen)
+  integer(c_int) :: nhk1(ntype), nhk2(ntype)
+  real(c_double) :: ratm(3, natom), frc(3, natom), prevr(3, natom, 3), hcell(3, 3)
+  character(len=256) :: fn
+  integer :: nf(2), i, nt, na, ncg, nmd, imd, h1(ntype), h2(ntype)
+  logical :: lc
+  fn = ' '
+  do i = 1, nlen
+    fn(i:i) = fname(i)
+  end do
+  call qxref_serial_world()
+  nf = 6
+  nt = ntype
+  na = natom
+  ncg = nstepcg
+  nmd = nstepmd
+  imd = ifmd
+  h1 = nhk1
+  h2 = nhk2
+  lc = lclust /= 0
+  call svacon(nf, 1, trim(fn), prevr, imd, ncg, nmd, ratm, frc, nt, h1, h2, lc, na, hcell)
+end subroutine

@@ -805,6 +805,35 @@ int qx_flocal_g_dev(qx_ctx* h, int ntype, const int* nhk1, const int* nhk2, cons
   return rc ? fail(rc, "qx_flocal_g_dev: bad ion ranges (%d)", rc) : 0;
 }
 
+// ---------------- restart files ---------------------------------------------
+int qx_svwfns_slot(qx_ctx* h, const char* fname, int slot, int nb, const double* eig,
+                   const double* occ, int lsreal8) {
+  Ctx* c = &h->c;
+  if (slot < 0 || slot > 4 || nb < 1 || nb > c->nbmax)
+    return fail(-1, "qx_svwfns_slot: bad slot or band count");
+  std::vector<double> g((size_t)c->nplwex * nb);
+  int rc = get_reference(c, c->d_slot[slot], g.data(), nb);
+  if (rc) return rc;
+  rc = qx_svwfns(fname, g.data(), c->nplwex, c->nplw, nb, eig, occ, lsreal8);
+  return rc ? fail(rc, "qx_svwfns_slot: cannot write %s", fname ? fname : "") : 0;
+}
+
+int qx_rdwfns_slot(qx_ctx* h, const char* fname, int slot, int nb, double* eig, double* occ,
+                   int* nband_file) {
+  Ctx* c = &h->c;
+  if (slot < 0 || slot > 1 || nb < 1 || nb > c->nbmax)
+    return fail(-1, "qx_rdwfns_slot: slot 0 or 1, nb <= nbmax");
+  std::vector<double> g((size_t)c->nplwex * nb, 0.0);
+  int nf = 0;
+  int rc = qx_rdwfns(fname, g.data(), c->nplwex, c->nplw, nb, eig, occ, &nf);
+  if (rc == -2) return fail(rc, "qx_rdwfns_slot: %s holds another G sphere", fname);
+  if (rc) return fail(rc, "qx_rdwfns_slot: cannot read %s", fname ? fname : "");
+  if ((rc = put_internal(c, c->d_slot[slot], g.data(), nb))) return rc;
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  if (nband_file) *nband_file = nf;
+  return 0;
+}
+
 // ---------------- wave-function extrapolation -------------------------------
 int qx_aspc_init(qx_ctx* h, int nprvmx, int ihest) {
   int rc = aspc_init(&h->c, nprvmx, ihest);

@@ -86,6 +86,12 @@ SIGNATURES = {
     "qx_flocal_g": [_vp, _dp, _i, _ip, _ip, _ip, _i, _ip, _i, _ip, _ip, _ip, _dp, _i, _i, _i,
                     _dp, _dp, _dp, _dp, _dp, _dp],
     "qx_flocal_g_dev": [_vp, _i, _ip, _ip, _ip, _ip, _i, _dp],
+    "qx_svwfns": [C.c_char_p, _dp, _i, _i, _i, _dp, _dp, _i],
+    "qx_rdwfns": [C.c_char_p, _dp, _i, _i, _i, _dp, _dp, _pi],
+    "qx_svwfns_slot": [_vp, C.c_char_p, _i, _i, _dp, _dp, _i],
+    "qx_rdwfns_slot": [_vp, C.c_char_p, _i, _i, _dp, _dp, _pi],
+    "qx_svacon": [C.c_char_p, _i, _ip, _ip, _i, _i, _i, _dp, _i, _dp, _dp, _dp],
+    "qx_rdacon": [C.c_char_p, _i, _ip, _ip, _pi, _pi, _pi, _dp, _pi, _dp, _dp, _dp],
     "qx_aspc_init": [_vp, _i, _i],
     "qx_sbalinaspc_slot": [_vp, _i, _i, _i, _i, _vp],
     "qx_roteig_slot": [_vp, _i, _i, _i],

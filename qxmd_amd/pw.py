@@ -338,6 +338,19 @@ class PlaneWave:
                                      i32(llking), int(natom), f), "qx_flocal_g_dev")
         return f
 
+    # ---- restart files of a slot (QM_eig)
+    def svwfns(self, fname, slot, nb, eig, occ, lsreal8=True):
+        check(load().qx_svwfns_slot(self.h, str(fname).encode(), slot, nb, f64(eig), f64(occ),
+                                    int(bool(lsreal8))), "qx_svwfns_slot")
+
+    def rdwfns(self, fname, slot, nb):
+        """Returns (eig, occ, nband in the file)."""
+        eig, occ = np.zeros(nb), np.zeros(nb)
+        nf = C.c_int(0)
+        check(load().qx_rdwfns_slot(self.h, str(fname).encode(), slot, nb, eig, occ,
+                                    C.byref(nf)), "qx_rdwfns_slot")
+        return eig, occ, nf.value
+
     # ---- wave-function extrapolation (sbalinaspc)
     def aspc_init(self, nprvmx, ihest):
         check(load().qx_aspc_init(self.h, int(nprvmx), int(ihest)), "qx_aspc_init")

@@ -108,6 +108,23 @@ def roteig_case(sp, nb=8, nprv=3, seed=1):
     return cur, prev
 
 
+def restart_case(sp, nb=5):
+    """Bands, eigenvalues and occupations for QM_eig, and an atom
+    configuration for QM_ion (two types over 8 atoms, a sheared cell)."""
+    from qxmd_amd.system import initial_wavefunctions
+    g = initial_wavefunctions(sp, nb, 25.0)
+    eig = np.linspace(-0.5, 0.3, nb)
+    occ = np.array([2.0, 2.0, 2.0, 1.5, 0.0][:nb])
+    rng = np.random.default_rng(0)
+    natom = 8
+    h = np.diag([10.26, 11.0, 12.5])
+    h[0, 1] = 0.3
+    ion = dict(nhk1=[1, 5], nhk2=[4, 8], nstepcg=3, nstepmd=17, lclust=False,
+               ratm=rng.uniform(size=(natom, 3)), ifmd=1, frc=rng.standard_normal((natom, 3)),
+               prevr=rng.standard_normal((3, natom, 3)), hcell=h)
+    return g, eig, occ, ion
+
+
 _SPHERES = {}
 
 

@@ -145,6 +145,30 @@ def main():
     out["roteig_prev"] = np.asarray(R.roteig(cur, prev, 2))[:, :, ::SUB]
     for ih in (1, 2, 3, 4):
         out[f"coef_aspc_{ih}"] = R.coef_aspc(ih)
+    # restart files written by the reference (svwfns both encodings, svacon):
+    # their SHA-256 digests
+    import tempfile
+    g, eig, occ, ion = conftest.restart_case(si8)
+    with tempfile.TemporaryDirectory() as d:
+        for l8 in (True, False):
+            f = os.path.join(d, "QM_eig")
+            R.svwfns(f, g, eig, occ, l8)
+            out[f"qm_eig_sha_{int(l8)}"] = np.array(sha(np.frombuffer(open(f, "rb").read(),
+                                                                   np.uint8)))
+        f = os.path.join(d, "QM_ion")
+        R.svacon(f, ion["nhk1"], ion["nhk2"], ion["nstepcg"], ion["nstepmd"], ion["lclust"],
+                 ion["ratm"], ion["ifmd"], ion["frc"], ion["prevr"], ion["hcell"])
+        out["qm_ion_sha"] = np.array(sha(np.frombuffer(open(f, "rb").read(), np.uint8)))
+    # flang's formatting of the text outputs' edit descriptors
+    import shutil
+    import subprocess
+    fc = shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang"
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "ff")
+        subprocess.run([fc, os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                         "fortran_formats.f90"), "-o", exe], check=True)
+        lines = subprocess.run([exe], stdout=subprocess.PIPE, text=True, check=True).stdout
+    out["fortran_format_lines"] = np.array(lines.splitlines())
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes,", len(out), "arrays")
 

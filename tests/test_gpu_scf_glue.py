@@ -186,3 +186,23 @@ def test_outchg_uses_the_context_density(si8):
     ctx.density(0, nb, occ, si8.cell.volume)
     ctx.outchg()
     assert np.array_equal(ctx.scf_get("rhgr"), a)
+
+
+def test_restart_files_from_slots(si8, tmp_path):
+    """QM_eig written from a device slot equals the host writer's file for the
+    same bands; read back into another slot, the bands are the same."""
+    import conftest
+    from qxmd_amd import restart
+    g, eig, occ, _ = conftest.restart_case(si8)
+    ctx = pw.PlaneWave(pw.GSphere(si8.cell.hcell), 5)
+    ctx.put(0, g)
+    for l8 in (True, False):
+        a, b = str(tmp_path / f"dev{l8}"), str(tmp_path / f"host{l8}")
+        ctx.svwfns(a, 0, 5, eig, occ, l8)
+        restart.svwfns(b, g, eig, occ, l8)
+        assert open(a, "rb").read() == open(b, "rb").read()
+        e2, o2, nf = ctx.rdwfns(a, 1, 5)
+        assert nf == 5 and np.array_equal(e2, eig) and np.array_equal(o2, occ)
+        back = ctx.get(1, 5)
+        host, _, _, _ = restart.rdwfns(a, si8.nplw, 5)
+        assert np.array_equal(back, host)

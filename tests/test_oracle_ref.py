@@ -216,3 +216,21 @@ def test_roteig_coef_aspc_vs_reference(si8):
         np.testing.assert_allclose(out, R.roteig(cur, prev, kp), rtol=0, atol=1e-13)
     for ih in (1, 2, 5):
         assert np.array_equal(O.coef_aspc(ih + 1), R.coef_aspc(ih))
+
+
+@pytest.mark.parametrize("lsreal8", [True, False])
+def test_restart_files_vs_reference(si64, tmp_path, lsreal8):
+    import conftest
+    from qxmd_amd import restart
+    from qxmd_amd.system import initial_wavefunctions
+    g = initial_wavefunctions(si64, 7, 3.0)
+    eig, occ = np.linspace(-1, 1, 7), np.full(7, 2.0)
+    a, b = str(tmp_path / "a"), str(tmp_path / "b")
+    restart.svwfns(a, g, eig, occ, lsreal8)
+    R.svwfns(b, g, eig, occ, lsreal8)
+    assert open(a, "rb").read() == open(b, "rb").read()
+    _, _, _, ion = conftest.restart_case(si64)
+    ion = dict(ion, lclust=True, ifmd=2)
+    restart.svacon(a, **ion)
+    R.svacon(b, **ion)
+    assert open(a, "rb").read() == open(b, "rb").read()
