@@ -225,7 +225,20 @@ __global__ void __launch_bounds__(256) k_gemv_t(int M, int K, double alpha,
   double acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-  for (int k = threadIdx.x; k < K; k += 256) {
+  // eight independent loads of A per thread in flight: the row is streamed
+  // from HBM, B (shared by every row) comes from L2
+  constexpr int U = 8;
+  int k = threadIdx.x;
+  for (; k + (U - 1) * 256 < K; k += U * 256) {
+    double av[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) av[u] = a[k + u * 256];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) acc[v] += av[u] * B[(size_t)v * ldb + k + u * 256];
+  }
+  for (; k < K; k += 256) {
     const double av = a[k];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] += av * B[(size_t)v * ldb + k];
@@ -260,7 +273,18 @@ __global__ void __launch_bounds__(256) k_gemv_n(int M, int K, int kchunk, double
   double acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-  for (int k = k0; k < k1; ++k) {
+  constexpr int U = 8;
+  int k = k0;
+  for (; k + U <= k1; k += U) {
+    double av[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) av[u] = A[(size_t)(k + u) * lda + m];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) acc[v] += av[u] * B[(size_t)v * ldb + k + u];
+  }
+  for (; k < k1; ++k) {
     const double av = A[(size_t)k * lda + m];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] += av * B[(size_t)v * ldb + k];
