@@ -409,10 +409,14 @@ __global__ void __launch_bounds__(kFastThreads) k_zfwd_f(FastArgs a, const doubl
 // Same plane layout and lane maps as k_plane_f.  A block (z plane, group of
 // band pairs) keeps its share of the plane's density in registers over all its
 // band pairs (RG row groups per wavefront at most) and writes it once.
-constexpr int kRhoGroups = 4;  // largest RG instantiated
+constexpr int kRhoGroups = 6;  // largest RG instantiated (2, 4 and 6 are)
+
+// more than four row groups run with at most four wavefronts per block, which
+// leaves each lane the register file's full width for the accumulators
+constexpr int rho_max_threads(int RG) { return RG > 4 ? 256 : kPlaneMaxThreads; }
 
 template <typename T, int NA, int NB, int RG>
-__global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_rho_f(FastArgs a,
+__global__ void __launch_bounds__(rho_max_threads(RG)) k_plane_rho_f(FastArgs a,
                                                                 const cplx<T>* __restrict__ t1,
                                                                 int nbp_batch, int bp_per_group,
                                                                 const double* __restrict__ w,
@@ -784,9 +788,13 @@ static int density_fast_nn(Ctx* c, const double* psi, int nb, const double* w) {
   const unsigned gz = (unsigned)((c->pc.npair + PB - 1) / PB);
   const size_t lp = plane_lds<T>(N, c->pc.nH, sa, sr, RW, nwave);
   if (lp > kCuLds) return -3;
-  const bool rg2 = nwave * 2 >= (N + RW - 1) / RW;
-  const void* kfn = rg2 ? reinterpret_cast<const void*>(k_plane_rho_f<T, NA, NB, 2>)
-                        : reinterpret_cast<const void*>(k_plane_rho_f<T, NA, NB, kRhoGroups>);
+  // the fewest row groups per wavefront that cover the plane's rows
+  const int ng_rows = (N + RW - 1) / RW;
+  const int rg = nwave * 2 >= ng_rows ? 2 : nwave * 4 >= ng_rows ? 4 : kRhoGroups;
+  if (64 * nwave > rho_max_threads(rg)) return -3;
+  const void* kfn = rg == 2   ? reinterpret_cast<const void*>(k_plane_rho_f<T, NA, NB, 2>)
+                    : rg == 4 ? reinterpret_cast<const void*>(k_plane_rho_f<T, NA, NB, 4>)
+                              : reinterpret_cast<const void*>(k_plane_rho_f<T, NA, NB, kRhoGroups>);
   if (lp > 65536)
     (void)hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lp);
   cplx<T>* t1 = reinterpret_cast<cplx<T>*>(c->d_t1);
@@ -804,12 +812,16 @@ static int density_fast_nn(Ctx* c, const double* psi, int nb, const double* w) {
     int ng = std::min(groups_per_batch, nbb);
     int per = (nbb + ng - 1) / ng;
     ng = (nbb + per - 1) / per;
-    if (rg2)
+    double* part = c->d_rho_part + (size_t)nparts * c->N;
+    if (rg == 2)
       hipLaunchKernelGGL((k_plane_rho_f<T, NA, NB, 2>), dim3(N, ng), dim3(64 * nwave), lp,
-                         c->stream, a, t1, nbb, per, w, c->d_rho_part + (size_t)nparts * c->N);
+                         c->stream, a, t1, nbb, per, w, part);
+    else if (rg == 4)
+      hipLaunchKernelGGL((k_plane_rho_f<T, NA, NB, 4>), dim3(N, ng), dim3(64 * nwave), lp,
+                         c->stream, a, t1, nbb, per, w, part);
     else
       hipLaunchKernelGGL((k_plane_rho_f<T, NA, NB, kRhoGroups>), dim3(N, ng), dim3(64 * nwave), lp,
-                         c->stream, a, t1, nbb, per, w, c->d_rho_part + (size_t)nparts * c->N);
+                         c->stream, a, t1, nbb, per, w, part);
     nparts += ng;
   }
   hipLaunchKernelGGL(k_rho_reduce_f, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream,

@@ -326,6 +326,26 @@ def test_fast_path_si216_sample(prec):
     assert _relerr(ctx.get(1, nb), H.hpsi(psi)) < tol
 
 
+@pytest.mark.parametrize("prec", [pw.FP64, pw.MIXED])
+def test_fast_path_si512_sample(prec):
+    """The 120^3 two-step (12 x 10) path on the 512-atom cell, a few bands:
+    the FP64 density there keeps six row groups per wavefront."""
+    import conftest
+    sph = conftest.Sphere(4)
+    assert sph.nd == (120, 120, 120)
+    nb = 7
+    gs, ctx, H, v = _setup(sph, nb, kb=False)
+    psi, _ = O.schmidt(initial_wavefunctions(sph, nb))
+    ctx.put(0, psi)
+    ctx.local(0, 1, nb, prec=prec)
+    tol = 1e-12 if prec == pw.FP64 else 3e-6
+    assert _relerr(ctx.get(1, nb), H.hpsi(psi)) < tol
+    occ = occupations(nb, 12)
+    rho = ctx.density(0, nb, occ, sph.cell.volume, prec=prec)
+    ref = O.softchg(psi, sph.nplw, occ, sph.cell.volume, sph.ijk, sph.nd)
+    assert _relerr(rho, ref) < tol
+
+
 def test_hpsi_mixed_si64(si64):
     """Mixed form: FP32 pencil FFT and FP32-MFMA KB contractions against the
     FP64 oracle."""
