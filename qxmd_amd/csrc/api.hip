@@ -248,6 +248,11 @@ int qx_create(qx_ctx** out, int n1, int n2, int n3, int nplw, const int* nga, co
   QX_CHECK(hipMemset(c->d_sync, 0, sizeof(int) * 16));
   c->dwork_len = 8 * (size_t)c->nbmax * c->nbmax + 16 * (size_t)c->nbmax + 64;
   QX_CHECK(hipMalloc(&c->d_dwork, sizeof(double) * c->dwork_len));
+  c->bb_nch = c->nplwex / 4096 + 1;
+  QX_CHECK(hipMalloc(&c->d_bb, sizeof(double) * (32 + 5 * (size_t)(c->bb_nch + 1))));
+  QX_CHECK(hipMemset(c->d_bb, 0, sizeof(double) * (32 + 5 * (size_t)(c->bb_nch + 1))));
+  QX_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_flag), sizeof(int) * 4, hipHostMallocDefault));
+  QX_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_hflag), c->h_flag, 0));
   for (auto& e : c->ev) QX_CHECK(hipEventCreate(&e));
   for (auto& e : c->pev) QX_CHECK(hipEventCreate(&e));
   *out = h;
@@ -260,9 +265,10 @@ int qx_destroy(qx_ctx* h) {
   (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {c->d_perm, c->d_pt_col, c->d_pt_z, c->d_pair_p0, c->d_col_hi, c->d_col_kmod,
                   c->d_kin, c->d_tw, c->d_V, c->d_Vf, c->d_Vp, c->d_Vpf, c->d_B, c->d_D, c->d_t1, c->d_t2, c->d_P,
-                  c->d_small, c->d_rho_part, c->d_rho, c->d_scal, c->d_sync, c->d_dwork};
+                  c->d_small, c->d_rho_part, c->d_rho, c->d_scal, c->d_sync, c->d_dwork, c->d_bb};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (c->h_flag) (void)hipHostFree(c->h_flag);
   for (auto* p : c->d_slot)
     if (p) (void)hipFree(p);
   for (auto& e : c->ev)

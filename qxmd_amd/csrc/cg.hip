@@ -219,15 +219,6 @@ static void axpby(Ctx* c, double* out, const double* x, const double* y, const d
                      cc, sa, sc, n, n, nb, mask);
 }
 
-static double pdot1(Ctx* c, const double* x, const double* y) {
-  double* d = c->d_scal + 60 * (size_t)c->nbmax;
-  pdot_batched(c, x, y, 1, d);
-  double h = 0.0;
-  (void)hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, c->stream);
-  (void)hipStreamSynchronize(c->stream);
-  return h;
-}
-
 // H h for one vector (kinetic + KB + local), order of Sources/eigen.f90:516-551
 static int hpsi1(Ctx* c, const double* x, double* hx, int prec) {
   int rc = local_apply(c, x, hx, 1, prec, 0);
@@ -235,95 +226,267 @@ static int hpsi1(Ctx* c, const double* x, double* hx, int prec) {
   return kb_apply(c, x, hx, 1, 1, nullptr, prec);
 }
 
+// ---- band-by-band schedule: the scalars of the band being updated stay on
+// the device; every inner product is a chunked pass whose partial sums the
+// next kernel finishes itself, and the host waits once per CG iteration for
+// the convergence flag.  part[0..nch) are chunk sums, part[nch] is the
+// x[0] y[0] correction of the packed product (stored by the pass, so a later
+// pass may overwrite either vector).
+struct BbScal {
+  double eigv, zansa, zansab, z000re, x000re, eigb, dinnerx, betk, zansap, alpsq, dtnew;
+  int iter, izcnt, lreset, active;
+};
+
+__device__ inline double bb_total(const double* part, int nch, int nq = 1, int q = 0) {
+  double s = 0.0;
+  for (int ch = 0; ch < nch; ++ch) s += part[ch * nq + q];
+  return s;
+}
+
+__global__ void __launch_bounds__(kDotThreads) k_bb_dot(const double* __restrict__ x,
+                                                        const double* __restrict__ y, int n,
+                                                        int nch, double* __restrict__ part) {
+  const int ch = blockIdx.x;
+  const int i0 = ch * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  double s[1] = {0.0};
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) s[0] += x[i] * y[i];
+  dot_store<1>(s, part, 0, ch, nch);
+  if (ch == 0 && threadIdx.x == 0) part[nch] = x[0] * y[0];
+}
+
+// eigv = <x|Hx>; g = eigv x - Hx; chunk sums of g.g (Sources/eigen.f90:380-400)
+__global__ void __launch_bounds__(kDotThreads) k_bb_start(const double* __restrict__ x,
+                                                          const double* __restrict__ hx,
+                                                          double* __restrict__ g, int n, int nch,
+                                                          const double* __restrict__ pin,
+                                                          BbScal* S, double* __restrict__ pout) {
+  const double eigv = 2.0 * bb_total(pin, nch) - pin[nch];
+  const int ch = blockIdx.x;
+  const int i0 = ch * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  double s[1] = {0.0};
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const double gv = eigv * x[i] + -1.0 * hx[i];
+    g[i] = gv;
+    s[0] += gv * gv;
+  }
+  dot_store<1>(s, pout, 0, ch, nch);
+  if (ch == 0 && threadIdx.x == 0) {
+    S->eigv = eigv;
+    pout[nch] = g[0] * g[0];
+  }
+}
+
+__global__ void k_bb_init(BbScal* S, const double* __restrict__ part, int nch) {
+  if (threadIdx.x) return;
+  const double zansa = 2.0 * bb_total(part, nch) - part[nch];
+  S->zansa = S->zansab = S->z000re = zansa;
+  S->x000re = 1e10;
+  S->eigb = S->eigv;
+  S->dinnerx = 0.0;
+  S->betk = 0.0;
+  S->iter = 0;
+  S->izcnt = 0;
+  S->lreset = 1;
+  S->active = 1;
+}
+
+// CG direction h = g + beta h, beta = zansap / betk (h = g after a reset)
+// from the chunk sums of the preconditioned g.g (Sources/eigen.f90:440-452)
+__global__ void __launch_bounds__(kDotThreads) k_bb_dir(const double* __restrict__ g,
+                                                        double* __restrict__ h, int n, int nch,
+                                                        const double* __restrict__ part,
+                                                        BbScal* S) {
+  const double zansap = 2.0 * bb_total(part, nch) - g[0] * g[0];
+  const bool reset = S->lreset != 0;
+  const double beta = reset ? 0.0 : zansap / S->betk;
+  const int i0 = blockIdx.x * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) h[i] = reset ? g[i] : g[i] + beta * h[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) S->zansap = zansap;
+}
+
+// subgs4: h <- h - <x|h> x; chunk sums of h.Hx for cai (Sources/engrad.f90:1232-1239,
+// eigen.f90:486-491)
+__global__ void __launch_bounds__(kDotThreads) k_bb_orth(const double* __restrict__ x,
+                                                         double* __restrict__ h,
+                                                         const double* __restrict__ hx, int n,
+                                                         int nch, const double* __restrict__ pin,
+                                                         double* __restrict__ pout) {
+  const double hhx = 2.0 * bb_total(pin, nch) - pin[nch];
+  const int ch = blockIdx.x;
+  const int i0 = ch * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  double s[1] = {0.0};
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const double hv = h[i] - hhx * x[i];
+    h[i] = hv;
+    s[0] += hv * hx[i];
+  }
+  dot_store<1>(s, pout, 0, ch, nch);
+  if (ch == 0 && threadIdx.x == 0) pout[nch] = h[0] * hx[0];
+}
+
+// the 2x2 step of BKL, or the line minimisation when h.h <= 0
+// (Sources/eigen.f90:575-693); pc: cai, pd: h.Hh | h.x | h.h
+__global__ void k_bb_step(BbScal* S, const double* __restrict__ pc, const double* __restrict__ pd,
+                          const double* __restrict__ h, const double* __restrict__ hh,
+                          const double* __restrict__ x, int nch, double threinn, double wdinner) {
+  if (threadIdx.x) return;
+  const double cai = 2.0 * bb_total(pc, nch) - pc[nch];
+  const double cbi = 2.0 * bb_total(pd, nch, 3, 0) - h[0] * hh[0];
+  const double hhx = 2.0 * bb_total(pd, nch, 3, 1) - h[0] * x[0];
+  const double hhi = 2.0 * bb_total(pd, nch, 3, 2) - h[0] * h[0];
+  double eigv = S->eigv, dtnew, alpsq;
+  if (hhi > 0.0) {
+    const double h11 = eigv, h12 = cai / sqrt(hhi), h21 = h12, h22 = cbi / hhi;
+    const double enew =
+        0.5 * (h11 + h22 - sqrt((h11 + h22) * (h11 + h22) - 4.0 * (h11 * h22 - h12 * h21)));
+    dtnew = (enew - h11) / h12 / sqrt(hhi);
+    alpsq = sqrt(1.0 / (1.0 + dtnew * (0.0 + dtnew * hhi)));
+    eigv = enew;
+  } else {
+    const double c0 = eigv, c1 = cai, c2 = cbi, c3 = hhi, c4 = 2.0 * hhx;
+    const double s0 = c0 * c4 - c1, s1 = c0 * c3 - c2, s2 = c1 * c3 - c2 * c4;
+    const double sr = s1 * s1 - s2 * s0;
+    dtnew = sr > 0.0 ? (-s1 - sqrt(sr)) / s2 : -1.0;
+    if (dtnew > 0.0) {
+      const double dinner = sqrt(1.0 / (1.0 + dtnew * dtnew * hhi));
+      if (dinner < threinn) dtnew = 0.0;
+      if (dinner < S->dinnerx * wdinner) dtnew = 0.0;
+      S->dinnerx = fmax(S->dinnerx, dinner);
+    }
+    const double dt2 = dtnew * dtnew, alpsq2 = 1.0 / (1.0 + dtnew * c4 + dt2 * c3);
+    eigv = (c0 + dtnew * c1 + dt2 * c2) * alpsq2;
+    alpsq = sqrt(alpsq2);
+  }
+  S->eigv = eigv;
+  S->dtnew = dtnew;
+  S->alpsq = alpsq;
+  S->iter += 1;
+  S->lreset = 0;
+  S->betk = S->zansap;
+}
+
+// x <- alpsq (x + dt h), Hx <- alpsq (Hx + dt Hh), g <- eig x - Hx; chunk
+// sums of g.g.  hh_g holds Hh on entry and the new g on exit.
+__global__ void __launch_bounds__(kDotThreads) k_bb_update(double* __restrict__ x,
+                                                           double* __restrict__ hx,
+                                                           double* hh_g,
+                                                           const double* __restrict__ h, int n,
+                                                           int nch, const BbScal* S,
+                                                           double* __restrict__ pout) {
+  const double a = S->alpsq, dt = S->dtnew, e = S->eigv;
+  const int ch = blockIdx.x;
+  const int i0 = ch * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  double s[1] = {0.0};
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const double xv = a * (x[i] + dt * h[i]);
+    const double hv = a * (hx[i] + dt * hh_g[i]);
+    const double gv = e * xv - hv;
+    x[i] = xv;
+    hx[i] = hv;
+    hh_g[i] = gv;
+    s[0] += gv * gv;
+  }
+  dot_store<1>(s, pout, 0, ch, nch);
+  if (ch == 0 && threadIdx.x == 0) pout[nch] = hh_g[0] * hh_g[0];
+}
+
+// convergence test (Sources/eigen.f90:755-778); the band's results go to
+// eig_out / bz_out / it_out and the flag the host waits for to *flag
+__global__ void k_bb_conv(BbScal* S, const double* __restrict__ part, int nch, int ib, int noband,
+                          int itermx, int iteremx, double tolera, double roughfzan,
+                          double roughfeig, double* eig_out, double* bz_out, int* it_out,
+                          int* flag) {
+  if (threadIdx.x) return;
+  const double zansa = 2.0 * bb_total(part, nch) - part[nch];
+  const double eig = S->eigv, eigb = S->eigb;
+  const int iter = S->iter;
+  S->zansa = zansa;
+  *eig_out = eig;
+  *bz_out = zansa;
+  *it_out = iter;
+  const bool stop = (ib <= noband && iter >= itermx) || (ib > noband && iter >= iteremx) ||
+                    zansa < tolera || fabs(eigb - eig) < tolera || zansa < S->z000re * roughfzan ||
+                    (iter > 1 && fabs(eigb - eig) < S->x000re * roughfeig);
+  if (stop) {
+    S->active = 0;
+  } else {
+    if (iter == 1) S->x000re = fabs(eigb - eig);
+    if (zansa > S->zansab) {
+      if (++S->izcnt > 20) {
+        S->izcnt = 0;
+        S->lreset = 1;
+      }
+    }
+    S->zansab = zansa;
+    S->eigb = eig;
+  }
+  *flag = S->active;
+}
+
 int bkl_band_by_band(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib, double* eig,
                      int noband, int itermx, int iteremx, double tolera, double threinn,
                      double wdinner, double roughfzan, double roughfeig, double* bzan, int* nriter,
                      int prec) {
-  const int n = c->nplwex;
+  const int n = c->nplwex, nch = (n + kDotChunk - 1) / kDotChunk;
+  if (!c->d_bb || !c->h_flag || nch > c->bb_nch) return -2;
   double* gnk = c->d_slot[4];
   double* hnk = c->d_slot[4] + n;
-  double* prod = c->d_scal + 8 * (size_t)c->nbmax;
-  double* ek = c->d_scal + 9 * (size_t)c->nbmax;
-  double* kin2 = c->d_kin;
-  (void)hipMemcpyAsync(ek, ekib, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream);
+  const size_t L = (size_t)c->nbmax;
+  double* prod = c->d_scal + 8 * L;
+  double* ek = c->d_scal + 9 * L;
+  double* eig_d = c->d_scal + 24 * L;
+  double* bz_d = c->d_scal + 25 * L;
+  int* it_d = reinterpret_cast<int*>(c->d_scal + 26 * L);
+  BbScal* S = reinterpret_cast<BbScal*>(c->d_bb);
+  double* p0 = c->d_bb + 32;                 // nch + 1
+  double* p1 = p0 + (nch + 1);               // nch + 1
+  double* p2 = p1 + (nch + 1);               // 3 nch
+  const int maxit = std::max(1, std::max(itermx, iteremx));
+  const dim3 grid(nch), blk(kDotThreads);
+  QX_CHECK(hipMemcpyAsync(ek, ekib, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
   for (int ibb = 0; ibb < nb; ++ibb) {
     const int ib = ibb + 1;
     double* x = psi + (size_t)n * ibb;
     double* hx = hpsi + (size_t)n * ibb;
-    double eigv = pdot1(c, x, hx);
-    axpby(c, gnk, x, hx, nullptr, nullptr, eigv, -1.0, 1, nullptr);
-    double zansa = pdot1(c, gnk, gnk);
-    int iter = 0, izcnt = 0, lreset = 1;
-    double zansab = zansa, z000re = zansa, x000re = 1e10, eigb = eigv, dinnerx = 0.0, betk = 0.0;
-    for (;;) {
-      ++iter;
-      hipLaunchKernelGGL(k_precond_mul, dim3(cdivu(n, 256)), dim3(256), 0, c->stream, gnk, kin2,
-                         ek + ibb, n, n, 1, nullptr);
-      double zansap = pdot1(c, gnk, gnk);
-      double beta = lreset ? 0.0 : zansap / betk;
-      // hnk = gnk + beta*hnk (hnk cleared on reset)
-      axpby(c, hnk, gnk, lreset ? nullptr : hnk, nullptr, nullptr, 1.0, beta, 1, nullptr);
-      lreset = 0;
-      betk = zansap;
-      if (ibb > 0) {  // subgs3 against updated bands 1..ib-1
+    hipLaunchKernelGGL(k_bb_dot, grid, blk, 0, c->stream, x, hx, n, nch, p0);
+    hipLaunchKernelGGL(k_bb_start, grid, blk, 0, c->stream, x, hx, gnk, n, nch, p0, S, p1);
+    hipLaunchKernelGGL(k_bb_init, dim3(1), dim3(64), 0, c->stream, S, p1, nch);
+    for (int it = 0; it < maxit; ++it) {
+      hipLaunchKernelGGL(k_precond_dot, grid, blk, 0, c->stream, gnk, c->d_kin, ek + ibb, n,
+                         nullptr, nch, p0);
+      hipLaunchKernelGGL(k_bb_dir, grid, blk, 0, c->stream, gnk, hnk, n, nch, p0, S);
+      if (ibb > 0) {  // subgs3 against the updated bands 1..ib-1
         int rc = gemm_tn(c, ibb, 1, n, 2.0, psi, n, hnk, n, 0.0, prod, ibb, 1);
         if (rc) return rc;
         rc = gemm_nn(c, n, 1, ibb, -1.0, psi, n, prod, ibb, 1.0, hnk, n);
         if (rc) return rc;
       }
-      double hhx = pdot1(c, x, hnk);  // subgs4
-      axpby(c, hnk, hnk, x, nullptr, nullptr, 1.0, -hhx, 1, nullptr);
-      double cai = pdot1(c, hnk, hx);
+      hipLaunchKernelGGL(k_bb_dot, grid, blk, 0, c->stream, x, hnk, n, nch, p0);
+      hipLaunchKernelGGL(k_bb_orth, grid, blk, 0, c->stream, x, hnk, hx, n, nch, p0, p1);
       int rc = hpsi1(c, hnk, gnk, prec);
       if (rc) return rc;
-      double cbi = pdot1(c, hnk, gnk);
-      double hhi = pdot1(c, hnk, hnk);
-      double dtnew, alpsq;
-      if (hhi > 0.0) {
-        double h11 = eigv, h12 = cai / std::sqrt(hhi), h21 = h12, h22 = cbi / hhi;
-        double enew =
-            0.5 * (h11 + h22 - std::sqrt((h11 + h22) * (h11 + h22) - 4.0 * (h11 * h22 - h12 * h21)));
-        dtnew = (enew - h11) / h12 / std::sqrt(hhi);
-        double alpsq2 = 1.0 / (1.0 + dtnew * (0.0 + dtnew * hhi));
-        alpsq = std::sqrt(alpsq2);
-        eigv = enew;
-      } else {
-        double hx2 = pdot1(c, hnk, x);
-        double c0 = eigv, c1 = cai, c2 = cbi, c3 = hhi, c4 = 2.0 * hx2;
-        double s0 = c0 * c4 - c1, s1 = c0 * c3 - c2, s2 = c1 * c3 - c2 * c4;
-        double s = s1 * s1 - s2 * s0;
-        dtnew = s > 0.0 ? (-s1 - std::sqrt(s)) / s2 : -1.0;
-        if (dtnew > 0.0) {
-          double dinner = std::sqrt(1.0 / (1.0 + dtnew * dtnew * hhi));
-          if (dinner < threinn) dtnew = 0.0;
-          if (dinner < dinnerx * wdinner) dtnew = 0.0;
-          dinnerx = std::max(dinnerx, dinner);
-        }
-        double dt2 = dtnew * dtnew, alpsq2 = 1.0 / (1.0 + dtnew * c4 + dt2 * c3);
-        eigv = (c0 + dtnew * c1 + dt2 * c2) * alpsq2;
-        alpsq = std::sqrt(alpsq2);
-      }
-      // x = alpsq (x + dt h), hx = alpsq (hx + dt H h), gnk = eig x - hx
-      axpby(c, x, x, hnk, nullptr, nullptr, alpsq, alpsq * dtnew, 1, nullptr);
-      axpby(c, hx, hx, gnk, nullptr, nullptr, alpsq, alpsq * dtnew, 1, nullptr);
-      axpby(c, gnk, x, hx, nullptr, nullptr, eigv, -1.0, 1, nullptr);
-      zansa = pdot1(c, gnk, gnk);
-      if ((ib <= noband && iter >= itermx) || (ib > noband && iter >= iteremx) || zansa < tolera ||
-          std::fabs(eigb - eigv) < tolera || zansa < z000re * roughfzan ||
-          (iter > 1 && std::fabs(eigb - eigv) < x000re * roughfeig))
-        break;
-      if (iter == 1) x000re = std::fabs(eigb - eigv);
-      if (zansa > zansab) {
-        if (++izcnt > 20) { izcnt = 0; lreset = 1; }
-      }
-      zansab = zansa;
-      eigb = eigv;
+      hipLaunchKernelGGL(k_dot3, grid, blk, 0, c->stream, hnk, gnk, x, n, nch, p2);
+      hipLaunchKernelGGL(k_bb_step, dim3(1), dim3(64), 0, c->stream, S, p1, p2, hnk, gnk, x, nch,
+                         threinn, wdinner);
+      hipLaunchKernelGGL(k_bb_update, grid, blk, 0, c->stream, x, hx, gnk, hnk, n, nch, S, p0);
+      hipLaunchKernelGGL(k_bb_conv, dim3(1), dim3(64), 0, c->stream, S, p0, nch, ib, noband,
+                         itermx, iteremx, tolera, roughfzan, roughfeig, eig_d + ibb, bz_d + ibb,
+                         it_d + ibb, c->d_hflag);
+      QX_CHECK(hipGetLastError());
+      QX_CHECK(hipStreamSynchronize(c->stream));
+      if (!*static_cast<volatile int*>(c->h_flag)) break;
     }
-    eig[ibb] = eigv;
-    if (bzan) bzan[ibb] = zansa;
-    if (nriter) nriter[ibb] = iter;
   }
-  QX_CHECK(hipGetLastError());
+  QX_CHECK(hipMemcpyAsync(eig, eig_d, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> bz(nb);
+  std::vector<int> itv(nb);
+  QX_CHECK(hipMemcpyAsync(bz.data(), bz_d, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+  QX_CHECK(hipMemcpyAsync(itv.data(), it_d, sizeof(int) * nb, hipMemcpyDeviceToHost, c->stream));
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  for (int b = 0; b < nb; ++b) {
+    if (bzan) bzan[b] = bz[b];
+    if (nriter) nriter[b] = itv[b];
+  }
   return 0;
 }
 
@@ -373,7 +536,7 @@ __global__ void k_bkl_step(BklState s, int nb) {
     double h12 = cai / sqrt(hhi), h22 = cbi / hhi;
     double enew = 0.5 * (h11 + h22 - sqrt((h11 + h22) * (h11 + h22) - 4.0 * (h11 * h22 - h12 * h12)));
     double dt = (enew - h11) / h12 / sqrt(hhi);
-    double a2 = 1.0 / (1.0 + dt * dt * hhi);
+    double a2 = 1.0 / (1.0 + dt * (0.0 + dt * hhi));
     s.alpha[b] = sqrt(a2);
     s.dt[b] = dt;
     s.eig[b] = enew;

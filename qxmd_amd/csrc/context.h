@@ -79,6 +79,9 @@ struct Ctx {
   double *d_scal = nullptr;               // scalar work (device)
   int* d_sync = nullptr;                  // grid-barrier counter and error flags
   double* d_dwork = nullptr;              // dense eigensolver work (8 nbmax^2 + ...)
+  double* d_bb = nullptr;                 // band-by-band CG: scalars, then chunk sums
+  int bb_nch = 0;                         // chunks of one band the chunk-sum space holds
+  int *h_flag = nullptr, *d_hflag = nullptr;  // pinned host flag and its device address
   size_t dwork_len = 0;
   hipEvent_t ev[8] = {};
   hipEvent_t pev[6] = {};                 // SCF band-step phase timers

@@ -175,6 +175,7 @@ def test_bkl_band_by_band_fp64(si8):
     e, bz, nr = ctx.eigen_bkl(nb, ekib, w)
     np.testing.assert_array_equal(nr, nr_ref)
     np.testing.assert_allclose(e, e_ref, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(bz, bz_ref, rtol=1e-7)
     assert _relerr(ctx.get(0, nb), p_ref) < 1e-9
     assert _relerr(ctx.get(1, nb), h_ref) < 1e-9
 
