@@ -118,8 +118,9 @@ int qx_rotate_slot(qx_ctx *ctx, int psi, int hpsi, int nb, double *eig);
 int qx_density_slot(qx_ctx *ctx, int slot, int nb, const double *occ, double rvol, double *rho,
                     int prec);
 /* BKL CG band update, lortho=.false. / lhcunt=.true. as the SCF driver calls it
- * (Sources/eigen.f90:199-938, Sources/pwlda.f90:3121).  psi/hpsi slots are
- * updated in place; ekib, eig in; eig, bzan, nriter out (host arrays). */
+ * (Sources/eigen.f90:199-938, Sources/pwlda.f90:3121).  psi/hpsi are slots 0
+ * and 1 (either order; slots 2-4 are the update's work space), updated in
+ * place; ekib, eig in; eig, bzan, nriter out (host arrays). */
 int qx_eigen_bkl_slot(qx_ctx *ctx, int psi, int hpsi, int nb, const double *ekib, double *eig,
                       int noband, int itermx, int iteremx, double tolera, double threinn,
                       double wdinner, double roughfzan, double roughfeig, double *bzan,
@@ -137,7 +138,8 @@ int qx_eigen_bkl_batched_slot(qx_ctx *ctx, int psi, int hpsi, int nb, const doub
  * the preconditioner scale of prpcg */
 int qx_ekib_slot(qx_ctx *ctx, int slot, int nb, double *ekib);
 /* one SCF band step with frozen potential (energy pass, rotation, CG,
- * Gram-Schmidt, density) -- see Sources/pwlda.f90:2087-3574 */
+ * Gram-Schmidt, density) -- see Sources/pwlda.f90:2087-3574; psi/hpsi are
+ * slots 0 and 1 as for qx_eigen_bkl_slot */
 int qx_scf_band_step(qx_ctx *ctx, int psi, int hpsi, int nb, const double *occ, int noband,
                      double rvol, double *eig, double *rho, int *lcholesky, int prec);
 /* qx_scf_band_step with the CG form chosen: batched != 0 uses

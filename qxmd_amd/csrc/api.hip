@@ -248,7 +248,7 @@ int qx_create(qx_ctx** out, int n1, int n2, int n3, int nplw, const int* nga, co
   QX_CHECK(hipMemset(c->d_sync, 0, sizeof(int) * 16));
   c->dwork_len = 8 * (size_t)c->nbmax * c->nbmax + 16 * (size_t)c->nbmax + 64;
   QX_CHECK(hipMalloc(&c->d_dwork, sizeof(double) * c->dwork_len));
-  c->bb_nch = c->nplwex / 4096 + 1;
+  c->bb_nch = c->nplwex / kBbChunk + 1;
   QX_CHECK(hipMalloc(&c->d_bb, sizeof(double) * (32 + 5 * (size_t)(c->bb_nch + 1))));
   QX_CHECK(hipMemset(c->d_bb, 0, sizeof(double) * (32 + 5 * (size_t)(c->bb_nch + 1))));
   QX_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_flag), sizeof(int) * 4, hipHostMallocDefault));
@@ -490,7 +490,8 @@ int qx_eigen_bkl_slot(qx_ctx* h, int psi, int hpsi, int nb, const double* ekib, 
                       double wdinner, double roughfzan, double roughfeig, double* bzan, int* nriter,
                       int prec) {
   Ctx* c = &h->c;
-  if (psi > 1 || hpsi > 1) return fail(-1, "qx_eigen_bkl_slot: psi/hpsi must be slots 0/1");
+  if (psi < 0 || psi > 1 || hpsi < 0 || hpsi > 1 || psi == hpsi || nb < 1 || nb > c->nbmax)
+    return fail(-1, "qx_eigen_bkl_slot: psi/hpsi must be slots 0 and 1, 1 <= nb <= nbmax");
   int rc = bkl_band_by_band(c, c->d_slot[psi], c->d_slot[hpsi], nb, ekib, eig, noband, itermx,
                             iteremx, tolera, threinn, wdinner, roughfzan, roughfeig, bzan, nriter,
                             prec);
@@ -504,6 +505,8 @@ int qx_eigen_bkl_batched_slot(qx_ctx* h, int psi, int hpsi, int nb, const double
                               int noband, int itermx, int iteremx, double tolera, double roughfzan,
                               double roughfeig, double* bzan, int* nriter, int prec) {
   Ctx* c = &h->c;
+  if (psi < 0 || psi > 1 || hpsi < 0 || hpsi > 1 || psi == hpsi || nb < 1 || nb > c->nbmax)
+    return fail(-1, "qx_eigen_bkl_batched_slot: psi/hpsi must be slots 0 and 1, 1 <= nb <= nbmax");
   return bkl_batched(c, c->d_slot[psi], c->d_slot[hpsi], nb, ekib, eig, noband, itermx, iteremx,
                      tolera, roughfzan, roughfeig, bzan, nriter, prec);
 }
@@ -527,6 +530,9 @@ int qx_scf_band_step_ex(qx_ctx* h, int psi, int hpsi, int nb, const double* occ,
                         double rvol, double* eig, double* rho, int* lcholesky, int prec,
                         int batched) {
   Ctx* c = &h->c;
+  // slots 2-4 are the CG's work space
+  if (psi < 0 || psi > 1 || hpsi < 0 || hpsi > 1 || psi == hpsi || nb < 1 || nb > c->nbmax)
+    return fail(-1, "qx_scf_band_step: psi/hpsi must be slots 0 and 1, 1 <= nb <= nbmax");
   double* P = c->d_slot[psi];
   double* HP = c->d_slot[hpsi];
   std::vector<double> ekib(nb), bz(nb);

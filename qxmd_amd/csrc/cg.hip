@@ -107,6 +107,7 @@ __device__ inline void dot_store(double (&s)[NQ], double* part, int b, int ch, i
 }
 
 // g <- preconditioned g for the active bands (as k_precond_mul); sums of g.g
+template <int CH>
 __global__ void __launch_bounds__(kDotThreads) k_precond_dot(double* __restrict__ g,
                                                              const double* __restrict__ kin2,
                                                              const double* __restrict__ ekib,
@@ -116,7 +117,7 @@ __global__ void __launch_bounds__(kDotThreads) k_precond_dot(double* __restrict_
   const bool act = !mask || mask[b];
   const double e15 = 1.5 * ekib[b];
   double* gb = g + (size_t)b * n;
-  const int i0 = ch * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  const int i0 = ch * CH, i1 = min(n, i0 + CH);
   double s[1] = {0.0};
   for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     double v = gb[i];
@@ -134,13 +135,14 @@ __global__ void __launch_bounds__(kDotThreads) k_precond_dot(double* __restrict_
 }
 
 // sums of h.hpsi, h.hh, h.h
+template <int CH>
 __global__ void __launch_bounds__(kDotThreads) k_dot3(const double* __restrict__ h,
                                                       const double* __restrict__ hp,
                                                       const double* __restrict__ hh, int n,
                                                       int nch, double* __restrict__ part) {
   const int b = blockIdx.y, ch = blockIdx.x;
   const size_t o = (size_t)b * n;
-  const int i0 = ch * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  const int i0 = ch * CH, i1 = min(n, i0 + CH);
   double s[3] = {0.0, 0.0, 0.0};
   for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     const double hv = h[o + i];
@@ -219,12 +221,6 @@ static void axpby(Ctx* c, double* out, const double* x, const double* y, const d
                      cc, sa, sc, n, n, nb, mask);
 }
 
-// H h for one vector (kinetic + KB + local), order of Sources/eigen.f90:516-551
-static int hpsi1(Ctx* c, const double* x, double* hx, int prec) {
-  int rc = local_apply(c, x, hx, 1, prec, 0);
-  if (rc) return rc;
-  return kb_apply(c, x, hx, 1, 1, nullptr, prec);
-}
 
 // ---- band-by-band schedule: the scalars of the band being updated stay on
 // the device; every inner product is a chunked pass whose partial sums the
@@ -232,14 +228,24 @@ static int hpsi1(Ctx* c, const double* x, double* hx, int prec) {
 // the convergence flag.  part[0..nch) are chunk sums, part[nch] is the
 // x[0] y[0] correction of the packed product (stored by the pass, so a later
 // pass may overwrite either vector).
+constexpr int kSvChunk = 1024;   // vector elements per chunk of the skinny projection
+constexpr int kProjRows = 8;     // columns per workgroup of the skinny projection
+constexpr int kProjBands = 64;   // columns per workgroup of the skinny back-projection
+
 struct BbScal {
   double eigv, zansa, zansab, z000re, x000re, eigb, dinnerx, betk, zansap, alpsq, dtnew;
   int iter, izcnt, lreset, active;
 };
 
+// fixed-order total of part[ch * nq + q] over ch < nch by one wavefront: lane
+// l sums ch = l, l + 64, ...; the butterfly leaves the same bits in every lane
+// (each level adds the same two values in both partners), so every wavefront
+// that calls it gets the same total
 __device__ inline double bb_total(const double* part, int nch, int nq = 1, int q = 0) {
+  const int lane = threadIdx.x & 63;
   double s = 0.0;
-  for (int ch = 0; ch < nch; ++ch) s += part[ch * nq + q];
+  for (int ch = lane; ch < nch; ch += 64) s += part[ch * nq + q];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   return s;
 }
 
@@ -247,11 +253,120 @@ __global__ void __launch_bounds__(kDotThreads) k_bb_dot(const double* __restrict
                                                         const double* __restrict__ y, int n,
                                                         int nch, double* __restrict__ part) {
   const int ch = blockIdx.x;
-  const int i0 = ch * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  const int i0 = ch * kBbChunk, i1 = min(n, i0 + kBbChunk);
   double s[1] = {0.0};
   for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) s[0] += x[i] * y[i];
   dot_store<1>(s, part, 0, ch, nch);
   if (ch == 0 && threadIdx.x == 0) part[nch] = x[0] * y[0];
+}
+
+// Products of one vector with a skinny matrix A (M columns of length n,
+// column j at A + j n), each pass reading A once, in fixed summation order:
+//   projection       pt[j nch + ch] = sum over chunk ch of A_j . v
+//   back-projection  pn[s n + i] = sum_{j in column chunk s} A_j[i] coef_j,
+//                    coef_j = f_j (2 sum_ch pt[j nch + ch] - A_j[0] v[0]
+//                             - im0 2 A_j[1] v[1]),  f_j = scale ? scale[j] : -1
+// (the packed product's G = 0 corrections; im0 also drops the Im part of the
+// G = 0 term, as the KB sums do).  The caller adds sum_s pn[s] to its vector.
+// subgs3 (Sources/engrad.f90:1072-1142): A = the updated bands, f = -1;
+// hckbpp_ib_g (Sources/nlkbpp.f90:1623): A = the KB projectors, f = vclalp, im0.
+__global__ void __launch_bounds__(kDotThreads) k_sv_proj_t(const double* __restrict__ A, int n,
+                                                           int M, const double* __restrict__ v,
+                                                           int nch, double* __restrict__ pt) {
+  constexpr int PT = kSvChunk / kDotThreads;
+  const int ch = blockIdx.x, j0 = blockIdx.y * kProjRows;
+  const int i0 = ch * kSvChunk + threadIdx.x;
+  double hv[PT];
+#pragma unroll
+  for (int u = 0; u < PT; ++u) {
+    const int i = i0 + u * kDotThreads;
+    hv[u] = i < n ? v[i] : 0.0;
+  }
+  // every load of the workgroup's columns is issued before the first reduction
+  double av[kProjRows][PT];
+#pragma unroll
+  for (int r = 0; r < kProjRows; ++r) {
+    const double* a = A + (size_t)min(j0 + r, M - 1) * n;
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+      const int i = i0 + u * kDotThreads;
+      av[r][u] = (j0 + r < M && i < n) ? a[i] : 0.0;
+    }
+  }
+  __shared__ double red[kProjRows][kDotThreads / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < kProjRows; ++r) {
+    double sum = 0.0;
+#pragma unroll
+    for (int u = 0; u < PT; ++u) sum += av[r][u] * hv[u];
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_down(sum, o, 64);
+    if (lane == 0) red[r][wv] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x < kProjRows && j0 + threadIdx.x < M) {
+    double t = 0.0;
+    for (int w = 0; w < kDotThreads / 64; ++w) t += red[threadIdx.x][w];
+    pt[(size_t)(j0 + threadIdx.x) * nch + ch] = t;
+  }
+}
+
+__global__ void __launch_bounds__(kDotThreads) k_sv_proj_n(const double* __restrict__ A, int n,
+                                                           int M, const double* __restrict__ v,
+                                                           int nch, const double* __restrict__ pt,
+                                                           const double* __restrict__ scale,
+                                                           int im0, double* __restrict__ pn) {
+  const int sidx = blockIdx.y, j0 = sidx * kProjBands, nj = min(M - j0, kProjBands);
+  // coef_j by kSub consecutive lanes per column: strided chunk sums, then a
+  // butterfly over the kSub lanes (fixed order)
+  constexpr int kSub = kDotThreads / kProjBands;
+  __shared__ double coef[kProjBands];
+  {
+    const int jj = threadIdx.x / kSub, sub = threadIdx.x % kSub;
+    const int j = j0 + min(jj, nj - 1);
+    double t = 0.0;
+    for (int ch = sub; ch < nch; ch += kSub) t += pt[(size_t)j * nch + ch];
+    for (int o = kSub / 2; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if (sub == 0 && jj < nj) {
+      const double* a = A + (size_t)j * n;
+      double c0 = a[0] * v[0];
+      if (im0) c0 = c0 + 2.0 * a[1] * v[1];
+      coef[jj] = (2.0 * t - c0) * (scale ? scale[j] : -1.0);
+    }
+  }
+  __syncthreads();
+  const int i = blockIdx.x * kDotThreads + threadIdx.x;
+  if (i >= n) return;
+  const double* a = A + (size_t)j0 * n + i;
+  double acc = 0.0;
+  if (nj == kProjBands) {
+#pragma unroll
+    for (int j = 0; j < kProjBands; ++j) acc += a[(size_t)j * n] * coef[j];
+  } else {
+    for (int j = 0; j < nj; ++j) acc += a[(size_t)j * n] * coef[j];
+  }
+  pn[(size_t)sidx * n + i] = acc;
+}
+
+// h <- h + sum_s pn[s] (subgs3's back-projection); chunk sums of x.h for
+// subgs4 (as k_bb_dot)
+__global__ void __launch_bounds__(kDotThreads) k_bb_dot_proj(const double* __restrict__ x,
+                                                             double* __restrict__ h,
+                                                             const double* __restrict__ pn,
+                                                             int ns, int n, int nch,
+                                                             double* __restrict__ part) {
+  const int ch = blockIdx.x;
+  const int i0 = ch * kBbChunk, i1 = min(n, i0 + kBbChunk);
+  double s[1] = {0.0};
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    double t = 0.0;
+    for (int k = 0; k < ns; ++k) t += pn[(size_t)k * n + i];
+    const double hv = h[i] + t;
+    h[i] = hv;
+    s[0] += x[i] * hv;
+  }
+  dot_store<1>(s, part, 0, ch, nch);
+  if (ch == 0 && threadIdx.x == 0) part[nch] = x[0] * h[0];
 }
 
 // eigv = <x|Hx>; g = eigv x - Hx; chunk sums of g.g (Sources/eigen.f90:380-400)
@@ -262,7 +377,7 @@ __global__ void __launch_bounds__(kDotThreads) k_bb_start(const double* __restri
                                                           BbScal* S, double* __restrict__ pout) {
   const double eigv = 2.0 * bb_total(pin, nch) - pin[nch];
   const int ch = blockIdx.x;
-  const int i0 = ch * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  const int i0 = ch * kBbChunk, i1 = min(n, i0 + kBbChunk);
   double s[1] = {0.0};
   for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     const double gv = eigv * x[i] + -1.0 * hx[i];
@@ -277,8 +392,8 @@ __global__ void __launch_bounds__(kDotThreads) k_bb_start(const double* __restri
 }
 
 __global__ void k_bb_init(BbScal* S, const double* __restrict__ part, int nch) {
-  if (threadIdx.x) return;
   const double zansa = 2.0 * bb_total(part, nch) - part[nch];
+  if (threadIdx.x) return;
   S->zansa = S->zansab = S->z000re = zansa;
   S->x000re = 1e10;
   S->eigb = S->eigv;
@@ -299,7 +414,7 @@ __global__ void __launch_bounds__(kDotThreads) k_bb_dir(const double* __restrict
   const double zansap = 2.0 * bb_total(part, nch) - g[0] * g[0];
   const bool reset = S->lreset != 0;
   const double beta = reset ? 0.0 : zansap / S->betk;
-  const int i0 = blockIdx.x * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  const int i0 = blockIdx.x * kBbChunk, i1 = min(n, i0 + kBbChunk);
   for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) h[i] = reset ? g[i] : g[i] + beta * h[i];
   if (blockIdx.x == 0 && threadIdx.x == 0) S->zansap = zansap;
 }
@@ -313,7 +428,7 @@ __global__ void __launch_bounds__(kDotThreads) k_bb_orth(const double* __restric
                                                          double* __restrict__ pout) {
   const double hhx = 2.0 * bb_total(pin, nch) - pin[nch];
   const int ch = blockIdx.x;
-  const int i0 = ch * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  const int i0 = ch * kBbChunk, i1 = min(n, i0 + kBbChunk);
   double s[1] = {0.0};
   for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     const double hv = h[i] - hhx * x[i];
@@ -324,16 +439,42 @@ __global__ void __launch_bounds__(kDotThreads) k_bb_orth(const double* __restric
   if (ch == 0 && threadIdx.x == 0) pout[nch] = h[0] * hx[0];
 }
 
+// g <- g + sum_s pn[s] (the KB back-projection of H h, ns = 0 when H h is
+// complete); chunk sums of h.g, h.x, h.h (the layout of k_dot3)
+__global__ void __launch_bounds__(kDotThreads) k_bb_dot3(const double* __restrict__ h,
+                                                         double* __restrict__ g,
+                                                         const double* __restrict__ x,
+                                                         const double* __restrict__ pn, int ns,
+                                                         int n, int nch, double* __restrict__ part) {
+  const int ch = blockIdx.x;
+  const int i0 = ch * kBbChunk, i1 = min(n, i0 + kBbChunk);
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    double gv = g[i];
+    if (ns) {
+      double t = 0.0;
+      for (int k = 0; k < ns; ++k) t += pn[(size_t)k * n + i];
+      gv = gv + t;
+      g[i] = gv;
+    }
+    const double hv = h[i];
+    s[0] += hv * gv;
+    s[1] += hv * x[i];
+    s[2] += hv * hv;
+  }
+  dot_store<3>(s, part, 0, ch, nch);
+}
+
 // the 2x2 step of BKL, or the line minimisation when h.h <= 0
 // (Sources/eigen.f90:575-693); pc: cai, pd: h.Hh | h.x | h.h
 __global__ void k_bb_step(BbScal* S, const double* __restrict__ pc, const double* __restrict__ pd,
                           const double* __restrict__ h, const double* __restrict__ hh,
                           const double* __restrict__ x, int nch, double threinn, double wdinner) {
-  if (threadIdx.x) return;
   const double cai = 2.0 * bb_total(pc, nch) - pc[nch];
   const double cbi = 2.0 * bb_total(pd, nch, 3, 0) - h[0] * hh[0];
   const double hhx = 2.0 * bb_total(pd, nch, 3, 1) - h[0] * x[0];
   const double hhi = 2.0 * bb_total(pd, nch, 3, 2) - h[0] * h[0];
+  if (threadIdx.x) return;
   double eigv = S->eigv, dtnew, alpsq;
   if (hhi > 0.0) {
     const double h11 = eigv, h12 = cai / sqrt(hhi), h21 = h12, h22 = cbi / hhi;
@@ -375,7 +516,7 @@ __global__ void __launch_bounds__(kDotThreads) k_bb_update(double* __restrict__ 
                                                            double* __restrict__ pout) {
   const double a = S->alpsq, dt = S->dtnew, e = S->eigv;
   const int ch = blockIdx.x;
-  const int i0 = ch * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  const int i0 = ch * kBbChunk, i1 = min(n, i0 + kBbChunk);
   double s[1] = {0.0};
   for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     const double xv = a * (x[i] + dt * h[i]);
@@ -396,8 +537,8 @@ __global__ void k_bb_conv(BbScal* S, const double* __restrict__ part, int nch, i
                           int itermx, int iteremx, double tolera, double roughfzan,
                           double roughfeig, double* eig_out, double* bz_out, int* it_out,
                           int* flag) {
-  if (threadIdx.x) return;
   const double zansa = 2.0 * bb_total(part, nch) - part[nch];
+  if (threadIdx.x) return;
   const double eig = S->eigv, eigb = S->eigb;
   const int iter = S->iter;
   S->zansa = zansa;
@@ -423,11 +564,26 @@ __global__ void k_bb_conv(BbScal* S, const double* __restrict__ part, int nch, i
   *flag = S->active;
 }
 
+// both passes of a single-vector product with A (M columns of length nplwex);
+// the back-projection's column-chunk partials go to pn (returned: their
+// count), the projection's chunk sums right after them
+static int sv_project(Ctx* c, const double* A, int M, const double* v, const double* scale,
+                      int im0, double* pn) {
+  const int n = c->nplwex, ns = (M + kProjBands - 1) / kProjBands;
+  const int nch = (n + kSvChunk - 1) / kSvChunk;
+  double* pt = pn + (size_t)ns * n;
+  hipLaunchKernelGGL(k_sv_proj_t, dim3(nch, (M + kProjRows - 1) / kProjRows), dim3(kDotThreads), 0,
+                     c->stream, A, n, M, v, nch, pt);
+  hipLaunchKernelGGL(k_sv_proj_n, dim3((n + kDotThreads - 1) / kDotThreads, ns), dim3(kDotThreads),
+                     0, c->stream, A, n, M, v, nch, pt, scale, im0, pn);
+  return ns;
+}
+
 int bkl_band_by_band(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib, double* eig,
                      int noband, int itermx, int iteremx, double tolera, double threinn,
                      double wdinner, double roughfzan, double roughfeig, double* bzan, int* nriter,
                      int prec) {
-  const int n = c->nplwex, nch = (n + kDotChunk - 1) / kDotChunk;
+  const int n = c->nplwex, nch = (n + kBbChunk - 1) / kBbChunk;
   if (!c->d_bb || !c->h_flag || nch > c->bb_nch) return -2;
   double* gnk = c->d_slot[4];
   double* hnk = c->d_slot[4] + n;
@@ -441,6 +597,14 @@ int bkl_band_by_band(Ctx* c, double* psi, double* hpsi, int nb, const double* ek
   double* p0 = c->d_bb + 32;                 // nch + 1
   double* p1 = p0 + (nch + 1);               // nch + 1
   double* p2 = p1 + (nch + 1);               // 3 nch
+  // subgs3 scratch after gnk and hnk in the same slot: the back-projection's
+  // band-chunk partials (ns x n), then the projection's chunk sums (ibb x nch)
+  double* pn = c->d_slot[4] + 2 * (size_t)n;
+  const size_t slot_len = (size_t)n * c->nbmax;
+  auto proj_fits = [&](int ibb) {
+    const size_t ns = (size_t)(ibb + kProjBands - 1) / kProjBands;
+    return (2 + ns) * n + (size_t)ibb * ((n + kSvChunk - 1) / kSvChunk) <= slot_len;
+  };
   const int maxit = std::max(1, std::max(itermx, iteremx));
   const dim3 grid(nch), blk(kDotThreads);
   QX_CHECK(hipMemcpyAsync(ek, ekib, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
@@ -452,20 +616,34 @@ int bkl_band_by_band(Ctx* c, double* psi, double* hpsi, int nb, const double* ek
     hipLaunchKernelGGL(k_bb_start, grid, blk, 0, c->stream, x, hx, gnk, n, nch, p0, S, p1);
     hipLaunchKernelGGL(k_bb_init, dim3(1), dim3(64), 0, c->stream, S, p1, nch);
     for (int it = 0; it < maxit; ++it) {
-      hipLaunchKernelGGL(k_precond_dot, grid, blk, 0, c->stream, gnk, c->d_kin, ek + ibb, n,
+      hipLaunchKernelGGL(k_precond_dot<kBbChunk>, grid, blk, 0, c->stream, gnk, c->d_kin, ek + ibb, n,
                          nullptr, nch, p0);
       hipLaunchKernelGGL(k_bb_dir, grid, blk, 0, c->stream, gnk, hnk, n, nch, p0, S);
-      if (ibb > 0) {  // subgs3 against the updated bands 1..ib-1
-        int rc = gemm_tn(c, ibb, 1, n, 2.0, psi, n, hnk, n, 0.0, prod, ibb, 1);
-        if (rc) return rc;
-        rc = gemm_nn(c, n, 1, ibb, -1.0, psi, n, prod, ibb, 1.0, hnk, n);
-        if (rc) return rc;
+      // subgs3 against the updated bands 1..ib-1, then the chunk sums of x.h
+      if (ibb > 0 && proj_fits(ibb)) {
+        const int ns = sv_project(c, psi, ibb, hnk, nullptr, 0, pn);
+        hipLaunchKernelGGL(k_bb_dot_proj, grid, blk, 0, c->stream, x, hnk, pn, ns, n, nch, p0);
+      } else {
+        if (ibb > 0) {
+          int rc = gemm_tn(c, ibb, 1, n, 2.0, psi, n, hnk, n, 0.0, prod, ibb, 1);
+          if (rc) return rc;
+          rc = gemm_nn(c, n, 1, ibb, -1.0, psi, n, prod, ibb, 1.0, hnk, n);
+          if (rc) return rc;
+        }
+        hipLaunchKernelGGL(k_bb_dot, grid, blk, 0, c->stream, x, hnk, n, nch, p0);
       }
-      hipLaunchKernelGGL(k_bb_dot, grid, blk, 0, c->stream, x, hnk, n, nch, p0);
       hipLaunchKernelGGL(k_bb_orth, grid, blk, 0, c->stream, x, hnk, hx, n, nch, p0, p1);
-      int rc = hpsi1(c, hnk, gnk, prec);
+      // H h: local part, then the KB part as a single-vector projection pair
+      // whose back-projection k_bb_dot3 adds (the FP32 form keeps kb_apply)
+      int rc = local_apply(c, hnk, gnk, 1, prec, 0);
       if (rc) return rc;
-      hipLaunchKernelGGL(k_dot3, grid, blk, 0, c->stream, hnk, gnk, x, n, nch, p2);
+      int ns_kb = 0;
+      if (prec == PREC_FP64 && proj_fits(c->npj)) {
+        if (c->npj) ns_kb = sv_project(c, c->d_B, c->npj, hnk, c->d_D, 1, pn);
+      } else if ((rc = kb_apply(c, hnk, gnk, 1, 1, nullptr, prec))) {
+        return rc;
+      }
+      hipLaunchKernelGGL(k_bb_dot3, grid, blk, 0, c->stream, hnk, gnk, x, pn, ns_kb, n, nch, p2);
       hipLaunchKernelGGL(k_bb_step, dim3(1), dim3(64), 0, c->stream, S, p1, p2, hnk, gnk, x, nch,
                          threinn, wdinner);
       hipLaunchKernelGGL(k_bb_update, grid, blk, 0, c->stream, x, hx, gnk, hnk, n, nch, S, p0);
@@ -613,7 +791,7 @@ int bkl_batched(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib_ho
   const dim3 dgrid(nch, nb);
   for (int it = 0; it < maxit; ++it) {
     // precondition, CG direction
-    hipLaunchKernelGGL(k_precond_dot, dgrid, dim3(kDotThreads), 0, c->stream, gnk, c->d_kin, ek, n,
+    hipLaunchKernelGGL(k_precond_dot<kDotChunk>, dgrid, dim3(kDotThreads), 0, c->stream, gnk, c->d_kin, ek, n,
                        s.active, nch, part);
     hipLaunchKernelGGL(k_dotq_final, dim3(gb), dim3(256), 0, c->stream, part, nch, nb, 1, gnk, gnk,
                        nullptr, nullptr, n, s.zansap, nullptr, nullptr);
@@ -627,7 +805,7 @@ int bkl_batched(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib_ho
     if (rc) return rc;
     rc = hpsi_batched(c, hnk, hh, nb, prec);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_dot3, dgrid, dim3(kDotThreads), 0, c->stream, hnk, hpsi, hh, n, nch, part);
+    hipLaunchKernelGGL(k_dot3<kDotChunk>, dgrid, dim3(kDotThreads), 0, c->stream, hnk, hpsi, hh, n, nch, part);
     hipLaunchKernelGGL(k_dotq_final, dim3(gb), dim3(256), 0, c->stream, part, nch, nb, 3, hnk, hpsi,
                        hh, hnk, n, s.cai, s.cbi, s.hhi);
     hipLaunchKernelGGL(k_bkl_step, dim3(gb), dim3(256), 0, c->stream, s, nb);

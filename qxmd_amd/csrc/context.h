@@ -7,6 +7,7 @@
 namespace qx {
 
 enum Prec { PREC_FP64 = 0, PREC_FP32 = 1 };
+constexpr int kBbChunk = 256;  // vector elements per chunk of the single-band CG passes
 
 // Pencil layout of the half sphere.  Columns are the (h,k) lines along z that
 // the full sphere (G and -G) touches; they come in mirror pairs (c, -c), with
@@ -80,7 +81,7 @@ struct Ctx {
   int* d_sync = nullptr;                  // grid-barrier counter and error flags
   double* d_dwork = nullptr;              // dense eigensolver work (8 nbmax^2 + ...)
   double* d_bb = nullptr;                 // band-by-band CG: scalars, then chunk sums
-  int bb_nch = 0;                         // chunks of one band the chunk-sum space holds
+  int bb_nch = 0;                         // chunks of kBbChunk the chunk-sum space holds
   int *h_flag = nullptr, *d_hflag = nullptr;  // pinned host flag and its device address
   size_t dwork_len = 0;
   hipEvent_t ev[8] = {};
