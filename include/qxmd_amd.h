@@ -249,6 +249,30 @@ int qx_rdacon(const char *fname, int ntype, const int *nhk1, const int *nhk2, in
               int *nstepMD, int *lclust, double *ratm, int *ifmd, double *frc, double *prevr,
               double *hcell);
 
+/* ---------------- velocity-Verlet updates of the MD loop ------------------- */
+/* Host arithmetic (O(natom)), bit-identical to the reference.  Scaled
+ * coordinates: x(3, n) in cell units, v the displacement per MD step, a the
+ * scaled acceleration with acon = dt^2 / (2 m) per species
+ * (Sources/remd.f90:3568-3571); is(n) the 1-based species of each atom;
+ * lfixion(ntype) != 0 fixes a species.  One constant-energy step is
+ * qx_normalized_a + qx_vkick after the forces (remd_semifinal,
+ * Sources/remd.f90:633-790), then qx_vkick + qx_updtps + qx_updtps_pbc at the
+ * next step's start (540-616). */
+/* rciprl (Sources/reclat.f90:1089-1128): b = reciprocal vectors / 2 pi of the
+ * cell a (3 x 3 column-major, a_i . b_j = delta_ij), *v = |det a| */
+int qx_rciprl(const double *a, double *b, double *v);
+/* normalized_a (Sources/remd.f90:6789-6830): forces a(3, n) -> acon(is) hi^T a
+ * in place, hi = rciprl(h), h the cell (3 x 3 column-major) */
+int qx_normalized_a(int n, int ntype, double *a, const int *is, const double *h,
+                    const double *acon);
+/* vkick (Sources/remd.f90:5408-5430): v += a for unfixed species */
+int qx_vkick(double *v, const double *a, const int *is, int n, int ntype, const int *lfixion);
+/* updtps (Sources/remd.f90:5755-5783): x += v for unfixed species */
+int qx_updtps(double *x, const double *v, const int *is, int n, int ntype, const int *lfixion);
+/* updtps_pbc (Sources/remd.f90:5786-5806): x into [0, 1); -2 if some |x| does
+ * not fit the reference's default INT */
+int qx_updtps_pbc(double *x, int n);
+
 /* ---------------- wave-function extrapolation between MD steps ------------- */
 /* set_coef_ASPC with ichest = 0 (Sources/sbalin.f90:77-170): predictor of
  * order ihest (kaspc = ihest + 1), history of nprvmx sets (<= 10) in HBM,

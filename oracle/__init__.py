@@ -637,3 +637,71 @@ class Hamiltonian:
                                      float(rvol), _cptr(rho) if rho is not None else None,
                                      C.byref(ek), C.byref(en), _cptr(cl) if cl is not None else None)
         return dict(psi=g, hpsi=r, eig=eig, rho=rho, ekin=ek.value, enl=en.value, ok=bool(ok))
+
+
+# ---- velocity-Verlet updates of the MD loop at constant energy -------------
+# Scaled coordinates: x in cell units, v the displacement per step, forces
+# turned into a = acon H^-1 f (acon = dt^2 / 2m per species,
+# Sources/remd.f90:3568-3571).  Scalar Python floats, in the reference's
+# operand order.
+
+def rciprl(hcell):
+    """Sources/reclat.f90:1089-1128: (b, v), b[:, j] the j-th reciprocal
+    vector / 2 pi of the cell with columns hcell[:, j], v = |det|."""
+    a = lambda i, j: float(hcell[i - 1][j - 1])  # noqa: E731
+    b = [[0.0] * 3 for _ in range(3)]
+    b[0][0] = a(2, 2) * a(3, 3) - a(3, 2) * a(2, 3)
+    b[1][0] = a(3, 2) * a(1, 3) - a(1, 2) * a(3, 3)
+    b[2][0] = a(1, 2) * a(2, 3) - a(2, 2) * a(1, 3)
+    v = a(1, 1) * b[0][0] + a(2, 1) * b[1][0] + a(3, 1) * b[2][0]
+    vr = 1.0 / v
+    for i in range(3):
+        b[i][0] = b[i][0] * vr
+    b[0][1] = (a(2, 3) * a(3, 1) - a(3, 3) * a(2, 1)) * vr
+    b[1][1] = (a(3, 3) * a(1, 1) - a(1, 3) * a(3, 1)) * vr
+    b[2][1] = (a(1, 3) * a(2, 1) - a(2, 3) * a(1, 1)) * vr
+    b[0][2] = (a(2, 1) * a(3, 2) - a(3, 1) * a(2, 2)) * vr
+    b[1][2] = (a(3, 1) * a(1, 2) - a(1, 1) * a(3, 2)) * vr
+    b[2][2] = (a(1, 1) * a(2, 2) - a(2, 1) * a(1, 2)) * vr
+    return np.array(b), abs(v)
+
+
+def normalized_a(frc, species, hcell, acon):
+    """Sources/remd.f90:6789-6830: a_i = acon(is_i) (hi(:,k) . f_i)_k."""
+    hi, _ = rciprl(hcell)
+    out = np.array(frc, dtype=np.float64, copy=True)
+    for i, f in enumerate(out):
+        arx, ary, arz = float(f[0]), float(f[1]), float(f[2])
+        ax = hi[0, 0] * arx + hi[1, 0] * ary + hi[2, 0] * arz
+        ay = hi[0, 1] * arx + hi[1, 1] * ary + hi[2, 1] * arz
+        az = hi[0, 2] * arx + hi[1, 2] * ary + hi[2, 2] * arz
+        c = float(acon[species[i] - 1])
+        out[i] = (ax * c, ay * c, az * c)
+    return out
+
+
+def vkick(v, a, species, lfixion):
+    """Sources/remd.f90:5408-5430: v + a for atoms of unfixed species
+    (updtps, 5755-5783, is the same update of x by v)."""
+    out = np.array(v, dtype=np.float64, copy=True)
+    for i in range(out.shape[0]):
+        if not lfixion[species[i] - 1]:
+            out[i] = out[i] + np.asarray(a[i], np.float64)
+    return out
+
+
+updtps = vkick
+
+
+def updtps_pbc(x):
+    """Sources/remd.f90:5786-5806: scaled coordinates into [0, 1)."""
+    out = np.array(x, dtype=np.float64, copy=True)
+    flat = out.reshape(-1)
+    for k in range(flat.size):
+        xi = float(flat[k])
+        if xi < 0.0:
+            xi = xi - float(int(xi)) + 1.0
+        if xi >= 1.0:
+            xi = xi - float(int(xi))
+        flat[k] = xi
+    return out

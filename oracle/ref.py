@@ -363,3 +363,53 @@ def svacon(fname, nhk1, nhk2, nstepcg, nstepmd, lclust, ratm, ifmd, frc, prevr, 
                    int(bool(lclust)), np.ascontiguousarray(ratm, np.float64), int(ifmd),
                    np.ascontiguousarray(frc, np.float64), np.ascontiguousarray(prevr, np.float64),
                    np.ascontiguousarray(np.asarray(hcell).T, np.float64), natom)
+
+
+_cd = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ci = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def md_kick(mode, x, v, species, lfixion):
+    """updtps (mode 0: x += v) or vkick (mode 1: v += a) of the reference on
+    copies; x, v (natom, 3); species 1-based; lfixion per species."""
+    L = lib()
+    L.qxref_md_kick.argtypes = [C.c_int, _cd, _cd, _ci, C.c_int, C.c_int, _ci]
+    out = np.ascontiguousarray(x, np.float64).copy()
+    vv = np.ascontiguousarray(v, np.float64)
+    sp = np.ascontiguousarray(species, np.int32)
+    lf = np.ascontiguousarray(np.asarray(lfixion, bool).astype(np.int32))
+    L.qxref_md_kick(int(mode), out, vv, sp, len(sp), len(lf), lf)
+    return out
+
+
+def updtps_pbc(x):
+    L = lib()
+    L.qxref_updtps_pbc.argtypes = [_cd, C.c_int]
+    out = np.ascontiguousarray(x, np.float64).copy()
+    L.qxref_updtps_pbc(out, out.shape[0])
+    return out
+
+
+def normalized_a(frc, species, hcell, acon):
+    """normalized_a of the reference: frc (natom, 3) Cartesian; hcell (3, 3)
+    with columns the lattice vectors; acon per species."""
+    L = lib()
+    L.qxref_normalized_a.argtypes = [C.c_int, C.c_int, _cd, _ci, _cd, _cd]
+    a = np.ascontiguousarray(frc, np.float64).copy()
+    h = np.zeros(18)
+    h[:9] = np.asarray(hcell, np.float64).T.ravel()
+    acon = np.ascontiguousarray(acon, np.float64)
+    L.qxref_normalized_a(a.shape[0], len(acon), a, np.ascontiguousarray(species, np.int32), h,
+                         acon)
+    return a
+
+
+def rciprl(hcell):
+    """(b, v): b[:, j] the j-th reciprocal vector / 2 pi of the cell whose
+    columns are the lattice vectors (a_i . b_j = delta_ij)."""
+    L = lib()
+    L.qxref_rciprl.argtypes = [_cd, _cd, C.POINTER(C.c_double)]
+    b = np.zeros(9)
+    v = C.c_double(0.0)
+    L.qxref_rciprl(np.ascontiguousarray(np.asarray(hcell, np.float64).T), b, C.byref(v))
+    return b.reshape(3, 3).T.copy(), v.value

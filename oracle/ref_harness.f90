@@ -763,3 +763,58 @@ subroutine qxref_svacon(fname, nlen, ntype, nhk1, nhk2, nstepcg, nstepmd, lclust
   lc = lclust /= 0
   call svacon(nf, 1, trim(fn), prevr, imd, ncg, nmd, ratm, frc, nt, h1, h2, lc, na, hcell)
 end subroutine
+
+! ---- velocity-Verlet updates of the MD loop (Sources/remd.f90) -------------
+subroutine qxref_md_kick(mode, x, v, is, n, ntype, lfix) bind(C, name="qxref_md_kick")
+  ! mode 0: updtps (x += v), 1: vkick (x += v as v += a)
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: mode, n, ntype
+  real(c_double) :: x(3 * n), v(3 * n)
+  integer(c_int) :: is(n), lfix(ntype)
+  integer :: nf(2), nn, nt, iss(n)
+  logical :: lf(ntype)
+  nf = 6
+  nn = n
+  nt = ntype
+  iss = is
+  lf = lfix /= 0
+  if (mode == 0) then
+    call updtps(nf, 0, 1, x, v, iss, nn, nt, lf)
+  else
+    call vkick(nf, 0, 1, x, v, iss, nn, nt, lf)
+  end if
+end subroutine
+
+subroutine qxref_updtps_pbc(x, n) bind(C, name="qxref_updtps_pbc")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: n
+  real(c_double) :: x(3 * n)
+  integer :: nf(2), nn, iss(n)
+  nf = 6
+  nn = n
+  iss = 1
+  call updtps_pbc(nf, 0, 1, x, iss, nn, 1)
+end subroutine
+
+subroutine qxref_normalized_a(n, ntype, a, is, h, acon) bind(C, name="qxref_normalized_a")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: n, ntype
+  real(c_double) :: a(3 * n), h(3, 3, 0:1), acon(ntype)
+  integer(c_int) :: is(n)
+  integer :: nf(2), nn, nt, iss(n)
+  nf = 6
+  nn = n
+  nt = ntype
+  iss = is
+  call normalized_a(nf, 0, 1, nn, nt, a, iss, h, acon)
+end subroutine
+
+subroutine qxref_rciprl(a, b, v) bind(C, name="qxref_rciprl")
+  use iso_c_binding
+  implicit none
+  real(c_double) :: a(3, 3), b(3, 3), v
+  call rciprl(a, b, v)
+end subroutine

@@ -24,7 +24,7 @@ ARCH = os.environ.get("QX_OFFLOAD_ARCH", "gfx950")
 HIP_SOURCES = ["pencil.hip", "pencil_fast.hip", "gemm.hip", "blaslt.hip", "dense.hip", "eigh.hip",
                "fullgrid.hip", "cg.hip", "nlsetup.hip", "forces.hip", "flocal.hip", "ewald.hip",
                "aspc.hip", "scfglue.hip", "api.hip"]
-HOST_SOURCES = ["gsphere.cpp", "restart.cpp"]
+HOST_SOURCES = ["gsphere.cpp", "restart.cpp", "mdstep.cpp"]
 FORTRAN_SOURCES = ["qxmd_amd_c.f90", "qxmd_dropin.f90"]
 FOUT = os.path.join(HERE, "libqxmd_amd_f.so")
 ROOT = os.path.dirname(HERE)

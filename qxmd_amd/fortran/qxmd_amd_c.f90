@@ -382,6 +382,39 @@ integer(c_int) function qx_flocal_g_dev(ctx, ntype, nhk1, nhk2, iatmpt, llking, 
   real(c_double), intent(inout) :: floc(*)
 end function
 
+! ---- velocity-Verlet updates of the MD loop (host arithmetic)
+integer(c_int) function qx_rciprl(a, b, v) bind(C, name="qx_rciprl")
+  import :: c_int, c_double
+  real(c_double) :: a(3, 3), b(3, 3), v
+end function
+
+integer(c_int) function qx_normalized_a(n, ntype, a, is, h, acon) bind(C, name="qx_normalized_a")
+  import :: c_int, c_double
+  integer(c_int), value :: n, ntype
+  real(c_double) :: a(*), h(3, 3), acon(*)
+  integer(c_int) :: is(*)
+end function
+
+integer(c_int) function qx_vkick(v, a, is, n, ntype, lfixion) bind(C, name="qx_vkick")
+  import :: c_int, c_double
+  real(c_double) :: v(*), a(*)
+  integer(c_int) :: is(*), lfixion(*)
+  integer(c_int), value :: n, ntype
+end function
+
+integer(c_int) function qx_updtps(x, v, is, n, ntype, lfixion) bind(C, name="qx_updtps")
+  import :: c_int, c_double
+  real(c_double) :: x(*), v(*)
+  integer(c_int) :: is(*), lfixion(*)
+  integer(c_int), value :: n, ntype
+end function
+
+integer(c_int) function qx_updtps_pbc(x, n) bind(C, name="qx_updtps_pbc")
+  import :: c_int, c_double
+  real(c_double) :: x(*)
+  integer(c_int), value :: n
+end function
+
 ! ---- wave-function extrapolation
 integer(c_int) function qx_aspc_init(ctx, nprvmx, ihest) bind(C, name="qx_aspc_init")
   import :: c_int, c_ptr

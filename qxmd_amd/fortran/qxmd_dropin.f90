@@ -554,6 +554,54 @@ end subroutine
 
 
 ! xclda (Sources/vxc.f90:961-1017): LDA exchange-correlation on the device.
+! ---- velocity-Verlet updates (Sources/remd.f90), same argument lists
+subroutine updtps(nfile, myid, nodes, x, v, is, n, ntype, lfixion)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: nfile(*), myid, nodes, n, ntype
+real*8 :: x(3*n), v(3*n)
+integer :: is(n)
+logical :: lfixion(ntype)
+integer(c_int) :: lf(ntype)
+lf = merge(1_c_int, 0_c_int, lfixion)
+call qxmd_gpu_check(qx_updtps(x, v, is, int(n, c_int), int(ntype, c_int), lf), 'updtps')
+end subroutine
+
+subroutine vkick(nfile, myid, nodes, v, a, is, n, ntype, lfixion)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: nfile(*), myid, nodes, n, ntype
+real*8 :: v(3*n), a(3*n)
+integer :: is(n)
+logical :: lfixion(ntype)
+integer(c_int) :: lf(ntype)
+lf = merge(1_c_int, 0_c_int, lfixion)
+call qxmd_gpu_check(qx_vkick(v, a, is, int(n, c_int), int(ntype, c_int), lf), 'vkick')
+end subroutine
+
+subroutine updtps_pbc(nfile, myid, nodes, x, is, n, ntype)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: nfile(*), myid, nodes, n, ntype
+real*8 :: x(3*n)
+integer :: is(n)
+call qxmd_gpu_check(qx_updtps_pbc(x, int(n, c_int)), 'updtps_pbc')
+end subroutine
+
+subroutine normalized_a(nfile, myid, nodes, n, ntype, a, is, h, acon)
+use qxmd_amd_c
+use qxmd_amd_gpu
+implicit none
+integer :: nfile(*), myid, nodes, n, ntype
+real*8 :: a(3*n), h(3,3,0:1), acon(ntype)
+integer :: is(n)
+call qxmd_gpu_check(qx_normalized_a(int(n, c_int), int(ntype, c_int), a, is, h(:,:,0), acon),  &
+&                   'normalized_a')
+end subroutine
+
 subroutine xclda(rho, rhocore, mshnod, rdelv, vexc, eeexc, eexc, evexc, ecor, evcor)
 use qxmd_amd_c
 use qxmd_amd_gpu

@@ -92,6 +92,11 @@ SIGNATURES = {
     "qx_rdwfns_slot": [_vp, C.c_char_p, _i, _i, _dp, _dp, _pi],
     "qx_svacon": [C.c_char_p, _i, _ip, _ip, _i, _i, _i, _dp, _i, _dp, _dp, _dp],
     "qx_rdacon": [C.c_char_p, _i, _ip, _ip, _pi, _pi, _pi, _dp, _pi, _dp, _dp, _dp],
+    "qx_rciprl": [_dp, _dp, _pd],
+    "qx_normalized_a": [_i, _i, _dp, _ip, _dp, _dp],
+    "qx_vkick": [_dp, _dp, _ip, _i, _i, _ip],
+    "qx_updtps": [_dp, _dp, _ip, _i, _i, _ip],
+    "qx_updtps_pbc": [_dp, _i],
     "qx_aspc_init": [_vp, _i, _i],
     "qx_sbalinaspc_slot": [_vp, _i, _i, _i, _i, _vp],
     "qx_roteig_slot": [_vp, _i, _i, _i],

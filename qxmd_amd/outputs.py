@@ -2,7 +2,10 @@
 qm_eng.d (energies of each SCF iteration, Sources/pwlda.f90:2592-2598,
 format (2i6,2es18.10,20es14.6)), qm_eig.d (eigenvalues and occupations,
 Sources/fermi.f90:1924-1937, '(10i7)' then '(i6,es13.5,f6.3)' per band, f7.3
-for negative occupations) and qm_fer.d (Fermi energy, '(2i7,2es13.5)').
+for negative occupations), qm_fer.d (Fermi energy, '(2i7,2es13.5)'), and
+qm_frc.d / qm_ion.d (forces and scaled coordinates of each MD step,
+Sources/datadump.f90:235-266: '(100i7)' nstep, ntype, nhk(1:ntype), then the
+scale '(es14.7)', then the values over the scale '(9f8.5)').
 The writers reproduce the Fortran edit descriptors so that files from this
 library and from the reference compare line by line."""
 from __future__ import annotations
@@ -85,3 +88,76 @@ def read_qm_fer(path):
         f = line.split()
         out.append((int(f[0]), int(f[1]), float(f[2])))
     return out
+
+
+# ---- qm_frc.d / qm_ion.d ---------------------------------------------------
+FRC_HEADER = "#  Atomic forces in [a.u.]       "
+ION_HEADER = "#  Atomic scaled coordinates     "
+
+
+def _i7_lines(vals, per=100):
+    vals = [int(v) for v in vals]
+    return ["".join(f"{v:7d}" for v in vals[k:k + per]) for k in range(0, len(vals), per)]
+
+
+def _f85_lines(vals):
+    vals = [float(v) for v in vals]
+    return ["".join(fortran_f(v, 8, 5) for v in vals[k:k + 9]) for k in range(0, len(vals), 9)]
+
+
+def frc_block(nstep, nhk, frc):
+    """The qm_frc.d lines of one MD step: frc (natom, 3) in a.u., nhk the atom
+    count of each species (Sources/datadump.f90:246-266)."""
+    f = np.asarray(frc, np.float64).reshape(-1, 3)
+    atconfx = float(np.abs(f).max()) / 9.999 if f.size else 0.0
+    lines = _i7_lines([nstep, len(nhk), *nhk])
+    lines.append(fortran_es(atconfx, 14, 7))
+    if abs(atconfx) < 1e-30:  # the reference's guard against dividing by zero
+        atconfx = 1.0
+    return lines + _f85_lines(x / atconfx for x in f.reshape(-1))
+
+
+def ion_block(nstep, nhk, vatm):
+    """The qm_ion.d lines of one MD step: scaled coordinates vatm (natom, 3)
+    written over 0.1 and capped at 9.99999 (Sources/datadump.f90:235-241)."""
+    v = np.asarray(vatm, np.float64).reshape(-1)
+    lines = _i7_lines([nstep, len(nhk), *nhk])
+    lines.append(fortran_es(0.1, 14, 7))
+    return lines + _f85_lines(min(x / 0.1, 9.99999) for x in v)
+
+
+def _read_blocks(path):
+    """Steps of a qm_frc.d / qm_ion.d file: (nstep, nhk, scale, values (natom,
+    3) as written).  The (9f8.5) fields can touch, so they are cut by width."""
+    lines = [ln.rstrip("\n") for ln in open(path)
+             if ln.strip() and not ln.lstrip().startswith("#")]
+    steps, i = [], 0
+    while i < len(lines):
+        h = [int(lines[i][k:k + 7]) for k in range(0, len(lines[i].rstrip()), 7)]
+        nstep, ntype = h[0], h[1]
+        while len(h) < 2 + ntype:  # more than 100 integers continue on the next line
+            i += 1
+            h += [int(lines[i][k:k + 7]) for k in range(0, len(lines[i].rstrip()), 7)]
+        nhk = np.array(h[2:2 + ntype])
+        scale = float(lines[i + 1])
+        nval = 3 * int(nhk.sum())
+        vals = []
+        i += 2
+        while len(vals) < nval:
+            ln = lines[i]
+            vals += [float(ln[k:k + 8]) for k in range(0, len(ln.rstrip()), 8)]
+            i += 1
+        steps.append((nstep, nhk, scale, np.array(vals[:nval]).reshape(-1, 3)))
+    return steps
+
+
+def read_qm_frc(path):
+    """Steps of qm_frc.d: (nstep, nhk, forces (natom, 3) in a.u.), the forces
+    as written (5 decimals of force / scale) times the scale."""
+    return [(n, nhk, vals * scale) for n, nhk, scale, vals in _read_blocks(path)]
+
+
+def read_qm_ion(path):
+    """Steps of qm_ion.d: (nstep, nhk, scaled coordinates (natom, 3)); values
+    written at the 9.99999 cap read back as 0.999999."""
+    return [(n, nhk, vals * scale) for n, nhk, scale, vals in _read_blocks(path)]

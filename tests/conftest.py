@@ -140,3 +140,22 @@ def si64():
     if 2 not in _SPHERES:
         _SPHERES[2] = Sphere(2)
     return _SPHERES[2]
+
+
+def md_case(seed=11):
+    """Inputs of the velocity-Verlet updates: 23 atoms of 3 species (the
+    second fixed), a skewed cell, scaled positions partly outside [0, 1)
+    (with the edge values 0, -0, +-1, +-2 and -2.5), displacements and
+    Cartesian forces."""
+    rng = np.random.default_rng(seed)
+    n = 23
+    species = np.array([1] * 8 + [2] * 5 + [3] * 10, np.int32)
+    lfixion = np.array([False, True, False])
+    hcell = np.array([[10.26, 0.4, -0.3], [0.2, 11.0, 0.5], [-0.1, 0.3, 9.7]])
+    x = rng.uniform(-0.2, 1.2, (n, 3))
+    x.reshape(-1)[:7] = [0.0, -0.0, 1.0, -1.0, 2.0, -2.0, -2.5]
+    v = rng.normal(0.0, 1e-3, (n, 3))
+    f = rng.normal(0.0, 0.05, (n, 3))
+    acon = np.array([0.5 * 40.0 ** 2 / 51196.0, 0.5 * 40.0 ** 2 / 21874.0,
+                     0.5 * 40.0 ** 2 / 1837.0])
+    return dict(species=species, lfixion=lfixion, hcell=hcell, x=x, v=v, f=f, acon=acon)

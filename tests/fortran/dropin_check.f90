@@ -187,6 +187,10 @@ call check_frcrec()
 ! ---- sbalinaspc: bands turning inside one subspace are predicted as they are
 call check_aspc()
 
+! ---- velocity Verlet (normalized_a, vkick, updtps, updtps_pbc): a harmonic
+! ---- crystal run forward and back returns to its start
+call check_verlet()
+
 call qxmd_gpu_finalize()
 write(*, '(a, es10.2)') 'DROPIN OK worst', worst
 
@@ -550,6 +554,54 @@ subroutine check_aspc()
     if (step > 1) e2 = max(e2, maxval(abs(g - g0)))
   end do
   call report('sbalinaspc keeps bands turning in their span', e2, 1.d-12)
+end subroutine
+
+subroutine check_verlet()
+  integer, parameter :: na = 4
+  real*8 :: h(3, 3, 0:1), x(3, na), x0(3, na), xs(3, na), v(3, na), a(3, na), acon(2), e
+  integer :: isp(na), step, dir
+  logical :: lfix(2)
+  h = 0.d0
+  h(1, 1, 0) = 10.d0
+  h(2, 2, 0) = 11.d0
+  h(3, 3, 0) = 9.5d0
+  h(1, 2, 0) = 0.4d0
+  isp = (/ 1, 2, 1, 2 /)
+  lfix = .false.
+  acon = 0.5d0 * 40.d0**2 / (/ 51196.d0, 1837.d0 /)
+  x0 = reshape((/ 0.3d0, 0.4d0, 0.5d0, 0.6d0, 0.35d0, 0.45d0, 0.55d0, 0.65d0, 0.5d0,  &
+  &               0.42d0, 0.58d0, 0.37d0 /), (/ 3, na /))
+  x = x0 + 0.01d0
+  xs = x
+  v = 0.d0
+  do dir = 1, 2
+    call spring(x, x0, h, a)
+    call normalized_a(nfile, 0, 1, na, 2, a, isp, h, acon)
+    do step = 1, 100
+      call vkick(nfile, 0, 1, v, a, isp, na, 2, lfix)
+      call updtps(nfile, 0, 1, x, v, isp, na, 2, lfix)
+      call updtps_pbc(nfile, 0, 1, x, isp, na, 2)
+      call spring(x, x0, h, a)
+      call normalized_a(nfile, 0, 1, na, 2, a, isp, h, acon)
+      call vkick(nfile, 0, 1, v, a, isp, na, 2, lfix)
+    end do
+    if (dir == 1) then
+      call expect(maxval(abs(x - xs)) > 1.d-3, 'velocity Verlet moves the atoms')
+      v = -v
+    end if
+  end do
+  e = maxval(abs(x - xs))
+  call report('velocity Verlet forward and back', e, 1.d-12)
+end subroutine
+
+! Cartesian forces -0.2 (r - r0) of independent springs, r = h x
+subroutine spring(x, x0, h, f)
+  real*8, intent(in) :: x(:, :), x0(:, :), h(3, 3, 0:1)
+  real*8, intent(out) :: f(:, :)
+  integer :: q
+  do q = 1, size(x, 2)
+    f(:, q) = -0.2d0 * matmul(h(:, :, 0), x(:, q) - x0(:, q))
+  end do
 end subroutine
 
 subroutine expect(cond, what)

@@ -145,6 +145,15 @@ def main():
     out["roteig_prev"] = np.asarray(R.roteig(cur, prev, 2))[:, :, ::SUB]
     for ih in (1, 2, 3, 4):
         out[f"coef_aspc_{ih}"] = R.coef_aspc(ih)
+    # velocity-Verlet updates of the MD loop
+    md = conftest.md_case()
+    out["md_updtps"] = R.md_kick(0, md["x"], md["v"], md["species"], md["lfixion"])
+    out["md_vkick"] = R.md_kick(1, md["v"], md["f"], md["species"], md["lfixion"])
+    out["md_normalized_a"] = R.normalized_a(md["f"], md["species"], md["hcell"], md["acon"])
+    out["md_updtps_pbc"] = R.updtps_pbc(md["x"])
+    b, vol = R.rciprl(md["hcell"])
+    out["md_rciprl_b"] = b
+    out["md_rciprl_v"] = np.array(vol)
     # restart files written by the reference (svwfns both encodings, svacon):
     # their SHA-256 digests
     import tempfile

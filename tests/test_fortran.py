@@ -1,7 +1,8 @@
 """The Fortran host layer: ISO_C_BINDING interfaces and the drop-in externals
 with the reference's names (rfft3, fft3, RsEIGQR, icholesky, softchg,
 hckbpp_g, hckbpp_ib_g, stscfn_pw, kbnlpp_frc2, xclda, harting, chgr2g, chgg2r,
-mxchgg_alloc, set_g2ming2max_in_chgdns, mxchgg, flocal_g).  The CPU test checks the built library exports them;
+mxchgg_alloc, set_g2ming2max_in_chgdns, mxchgg, flocal_g, updtps, vkick,
+updtps_pbc, normalized_a).  The CPU test checks the built library exports them;
 the GPU test runs the Fortran driver that calls each by its reference
 argument list and checks closed forms."""
 import os
@@ -15,7 +16,8 @@ FLIB = os.path.join(ROOT, "qxmd_amd", "libqxmd_amd_f.so")
 DRIVER = os.path.join(ROOT, "tests", "fortran", "dropin_check")
 DROPINS = ["rfft3_", "fft3_", "rseigqr_", "icholesky_", "softchg_", "hckbpp_g_", "hckbpp_ib_g_",
            "stscfn_pw_", "kbnlpp_frc2_", "xclda_", "harting_", "chgr2g_", "chgg2r_", "mxchgg_",
-           "mxchgg_alloc_", "set_g2ming2max_in_chgdns_", "flocal_g_"]
+           "mxchgg_alloc_", "set_g2ming2max_in_chgdns_", "flocal_g_", "updtps_", "vkick_",
+           "updtps_pbc_", "normalized_a_"]
 
 needs_fortran = pytest.mark.skipif(not os.path.exists(FLIB),
                                    reason="Fortran layer not built (no flang)")

@@ -234,3 +234,18 @@ def test_restart_files_vs_reference(si64, tmp_path, lsreal8):
     restart.svacon(a, **ion)
     R.svacon(b, **ion)
     assert open(a, "rb").read() == open(b, "rb").read()
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_verlet_updates_vs_reference(seed):
+    import conftest
+    c = conftest.md_case(seed)
+    sp, lf = c["species"], c["lfixion"]
+    assert np.array_equal(O.updtps(c["x"], c["v"], sp, lf), R.md_kick(0, c["x"], c["v"], sp, lf))
+    assert np.array_equal(O.vkick(c["v"], c["f"], sp, lf), R.md_kick(1, c["v"], c["f"], sp, lf))
+    assert np.array_equal(O.normalized_a(c["f"], sp, c["hcell"], c["acon"]),
+                          R.normalized_a(c["f"], sp, c["hcell"], c["acon"]))
+    assert np.array_equal(O.updtps_pbc(c["x"]), R.updtps_pbc(c["x"]))
+    b, v = O.rciprl(c["hcell"])
+    rb, rv = R.rciprl(c["hcell"])
+    assert np.array_equal(b, rb) and v == rv

@@ -79,3 +79,30 @@ def test_text_output_formats(tmp_path):
     steps = outputs.read_qm_eig(q)
     assert len(steps) == 2 and list(steps[1][2]) == [1, 2, 3, 4]
     assert steps[0][4][3] == -0.25
+
+
+def test_frc_and_ion_blocks(tmp_path):
+    """qm_frc.d / qm_ion.d blocks against flang's formatting of the same edit
+    descriptors (tests/golden/fortran_formats.f90), and read back."""
+    from qxmd_amd import outputs
+    want = [str(x) for x in G["fortran_format_lines"]]
+    frc = np.array([0.0123, -0.5, 0.25, 1.0e-9, -0.031, 0.0, 0.49999, -0.125, 0.3333333, -0.2,
+                    0.07, -0.0049, 0.1, 0.45, -0.0001]).reshape(5, 3)
+    vatm = np.array([0.0, 0.25, 0.5, 0.999999, 0.1234567, 0.75, 1.5, 0.0625, 0.3, 0.87654321,
+                     0.01, 0.999, 0.4, 0.6, 0.05]).reshape(5, 3)
+    fb = outputs.frc_block(17, [3, 2], frc)
+    ib = outputs.ion_block(17, [3, 2], vatm)
+    assert fb == want[7:11]
+    assert ib == want[11:15]
+    p = tmp_path / "qm_frc.d"
+    p.write_text(outputs.FRC_HEADER + "\n" + "\n".join(fb + outputs.frc_block(18, [3, 2], -frc))
+                 + "\n")
+    steps = outputs.read_qm_frc(p)
+    assert [s[0] for s in steps] == [17, 18] and list(steps[0][1]) == [3, 2]
+    scale = 0.5 / 9.999
+    np.testing.assert_allclose(steps[0][2], frc, rtol=0, atol=0.5e-5 * scale * 1.0001)
+    np.testing.assert_allclose(steps[1][2], -frc, rtol=0, atol=0.5e-5 * scale * 1.0001)
+    q = tmp_path / "qm_ion.d"
+    q.write_text(outputs.ION_HEADER + "\n" + "\n".join(ib) + "\n")
+    (n, nhk, x), = outputs.read_qm_ion(q)
+    np.testing.assert_allclose(x, np.minimum(vatm, 0.999999), rtol=0, atol=0.5e-6)
