@@ -186,6 +186,112 @@ def extrapolation(ctx, psi, nband, reps=3):
     return {"step_ms": round(1e3 * float(np.mean(times)), 3), "iaspc": ia, "lcholesky": ok}
 
 
+def md_run(ctx, cell, gs, nband, occ, psi, nsteps=3, max_scf=40, tol_ha_atom=1e-6,
+           dt=40.0):
+    """Converged MD steps at constant energy (Si masses, dt in a.u.), the SCF
+    in mixed precision, device-resident except the O(natom) Verlet updates.
+    Per step: first half kick and position update (host); stscfn_pw,
+    setnlc_g (KB table) and vlocal_g (local potential) at the new positions
+    on the device; sbalinaspc; SCF iterations (band step + outchg + Pulay +
+    potential) until the Harris-Foulkes energy changes by less than
+    tol_ha_atom per atom (Pulay history reset at the step's first iteration);
+    KB, local and Ewald reciprocal-space forces on the device; second half
+    kick.  The local form factors are the Fourier transform of the Gaussian
+    wells of model_local_potential (the same potential), so the local forces
+    are the gradient of the potential the SCF sees; the Ewald real-space sum
+    is not part of the path.  Step 0 converges from the starting bands.
+    Runs last: it replaces the context's local potential tables."""
+    from qxmd_amd import md, nlpp, pw
+    natom = len(cell.tau)
+    hcell = cell.hcell
+    nw = gs.nplw + 1
+    ctx.set_nlpp_tables(nlpp.model_tables(gs.gx[:nw], gs.gy[:nw], gs.gz[:nw]))
+    n5 = gs.nplw5 + 1
+    g = [np.asarray(a[:n5], np.int32) for a in (gs.nga, gs.ngb, gs.ngc)]
+    kmax = [int(np.abs(a).max()) for a in g]
+    nd = np.array(gs.nd, np.int32)
+    one, nat = np.array([1], np.int32), np.array([natom], np.int32)
+    species = np.ones(natom, np.int32)
+    lfix = np.zeros(1, bool)
+    acon = md.acon([28.0855 * 1822.888486], dt)
+    rx = np.zeros(gs.nplw5 + 1)
+    rx[1:] = 4 * np.pi / gs.recnrm[1: gs.nplw5 + 1]
+    x = np.linalg.solve(hcell, cell.tau.T).T
+    v = np.zeros_like(x)
+    # Gaussian wells -depth exp(-r^2 / 2 w^2) per atom plus a constant shift
+    # (model_local_potential's defaults), as form factors over the density sphere
+    depth, w, shift = 1.2, 1.1, 0.4
+    g2 = np.asarray(gs.recnrm[: gs.nplw5 + 1])
+    svco = -depth * (2 * np.pi * w * w) ** 1.5 * np.exp(-0.5 * g2 * w * w)
+    svco[0] += shift * cell.volume / natom
+    ctx.set_lopp(svco[None, :])
+    ctx.scf_set_grid(rx)
+    ions = np.arange(1, natom + 1)
+
+    def refresh(x):
+        ctx.stscfn_pw(x, hcell, nd, nd, *g, gs.nplw, kmax, copy_out=False)
+        ctx.setnlc_g(1, one, nat, natom, cell.volume)
+        ctx.vlocal_g(1, one, nat, ions, [0], natom, cell.volume, copy_out=False)
+
+    def scf():
+        eprev = None
+        for it in range(max_scf):
+            eig, _, ok = ctx.scf_band_step(nband, occ, cell.volume, prec=pw.MIXED, batched=True,
+                                           want_rho=False)
+            ctx.outchg()
+            ctx.pwpmxchgg(0.8, 0.6, lreset=(it == 0))
+            en = ctx.scf_potential()
+            e = (float(np.sum(occ * np.sort(eig))) - en["ehar"] + en["eexc"] + en["ecor"]
+                 - en["evexc"] - en["evcor"])
+            if eprev is not None and abs(e - eprev) / 2.0 / natom < tol_ha_atom:
+                return it + 1, True
+            eprev = e
+        return max_scf, False
+
+    def forces():
+        f = ctx.kb_forces(0, nband, occ)
+        f = f + ctx.flocal_g_dev(1, one, nat, ions, [0], natom)
+        fe, _, _ = ctx.frcrec(1, one, nat, [4.0], 4 * natom, natom,
+                              np.ones(natom, np.int32), np.arange(1, natom + 1))
+        return f + fe
+
+    # step 0: converge from the starting bands and density
+    ctx.put(0, psi)
+    ctx.aspc_init(3, 2)
+    ctx.mxchgg_alloc(imxchg=1, itratn=10)
+    refresh(x)
+    ctx.pwpchg_slot(0, nband, occ)
+    ctx.scf_put("rinr", ctx.scf_get("rhgr"))
+    ctx.pwpmxchgg(0.8, 0.6)
+    ctx.scf_potential()
+    ctx.sync()
+    t0 = time.perf_counter()
+    n0, conv0 = scf()
+    a = md.normalized_a(forces(), species, hcell, acon)
+    v = md.vkick(v, a, species, lfix)
+    ctx.sync()
+    first = {"scf_iterations": n0, "converged": conv0,
+             "ms": round((time.perf_counter() - t0) * 1e3, 2)}
+    steps, iters, convs = [], [], []
+    for k in range(1, nsteps + 1):
+        ctx.sync()
+        t0 = time.perf_counter()
+        v = md.vkick(v, a, species, lfix)
+        x = md.updtps_pbc(md.updtps(x, v, species, lfix))
+        refresh(x)
+        ctx.sbalinaspc(0, nband, k)
+        n, conv = scf()
+        a = md.normalized_a(forces(), species, hcell, acon)
+        v = md.vkick(v, a, species, lfix)
+        ctx.sync()
+        steps.append((time.perf_counter() - t0) * 1e3)
+        iters.append(n)
+        convs.append(conv)
+    return {"step_ms": round(float(np.mean(steps)), 2), "steps": nsteps,
+            "scf_iterations": iters, "converged": all(convs), "tol_ha_atom": tol_ha_atom,
+            "scf_prec": "mixed", "first_step": first}
+
+
 def fft_stage_bytes(gs, s):
     """Algorithmic FFT-stage bytes per band-apply: 8*N_h*s_c + N*s_r + 2*(nplw+1)*s_c
     (two real 3-D transforms as two HBM passes each, V(r) read, sphere I/O)."""
@@ -436,6 +542,11 @@ def main():
     extra["sbalinaspc_ms"] = extrapolation(ctx, psi_orth, nband)
     print(f"# sbalinaspc ({nband} bands, 3 previous steps): {extra['sbalinaspc_ms']}",
           file=sys.stderr, flush=True)
+
+    # converged MD steps (device KB refresh, extrapolation, SCF to 1e-6 Ha/atom,
+    # forces, Verlet); it re-seeds the bands, mixer and potential it uses
+    extra["md_step"] = md_run(ctx, cell, gs, nband, occupations(nband, nel), psi_orth)
+    print(f"# converged MD step: {extra['md_step']}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:

@@ -221,6 +221,17 @@ int qx_flocal_g(qx_ctx *ctx, double *floc, int ntype, const int *nhk1, const int
  * qx_stscfn_pw (called with the nplw5 G list), svco of qx_set_lopp. */
 int qx_flocal_g_dev(qx_ctx *ctx, int ntype, const int *nhk1, const int *nhk2,
                     const int *iatmpt, const int *llking, int natom, double *floc);
+/* vlocal_g (Sources/nlkbpp.f90:862-1032, caller Sources/pwlda.f90:1495): the
+ * local pseudopotential on the grid from the svco of qx_set_lopp and the
+ * device axis tables of the last qx_stscfn_pw over the density sphere, into
+ * the external potential the SCF glue adds (qx_scf_set_grid's vext):
+ * vext = (accumulate ? vext : 0) + FFT^-1[ sum_type svco(G) / rvol
+ * conj(S_type(G)) ], S_type the structure factor of the type's ions (llking
+ * types skipped), G = 0 term nhk svco(0) / rvol.  Optional host copies: vext
+ * (FFT order, N) and thrhgr (2 (nplw5 + 1), its G = 0 entry zero). */
+int qx_vlocal_g(qx_ctx *ctx, int ntype, const int *nhk1, const int *nhk2, const int *iatmpt,
+                const int *llking, int natom, double rvol, int accumulate, double *vext,
+                double *thrhgr);
 
 /* ---------------- restart files (the reference's own layout) ---------------- */
 /* svwfns (Sources/savedd.f90:2441-2810), one spin, ncscale = 1: QM_eig with

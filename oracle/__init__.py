@@ -116,6 +116,9 @@ def _declare(L):
                              C.POINTER(_d), _dp]
     L.qxo_flocal_g.argtypes = [_i, _ip, _ip, _ip, _ip, _i, _i, _ip, _ip, _ip, _dp, _dp, _dp, _dp,
                                _dp, _ip, C.POINTER(_vp), C.POINTER(_vp), _dp]
+    L.qxo_vlocal_g.argtypes = [_i, _ip, _ip, _ip, _ip, _i, _i, _ip, _ip, _ip, _dp, _d, _ip,
+                               C.POINTER(_vp), C.POINTER(_vp), _dp]
+    L.qxo_vlocal_g.restype = _d
     L.qxo_kbnlpp_frc2.argtypes = [_dp, _i, _i, _i, _ip, _dp, _dp, _i, _dp, _dp, _dp, _dp, _dp,
                                   _dp]
     L.qxo_stscfn_pw.argtypes = [_dp, _dp, _ip, _ip, _i, _i, _i, _i, _i, _i, _i, _ip, _ip, _ip,
@@ -396,6 +399,27 @@ def flocal_g(ntype, nhk1, nhk2, iatmpt, llking, natom, nga, ngb, ngc, gx, gy, gz
                        _f(np.asarray(gx)[: n5 + 1]), _f(np.asarray(gy)[: n5 + 1]),
                        _f(np.asarray(gz)[: n5 + 1]), rh, sv, _f(kmax, np.int32), dc, ds, f)
     return f
+
+
+def vlocal_g(ntype, nhk1, nhk2, iatmpt, llking, natom, nga, ngb, ngc, svco, rvol, kmax, tables,
+             ijk, nd):
+    """vlocal_g (Sources/nlkbpp.f90:862-1032) for a zeroed vext: returns (vext
+    on the grid in FFT order, thrhgr over ig = 0..nplw5).  The G = 0 grid
+    value and the +-G scatter are those of chgg2r's inverse transform."""
+    sv = _f(svco)
+    n5 = sv.shape[-1] - 1
+    thr = np.zeros(2 * (n5 + 1))
+    dc = (_vp * 3)(*[t[0].ctypes.data for t in tables])
+    ds = (_vp * 3)(*[t[1].ctypes.data for t in tables])
+    v0 = lib().qxo_vlocal_g(int(ntype), _f(nhk1, np.int32), _f(nhk2, np.int32),
+                            _f(iatmpt, np.int32), _f(llking, np.int32), int(natom), n5,
+                            _f(np.asarray(nga)[: n5 + 1], np.int32),
+                            _f(np.asarray(ngb)[: n5 + 1], np.int32),
+                            _f(np.asarray(ngc)[: n5 + 1], np.int32), sv, float(rvol),
+                            _f(kmax, np.int32), dc, ds, thr)
+    z = thr.copy()
+    z[0], z[1] = v0, 0.0
+    return chgg2r(z, ijk, nd), thr
 
 
 def frcrec(ntype, nhk1, nhk2, zv, nel, natom, iatoit, iatmpt, ew, tables, fclm=None,

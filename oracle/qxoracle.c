@@ -1175,6 +1175,59 @@ void qxo_flocal_g(int ntype, const int *nhk1, const int *nhk2, const int *iatmpt
   }
 }
 
+/* vlocal_g, reciprocal-space part (Sources/nlkbpp.f90:908-972): per type
+ * (llking false) the structure factor rhmur(G) = sum_k (cos, sin)(G.tau_k)
+ * from the axis tables (the three complex products of flocal_g), then
+ * thrhgr(G) += svco(G, type) / rvol (Re, -Im) rhmur(G) for G >= 1; thrhgr over
+ * ig = 0..nplw5, its G = 0 entry left zero.  Returns the G = 0 grid value
+ * sum_type nhk(type) svco(0, type) / rvol (912-924). */
+double qxo_vlocal_g(int ntype, const int *nhk1, const int *nhk2, const int *iatmpt,
+                    const int *llking, int natom, int nplw5, const int *nga, const int *ngb,
+                    const int *ngc, const double *svco, double rvol, const int *kmax,
+                    const double *const *dcq, const double *const *dsq, double *thrhgr)
+{
+  int it, k, ig;
+  const double rvolr = 1.0 / rvol;
+  double v0 = 0.0;
+  double *rhmur = (double *)malloc(sizeof(double) * 2 * ((size_t)nplw5 + 1));
+  (void)natom;
+  for (it = 0; it < ntype; ++it) {
+    if (llking[it]) continue;
+    const double dcsn = (double)(nhk2[it] - nhk1[it] + 1) * svco[(size_t)(nplw5 + 1) * it];
+    v0 = v0 + dcsn * rvolr;
+  }
+  for (ig = 0; ig < 2 * (nplw5 + 1); ++ig) thrhgr[ig] = 0.0;
+  for (it = 0; it < ntype; ++it) {
+    if (llking[it]) continue;
+    for (ig = 0; ig < 2 * (nplw5 + 1); ++ig) rhmur[ig] = 0.0;
+    for (k = nhk1[it]; k <= nhk2[it]; ++k) {
+      const int ia = iatmpt[k - 1] - 1;
+      const double *c1 = dcq[0] + (size_t)ia * (2 * kmax[0] + 1) + kmax[0];
+      const double *s1 = dsq[0] + (size_t)ia * (2 * kmax[0] + 1) + kmax[0];
+      const double *c2 = dcq[1] + (size_t)ia * (2 * kmax[1] + 1) + kmax[1];
+      const double *s2 = dsq[1] + (size_t)ia * (2 * kmax[1] + 1) + kmax[1];
+      const double *c3 = dcq[2] + (size_t)ia * (2 * kmax[2] + 1) + kmax[2];
+      const double *s3 = dsq[2] + (size_t)ia * (2 * kmax[2] + 1) + kmax[2];
+      for (ig = 1; ig <= nplw5; ++ig) {
+        const int k1 = nga[ig], k2 = ngb[ig], k3 = ngc[ig];
+        const double dcc12 = c1[k1] * c2[k2] - s1[k1] * s2[k2];
+        const double dsc12 = s1[k1] * c2[k2] + c1[k1] * s2[k2];
+        rhmur[2 * ig] = rhmur[2 * ig] + (dcc12 * c3[k3] - dsc12 * s3[k3]);
+        rhmur[2 * ig + 1] = rhmur[2 * ig + 1] + (dsc12 * c3[k3] + dcc12 * s3[k3]);
+      }
+    }
+    for (ig = 1; ig <= nplw5; ++ig) {
+      const double dggg = svco[ig + (size_t)(nplw5 + 1) * it] * rvolr;
+      const double phar = rhmur[2 * ig] * dggg;
+      const double phai = -rhmur[2 * ig + 1] * dggg;
+      thrhgr[2 * ig] = thrhgr[2 * ig] + phar;
+      thrhgr[2 * ig + 1] = thrhgr[2 * ig + 1] + phai;
+    }
+  }
+  free(rhmur);
+  return v0;
+}
+
 /* frcrec without stress, VECTOR form (Sources/bulk.f90:1880-2026): Ewald
  * reciprocal-space forces, energy and per-atom energies.  Per G (m =
  * 1..numk2, arrays at [m-1]) the ion sum S = sum_k zv (cos, sin)(G.tau_k)

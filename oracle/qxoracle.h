@@ -80,6 +80,10 @@ int qxo_mxchgg_pulay(double *rhgr, double *rinr, int nplwcex, int nplw5ex, const
                      int lreset, int lmixhold, int itratn, int *kntchg_io, double *unr,
                      double *dfmr, double *aij, double *cmk_out);
 void qxo_mxchgg_simple(double *rhgr, double *rinr, int nplw5ex, double aslh, int none);
+double qxo_vlocal_g(int ntype, const int *nhk1, const int *nhk2, const int *iatmpt,
+                    const int *llking, int natom, int nplw5, const int *nga, const int *ngb,
+                    const int *ngc, const double *svco, double rvol, const int *kmax,
+                    const double *const *dcq, const double *const *dsq, double *thrhgr);
 void qxo_flocal_g(int ntype, const int *nhk1, const int *nhk2, const int *iatmpt,
                   const int *llking, int natom, int nplw5, const int *nga, const int *ngb,
                   const int *ngc, const double *gx, const double *gy, const double *gz,

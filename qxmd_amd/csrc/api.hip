@@ -817,6 +817,29 @@ int qx_flocal_g_dev(qx_ctx* h, int ntype, const int* nhk1, const int* nhk2, cons
   return rc ? fail(rc, "qx_flocal_g_dev: bad ion ranges (%d)", rc) : 0;
 }
 
+int qx_vlocal_g(qx_ctx* h, int ntype, const int* nhk1, const int* nhk2, const int* iatmpt,
+                const int* llking, int natom, double rvol, int accumulate, double* vext,
+                double* thrhgr) {
+  Ctx* c = &h->c;
+  if (!c->scf.d_ijk || !c->scf.d_vext || c->scf.nplw5 != c->lopp.nplw5)
+    return fail(-2, "qx_vlocal_g: the density sphere (qx_scf_set_grid) and qx_set_lopp must "
+                    "have the same nplw5");
+  int rc = vlocal_sphere(c, ntype, nhk1, nhk2, iatmpt, llking, natom, rvol, c->scf.nplw5);
+  if (rc == -5) return fail(rc, "qx_vlocal_g: call qx_stscfn_pw over the density sphere "
+                                "(nplw5) for these atoms first");
+  if (rc == -4) return fail(rc, "qx_vlocal_g: call qx_set_lopp first");
+  if (rc) return fail(rc, "qx_vlocal_g: bad ion ranges or volume (%d)", rc);
+  if ((rc = vext_from_sphere(c, c->lopp.d_zin, accumulate))) return fail(rc, "qx_vlocal_g: FFT");
+  if (vext)
+    QX_CHECK(hipMemcpyAsync(vext, c->scf.d_vext, sizeof(double) * c->N, hipMemcpyDeviceToHost,
+                            c->stream));
+  if (thrhgr)
+    QX_CHECK(hipMemcpyAsync(thrhgr, c->lopp.d_thr, sizeof(double) * 2 * (c->scf.nplw5 + 1),
+                            hipMemcpyDeviceToHost, c->stream));
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 // ---------------- restart files ---------------------------------------------
 int qx_svwfns_slot(qx_ctx* h, const char* fname, int slot, int nb, const double* eig,
                    const double* occ, int lsreal8) {

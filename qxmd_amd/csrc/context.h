@@ -121,6 +121,9 @@ struct Ctx {
   struct Lopp {
     int ntype = 0, nplw5 = -1;
     double *d_svco = nullptr, *d_g = nullptr;  // d_g: gx | gy | gz
+    double *d_thr = nullptr, *d_zin = nullptr; // vlocal_g: thrhgr and the transform input
+    void* d_ions = nullptr;                    // vlocal_g: ion list (capacity natom)
+    int ion_cap = 0;
   } lopp;
   // Ewald reciprocal-space G list (clmrcp) with hep = (exp(-(pi/gamma)^2 g^2)
   // - recnrmex) / g^2 per G, for frcrec
@@ -262,6 +265,12 @@ int ewald_frcrec_dev(Ctx* c, double* fclm, int ntype, const int* nhk1, const int
                      const int* iatoit, const int* iatmpt, int natom, int nion);
 int flocal_dev(Ctx* c, int ntype, const int* nhk1, const int* nhk2, const int* iatmpt,
                const int* llking, int natom, const double* d_rhgr, int nplw5, double* floc);
+// vlocal_g: thrhgr (lopp.d_thr) and the transform input lopp.d_zin (thrhgr
+// with the G = 0 grid value) on the density sphere, from the device tables
+int vlocal_sphere(Ctx* c, int ntype, const int* nhk1, const int* nhk2, const int* iatmpt,
+                  const int* llking, int natom, double rvol, int nplw5);
+// vext (scf) = (accumulate ? vext : 0) + the inverse transform of zin
+int vext_from_sphere(Ctx* c, const double* d_zin, int accumulate);
 int flocal_host(Ctx* c, double* floc, int ntype, const int* nhk1, const int* nhk2,
                 const int* iatmpt, int natom, const int* llking, int nplw5, const int* nga,
                 const int* ngb, const int* ngc, const double* rhgr, const int* kmax,

@@ -86,6 +86,54 @@ __global__ void __launch_bounds__(kFlThreads)
   }
 }
 
+// vlocal_g, reciprocal-space part (Sources/nlkbpp.f90:928-972), one thread per
+// G of the density sphere: the ions of each type in the reference's order
+// (ions are grouped by type) give rhmur = sum (cos, sin)(G.tau), then
+// thrhgr += svco / rvol (Re, -Im) rhmur, type by type.  zin = thrhgr with the
+// G = 0 grid value v0 (912-924) in its G = 0 slot.
+__global__ void __launch_bounds__(kFlThreads)
+    k_vlocal(const Ion* __restrict__ ions, int nion, const double* __restrict__ tab, int ld,
+             int km1, int km2, int km3, int natom_tab, const int* __restrict__ nga,
+             const int* __restrict__ ngb, const int* __restrict__ ngc, int nplw5,
+             const double* __restrict__ svco, double rvolr, double v0, double* __restrict__ thr,
+             double* __restrict__ zin) {
+#pragma clang fp contract(off)
+  const int ig = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ig > nplw5) return;
+  if (ig == 0) {
+    thr[0] = thr[1] = 0.0;
+    zin[0] = v0;
+    zin[1] = 0.0;
+    return;
+  }
+  const double* dc = tab;
+  const double* ds = tab + 3 * (size_t)natom_tab * ld;
+  const int k1 = nga[ig], k2 = ngb[ig], k3 = ngc[ig];
+  double tr = 0.0, ti = 0.0;
+  int j = 0;
+  while (j < nion) {
+    const int it = ions[j].it;
+    double rc = 0.0, rs = 0.0;
+    for (; j < nion && ions[j].it == it; ++j) {
+      const size_t r = (size_t)(3 * ions[j].ia) * ld;
+      const double c1 = dc[r + km1 + k1], s1 = ds[r + km1 + k1];
+      const double c2 = dc[r + ld + km2 + k2], s2 = ds[r + ld + km2 + k2];
+      const double c3 = dc[r + 2 * ld + km3 + k3], s3 = ds[r + 2 * ld + km3 + k3];
+      const double dcc12 = c1 * c2 - s1 * s2;
+      const double dsc12 = s1 * c2 + c1 * s2;
+      rc = rc + (dcc12 * c3 - dsc12 * s3);
+      rs = rs + (dsc12 * c3 + dcc12 * s3);
+    }
+    const double dggg = svco[(size_t)it * (nplw5 + 1) + ig] * rvolr;
+    const double phar = rc * dggg;
+    const double phai = -rs * dggg;
+    tr = tr + phar;
+    ti = ti + phai;
+  }
+  thr[2 * ig] = zin[2 * ig] = tr;
+  thr[2 * ig + 1] = zin[2 * ig + 1] = ti;
+}
+
 // out[j * 3 + c] = sum over chunks, in chunk order
 __global__ void k_flocal_sum(const double* __restrict__ part, int nchunk, int nion,
                              double* __restrict__ out) {
@@ -163,9 +211,55 @@ void flocal_finish(int ntype, const int* nhk1, const int* nhk2, const int* iatmp
 }  // namespace
 
 void lopp_destroy(Ctx* c) {
-  if (c->lopp.d_svco) (void)hipFree(c->lopp.d_svco);
-  if (c->lopp.d_g) (void)hipFree(c->lopp.d_g);
+  for (void* p : {(void*)c->lopp.d_svco, (void*)c->lopp.d_g, (void*)c->lopp.d_thr,
+                  (void*)c->lopp.d_zin, c->lopp.d_ions})
+    if (p) (void)hipFree(p);
   c->lopp = Ctx::Lopp{};
+}
+
+int vlocal_sphere(Ctx* c, int ntype, const int* nhk1, const int* nhk2, const int* iatmpt,
+                  const int* llking, int natom, double rvol, int nplw5) {
+  auto& L = c->lopp;
+  if (!L.d_svco || L.ntype != ntype || L.nplw5 != nplw5) return -4;
+  if (!c->sf.d_tab || c->sf.ngmax < nplw5 || c->sf.natom != natom) return -5;
+  if (!(rvol > 0.0)) return -2;
+  std::vector<Ion> ions;
+  int rc = ion_list(ntype, nhk1, nhk2, iatmpt, llking, natom, ions);
+  if (rc) return rc;
+  const size_t n5x = 2 * ((size_t)nplw5 + 1);
+  if (!L.d_thr) {
+    QX_CHECK(hipMalloc(&L.d_thr, sizeof(double) * n5x));
+    QX_CHECK(hipMalloc(&L.d_zin, sizeof(double) * n5x));
+  }
+  if (L.ion_cap < (int)ions.size() || !L.d_ions) {
+    if (L.d_ions) (void)hipFree(L.d_ions);
+    L.ion_cap = std::max<int>(1, (int)ions.size());
+    QX_CHECK(hipMalloc(&L.d_ions, sizeof(Ion) * L.ion_cap));
+  }
+  if (!ions.empty())
+    QX_CHECK(hipMemcpyAsync(L.d_ions, ions.data(), sizeof(Ion) * ions.size(),
+                            hipMemcpyHostToDevice, c->stream));
+  // G = 0 grid value, type by type (nhk = nhk2 - nhk1 + 1 atoms of the type)
+  std::vector<double> svco0(ntype);
+  for (int it = 0; it < ntype; ++it)
+    QX_CHECK(hipMemcpyAsync(&svco0[it], L.d_svco + (size_t)it * (nplw5 + 1), sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  const double rvolr = 1.0 / rvol;
+  double v0 = 0.0;
+  for (int it = 0; it < ntype; ++it) {
+    if (llking[it]) continue;
+    const double dcsn = (double)(nhk2[it] - nhk1[it] + 1) * svco0[it];
+    v0 = v0 + dcsn * rvolr;
+  }
+  const int* g = c->sf.d_g;
+  const int ng1 = c->sf.ngmax + 1;
+  hipLaunchKernelGGL(k_vlocal, dim3(cdivu((long)nplw5 + 1, kFlThreads)), dim3(kFlThreads), 0,
+                     c->stream, static_cast<const Ion*>(L.d_ions), (int)ions.size(), c->sf.d_tab,
+                     c->sf.ld, c->sf.km[0], c->sf.km[1], c->sf.km[2], c->sf.natom, g, g + ng1,
+                     g + 2 * ng1, nplw5, L.d_svco, rvolr, v0, L.d_thr, L.d_zin);
+  QX_CHECK(hipGetLastError());
+  return 0;
 }
 
 int lopp_set(Ctx* c, int ntype, int nplw5, const double* svco, const double* gx,

@@ -425,6 +425,13 @@ int g2r_dev(Ctx* c, const double* rhgr) {
   return 0;
 }
 
+__global__ void k_sg_vext(const double2* __restrict__ z, double* __restrict__ vext, long N,
+                          int accumulate) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  vext[i] = accumulate ? vext[i] + z[i].x : z[i].x;
+}
+
 // mixing part of mxchgg (Sources/chgdns.f90:796-1210) on d_rhgr / d_rinr
 int mix_dev(Ctx* c, int lreset, double aslh, double bslh, int lmixhold) {
   auto& S = c->scf;
@@ -552,6 +559,26 @@ int potential_dev(Ctx* c, double* ene) {
 }
 
 }  // namespace
+
+// vlocal_g's grid step (Sources/nlkbpp.f90:974-1027): zin scattered to +-G
+// (G = 0 real), the unnormalised inverse transform, vext (+)= its real part
+int vext_from_sphere(Ctx* c, const double* d_zin, int accumulate) {
+  auto& S = c->scf;
+  if (!S.d_ijk || !S.d_vext) return -4;
+  const long N = c->N;
+  hipLaunchKernelGGL(k_sg_zero, dim3(cdivu(N, 256)), dim3(256), 0, c->stream, c->fg.d_g, N);
+  hipLaunchKernelGGL(k_sg_scatter, dim3(cdivu(S.nplw5 + 1, 256)), dim3(256), 0, c->stream, d_zin,
+                     nullptr, S.d_ijk + S.nplw5, S.nplw5, c->fg.d_g);
+  QX_CHECK(hipGetLastError());
+  int rc = fg_transform(c->fg, c->stream, +1, 1.0);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_sg_vext, dim3(cdivu(N, 256)), dim3(256), 0, c->stream, c->fg.d_g, S.d_vext,
+                     N, accumulate);
+  QX_CHECK(hipGetLastError());
+  S.z_vhar = false;  // fg.d_g no longer holds rho + i vhar
+  return 0;
+}
+
 
 void scf_destroy(Ctx* c) {
   auto& S = c->scf;

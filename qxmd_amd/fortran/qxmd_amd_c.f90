@@ -382,6 +382,16 @@ integer(c_int) function qx_flocal_g_dev(ctx, ntype, nhk1, nhk2, iatmpt, llking, 
   real(c_double), intent(inout) :: floc(*)
 end function
 
+integer(c_int) function qx_vlocal_g(ctx, ntype, nhk1, nhk2, iatmpt, llking, natom, rvol,  &
+&     accumulate, vext, thrhgr) bind(C, name="qx_vlocal_g")
+  import :: c_int, c_double, c_ptr
+  type(c_ptr), value :: ctx
+  integer(c_int), value :: ntype, natom, accumulate
+  integer(c_int), intent(in) :: nhk1(*), nhk2(*), iatmpt(*), llking(*)
+  real(c_double), value :: rvol
+  real(c_double) :: vext(*), thrhgr(*)
+end function
+
 ! ---- velocity-Verlet updates of the MD loop (host arithmetic)
 integer(c_int) function qx_rciprl(a, b, v) bind(C, name="qx_rciprl")
   import :: c_int, c_double

@@ -228,6 +228,25 @@ end subroutine
 
 ! svco(nplw5+1, ntype) of lopp_variables (Sources/ppkb.f90:66-79) and the
 ! Cartesian G of the density sphere, once per cell, for flocal_g
+! vlocal_g on the device (Sources/nlkbpp.f90:862-1032): the local
+! pseudopotential from the svco of qxmd_gpu_set_lopp and the device structure
+! factors of the last stscfn_pw over the density sphere, into the SCF glue's
+! external potential (vext += ... when accumulate, as the reference's
+! vext(i) = vext(i) + fft3x(i)); host copies of vext (FFT order) and thrhgr.
+subroutine qxmd_vlocal_g(ntype, nhk1, nhk2, iatmpt, llking, natom, rvol, accumulate, vext,  &
+&                        thrhgr)
+  integer, intent(in) :: ntype, natom, nhk1(ntype), nhk2(ntype), iatmpt(natom)
+  logical, intent(in) :: llking(ntype), accumulate
+  real*8, intent(in) :: rvol
+  real*8, intent(inout) :: vext(*), thrhgr(*)
+  integer(c_int) :: lk(ntype)
+  call qxmd_gpu_need_ctx('qxmd_vlocal_g')
+  lk = merge(1_c_int, 0_c_int, llking)
+  call qxmd_gpu_check(qx_vlocal_g(qxctx, int(ntype, c_int), nhk1, nhk2, iatmpt, lk,  &
+&      int(natom, c_int), rvol, merge(1_c_int, 0_c_int, accumulate), vext, thrhgr),  &
+&      'qxmd_vlocal_g')
+end subroutine
+
 subroutine qxmd_gpu_set_lopp(ntype, nplw5, svco, gx, gy, gz)
   integer, intent(in) :: ntype, nplw5
   real*8, intent(in) :: svco(nplw5 + 1, ntype), gx(0:nplw5), gy(0:nplw5), gz(0:nplw5)

@@ -86,6 +86,7 @@ SIGNATURES = {
     "qx_flocal_g": [_vp, _dp, _i, _ip, _ip, _ip, _i, _ip, _i, _ip, _ip, _ip, _dp, _i, _i, _i,
                     _dp, _dp, _dp, _dp, _dp, _dp],
     "qx_flocal_g_dev": [_vp, _i, _ip, _ip, _ip, _ip, _i, _dp],
+    "qx_vlocal_g": [_vp, _i, _ip, _ip, _ip, _ip, _i, _d, _i, _vp, _vp],
     "qx_svwfns": [C.c_char_p, _dp, _i, _i, _i, _dp, _dp, _i],
     "qx_rdwfns": [C.c_char_p, _dp, _i, _i, _i, _dp, _dp, _pi],
     "qx_svwfns_slot": [_vp, C.c_char_p, _i, _i, _dp, _dp, _i],

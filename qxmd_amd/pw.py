@@ -352,6 +352,20 @@ class PlaneWave:
         return eig, occ, nf.value
 
     # ---- wave-function extrapolation (sbalinaspc)
+    def vlocal_g(self, ntype, nhk1, nhk2, iatmpt, llking, natom, rvol, accumulate=False,
+                 copy_out=True):
+        """Local pseudopotential into the context's external potential
+        (vlocal_g on the device); returns (vext (FFT order), thrhgr) when
+        copy_out, else None."""
+        vx = np.zeros(self.gs.N) if copy_out else None
+        th = np.zeros(2 * (self.gs.nplw5 + 1)) if copy_out else None
+        check(load().qx_vlocal_g(self.h, int(ntype), i32(nhk1), i32(nhk2), i32(iatmpt),
+                                 i32(llking), int(natom), float(rvol), int(bool(accumulate)),
+                                 vx.ctypes.data_as(C.c_void_p) if copy_out else None,
+                                 th.ctypes.data_as(C.c_void_p) if copy_out else None),
+              "qx_vlocal_g")
+        return (vx, th) if copy_out else None
+
     def aspc_init(self, nprvmx, ihest):
         check(load().qx_aspc_init(self.h, int(nprvmx), int(ihest)), "qx_aspc_init")
 
