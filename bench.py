@@ -312,14 +312,16 @@ def time_region(ctx, fn, reps):
 
 
 def cpu_worker(ncell, budget_s):
-    """Oracle H|psi> (FP64) on the host for as many bands as fit the budget."""
+    """Oracle H|psi> (FP64) on the host over the workload's bands, for as many
+    as fit the budget (about budget_s of CPU work)."""
     import oracle as O
-    cell, gs, v, eq, lmx, vcl, cl, psi = build_inputs(ncell, 64)
+    nband = int(2.5 * 8 * ncell ** 3)
+    cell, gs, v, eq, lmx, vcl, cl, psi = build_inputs(ncell, nband)
     H = O.Hamiltonian(gs.nplw, gs.nd, gs.ijkgd(), v, gs.dkgnrm(), eq, lmx, vcl)
     H.hpsi(psi[:1])  # warm the twiddle tables
     t0 = time.perf_counter()
     n = 0
-    while n < 64:
+    while n + 2 <= nband:
         H.hpsi(psi[n:n + 2])
         n += 2
         if time.perf_counter() - t0 > budget_s:
