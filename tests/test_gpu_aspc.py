@@ -18,6 +18,8 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("kpwvx", [1, 2, 3])
 def test_roteig_device(si8, kpwvx):
+    """Previous sets with large random components (the overlap is far from
+    orthogonal, so the polar factor goes through the eigenvalues of A^T A)."""
     import conftest
     cur, prev = conftest.roteig_case(si8, nb=8, nprv=3)
     ctx = pw.PlaneWave(pw.GSphere(si8.cell.hcell), 8)
@@ -56,3 +58,32 @@ def test_sbalinaspc_device(si8, si64, ncell, nb, ihest):
         assert _rel(ctx.get(0, nb), pred) < 1e-11, (step, _rel(ctx.get(0, nb), pred))
         for k in range(3):
             assert _rel(ctx.aspc_get(k, nb), st.prveig[k]) < 1e-11 or not st.prveig[k].any()
+
+
+@pytest.mark.parametrize("eigh", [False, True])
+def test_roteig_near_orthogonal_overlap(si64, eigh, monkeypatch):
+    """Previous sets that are small rotations of the current bands (as between
+    MD steps): the polar factor by Newton-Schulz and by the eigenvalues of
+    A^T A (QX_ROTEIG_EIGH) agree with the oracle's roteig2."""
+    import conftest
+    if eigh:
+        monkeypatch.setenv("QX_ROTEIG_EIGH", "1")
+    nb = 24
+    cur, _ = conftest.roteig_case(si64, nb=nb, nprv=1)
+    rng = np.random.default_rng(8)
+    prev = []
+    for k in range(3):
+        q, _ = np.linalg.qr(np.eye(nb) + 0.05 * (k + 1) * rng.standard_normal((nb, nb)))
+        prev.append(O.schmidt(q.T @ cur + 1e-3 * rng.standard_normal(cur.shape))[0])
+    prev = np.stack(prev)
+    ctx = pw.PlaneWave(pw.GSphere(si64.cell.hcell), nb)
+    ctx.aspc_init(3, 2)
+    for k in range(3):
+        ctx.aspc_put(k, prev[k], 1)
+    ctx.put(0, cur)
+    ctx.roteig(0, nb, 3)
+    ref, _, rc = O.roteig(cur, prev, 3)
+    assert rc == 0
+    for k in range(3):
+        assert _rel(ctx.aspc_get(k, nb), ref[k]) < 1e-12, k
+
