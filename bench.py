@@ -42,6 +42,22 @@ def build_inputs(ncell, nband):
     return cell, gs, v, eq, lmx, vcl, cl, psi
 
 
+def device_kb_table(ctx, cell, gs):
+    """Register the model s + p tables and build the KB table on the device
+    from the structure factors of the cell's atoms."""
+    from qxmd_amd import nlpp
+    nw = gs.nplw + 1
+    ctx.set_nlpp_tables(nlpp.model_tables(gs.gx[:nw], gs.gy[:nw], gs.gz[:nw]))
+    frac = np.linalg.solve(cell.hcell, cell.tau.T).T
+    n5 = gs.nplw5 + 1
+    g = [np.asarray(a[:n5], np.int32) for a in (gs.nga, gs.ngb, gs.ngc)]
+    kmax = [int(np.abs(a).max()) for a in g]
+    nd = np.array(gs.nd, np.int32)
+    natom = len(frac)
+    ctx.stscfn_pw(frac, cell.hcell, nd, nd, *g, gs.nplw, kmax, copy_out=False)
+    ctx.setnlc_g(1, np.array([1], np.int32), np.array([natom], np.int32), natom, cell.volume)
+
+
 def md_kb_setup(ctx, cell, gs, eq, lmx, vcl, reps=3):
     """Time the per-MD-step KB inputs: stscfn_pw + setnlc_g on the device
     (model s + p tables) against qx_set_kb's upload of the host-built table;
@@ -485,6 +501,10 @@ def main():
 
     extra = {}
     if not args.no_scf:
+        # the SCF steps run on the KB table an MD run builds on the device at
+        # every step (stscfn_pw + setnlc_g, model s + p tables): its band-by-
+        # band CG then applies KB from the table's factors
+        device_kb_table(ctx, cell, gs)
         occ = occupations(nband, nel)
         res = {}
         for name, prec, batched in (("fp64_batched", pw.FP64, True),

@@ -281,6 +281,7 @@ int qx_destroy(qx_ctx* h) {
   nl_destroy(c);
   scf_destroy(c);
   lopp_destroy(c);
+  kb1_destroy(c);
   ewald_destroy(c);
   aspc_destroy(c);
   (void)hipStreamDestroy(c->stream);
@@ -329,6 +330,7 @@ int qx_set_kb(qx_ctx* h, const double* eqlmr, int lmmax, int natom, const int* l
   const int npj = (int)pa.size();
   const size_t n = (size_t)c->nplwex;
   lt_forget_kb(c);
+  kb1_destroy(c);  // a host table has no factors
   if (c->d_B) { (void)hipFree(c->d_B); c->d_B = nullptr; }
   if (c->d_D) { (void)hipFree(c->d_D); c->d_D = nullptr; }
   if (c->d_P) { (void)hipFree(c->d_P); c->d_P = nullptr; }

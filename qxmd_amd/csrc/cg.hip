@@ -11,6 +11,7 @@
 //     kernel, so an iteration never waits on the host.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 #include "context.h"
 
@@ -605,6 +606,8 @@ int bkl_band_by_band(Ctx* c, double* psi, double* hpsi, int nb, const double* ek
     const size_t ns = (size_t)(ibb + kProjBands - 1) / kProjBands;
     return (2 + ns) * n + (size_t)ibb * ((n + kSvChunk - 1) / kSvChunk) <= slot_len;
   };
+  const bool use_kb1 = c->kb1.valid && c->npj > 0 && !std::getenv("QX_KB1_OFF") &&
+                       2 * (size_t)n + kb1_scratch(c) <= slot_len;
   const int maxit = std::max(1, std::max(itermx, iteremx));
   const dim3 grid(nch), blk(kDotThreads);
   QX_CHECK(hipMemcpyAsync(ek, ekib, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
@@ -638,7 +641,9 @@ int bkl_band_by_band(Ctx* c, double* psi, double* hpsi, int nb, const double* ek
       int rc = local_apply(c, hnk, gnk, 1, prec, 0);
       if (rc) return rc;
       int ns_kb = 0;
-      if (prec == PREC_FP64 && proj_fits(c->npj)) {
+      if (prec == PREC_FP64 && use_kb1) {
+        ns_kb = kb1_project(c, hnk, pn);  // from the device-built table's factors
+      } else if (prec == PREC_FP64 && proj_fits(c->npj)) {
         if (c->npj) ns_kb = sv_project(c, c->d_B, c->npj, hnk, c->d_D, 1, pn);
       } else if ((rc = kb_apply(c, hnk, gnk, 1, 1, nullptr, prec))) {
         return rc;

@@ -118,6 +118,14 @@ struct Ctx {
   } sf;
   // local pseudopotential form factors svco(ig, it), ig = 0..nplw5, and the
   // Cartesian G of the density sphere, for flocal_g
+  // factors of a device-built KB table for the one-band apply (kb1.hip):
+  // phases (natom x nplw1 complex), z rows per (type, l), projector ranges
+  struct Kb1 {
+    bool valid = false;
+    double *d_ph = nullptr, *d_z = nullptr;
+    int *d_p0 = nullptr, *d_pcls = nullptr, *d_ppar = nullptr;
+    int ncls = 0, natom = 0, nplw1 = 0, npj = 0, apg = 1;
+  } kb1;
   struct Lopp {
     int ntype = 0, nplw5 = -1;
     double *d_svco = nullptr, *d_g = nullptr;  // d_g: gx | gy | gz
@@ -269,6 +277,12 @@ int flocal_dev(Ctx* c, int ntype, const int* nhk1, const int* nhk2, const int* i
 // with the G = 0 grid value) on the density sphere, from the device tables
 int vlocal_sphere(Ctx* c, int ntype, const int* nhk1, const int* nhk2, const int* iatmpt,
                   const int* llking, int natom, double rvol, int nplw5);
+// one-band KB apply from the factors of a device-built table (kb1.hip)
+void kb1_destroy(Ctx* c);
+int kb1_build(Ctx* c, const int* type, int natom, const int* lkbppi, const int* lking,
+              const double* dyc, const double* dys, int ldy);
+size_t kb1_scratch(const Ctx* c);
+int kb1_project(Ctx* c, const double* v, double* pn);
 // vext (scf) = (accumulate ? vext : 0) + the inverse transform of zin
 int vext_from_sphere(Ctx* c, const double* d_zin, int accumulate);
 int flocal_host(Ctx* c, double* floc, int ntype, const int* nhk1, const int* nhk2,

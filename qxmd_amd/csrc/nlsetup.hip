@@ -359,6 +359,9 @@ int nl_setnlc_g(Ctx* c, int nplw, int ntype, const int* nhk1, const int* nhk2, i
     QX_CHECK(hipStreamSynchronize(c->stream));
     (void)hipFree(d_ps);
   }
+  // the table's factors for the one-band apply (before the structure factors
+  // uploaded for this call are freed)
+  if (int rc = kb1_build(c, type.data(), natom, lkbppi, lking, dyc, dys, nplwcs + 1)) return rc;
   if (up_c) (void)hipFree(up_c);
   if (up_s) (void)hipFree(up_s);
   return 0;

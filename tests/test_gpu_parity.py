@@ -477,3 +477,52 @@ def test_kb_forces_device(si64):
     _, sl = O.hckbpp_g(eq, lmx, vcl, psi, np.zeros_like(psi))
     fo = O.kbnlpp_frc2(eq, lmx, vcl, psi, occ, sl, si64.gx[:nw], si64.gy[:nw], si64.gz[:nw])
     assert np.abs((f - 0.25) - fo).max() < 1e-10 * np.abs(fo).max()
+
+
+@pytest.mark.parametrize("lking", [False, True])
+def test_bkl_band_by_band_device_table(si64, lking, monkeypatch):
+    """Band-by-band CG with a KB table built on the device (stscfn_pw +
+    setnlc_g): its one-band KB apply runs from the table's factors (phases and
+    (type, l) rows) and matches the oracle's CG on the same table, and the
+    table-streaming form (QX_KB1_OFF)."""
+    import conftest
+    c = conftest.nlpp_case(si64, ntype=2)
+    t = c["tables"]
+    nb = 10
+    gs = pw.GSphere(si64.cell.hcell)
+    v = model_local_potential(si64, si64.cell)
+    lk = np.array([0, 1], np.int32) if lking else None
+    yo, so = O.stscfn_pw(c["frac"], si64.cell.hcell, c["nd"], c["nd"], *c["g"], c["nplwcs"],
+                         c["kmax"])
+    eo, vo = O.setnlc_g(t, c["nhk1"], c["nhk2"], c["natom"], yo, so, c["recnrm"],
+                        si64.cell.volume, lking=lk)
+    types = np.repeat(np.arange(2), c["natom"] // 2)
+    lmx_atom = np.array([0 if (lking and types[ia] == 1) else t.lmx[types[ia]]
+                         for ia in range(c["natom"])], np.int32)
+    H = O.Hamiltonian(si64.nplw, si64.nd, si64.ijk, v, si64.dkg, eo, lmx_atom, vo[types])
+    psi, _ = O.schmidt(initial_wavefunctions(si64, nb))
+    hp = H.hpsi(psi)
+    hm, _ = O.hmatrix(psi, hp)
+    w, u = O.eigh(hm)
+    psi, hp = O.untrycore(psi, u), O.untrycore(hp, u)
+    ekib = O.ekinetic(si64.dkg, psi)[1]
+    e_ref, p_ref, h_ref, bz_ref, nr_ref = H.eigen_bkl(w, psi, hp, ekib)
+    out = []
+    for off in (False, True):
+        if off:
+            monkeypatch.setenv("QX_KB1_OFF", "1")
+        ctx = pw.PlaneWave(gs, nb)
+        ctx.set_glocal(v)
+        ctx.set_nlpp_tables(t)
+        ctx.stscfn_pw(c["frac"], si64.cell.hcell, c["nd"], c["nd"], *c["g"], c["nplwcs"],
+                      c["kmax"], copy_out=False)
+        ctx.setnlc_g(2, c["nhk1"], c["nhk2"], c["natom"], si64.cell.volume, lking=lk)
+        ctx.put(0, psi)
+        ctx.put(1, hp)
+        e, bz, nr = ctx.eigen_bkl(nb, ekib, w)
+        np.testing.assert_array_equal(nr, nr_ref)
+        np.testing.assert_allclose(e, e_ref, rtol=0, atol=1e-11)
+        assert _relerr(ctx.get(0, nb), p_ref) < 1e-9
+        assert _relerr(ctx.get(1, nb), h_ref) < 1e-9
+        out.append(ctx.get(0, nb))
+    assert _relerr(out[0], out[1]) < 1e-12
