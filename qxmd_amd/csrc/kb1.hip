@@ -141,17 +141,28 @@ __global__ void __launch_bounds__(kK1Threads)
   const int ip = blockIdx.x * kK1Threads + threadIdx.x;
   if (ip >= nplw1) return;
   double re = 0.0, im = 0.0;
-  int ia_prev = -1;
-  double2 h = make_double2(0.0, 0.0);
-  for (int j = 0; j < npr; ++j) {
+  double zc[kK1MaxL];
+  int cls0 = -1, ncached = 0;
+  for (int j = 0; j < npr;) {  // atom by atom
     const int ia = aof[j];
-    if (ia != ia_prev) {
-      h = ph[(size_t)ia * nplw1 + ip];
-      ia_prev = ia;
+    const int pa = pb + j, na = min(p0[ia + 1] - pa, kK1MaxL);
+    if (pcls[pa] != cls0 || na != ncached) {
+      cls0 = pcls[pa];
+      ncached = na;
+#pragma unroll
+      for (int q = 0; q < kK1MaxL; ++q)
+        zc[q] = q < na ? z[(size_t)pcls[pa + q] * nplw1 + ip] : 0.0;
     }
-    const double2 b = kb1_entry(h, z[(size_t)pcls[pb + j] * nplw1 + ip], ppar[pb + j]);
-    re += b.x * coef[j];
-    im += b.y * coef[j];
+    const double2 h = ph[(size_t)ia * nplw1 + ip];
+#pragma unroll
+    for (int q = 0; q < kK1MaxL; ++q) {
+      if (q < na) {
+        const double2 b = kb1_entry(h, zc[q], ppar[pa + q]);
+        re += b.x * coef[j + q];
+        im += b.y * coef[j + q];
+      }
+    }
+    j += na;
   }
   pn[(size_t)grp * nex + 2 * ip] = re;
   pn[(size_t)grp * nex + 2 * ip + 1] = im;
