@@ -580,6 +580,13 @@ def main():
                    "cores": threads, "kind": "port",
                    "sample": f"oracle FP64 H|psi> (C, OpenMP) on {d['bands']} bands of the same "
                              f"Si{natom} cell, {d['seconds']:.1f} s"}
+        # the same restatement on one core (the reference itself is serial)
+        d1 = cpu_baseline(args.ncell, 1, budget_s=6.0)
+        if d1:
+            extra["cpu_baseline_1core"] = {
+                "value": round(d1["bands"] / d1["seconds"], 3), "unit": "band-applies/s",
+                "cores": 1, "kind": "port",
+                "sample": f"oracle FP64 H|psi> (C) on {d1['bands']} bands, {d1['seconds']:.1f} s"}
 
     value = nband / (ms_mixed * 1e-3)
     line = {
