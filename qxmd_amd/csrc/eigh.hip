@@ -709,7 +709,7 @@ int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int
   hipLaunchKernelGGL(k_gersh, dim3(1), dim3(256), 0, st, d, e, n, bounds);
   hipLaunchKernelGGL(k_bisect_ms, dim3((n + 3) / 4), dim3(256), 0, st, d, e, n, bounds, w);
   // 3. eigenvectors: inverse iteration + re-orthonormalisation, twice; the LU
-  // factors of round 0 are reused by round 1.  Vectors are orthonormalised
+  // factors of pass 0 are reused by pass 1.  Vectors are orthonormalised
   // within eigenvalue clusters, or, when a cluster is too large for that, all
   // together by CholeskyQR (scratch X).
   hipLaunchKernelGGL(k_cluster_scan, dim3(1), dim3(64), 0, st, w, n, bounds, cl);
@@ -718,18 +718,18 @@ int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int
   QX_CHECK(hipStreamSynchronize(st));
   const bool local = maxcl <= kClusterMax && std::getenv("QX_EIGH_CHOLQR") == nullptr;
   const dim3 tgrid((n + 31) / 32, (n + 31) / 32);
-  for (int round = 0; round < 2; ++round) {
-    if (round > 0) hipLaunchKernelGGL(k_transpose_sq, tgrid, dim3(256), 0, st, z, n, X);
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass > 0) hipLaunchKernelGGL(k_transpose_sq, tgrid, dim3(256), 0, st, z, n, X);
     hipLaunchKernelGGL(k_inviter_par, dim3((n + 63) / 64), dim3(64), 0, st, d, e, n, w, bounds, lu,
-                       X, round == 0 ? 1 : 0);
+                       X, pass == 0 ? 1 : 0);
     hipLaunchKernelGGL(k_transpose_sq, tgrid, dim3(256), 0, st, X, n, z);
     if (local) {
       hipLaunchKernelGGL(k_cluster_orth, dim3(n), dim3(256), 0, st, z, n, cl);
       continue;
     }
-    // round 0 starts from random vectors (shifted first pass); round 1 from
-    // orthonormal ones, which the solve keeps near orthonormal
-    int rc = chol_qr(c, z, n, gram, linv, X, ok + 8 * round, round == 0);
+    // pass 0 starts from random vectors (shifted first CholeskyQR); pass 1
+    // from orthonormal ones, which the solve keeps near orthonormal
+    int rc = chol_qr(c, z, n, gram, linv, X, ok + 8 * pass, pass == 0);
     if (rc) return rc;
   }
   // 4. back-transformation
