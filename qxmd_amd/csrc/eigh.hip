@@ -367,7 +367,7 @@ __global__ void __launch_bounds__(64) k_inviter_par(const double* __restrict__ d
   double* pv = dl + nn;          // 1 when rows i, i+1 were swapped
 #define L_(arr, i) arr[(size_t)(i) * n]
   if (first) {
-    // factor T - lambda I once; the second round reuses it
+    // factor T - lambda I once; the second pass reuses it
     const double eps = 2.220446049250313e-16;
     const double pert = eps * bounds[3];
     double lam = w[j];
