@@ -50,6 +50,12 @@ int qx_create(qx_ctx **ctx, int n1, int n2, int n3, int nplw, const int *nga, co
               const int *ngc, const double *recnrm, int nbmax);
 int qx_destroy(qx_ctx *ctx);
 int qx_sync(qx_ctx *ctx);
+/* releases the process-wide device resources of the context-free drop-ins
+ * (rfft3, fft3, xclda, harting, chgr2g, chgg2r, mxchgg: their grid and
+ * stream); call after the last qx_destroy and before the process exits, as
+ * QXMD's end of run would (the Fortran qxmd_gpu_finalize does).  Later
+ * drop-in calls set them up again.  No reference counterpart. */
+int qx_shutdown(void);
 /* local potential glocal on the FFT grid, Ry (VECTOR build, Sources/engrad.f90:487-495) */
 int qx_set_glocal(qx_ctx *ctx, const double *glocal);
 /* KB projector tables as setnlc_g leaves them (Sources/nlkbpp.f90:1236-1279):

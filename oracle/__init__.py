@@ -37,7 +37,10 @@ PINNING = {
     "untrycore": "reference untrycore (oracle/_ref)",
     "softchg": "integral = sum occ; numpy FFT of psi(r)^2",
     "prpcg": "reference prpcg (oracle/_ref)",
-    "eigen_bkl": "Rayleigh quotient decrease, residual invariants; per-step pieces above",
+    "eigen_bkl": "per-iteration trace against numpy: the 2x2 Rayleigh-Ritz eigenvalue on "
+                 "span{x, h} of a dense H, the direction h = P g + beta h_prev up to the "
+                 "projected span, beta = zansap/betk, h orthogonal to the updated bands, exits at "
+                 "their thresholds; converged eigenvalues = numpy eigvalsh",
 }
 
 
@@ -140,6 +143,10 @@ def _declare(L):
     L.qxo_hpsi.argtypes = [C.POINTER(Ham), _dp, _dp, _i, _vp]
     L.qxo_eigen_bkl.argtypes = [C.POINTER(Ham), _dp, _dp, _dp, _dp, _i, _i, _i, _i, _d, _d, _d,
                                 _d, _d, _dp, _ip]
+    L.qxo_bkl_trace.argtypes = [_vp, _vp, _i]
+    L.qxo_bkl_trace.restype = None
+    L.qxo_bkl_trace_count.argtypes = []
+    L.qxo_bkl_trace_count.restype = _i
     L.qxo_scf_band_step.argtypes = [C.POINTER(Ham), _dp, _dp, _dp, _dp, _i, _i, _d, _vp,
                                     C.POINTER(_d), C.POINTER(_d), _vp]
     L.qxo_scf_band_step.restype = _i
@@ -648,6 +655,29 @@ class Hamiltonian:
                             int(iteremx), tolera, threinn, wdinner, roughfzan, roughfeig, bzan,
                             nriter)
         return e, g, r, bzan, nriter
+
+    BKL_TRACE_FIELDS = ("ib", "iter", "h11", "zansap", "beta", "cai", "cbi", "hhi", "eig",
+                        "dtnew", "alpsq", "zansa", "bkl", "stop", "eigb", "z000re", "x000re")
+
+    def eigen_bkl_trace(self, eig, gdcr, rhcr, ekib, cap=4096, vectors=True, **kw):
+        """eigen_bkl with its per-iteration trace: returns (eig, gdcr, rhcr,
+        bzan, nriter, records, vecs) where records is a structured list of
+        dicts (BKL_TRACE_FIELDS) and vecs[i] = (x before the update, h after
+        the projections) of record i."""
+        n = _f(gdcr).shape[1]
+        rec = np.zeros((cap, len(self.BKL_TRACE_FIELDS)))
+        vec = np.zeros((cap, 2, n)) if vectors else None
+        lib().qxo_bkl_trace(rec.ctypes.data, vec.ctypes.data if vectors else None, cap)
+        try:
+            out = self.eigen_bkl(eig, gdcr, rhcr, ekib, **kw)
+            cnt = lib().qxo_bkl_trace_count()
+        finally:
+            lib().qxo_bkl_trace(None, None, 0)
+        recs = [dict(zip(self.BKL_TRACE_FIELDS, r)) for r in rec[:cnt]]
+        for r in recs:
+            for k in ("ib", "iter", "bkl", "stop"):
+                r[k] = int(r[k])
+        return (*out, recs, vec[:cnt] if vectors else None)
 
     def scf_band_step(self, gdcr, occ, rvol, noband=0, clalp_atom=None, want_rho=True):
         g = _f(gdcr).copy()

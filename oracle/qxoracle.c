@@ -1772,6 +1772,26 @@ static void hpsi_one(const qxo_ham *h, const double *x, double *hx, double *fx, 
   qxo_hcnonloc(h->glocal, x, hx, h->nplw, h->ijkgd_c, h->n1, h->n2, h->n3, fx, fy);
 }
 
+/* Trace of the BKL schedule (test hook): when a buffer is registered,
+ * qxo_eigen_bkl appends one record of QXO_BKL_TRACE_W doubles per CG iteration
+ * -- ib, iter, h11 (eig before the step), zansap, beta, cai, cbi, hhi, eig
+ * after the step, dtnew, alpsq, zansa, the ifhhi branch (1 BKL, 0 line
+ * minimisation), the exit flag, eigb, z000re, x000re -- and, if `vec` is
+ * given, the vectors x (before the update) and h (after subgs3/subgs4) of that
+ * iteration, 2 nplwex doubles per record.  Records past `cap` are dropped. */
+static double *g_bkl_rec = NULL, *g_bkl_vec = NULL;
+static int g_bkl_cap = 0, g_bkl_n = 0;
+
+void qxo_bkl_trace(double *rec, double *vec, int cap)
+{
+  g_bkl_rec = rec;
+  g_bkl_vec = vec;
+  g_bkl_cap = rec ? cap : 0;
+  g_bkl_n = 0;
+}
+
+int qxo_bkl_trace_count(void) { return g_bkl_n; }
+
 /* eigen_BKL with lortho=.false. and lhcunt=.true., the form the SCF driver
  * calls (Sources/pwlda.f90:2940, 3121; Sources/eigen.f90:199-938).  gdcr and
  * the all-band copy cgjr coincide on one FP64 rank, so one array serves both. */
@@ -1824,6 +1844,16 @@ void qxo_eigen_bkl(const qxo_ham *h, double *eig, double *gdcr, double *rhcr,
       hpsi_one(h, hnk, gnk, fx, fy);
       cbi = pdot(hnk, gnk, n);
       hhi = pdot(hnk, hnk, n);
+      const double h11_trace = eig[ibb];
+      double *rec = NULL;
+      if (g_bkl_rec && g_bkl_n < g_bkl_cap) {
+        rec = g_bkl_rec + (size_t)QXO_BKL_TRACE_W * g_bkl_n;
+        if (g_bkl_vec) {
+          memcpy(g_bkl_vec + 2 * (size_t)n * g_bkl_n, x, sizeof(double) * (size_t)n);
+          memcpy(g_bkl_vec + (2 * (size_t)g_bkl_n + 1) * n, hnk, sizeof(double) * (size_t)n);
+        }
+        ++g_bkl_n;
+      }
       if (hhi > 0.0) {
         double h11 = eig[ibb], h12 = cai / sqrt(hhi), h21 = cai / sqrt(hhi), h22 = cbi / hhi;
         double enew = 0.5 * (h11 + h22 - sqrt((h11 + h22) * (h11 + h22) - 4.0 * (h11 * h22 - h12 * h21)));
@@ -1859,10 +1889,19 @@ void qxo_eigen_bkl(const qxo_ham *h, double *eig, double *gdcr, double *rhcr,
       for (ig = 0; ig < n; ++ig) hx[ig] = alpsq * (hx[ig] + dtnew * gnk[ig]);
       for (ig = 0; ig < n; ++ig) gnk[ig] = eig[ibb] * x[ig] - hx[ig];
       zansa = pdot(gnk, gnk, n);
-      if ((ib <= noband && iter >= itermx) || (ib > noband && iter >= iteremx) ||
-          zansa < tolera || fabs(eigb - eig[ibb]) < tolera || zansa < z000re * roughfzan ||
-          (iter > 1 && fabs(eigb - eig[ibb]) < x000re * roughfeig))
-        break;
+      {
+        const int stop = (ib <= noband && iter >= itermx) || (ib > noband && iter >= iteremx) ||
+                         zansa < tolera || fabs(eigb - eig[ibb]) < tolera ||
+                         zansa < z000re * roughfzan ||
+                         (iter > 1 && fabs(eigb - eig[ibb]) < x000re * roughfeig);
+        if (rec) {
+          const double r[QXO_BKL_TRACE_W] = {ib, iter, h11_trace, zansap, beta, cai, cbi, hhi,
+                                             eig[ibb], dtnew, alpsq, zansa, hhi > 0.0, stop,
+                                             eigb, z000re, x000re};
+          memcpy(rec, r, sizeof(r));
+        }
+        if (stop) break;
+      }
       if (iter == 1) x000re = fabs(eigb - eig[ibb]);
       if (zansa > zansab) {
         ++izcnt;

@@ -129,6 +129,10 @@ typedef struct {
 } qxo_ham;
 
 void qxo_hpsi(const qxo_ham *h, const double *psi, double *hpsi, int nb, double *slmir);
+/* test hook: per-iteration trace of qxo_eigen_bkl (see qxoracle.c) */
+#define QXO_BKL_TRACE_W 17
+void qxo_bkl_trace(double *rec, double *vec, int cap);
+int qxo_bkl_trace_count(void);
 void qxo_eigen_bkl(const qxo_ham *h, double *eig, double *gdcr, double *rhcr,
                    const double *ekib, int nband, int noband, int itermx, int iteremx,
                    double tolera, double threinn, double wdinner, double roughfzan,

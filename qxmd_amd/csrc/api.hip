@@ -63,6 +63,30 @@ __global__ void k_to_reference(const double* __restrict__ in, double* __restrict
 
 static unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
+// Argument check shared by the slot entry points, run before anything touches
+// the device: every slot index names one of the kSlots band arrays, 1 <= nb <=
+// nbmax, the slots a routine reads and writes (a, b) are distinct, `staged`
+// refuses slot 5 (the routine stages its result there) and `cg_work` refuses
+// slots 2-4 (the CG's g, h and H h).
+constexpr int kSlotStaged = 1, kSlotCgWork = 2, kSlotPair01 = 4;
+static int check_slots(const Ctx* c, const char* who, int nb, int a, int b, int rules) {
+  auto bad_slot = [&](int s) {
+    if (s < 0 || s >= Ctx::kSlots) return true;
+    if ((rules & kSlotStaged) && s == 5) return true;
+    if ((rules & kSlotCgWork) && s >= 2 && s <= 4) return true;
+    if ((rules & kSlotPair01) && s > 1) return true;
+    return false;
+  };
+  if (bad_slot(a) || (b != -1 && bad_slot(b)))
+    return fail(-1, "%s: slot out of range or reserved (slots 0..%d%s%s%s)", who, Ctx::kSlots - 1,
+                (rules & kSlotPair01) ? "; only 0 and 1 here" : "",
+                (rules & kSlotCgWork) ? "; 2-4 are the CG work space" : "",
+                (rules & kSlotStaged) ? "; 5 is the staging slot" : "");
+  if (b != -1 && a == b) return fail(-1, "%s: the two slots must differ", who);
+  if (nb < 1 || nb > c->nbmax) return fail(-1, "%s: need 1 <= nb <= nbmax (%d)", who, c->nbmax);
+  return 0;
+}
+
 // acc += (T+V) psi - T psi  (the local part alone, hcnonloc)
 __global__ void k_add_minus_kin(double* acc, const double* tv, const double* psi, const double* kin,
                                 int n) {
@@ -289,6 +313,11 @@ int qx_destroy(qx_ctx* h) {
   return 0;
 }
 
+int qx_shutdown(void) {
+  dropin_release();
+  return 0;
+}
+
 int qx_sync(qx_ctx* h) {
   QX_CHECK(hipStreamSynchronize(h->c.stream));
   return 0;
@@ -422,8 +451,9 @@ int qx_get_kb(qx_ctx* h, double* eqlmr, double* vclalp_atom) {
 }
 
 int qx_slot_put(qx_ctx* h, int slot, const double* gdcr, int nb) {
+  if (!h) return fail(-1, "qx_slot_put: null context");
   Ctx* c = &h->c;
-  if (slot < 0 || slot > 1 || nb > c->nbmax) return fail(-1, "qx_slot_put: slot 0/1, nb <= nbmax");
+  if (int e = check_slots(c, "qx_slot_put", nb, slot, -1, kSlotPair01)) return e;
   int rc = put_internal(c, c->d_slot[slot], gdcr, nb);
   if (rc) return rc;
   QX_CHECK(hipStreamSynchronize(c->stream));
@@ -431,30 +461,39 @@ int qx_slot_put(qx_ctx* h, int slot, const double* gdcr, int nb) {
 }
 
 int qx_slot_get(qx_ctx* h, int slot, double* gdcr, int nb) {
+  if (!h) return fail(-1, "qx_slot_get: null context");
   Ctx* c = &h->c;
-  if (slot < 0 || slot > 4 || nb > c->nbmax) return fail(-1, "qx_slot_get: bad slot/nb");
+  if (int e = check_slots(c, "qx_slot_get", nb, slot, -1, kSlotStaged)) return e;
   return get_reference(c, c->d_slot[slot], gdcr, nb);
 }
 
 int qx_slot_copy(qx_ctx* h, int src, int dst, int nb) {
+  if (!h) return fail(-1, "qx_slot_copy: null context");
   Ctx* c = &h->c;
+  if (int e = check_slots(c, "qx_slot_copy", nb, src, dst, 0)) return e;
   QX_CHECK(hipMemcpyAsync(c->d_slot[dst], c->d_slot[src], sizeof(double) * c->nplwex * nb,
                           hipMemcpyDeviceToDevice, c->stream));
   return 0;
 }
 
 int qx_hpsi_slot(qx_ctx* h, int src, int dst, int nb, int prec) {
+  if (!h) return fail(-1, "qx_hpsi_slot: null context");
   Ctx* c = &h->c;
+  if (int e = check_slots(c, "qx_hpsi_slot", nb, src, dst, 0)) return e;
   return hpsi_batched(c, c->d_slot[src], c->d_slot[dst], nb, prec);
 }
 
 int qx_local_slot(qx_ctx* h, int src, int dst, int nb, int prec, int accumulate) {
+  if (!h) return fail(-1, "qx_local_slot: null context");
   Ctx* c = &h->c;
+  if (int e = check_slots(c, "qx_local_slot", nb, src, dst, 0)) return e;
   return local_apply(c, c->d_slot[src], c->d_slot[dst], nb, prec, accumulate);
 }
 
 int qx_kb_slot(qx_ctx* h, int src, int dst, int nb, double* slmir_host) {
+  if (!h) return fail(-1, "qx_kb_slot: null context");
   Ctx* c = &h->c;
+  if (int e = check_slots(c, "qx_kb_slot", nb, src, dst, 0)) return e;
   int rc = kb_apply(c, c->d_slot[src], c->d_slot[dst], nb, 1, nullptr, PREC_FP64);
   if (rc) return rc;
   (void)slmir_host;
@@ -462,18 +501,24 @@ int qx_kb_slot(qx_ctx* h, int src, int dst, int nb, double* slmir_host) {
 }
 
 int qx_schmidt_slot(qx_ctx* h, int slot, int nb, int* lcholesky) {
+  if (!h) return fail(-1, "qx_schmidt_slot: null context");
   Ctx* c = &h->c;
+  if (int e = check_slots(c, "qx_schmidt_slot", nb, slot, -1, kSlotStaged)) return e;
   return schmidt_dev(c, c->d_slot[slot], nb, lcholesky);
 }
 
 int qx_rotate_slot(qx_ctx* h, int psi, int hpsi, int nb, double* eig) {
+  if (!h) return fail(-1, "qx_rotate_slot: null context");
   Ctx* c = &h->c;
+  if (int e = check_slots(c, "qx_rotate_slot", nb, psi, hpsi, kSlotStaged)) return e;
   return rotate_dev(c, c->d_slot[psi], c->d_slot[hpsi], nb, eig);
 }
 
 int qx_density_slot(qx_ctx* h, int slot, int nb, const double* occ, double rvol, double* rho,
                     int prec) {
+  if (!h) return fail(-1, "qx_density_slot: null context");
   Ctx* c = &h->c;
+  if (int e = check_slots(c, "qx_density_slot", nb, slot, -1, 0)) return e;
   double* w = c->d_scal + 41 * (size_t)c->nbmax;
   std::vector<double> wh(nb + 1, 0.0);
   for (int b = 0; b < nb; ++b) wh[b] = occ[b] / rvol;
@@ -514,7 +559,9 @@ int qx_eigen_bkl_batched_slot(qx_ctx* h, int psi, int hpsi, int nb, const double
 }
 
 int qx_ekib_slot(qx_ctx* h, int slot, int nb, double* ekib_host) {
+  if (!h) return fail(-1, "qx_ekib_slot: null context");
   Ctx* c = &h->c;
+  if (int e = check_slots(c, "qx_ekib_slot", nb, slot, -1, kSlotStaged)) return e;
   const int n = c->nplwex;
   double* t = c->d_slot[5];
   double* d = c->d_scal + 42 * (size_t)c->nbmax;
@@ -879,8 +926,9 @@ int qx_aspc_init(qx_ctx* h, int nprvmx, int ihest) {
 }
 
 int qx_sbalinaspc_slot(qx_ctx* h, int slot, int nb, int nstep, int nstep_ini, int* info) {
+  if (!h) return fail(-1, "qx_sbalinaspc_slot: null context");
   Ctx* c = &h->c;
-  if (slot < 0 || slot > 1) return fail(-1, "qx_sbalinaspc_slot: slot 0 or 1");
+  if (int e = check_slots(c, "qx_sbalinaspc_slot", nb, slot, -1, kSlotPair01)) return e;
   int iaspc = 0, lchol = 1;
   int rc = aspc_step(c, c->d_slot[slot], nb, nstep, nstep_ini, &iaspc, &lchol);
   if (rc == -21)
@@ -894,8 +942,9 @@ int qx_sbalinaspc_slot(qx_ctx* h, int slot, int nb, int nstep, int nstep_ini, in
 }
 
 int qx_roteig_slot(qx_ctx* h, int slot, int nb, int kpwvx) {
+  if (!h) return fail(-1, "qx_roteig_slot: null context");
   Ctx* c = &h->c;
-  if (slot < 0 || slot > 1) return fail(-1, "qx_roteig_slot: slot 0 or 1");
+  if (int e = check_slots(c, "qx_roteig_slot", nb, slot, -1, kSlotPair01)) return e;
   int rc = aspc_roteig(c, c->d_slot[slot], nb, kpwvx);
   if (rc == -21) return fail(rc, "qx_roteig_slot: A^T A is not positive definite");
   if (rc) return fail(rc, "qx_roteig_slot: call qx_aspc_init first (%d)", rc);

@@ -40,6 +40,7 @@ int fg_transform(FullGrid& f, hipStream_t st, int sign, double scale);
 // the grid and stream of the context-free drop-ins (rfft3, fft3, harting, ...)
 int dropin_grid(int n1, int n2, int n3, FullGrid** f, hipStream_t* st);
 int dropin_stream(hipStream_t* st);
+void dropin_release();
 
 struct Ctx {
   int dev = 0;

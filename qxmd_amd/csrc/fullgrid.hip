@@ -177,6 +177,15 @@ int dropin_stream(hipStream_t* st) {
   return 0;
 }
 
+// releases the drop-ins' grid and stream (qx_shutdown); the next drop-in
+// call creates them again
+void dropin_release() {
+  if (g_fg_stream) (void)hipStreamSynchronize(g_fg_stream);
+  fg_release(g_fg);
+  if (g_fg_stream) (void)hipStreamDestroy(g_fg_stream);
+  g_fg_stream = nullptr;
+}
+
 int dropin_grid(int n1, int n2, int n3, FullGrid** f, hipStream_t* st) {
   int rc = dropin_stream(st);
   if (rc) return rc;

@@ -61,6 +61,10 @@ integer(c_int) function qx_destroy(ctx) bind(C, name="qx_destroy")
   type(c_ptr), value :: ctx
 end function
 
+integer(c_int) function qx_shutdown() bind(C, name="qx_shutdown")
+  import :: c_int
+end function
+
 integer(c_int) function qx_sync(ctx) bind(C, name="qx_sync")
   import :: c_int, c_ptr
   type(c_ptr), value :: ctx

@@ -110,11 +110,13 @@ subroutine qxmd_gpu_init_cell(hcell, ecut, ecutdens, nbmax, nd, nplw)
 &                    rec(1:nplw + 1), nbmax)
 end subroutine
 
+! releases the context and the drop-ins' process-wide device resources
 subroutine qxmd_gpu_finalize()
   integer(c_int) :: rc
   if (c_associated(qxctx)) rc = qx_destroy(qxctx)
   qxctx = c_null_ptr
   qx_nplw = -1
+  rc = qx_shutdown()
 end subroutine
 
 ! glocal = V_H + V_xc + V_loc on the FFT grid, Ry (Sources/engrad.f90:487-495)

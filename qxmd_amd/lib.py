@@ -6,8 +6,10 @@ a machine without a HIP device fails with QxError.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -22,6 +24,9 @@ class QxError(RuntimeError):
     pass
 
 
+_LIVE = weakref.WeakSet()  # open device contexts (PlaneWave), closed at exit
+
+
 def load():
     global _LIB
     if _LIB is None:
@@ -29,7 +34,26 @@ def load():
             raise QxError(f"{SO} is missing: build it with `python -m qxmd_amd.build`")
         _LIB = C.CDLL(SO)
         _declare(_LIB)
+        atexit.register(_teardown)
     return _LIB
+
+
+def track(ctx):
+    """Register an object with a close() method to be closed at interpreter
+    exit, before the HIP runtime's own teardown."""
+    _LIVE.add(ctx)
+
+
+def _teardown():
+    """Release every open context and the drop-ins' process-wide device
+    resources while the HIP runtime is still up."""
+    for ctx in list(_LIVE):
+        try:
+            ctx.close()
+        except Exception:
+            pass
+    if _LIB is not None:
+        _LIB.qx_shutdown()
 
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -47,6 +71,7 @@ SIGNATURES = {
     "qx_create": [C.POINTER(_vp), _i, _i, _i, _i, _ip, _ip, _ip, _dp, _i],
     "qx_destroy": [_vp],
     "qx_sync": [_vp],
+    "qx_shutdown": [],
     "qx_set_glocal": [_vp, _dp],
     "qx_set_kb": [_vp, _dp, _i, _i, _ip, _dp],
     "qx_info": [_vp, _ip],

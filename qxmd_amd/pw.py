@@ -11,7 +11,7 @@ import ctypes as C
 
 import numpy as np
 
-from .lib import QxError, check, f64, i32, load
+from .lib import QxError, check, f64, i32, load, track
 
 FP64, MIXED = 0, 1
 
@@ -71,6 +71,7 @@ class PlaneWave:
                           self.nbmax), "qx_create")
         self.h = h
         self.npj = 0
+        track(self)
 
     def close(self):
         if self.h:

@@ -211,10 +211,11 @@ def test_multi_batch_pencil(si64, monkeypatch):
 
 
 @pytest.mark.parametrize("path", ["coop", "legacy"])
-@pytest.mark.parametrize("n", [1, 2, 7, 64, 65, 129, 200, 540])
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 65, 129, 200, 540, 1280])
 def test_icholesky(si8, n, path, monkeypatch):
     """Cholesky + inverse: the one-launch cooperative kernel and the
-    multi-launch fallback against rsdecmp/rsinvcd."""
+    multi-launch fallback against rsdecmp/rsinvcd (n = 1280: the Si512 band
+    count)."""
     if path == "legacy":
         monkeypatch.setenv("QX_DENSE_LEGACY", "1")
     gs, ctx, H, v = _setup(si8, max(n, 2), kb=False)
@@ -265,10 +266,12 @@ def test_rseigqr(si8, case, path, monkeypatch):
     assert np.abs(a @ z - z * w).max() < 1e-10 * np.abs(wr).max()
 
 
-@pytest.mark.parametrize("n", [3, 4, 5, 17, 64, 65, 300, 540])
+@pytest.mark.parametrize("n", [3, 4, 5, 17, 64, 65, 300, 540, 640, 641, 1280])
 def test_rseigqr_sizes(si8, n):
     """The cooperative eigensolver over the band counts the workloads use
-    (20, 160, 540) and odd sizes around the 64-row tiles."""
+    (20, 160, 540, 1280), odd sizes around the 64-row tiles, and both sides
+    of the switch from four to eight rows per workgroup in the
+    tridiagonalisation (n = 640 / 641)."""
     gs, ctx, H, v = _setup(si8, max(n, 4), kb=False)
     rng = np.random.default_rng(100 + n)
     x = rng.standard_normal((n, n))
