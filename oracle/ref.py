@@ -413,3 +413,93 @@ def rciprl(hcell):
     v = C.c_double(0.0)
     L.qxref_rciprl(np.ascontiguousarray(np.asarray(hcell, np.float64).T), b, C.byref(v))
     return b.reshape(3, 3).T.copy(), v.value
+
+
+def _ncpp_child(workdir):
+    """Child-process side of ncpp(): reads ncpp_in.npz in workdir, runs
+    qxref_ncpp there and writes ncpp_out.npz."""
+    os.chdir(workdir)
+    a = np.load("ncpp_in.npz")
+    gx, gy, gz, recnrm = a["gx"], a["gy"], a["gz"], a["recnrm"]
+    recnrm5, recnrmex = a["recnrm5"], a["recnrmex"]
+    L = lib()
+    nplw, nplw5 = len(gx) - 1, len(recnrm5) - 1
+    lmcap, lncap, nqcap = 16, 8, 16
+    L.qxref_ncpp.argtypes = [C.c_double, C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, _dp, C.c_int,
+                             _dp, _dp, C.c_int, C.c_int, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp,
+                             _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp]
+    dims = np.zeros(8, np.int32)
+    zv = np.zeros(1)
+    clalp = np.zeros(lmcap)
+    tb = np.zeros((551, lncap), order="F")
+    tba = np.zeros((551, lncap), order="F")
+    dl = np.zeros(1)
+    ito = [np.zeros(lmcap, np.int32) for _ in range(4)]
+    qr = np.zeros((nplw + 1, nqcap), order="F")
+    qi = np.zeros((nplw + 1, nqcap), order="F")
+    svco = np.zeros(nplw5 + 1)
+    rc = np.zeros(1001)
+    rca = np.zeros(1001)
+    L.qxref_ncpp(float(a["zatom"]), int(a["jgga"]), float(a["ecutdens"]), nplw, _F(gx), _F(gy),
+                 _F(gz), _F(recnrm), nplw5, _F(recnrm5), _F(recnrmex), lmcap, lncap, nqcap, dims,
+                 zv, clalp, tb, tba, dl, *ito, qr, qi, svco, rc, rca)
+    lmmax, lnref, nqlmmx = int(dims[0]), int(dims[1]), int(dims[2])
+    if dims[7] != 0:
+        raise RuntimeError(f"qxref_ncpp failed: dims {dims.tolist()}")
+    np.savez("ncpp_out.npz", zv=zv[0], lmmax=lmmax, lnref=lnref, nqlmmx=nqlmmx, lmx=dims[3],
+             mxl=dims[4], lmax=dims[5], lclno=dims[6], clalp=clalp[:lmmax],
+             tbqlm=np.ascontiguousarray(tb[:, :lnref].T),
+             tbqlma=np.ascontiguousarray(tba[:, :lnref].T), dlqlm=dl[0], itola=ito[0][:lmmax],
+             itolm=ito[1][:lmmax], itolmf=ito[2][:lmmax], litoll=ito[3][:lmmax],
+             qlmr=np.ascontiguousarray(qr[:, :nqlmmx].T),
+             qlmi=np.ascontiguousarray(qi[:, :nqlmmx].T), svco=svco, rhocr=rc, rhocra=rca)
+
+
+def ncpp(zatom, element, ppdir, jgga, ecutdens, gx, gy, gz, recnrm, recnrm5):
+    """The reference's set-up of one norm-conserving pseudopotential (read_pp,
+    rdtmpp, lotmpp, lotmpp_g, nlkbpp_g through qxref_ncpp) on the given
+    wave-function sphere (gx, gy, gz, recnrm over 0..nplw) and density sphere
+    (recnrm5 over 0..nplw5), with recnrmex = 4 pi / |G|^2 as set_recnrmex
+    makes it.  ppdir holds the element's files (<EL>_A.rvs, ...); they are
+    copied under control/NCPP/<EL>/ of a scratch directory, where the
+    reference's openpp looks for LDA (jgga = 1) data, and for jgga = 1 the
+    functional marker of <EL>_A.rvs is set to -1 (the reference stops on
+    another marker with LDA, Sources/ppkb.f90:1181-1189).  Runs in a child
+    process (nlkbpp_g keeps the table range of its first call)."""
+    import re
+    import shutil
+    import subprocess
+    import sys
+    import tempfile
+    recnrm5 = np.asarray(recnrm5, np.float64)
+    pi4 = 4.0 * np.arccos(-1.0)
+    recnrmex = np.zeros_like(recnrm5)
+    recnrmex[1:] = pi4 / recnrm5[1:]
+    with tempfile.TemporaryDirectory() as tmp:
+        sub = "" if jgga == 1 else (".PBE" if jgga == 2 else ".RPBE")
+        dst = os.path.join(tmp, "control", "NCPP", element.upper() + sub)
+        shutil.copytree(ppdir, dst)
+        if jgga == 1:
+            a = os.path.join(dst, f"{element.upper()}_A.rvs")
+            os.chmod(a, 0o644)
+            lines = open(a).read().split("\n")
+            f = lines[0].split()
+            f[2] = "-1"
+            lines[0] = "  " + "   ".join(f)
+            open(a, "w").write("\n".join(lines))
+            assert re.match(r"\s*\S+\s+\S+\s+-1\s+\S+", lines[0])
+        np.savez(os.path.join(tmp, "ncpp_in.npz"), zatom=float(zatom), jgga=int(jgga),
+                 ecutdens=float(ecutdens), gx=np.asarray(gx, np.float64),
+                 gy=np.asarray(gy, np.float64), gz=np.asarray(gz, np.float64),
+                 recnrm=np.asarray(recnrm, np.float64), recnrm5=recnrm5, recnrmex=recnrmex)
+        root = os.path.dirname(_HERE)
+        code = (f"import sys; sys.path.insert(0, {root!r}); import oracle.ref as R; "
+                f"R._ncpp_child({tmp!r})")
+        r = subprocess.run([sys.executable, "-c", code], stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError("qxref_ncpp child failed:\n" + r.stdout[-3000:])
+        o = np.load(os.path.join(tmp, "ncpp_out.npz"))
+        out = {k: (o[k].item() if o[k].ndim == 0 else o[k].copy()) for k in o.files}
+    out["recnrmex"] = recnrmex
+    return out

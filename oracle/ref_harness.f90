@@ -818,3 +818,126 @@ subroutine qxref_rciprl(a, b, v) bind(C, name="qxref_rciprl")
   real(c_double) :: a(3, 3), b(3, 3), v
   call rciprl(a, b, v)
 end subroutine
+
+!-----------------------------------------------------------------------
+! Norm-conserving pseudopotential of one species, set up as the serial
+! reference's initialisation does it: read_pp (pptmpp, mxl_alloc,
+! set_nlpp_parameters), then setpp's steps for a KB pseudopotential in
+! reciprocal space without core correction or smoothing (rdtmpp, lotmpp,
+! lotmpp_g, nlkbpp_g; Sources/input.f90:5240-5330, Sources/ppkb.f90:816-1120).
+! The files are opened by the reference's openpp under the current
+! directory (control/NCPP/<element>/ for LDA).  Inputs: the wave-function
+! sphere gx, gy, gz, recnrm(0:nplw) and the density sphere recnrm5,
+! recnrmex(0:nplw5).  Outputs, in the caller's capacities (lmcap projector
+! rows, lncap radial rows, nqcap angular rows): the nlpp_variables tables
+! setqlm reads, svco(0:nplw5) of lopp_variables and lotmpp_g's radial table
+! rhocr/rhocra(0:1000).  Call once per process: nlkbpp_g keeps the table
+! range of its first call.
+!-----------------------------------------------------------------------
+subroutine qxref_ncpp(zatom_in, jgga_in, ecutdens_in, nplw, gx, gy, gz, recnrm, nplw5, recnrm5, &
+&   recnrmex, lmcap, lncap, nqcap, dims, zv_o, clalp_o, tbqlm_o, tbqlma_o, dlqlm_o, itola_o, &
+&   itolm_o, itolmf_o, litoll_o, qlmr_o, qlmi_o, svco_o, rhocr_o, rhocra_o) &
+&   bind(C, name="qxref_ncpp")
+  use iso_c_binding
+  use param, only: ntype, mxl, jgga, lrela, mx1, mx1loc, pwscale
+  use param_atom, only: zatom, lkbpp, lvand, lkbppi, lvandi, lvflag, lking, llking, rctflc, &
+&     lpcci, rpcc, lpking, rpking, gpkgmax, gpkgexct, zv, lmax, lclno, lchk, lsomax
+  use constants, only: ms, aname
+  use nlpp_parameters, only: lmmax, lnref, nqlmmx
+  use nlpp_variables, only: lmx, clalp, tbqlm, tbqlma, dlqlm, itola, itolm, itolmf, litoll, &
+&     qlmr, qlmi, mxqlm
+  use lopp_variables, only: svco
+  use for_lotmpp_g, only: rhocr, rhocra, mxqab_lc => mxqab
+  use outfile, only: loutfile
+  implicit none
+  real(c_double), value :: zatom_in, ecutdens_in
+  integer(c_int), value :: jgga_in, nplw, nplw5, lmcap, lncap, nqcap
+  real(c_double) :: gx(0:nplw), gy(0:nplw), gz(0:nplw), recnrm(0:nplw)
+  real(c_double) :: recnrm5(0:nplw5), recnrmex(0:nplw5)
+  integer(c_int) :: dims(8)
+  real(c_double) :: zv_o, clalp_o(lmcap), dlqlm_o
+  real(c_double) :: tbqlm_o(0:550, lncap), tbqlma_o(0:550, lncap)
+  integer(c_int) :: itola_o(lmcap), itolm_o(lmcap), itolmf_o(lmcap), litoll_o(lmcap)
+  real(c_double) :: qlmr_o(0:nplw, nqcap), qlmi_o(0:nplw, nqcap)
+  real(c_double) :: svco_o(0:nplw5), rhocr_o(0:1000), rhocra_o(0:1000)
+  integer :: nfile(2), myid, nodes, nvlcl, nplw_dummy(1), ierror, nplw5ex
+  real*8 :: alloc_mem, ecutdens
+  real*8, allocatable :: vlocli(:), xitgrd(:), rr(:)
+  real*8, allocatable :: tablc(:, :), tablca(:, :), tbflc(:, :), tbflca(:, :)
+  real*8 :: dltlc(1), rmxlc(1), dltflc(1), rmxflc(1)
+  nfile = 6
+  myid = 0
+  nodes = 1
+  loutfile(1) = .true.   ! the reference's log lines go to the child's stdout
+  call qxref_serial_world()
+  alloc_mem = 0.d0
+  ecutdens = ecutdens_in
+  ntype = 1
+  jgga = jgga_in
+  lrela = .false.
+  lkbpp = .true.
+  lvand = .false.
+  allocate(zatom(1), lkbppi(1), lvandi(1), lvflag(1), lking(1), llking(1), lpcci(1), rpcc(1), &
+&          lpking(1), rpking(1), gpkgmax(1), gpkgexct(1), zv(1), lmax(1), lclno(1), lsomax(1))
+  zatom(1) = zatom_in
+  lkbppi = .true.
+  lvandi = .false.
+  lvflag = .false.
+  lking = .false.
+  llking = .false.
+  lpcci = .false.
+  lpking = .false.
+  rpcc = 0.d0
+  rpking = 0.d0
+  gpkgmax = 0.d0
+  gpkgexct = 0.d0
+  call psvariables_alloc(nfile, myid, nodes, alloc_mem, ntype)
+  call read_pp(nfile, myid, nodes)
+  call planewave_decomp_variables_alloc(nfile, alloc_mem)
+  nplw5ex = 2 * (nplw5 + 1)
+  call planewave_decomp(nfile, nplw5ex, nplw5)
+  call nlpp_variables_alloc(nfile, myid, nodes, alloc_mem, ntype, mxl, nplw, pwscale)
+  call nlppg_variables_alloc(nfile, myid, nodes, alloc_mem, ntype, mxl, nplw, recnrm, pwscale)
+  nvlcl = max(mx1, mx1loc)
+  allocate(vlocli(0:nvlcl), xitgrd(0:nvlcl), rr(0:nvlcl))
+  allocate(tablc(0:mx1loc, 1), tablca(0:mx1loc, 1), tbflc(0:mx1loc, 1), tbflca(0:mx1loc, 1))
+  call rdtmpp(nfile, myid, nodes, jgga, lrela, ntype, zatom, zv, lmax, lclno, lchk, ecutdens, &
+&   lkbppi, lpcci, rpcc, lpking, rpking, gpkgmax, gpkgexct, ms, aname, mxl, mx1, vlocli, xitgrd, &
+&   rr, mx1)
+  call lotmpp(nfile, myid, nodes, ntype, zv, rctflc, llking, tablc, tablca, dltlc, rmxlc, &
+&   tbflc, tbflca, dltflc, rmxflc, mx1loc)
+  call lopp_variables_alloc(nfile, myid, nodes, alloc_mem, ntype, nplw5, .false., pwscale)
+  call lotmpp_g(nfile, myid, nodes, ntype, zv, llking, dltlc, rmxlc, recnrm5, recnrmex, nplw5, &
+&   vlocli, xitgrd, rr, nvlcl, .false., .true.)
+  nplw_dummy(1) = nplw
+  call nlkbpp_g(nfile, myid, nodes, jgga, lrela, ntype, zatom, zv, lmax, lclno, lchk, lkbppi, &
+&   lking, ms, aname, mxl, nplw_dummy, gx, gy, gz, nplw, vlocli, xitgrd, rr, mx1, ierror)
+  dims = 0
+  dims(1) = lmmax
+  dims(2) = lnref
+  dims(3) = nqlmmx
+  dims(4) = lmx(1)
+  dims(5) = mxl
+  dims(6) = lmax(1)
+  dims(7) = lclno(1)
+  dims(8) = ierror
+  if (lmmax > lmcap .or. lnref > lncap .or. nqlmmx > nqcap .or. mxqlm /= 550 .or. &
+&     mxqab_lc /= 1000) then
+    dims(8) = -1
+    return
+  end if
+  zv_o = zv(1)
+  clalp_o(1:lmmax) = clalp(1:lmmax, 1)
+  tbqlm_o(:, 1:lnref) = tbqlm(:, 1:lnref, 1)
+  tbqlma_o(:, 1:lnref) = tbqlma(:, 1:lnref, 1)
+  dlqlm_o = dlqlm(1)
+  itola_o(1:lmmax) = itola(1:lmmax, 1)
+  itolm_o(1:lmmax) = itolm(1:lmmax, 1)
+  itolmf_o(1:lmmax) = itolmf(1:lmmax, 1)
+  litoll_o(1:lmmax) = merge(1, 0, litoll(1:lmmax, 1))
+  qlmr_o(:, 1:nqlmmx) = qlmr(0:nplw, 1:nqlmmx, 1, 1)
+  qlmi_o(:, 1:nqlmmx) = qlmi(0:nplw, 1:nqlmmx, 1, 1)
+  svco_o(0:nplw5) = svco(1:nplw5 + 1, 1)
+  rhocr_o = rhocr(0:1000)
+  rhocra_o = rhocra(0:1000)
+end subroutine
