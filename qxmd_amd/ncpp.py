@@ -136,3 +136,15 @@ def si_lda(fixture, tag):
                 litoll=f["litoll"], tbqlm=f[f"{tag}_tbqlm"], tbqlma=f[f"{tag}_tbqlma"],
                 dlqlm=float(f[f"{tag}_dlqlm"]), rhocr=f[f"{tag}_rhocr"],
                 rhocra=f[f"{tag}_rhocra"])
+
+
+def silicon_cell_inputs(gs, fixture, tag):
+    """Everything the SCF needs from the silicon pseudopotential for one
+    plane-wave sphere `gs` (qxmd_amd.pw.GSphere): the NlppTables of
+    qx_set_nlpp_tables, the local form factors svco (1, nplw5 + 1) and the
+    valence charges zv (1,)."""
+    sp = si_lda(fixture, tag)
+    nw = gs.nplw + 1
+    t = nlpp_tables(gs.gx[:nw], gs.gy[:nw], gs.gz[:nw], [sp])
+    svco = form_factors(gs.recnrm[: gs.nplw5 + 1], sp["rhocr"], sp["rhocra"], sp["zv"])
+    return t, svco[None, :], np.array([sp["zv"]])
