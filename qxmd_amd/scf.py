@@ -87,24 +87,38 @@ class Scf:
                 + en["ecor"] - en["evexc"] - en["evcor"])
 
     def iterate(self, occ, tol_ha_atom=1e-6, max_scf=60, prec=pw.FP64, batched=True,
-                reset_history=True):
+                reset_history=True, refine_ha_atom=None):
         """SCF iterations until the Harris-Foulkes energy changes by less than
         tol_ha_atom (Hartree per atom).  Returns (iterations, converged,
-        energy history in Ry)."""
+        energy history in Ry).
+
+        refine_ha_atom (mixed precision only): once the energy changes by less
+        than this, the remaining band steps run in FP64 -- the final FP64
+        refinement sweep -- and convergence is only declared on an FP64 step,
+        so the converged bands, energy and forces carry FP64 accuracy.
+        ``self.fp64_steps`` counts the refinement steps taken."""
         ctx = self.ctx
         nb = len(occ)
         hist = []
+        self.fp64_steps = 0
+        cur = prec
         for it in range(max_scf):
-            eig, _, ok = ctx.scf_band_step(nb, occ, self.gs.volume, prec=prec, batched=batched,
+            eig, _, ok = ctx.scf_band_step(nb, occ, self.gs.volume, prec=cur, batched=batched,
                                            want_rho=False)
             if not ok:
                 raise RuntimeError("schmidt: overlap not positive definite")
+            if cur == pw.FP64 and prec != pw.FP64:
+                self.fp64_steps += 1
             ctx.outchg()
             ctx.pwpmxchgg(self.aslh, self.bslh, lreset=(reset_history and it == 0))
             self.energies = en = ctx.scf_potential()
             hist.append(self.harris_foulkes(occ, eig, en))
-            if len(hist) > 1 and abs(hist[-1] - hist[-2]) / 2.0 / self.natom < tol_ha_atom:
-                return it + 1, True, hist
+            if len(hist) > 1:
+                de = abs(hist[-1] - hist[-2]) / 2.0 / self.natom
+                if de < tol_ha_atom and (cur == pw.FP64 or refine_ha_atom is None):
+                    return it + 1, True, hist
+                if refine_ha_atom is not None and cur != pw.FP64 and de < refine_ha_atom:
+                    cur = pw.FP64
         return max_scf, False, hist
 
     def finish(self, occ):
