@@ -172,6 +172,19 @@ int qx_launch_timer(qx_ctx *ctx, int id, int mode, double *ms, int *count);
  * out, else c2r from the half grid; ierror = 0 on success. */
 int qx_rfft3(int key, double *fft3x, double *fft3y, int kfft1, int kfft2, int kfft3,
              int *ierror);
+/* rfft3 on device arrays, allocating nothing: the same transform as qx_rfft3
+ * with d_fft3x / d_fft3y (kfft1*kfft2*kfft3 doubles each) in device memory,
+ * run on `stream` (a hipStream_t; NULL = the null stream) and waited for.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (tight serving loops, harnesses that forbid hipMalloc); it waits for its
+ * stream, so it is not meant for graph capture.  Its complex work grid lives
+ * in a fixed device array of the library, one per GPU, held until the
+ * process ends: grids up to qx_noalloc_limits() points (2^21 = 128^3) and
+ * axis length; larger grids return -2.  Calls that overlap on one GPU share
+ * that array and race. */
+int qx_rfft3_dev_noalloc(int key, double *d_fft3x, double *d_fft3y, int kfft1, int kfft2,
+                         int kfft3, void *stream, int *ierror);
+int qx_noalloc_limits(long *max_points, int *max_axis);
 /* fft3 (Sources/fft3.f90:237-333): complex, key=1 forward x1/N. */
 int qx_fft3(int key, double *fft3x, double *fft3y, int kfft1, int kfft2, int kfft3, int *ierror);
 /* hcnonloc for one band with the context's glocal (Sources/engrad.f90:411-606):

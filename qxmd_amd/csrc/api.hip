@@ -16,6 +16,8 @@ void get_mesh(const double* hcell, double ecutdens, int* nd);
 void ijkgd_map(int nplw5, const int* nga, const int* ngb, const int* ngc, const int* nd, int* ijkgd);
 int fullgrid_rfft3(int key, double* fx, double* fy, int n1, int n2, int n3);
 int fullgrid_fft3(int key, double* fx, double* fy, int n1, int n2, int n3);
+int fullgrid_rfft3_dev(int key, double* dx, double* dy, int n1, int n2, int n3, hipStream_t st);
+int noalloc_limits(long* max_points, int* max_axis);
 int dense_chol_inv(Ctx* c, double* s, int n, double* x, int* ok, int* flag);
 int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int* info);
 int dense_symmetrize(Ctx* c, const double* C, int n, double* H, double* eig);
@@ -695,6 +697,27 @@ int qx_rfft3(int key, double* fft3x, double* fft3y, int k1, int k2, int k3, int*
   int rc = fullgrid_rfft3(key, fft3x, fft3y, k1, k2, k3);
   if (ierror) *ierror = rc ? 1 : 0;
   return rc ? fail(rc, "qx_rfft3: failed (%d)", rc) : 0;
+}
+
+int qx_rfft3_dev_noalloc(int key, double* d_fft3x, double* d_fft3y, int k1, int k2, int k3,
+                         void* stream, int* ierror) {
+  long maxp = 0;
+  int maxa = 0;
+  noalloc_limits(&maxp, &maxa);
+  if (k1 < 1 || k2 < 1 || k3 < 1 || k1 > maxa || k2 > maxa || k3 > maxa ||
+      (long)k1 * k2 * k3 > maxp) {
+    if (ierror) *ierror = 1;
+    return fail(-2, "qx_rfft3_dev_noalloc: grid %d x %d x %d exceeds the scratch limit "
+                "(%ld points, %d per axis)", k1, k2, k3, maxp, maxa);
+  }
+  int rc = fullgrid_rfft3_dev(key, d_fft3x, d_fft3y, k1, k2, k3,
+                              static_cast<hipStream_t>(stream));
+  if (ierror) *ierror = rc ? 1 : 0;
+  return rc ? fail(rc, "qx_rfft3_dev_noalloc: failed (%d)", rc) : 0;
+}
+
+int qx_noalloc_limits(long* max_points, int* max_axis) {
+  return noalloc_limits(max_points, max_axis);
 }
 
 int qx_fft3(int key, double* fft3x, double* fft3y, int k1, int k2, int k3, int* ierror) {

@@ -6,6 +6,8 @@
 // (y, z) lines are still read in coalesced rows.
 #include <algorithm>
 #include <cstdlib>
+#include <cmath>
+#include <mutex>
 #include "context.h"
 
 namespace qx {
@@ -218,6 +220,91 @@ int fullgrid_rfft3(int key, double* fx, double* fy, int n1, int n2, int n3) {
   }
   QX_CHECK(hipMemcpyAsync(fx, f->d_x, sizeof(double) * N, hipMemcpyDeviceToHost, st));
   QX_CHECK(hipMemcpyAsync(fy, f->d_y, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+  QX_CHECK(hipStreamSynchronize(st));
+  return 0;
+}
+
+// ---- rfft3 on device arrays without allocation ----------------------------
+// The complex work grid and the three twiddle tables live in one file-scope
+// device array, sized for grids of up to kNaMaxPoints points with each axis
+// at most kNaMaxAxis.  Every call on a GPU uses that GPU's copy; calls that
+// overlap on one GPU race.  The array's device address is looked up once per
+// device (hipGetSymbolAddress, with that device current) and kept in
+// g_na_ptr: in host code the array's name is not a device address.
+constexpr long kNaMaxPoints = 1L << 21;  // 128^3
+constexpr int kNaMaxAxis = 2 * 4 * kFgThreads;
+constexpr long kNaDoubles = 2 * kNaMaxPoints + 3L * 2 * kNaMaxAxis;
+static __device__ __align__(256) double scratch[kNaDoubles];
+
+constexpr int kNaMaxDev = 64;
+static double* g_na_ptr[kNaMaxDev] = {};
+static std::mutex g_na_mu;
+
+static int noalloc_scratch(double** p) {
+  int dev = 0;
+  QX_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNaMaxDev) return -1;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void* a = nullptr;
+    QX_CHECK(hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)));
+    g_na_ptr[dev] = static_cast<double*>(a);
+  }
+  *p = g_na_ptr[dev];
+  return 0;
+}
+
+int noalloc_limits(long* max_points, int* max_axis) {
+  *max_points = kNaMaxPoints;
+  *max_axis = kNaMaxAxis;
+  return 0;
+}
+
+int fullgrid_rfft3_dev(int key, double* dx, double* dy, int n1, int n2, int n3, hipStream_t st) {
+  const int dims[3] = {n1, n2, n3};
+  for (int a = 0; a < 3; ++a)
+    if (dims[a] < 1 || dims[a] > kNaMaxAxis) return -2;
+  const long N = (long)n1 * n2 * n3;
+  if (N > kNaMaxPoints) return -2;
+  if (!dx || !dy) return -1;
+  FullGrid f;  // views into the scratch array; nothing here is freed
+  for (int a = 0; a < 3; ++a) {
+    f.p[a] = make_plan(dims[a]);
+    if (f.p[a].npass < 0) return -6;
+  }
+  double* base = nullptr;
+  int rc = noalloc_scratch(&base);
+  if (rc) return rc;
+  f.d_g = reinterpret_cast<double2*>(base);
+  // the twiddles of fg_prepare, cos/sin of 2 pi k / n from the host libm,
+  // staged through a static host table (the call waits for the copies)
+  static double tw_host[3 * 2 * kNaMaxAxis];
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  for (int a = 0; a < 3; ++a) {
+    f.d_tw[a] = base + 2 * kNaMaxPoints + (long)a * 2 * kNaMaxAxis;
+    double* t = tw_host + (long)a * 2 * kNaMaxAxis;
+    for (int k = 0; k < dims[a]; ++k) {
+      const double ang = 2.0 * M_PI * (double)k / (double)dims[a];
+      t[2 * k] = std::cos(ang);
+      t[2 * k + 1] = std::sin(ang);
+    }
+    QX_CHECK(hipMemcpyAsync(f.d_tw[a], t, sizeof(double) * 2 * dims[a], hipMemcpyHostToDevice,
+                            st));
+  }
+  f.n1 = n1;
+  f.n2 = n2;
+  f.n3 = n3;
+  const unsigned nb = (unsigned)((N + 255) / 256);
+  if (key == 1) {
+    hipLaunchKernelGGL(k_fg_pack, dim3(nb), dim3(256), 0, st, dx, nullptr, f.d_g, N);
+    if ((rc = fg_transform(f, st, -1, 1.0 / (double)N))) return rc;
+    hipLaunchKernelGGL(k_fg_unpack, dim3(nb), dim3(256), 0, st, f.d_g, dx, dy, N, 0);
+  } else {
+    hipLaunchKernelGGL(k_fg_hermitian, dim3(nb), dim3(256), 0, st, dx, dy, f.d_g, n1, n2, n3);
+    if ((rc = fg_transform(f, st, +1, 1.0))) return rc;
+    hipLaunchKernelGGL(k_fg_unpack, dim3(nb), dim3(256), 0, st, f.d_g, dx, dy, N, 1);
+  }
+  QX_CHECK(hipGetLastError());
   QX_CHECK(hipStreamSynchronize(st));
   return 0;
 }

@@ -100,6 +100,8 @@ SIGNATURES = {
     "qx_launch_timer": [_vp, _i, _i, C.POINTER(C.c_double), C.POINTER(C.c_int)],
     "qx_rfft3": [_i, _dp, _dp, _i, _i, _i, _pi],
     "qx_fft3": [_i, _dp, _dp, _i, _i, _i, _pi],
+    "qx_rfft3_dev_noalloc": [_i, _vp, _vp, _i, _i, _i, _vp, _pi],
+    "qx_noalloc_limits": [C.POINTER(C.c_long), _pi],
     "qx_hcnonloc": [_vp, _dp, _dp],
     "qx_hckbpp_g": [_vp, _dp, _dp, _i, _vp],
     "qx_hckbpp_ib_g": [_vp, _dp, _dp],
