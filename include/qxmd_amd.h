@@ -39,6 +39,13 @@ int qx_get_mesh(const double *hcell, double ecutdens, int *nd);
  * entries (index 0 is G = 0).  Returns num or -(num+1) if cap is too small. */
 int qx_strvec(const double *rba, double ecutdens, double ecut, int cap, int *counts, int *nga,
               int *ngb, int *ngc, double *recnrm, double *gx, double *gy, double *gz);
+/* clmrcp (Sources/bulk.f90:556-1050, no double grid): the Ewald reciprocal
+ * lattice list for |G|^2 <= dkmax2, b(3,3) the reciprocal vectors without
+ * 2 pi (rciprl's output, columns); arrays of cap entries m = 1..numk2 in the
+ * reference's order, kcsma(3) the largest |nga|, |ngb|, |ngc|.  Returns
+ * numk2, or -numk2 when cap is smaller. */
+int qx_clmrcp(const double *b, double dkmax2, int cap, int *nga, int *ngb, int *ngc, double *gx,
+              double *gy, double *gz, double *recnrm, int *kcsma);
 /* reclat: ijkgd(-nplw5:nplw5), 1-based grid indices (Sources/reclat.f90:259-276). */
 int qx_ijkgd(int nplw5, const int *nga, const int *ngb, const int *ngc, const int *nd,
              int *ijkgd);
@@ -131,6 +138,12 @@ int qx_eigen_bkl_slot(qx_ctx *ctx, int psi, int hpsi, int nb, const double *ekib
                       int noband, int itermx, int iteremx, double tolera, double threinn,
                       double wdinner, double roughfzan, double roughfeig, double *bzan,
                       int *nriter, int prec);
+/* the batched CG captures one iteration as a HIP graph on its first call for
+ * a (bands, precision) and replays it for every later iteration (on by
+ * default; off = every iteration launched eagerly, the graphs released);
+ * qx_cg_graph_count returns the number of captured iterations held. */
+int qx_set_cg_graph(qx_ctx *ctx, int on);
+int qx_cg_graph_count(qx_ctx *ctx);
 /* the same CG update with all bands advanced together: every iteration is one
  * batched H|psi> over the bands still active, and the projection against the
  * other bands (subgs3, Sources/engrad.f90:1072-1142) is done against all bands
@@ -166,6 +179,14 @@ int qx_scf_phase_ms(qx_ctx *ctx, float *ms5);
  * starts the timer; mode 0 stops it and returns the summed ms and the number
  * of timed launches.  (Measurement only; no counterpart in the reference.) */
 int qx_launch_timer(qx_ctx *ctx, int id, int mode, double *ms, int *count);
+
+/* which kernels run the sizable contractions (M, N >= 256), process-wide:
+ * 0 = ours (FP64 MFMA; split bf16x3 products for the mixed form, the
+ * default), 1 = hipBLASLt (a cross-check of ours), 2 = ours with plain FP32
+ * MFMA for the mixed form.  The environment's QX_GEMM_PATH sets the
+ * default.  (No counterpart in the reference: its contractions are loops.) */
+int qx_set_gemm_path(int path);
+int qx_gemm_path(void);
 
 /* ---------------- Fortran-shaped drop-ins (host arrays) -------------------- */
 /* rfft3 (Sources/fft3.f90:113-232): key=1 r2c x1/N with full Hermitian grid

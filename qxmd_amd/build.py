@@ -21,7 +21,8 @@ OUT = os.path.join(HERE, "libqxmd_amd.so")
 BLD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("QX_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["pencil.hip", "pencil_fast.hip", "gemm.hip", "blaslt.hip", "dense.hip", "eigh.hip",
+HIP_SOURCES = ["pencil.hip", "pencil_fast.hip", "gemm.hip", "mfma_gemm.hip", "blaslt.hip", "dense.hip",
+               "eigh.hip",
                "fullgrid.hip", "cg.hip", "nlsetup.hip", "kb1.hip", "forces.hip", "flocal.hip", "ewald.hip",
                "aspc.hip", "scfglue.hip", "api.hip"]
 HOST_SOURCES = ["gsphere.cpp", "restart.cpp", "mdstep.cpp"]
@@ -104,6 +105,7 @@ def build_fortran(verbose: bool = False):
     tmp = FOUT + ".tmp"
     _run([fc, "-shared", "-o", tmp, *objs, "-L" + HERE, "-lqxmd_amd", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, FOUT)
+    build_module_dropins(fc, verbose)
     drv_src = os.path.join(ROOT, "tests", "fortran", "dropin_check.f90")
     if os.path.exists(drv_src):
         drv = os.path.join(ROOT, "tests", "fortran", "dropin_check")
@@ -111,6 +113,38 @@ def build_fortran(verbose: bool = False):
               "-lqxmd_amd", "-Wl,-rpath,$ORIGIN/../../qxmd_amd"])
         os.replace(drv + ".tmp", drv)
     return FOUT
+
+
+def build_module_dropins(fc, verbose=False):
+    """qxmd_dropin_qxmd.f90 (schmidt, pwpchg and eigen by name) reads QXMD's
+    own modules, so it is compiled inside a QXMD build.  Where the
+    reference's flang build of the checker (oracle/_ref) is present, it is
+    compiled here against that build's modules and linked into the check
+    driver tests/fortran/dropin_modules_check (the reference's module data
+    comes from oracle/_ref/libqxref.so; the executable's schmidt / pwpchg /
+    eigen and libqxmd_amd_f.so's softchg / chgr2g take precedence over the
+    reference's).  Skipped elsewhere."""
+    ref = os.path.join(ROOT, "oracle", "_ref")
+    refmod = os.path.join(ref, "build")
+    if not (os.path.exists(os.path.join(ref, "libqxref.so"))
+            and os.path.exists(os.path.join(refmod, "ifcholesky.mod"))):
+        if verbose:
+            print("oracle/_ref not built: module drop-in check skipped")
+        return None
+    mdir = os.path.join(BLD, "qxmd_modules")
+    os.makedirs(mdir, exist_ok=True)
+    obj = os.path.join(mdir, "qxmd_dropin_qxmd.o")
+    _run([fc, "-O2", "-fPIC", "-I" + refmod, "-I" + BLD, "-module-dir", mdir, "-c",
+          os.path.join(HERE, "fortran", "qxmd_dropin_qxmd.f90"), "-o", obj])
+    src = os.path.join(ROOT, "tests", "fortran", "dropin_modules_check.f90")
+    drv = os.path.join(ROOT, "tests", "fortran", "dropin_modules_check")
+    dobj = os.path.join(mdir, "dropin_modules_check.o")
+    _run([fc, "-O2", "-I" + refmod, "-I" + BLD, "-module-dir", mdir, "-c", src, "-o", dobj])
+    _run([fc, dobj, obj, "-o", drv + ".tmp", "-L" + HERE, "-lqxmd_amd_f", "-lqxmd_amd", "-L" + ref,
+          "-lqxref", "-Wl,--allow-shlib-undefined",
+          "-Wl,-rpath,$ORIGIN/../../qxmd_amd:$ORIGIN/../../oracle/_ref"])
+    os.replace(drv + ".tmp", drv)
+    return drv
 
 
 if __name__ == "__main__":

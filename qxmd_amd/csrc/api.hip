@@ -14,6 +14,8 @@ int strvec(const double* rba, double ecutdens, double ecut, int cap, int* counts
            int* ngb, int* ngc, double* recnrm, double* gx, double* gy, double* gz);
 void get_mesh(const double* hcell, double ecutdens, int* nd);
 void ijkgd_map(int nplw5, const int* nga, const int* ngb, const int* ngc, const int* nd, int* ijkgd);
+int clmrcp(const double* B, double dkmax2, int cap, int* nga, int* ngb, int* ngc, double* gx,
+           double* gy, double* gz, double* recnrm, int* kcsma);
 int fullgrid_rfft3(int key, double* fx, double* fy, int n1, int n2, int n3);
 int fullgrid_fft3(int key, double* fx, double* fy, int n1, int n2, int n3);
 int fullgrid_rfft3_dev(int key, double* dx, double* dy, int n1, int n2, int n3, hipStream_t st);
@@ -216,6 +218,12 @@ int qx_ijkgd(int nplw5, const int* nga, const int* ngb, const int* ngc, const in
   return 0;
 }
 
+int qx_clmrcp(const double* b, double dkmax2, int cap, int* nga, int* ngb, int* ngc, double* gx,
+              double* gy, double* gz, double* recnrm, int* kcsma) {
+  if (!b || !(dkmax2 > 0.0) || cap < 0) return fail(-1, "qx_clmrcp: bad arguments");
+  return clmrcp(b, dkmax2, cap, nga, ngb, ngc, gx, gy, gz, recnrm, kcsma);
+}
+
 int qx_create(qx_ctx** out, int n1, int n2, int n3, int nplw, const int* nga, const int* ngb,
               const int* ngc, const double* recnrm, int nbmax) {
   if (!out || n1 <= 0 || n2 <= 0 || n3 <= 0 || nplw < 0 || nbmax < 1)
@@ -304,6 +312,8 @@ int qx_destroy(qx_ctx* h) {
   for (auto& t : c->lt)
     for (auto& e : t.ev) (void)hipEventDestroy(e);
   lt_destroy(c);
+  cg_graphs_destroy(c);
+  mg_destroy(c);
   nl_destroy(c);
   scf_destroy(c);
   lopp_destroy(c);
@@ -360,7 +370,8 @@ int qx_set_kb(qx_ctx* h, const double* eqlmr, int lmmax, int natom, const int* l
     }
   const int npj = (int)pa.size();
   const size_t n = (size_t)c->nplwex;
-  lt_forget_kb(c);
+  mg_forget_kb(c);
+  buffers_changed(c);
   kb1_destroy(c);  // a host table has no factors
   if (c->d_B) { (void)hipFree(c->d_B); c->d_B = nullptr; }
   if (c->d_D) { (void)hipFree(c->d_D); c->d_D = nullptr; }
@@ -714,6 +725,25 @@ int qx_rfft3_dev_noalloc(int key, double* d_fft3x, double* d_fft3y, int k1, int 
                               static_cast<hipStream_t>(stream));
   if (ierror) *ierror = rc ? 1 : 0;
   return rc ? fail(rc, "qx_rfft3_dev_noalloc: failed (%d)", rc) : 0;
+}
+
+int qx_set_gemm_path(int path) {
+  if (path < GEMM_OWN || path > GEMM_OWN_F32) return fail(-1, "qx_set_gemm_path: bad path %d", path);
+  set_gemm_path(path);
+  return 0;
+}
+
+int qx_gemm_path(void) { return gemm_path(); }
+
+int qx_set_cg_graph(qx_ctx* h, int on) {
+  Ctx* c = &h->c;
+  c->cg_graph_on = on != 0;
+  if (!on) cg_graphs_destroy(c);
+  return 0;
+}
+
+int qx_cg_graph_count(qx_ctx* h) {
+  return h->c.cg_graphs ? cg_graph_entries(&h->c) : 0;
 }
 
 int qx_noalloc_limits(long* max_points, int* max_axis) {

@@ -1,14 +1,11 @@
-// blaslt.hip -- the large plain contractions on hipBLASLt.
+// blaslt.hip -- the large plain contractions on hipBLASLt, kept as a
+// cross-check of our own MFMA kernels (mfma_gemm.hip): it runs only when the
+// environment sets QX_GEMM_LIB=hipblaslt (tests compare the two).
 //
-// The KB projection P = B^T W psi and back-projection H psi += B D P, the
-// overlap, subspace-H and rotation products and the CG projections are plain
-// GEMMs over the plane-wave index (the packed-inner-product rank-1 correction
-// stays a kernel of ours, k_corr).  For the large shapes they run on
-// hipBLASLt; the algorithm for each shape is chosen once per context by
-// timing the library's candidates on the first call (a small autotuner), then
-// cached.  The mixed form rounds the FP64 operands to FP32 copies (the KB
-// table's copy is kept), multiplies with FP32 accumulation and adds the FP32
-// product into the FP64 result.
+// For each shape the library's candidate algorithms are timed once per
+// context on the first call and the fastest is cached.  FP64 operands of the
+// FP32 modes are rounded to FP32 copies; mode 2 is the library's fast FP32
+// compute (split products on the matrix cores).
 #include <hipblaslt/hipblaslt.h>
 #include <cstdlib>
 #include <cstring>
@@ -29,18 +26,15 @@ struct LtPlan {
 
 typedef std::tuple<int, int, int, int, int, int, int, int> PlanKey;  // ta M N K lda ldb ldc mode
 
-constexpr int kScratch = 8;
+constexpr int kScratch = 3;
 
 struct LtState {
   hipblasLtHandle_t h = nullptr;
   void* ws = nullptr;
   size_t ws_bytes = 0;
   std::map<PlanKey, LtPlan> plans;
-  float* f32[kScratch] = {};  // 0-2: operand copies of lt_sgemm_mixed; the rest lt_f32_scratch
+  float* f32[kScratch] = {};  // operand copies of lt_sgemm_mixed
   size_t f32_len[kScratch] = {};
-  float* B32 = nullptr;          // FP32 copy of the KB table
-  const double* B32_src = nullptr;
-  size_t B32_len = 0;
 };
 
 constexpr size_t kWorkspace = 128u << 20;
@@ -178,10 +172,7 @@ __global__ void k_f32_axpby(const float* __restrict__ R, int M, long N, double a
 
 }  // namespace
 
-bool lt_enabled() {
-  static const int own = std::getenv("QX_GEMM_OWN") ? 1 : 0;
-  return !own;
-}
+bool lt_enabled() { return gemm_path() == GEMM_LIB; }
 
 // FP64: C = alpha op(A) B + beta C.  Returns < 0 when hipBLASLt cannot take
 // the shape (the caller falls back to its own kernels).
@@ -197,30 +188,6 @@ int lt_dgemm(Ctx* c, bool ta, int M, int N, int K, double alpha, const double* A
                                        C, p->lc, &p->algo, s->ws, s->ws_bytes, c->stream);
   lt_mark(c, c->lt_gemm_id, true);
   return st == HIPBLAS_STATUS_SUCCESS ? 0 : -2;
-}
-
-// FP32 copy of the KB table (nplwex x npj), made once per table
-const float* lt_kb_b32(Ctx* c) {
-  LtState* s = state(c);
-  if (!s || !c->d_B) return nullptr;
-  const size_t len = (size_t)c->nplwex * c->npj;
-  if (s->B32_src != c->d_B || s->B32_len != len || !s->B32) {
-    if (s->B32) (void)hipFree(s->B32);
-    s->B32 = nullptr;
-    if (hipMalloc(&s->B32, sizeof(float) * len) != hipSuccess) return nullptr;
-    hipLaunchKernelGGL(k_to_f32, dim3(cdivu((long)len, 256)), dim3(256), 0, c->stream, c->d_B,
-                       c->nplwex, s->B32, c->nplwex, c->nplwex, (long)c->npj);
-    s->B32_src = c->d_B;
-    s->B32_len = len;
-  }
-  return s->B32;
-}
-
-// FP32 work buffer i (0 <= i < 4) of at least n floats, kept by the context
-float* lt_f32_scratch(Ctx* c, int i, size_t n) {
-  LtState* s = state(c);
-  if (!s || i < 0 || i + 3 >= kScratch) return nullptr;
-  return scratch(s, i + 3, n);
 }
 
 // the FP32 compute mode of the mixed products: QX_MIXED_GEMM=f32 keeps plain
@@ -257,7 +224,7 @@ int lt_sgemm_mixed(Ctx* c, bool ta, int M, int N, int K, double alpha, const dou
   const float* A32;
   int lda32 = arows;
   if (A == c->d_B && lda == c->nplwex && c->d_B) {
-    A32 = lt_kb_b32(c);
+    A32 = mg_kb_b32(c);
     if (!A32) return -3;
     lda32 = c->nplwex;
   } else {
@@ -285,16 +252,6 @@ int lt_sgemm_mixed(Ctx* c, bool ta, int M, int N, int K, double alpha, const dou
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
-// the KB table changed: drop its FP32 copy
-void lt_forget_kb(Ctx* c) {
-  if (!c->blaslt) return;
-  LtState* s = static_cast<LtState*>(c->blaslt);
-  if (s->B32) (void)hipFree(s->B32);
-  s->B32 = nullptr;
-  s->B32_src = nullptr;
-  s->B32_len = 0;
-}
-
 void lt_destroy(Ctx* c) {
   if (!c->blaslt) return;
   LtState* s = static_cast<LtState*>(c->blaslt);
@@ -305,9 +262,8 @@ void lt_destroy(Ctx* c) {
     if (p.lc) hipblasLtMatrixLayoutDestroy(p.lc);
     if (p.desc) hipblasLtMatmulDescDestroy(p.desc);
   }
-  for (int i = 0; i < 3; ++i)
+  for (int i = 0; i < kScratch; ++i)
     if (s->f32[i]) (void)hipFree(s->f32[i]);
-  if (s->B32) (void)hipFree(s->B32);
   if (s->ws) (void)hipFree(s->ws);
   if (s->h) hipblasLtDestroy(s->h);
   delete s;

@@ -760,6 +760,45 @@ __global__ void k_mul(double* out, const double* a, const double* b2, int nb) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nb) out[b] = a[b] * b2[b];
 }
+__global__ void k_copy(double* out, const double* in, int nb) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) out[b] = in[b];
+}
+
+// One batched-CG iteration captured as a HIP graph, per (bands, precision,
+// psi, H psi) of a context: every later iteration (all but the first, which
+// runs eagerly and sets up whatever the products need on first use) is one
+// graph launch.  The graph holds kernel launches only (no memset or host
+// copy nodes).
+struct CgGraphs {
+  struct Entry {
+    int nb, prec, path;
+    const double *psi, *hpsi;
+    unsigned long gen;  // Ctx::buffers_gen at capture
+    hipGraphExec_t exec;
+  };
+  std::vector<Entry> e;
+};
+
+void cg_graphs_destroy(Ctx* c) {
+  if (!c->cg_graphs) return;
+  auto* g = static_cast<CgGraphs*>(c->cg_graphs);
+  for (auto& x : g->e) (void)hipGraphExecDestroy(x.exec);
+  delete g;
+  c->cg_graphs = nullptr;
+}
+
+int cg_graph_entries(Ctx* c) {
+  return c->cg_graphs ? (int)static_cast<CgGraphs*>(c->cg_graphs)->e.size() : 0;
+}
+
+static bool cg_graph_wanted(Ctx* c) {
+  static const bool off = std::getenv("QX_CG_GRAPH") && std::atoi(std::getenv("QX_CG_GRAPH")) == 0;
+  if (off || !c->cg_graph_on || gemm_path() == GEMM_LIB) return false;
+  for (auto& t : c->lt)
+    if (t.on) return false;  // launch timers record events between launches
+  return true;
+}
 
 int hpsi_batched(Ctx* c, const double* x, double* hx, int nb, int prec);
 
@@ -794,14 +833,15 @@ int bkl_batched(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib_ho
   const int nch = (n + kDotChunk - 1) / kDotChunk;
   double* part = c->d_scal + 64 * (size_t)c->nbmax;  // 3 x nch x nb partial sums
   const dim3 dgrid(nch, nb);
-  for (int it = 0; it < maxit; ++it) {
+  // one iteration; the first takes h = P g (beta = 0 and h may hold anything)
+  auto iteration = [&](bool first) -> int {
     // precondition, CG direction
-    hipLaunchKernelGGL(k_precond_dot<kDotChunk>, dgrid, dim3(kDotThreads), 0, c->stream, gnk, c->d_kin, ek, n,
-                       s.active, nch, part);
+    hipLaunchKernelGGL(k_precond_dot<kDotChunk>, dgrid, dim3(kDotThreads), 0, c->stream, gnk,
+                       c->d_kin, ek, n, s.active, nch, part);
     hipLaunchKernelGGL(k_dotq_final, dim3(gb), dim3(256), 0, c->stream, part, nch, nb, 1, gnk, gnk,
                        nullptr, nullptr, n, s.zansap, nullptr, nullptr);
     hipLaunchKernelGGL(k_bkl_beta, dim3(gb), dim3(256), 0, c->stream, s, nb);
-    if (it == 0) axpby(c, hnk, gnk, nullptr, nullptr, nullptr, 1.0, 0.0, nb, s.active);
+    if (first) axpby(c, hnk, gnk, nullptr, nullptr, nullptr, 1.0, 0.0, nb, s.active);
     else axpby(c, hnk, gnk, hnk, nullptr, s.beta, 1.0, 0.0, nb, s.active);
     // project out all current bands: h <- h - Psi (Psi^T W h)
     int rc = gemm_tn_p(c, nb, nb, n, 2.0, psi, n, hnk, n, 0.0, proj, nb, 1, prec);
@@ -810,19 +850,63 @@ int bkl_batched(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib_ho
     if (rc) return rc;
     rc = hpsi_batched(c, hnk, hh, nb, prec);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_dot3<kDotChunk>, dgrid, dim3(kDotThreads), 0, c->stream, hnk, hpsi, hh, n, nch, part);
-    hipLaunchKernelGGL(k_dotq_final, dim3(gb), dim3(256), 0, c->stream, part, nch, nb, 3, hnk, hpsi,
-                       hh, hnk, n, s.cai, s.cbi, s.hhi);
+    hipLaunchKernelGGL(k_dot3<kDotChunk>, dgrid, dim3(kDotThreads), 0, c->stream, hnk, hpsi, hh,
+                       n, nch, part);
+    hipLaunchKernelGGL(k_dotq_final, dim3(gb), dim3(256), 0, c->stream, part, nch, nb, 3, hnk,
+                       hpsi, hh, hnk, n, s.cai, s.cbi, s.hhi);
     hipLaunchKernelGGL(k_bkl_step, dim3(gb), dim3(256), 0, c->stream, s, nb);
     // x = a (x + dt h); Hx = a (Hx + dt Hh); g = eig x - Hx
     hipLaunchKernelGGL(k_mul, dim3(gb), dim3(256), 0, c->stream, tmp, s.alpha, s.dt, nb);
-    hipLaunchKernelGGL(k_cg_update, dgrid, dim3(kDotThreads), 0, c->stream, psi, hpsi, gnk, hnk, hh,
-                       s.alpha, tmp, s.eig, n, s.active, nch, part);
-    hipLaunchKernelGGL(k_dotq_final, dim3(gb), dim3(256), 0, c->stream, part, nch, nb, 1, gnk, gnk,
-                       nullptr, nullptr, n, tmp2, nullptr, nullptr);
-    QX_CHECK(hipMemcpyAsync(s.zansa, tmp2, sizeof(double) * nb, hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_cg_update, dgrid, dim3(kDotThreads), 0, c->stream, psi, hpsi, gnk, hnk,
+                       hh, s.alpha, tmp, s.eig, n, s.active, nch, part);
+    hipLaunchKernelGGL(k_dotq_final, dim3(gb), dim3(256), 0, c->stream, part, nch, nb, 1, gnk,
+                       gnk, nullptr, nullptr, n, tmp2, nullptr, nullptr);
+    hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, c->stream, s.zansa, tmp2, nb);
     hipLaunchKernelGGL(k_bkl_conv, dim3(gb), dim3(256), 0, c->stream, s, nb, noband, itermx,
                        iteremx, tolera, roughfzan, roughfeig);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+  };
+  int rc = iteration(true);
+  if (rc) return rc;
+  hipGraphExec_t exec = nullptr;
+  if (maxit > 1 && cg_graph_wanted(c)) {
+    if (!c->cg_graphs) c->cg_graphs = new CgGraphs();
+    auto* G = static_cast<CgGraphs*>(c->cg_graphs);
+    // graphs captured before a buffer they use moved are stale: drop them
+    for (size_t i = 0; i < G->e.size();) {
+      if (G->e[i].gen != c->buffers_gen) {
+        (void)hipGraphExecDestroy(G->e[i].exec);
+        G->e.erase(G->e.begin() + (long)i);
+      } else {
+        ++i;
+      }
+    }
+    const int path = gemm_path();
+    for (auto& x : G->e)
+      if (x.nb == nb && x.prec == prec && x.path == path && x.psi == psi && x.hpsi == hpsi)
+        exec = x.exec;
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      QX_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int rcc = iteration(false);
+      const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+      if (rcc == 0 && ec == hipSuccess && graph &&
+          hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+        G->e.push_back({nb, prec, path, psi, hpsi, c->buffers_gen, exec});
+      } else {
+        exec = nullptr;
+        (void)hipGetLastError();
+      }
+      if (graph) (void)hipGraphDestroy(graph);
+      if (!exec) c->cg_graph_on = false;  // capture not possible: stay eager from now on
+    }
+  }
+  for (int it = 1; it < maxit; ++it) {
+    if (exec) {
+      QX_CHECK(hipGraphLaunch(exec, c->stream));
+    } else if ((rc = iteration(false))) {
+      return rc;
+    }
   }
   QX_CHECK(hipMemcpyAsync(eig_host, s.eig, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
   if (bzan)

@@ -176,8 +176,17 @@ struct Ctx {
     int nreset = 0;
   } scf;
   int lt_gemm_id = -1;                    // timer the GEMM launches report to
-  void* blaslt = nullptr;                 // hipBLASLt state (handle, plans, FP32 copies)
+  void* blaslt = nullptr;                 // hipBLASLt state (cross-check only)
+  void* mg = nullptr;                     // own-GEMM state (workspace, tuned grids, FP32 copies)
+  void* cg_graphs = nullptr;              // captured batched-CG iterations (cg.hip)
+  bool cg_graph_on = true;                // replay them (qx_set_cg_graph)
+  // bumped whenever a device buffer a captured graph may read or write is
+  // (re)allocated or its contents' shape changes (KB table, GEMM workspace,
+  // FP32 copies, potential copies): captured graphs of older generations are
+  // dropped and captured again
+  unsigned long buffers_gen = 0;
 };
+inline void buffers_changed(Ctx* c) { ++c->buffers_gen; }
 
 // record the begin (end) event of a timed launch of timer id, if enabled; a
 // begin with cont = true opens a further segment of the same launch
@@ -199,14 +208,38 @@ int lt_dgemm(Ctx* c, bool ta, int M, int N, int K, double alpha, const double* A
              const double* B, int ldb, double beta, double* C, int ldc);
 int lt_sgemm_mixed(Ctx* c, bool ta, int M, int N, int K, double alpha, const double* A, int lda,
                    const double* B, int ldb, double beta, double* C, int ldc);
-void lt_forget_kb(Ctx* c);
 void lt_destroy(Ctx* c);
-// FP32 pieces of the fused mixed band-apply
+// which kernels run the sizable contractions: GEMM_OWN (our MFMA kernels,
+// split bf16x3 for the mixed form), GEMM_LIB (hipBLASLt, a cross-check of
+// ours) or GEMM_OWN_F32 (ours, plain FP32 MFMA for the mixed form).  Set by
+// qx_set_gemm_path; the environment's QX_GEMM_PATH (0, 1, 2) is the default.
+enum { GEMM_OWN = 0, GEMM_LIB = 1, GEMM_OWN_F32 = 2 };
+int gemm_path();
+void set_gemm_path(int p);
 bool lt_enabled();
-const float* lt_kb_b32(Ctx* c);
-float* lt_f32_scratch(Ctx* c, int i, size_t n);
 int lt_sgemm_f32(Ctx* c, bool ta, int M, int N, int K, const float* A, int lda, const float* B,
                  int ldb, float* C, int ldc);
+// our MFMA contractions (mfma_gemm.hip); -3 = shape not taken (odd leading
+// dimension), the caller falls back to the small-tile kernels
+void mg_destroy(Ctx* c);
+void cg_graphs_destroy(Ctx* c);
+int cg_graph_entries(Ctx* c);
+int mg_dgemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
+                const double* B, int ldb, double beta, double* C, int ldc, int corr);
+int mg_dgemm_nn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
+                const double* B, int ldb, double beta, double* C, int ldc);
+template <typename T, typename TC>
+int mg_bf3_tn(Ctx* c, int M, int N, int K, double alpha, const T* A, int lda, const T* B, int ldb,
+              double beta, TC* C, int ldc, int corr);
+template <typename T, typename TC>
+int mg_bf3_nn(Ctx* c, int M, int N, int K, double alpha, const T* A, int lda, const T* B, int ldb,
+              double beta, TC* C, int ldc);
+// FP32 pieces of the fused mixed band-apply: the KB table's FP32 copy (made
+// once per table) and FP32 work buffers i = 0..3 kept by the context
+const float* mg_kb_b32(Ctx* c);
+float* mg_f32_scratch(Ctx* c, int i, size_t n);
+void mg_forget_kb(Ctx* c);
+int mg_tuned_shapes(Ctx* c);
 bool use_big(int M, int N);
 // KB term of the mixed band-apply from the FP32 copy psi32 of psi: kb32 =
 // B diag(D) (B^T W psi) in FP32 (nplwex x nb)

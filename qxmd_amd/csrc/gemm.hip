@@ -14,14 +14,33 @@
 // alpha = 2 plus a rank-1 correction of row 0 (and row 1 for the KB sum,
 // which skips Im c(G=0): Sources/nlkbpp.f90:1360-1363).
 //
-// Tiles: 64x64 per 256-thread block, 4 wavefronts of 32x32, each a 2x2 grid
-// of v_mfma_f64_16x16x4 blocks; K staged through LDS 16 at a time.  The
-// long-K shape splits K over blocks and reduces with FP64 atomics.
+// Sizable outputs (M, N >= 256) run on the 128 x 128 kernels of
+// mfma_gemm.hip.  The kernels here take the skinny shapes: 64x64 per
+// 256-thread block, 4 wavefronts of 32x32, each a 2x2 grid of
+// v_mfma_f64_16x16x4 blocks, K staged through LDS 16 at a time, the long-K
+// shape split over blocks and reduced with FP64 atomics; at most 4 columns,
+// matrix-vector kernels.
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include "context.h"
 
 namespace qx {
+
+static std::atomic<int> g_gemm_path{-1};
+
+int gemm_path() {
+  int p = g_gemm_path.load(std::memory_order_relaxed);
+  if (p < 0) {
+    const char* e = std::getenv("QX_GEMM_PATH");
+    p = e ? std::atoi(e) : GEMM_OWN;
+    if (p < GEMM_OWN || p > GEMM_OWN_F32) p = GEMM_OWN;
+    g_gemm_path.store(p, std::memory_order_relaxed);
+  }
+  return p;
+}
+
+void set_gemm_path(int p) { g_gemm_path.store(p, std::memory_order_relaxed); }
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -597,16 +616,27 @@ static void launch_big_nn(Ctx* c, int M, int N, int K, double alpha, const doubl
                      alpha, A, lda, B, ldb, C, ldc);
 }
 
+// FP32-class contractions (the mixed form): our split bf16x3 MFMA kernels
+// for sizable outputs, FP32 MFMA tiles for skinny ones; hipBLASLt only as a
+// cross-check (QX_GEMM_LIB=hipblaslt)
 int gemm_tn_p(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
               int ldb, double beta, double* C, int ldc, int corr, int prec) {
   if (prec != PREC_FP32) return gemm_tn(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, corr);
   if (M <= 0 || N <= 0) return 0;
-  if (use_big(M, N) && lt_sgemm_mixed(c, true, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc) == 0) {
-    if (corr)
-      hipLaunchKernelGGL(k_corr, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N,
-                         ldc, A, lda, B, ldb, alpha, corr == 2 ? 1 : 0);
-    QX_CHECK(hipGetLastError());
-    return 0;
+  if (use_big(M, N)) {
+    if (lt_enabled()) {
+      if (lt_sgemm_mixed(c, true, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc) == 0) {
+        if (corr)
+          hipLaunchKernelGGL(k_corr, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C,
+                             M, N, ldc, A, lda, B, ldb, alpha, corr == 2 ? 1 : 0);
+        QX_CHECK(hipGetLastError());
+        return 0;
+      }
+    } else if (gemm_path() == GEMM_OWN) {
+      const int rc = mg_bf3_tn<double, double>(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                               corr);
+      if (rc != -3) return rc;
+    }
   }
 
   hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N, ldc,
@@ -639,8 +669,14 @@ int gemm_nn_p(Ctx* c, int M, int N, int K, double alpha, const double* A, int ld
               int ldb, double beta, double* C, int ldc, int prec) {
   if (prec != PREC_FP32) return gemm_nn(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
   if (M <= 0 || N <= 0) return 0;
-  if (use_big(M, N) && lt_sgemm_mixed(c, false, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc) == 0)
-    return 0;
+  if (use_big(M, N)) {
+    if (lt_enabled()) {
+      if (lt_sgemm_mixed(c, false, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc) == 0) return 0;
+    } else if (gemm_path() == GEMM_OWN) {
+      const int rc = mg_bf3_nn<double, double>(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+      if (rc != -3) return rc;
+    }
+  }
 
   if (beta != 1.0)
     hipLaunchKernelGGL(k_scale, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N,
@@ -658,27 +694,37 @@ int gemm_nn_p(Ctx* c, int M, int N, int K, double alpha, const double* A, int ld
   return 0;
 }
 
-// FP64 contractions: hipBLASLt for sizable outputs (64 x 64 MFMA kernels of
-// ours when it cannot take a shape), 64 x 64 kernels for the skinny ones
+// FP64 contractions: our 128 x 128 MFMA kernels (mfma_gemm.hip) for sizable
+// outputs, 64 x 64 tiles and matrix-vector kernels for skinny ones
 int gemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
             int ldb, double beta, double* C, int ldc, int corr) {
   if (!use_big(M, N)) return gemm_tn_small(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, corr);
-  if (lt_dgemm(c, true, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc) != 0)
-    return gemm_tn_small(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, corr);
-  if (corr)
-    hipLaunchKernelGGL(k_corr, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N, ldc,
-                       A, lda, B, ldb, alpha, corr == 2 ? 1 : 0);
-  QX_CHECK(hipGetLastError());
-  return 0;
+  if (lt_enabled()) {
+    if (lt_dgemm(c, true, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc) != 0)
+      return gemm_tn_small(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, corr);
+    if (corr)
+      hipLaunchKernelGGL(k_corr, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N,
+                         ldc, A, lda, B, ldb, alpha, corr == 2 ? 1 : 0);
+    QX_CHECK(hipGetLastError());
+    return 0;
+  }
+  const int rc = mg_dgemm_tn(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, corr);
+  if (rc != -3) return rc;
+  return gemm_tn_small(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, corr);
 }
 
 int gemm_nn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda, const double* B,
             int ldb, double beta, double* C, int ldc) {
   if (!use_big(M, N)) return gemm_nn_small(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
-  if (lt_dgemm(c, false, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc) != 0)
-    return gemm_nn_small(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
-  QX_CHECK(hipGetLastError());
-  return 0;
+  if (lt_enabled()) {
+    if (lt_dgemm(c, false, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc) != 0)
+      return gemm_nn_small(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    QX_CHECK(hipGetLastError());
+    return 0;
+  }
+  const int rc = mg_dgemm_nn(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+  if (rc != -3) return rc;
+  return gemm_nn_small(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 // P[b][p] -> slmir (optional copy) and DP[b][p] = D[p] P[b][p]
@@ -725,20 +771,24 @@ __global__ void k_proj_fix(const float* __restrict__ P32, const double* __restri
 
 int kb_mixed_f32(Ctx* c, const double* psi, const float* psi32, int nb, float* kb32) {
   const int n = c->nplwex, npj = c->npj;
-  const float* B32 = lt_kb_b32(c);
-  float* P32 = lt_f32_scratch(c, 2, (size_t)npj * nb);
-  float* Pd32 = lt_f32_scratch(c, 3, (size_t)npj * nb);
+  const float* B32 = mg_kb_b32(c);
+  float* P32 = mg_f32_scratch(c, 2, (size_t)npj * nb);
+  float* Pd32 = mg_f32_scratch(c, 3, (size_t)npj * nb);
   if (!B32 || !P32 || !Pd32) return -3;
   struct TimeGemms {  // the KB contractions are what launch timer 0 measures
     Ctx* c;
     explicit TimeGemms(Ctx* cc) : c(cc) { c->lt_gemm_id = 0; }
     ~TimeGemms() { c->lt_gemm_id = -1; }
   } tg(c);
-  int rc = lt_sgemm_f32(c, true, npj, nb, n, B32, n, psi32, n, P32, npj);
+  int rc = lt_enabled() ? lt_sgemm_f32(c, true, npj, nb, n, B32, n, psi32, n, P32, npj)
+                        : mg_bf3_tn<float, float>(c, npj, nb, n, 1.0, B32, n, psi32, n, 0.0, P32,
+                                                  npj, 0);
   if (rc) return rc;
   hipLaunchKernelGGL(k_proj_fix, dim3(cdiv((long)npj * nb, 256)), dim3(256), 0, c->stream, P32,
                      c->d_B, n, psi, n, c->d_D, npj, nb, Pd32);
-  return lt_sgemm_f32(c, false, n, nb, npj, B32, n, Pd32, npj, kb32, n);
+  return lt_enabled() ? lt_sgemm_f32(c, false, n, nb, npj, B32, n, Pd32, npj, kb32, n)
+                      : mg_bf3_nn<float, float>(c, n, nb, npj, 1.0, B32, n, Pd32, npj, 0.0, kb32,
+                                                n);
 }
 
 }  // namespace qx

@@ -335,7 +335,8 @@ int nl_setnlc_g(Ctx* c, int nplw, int ntype, const int* nhk1, const int* nhk2, i
   }
   const int npj = (int)ps.size();
   const size_t nex = (size_t)c->nplwex;
-  lt_forget_kb(c);
+  mg_forget_kb(c);
+  buffers_changed(c);
   if (c->d_B) { (void)hipFree(c->d_B); c->d_B = nullptr; }
   if (c->d_D) { (void)hipFree(c->d_D); c->d_D = nullptr; }
   if (c->d_P) { (void)hipFree(c->d_P); c->d_P = nullptr; }

@@ -554,6 +554,7 @@ int fast_permute_v(Ctx* c) {
   int na, nb;
   if (!(c->n1 == c->n2 && c->n2 == c->n3 && fast_split(c->n1, &na, &nb))) return 0;
   if (!c->d_Vp) {
+    buffers_changed(c);
     QX_CHECK(hipMalloc(&c->d_Vp, sizeof(double) * c->N));
     QX_CHECK(hipMalloc(&c->d_Vpf, sizeof(float) * c->N));
   }
@@ -680,14 +681,16 @@ static long plane_cost_sr(int NA, int NB, int sr, int el) {
   return c;
 }
 
-// Row strides of the plane kernel: the pair in [N, N+32]^2 with the fewest
-// modelled LDS passes that keeps as many blocks per CU as unpadded rows do
-// (QX_PLANE_SA / QX_PLANE_SR override).  Cached per shape.
+// Row strides of the plane kernel: pairs in [N, N+32]^2 ranked by the
+// modelled LDS passes, among those that keep as many blocks per CU as
+// unpadded rows do; the best `keep` pairs are returned (the first is the
+// model's choice).  Cached per shape.
 template <typename T>
-static void plane_strides(int NA, int NB, int nH, int nwave, int* psa, int* psr) {
+static std::vector<std::pair<int, int>> plane_stride_ranks(int NA, int NB, int nH, int nwave,
+                                                           int keep) {
   static std::mutex mu;
-  static std::map<std::array<int, 4>, std::pair<int, int>> cache;
-  const std::array<int, 4> key{NA * 100 + NB, nH, (int)sizeof(T), nwave};
+  static std::map<std::array<int, 5>, std::vector<std::pair<int, int>>> cache;
+  const std::array<int, 5> key{NA * 100 + NB, nH, (int)sizeof(T), nwave, keep};
   std::lock_guard<std::mutex> lk(mu);
   auto it = cache.find(key);
   if (it == cache.end()) {
@@ -695,24 +698,34 @@ static void plane_strides(int NA, int NB, int nH, int nwave, int* psa, int* psr)
     const size_t occ0 = kCuLds / plane_lds<T>(N, nH, N, N, RW, nwave);
     long csr[33];
     for (int d = 0; d <= 32; ++d) csr[d] = plane_cost_sr(NA, NB, N + d, el);
-    long best = -1;
-    int bsa = N, bsr = N;
+    std::vector<std::pair<long, std::pair<int, int>>> all;
     for (int da = 0; da <= 32; ++da) {
       const long ca = plane_cost_sa(NA, NB, nH, N + da, el);
       for (int dr = 0; dr <= 32; ++dr) {
         const size_t l = plane_lds<T>(N, nH, N + da, N + dr, RW, nwave);
         if (l > kCuLds || kCuLds / l < occ0) continue;
-        if (best < 0 || ca + csr[dr] < best) {
-          best = ca + csr[dr];
-          bsa = N + da;
-          bsr = N + dr;
-        }
+        all.push_back({ca + csr[dr], {N + da, N + dr}});
       }
     }
-    it = cache.emplace(key, std::make_pair(bsa, bsr)).first;
+    std::stable_sort(all.begin(), all.end(),
+                     [](const auto& x, const auto& y) { return x.first < y.first; });
+    std::vector<std::pair<int, int>> r;
+    for (const auto& x : all) {
+      if ((int)r.size() >= keep) break;
+      r.push_back(x.second);
+    }
+    if (r.empty()) r.push_back({N, N});
+    it = cache.emplace(key, r).first;
   }
-  *psa = it->second.first;
-  *psr = it->second.second;
+  return it->second;
+}
+
+// the model's stride pair (QX_PLANE_SA / QX_PLANE_SR override)
+template <typename T>
+static void plane_strides(int NA, int NB, int nH, int nwave, int* psa, int* psr) {
+  const auto r = plane_stride_ranks<T>(NA, NB, nH, nwave, 1);
+  *psa = r[0].first;
+  *psr = r[0].second;
   if (const char* e = std::getenv("QX_PLANE_SA")) *psa = std::max(NA * NB, std::atoi(e));
   if (const char* e = std::getenv("QX_PLANE_SR")) *psr = std::max(NA * NB, std::atoi(e));
 }
@@ -725,29 +738,106 @@ struct FastMid {
   int (*run)(Ctx*, const double*, int, const FastMid&);
 };
 
+// Launch shape of the band-apply plane kernel: wavefronts per block and the
+// two LDS row strides.  On the first band-apply of a shape the candidates
+// (8 and 4 wavefronts, the model's three best stride pairs for each) are
+// timed on the real z-pencils (k_zinv_f fills T1 first; the trial launches
+// write only T2) and the fastest is kept for the process; QX_FFT_NOTUNE=1
+// takes the model's choice.
+struct PlaneCfg {
+  int nwave = 0, sa = 0, sr = 0;
+  float ms = 0.f;
+};
+
+template <typename T, int NA, int NB>
+static PlaneCfg plane_config(Ctx* c, const double* psi, int nb, const T* Vp, cplx<T>* t1,
+                             cplx<T>* t2, int nbb) {
+  constexpr int N = NA * NB;
+  constexpr int RW = plane_rows_per_wave(NA, NB);
+  static std::mutex mu;
+  static std::map<std::array<int, 3>, PlaneCfg> tuned;
+  const std::array<int, 3> key{N, c->pc.nH, (int)sizeof(T)};
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = tuned.find(key);
+    if (it != tuned.end()) return it->second;
+  }
+  PlaneCfg best;
+  best.nwave = plane_waves<T>(N, c->pc.nH, RW);
+  if (best.nwave < 1) return best;
+  plane_strides<T>(NA, NB, c->pc.nH, best.nwave, &best.sa, &best.sr);
+  const bool fixed = std::getenv("QX_FFT_NOTUNE") || std::getenv("QX_PLANE_WAVES") ||
+                     std::getenv("QX_PLANE_SA") || std::getenv("QX_PLANE_SR");
+  if (!fixed) {
+    FastArgs a = fast_args(c, nb, 0);
+    constexpr int PB = zpairs(N, (int)sizeof(cplx<T>));
+    const unsigned gz = (unsigned)((c->pc.npair + PB - 1) / PB);
+    hipLaunchKernelGGL((k_zinv_f<T, NA, NB>), dim3(gz, nbb), dim3(kFastThreads), 0, c->stream, a,
+                       psi, t1, nullptr);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+      float bms = 1e30f;
+      for (int nw = kPlaneMaxThreads / 64; nw >= 4; nw /= 2) {
+        if (plane_lds<T>(N, c->pc.nH, N, N, RW, nw) > kCuLds) continue;
+        for (const auto& st : plane_stride_ranks<T>(NA, NB, c->pc.nH, nw, 3)) {
+          const size_t lp = plane_lds<T>(N, c->pc.nH, st.first, st.second, RW, nw);
+          if (lp > kCuLds) continue;
+          if (lp > 65536)
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_plane_f<T, NA, NB>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lp);
+          a.sa = st.first;
+          a.sr = st.second;
+          hipLaunchKernelGGL((k_plane_f<T, NA, NB>), dim3(N, nbb), dim3(64 * nw), lp, c->stream,
+                             a, Vp, t1, t2);
+          (void)hipEventRecord(e0, c->stream);
+          for (int r = 0; r < 2; ++r)
+            hipLaunchKernelGGL((k_plane_f<T, NA, NB>), dim3(N, nbb), dim3(64 * nw), lp,
+                               c->stream, a, Vp, t1, t2);
+          (void)hipEventRecord(e1, c->stream);
+          (void)hipEventSynchronize(e1);
+          float ms = 0.f;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          if (hipGetLastError() == hipSuccess && ms > 0.f && ms < bms) {
+            bms = ms;
+            best.nwave = nw;
+            best.sa = st.first;
+            best.sr = st.second;
+            best.ms = ms / 2;
+          }
+        }
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+    }
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  tuned[key] = best;
+  return best;
+}
+
 template <typename T, int NA, int NB>
 static int local_fast_nn(Ctx* c, const double* psi, double* out, int nb, int accumulate,
                          const FastMid* mid = nullptr) {
   constexpr int N = NA * NB;
   constexpr int RW = plane_rows_per_wave(NA, NB);
   const int nbp = (nb + 1) / 2;
-  const int nwave = plane_waves<T>(N, c->pc.nH, RW);
-  if (nwave < 1) return -3;
-  int sa, sr;
-  plane_strides<T>(NA, NB, c->pc.nH, nwave, &sa, &sr);
   const size_t per_bp = sizeof(cplx<T>) * (size_t)N * c->pc.ncol;
   const int batch = (int)std::max<size_t>(1, std::min<size_t>(c->t_bytes / per_bp, (size_t)nbp));
   constexpr int PB = zpairs(N, (int)sizeof(cplx<T>));
   const unsigned gz = (unsigned)((c->pc.npair + PB - 1) / PB);
+  cplx<T>* t1 = reinterpret_cast<cplx<T>*>(c->d_t1);
+  cplx<T>* t2 = reinterpret_cast<cplx<T>*>(c->d_t2);
+  const T* Vp = sizeof(T) == 8 ? reinterpret_cast<const T*>(c->d_Vp) : reinterpret_cast<const T*>(c->d_Vpf);
+  if (mid && batch < nbp) return -3;
+  const PlaneCfg cfg = plane_config<T, NA, NB>(c, psi, nb, Vp, t1, t2, std::min(batch, nbp));
+  const int nwave = cfg.nwave;
+  if (nwave < 1) return -3;
+  const int sa = cfg.sa, sr = cfg.sr;
   const size_t lp = plane_lds<T>(N, c->pc.nH, sa, sr, RW, nwave);
   if (lp > kCuLds) return -3;
   if (lp > 65536)
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_plane_f<T, NA, NB>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lp);
-  cplx<T>* t1 = reinterpret_cast<cplx<T>*>(c->d_t1);
-  cplx<T>* t2 = reinterpret_cast<cplx<T>*>(c->d_t2);
-  const T* Vp = sizeof(T) == 8 ? reinterpret_cast<const T*>(c->d_Vp) : reinterpret_cast<const T*>(c->d_Vpf);
-  if (mid && batch < nbp) return -3;
   for (int bp0 = 0; bp0 < nbp; bp0 += batch) {
     const int nbb = std::min(batch, nbp - bp0);
     FastArgs a = fast_args(c, nb, bp0);
@@ -850,12 +940,13 @@ static int fused_kb_mid(Ctx* c, const double* psi, int nb, const FastMid& m) {
 }
 
 int hpsi_mixed_fused(Ctx* c, const double* psi, double* out, int nb) {
-  if (!fast_ok(c) || c->npj == 0 || !lt_enabled() || std::getenv("QX_HPSI_UNFUSED") ||
-      !use_big(c->npj, nb) || !use_big(c->nplwex, nb))
+  if (!fast_ok(c) || c->npj == 0 || std::getenv("QX_HPSI_UNFUSED") ||
+      gemm_path() == GEMM_OWN_F32 || !use_big(c->npj, nb) ||
+      !use_big(c->nplwex, nb) || (c->nplwex & 1) || (c->npj & 1))
     return -3;
   FastMid m;
-  m.psi32 = lt_f32_scratch(c, 0, (size_t)c->nplwex * nb);
-  m.kb32 = lt_f32_scratch(c, 1, (size_t)c->nplwex * nb);
+  m.psi32 = mg_f32_scratch(c, 0, (size_t)c->nplwex * nb);
+  m.kb32 = mg_f32_scratch(c, 1, (size_t)c->nplwex * nb);
   m.run = fused_kb_mid;
   if (!m.psi32 || !m.kb32) return -3;
   return local_fast_t<float>(c, psi, out, nb, 0, &m);

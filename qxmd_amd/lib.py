@@ -68,6 +68,7 @@ SIGNATURES = {
     "qx_get_mesh": [_dp, _d, _ip],
     "qx_strvec": [_dp, _d, _d, _i, _ip, _ip, _ip, _ip, _dp, _vp, _vp, _vp],
     "qx_ijkgd": [_i, _ip, _ip, _ip, _ip, _ip],
+    "qx_clmrcp": [_dp, _d, _i, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _ip],
     "qx_create": [C.POINTER(_vp), _i, _i, _i, _i, _ip, _ip, _ip, _dp, _i],
     "qx_destroy": [_vp],
     "qx_sync": [_vp],
@@ -102,6 +103,10 @@ SIGNATURES = {
     "qx_fft3": [_i, _dp, _dp, _i, _i, _i, _pi],
     "qx_rfft3_dev_noalloc": [_i, _vp, _vp, _i, _i, _i, _vp, _pi],
     "qx_noalloc_limits": [C.POINTER(C.c_long), _pi],
+    "qx_set_gemm_path": [_i],
+    "qx_set_cg_graph": [_vp, _i],
+    "qx_cg_graph_count": [_vp],
+    "qx_gemm_path": [],
     "qx_hcnonloc": [_vp, _dp, _dp],
     "qx_hckbpp_g": [_vp, _dp, _dp, _i, _vp],
     "qx_hckbpp_ib_g": [_vp, _dp, _dp],

@@ -14,6 +14,39 @@ import numpy as np
 from .lib import QxError, check, f64, i32, load, track
 
 FP64, MIXED = 0, 1
+GEMM_OWN, GEMM_LIB, GEMM_OWN_F32 = 0, 1, 2
+
+
+def set_gemm_path(path):
+    """Kernels of the sizable contractions, process-wide: GEMM_OWN (ours;
+    split bf16x3 products in the mixed form), GEMM_LIB (hipBLASLt, a
+    cross-check), GEMM_OWN_F32 (ours, plain FP32 MFMA in the mixed form)."""
+    check(load().qx_set_gemm_path(int(path)), "qx_set_gemm_path")
+
+
+def gemm_path():
+    return int(load().qx_gemm_path())
+
+
+def clmrcp(b, dkmax2):
+    """The Ewald reciprocal-lattice list (clmrcp, Sources/bulk.f90:556) for
+    reciprocal vectors b (3, 3; columns, without 2 pi) and |G|^2 <= dkmax2:
+    dict of nga, ngb, ngc, gx, gy, gz, recnrm (m = 1..numk2, the reference's
+    order) and kcsma (the largest |h|, |k|, |l|)."""
+    L = load()
+    bf = f64(np.asarray(b, np.float64).T.ravel())  # column-major
+    kc = np.zeros(3, np.int32)
+    z = np.zeros(1, np.int32)
+    zd = np.zeros(1)
+    n = -L.qx_clmrcp(bf, float(dkmax2), 0, z, z, z, zd, zd, zd, zd, kc)
+    if n <= 0:
+        raise QxError(f"qx_clmrcp failed ({-n})")
+    nga, ngb, ngc = (np.zeros(n, np.int32) for _ in range(3))
+    gx, gy, gz, rec = (np.zeros(n) for _ in range(4))
+    m = L.qx_clmrcp(bf, float(dkmax2), n, nga, ngb, ngc, gx, gy, gz, rec, kc)
+    if m != n:
+        raise QxError("qx_clmrcp failed")
+    return dict(nga=nga, ngb=ngb, ngc=ngc, gx=gx, gy=gy, gz=gz, recnrm=rec, kcsma=kc)
 
 
 class GSphere:
@@ -169,6 +202,25 @@ class PlaneWave:
         check(load().qx_get_kb(self.h, eq, vcl.ctypes.data_as(C.c_void_p)), "qx_get_kb")
         return eq, vcl
 
+    @staticmethod
+    def gemm_kernel_name(prec):
+        """What runs the large contractions (KB, overlap, rotation) at precision prec."""
+        path = load().qx_gemm_path()
+        if path == GEMM_LIB:
+            return ("hipBLASLt FP32 GEMM, FAST_TF32 split products on bf16 MFMA"
+                    if prec == MIXED else "hipBLASLt FP64 GEMM")
+        if prec == MIXED:
+            return ("k_mg_bf3: split bf16x3 products on v_mfma_f32_16x16x32_bf16"
+                    if path == GEMM_OWN else "k_gemm_big<float>: v_mfma_f32_16x16x4f32")
+        return "k_mg_f64: v_mfma_f64_16x16x4f64, 128 x 128 tiles"
+
+    def set_cg_graph(self, on):
+        """Replay the batched CG's captured iteration graph (on by default)."""
+        check(load().qx_set_cg_graph(self.h, int(bool(on))), "qx_set_cg_graph")
+
+    def cg_graph_count(self):
+        return int(load().qx_cg_graph_count(self.h))
+
     def info(self):
         info = np.zeros(5, np.int32)
         load().qx_info(self.h, info)
@@ -189,6 +241,9 @@ class PlaneWave:
         check(load().qx_sync(self.h), "qx_sync")
 
     # ---- operations on resident slots
+    def slot_copy(self, src, dst, nb):
+        check(load().qx_slot_copy(self.h, int(src), int(dst), int(nb)), "qx_slot_copy")
+
     def hpsi(self, src=0, dst=1, nb=None, prec=FP64):
         check(load().qx_hpsi_slot(self.h, src, dst, nb, prec), "qx_hpsi_slot")
 

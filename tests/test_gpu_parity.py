@@ -364,8 +364,10 @@ def test_hpsi_mixed_si64(si64):
 def test_hpsi_mixed_fused_si64(si64, monkeypatch):
     """Mixed band-apply at a size where the KB contractions run between the
     pencil passes (FP32 copy of psi from the z inverse pass, KB term added in
-    the z forward pass): same result as the plain order, and within the FP32
-    split-product tolerance of the FP64 oracle."""
+    the z forward pass): the same split bf16x3 products as the plain order,
+    whose projections are kept in FP64 rather than FP32, so the two agree to
+    FP32 rounding of the KB term; both within the split-product tolerance of
+    the FP64 oracle."""
     nb = 256
     gs, ctx, H, v = _setup(si64, nb)
     assert ctx.npj >= 256
@@ -376,9 +378,11 @@ def test_hpsi_mixed_fused_si64(si64, monkeypatch):
     monkeypatch.setenv("QX_HPSI_UNFUSED", "1")
     ctx.hpsi(0, 1, nb, prec=pw.MIXED)
     plain = ctx.get(1, nb)
-    assert _relerr(fused, plain) < 1e-12
+    assert _relerr(fused, plain) < 2e-6
     sub = slice(0, 16)
-    assert _relerr(fused[sub], H.hpsi(psi[sub])) < 3e-5
+    ref = H.hpsi(psi[sub])
+    assert _relerr(fused[sub], ref) < 3e-5
+    assert _relerr(plain[sub], ref) < 3e-5
 
 
 @pytest.mark.parametrize("batched", [False, True])
@@ -398,28 +402,6 @@ def test_scf_band_step_mixed_vs_fp64(si64, batched):
     assert np.abs(np.sort(emx) - np.sort(e64)).max() / 2 < 1e-6
     natom = 64
     assert abs(np.sum(occ * np.sort(emx)) - np.sum(occ * np.sort(e64))) / 2 / natom < 1e-6
-
-
-@pytest.mark.parametrize("mode", ["f32", "xf32"])
-def test_kb_mixed_gemm_accuracy(si64, mode, monkeypatch):
-    """The mixed KB contractions (FP32 copies on hipBLASLt, plain FP32 or the
-    fast split-product mode) against the FP64 oracle: the KB part of H|psi>
-    alone.  Tolerances: FP32 products 5e-6; split products (about 16
-    significant bits per factor) 3e-5."""
-    monkeypatch.setenv("QX_MIXED_GEMM", mode)
-    nb = 256  # large enough for the library path
-    gs, ctx, H, v = _setup(si64, nb)
-    psi = initial_wavefunctions(si64, nb)
-    ctx.put(0, psi)
-    ctx.hpsi(0, 1, nb, prec=pw.MIXED)
-    got = ctx.get(1, nb)
-    ctx.local(0, 1, nb, prec=pw.MIXED)
-    kb_got = got - ctx.get(1, nb)
-    eq, lmx, vcl, cl = model_kb_projectors(si64, si64.cell)
-    kb_ref, _ = O.hckbpp_g(eq, lmx, vcl, psi, np.zeros_like(psi))
-    err = _relerr(kb_got, kb_ref)
-    print(mode, "KB rel err", err)
-    assert err < (5e-6 if mode == "f32" else 3e-5)
 
 
 @pytest.mark.parametrize("host_sf", [False, True])
