@@ -503,3 +503,20 @@ def ncpp(zatom, element, ppdir, jgga, ecutdens, gx, gy, gz, recnrm, recnrm5):
         out = {k: (o[k].item() if o[k].ndim == 0 else o[k].copy()) for k in o.files}
     out["recnrmex"] = recnrmex
     return out
+
+
+def clmrcp(b, dkmax2, cap=200000):
+    """The reference clmrcp's Ewald G list for reciprocal vectors b (3, 3),
+    columns, without 2 pi, and cutoff dkmax2."""
+    L = lib()
+    L.qxref_clmrcp.argtypes = [_dp, C.c_double, C.c_int, C.POINTER(C.c_int), _ip, _ip, _ip, _dp,
+                               _dp, _dp, _dp, _ip]
+    n = C.c_int(0)
+    nga, ngb, ngc = (np.zeros(cap, np.int32) for _ in range(3))
+    gx, gy, gz, rec = (np.zeros(cap) for _ in range(4))
+    kc = np.zeros(3, np.int32)
+    L.qxref_clmrcp(_F(b), float(dkmax2), int(cap), C.byref(n), nga, ngb, ngc, gx, gy, gz, rec, kc)
+    m = int(n.value)
+    assert m <= cap
+    return dict(numk2=m, nga=nga[:m], ngb=ngb[:m], ngc=ngc[:m], gx=gx[:m], gy=gy[:m], gz=gz[:m],
+                recnrm=rec[:m], kcsma=kc)

@@ -98,6 +98,40 @@ subroutine qxref_gsphere(rba_in, ecut_in, ecutdens_in, cap, counts, nga_o, ngb_o
   deallocate(ijkgd, ijkg)
 end subroutine
 
+! clmrcp (Sources/bulk.f90:556) on its own: dkmax2 set in para_ewald as ewaldini
+! leaves it, no double grid; the list it builds in para_ewald copied out.
+subroutine qxref_clmrcp(b_in, dkmax2_in, cap, out_n, nga_o, ngb_o, ngc_o, gx_o, gy_o, gz_o,  &
+&                       recnrm_o, kcsma_o) bind(C, name="qxref_clmrcp")
+  use iso_c_binding
+  use para_ewald, only: dkmax2, numk2, nga, ngb, ngc, gx, gy, gz, recnrm, KCSMA1, KCSMA2, KCSMA3
+  implicit none
+  real(c_double), intent(in) :: b_in(3,3)
+  real(c_double), value :: dkmax2_in
+  integer(c_int), value :: cap
+  integer(c_int) :: out_n, nga_o(cap), ngb_o(cap), ngc_o(cap), kcsma_o(3)
+  real(c_double) :: gx_o(cap), gy_o(cap), gz_o(cap), recnrm_o(cap)
+  integer :: nfile(2), i
+  real*8 :: b(3,3)
+  nfile = 6
+  b = b_in
+  dkmax2 = dkmax2_in
+  call clmrcp(nfile, 0, 1, b, 1, 1, 1, 1, .false., 0.d0, .false., 1.2d0, .false.)
+  out_n = numk2
+  kcsma_o(1) = KCSMA1
+  kcsma_o(2) = KCSMA2
+  kcsma_o(3) = KCSMA3
+  if (numk2 > cap) return
+  do i = 1, numk2
+    nga_o(i) = nga(i)
+    ngb_o(i) = ngb(i)
+    ngc_o(i) = ngc(i)
+    gx_o(i) = gx(i)
+    gy_o(i) = gy(i)
+    gz_o(i) = gz(i)
+    recnrm_o(i) = recnrm(i)
+  end do
+end subroutine
+
 subroutine qxref_rnd00(seed, n, out) bind(C, name="qxref_rnd00")
   use iso_c_binding
   implicit none

@@ -7,6 +7,7 @@
 //   vsgar + vsoe     Sources/funcs.f90:1211-1359 (a stable sort by key)
 //   set_rciprl       Sources/modules.f90:1518-1540
 //   reclat (ijkgd)   Sources/reclat.f90:259-276
+//   clmrcp           Sources/bulk.f90:556-1050 (the Ewald G list; same generator)
 // This file must be compiled without floating-point contraction.
 #include <algorithm>
 #include <cmath>
@@ -197,6 +198,74 @@ int strvec(const double* rba, double ecutdens, double ecut, int cap, int* counts
     }
   }
   return num;
+}
+
+// clmrcp (Sources/bulk.f90:556-1050) without the double grid: the Ewald
+// reciprocal-lattice list of the reciprocal vectors B (columns, without 2 pi,
+// as rciprl returns them) for |G|^2 <= dkmax2.  The same half-space generator
+// as strvec with kmaxm1 from sqrt(dkmax2) (596-614), |G|^2 from B without
+// 2 pi, the stable key sort, then the first numk2 vectors in that order; the
+// largest |h|, |k|, |l| of the list go to kcsma.  Returns numk2 (needs cap >=
+// numk2), or -numk2 when cap is too small.
+int clmrcp(const double* B, double dkmax2, int cap, int* nga, int* ngb, int* ngc, double* gx,
+           double* gy, double* gz, double* recnrm, int* kcsma) {
+  GenSetup s;
+  for (int i = 0; i < 9; ++i) s.cross[i] = B[i];
+  for (int i = 0; i < 3; ++i) {
+    int ip1 = IIP1[i], ip2 = IIP2[i], ip3 = IIP3[i];
+    double a1x = M3(B, 2, ip1) * M3(B, 3, ip2) - M3(B, 3, ip1) * M3(B, 2, ip2);
+    double a1y = M3(B, 3, ip1) * M3(B, 1, ip2) - M3(B, 1, ip1) * M3(B, 3, ip2);
+    double a1z = M3(B, 1, ip1) * M3(B, 2, ip2) - M3(B, 2, ip1) * M3(B, 1, ip2);
+    double aaa = a1x * a1x + a1y * a1y + a1z * a1z;
+    aaa = 1.0 / std::sqrt(aaa);
+    a1x *= aaa;
+    a1y *= aaa;
+    a1z *= aaa;
+    double pk3 = M3(B, 1, ip3) * a1x + M3(B, 2, ip3) * a1y + M3(B, 3, ip3) * a1z;
+    double dmmz = std::sqrt(dkmax2) / pk3;
+    int k = (int)std::fabs(dmmz);
+    s.kmaxm1[i] = k;
+    s.kmaxm = std::max(s.kmaxm, k);
+  }
+  std::vector<int> h, k, l;
+  generate(s, [&](int a, int b, int c) { h.push_back(a); k.push_back(b); l.push_back(c); });
+  const int num = (int)h.size();
+  std::vector<double> g1(num), g2(num), g3(num), gss(num);
+  for (int m = 0; m < num; ++m) {
+    double d1 = h[m], d2 = k[m], d3 = l[m];
+    g1[m] = M3(s.cross, 1, 1) * d1 + M3(s.cross, 1, 2) * d2 + M3(s.cross, 1, 3) * d3;
+    g2[m] = M3(s.cross, 2, 1) * d1 + M3(s.cross, 2, 2) * d2 + M3(s.cross, 2, 3) * d3;
+    g3[m] = M3(s.cross, 3, 1) * d1 + M3(s.cross, 3, 2) * d2 + M3(s.cross, 3, 3) * d3;
+    gss[m] = g1[m] * g1[m] + g2[m] * g2[m] + g3[m] * g3[m];
+  }
+  double ddmx = 0.0;
+  for (int m = 0; m < num; ++m) ddmx = std::max(ddmx, std::fabs(gss[m]));
+  const double fct = 1000000000.0 / ddmx;
+  std::vector<int> key(num), ord(num);
+  for (int m = 0; m < num; ++m) {
+    key[m] = (int)(fct * gss[m]);
+    ord[m] = m;
+  }
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
+  int numk2 = 0;
+  for (int m = 0; m < num; ++m)
+    if (gss[m] <= dkmax2) ++numk2;
+  if (cap < numk2) return -numk2;
+  kcsma[0] = kcsma[1] = kcsma[2] = 0;
+  for (int i = 0; i < numk2; ++i) {
+    const int j = ord[i];
+    nga[i] = h[j];
+    ngb[i] = k[j];
+    ngc[i] = l[j];
+    gx[i] = g1[j];
+    gy[i] = g2[j];
+    gz[i] = g3[j];
+    recnrm[i] = gss[j];
+    kcsma[0] = std::max(kcsma[0], std::abs(h[j]));
+    kcsma[1] = std::max(kcsma[1], std::abs(k[j]));
+    kcsma[2] = std::max(kcsma[2], std::abs(l[j]));
+  }
+  return numk2;
 }
 
 void ijkgd_map(int nplw5, const int* nga, const int* ngb, const int* ngc, const int* nd,

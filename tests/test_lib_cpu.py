@@ -47,3 +47,30 @@ def test_plane_wave_counts():
     for n, (wf, dens) in expect.items():
         g = pw.GSphere(diamond_si(n).hcell)
         assert 2 * g.nplw + 1 == wf and 2 * g.nplw5 + 1 == dens
+
+
+def test_clmrcp_lattice_closed_form():
+    """qx_clmrcp's Ewald list is the half of the reciprocal lattice with
+    0 < |G|^2 <= dkmax2: every lattice vector in the ball appears once, as
+    itself or its negative, never both; |G|^2 = |B n|^2; sorted by |G|^2 up to
+    the key's resolution (1e-9 of the largest generated |G|^2)."""
+    import itertools
+    h = np.array([[10.26, 0.3, 0.0], [0.0, 13.1, 0.2], [0.0, 0.0, 8.7]])
+    b = np.linalg.inv(h).T
+    dk = 0.5
+    g = pw.clmrcp(b, dk)
+    n = np.stack([g["nga"], g["ngb"], g["ngc"]], 1)
+    got = {tuple(v) for v in n}
+    assert len(got) == len(n)
+    assert not any(tuple(-v) in got for v in n)
+    r = 12
+    ball = set()
+    for v in itertools.product(range(-r, r + 1), repeat=3):
+        if v != (0, 0, 0) and float(np.sum((b @ np.array(v)) ** 2)) <= dk:
+            ball.add(v)
+    assert all(v in ball for v in got)
+    assert {v if v in got else tuple(-np.array(v)) for v in ball} == got
+    gv = (b @ n.T).T
+    assert np.allclose(g["recnrm"], np.sum(gv ** 2, 1), rtol=1e-14, atol=0)
+    assert np.all(np.diff(g["recnrm"]) >= -1e-9 * g["recnrm"].max() * 10)
+    assert list(g["kcsma"]) == [int(np.abs(n[:, i]).max()) for i in range(3)]

@@ -249,3 +249,22 @@ def test_verlet_updates_vs_reference(seed):
     b, v = O.rciprl(c["hcell"])
     rb, rv = R.rciprl(c["hcell"])
     assert np.array_equal(b, rb) and v == rv
+
+
+@pytest.mark.parametrize("which", ["si8", "si64", "sheared"])
+def test_clmrcp_bitexact(which):
+    """The library's Ewald G list (qx_clmrcp) equals the reference clmrcp's
+    (Sources/bulk.f90:556) bit for bit: order, integer vectors, Cartesian G
+    and |G|^2, kcsma."""
+    from qxmd_amd import pw
+    if which == "sheared":
+        h = np.array([[10.26, 0.3, 0.0], [0.0, 13.1, 0.2], [0.0, 0.0, 8.7]])
+    else:
+        h = diamond_si(1 if which == "si8" else 2).hcell
+    b = np.linalg.inv(h).T  # reciprocal vectors (columns), without 2 pi
+    dkmax2 = 37.0 * 0.35 ** 2 / np.pi ** 2
+    r = R.clmrcp(b, dkmax2)
+    g = pw.clmrcp(b, dkmax2)
+    assert r["numk2"] == len(g["nga"]) > 100
+    for k in ("nga", "ngb", "ngc", "gx", "gy", "gz", "recnrm", "kcsma"):
+        assert np.array_equal(r[k], g[k]), k
