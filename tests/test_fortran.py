@@ -44,3 +44,18 @@ def test_fortran_dropin_driver():
     print(r.stdout)
     assert r.returncode == 0, r.stdout
     assert "DROPIN OK" in r.stdout
+
+
+MODULE_DRIVER = os.path.join(ROOT, "tests", "fortran", "dropin_modules_check")
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(MODULE_DRIVER),
+                    reason="built only where the reference's flang build (oracle/_ref) exists")
+def test_fortran_module_dropins():
+    """schmidt, pwpchg and eigen by name, on the reference's own module data."""
+    r = subprocess.run([MODULE_DRIVER], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=300, cwd=ROOT)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
+    assert "MODULE DROPIN OK" in r.stdout
