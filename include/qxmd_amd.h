@@ -424,6 +424,42 @@ int qx_scf_potential(qx_ctx *ctx, double *ene);
 int qx_scf_get(qx_ctx *ctx, int what, double *host);
 int qx_scf_put(qx_ctx *ctx, int what, const double *host);
 
+/* ---------------- Bloch bands at several k-points ------------------------- */
+/* The reference's band algebra is Gamma-only (Sources/kpoints.f90:33-39; its
+ * k-point input block, Sources/input.f90:2016-2126, is commented out); these
+ * extend its H|psi> (hcnonloc + ekinetic, Sources/engrad.f90:411-606, 66-127;
+ * hckbpp_g, Sources/nlkbpp.f90:1284-1393) to complex Bloch bands c_nk(G),
+ * |k+G|^2 <= ecut, on the context's FFT grid and local potential
+ * (qx_set_glocal).  No reference interface exists for them.
+ *
+ * qx_kpoints_set: nk k-points kfrac (3 x nk, units of the reciprocal vectors),
+ * bvec the reciprocal vectors (3 x 3 column-major, columns b_i with 2 pi),
+ * ecut in Ry, nb bands per k-point; npw(nk) receives the sphere sizes.
+ * Allocates four k-slots of nk x nb bands. */
+int qx_kpoints_set(qx_ctx *ctx, int nk, const double *kfrac, const double *bvec, double ecut,
+                   int nb, int *npw);
+/* The sphere of k-point kfrac on grid nd(3) without a context: hkl(3, npw),
+ * kin(npw) as qx_kpoints_sphere; returns npw (-npw when cap is smaller; -1 on
+ * error). */
+int qx_ksphere(const double *bvec, const double *kfrac, double ecut, const int *nd, int cap,
+               int *hkl, double *kin);
+/* The sphere of k-point ik in its storage order (points grouped by (h, k)
+ * column, columns in lexicographic order, l rising): hkl(3, npw), kin(npw) =
+ * |k+G|^2. */
+int qx_kpoints_sphere(qx_ctx *ctx, int ik, int *hkl, double *kin);
+/* Bands 0..nb-1 of k-point ik in / out of a k-slot: psi(2 npw_ik, nb),
+ * complex interleaved. */
+int qx_kslot_put(qx_ctx *ctx, int slot, int ik, const double *psi, int nb);
+int qx_kslot_get(qx_ctx *ctx, int slot, int ik, double *psi, int nb);
+/* KB projectors of k-point ik: beta(2 npw_ik, npj) the coefficients
+ * beta_p(k+G) (complex interleaved), D(npj) their energies (vclalp);
+ * <beta_p|c> = sum_G conj(beta_p(G)) c(G).  Every k-point takes the same npj. */
+int qx_kset_kb(qx_ctx *ctx, int ik, int npj, const double *beta, const double *D);
+/* H_k c for bands 0..nb-1 of every k-point, k-slot src -> k-slot dst.
+ * per_band = 0: each stage over all (k, band) blocks in one launch; 1: one
+ * launch per (k, band) block (the per-band loop the batched form replaces). */
+int qx_khpsi_slot(qx_ctx *ctx, int src, int dst, int nb, int per_band);
+
 #ifdef __cplusplus
 }
 #endif

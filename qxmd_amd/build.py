@@ -23,7 +23,7 @@ ARCH = os.environ.get("QX_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["pencil.hip", "pencil_fast.hip", "gemm.hip", "mfma_gemm.hip", "blaslt.hip", "dense.hip",
                "eigh.hip",
-               "fullgrid.hip", "cg.hip", "nlsetup.hip", "kb1.hip", "forces.hip", "flocal.hip", "ewald.hip",
+               "fullgrid.hip", "kpoint.hip", "cg.hip", "nlsetup.hip", "kb1.hip", "forces.hip", "flocal.hip", "ewald.hip",
                "aspc.hip", "scfglue.hip", "api.hip"]
 HOST_SOURCES = ["gsphere.cpp", "restart.cpp", "mdstep.cpp"]
 FORTRAN_SOURCES = ["qxmd_amd_c.f90", "qxmd_dropin.f90"]

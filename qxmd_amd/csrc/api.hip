@@ -313,6 +313,7 @@ int qx_destroy(qx_ctx* h) {
     for (auto& e : t.ev) (void)hipEventDestroy(e);
   lt_destroy(c);
   cg_graphs_destroy(c);
+  kp_destroy(c);
   mg_destroy(c);
   nl_destroy(c);
   scf_destroy(c);
@@ -480,12 +481,24 @@ int qx_slot_get(qx_ctx* h, int slot, double* gdcr, int nb) {
   return get_reference(c, c->d_slot[slot], gdcr, nb);
 }
 
+__global__ void k_slot_copy(const double2* __restrict__ in, double2* __restrict__ out, long n2) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x)
+    out[i] = in[i];
+}
+
 int qx_slot_copy(qx_ctx* h, int src, int dst, int nb) {
   if (!h) return fail(-1, "qx_slot_copy: null context");
   Ctx* c = &h->c;
   if (int e = check_slots(c, "qx_slot_copy", nb, src, dst, 0)) return e;
-  QX_CHECK(hipMemcpyAsync(c->d_slot[dst], c->d_slot[src], sizeof(double) * c->nplwex * nb,
-                          hipMemcpyDeviceToDevice, c->stream));
+  // one dispatch of a plain streaming copy (16 bytes per lane): each byte of
+  // the bands is read once and written once, which also makes this launch the
+  // known-size reference of the HBM byte counters
+  const long n2 = (long)c->nplwex * nb / 2;  // nplwex = 2 (nplw + 1) is even
+  const int grid = (int)std::min<long>(cdiv(n2, 256), 8192);
+  hipLaunchKernelGGL(k_slot_copy, dim3(grid), dim3(256), 0, c->stream,
+                     reinterpret_cast<const double2*>(c->d_slot[src]),
+                     reinterpret_cast<double2*>(c->d_slot[dst]), n2);
+  QX_CHECK(hipGetLastError());
   return 0;
 }
 
@@ -1141,6 +1154,53 @@ int qx_scf_put(qx_ctx* h, int what, const double* host) {
   int rc = scf_io(&h->c, what, const_cast<double*>(host), true);
   return rc ? fail(rc, "qx_scf_put: only rho (0), rhgr (1) and rinr (2) can be set (%d)", what)
             : 0;
+}
+
+// ---- Bloch bands at several k-points (kpoint.hip)
+int qx_ksphere(const double* bvec, const double* kfrac, double ecut, const int* nd, int cap,
+               int* hkl, double* kin) {
+  if (!bvec || !kfrac || !nd || !(ecut > 0.0) || cap < 0)
+    return fail(-1, "qx_ksphere: bad arguments");
+  int n = kp_sphere_host(bvec, kfrac, ecut, nd[0], nd[1], nd[2], cap, hkl, kin);
+  if (n < -100000000) return fail(-1, "qx_ksphere: the sphere does not fit the grid");
+  return n;
+}
+
+int qx_kpoints_set(qx_ctx* h, int nk, const double* kfrac, const double* bvec, double ecut,
+                   int nb, int* npw) {
+  if (!h || !kfrac || !bvec) return fail(-1, "qx_kpoints_set: null argument");
+  int rc = kp_set(&h->c, nk, kfrac, bvec, ecut, nb, npw);
+  if (rc == -1) return fail(rc, "qx_kpoints_set: bad arguments");
+  if (rc == -2)
+    return fail(rc, "qx_kpoints_set: a k-point sphere does not fit the FFT grid (or the grid "
+                    "exceeds the plane kernel: n1 n2 <= 8192, n3 <= 128)");
+  return rc ? fail(rc, "qx_kpoints_set: device error %d", rc) : 0;
+}
+
+int qx_kpoints_sphere(qx_ctx* h, int ik, int* hkl, double* kin) {
+  int rc = h ? kp_sphere(&h->c, ik, hkl, kin) : -1;
+  return rc ? fail(rc, "qx_kpoints_sphere: no k-points set or ik out of range") : 0;
+}
+
+int qx_kslot_put(qx_ctx* h, int slot, int ik, const double* psi, int nb) {
+  int rc = h && psi ? kp_slot_put(&h->c, slot, ik, psi, nb) : -1;
+  return rc ? fail(rc, "qx_kslot_put: bad slot, k-point or band count (%d)", rc) : 0;
+}
+
+int qx_kslot_get(qx_ctx* h, int slot, int ik, double* psi, int nb) {
+  int rc = h && psi ? kp_slot_get(&h->c, slot, ik, psi, nb) : -1;
+  return rc ? fail(rc, "qx_kslot_get: bad slot, k-point or band count (%d)", rc) : 0;
+}
+
+int qx_kset_kb(qx_ctx* h, int ik, int npj, const double* beta, const double* D) {
+  int rc = h && (npj == 0 || (beta && D)) ? kp_set_kb(&h->c, ik, npj, beta, D) : -1;
+  return rc ? fail(rc, "qx_kset_kb: no k-points set, bad k-point or projector count (%d)", rc)
+            : 0;
+}
+
+int qx_khpsi_slot(qx_ctx* h, int src, int dst, int nb, int per_band) {
+  int rc = h ? kp_hpsi(&h->c, src, dst, nb, per_band) : -1;
+  return rc ? fail(rc, "qx_khpsi_slot: bad slots or band count, or no k-points set (%d)", rc) : 0;
 }
 
 }  // extern "C"

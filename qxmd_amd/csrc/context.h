@@ -179,6 +179,7 @@ struct Ctx {
   void* blaslt = nullptr;                 // hipBLASLt state (cross-check only)
   void* mg = nullptr;                     // own-GEMM state (workspace, tuned grids, FP32 copies)
   void* cg_graphs = nullptr;              // captured batched-CG iterations (cg.hip)
+  void* kp = nullptr;                     // Bloch bands at several k-points (kpoint.hip)
   bool cg_graph_on = true;                // replay them (qx_set_cg_graph)
   // bumped whenever a device buffer a captured graph may read or write is
   // (re)allocated or its contents' shape changes (KB table, GEMM workspace,
@@ -223,6 +224,17 @@ int lt_sgemm_f32(Ctx* c, bool ta, int M, int N, int K, const float* A, int lda, 
 // dimension), the caller falls back to the small-tile kernels
 void mg_destroy(Ctx* c);
 void cg_graphs_destroy(Ctx* c);
+// k-points (kpoint.hip)
+void kp_destroy(Ctx* c);
+int kp_set(Ctx* c, int nk, const double* kfrac, const double* bvec, double ecut, int nb,
+           int* npw_out);
+int kp_sphere(Ctx* c, int ik, int* hkl, double* kin);
+int kp_sphere_host(const double* bvec, const double* kf, double ecut, int n1, int n2, int n3,
+                   int cap, int* hkl, double* kin);
+int kp_slot_put(Ctx* c, int slot, int ik, const double* psi, int nb);
+int kp_slot_get(Ctx* c, int slot, int ik, double* psi, int nb);
+int kp_set_kb(Ctx* c, int ik, int npj, const double* beta, const double* D);
+int kp_hpsi(Ctx* c, int src, int dst, int nb, int per_band);
 int cg_graph_entries(Ctx* c);
 int mg_dgemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
                 const double* B, int ldb, double beta, double* C, int ldc, int corr);
@@ -238,12 +250,19 @@ int mg_bf3_nn(Ctx* c, int M, int N, int K, double alpha, const T* A, int lda, co
 // once per table) and FP32 work buffers i = 0..3 kept by the context
 const float* mg_kb_b32(Ctx* c);
 float* mg_f32_scratch(Ctx* c, int i, size_t n);
+const unsigned short* mg_kb_split(Ctx* c, int* ld, int* rows_pad, long* lo);
+unsigned short* mg_u16_scratch(Ctx* c, int i, size_t n);
+int mg_ps3_tn(Ctx* c, int M, int N, int K, const unsigned short* A, int lda, long alo,
+              const unsigned short* B, int ldb, long blo, float* C, int ldc);
+int mg_ps3_nn(Ctx* c, int M, int N, int K, const unsigned short* A, int lda, long alo,
+              const unsigned short* B, int ldb, long blo, float* C, int ldc);
 void mg_forget_kb(Ctx* c);
 int mg_tuned_shapes(Ctx* c);
 bool use_big(int M, int N);
 // KB term of the mixed band-apply from the FP32 copy psi32 of psi: kb32 =
 // B diag(D) (B^T W psi) in FP32 (nplwex x nb)
-int kb_mixed_f32(Ctx* c, const double* psi, const float* psi32, int nb, float* kb32);
+int kb_mixed_f32(Ctx* c, const double* psi, const float* psi32, const unsigned short* psis,
+                 int ldk, long slo, int nb, float* kb32);
 // mixed band-apply with the KB contractions between the pencil FFT passes:
 // the FP32 copy of psi comes out of the z inverse pass and the KB term goes
 // into the z forward pass.  Returns -3 when the shape is not covered.

@@ -152,6 +152,13 @@ SIGNATURES = {
     "qx_scf_potential": [_vp, _dp],
     "qx_scf_get": [_vp, _i, _dp],
     "qx_scf_put": [_vp, _i, _dp],
+    "qx_ksphere": [_dp, _dp, _d, _ip, _i, _vp, _vp],
+    "qx_kpoints_set": [_vp, _i, _dp, _dp, _d, _i, _ip],
+    "qx_kpoints_sphere": [_vp, _i, _ip, _dp],
+    "qx_kslot_put": [_vp, _i, _i, _dp, _i],
+    "qx_kslot_get": [_vp, _i, _i, _dp, _i],
+    "qx_kset_kb": [_vp, _i, _i, _dp, _dp],
+    "qx_khpsi_slot": [_vp, _i, _i, _i, _i],
 }
 # entry points beyond the header's core set (batched CG, kinetic energies)
 EXTRA = {

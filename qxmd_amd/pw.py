@@ -49,6 +49,24 @@ def clmrcp(b, dkmax2):
     return dict(nga=nga, ngb=ngb, ngc=ngc, gx=gx, gy=gy, gz=gz, recnrm=rec, kcsma=kc)
 
 
+def ksphere(bvec, kfrac, ecut, nd):
+    """The sphere |k+G|^2 <= ecut of k-point kfrac (units of the reciprocal
+    vectors bvec, columns with 2 pi) on grid nd, in the library's storage
+    order: (hkl (npw, 3) int32, |k+G|^2 (npw,))."""
+    L = load()
+    b = f64(np.asarray(bvec, np.float64).T.ravel())
+    kf = f64(np.asarray(kfrac, np.float64))
+    n = -L.qx_ksphere(b, kf, float(ecut), i32(nd), 0, None, None)
+    if n <= 0:
+        raise QxError("qx_ksphere failed")
+    hkl = np.zeros((n, 3), np.int32)
+    kin = np.zeros(n)
+    if L.qx_ksphere(b, kf, float(ecut), i32(nd), n, hkl.ctypes.data_as(C.c_void_p),
+                    kin.ctypes.data_as(C.c_void_p)) != n:
+        raise QxError("qx_ksphere failed")
+    return hkl, kin
+
+
 class GSphere:
     """get_mesh + strvec + set_rciprl + reclat for cell ``hcell`` (columns are
     lattice vectors, bohr) and cutoffs in Ry."""
@@ -239,6 +257,50 @@ class PlaneWave:
 
     def sync(self):
         check(load().qx_sync(self.h), "qx_sync")
+
+    # ---- Bloch bands at several k-points (k-slots, complex interleaved)
+    def kpoints_set(self, kfrac, bvec, ecut, nb):
+        """k-points kfrac (nk, 3) in units of the reciprocal vectors bvec
+        (columns, 2 pi included), sphere |k+G|^2 <= ecut (Ry), nb bands each.
+        Returns the sphere sizes (nk,)."""
+        kf = f64(np.asarray(kfrac, np.float64).reshape(-1, 3))
+        nk = kf.shape[0]
+        npw = np.zeros(nk, np.int32)
+        check(load().qx_kpoints_set(self.h, nk, kf, f64(np.asarray(bvec).T), float(ecut), int(nb),
+                                    npw), "qx_kpoints_set")
+        self._knpw = npw.copy()
+        self._knb = int(nb)
+        return npw
+
+    def kpoints_sphere(self, ik):
+        """(hkl (npw, 3), |k+G|^2 (npw,)) of k-point ik, in storage order."""
+        n = int(self._knpw[ik])
+        hkl = np.zeros((n, 3), np.int32)
+        kin = np.zeros(n)
+        check(load().qx_kpoints_sphere(self.h, int(ik), hkl, kin), "qx_kpoints_sphere")
+        return hkl, kin
+
+    def kput(self, slot, ik, psi):
+        """psi (nb, npw) complex -> k-slot."""
+        p = np.asarray(psi, np.complex128)
+        x = f64(p.view(np.float64).reshape(p.shape[0], -1))
+        check(load().qx_kslot_put(self.h, int(slot), int(ik), x, p.shape[0]), "qx_kslot_put")
+
+    def kget(self, slot, ik, nb=None):
+        nb = self._knb if nb is None else int(nb)
+        out = np.zeros((nb, 2 * int(self._knpw[ik])))
+        check(load().qx_kslot_get(self.h, int(slot), int(ik), out, nb), "qx_kslot_get")
+        return out.view(np.complex128)
+
+    def kset_kb(self, ik, beta, D):
+        """beta (npj, 2 npw_ik) complex interleaved, D (npj,)."""
+        b = f64(beta)
+        check(load().qx_kset_kb(self.h, int(ik), b.shape[0], b, f64(D)), "qx_kset_kb")
+
+    def khpsi(self, src=0, dst=1, nb=None, per_band=False):
+        nb = self._knb if nb is None else int(nb)
+        check(load().qx_khpsi_slot(self.h, int(src), int(dst), nb, int(bool(per_band))),
+              "qx_khpsi_slot")
 
     # ---- operations on resident slots
     def slot_copy(self, src, dst, nb):
