@@ -30,156 +30,94 @@ FP32_PEAK_TFS = 157.3
 BF16_PEAK_TFS = 2500.0
 
 
-def build_inputs(ncell, nband):
+NCPP = os.path.join(ROOT, "tests", "golden", "si_ncpp_lda.npz")
+CELL_TAG = {1: "si8", 2: "si64", 3: "si216", 4: "si512"}
+
+
+def ewald_list(gs, hcell, gamma=0.35):
+    """Ewald G list from the reference's clmrcp generator (its order): the
+    half space of reciprocal vectors with (pi |G| / gamma)^2 <= 37, in the
+    units clmrcp uses (no 2 pi)."""
     from qxmd_amd import pw
-    from qxmd_amd.system import (diamond_si, initial_wavefunctions, model_kb_projectors,
-                                 model_local_potential)
-    cell = diamond_si(ncell)
-    gs = pw.GSphere(cell.hcell)
-    v = model_local_potential(gs, cell)
-    eq, lmx, vcl, cl = model_kb_projectors(gs, cell)
-    psi = initial_wavefunctions(gs, nband)
-    return cell, gs, v, eq, lmx, vcl, cl, psi
+    ew = pw.clmrcp(np.linalg.inv(hcell).T, 37.0 * gamma ** 2 / np.pi ** 2)
+    n = len(ew["nga"])
+    return dict(nga=ew["nga"], ngb=ew["ngb"], ngc=ew["ngc"], gx=ew["gx"], gy=ew["gy"],
+                gz=ew["gz"], recnrm=ew["recnrm"], recnrmex=np.zeros(n + 1), gamma=gamma,
+                volume=gs.volume)
 
 
-def device_kb_table(ctx, cell, gs):
-    """Register the model s + p tables and build the KB table on the device
-    from the structure factors of the cell's atoms."""
-    from qxmd_amd import nlpp
-    nw = gs.nplw + 1
-    ctx.set_nlpp_tables(nlpp.model_tables(gs.gx[:nw], gs.gy[:nw], gs.gz[:nw]))
-    frac = np.linalg.solve(cell.hcell, cell.tau.T).T
-    n5 = gs.nplw5 + 1
-    g = [np.asarray(a[:n5], np.int32) for a in (gs.nga, gs.ngb, gs.ngc)]
-    kmax = [int(np.abs(a).max()) for a in g]
-    nd = np.array(gs.nd, np.int32)
-    natom = len(frac)
-    ctx.stscfn_pw(frac, cell.hcell, nd, nd, *g, gs.nplw, kmax, copy_out=False)
-    ctx.setnlc_g(1, np.array([1], np.int32), np.array([natom], np.int32), natom, cell.volume)
+class Workload:
+    """The silicon cell of BASELINE's configurations: n x n x n diamond cells
+    (a0 = 10.26 bohr) with seeded displacements, 20 / 80 Ry, the reference's
+    Si LDA norm-conserving pseudopotential (tables from its own set-up
+    routines), seeded damped random starting bands."""
+
+    def __init__(self, ncell, nband):
+        from qxmd_amd import ncpp, pw
+        from qxmd_amd.system import diamond_si, initial_wavefunctions, occupations
+        self.cell = diamond_si(ncell)
+        self.gs = pw.GSphere(self.cell.hcell)
+        self.natom = len(self.cell.tau)
+        self.nband = nband
+        self.tables, self.svco, self.zv = ncpp.silicon_cell_inputs(self.gs, np.load(NCPP),
+                                                                   CELL_TAG[ncell])
+        self.frac = np.linalg.solve(self.cell.hcell, self.cell.tau.T).T
+        self.psi0 = initial_wavefunctions(self.gs, nband)
+        self.occ = occupations(nband, 4 * self.natom)
+        self.ew = ewald_list(self.gs, self.cell.hcell)
+
+    def scf(self, ctx, x=None):
+        """A fresh SCF driver on ctx (new Pulay history) at positions x."""
+        from qxmd_amd.scf import Scf
+        sc = Scf(ctx, self.natom, self.tables, self.svco, self.zv)
+        sc.set_ewald(self.ew)
+        sc.set_positions(self.frac if x is None else x)
+        return sc
 
 
-def md_kb_setup(ctx, cell, gs, eq, lmx, vcl, reps=3):
-    """Time the per-MD-step KB inputs: stscfn_pw + setnlc_g on the device
-    (model s + p tables) against qx_set_kb's upload of the host-built table;
-    leaves the host-built table registered again."""
-    from qxmd_amd import nlpp
-    nw = gs.nplw + 1
-    t = nlpp.model_tables(gs.gx[:nw], gs.gy[:nw], gs.gz[:nw])
-    ctx.set_nlpp_tables(t)
-    frac = np.linalg.solve(cell.hcell, cell.tau.T).T
-    n5 = gs.nplw5 + 1
-    g = [np.asarray(a[:n5], np.int32) for a in (gs.nga, gs.ngb, gs.ngc)]
-    kmax = [int(np.abs(a).max()) for a in g]
-    nd = np.array(gs.nd, np.int32)
-    natom = len(frac)
-    nhk = np.array([1], np.int32), np.array([natom], np.int32)
-
-    def device():
-        ctx.stscfn_pw(frac, cell.hcell, nd, nd, *g, gs.nplw, kmax, copy_out=False)
-        ctx.setnlc_g(1, nhk[0], nhk[1], natom, cell.volume)
-
-    def host():
-        ctx.set_kb(eq, lmx, vcl)
-
-    out = {}
-    for name, fn in (("device_ms", device), ("host_upload_ms", host)):
+def time_host(ctx, fn, reps):
+    fn()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
         fn()
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            fn()
-        ctx.sync()
-        out[name] = round((time.perf_counter() - t0) / reps * 1e3, 3)
-    out["table_mb"] = round(8.0 * gs.nplwex * ctx.npj / 1e6, 1)
-    return out
+    ctx.sync()
+    return round((time.perf_counter() - t0) / reps * 1e3, 4)
 
 
-def scf_glue(ctx, cell, gs, v, nband, occ, reps=6, itratn=10):
-    """Time the SCF glue between two band updates, device-resident: pwpchg
-    (density of all bands + chgr2g), pwpmxchgg (Pulay over a filled, sliding
-    history + chgg2r and harting in one inverse transform) and scf_potential
-    (xclda + potential assembly + energies).  The occupations are perturbed at
-    random from call to call so that every mixing step sees a new output
-    density.  Runs after
-    everything else: it replaces the context's local potential."""
+def md_parts(ctx, w, sc, reps=3):
+    """Per-MD-step pieces on the device with the SCF's own state: the refresh
+    at new positions (stscfn_pw, setnlc_g's KB table, vlocal_g), the SCF glue
+    between band steps (outchg, Pulay mixing with the history held,
+    potential), the KB / local / reciprocal-Ewald forces."""
     from qxmd_amd import pw
-    rx = np.zeros(gs.nplw5 + 1)
-    rx[1:] = 4 * np.pi / gs.recnrm[1: gs.nplw5 + 1]
-    ctx.scf_set_grid(rx, v)
-    ctx.mxchgg_alloc(imxchg=1, itratn=itratn)
-    rng = np.random.default_rng(7)
-
-    def occ_k():
-        # independent perturbations keep successive residuals linearly independent
-        return occ * (1.0 + 0.02 * rng.standard_normal(len(occ)))
-
-    def glue():
-        ctx.pwpchg_slot(0, nband, occ_k())
-        ctx.pwpmxchgg(0.8, 0.6, lmixhold=True)
+    nb, occ = w.nband, w.occ
+    out = {"refresh_ms": time_host(ctx, lambda: sc.set_positions(w.frac), reps)}
+    ctx.put(0, w.psi_orth)
+    sc.start(None, occ)
+    for it in range(4):  # fill part of the Pulay history
+        ctx.scf_band_step(nb, occ, w.gs.volume, prec=pw.MIXED, batched=True, want_rho=False)
+        ctx.outchg()
+        ctx.pwpmxchgg(sc.aslh, sc.bslh, lreset=(it == 0))
         ctx.scf_potential()
 
-    ctx.pwpchg_slot(0, nband, occ)
-    ctx.scf_put("rinr", ctx.scf_get("rhgr"))
-    for _ in range(itratn + 2):  # fill the history
-        glue()
-    ctx.pwpchg_slot(0, nband, occ, pw.MIXED)  # first FP32 density call sets up its buffers
-    out = {"glue_total": time_region(ctx, glue, reps),
-           "pwpchg": time_region(ctx, lambda: ctx.pwpchg_slot(0, nband, occ_k()), reps),
-           "scf_potential": time_region(ctx, ctx.scf_potential, reps),
-           "pwpchg_fp32": time_region(ctx, lambda: ctx.pwpchg_slot(0, nband, occ_k(), pw.MIXED),
-                                      reps)}
-    out["pwpmxchgg"] = out["glue_total"] - out["pwpchg"] - out["scf_potential"]
-    out = {k: round(x, 4) for k, x in out.items()}
-    ctx.pwpchg_slot(0, nband, occ_k())
-    out["pulay_history"], out["pulay_resets"] = ctx.pwpmxchgg(0.8, 0.6, lmixhold=True)
+    def glue():
+        ctx.outchg()
+        ctx.pwpmxchgg(sc.aslh, sc.bslh, lmixhold=True)
+        ctx.scf_potential()
+
+    out["scf_glue_ms"] = time_host(ctx, glue, reps)
+    out["pwpchg_ms"] = time_host(ctx, lambda: ctx.pwpchg_slot(0, nb, occ), reps)
+    out["kb_forces_ms"] = time_host(ctx, lambda: ctx.kb_forces(0, nb, occ), reps)
+    one, nat = sc.nhk1, sc.nhk2
+    out["flocal_ms"] = time_host(
+        ctx, lambda: ctx.flocal_g_dev(sc.ntype, one, nat, sc.ions, sc._zero, w.natom), reps)
+    it = np.ones(w.natom, np.int32)
+    out["ewald_frcrec_ms"] = time_host(
+        ctx, lambda: ctx.frcrec(sc.ntype, one, nat, sc.zv, 4 * w.natom, w.natom, it, sc.ions),
+        reps)
+    out["ewald_g"] = len(w.ew["nga"])
     return out
-
-
-def flocal(ctx, cell, gs, reps=5):
-    """Time flocal_g on the device: axis tables from the device stscfn_pw over
-    the density sphere, a model local form factor (Gaussian-screened 1/G^2),
-    the density the SCF glue left in the context."""
-    n5 = gs.nplw5
-    g2 = gs.recnrm[: n5 + 1].copy()
-    g2[0] = 1.0
-    svco = (-8 * np.pi * 4.0 / gs.volume * np.exp(-0.25 * g2) / g2)[None, :]
-    svco[0, 0] = 0.0
-    ctx.set_lopp(svco)
-    frac = np.linalg.solve(cell.hcell, cell.tau.T).T
-    g = [np.asarray(a[: n5 + 1], np.int32) for a in (gs.nga, gs.ngb, gs.ngc)]
-    kmax = [int(np.abs(a).max()) for a in g]
-    ctx.stscfn_pw(frac, cell.hcell, gs.nd, gs.nd, *g, gs.nplw, kmax, copy_out=False)
-    natom = len(frac)
-    args = (1, [1], [natom], np.arange(1, natom + 1), [0], natom)
-    ctx.flocal_g_dev(*args)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        ctx.flocal_g_dev(*args)
-    return round((time.perf_counter() - t0) * 1e3 / reps, 4)
-
-
-def ewald(ctx, cell, gs, gamma=0.35, reps=5):
-    """Time frcrec on the device: the Ewald G list = the half-sphere G with
-    (pi g / gamma)^2 < 37 (units without 2 pi, as clmrcp builds it), one type
-    with zv = 4, the per-atom axis tables of the last device stscfn_pw."""
-    tp = 2 * np.pi
-    g2 = gs.recnrm[1: gs.nplw5 + 1] / tp ** 2
-    n = int(np.searchsorted(g2, 37.0 * gamma ** 2 / np.pi ** 2))
-    m = slice(1, n + 1)
-    ew = dict(nga=gs.nga[m], ngb=gs.ngb[m], ngc=gs.ngc[m], gx=gs.gx[m] / tp, gy=gs.gy[m] / tp,
-              gz=gs.gz[m] / tp, recnrm=g2[:n], recnrmex=np.zeros(n + 1), gamma=gamma,
-              volume=gs.volume)
-    ctx.set_ewald(ew)
-    natom = len(cell.tau)
-    args = (1, [1], [natom], [4.0], 4 * natom, natom, np.ones(natom, np.int32),
-            np.arange(1, natom + 1))
-    ctx.frcrec(*args)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        ctx.frcrec(*args)
-    return round((time.perf_counter() - t0) * 1e3 / reps, 4), n
 
 
 def extrapolation(ctx, psi, nband, reps=3):
@@ -203,88 +141,61 @@ def extrapolation(ctx, psi, nband, reps=3):
     return {"step_ms": round(1e3 * float(np.mean(times)), 3), "iaspc": ia, "lcholesky": ok}
 
 
-def md_run(ctx, cell, gs, nband, occ, psi, nsteps=3, max_scf=40, tol_ha_atom=1e-6,
-           dt=40.0):
-    """Converged MD steps at constant energy (Si masses, dt in a.u.), the SCF
-    in mixed precision, device-resident except the O(natom) Verlet updates.
-    Per step: first half kick and position update (host); stscfn_pw,
-    setnlc_g (KB table) and vlocal_g (local potential) at the new positions
-    on the device; sbalinaspc; SCF iterations (band step + outchg + Pulay +
-    potential) until the Harris-Foulkes energy changes by less than
-    tol_ha_atom per atom (Pulay history reset at the step's first iteration);
-    KB, local and Ewald reciprocal-space forces on the device; second half
-    kick.  The local form factors are the Fourier transform of the Gaussian
-    wells of model_local_potential (the same potential), so the local forces
-    are the gradient of the potential the SCF sees; the Ewald real-space sum
-    is not part of the path.  Step 0 converges from the starting bands.
-    Runs last: it replaces the context's local potential tables."""
-    from qxmd_amd import md, nlpp, pw
-    natom = len(cell.tau)
-    hcell = cell.hcell
-    nw = gs.nplw + 1
-    ctx.set_nlpp_tables(nlpp.model_tables(gs.gx[:nw], gs.gy[:nw], gs.gz[:nw]))
-    n5 = gs.nplw5 + 1
-    g = [np.asarray(a[:n5], np.int32) for a in (gs.nga, gs.ngb, gs.ngc)]
-    kmax = [int(np.abs(a).max()) for a in g]
-    nd = np.array(gs.nd, np.int32)
-    one, nat = np.array([1], np.int32), np.array([natom], np.int32)
+def scf_converge(ctx, w, prec, batched, refine=None, tol=1e-6, max_scf=80):
+    """SCF from the shared start (orthonormal seeded bands, their density) to
+    a change of the Harris-Foulkes energy below tol Ha/atom; then the final
+    Rayleigh-Ritz and the forces.  Timed from the first band step to the
+    forces."""
+    sc = w.scf(ctx)
+    ctx.put(0, w.psi_orth)
+    sc.start(None, w.occ)
+    ctx.sync()
+    t0 = time.perf_counter()
+    it, conv, hist = sc.iterate(w.occ, tol_ha_atom=tol, max_scf=max_scf, prec=prec,
+                                batched=batched, refine_ha_atom=refine)
+    eig, e = sc.finish(w.occ)
+    frc = sc.forces(w.occ)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    return {"ms": round(dt * 1e3, 2), "iterations": it, "converged": conv,
+            "fp64_refinement_steps": sc.fp64_steps, "energy_ry": e, "eig": eig, "forces": frc}
+
+
+def scf_errors(r, ref, natom):
+    """Errors of run r against ref (Ha units): energy per atom, eigenvalues of
+    the occupied bands, forces."""
+    nocc = 2 * natom
+    return {"energy_ha_per_atom": float(abs(r["energy_ry"] - ref["energy_ry"]) / 2 / natom),
+            "max_occ_eig_ha": float(np.abs(r["eig"][:nocc] - ref["eig"][:nocc]).max() / 2),
+            "max_force_ha_bohr": float(np.abs(r["forces"] - ref["forces"]).max() / 2)}
+
+
+def md_run(ctx, w, nsteps=3, max_scf=40, tol_ha_atom=1e-6, dt=40.0):
+    """Converged MD steps at constant energy (Si masses, dt in a.u.), each SCF
+    in mixed precision with the final FP64 refinement sweep, device-resident
+    except the O(natom) Verlet updates.  Per step: half kick and position
+    update (host); structure factors, KB table and local potential at the new
+    positions (device); sbalinaspc; SCF iterations until the Harris-Foulkes
+    energy changes by less than tol_ha_atom per atom; KB, local and
+    reciprocal-space Ewald forces (the Ewald real-space sum is not part of the
+    path); second half kick.  Step 0 converges from the starting bands."""
+    from qxmd_amd import md, pw
+    natom, nb, occ = w.natom, w.nband, w.occ
+    hcell = w.cell.hcell
     species = np.ones(natom, np.int32)
     lfix = np.zeros(1, bool)
     acon = md.acon([28.0855 * 1822.888486], dt)
-    rx = np.zeros(gs.nplw5 + 1)
-    rx[1:] = 4 * np.pi / gs.recnrm[1: gs.nplw5 + 1]
-    x = np.linalg.solve(hcell, cell.tau.T).T
+    x = w.frac.copy()
     v = np.zeros_like(x)
-    # Gaussian wells -depth exp(-r^2 / 2 w^2) per atom plus a constant shift
-    # (model_local_potential's defaults), as form factors over the density sphere
-    depth, w, shift = 1.2, 1.1, 0.4
-    g2 = np.asarray(gs.recnrm[: gs.nplw5 + 1])
-    svco = -depth * (2 * np.pi * w * w) ** 1.5 * np.exp(-0.5 * g2 * w * w)
-    svco[0] += shift * cell.volume / natom
-    ctx.set_lopp(svco[None, :])
-    ctx.scf_set_grid(rx)
-    ions = np.arange(1, natom + 1)
-
-    def refresh(x):
-        ctx.stscfn_pw(x, hcell, nd, nd, *g, gs.nplw, kmax, copy_out=False)
-        ctx.setnlc_g(1, one, nat, natom, cell.volume)
-        ctx.vlocal_g(1, one, nat, ions, [0], natom, cell.volume, copy_out=False)
-
-    def scf():
-        eprev = None
-        for it in range(max_scf):
-            eig, _, ok = ctx.scf_band_step(nband, occ, cell.volume, prec=pw.MIXED, batched=True,
-                                           want_rho=False)
-            ctx.outchg()
-            ctx.pwpmxchgg(0.8, 0.6, lreset=(it == 0))
-            en = ctx.scf_potential()
-            e = (float(np.sum(occ * np.sort(eig))) - en["ehar"] + en["eexc"] + en["ecor"]
-                 - en["evexc"] - en["evcor"])
-            if eprev is not None and abs(e - eprev) / 2.0 / natom < tol_ha_atom:
-                return it + 1, True
-            eprev = e
-        return max_scf, False
-
-    def forces():
-        f = ctx.kb_forces(0, nband, occ)
-        f = f + ctx.flocal_g_dev(1, one, nat, ions, [0], natom)
-        fe, _, _ = ctx.frcrec(1, one, nat, [4.0], 4 * natom, natom,
-                              np.ones(natom, np.int32), np.arange(1, natom + 1))
-        return f + fe
-
-    # step 0: converge from the starting bands and density
-    ctx.put(0, psi)
+    sc = w.scf(ctx, x)
+    ctx.put(0, w.psi_orth)
     ctx.aspc_init(3, 2)
-    ctx.mxchgg_alloc(imxchg=1, itratn=10)
-    refresh(x)
-    ctx.pwpchg_slot(0, nband, occ)
-    ctx.scf_put("rinr", ctx.scf_get("rhgr"))
-    ctx.pwpmxchgg(0.8, 0.6)
-    ctx.scf_potential()
+    sc.start(None, occ)
     ctx.sync()
     t0 = time.perf_counter()
-    n0, conv0 = scf()
-    a = md.normalized_a(forces(), species, hcell, acon)
+    n0, conv0, _ = sc.iterate(occ, tol_ha_atom, max_scf, pw.MIXED, True,
+                              refine_ha_atom=tol_ha_atom)
+    a = md.normalized_a(sc.forces(occ), species, hcell, acon)
     v = md.vkick(v, a, species, lfix)
     ctx.sync()
     first = {"scf_iterations": n0, "converged": conv0,
@@ -295,10 +206,11 @@ def md_run(ctx, cell, gs, nband, occ, psi, nsteps=3, max_scf=40, tol_ha_atom=1e-
         t0 = time.perf_counter()
         v = md.vkick(v, a, species, lfix)
         x = md.updtps_pbc(md.updtps(x, v, species, lfix))
-        refresh(x)
-        ctx.sbalinaspc(0, nband, k)
-        n, conv = scf()
-        a = md.normalized_a(forces(), species, hcell, acon)
+        sc.set_positions(x)
+        ctx.sbalinaspc(0, nb, k)
+        n, conv, _ = sc.iterate(occ, tol_ha_atom, max_scf, pw.MIXED, True,
+                                refine_ha_atom=tol_ha_atom)
+        a = md.normalized_a(sc.forces(occ), species, hcell, acon)
         v = md.vkick(v, a, species, lfix)
         ctx.sync()
         steps.append((time.perf_counter() - t0) * 1e3)
@@ -306,16 +218,22 @@ def md_run(ctx, cell, gs, nband, occ, psi, nsteps=3, max_scf=40, tol_ha_atom=1e-
         convs.append(conv)
     return {"step_ms": round(float(np.mean(steps)), 2), "steps": nsteps,
             "scf_iterations": iters, "converged": all(convs), "tol_ha_atom": tol_ha_atom,
-            "scf_prec": "mixed", "first_step": first}
+            "scf_prec": "mixed + FP64 refinement", "first_step": first}
 
 
-def fft_stage_bytes(gs, s):
-    """Algorithmic FFT-stage bytes per band-apply: 8*N_h*s_c + N*s_r + 2*(nplw+1)*s_c
-    (two real 3-D transforms as two HBM passes each, V(r) read, sphere I/O)."""
+def fft_stage_lower_bytes(gs, ncol, nb):
+    """Lower bound of the HBM bytes one launch of the fused mixed FFT stage
+    (k_zinv_f + k_plane_f + k_zfwd_f over nb bands) must move in this design:
+    per band the sphere psi in and H psi out (FP64, 8 nplwex bytes each), the
+    FP32 copy of psi handed to the KB products and their FP32 result read
+    back (4 nplwex each), and the z-pencil intermediates T1 and T2, each
+    written once and read once (ncol columns x n points, FP32 complex, per
+    band pair: 16 ncol n bytes per band); V(r) once per launch (FP32).  The
+    kinetic term's second read of psi and V(r) re-reads beyond the first are
+    not counted: the design could avoid them."""
     n1, n2, n3 = gs.nd
-    N = n1 * n2 * n3
-    Nh = (n1 // 2 + 1) * n2 * n3
-    return 8 * Nh * 2 * s + N * s + 2 * (gs.nplw + 1) * 2 * s
+    per_band = 24 * gs.nplwex + 16 * ncol * n3
+    return nb * per_band + 4 * n1 * n2 * n3, per_band
 
 
 def time_region(ctx, fn, reps):
@@ -330,10 +248,19 @@ def time_region(ctx, fn, reps):
 
 def cpu_worker(ncell, budget_s):
     """Oracle H|psi> (FP64) on the host over the workload's bands, for as many
-    as fit the budget (about budget_s of CPU work)."""
+    as fit the budget (about budget_s of CPU work).  The potential and the KB
+    table are the model ones of qxmd_amd.system (same grid, sphere and
+    projector count, so the same operation count)."""
     import oracle as O
+    from qxmd_amd import pw
+    from qxmd_amd.system import (diamond_si, initial_wavefunctions, model_kb_projectors,
+                                 model_local_potential)
     nband = int(2.5 * 8 * ncell ** 3)
-    cell, gs, v, eq, lmx, vcl, cl, psi = build_inputs(ncell, nband)
+    cell = diamond_si(ncell)
+    gs = pw.GSphere(cell.hcell)
+    v = model_local_potential(gs, cell)
+    eq, lmx, vcl, _ = model_kb_projectors(gs, cell)
+    psi = initial_wavefunctions(gs, nband)
     H = O.Hamiltonian(gs.nplw, gs.nd, gs.ijkgd(), v, gs.dkgnrm(), eq, lmx, vcl)
     H.hpsi(psi[:1])  # warm the twiddle tables
     t0 = time.perf_counter()
@@ -347,30 +274,15 @@ def cpu_worker(ncell, budget_s):
     print(json.dumps({"bands": n, "seconds": dt}))
 
 
-def measured_traffic(key):
-    """HBM bytes per launch measured by the calibrated rocprofv3 FETCH_SIZE /
-    WRITE_SIZE runs summarized in profiles/ (None when not measured)."""
-    import glob
-    for path in glob.glob(os.path.join(ROOT, "profiles", "pmc_*.md")):
-        with open(path) as f:
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
             for line in f:
-                parts = line.split()
-                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == key:
-                    return float(parts[2])
-    return None
-
-
-def copy_probe(nbytes=1 << 30, reps=3):
-    """Streaming kernels of known size whose HBM byte counters (FETCH_SIZE,
-    WRITE_SIZE) calibrate those of the path's kernels: a fill (writes nbytes)
-    and an elementwise scale y = 2x (reads nbytes, writes nbytes)."""
-    import torch
-    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
-    for _ in range(reps):
-        torch.mul(x, 2.0, out=y)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes_each_way": nbytes, "reps": reps}), file=sys.stderr)
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(ncell, threads, budget_s=12.0):
@@ -381,8 +293,236 @@ def cpu_baseline(ncell, threads, budget_s=12.0):
                        timeout=600)
     if r.returncode != 0:
         return None
-    d = json.loads(r.stdout.strip().splitlines()[-1])
-    return d
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def kpoint_bench(steps, nb=160):
+    """The 64-atom cell at the 8 k-points {0, 1/2}^3 (complex bands, each
+    k-point its own sphere), nb bands each, FP64: H_k|psi> of all 8 x nb
+    (k-point, band) blocks with one launch per stage, against the per-band
+    launch loop.  Local potential: a smooth synthetic V(r); projectors: the
+    reference's Si tables at k+G."""
+    from qxmd_amd import kpoints, ncpp, pw
+    from qxmd_amd.system import diamond_si
+    cell = diamond_si(2)
+    gs = pw.GSphere(cell.hcell)
+    ctx = pw.PlaneWave(gs, 2)
+    n1, n2, n3 = gs.nd
+    x = np.arange(n1) / n1
+    V = -0.3 + 0.05 * np.cos(2 * np.pi * (x[None, None, :] + 2 * x[None, :, None]
+                                           + 3 * x[:, None, None]))
+    ctx.set_glocal(np.ascontiguousarray(V.ravel()))
+    b = 2 * np.pi * np.linalg.inv(cell.hcell).T
+    kf = kpoints.gamma_folding_kpoints(2)
+    npw = ctx.kpoints_set(kf, b, 20.0, nb)
+    t, _, _ = ncpp.silicon_cell_inputs(gs, np.load(NCPP), "si512")
+    rng = np.random.default_rng(7)
+    for ik in range(len(kf)):
+        hkl, kin = ctx.kpoints_sphere(ik)
+        beta, D = kpoints.kb_projectors(t, cell.tau, np.zeros(cell.natom, int), b, kf[ik], hkl,
+                                        cell.volume)
+        ctx.kset_kb(ik, beta, D)
+        c = (rng.standard_normal((nb, len(kin))) + 1j * rng.standard_normal((nb, len(kin))))
+        ctx.kput(0, ik, c / (1.0 + kin)[None, :])
+    ctx.khpsi(0, 1)  # first use (GEMM grid tuning)
+    ctx.khpsi(0, 2, per_band=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.khpsi(0, 1)
+    ctx.sync()
+    ms_b = (time.perf_counter() - t0) * 1e3 / steps
+    t0 = time.perf_counter()
+    ctx.khpsi(0, 2, per_band=True)
+    ctx.sync()
+    ms_l = (time.perf_counter() - t0) * 1e3
+    diff = max(float(np.abs(ctx.kget(1, ik) - ctx.kget(2, ik)).max()) for ik in range(len(kf)))
+    ctx.close()
+    nkb = len(kf) * nb
+    return {"workload": f"Si64, 8 k-points {{0,1/2}}^3, {nb} bands each, FP64, grid {n1}^3",
+            "npw_per_k": [int(v) for v in npw], "npj": int(t.lmx[0]) * cell.natom,
+            "batched_ms": round(ms_b, 3), "batched_band_applies_per_s": round(nkb / ms_b * 1e3, 1),
+            "per_band_loop_ms": round(ms_l, 2),
+            "per_band_loop_band_applies_per_s": round(nkb / ms_l * 1e3, 1),
+            "speedup_batched_vs_loop": round(ms_l / ms_b, 2),
+            "max_abs_diff_batched_vs_loop": diff}
+
+
+# ---- hardware counters of the FFT stage, measured in this run --------------
+FFT_REGEX = "k_zinv_f|k_plane_f|k_zfwd_f|k_slot_copy"
+PMC_PASSES = {
+    "fetch": (["FETCH_SIZE"], FFT_REGEX),
+    "write": (["WRITE_SIZE"], FFT_REGEX),
+    "sq": (["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT",
+            "SQ_ACTIVE_INST_LDS", "SQ_BUSY_CYCLES"], FFT_REGEX),
+    # MFMA work and busy cycles of our GEMM kernels (MOPS counters count
+    # units of 512 flops)
+    # MFMA work and busy share of our GEMM kernels: rocprofv3's derived
+    # MfmaUtil = sum SQ_VALU_MFMA_BUSY_CYCLES / (max GRBM_GUI_ACTIVE x SIMDs) x 100
+    # and MfmaFlops* = SQ_INSTS_VALU_MFMA_MOPS_* x 512
+    "mfma": (["MfmaUtil", "MfmaFlopsBF16", "MfmaFlopsF64"], "k_mg_bf3|k_mg_f64"),
+}
+PMC_KERNELS = ("k_zinv_f<float", "k_plane_f<float", "k_zfwd_f<float")
+MG_KERNELS = {"k_mg_bf3<float, float, true>": "bf3_tn", "k_mg_bf3<float, float, false>": "bf3_nn",
+              "k_mg_f64<true>": "f64_tn", "k_mg_f64<false>": "f64_nn"}
+
+
+def pmc_worker(ncell, nband, steps):
+    """Runs under rocprofv3 --pmc: a device copy of all bands (slot 0 -> slot
+    5: known bytes each way, the counters' calibration), then `steps` mixed
+    H|psi> launches of the workload; the counters are read per dispatch."""
+    from qxmd_amd import pw
+    w = Workload(ncell, nband)
+    ctx = pw.PlaneWave(w.gs, nband)
+    sc = w.scf(ctx)
+    ctx.put(0, w.psi0)
+    sc.start(None, w.occ)
+    ctx.slot_copy(0, 5, nband)  # nband * nplwex * 8 bytes read and written
+    ctx.sync()
+    for _ in range(steps):
+        ctx.hpsi(0, 1, nband, pw.MIXED)
+    ctx.sync()
+    for _ in range(steps):  # the FP64 KB products (k_mg_f64) for the MFMA counters
+        ctx.hpsi(0, 1, nband, pw.FP64)
+    ctx.sync()
+    ctx.close()
+    print("pmc worker done", flush=True)
+
+
+def _pmc_rows(outdir):
+    import csv
+    import glob
+    rows = []
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            rows.extend(csv.DictReader(f))
+    return rows
+
+
+def _pmc_last(rows, keep):
+    """Counter values of the last `keep` dispatches of each kernel of the
+    stage (the worker's timed launches; earlier dispatches are the tuners'
+    trial launches at other shapes), and of the last slot copy."""
+    per = {}
+    for row in rows:
+        k = row.get("Kernel_Name", "")
+        kern = next((p.split("<")[0] for p in PMC_KERNELS if p in k), None)
+        if kern is None and "k_slot_copy" in k:
+            kern = "copy"
+        if kern is None:
+            kern = next((v for n, v in MG_KERNELS.items() if n in k), None)
+        if kern is None:
+            continue
+        did = int(row.get("Dispatch_Id") or row.get("Correlation_Id") or 0)
+        per.setdefault((kern, row["Counter_Name"]), {})
+        d = per[(kern, row["Counter_Name"])]
+        d[did] = d.get(did, 0.0) + float(row["Counter_Value"])
+    out = {}
+    for (kern, cn), d in per.items():
+        ids = sorted(d)[-(1 if kern == "copy" else keep):]
+        out[(kern, cn)] = [d[i] for i in ids]
+    return out
+
+
+def _pmc_pass(cmd, outdir, limit_s):
+    """Run one counter pass as a child process group.  rocprofv3 writes its
+    CSV at the tool's finalisation, after the worker's last line; a pass whose
+    CSV has been written and stopped growing is ended there (the profiler's
+    own exit can stall in its signal handlers after that).  Returns (rows,
+    worker finished, seconds, return code)."""
+    import signal
+    logp = outdir + ".log"
+    os.makedirs(os.path.dirname(logp), exist_ok=True)
+    t0 = time.perf_counter()
+    with open(logp, "w") as log:
+        p = subprocess.Popen(cmd, stdout=log, stderr=subprocess.STDOUT, start_new_session=True)
+        last, stable = -1, 0
+        while p.poll() is None:
+            time.sleep(0.5)
+            if time.perf_counter() - t0 > limit_s:
+                break
+            sizes = sum(os.path.getsize(os.path.join(r, f)) for r, _, fs in os.walk(outdir)
+                        for f in fs if f.endswith("counter_collection.csv")) \
+                if os.path.isdir(outdir) else 0
+            stable = stable + 1 if sizes > 0 and sizes == last else 0
+            last = sizes
+            if stable >= 6:  # 3 s without growth after the CSV appeared
+                break
+        if p.poll() is None:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.wait()
+    with open(logp) as f:
+        done = "pmc worker done" in f.read()
+    return _pmc_rows(outdir), done, time.perf_counter() - t0, p.returncode
+
+
+def pmc_measure(ncell, nband, steps=3, limit_s=120):
+    """Three rocprofv3 --pmc passes over pmc_worker (FETCH_SIZE, WRITE_SIZE, SQ
+    counters), each a child process under its own time limit.  Returns the
+    per-dispatch means per kernel and counter, and the calibration factors
+    of FETCH_SIZE / WRITE_SIZE from the device copy of the bands."""
+    import shutil
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None, "rocprofv3 not found"
+    base = tempfile.mkdtemp(prefix="qx_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
+    stats = {}
+    for name, (counters, regex) in PMC_PASSES.items():
+        d = os.path.join(base, name)
+        cmd = ["timeout", "-s", "KILL", str(limit_s), prof, "--pmc", *counters,
+               "--kernel-include-regex", regex,
+               "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
+               os.path.abspath(__file__), "--pmc-worker", "--ncell", str(ncell),
+               "--nband", str(nband), "--steps", str(steps)]
+        print(f"# counter pass {name} ({' '.join(counters)})", file=sys.stderr, flush=True)
+        rows, done, sec, rc = _pmc_pass(cmd, d, limit_s)
+        print(f"# counter pass {name}: {len(rows)} rows, worker finished {done}, rc {rc}, "
+              f"{sec:.0f} s", file=sys.stderr, flush=True)
+        if not done or not rows:
+            return None, f"pass {name}: rc {rc}, worker finished {done}, {len(rows)} rows"
+        for key, vals in _pmc_last(rows, steps).items():
+            stats.setdefault(key, []).extend(vals)
+    mean = {f"{k[0]}:{k[1]}": float(np.mean(v)) for k, v in stats.items()}
+    return mean, None
+
+
+def fft_counter_roof(mean, fft_avg_ms, lower_bytes, copy_bytes):
+    """Traffic and VALU figures of the FFT stage from the counters: bytes =
+    counter (KiB) x 1024 / calibration, the calibration being what the same
+    counter reads, per byte, for the device copy of all bands (copy_bytes
+    read and written)."""
+    wcal = mean.get("copy:WRITE_SIZE", 0.0) * 1024 / copy_bytes
+    fcal = mean.get("copy:FETCH_SIZE", 0.0) * 1024 / copy_bytes
+    if wcal <= 0 or fcal <= 0:
+        return None
+    out = {"fetch_calibration": round(fcal, 4), "write_calibration": round(wcal, 4),
+           "per_kernel_bytes": {}}
+    if not (0.25 <= fcal <= 2.0 and 0.25 <= wcal <= 2.0):
+        # a streaming copy of known size reads outside this range only when the
+        # counters did not see the whole dispatch: no traffic figure then
+        out["calibration_implausible"] = True
+        return out
+    tot = 0.0
+    for k in ("k_zinv_f", "k_plane_f", "k_zfwd_f"):
+        rd = mean.get(f"{k}:FETCH_SIZE", 0.0) * 1024 / fcal
+        wr = mean.get(f"{k}:WRITE_SIZE", 0.0) * 1024 / wcal
+        out["per_kernel_bytes"][k] = {"read": round(rd), "written": round(wr)}
+        tot += rd + wr
+    out["traffic"] = round(tot)
+    out["traffic_gbs"] = round(tot / (fft_avg_ms * 1e-3) / 1e9, 1)
+    out["traffic_frac"] = round(out["traffic_gbs"] / HBM_PEAK_GBS, 4)
+    out["traffic_over_lower_bound"] = round(tot / lower_bytes, 3)
+    waves = mean.get("k_plane_f:SQ_WAVES", 0.0)
+    valu = mean.get("k_plane_f:SQ_INSTS_VALU", 0.0)
+    if waves > 0:
+        out["k_plane_f"] = {
+            "waves": waves, "valu_instr_per_wave": round(valu / waves, 1),
+            "valu_instr": valu,
+            "lds_bank_conflict_over_active_lds": round(
+                mean.get("k_plane_f:SQ_LDS_BANK_CONFLICT", 0.0) /
+                max(mean.get("k_plane_f:SQ_ACTIVE_INST_LDS", 1.0), 1.0), 3)}
+    return out
 
 
 def main():
@@ -392,40 +532,41 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--ncell", type=int, default=3)
     ap.add_argument("--nband", type=int, default=0)
-    ap.add_argument("--no-scf", action="store_true")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-scf", action="store_true", help="skip the SCF and MD measurements")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--no-kpoints", action="store_true", help="skip the k-point measurement")
     ap.add_argument("--cpu-worker", action="store_true")
+    ap.add_argument("--pmc-worker", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="first run a plain 1 GiB device copy (calibrates the HBM byte counters "
-                         "of a rocprofv3 --pmc run over the same process)")
     args = ap.parse_args()
+    natom = 8 * args.ncell ** 3
+    nband = args.nband or int(2.5 * natom)  # 20 / 160 / 540 / 1280
     if args.cpu_worker:
         cpu_worker(args.ncell, args.cpu_budget)
         return
-    if args.copy_probe:
-        copy_probe()
+    if args.pmc_worker:
+        pmc_worker(args.ncell, nband, args.steps)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs the single-GPU path only (one rank, one MI355X); "
               "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
 
     from qxmd_amd import pw
-    from qxmd_amd.system import occupations
-    natom = 8 * args.ncell ** 3
-    nband = args.nband or int(2.5 * natom)  # 20 / 160 / 540 / 1280
-    nel = 4 * natom
     t_setup = time.perf_counter()
-    cell, gs, v, eq, lmx, vcl, cl, psi0 = build_inputs(args.ncell, nband)
+    w = Workload(args.ncell, nband)
+    gs = w.gs
     ctx = pw.PlaneWave(gs, nband)
-    ctx.set_glocal(v)
-    ctx.set_kb(eq, lmx, vcl)
-    ctx.put(0, psi0)
+    sc = w.scf(ctx)
+    ctx.put(0, w.psi0)
     assert ctx.schmidt(0, nband)
-    psi_orth = ctx.get(0, nband)
+    w.psi_orth = ctx.get(0, nband)
+    sc.start(None, w.occ)  # V(r) of the starting density; KB table of the cell
+    ncol = ctx.info()["ncol"]
     t_setup = time.perf_counter() - t_setup
     print(f"# setup {t_setup:.1f}s: Si{natom}, grid {gs.nd}, nplw {gs.nplw}, nband {nband}, "
-          f"npj {ctx.npj}", file=sys.stderr, flush=True)
+          f"npj {ctx.npj}, ncol {ncol}", file=sys.stderr, flush=True)
 
     def hpsi(prec):
         return lambda: ctx.hpsi(0, 1, nband, prec)
@@ -445,7 +586,6 @@ def main():
           f"{fft_ms / max(fft_n, 1):.3f} ms", file=sys.stderr, flush=True)
     ms_fp64 = time_region(ctx, hpsi(pw.FP64), args.steps)
     print(f"# H|psi> fp64 : {ms_fp64:.3f} ms/step", file=sys.stderr, flush=True)
-    # stage timings (same stream, HIP events)
     reps = max(3, args.steps // 2)
     ms_loc32 = time_region(ctx, lambda: ctx.local(0, 1, nband, pw.MIXED), reps)
     ms_loc64 = time_region(ctx, lambda: ctx.local(0, 1, nband, pw.FP64), reps)
@@ -453,139 +593,133 @@ def main():
     print(f"# stages: local fp32 {ms_loc32:.3f} ms, local fp64 {ms_loc64:.3f} ms, "
           f"KB fp64 {ms_kb:.3f} ms", file=sys.stderr, flush=True)
 
-    # roofline of the dominant kernel of the mixed step, from the launch timers:
-    #  - KB: each GEMM launch (projection, back-projection) does 2*npj*nplwex*nb
-    #    flops on FP32 MFMA;
-    #  - FFT stage (zinv + plane + zfwd of one band batch): the algorithmic bytes
-    #    8*N_h*s_c + N*s_r + 2(nplw+1)*s_c per band, FP32.
+    # rooflines of the two stages of the mixed step, from the launch timers
     nplwex = gs.nplwex
     kb_launch_flops = 2.0 * ctx.npj * nplwex * nband
-    fft_bands_per_launch = nband * args.steps / max(fft_n, 1)
-    fft_launch_bytes = fft_stage_bytes(gs, 4) * fft_bands_per_launch
     kb_avg = kb_ms / max(kb_n, 1)
     fft_avg = fft_ms / max(fft_n, 1)
-    # FP32 KB products run on hipBLASLt; its default FAST_TF32 compute type splits
-    # the FP32 operands over the bf16 MFMA pipe (the only way past the 157.3 TF/s
-    # FP32 MFMA peak), so that pipe's dense peak is the ceiling quoted.
-    split = os.environ.get("QX_MIXED_GEMM", "") != "f32"
-    kb_peak = BF16_PEAK_TFS if split else FP32_PEAK_TFS
+    fft_bands_per_launch = int(round(nband * args.steps / max(fft_n, 1)))
+    lower, per_band = fft_stage_lower_bytes(gs, ncol, fft_bands_per_launch)
+    kb_peak = BF16_PEAK_TFS
     kb_roof = {"bound": "mfma", "achieved": round(kb_launch_flops / (kb_avg * 1e-3) / 1e12, 3),
                "peak": kb_peak, "unit": "TFLOP/s", "traffic": None,
-               "kernel": ("hipBLASLt FP32 GEMM, FAST_TF32 split products on bf16 MFMA"
-                          if split else "hipBLASLt FP32 GEMM on FP32 MFMA")
-               + " (KB projection / back-projection of a band batch)",
+               "kernel": ctx.gemm_kernel_name(pw.MIXED)
+               + " (KB projection / back-projection of a band batch; achieved counts the "
+                 "FP32-equivalent flops, peak is the dense bf16 MFMA peak the split products run on)",
                "launch_ms": round(kb_avg, 4), "launches": kb_n,
                "algorithmic_per_launch": kb_launch_flops}
     kb_roof["frac"] = round(kb_roof["achieved"] / kb_peak, 4)
-    fft_roof = {"bound": "hbm", "achieved": round(fft_launch_bytes / (fft_avg * 1e-3) / 1e9, 1),
+    fft_roof = {"bound": "hbm", "achieved": round(lower / (fft_avg * 1e-3) / 1e9, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
                 "kernel": "k_zinv_f + k_plane_f + k_zfwd_f (FP32 pencil FFT stage of a band batch)",
                 "launch_ms": round(fft_avg, 4), "launches": fft_n,
-                "algorithmic_per_launch": fft_launch_bytes}
+                "bands_per_launch": fft_bands_per_launch,
+                "algorithmic_per_launch": lower, "algorithmic_per_band": per_band,
+                "model": "lower bound of this design: sphere psi in + H psi out (FP64), FP32 psi "
+                         "copy out + FP32 KB term in, z-pencil T1/T2 written and read once "
+                         "(FP32 complex), V(r) once per launch"}
     fft_roof["frac"] = round(fft_roof["achieved"] / HBM_PEAK_GBS, 4)
-    if natom == 216 and round(fft_bands_per_launch) == 540:
-        fft_roof["traffic"] = measured_traffic("fft_stage_fp32_si216_540")
-    fft_roof["note"] = ("algorithmic bytes model two full-grid HBM passes per 3-D transform; "
-                        "the sphere-pruned pencil kernels keep xy planes in LDS and move far "
-                        "less, so achieved/peak can exceed 1; traffic_gbs is the measured "
-                        "HBM rate")
-    if fft_roof["traffic"]:
-        fft_roof["traffic_gbs"] = round(fft_roof["traffic"] / (fft_avg * 1e-3) / 1e9, 1)
-        fft_roof["traffic_frac"] = round(fft_roof["traffic_gbs"] / HBM_PEAK_GBS, 4)
-    if fft_ms >= kb_ms:
-        roof, other = fft_roof, kb_roof
-    else:
-        roof, other = kb_roof, fft_roof
-    roof = {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")} | {
-        k: roof[k] for k in roof if k not in ("bound", "achieved", "peak", "unit", "frac", "traffic")}
+    if not args.no_pmc:
+        mean, err = pmc_measure(args.ncell, nband)
+        if mean:
+            cr = fft_counter_roof(mean, fft_avg, lower, 8.0 * nband * gs.nplwex)
+            mf = {}
+            for key, tag in (("bf3_tn", "kb_bf3_projection"), ("bf3_nn", "kb_bf3_back_projection"),
+                             ("f64_tn", "kb_fp64_projection"), ("f64_nn", "kb_fp64_back_projection")):
+                u = mean.get(f"{key}:MfmaUtil")
+                if u is None:
+                    continue
+                fl = mean.get(f"{key}:MfmaFlopsBF16", 0.0) + mean.get(f"{key}:MfmaFlopsF64", 0.0)
+                mf[tag] = {"mfma_util_pct": round(u, 2), "mfma_flops_per_dispatch": fl,
+                           "over_algorithmic": round(fl / kb_launch_flops, 3)}
+            if mf:
+                kb_roof["counters"] = mf
+            if cr:
+                fft_roof["traffic"] = cr.pop("traffic", None)
+                fft_roof["counters"] = cr
+                kp = cr.get("k_plane_f")
+                if kp:
+                    # VALU issue roofline of the plane kernel: one wave64 VALU
+                    # instruction per cycle per CU (four 16-lane SIMDs, 4 cycles
+                    # each); its share of the stage time from the launch timer is
+                    # not separated, so the stage time bounds it from above
+                    issue = 256 * 2.4e9
+                    kp["valu_issue_frac_of_stage"] = round(kp["valu_instr"] / (fft_avg * 1e-3)
+                                                           / issue, 4)
+        else:
+            fft_roof["counters_error"] = err
+        print(f"# FFT-stage counters: {fft_roof.get('counters', err)}", file=sys.stderr,
+              flush=True)
+    roof, other = (fft_roof, kb_roof) if fft_ms >= kb_ms else (kb_roof, fft_roof)
+    order = ("bound", "achieved", "peak", "unit", "frac", "traffic")
+    roof = {k: roof[k] for k in order} | {k: roof[k] for k in roof if k not in order}
 
     extra = {}
     if not args.no_scf:
-        # the SCF steps run on the KB table an MD run builds on the device at
-        # every step (stscfn_pw + setnlc_g, model s + p tables): its band-by-
-        # band CG then applies KB from the table's factors
-        device_kb_table(ctx, cell, gs)
-        occ = occupations(nband, nel)
         res = {}
         for name, prec, batched in (("fp64_batched", pw.FP64, True),
                                     ("mixed_batched", pw.MIXED, True),
                                     ("fp64_band_by_band", pw.FP64, False)):
-            # one untimed step first: GEMM shapes met for the first time are
-            # autotuned once per context, an SCF run repeats the step many times
+            # one untimed step first (first-use set-up of a shape), then the timed one
             for timed in (False, True):
-                ctx.put(0, psi_orth)
+                ctx.put(0, w.psi_orth)
                 ctx.sync()
                 t0 = time.perf_counter()
-                eig, rho, ok = ctx.scf_band_step(nband, occ, cell.volume, prec=prec,
-                                                 batched=batched, want_rho=True)
+                eig, _, ok = ctx.scf_band_step(nband, w.occ, gs.volume, prec=prec,
+                                               batched=batched, want_rho=False)
                 dt = time.perf_counter() - t0
             res[name] = (dt, eig, ok, ctx.scf_phase_ms())
             print(f"# SCF band step {name}: {dt * 1e3:.1f} ms ok={ok} phases {res[name][3]}",
                   file=sys.stderr, flush=True)
-        e64 = np.sum(occ * np.sort(res["fp64_batched"][1])) / 2.0
-        emx = np.sum(occ * np.sort(res["mixed_batched"][1])) / 2.0
-        extra = {
-            "scf_step_ms": {k: round(v[0] * 1e3, 2) for k, v in res.items()},
-            "scf_phase_ms": {k: v[3] for k, v in res.items()},
-            "mixed_vs_fp64_band_energy_err_ha_per_atom": float(abs(emx - e64) / natom),
-            "mixed_vs_fp64_max_eig_err_ha": float(np.abs(np.sort(res["mixed_batched"][1]) -
-                                                         np.sort(res["fp64_batched"][1])).max() / 2),
-        }
-
-    # per-MD-step KB inputs: structure factors + eqlmr_ built on the device
-    # (stscfn_pw, setnlc_g) against uploading a host-built table (qx_set_kb)
-    kb_setup = md_kb_setup(ctx, cell, gs, eq, lmx, vcl)
-    print(f"# KB set-up per MD step: device {kb_setup['device_ms']:.2f} ms, host table upload "
-          f"{kb_setup['host_upload_ms']:.2f} ms", file=sys.stderr, flush=True)
-    extra["md_kb_setup_ms"] = kb_setup
-    # KB Hellmann-Feynman forces of all bands (kbnlpp_frc2) on the device
-    occ_f = occupations(nband, nel)
-    ctx.put(0, psi_orth)
-    ctx.kb_forces(0, nband, occ_f)
-    ctx.sync()
-    t0 = time.perf_counter()
-    ctx.kb_forces(0, nband, occ_f)
-    extra["kb_forces_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
-    print(f"# KB forces ({natom} atoms, {nband} bands): {extra['kb_forces_ms']:.2f} ms",
-          file=sys.stderr, flush=True)
-
-    # SCF glue between band updates (density -> mixing -> Hartree + XC -> V)
-    extra["scf_glue_ms"] = scf_glue(ctx, cell, gs, v, nband, occ_f)
-    print(f"# SCF glue: {extra['scf_glue_ms']}", file=sys.stderr, flush=True)
-    # local-pseudopotential forces on the mixed density the glue left
-    extra["flocal_ms"] = flocal(ctx, cell, gs)
-    print(f"# flocal_g ({natom} atoms, {gs.nplw5} G): {extra['flocal_ms']:.3f} ms",
-          file=sys.stderr, flush=True)
-    # Ewald reciprocal-space forces on the device axis tables flocal left
-    extra["ewald_frcrec_ms"], numk2 = ewald(ctx, cell, gs)
-    print(f"# frcrec ({natom} ions, {numk2} Ewald G): {extra['ewald_frcrec_ms']:.3f} ms",
-          file=sys.stderr, flush=True)
-
-    # wave-function extrapolation of an MD step (roteig + ASPC + Gram-Schmidt)
-    extra["sbalinaspc_ms"] = extrapolation(ctx, psi_orth, nband)
-    print(f"# sbalinaspc ({nband} bands, 3 previous steps): {extra['sbalinaspc_ms']}",
-          file=sys.stderr, flush=True)
-
-    # converged MD steps (device KB refresh, extrapolation, SCF to 1e-6 Ha/atom,
-    # forces, Verlet); it re-seeds the bands, mixer and potential it uses
-    extra["md_step"] = md_run(ctx, cell, gs, nband, occupations(nband, nel), psi_orth)
-    print(f"# converged MD step: {extra['md_step']}", file=sys.stderr, flush=True)
+        extra["scf_step_ms"] = {k: round(v[0] * 1e3, 2) for k, v in res.items()}
+        extra["scf_phase_ms"] = {k: v[3] for k, v in res.items()}
+        # converged SCF from identical starts (1e-6 Ha/atom on the energy change)
+        conv = {}
+        for name, prec, batched, refine in (
+                ("fp64_batched", pw.FP64, True, None),
+                ("mixed_batched_fp64_refined", pw.MIXED, True, 1e-6),
+                ("mixed_batched", pw.MIXED, True, None),
+                ("fp64_band_by_band", pw.FP64, False, None)):
+            conv[name] = scf_converge(ctx, w, prec, batched, refine)
+            print(f"# SCF to 1e-6 Ha/atom {name}: {conv[name]['ms']:.0f} ms, "
+                  f"{conv[name]['iterations']} iterations "
+                  f"({conv[name]['fp64_refinement_steps']} FP64 refinement)",
+                  file=sys.stderr, flush=True)
+        ref = conv["fp64_batched"]
+        extra["scf_converged"] = {
+            k: {kk: v[kk] for kk in ("ms", "iterations", "converged", "fp64_refinement_steps")}
+            | {"energy_ha_per_atom": v["energy_ry"] / 2 / natom}
+            for k, v in conv.items()}
+        extra["scf_converged_errors_vs_fp64_batched"] = {
+            k: scf_errors(conv[k], ref, natom) for k in conv if k != "fp64_batched"}
+        extra["scf_converged_speedup_vs_fp64_batched"] = {
+            k: round(ref["ms"] / conv[k]["ms"], 3) for k in conv if k != "fp64_batched"}
+        extra["md_parts"] = md_parts(ctx, w, sc)
+        print(f"# MD-step pieces: {extra['md_parts']}", file=sys.stderr, flush=True)
+        extra["sbalinaspc_ms"] = extrapolation(ctx, w.psi_orth, nband)
+        print(f"# sbalinaspc: {extra['sbalinaspc_ms']}", file=sys.stderr, flush=True)
+        extra["md_step"] = md_run(ctx, w)
+        print(f"# converged MD step: {extra['md_step']}", file=sys.stderr, flush=True)
+    if not args.no_kpoints:
+        extra["kpoints"] = kpoint_bench(args.steps)
+        print(f"# k-points: {extra['kpoints']}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         d = cpu_baseline(args.ncell, threads)
+        model = cpu_model()
         if d:
             cpu = {"value": round(d["bands"] / d["seconds"], 3), "unit": "band-applies/s",
                    "cores": threads, "kind": "port",
-                   "sample": f"oracle FP64 H|psi> (C, OpenMP) on {d['bands']} bands of the same "
-                             f"Si{natom} cell, {d['seconds']:.1f} s"}
-        # the same restatement on one core (the reference itself is serial)
+                   "sample": f"oracle FP64 H|psi> (C, OpenMP, textbook FFT) on {d['bands']} "
+                             f"bands of the same Si{natom} grid and sphere, {d['seconds']:.1f} s",
+                   "cpu_model": model, "nproc": os.cpu_count()}
         d1 = cpu_baseline(args.ncell, 1, budget_s=6.0)
         if d1:
             extra["cpu_baseline_1core"] = {
                 "value": round(d1["bands"] / d1["seconds"], 3), "unit": "band-applies/s",
-                "cores": 1, "kind": "port",
+                "cores": 1, "kind": "port", "cpu_model": model,
                 "sample": f"oracle FP64 H|psi> (C) on {d1['bands']} bands, {d1['seconds']:.1f} s"}
 
     value = nband / (ms_mixed * 1e-3)
@@ -600,8 +734,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 FFT + fp32 KB (hipBLASLt FAST_TF32 split products) (mixed); fp64 dense",
-        "data": "synthetic",
+        "dtype": "fp32 FFT + bf16x3 split-product KB (mixed); fp64 dense",
+        "data": "synthetic (seeded Si cell and bands; the reference's Si LDA pseudopotential)",
         "config": {"workload": f"Si{natom} Gamma H|psi>, {nband} bands, grid {gs.nd[0]}^3, "
                                f"nplw {gs.nplw}, {ctx.npj} KB projectors",
                    "natom": natom, "nband": nband, "grid": list(gs.nd), "nplw": gs.nplw,
@@ -609,7 +743,8 @@ def main():
         "fp64_band_applies_per_s": round(nband / (ms_fp64 * 1e-3), 1),
         "fp64_ms_per_step": round(ms_fp64, 4),
         "stage_ms": {"local_fp32": round(ms_loc32, 4), "local_fp64": round(ms_loc64, 4),
-                     "kb_fp64": round(ms_kb, 4), "kb_gemm_fp32_launches": round(kb_ms / args.steps, 4)},
+                     "kb_fp64": round(ms_kb, 4),
+                     "kb_gemm_mixed_launches": round(kb_ms / args.steps, 4)},
         "roofline": roof,
         "roofline_other": other,
         "cpu_baseline": cpu,
