@@ -360,11 +360,12 @@ PMC_PASSES = {
     # MFMA work and busy share of our GEMM kernels: rocprofv3's derived
     # MfmaUtil = sum SQ_VALU_MFMA_BUSY_CYCLES / (max GRBM_GUI_ACTIVE x SIMDs) x 100
     # and MfmaFlops* = SQ_INSTS_VALU_MFMA_MOPS_* x 512
-    "mfma": (["MfmaUtil", "MfmaFlopsBF16", "MfmaFlopsF64"], "k_mg_bf3|k_mg_f64"),
+    "mfma": (["MfmaUtil", "MfmaFlopsBF16", "MfmaFlopsF64"], "k_mg_bf3|k_mg_ps3|k_mg_f64"),
 }
 PMC_KERNELS = ("k_zinv_f<float", "k_plane_f<float", "k_zfwd_f<float")
-MG_KERNELS = {"k_mg_bf3<float, float, true>": "bf3_tn", "k_mg_bf3<float, float, false>": "bf3_nn",
-              "k_mg_f64<true>": "f64_tn", "k_mg_f64<false>": "f64_nn"}
+MG_KERNELS = {"k_mg_ps3<true": "bf3_tn", "k_mg_ps3<false": "bf3_nn",
+              "k_mg_bf3<float, float, true>": "bf3_tn", "k_mg_bf3<float, float, false>": "bf3_nn",
+              "k_mg_f64<true": "f64_tn", "k_mg_f64<false": "f64_nn"}
 
 
 def pmc_worker(ncell, nband, steps):

@@ -313,6 +313,25 @@ __device__ inline void lds_fft(cplx<T>* buf, int nlines, int lstride, int estrid
   }
 }
 
+// Quotient by a divisor known only at run time, without the integer divide
+// sequence: n / d = (n * m) >> 21 with m = ceil(2^21 / d), exact for
+// 0 <= n < 8192 and 1 <= d <= 256 (checked over that whole range).
+__host__ __device__ inline unsigned div_magic(unsigned d) { return ((1u << 21) + d - 1) / d; }
+__device__ inline int fdiv(int n, unsigned m) {
+  return (int)(((unsigned long long)(unsigned)n * m) >> 21);
+}
+
+// Split of two FP32 values into bf16 parts for the split bf16x3 products:
+// hi = bf16(x) (round to nearest even), lo = bf16(x - hi); each returned as a
+// packed pair (element 0 in the low half).
+typedef __bf16 qx_bf2 __attribute__((ext_vector_type(2)));
+__device__ inline void split_bf16x2(float a, float b, unsigned& hi, unsigned& lo) {
+  const qx_bf2 h = {(__bf16)a, (__bf16)b};
+  const qx_bf2 l = {(__bf16)(a - (float)h[0]), (__bf16)(b - (float)h[1])};
+  hi = __builtin_bit_cast(unsigned, h);
+  lo = __builtin_bit_cast(unsigned, l);
+}
+
 // load the twiddle table exp(2*pi*i*k/n) from global into LDS
 template <typename T>
 __device__ inline void load_tw(cplx<T>* dst, const double* src, int n) {

@@ -769,8 +769,51 @@ __global__ void k_proj_fix(const float* __restrict__ P32, const double* __restri
   Pd32[i] = (float)((2.0 * (double)P32[i] - c) * D[m]);
 }
 
-int kb_mixed_f32(Ctx* c, const double* psi, const float* psi32, int nb, float* kb32) {
+// the same scaled projection written as its bf16 hi | lo split for the
+// split bf16x3 back-projection: rows of ldp (zero pads kept), lo at plo; one
+// thread per pair of projectors (npj even)
+__global__ void k_proj_fix_split(const float* __restrict__ P32, const double* __restrict__ B,
+                                 int ldb, const double* __restrict__ psi, int ldp_psi,
+                                 const double* __restrict__ D, int npj, int nb,
+                                 unsigned short* __restrict__ Pds, int ldp, long plo) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int hp = npj / 2;
+  if (i >= (long)hp * nb) return;
+  const int m = 2 * (int)(i % hp), n = (int)(i / hp);
+  float v[2];
+  for (int u = 0; u < 2; ++u) {
+    const double c = B[(size_t)(m + u) * ldb] * psi[(size_t)n * ldp_psi] +
+                     2.0 * B[(size_t)(m + u) * ldb + 1] * psi[(size_t)n * ldp_psi + 1];
+    v[u] = (float)((2.0 * (double)P32[(size_t)n * npj + m + u] - c) * D[m + u]);
+  }
+  unsigned h, l;
+  split_bf16x2(v[0], v[1], h, l);
+  *reinterpret_cast<unsigned*>(Pds + (size_t)n * ldp + m) = h;
+  *reinterpret_cast<unsigned*>(Pds + plo + (size_t)n * ldp + m) = l;
+}
+
+int kb_mixed_f32(Ctx* c, const double* psi, const float* psi32, const unsigned short* psis,
+                 int ldk, long slo, int nb, float* kb32) {
   const int n = c->nplwex, npj = c->npj;
+  if (psis) {  // split bf16x3 on operands split beforehand
+    int ld = 0, rows = 0;
+    long tlo = 0;
+    const unsigned short* Ts = mg_kb_split(c, &ld, &rows, &tlo);
+    float* P32 = mg_f32_scratch(c, 2, (size_t)npj * nb);
+    const long plo = (long)c->nbmax * rows;
+    unsigned short* Pds = mg_u16_scratch(c, 1, 2 * (size_t)plo);
+    if (!Ts || !P32 || !Pds || ld != ldk || (npj & 1)) return -3;
+    struct TimeGemms {
+      Ctx* c;
+      explicit TimeGemms(Ctx* cc) : c(cc) { c->lt_gemm_id = 0; }
+      ~TimeGemms() { c->lt_gemm_id = -1; }
+    } tg(c);
+    int rc = mg_ps3_tn(c, npj, nb, ldk, Ts, ld, tlo, psis, ldk, slo, P32, npj);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_proj_fix_split, dim3(cdiv((long)(npj / 2) * nb, 256)), dim3(256), 0,
+                       c->stream, P32, c->d_B, n, psi, n, c->d_D, npj, nb, Pds, rows, plo);
+    return mg_ps3_nn(c, n, nb, rows, Ts, ld, tlo, Pds, rows, plo, kb32, n);
+  }
   const float* B32 = mg_kb_b32(c);
   float* P32 = mg_f32_scratch(c, 2, (size_t)npj * nb);
   float* Pd32 = mg_f32_scratch(c, 3, (size_t)npj * nb);

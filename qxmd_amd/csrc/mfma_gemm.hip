@@ -61,6 +61,8 @@ struct Sk {
   int G;            // workgroups
   double alpha, beta;
   int ldc, corr;
+  long alo, blo;  // pre-split operands: element offset of the lo half after the hi half
+  int tm, tn;     // output tile (TE x TE, or 96 along an edge for the FP64 kernels)
 };
 
 // logical workgroup: XCD x = blockIdx % 8 takes the x-th contiguous eighth
@@ -80,99 +82,99 @@ __device__ inline double corr_term(int corr, const T* A, int lda, const T* B, in
 }
 
 // ---------------------------------------------------------------- FP64 ----
-template <bool AK>
+// Output tile 32 BI x 32 BJ (BI, BJ = 4 or 3: 128 or 96 along each edge, so
+// that 540 bands or 864 projectors waste little of the last tile), four
+// wavefronts of 16 BI x 16 BJ in a 2 x 2 layout.  Operands move in 16-byte
+// chunks: a k-contiguous row of the K stage is 8 chunks (8 threads read 128
+// consecutive bytes), an m-contiguous k row is TM / 2 chunks.
+template <bool AK, int BI, int BJ>
 __global__ void __launch_bounds__(256, 2)
     k_mg_f64(Sk sk, const double* __restrict__ A, int lda, const double* __restrict__ B, int ldb,
              double* __restrict__ C, double* __restrict__ Wp) {
-  __shared__ double As[2][KT64][LP64];
-  __shared__ double Bs[2][KT64][LP64];
+  constexpr int TM = 32 * BI, TN = 32 * BJ;
+  constexpr int LPM = TM + 4, LPN = TN + 4;
+  constexpr int CA = TM * KT64 / 2 / 256, CB = TN * KT64 / 2 / 256;  // chunks per thread
+  static_assert(TM * KT64 / 2 % 256 == 0 && TN * KT64 / 2 % 256 == 0, "chunk split");
+  __shared__ double As[2][KT64][LPM];
+  __shared__ double Bs[2][KT64][LPN];
   const int M = sk.M, N = sk.N, K = sk.K;
   const int lb = logical_wg(sk.G);
-  const long I = (long)sk.tiles_m * cdivu(N, TE) * sk.nk;
+  const long I = (long)sk.tiles_m * cdivu(N, TN) * sk.nk;
   const long g0 = lb * sk.L, g1 = min(I, g0 + sk.L);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
-  // k-contiguous operand: thread -> row tid >> 1, k run (tid & 1) * 8
-  const int kr_r = tid >> 1, kr_k = (tid & 1) * 8;
-  // m-contiguous operand: thread -> k row tid >> 4, m run (tid & 15) * 8
-  const int mr_k = tid >> 4, mr_m = (tid & 15) * 8;
+  const int wm = (wave & 1) * (TM / 2), wn = (wave >> 1) * (TN / 2);
   for (long g = g0; g < g1;) {
     const int t = (int)(g / sk.nk);
     const int s0 = (int)(g - (long)t * sk.nk);
     const int s1 = (int)min((long)sk.nk, g1 - (long)t * sk.nk);
-    const int m0 = (t % sk.tiles_m) * TE, n0 = (t / sk.tiles_m) * TE;
+    const int m0 = (t % sk.tiles_m) * TM, n0 = (t / sk.tiles_m) * TN;
     const int kb = s0 * KT64, ke = min(K, s1 * KT64);
-    d4 acc[4][4];
+    d4 acc[BI][BJ];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < BI; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-    double ra[8], rb[8];
-    const double* Ar = A + (AK ? (size_t)min(m0 + kr_r, M - 1) * lda : 0);
-    const double* Br = B + (size_t)min(n0 + kr_r, N - 1) * ldb;
+      for (int j = 0; j < BJ; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    double2 ra[CA], rb[CB];
+    // chunk u of this thread: k-contiguous -> row c / 8, k 2 (c % 8);
+    // m-contiguous -> k row c / (TM / 2), m 2 (c % (TM / 2))
     auto fetch = [&](int k0) {
-      if (k0 + KT64 <= ke) {
+      const bool full = k0 + KT64 <= ke;
+#pragma unroll
+      for (int u = 0; u < CA; ++u) {
+        const int c = tid + 256 * u;
         if (AK) {
-          const double2* p = reinterpret_cast<const double2*>(Ar + k0 + kr_k);
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const double2 v = p[u];
-            ra[2 * u] = v.x;
-            ra[2 * u + 1] = v.y;
-          }
-        } else {
-          const int gm = m0 + mr_m;
-          const double* p = A + (size_t)(k0 + mr_k) * lda + gm;
-          if (gm + 8 <= M) {
-            const double2* q = reinterpret_cast<const double2*>(p);
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const double2 v = q[u];
-              ra[2 * u] = v.x;
-              ra[2 * u + 1] = v.y;
-            }
+          const int row = c >> 3, k = k0 + 2 * (c & 7);
+          const double* p = A + (size_t)min(m0 + row, M - 1) * lda + k;
+          if (full) {
+            ra[u] = *reinterpret_cast<const double2*>(p);
           } else {
-#pragma unroll
-            for (int u = 0; u < 8; ++u) ra[u] = (gm + u < M) ? p[u] : 0.0;
-          }
-        }
-        const double2* q = reinterpret_cast<const double2*>(Br + k0 + kr_k);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const double2 v = q[u];
-          rb[2 * u] = v.x;
-          rb[2 * u + 1] = v.y;
-        }
-      } else {  // the last, partial K stage
-        if (AK) {
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const int k = k0 + kr_k + u;
-            ra[u] = k < ke ? Ar[k] : 0.0;
+            ra[u].x = k < ke ? p[0] : 0.0;
+            ra[u].y = k + 1 < ke ? p[1] : 0.0;
           }
         } else {
-          const int gm = m0 + mr_m, k = k0 + mr_k;
-          const double* p = A + (size_t)min(k, ke - 1) * lda;
-#pragma unroll
-          for (int u = 0; u < 8; ++u) ra[u] = (k < ke && gm + u < M) ? p[gm + u] : 0.0;
+          const int kr = c / (TM / 2), m = m0 + 2 * (c % (TM / 2)), k = k0 + kr;
+          const double* p = A + (size_t)min(k, K - 1) * lda + m;
+          if (k < ke && m + 2 <= M) {
+            ra[u] = *reinterpret_cast<const double2*>(p);
+          } else {
+            ra[u].x = (k < ke && m < M) ? p[0] : 0.0;
+            ra[u].y = (k < ke && m + 1 < M) ? p[1] : 0.0;
+          }
         }
+      }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int k = k0 + kr_k + u;
-          rb[u] = k < ke ? Br[k] : 0.0;
+      for (int u = 0; u < CB; ++u) {
+        const int c = tid + 256 * u;
+        const int row = c >> 3, k = k0 + 2 * (c & 7);
+        const double* p = B + (size_t)min(n0 + row, N - 1) * ldb + k;
+        if (full) {
+          rb[u] = *reinterpret_cast<const double2*>(p);
+        } else {
+          rb[u].x = k < ke ? p[0] : 0.0;
+          rb[u].y = k + 1 < ke ? p[1] : 0.0;
         }
       }
     };
     auto stash = [&](int buf) {
-      if (AK) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) As[buf][kr_k + u][kr_r] = ra[u];
-      } else {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) As[buf][mr_k][mr_m + u] = ra[u];
+      for (int u = 0; u < CA; ++u) {
+        const int c = tid + 256 * u;
+        if (AK) {
+          const int row = c >> 3, k = 2 * (c & 7);
+          As[buf][k][row] = ra[u].x;
+          As[buf][k + 1][row] = ra[u].y;
+        } else {
+          const int kr = c / (TM / 2), m = 2 * (c % (TM / 2));
+          *reinterpret_cast<double2*>(&As[buf][kr][m]) = ra[u];
+        }
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) Bs[buf][kr_k + u][kr_r] = rb[u];
+      for (int u = 0; u < CB; ++u) {
+        const int c = tid + 256 * u;
+        const int row = c >> 3, k = 2 * (c & 7);
+        Bs[buf][k][row] = rb[u].x;
+        Bs[buf][k + 1][row] = rb[u].y;
+      }
     };
     int buf = 0;
     fetch(kb);
@@ -184,16 +186,15 @@ __global__ void __launch_bounds__(256, 2)
 #pragma unroll
       for (int k4 = 0; k4 < KT64; k4 += 4) {
         const int kr = k4 + (lane >> 4), c = lane & 15;
-        double a[4], b[4];
+        double a[BI], b[BJ];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          a[i] = As[buf][kr][wm + 16 * i + c];
-          b[i] = Bs[buf][kr][wn + 16 * i + c];
-        }
+        for (int i = 0; i < BI; ++i) a[i] = As[buf][kr][wm + 16 * i + c];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < BJ; ++j) b[j] = Bs[buf][kr][wn + 16 * j + c];
 #pragma unroll
-          for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < BI; ++i)
+#pragma unroll
+          for (int j = 0; j < BJ; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
       }
       if (more) stash(buf ^ 1);
@@ -204,15 +205,15 @@ __global__ void __launch_bounds__(256, 2)
     const bool whole = s0 == 0 && s1 == sk.nk;
     double* P = Wp + ((size_t)lb * 2 + (g == g0 ? 0 : 1)) * TE * TE;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < BI; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < BJ; ++j) {
         const int nl = wn + 16 * j + (lane & 15), n = n0 + nl;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int ml = wm + 16 * i + (lane >> 4) + 4 * v, m = m0 + ml;
           if (!whole) {
-            P[nl * TE + ml] = acc[i][j][v];
+            P[nl * TM + ml] = acc[i][j][v];
           } else if (m < M && n < N) {
             double* p = C + (size_t)n * sk.ldc + m;
             const double x = acc[i][j][v] - corr_term(sk.corr, A, lda, B, ldb, m, n);
@@ -235,15 +236,15 @@ __global__ void k_mg_fixup(Sk sk, const TW* __restrict__ Wp, const T* __restrict
   const int b_lo = (int)(lo / sk.L), b_hi = (int)(hi / sk.L);
   if (b_lo == b_hi) return;
   const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (e >= TE * TE) return;
-  const int ml = e % TE, nl = e / TE;
-  const int m = (t % sk.tiles_m) * TE + ml, n = (t / sk.tiles_m) * TE + nl;
+  if (e >= sk.tm * sk.tn) return;
+  const int ml = e % sk.tm, nl = e / sk.tm;
+  const int m = (t % sk.tiles_m) * sk.tm + ml, n = (t / sk.tiles_m) * sk.tn + nl;
   if (m >= sk.M || n >= sk.N) return;
   double s = 0.0;
   for (int b = b_lo; b <= b_hi; ++b) {
     const int first = (int)((long)b * sk.L / sk.nk);  // the workgroup's first tile
     const size_t slot = (size_t)b * 2 + (first == t ? 0 : 1);
-    s += (double)Wp[slot * TE * TE + (size_t)nl * TE + ml];
+    s += (double)Wp[slot * TE * TE + (size_t)nl * sk.tm + ml];
   }
   s -= corr_term(sk.corr, A, lda, B, ldb, m, n);
   TC* p = C + (size_t)n * sk.ldc + m;
@@ -251,15 +252,9 @@ __global__ void k_mg_fixup(Sk sk, const TW* __restrict__ Wp, const T* __restrict
 }
 
 // ------------------------------------------------------------ bf16x3 ------
-typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
-
-// two values split at once: hi = bf16(x) (round to nearest even, one packed
-// conversion for the pair), lo = bf16(x - hi); returned as packed bf16 pairs
+// two values split at once (split_bf16x2: hi = bf16(x), lo = bf16(x - hi))
 __device__ inline void split2(float a, float b, unsigned& hi, unsigned& lo) {
-  const bf2 h = {(__bf16)a, (__bf16)b};
-  const bf2 l = {(__bf16)(a - (float)h[0]), (__bf16)(b - (float)h[1])};
-  hi = __builtin_bit_cast(unsigned, h);
-  lo = __builtin_bit_cast(unsigned, l);
+  split_bf16x2(a, b, hi, lo);
 }
 
 // 16 consecutive elements of a k-contiguous row (k0 + 16 <= end), as floats
@@ -454,18 +449,208 @@ __global__ void __launch_bounds__(256, 2)
   }
 }
 
+// Split bf16x3 on operands split beforehand: each operand is a bf16 hi array
+// followed (sk.alo / sk.blo elements later) by its bf16 lo array, with K
+// padded by zeros to a multiple of KP, so the K loop has no guards and no
+// conversion work; the m-contiguous A of NN is read up to TE elements past
+// its last row (the producers allocate that margin).  TN: A rows k-contiguous.
+// The K stage is 64 deep, so that a stage's 96 MFMAs per wavefront cover the
+// latency of the next stage's loads.
+typedef unsigned short u16;
+constexpr int KP = 64, LPP = KP + 8;  // pre-split K stage, LDS row pitch (bf16)
+
+// 4 k rows of 4 m values (each row a pair of packed bf16 pairs) -> the run
+// of 4 k values of m = u (16-bit transpose)
+__device__ inline uint2 t4(uint2 r0, uint2 r1, uint2 r2, uint2 r3, int u) {
+  const unsigned w0 = u < 2 ? r0.x : r0.y, w1 = u < 2 ? r1.x : r1.y;
+  const unsigned w2 = u < 2 ? r2.x : r2.y, w3 = u < 2 ? r3.x : r3.y;
+  const int sh = (u & 1) * 16;
+  return make_uint2(((w0 >> sh) & 0xffffu) | (((w1 >> sh) & 0xffffu) << 16),
+                    ((w2 >> sh) & 0xffffu) | (((w3 >> sh) & 0xffffu) << 16));
+}
+
+template <bool AK, typename TC>
+__global__ void __launch_bounds__(256, 2)
+    k_mg_ps3(Sk sk, const u16* __restrict__ A, int lda, const u16* __restrict__ B, int ldb,
+             TC* __restrict__ C, float* __restrict__ Wp) {
+  __shared__ u16 Ah[TE][LPP], Al[TE][LPP], Bh[TE][LPP], Bl[TE][LPP];
+  const int M = sk.M, N = sk.N;
+  const int lb = logical_wg(sk.G);
+  const long I = (long)sk.tiles_m * cdivu(N, TE) * sk.nk;
+  const long g0 = lb * sk.L, g1 = min(I, g0 + sk.L);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  // k-contiguous rows: thread -> row tid >> 1, k run (tid & 1) * 32 .. + 31
+  const int lr = tid >> 1, lk = (tid & 1) * 32;
+  // m-contiguous A (NN): thread -> m run (tid & 31) * 4, k rows (tid >> 5) * 8 .. + 7
+  const int am = (tid & 31) * 4, ak = (tid >> 5) * 8;
+  for (long g = g0; g < g1;) {
+    const int t = (int)(g / sk.nk);
+    const int s0 = (int)(g - (long)t * sk.nk);
+    const int s1 = (int)min((long)sk.nk, g1 - (long)t * sk.nk);
+    const int m0 = (t % sk.tiles_m) * TE, n0 = (t / sk.tiles_m) * TE;
+    const int kb = s0 * KP, ke = s1 * KP;
+    f4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    const u16* Ar = A + (AK ? (size_t)min(m0 + lr, M - 1) * lda : (size_t)m0 + am);
+    const u16* Br = B + (size_t)min(n0 + lr, N - 1) * ldb;
+    // operand registers of the next stage: named scalars (arrays of vectors
+    // would be placed in scratch memory)
+    uint4 ah0, ah1, ah2, ah3, al0, al1, al2, al3, bh0, bh1, bh2, bh3, bl0, bl1, bl2, bl3;
+    uint2 nh0, nh1, nh2, nh3, nh4, nh5, nh6, nh7, nl0, nl1, nl2, nl3, nl4, nl5, nl6, nl7;
+    auto fetch = [&](int k0) {
+      if constexpr (AK) {
+        const uint4* p = reinterpret_cast<const uint4*>(Ar + k0 + lk);
+        const uint4* q = reinterpret_cast<const uint4*>(Ar + sk.alo + k0 + lk);
+        ah0 = p[0]; ah1 = p[1]; ah2 = p[2]; ah3 = p[3];
+        al0 = q[0]; al1 = q[1]; al2 = q[2]; al3 = q[3];
+      } else {
+        const u16* p = Ar + (size_t)(k0 + ak) * lda;
+        const u16* q = p + sk.alo;
+        const size_t d = (size_t)lda;
+        nh0 = *reinterpret_cast<const uint2*>(p + 0 * d);
+        nh1 = *reinterpret_cast<const uint2*>(p + 1 * d);
+        nh2 = *reinterpret_cast<const uint2*>(p + 2 * d);
+        nh3 = *reinterpret_cast<const uint2*>(p + 3 * d);
+        nh4 = *reinterpret_cast<const uint2*>(p + 4 * d);
+        nh5 = *reinterpret_cast<const uint2*>(p + 5 * d);
+        nh6 = *reinterpret_cast<const uint2*>(p + 6 * d);
+        nh7 = *reinterpret_cast<const uint2*>(p + 7 * d);
+        nl0 = *reinterpret_cast<const uint2*>(q + 0 * d);
+        nl1 = *reinterpret_cast<const uint2*>(q + 1 * d);
+        nl2 = *reinterpret_cast<const uint2*>(q + 2 * d);
+        nl3 = *reinterpret_cast<const uint2*>(q + 3 * d);
+        nl4 = *reinterpret_cast<const uint2*>(q + 4 * d);
+        nl5 = *reinterpret_cast<const uint2*>(q + 5 * d);
+        nl6 = *reinterpret_cast<const uint2*>(q + 6 * d);
+        nl7 = *reinterpret_cast<const uint2*>(q + 7 * d);
+      }
+      const uint4* p = reinterpret_cast<const uint4*>(Br + k0 + lk);
+      const uint4* q = reinterpret_cast<const uint4*>(Br + sk.blo + k0 + lk);
+      bh0 = p[0]; bh1 = p[1]; bh2 = p[2]; bh3 = p[3];
+      bl0 = q[0]; bl1 = q[1]; bl2 = q[2]; bl3 = q[3];
+    };
+    auto stash = [&]() {
+      if constexpr (AK) {
+        *reinterpret_cast<uint4*>(&Ah[lr][lk + 0]) = ah0;
+        *reinterpret_cast<uint4*>(&Ah[lr][lk + 8]) = ah1;
+        *reinterpret_cast<uint4*>(&Ah[lr][lk + 16]) = ah2;
+        *reinterpret_cast<uint4*>(&Ah[lr][lk + 24]) = ah3;
+        *reinterpret_cast<uint4*>(&Al[lr][lk + 0]) = al0;
+        *reinterpret_cast<uint4*>(&Al[lr][lk + 8]) = al1;
+        *reinterpret_cast<uint4*>(&Al[lr][lk + 16]) = al2;
+        *reinterpret_cast<uint4*>(&Al[lr][lk + 24]) = al3;
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          *reinterpret_cast<uint2*>(&Ah[am + u][ak]) = t4(nh0, nh1, nh2, nh3, u);
+          *reinterpret_cast<uint2*>(&Ah[am + u][ak + 4]) = t4(nh4, nh5, nh6, nh7, u);
+          *reinterpret_cast<uint2*>(&Al[am + u][ak]) = t4(nl0, nl1, nl2, nl3, u);
+          *reinterpret_cast<uint2*>(&Al[am + u][ak + 4]) = t4(nl4, nl5, nl6, nl7, u);
+        }
+      }
+      *reinterpret_cast<uint4*>(&Bh[lr][lk + 0]) = bh0;
+      *reinterpret_cast<uint4*>(&Bh[lr][lk + 8]) = bh1;
+      *reinterpret_cast<uint4*>(&Bh[lr][lk + 16]) = bh2;
+      *reinterpret_cast<uint4*>(&Bh[lr][lk + 24]) = bh3;
+      *reinterpret_cast<uint4*>(&Bl[lr][lk + 0]) = bl0;
+      *reinterpret_cast<uint4*>(&Bl[lr][lk + 8]) = bl1;
+      *reinterpret_cast<uint4*>(&Bl[lr][lk + 16]) = bl2;
+      *reinterpret_cast<uint4*>(&Bl[lr][lk + 24]) = bl3;
+    };
+    fetch(kb);
+    for (int k0 = kb; k0 < ke; k0 += KP) {
+      stash();
+      __syncthreads();
+      if (k0 + KP < ke) fetch(k0 + KP);
+      // v_mfma_f32_16x16x32_bf16 operands: lane l holds row l % 16, k 8 (l / 16) .. + 7
+      const int r = lane & 15;
+#pragma unroll
+      for (int ks = 0; ks < KP; ks += 32) {
+        const int kk = ks + 8 * (lane >> 4);
+        bf8 xh[4], xl[4], yh[4], yl[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          xh[i] = *reinterpret_cast<const bf8*>(&Ah[wm + 16 * i + r][kk]);
+          xl[i] = *reinterpret_cast<const bf8*>(&Al[wm + 16 * i + r][kk]);
+          yh[i] = *reinterpret_cast<const bf8*>(&Bh[wn + 16 * i + r][kk]);
+          yl[i] = *reinterpret_cast<const bf8*>(&Bl[wn + 16 * i + r][kk]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[i], yh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[i], yl[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[i], yh[j], acc[i][j], 0, 0, 0);
+          }
+      }
+      __syncthreads();
+    }
+    const bool whole = s0 == 0 && s1 == sk.nk;
+    float* P = Wp + ((size_t)lb * 2 + (g == g0 ? 0 : 1)) * TE * TE;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int nl2 = wn + 16 * j + (lane & 15), n = n0 + nl2;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int ml = wm + 16 * i + 4 * (lane >> 4) + v, m = m0 + ml;
+          if (!whole) {
+            P[nl2 * TE + ml] = acc[i][j][v];
+          } else if (m < M && n < N) {
+            TC* p = C + (size_t)n * sk.ldc + m;
+            *p = (TC)(sk.alpha * (double)acc[i][j][v] +
+                      (sk.beta == 0.0 ? 0.0 : sk.beta * (double)*p));
+          }
+        }
+      }
+    g = (long)t * sk.nk + s1;
+  }
+}
+
 // ------------------------------------------------------------ host side ---
 struct MgState {
   void* ws = nullptr;  // partial tiles of the stream-K workgroups
   size_t ws_bytes = 0;
   // tuned workgroup count per (kind, M, N, K, lda, ldb)
   std::map<std::tuple<int, int, int, int, int, int>, int> grid;
+  // FP64 output tile per (kind, M, N, K, lda, ldb): index into the variants
+  std::map<std::tuple<int, int, int, int, int, int>, int> tile;
   float* f32[4] = {};  // FP32 work buffers of the fused mixed band-apply
   size_t f32_len[4] = {};
   float* B32 = nullptr;  // FP32 copy of the KB table
   const double* B32_src = nullptr;
   size_t B32_len = 0;
+  // the KB table split into bf16 hi | lo (rows padded to 32, K to 32)
+  u16* Ts = nullptr;
+  const double* Ts_src = nullptr;
+  size_t Ts_len = 0;
+  u16* u16s[2] = {};  // zero-initialised split work arrays
+  size_t u16_len[2] = {};
 };
+
+__global__ void k_split_rows(const double* __restrict__ src, int rows, int cols, int lds,
+                             u16* __restrict__ dst, int ld, int rows_pad, long lo) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;  // pair index
+  const long npair = lo / 2;
+  if (i >= npair) return;
+  const long e = 2 * i;
+  const long r = e / ld;
+  const int col = (int)(e - r * ld);
+  float a = 0.f, b = 0.f;
+  if (r < rows && col < cols) a = (float)src[r * lds + col];
+  if (r < rows && col + 1 < cols) b = (float)src[r * lds + col + 1];
+  (void)rows_pad;
+  unsigned h, l;
+  split_bf16x2(a, b, h, l);
+  *reinterpret_cast<unsigned*>(dst + e) = h;
+  *reinterpret_cast<unsigned*>(dst + lo + e) = l;
+}
 
 __global__ void k_to_f32(const double* __restrict__ x, float* __restrict__ y, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -489,20 +674,24 @@ void* workspace(Ctx* c, size_t bytes) {
   return s->ws;
 }
 
-Sk make_sk(int M, int N, int K, int kt, int G, double alpha, double beta, int ldc, int corr) {
+Sk make_sk(int M, int N, int K, int kt, int G, double alpha, double beta, int ldc, int corr,
+           int tm = TE, int tn = TE) {
   Sk sk;
   sk.M = M;
   sk.N = N;
   sk.K = K;
-  sk.tiles_m = (int)cdivu(M, TE);
+  sk.tm = tm;
+  sk.tn = tn;
+  sk.tiles_m = (int)cdivu(M, tm);
   sk.nk = (int)cdivu(K, kt);
-  const long I = (long)sk.tiles_m * cdivu(N, TE) * sk.nk;
+  const long I = (long)sk.tiles_m * cdivu(N, tn) * sk.nk;
   sk.G = G;
   sk.L = (long)cdivu(I, G);
   sk.alpha = alpha;
   sk.beta = beta;
   sk.ldc = ldc;
   sk.corr = corr;
+  sk.alo = sk.blo = 0;
   return sk;
 }
 
@@ -569,7 +758,7 @@ template <typename TW, typename T, typename TC, typename K1>
 int launch_sk(Ctx* c, const Sk& sk, K1 kernel, const T* A, int lda, const T* B, int ldb, TC* C) {
   TW* W = static_cast<TW*>(state(c)->ws);
   hipLaunchKernelGGL(kernel, dim3(sk.G), dim3(256), 0, c->stream, sk, A, lda, B, ldb, C, W);
-  const int tiles = sk.tiles_m * (int)cdivu(sk.N, TE);
+  const int tiles = sk.tiles_m * (int)cdivu(sk.N, sk.tn);
   hipLaunchKernelGGL((k_mg_fixup<TW, T, TC>), dim3(TE * TE / 256, tiles), dim3(256), 0,
                      c->stream, sk, W, A, lda, B, ldb, C);
   return hipGetLastError() == hipSuccess ? 0 : -4;
@@ -578,7 +767,7 @@ int launch_sk(Ctx* c, const Sk& sk, K1 kernel, const T* A, int lda, const T* B, 
 template <typename TW, typename T, typename TC, typename K1>
 int run_product(Ctx* c, int kind, int kt, int per_cu_max, K1 kernel, int M, int N, int K,
                 double alpha, const T* A, int lda, const T* B, int ldb, double beta, TC* C,
-                int ldc, int corr) {
+                int ldc, int corr, long alo = 0, long blo = 0, int tm = TE, int tn = TE) {
   if (M <= 0 || N <= 0) return 0;
   if ((lda | ldb) & 1) return -3;  // 8 / 16-byte row loads need even leading dimensions
   MgState* s = state(c);
@@ -590,14 +779,18 @@ int run_product(Ctx* c, int kind, int kt, int per_cu_max, K1 kernel, int M, int 
     TrialOut<TC> out((size_t)M * N);
     if (!out.p) return -3;
     G = choose_grid(c, kind, M, N, K, lda, ldb, per_cu_max, sizeof(TW), [&](int g) {
-      const Sk tk = make_sk(M, N, K, kt, g, alpha, 0.0, M, corr);
+      Sk tk = make_sk(M, N, K, kt, g, alpha, 0.0, M, corr, tm, tn);
+      tk.alo = alo;
+      tk.blo = blo;
       return launch_sk<TW>(c, tk, kernel, A, lda, B, ldb, out.p);
     });
     QX_CHECK(hipStreamSynchronize(c->stream));  // trials done before the scratch goes
   }
   if (G < 0) return G;
   if (!workspace(c, sizeof(TW) * 2 * TE * TE * (size_t)G)) return -3;
-  const Sk sk = make_sk(M, N, K, kt, G, alpha, beta, ldc, corr);
+  Sk sk = make_sk(M, N, K, kt, G, alpha, beta, ldc, corr, tm, tn);
+  sk.alo = alo;
+  sk.blo = blo;
   lt_mark(c, c->lt_gemm_id, false);
   const int rc = launch_sk<TW>(c, sk, kernel, A, lda, B, ldb, C);
   lt_mark(c, c->lt_gemm_id, true);
@@ -612,14 +805,64 @@ void mg_destroy(Ctx* c) {
   if (s->ws) (void)hipFree(s->ws);
   for (float* p : s->f32)
     if (p) (void)hipFree(p);
+  for (u16* p : s->u16s)
+    if (p) (void)hipFree(p);
   if (s->B32) (void)hipFree(s->B32);
+  if (s->Ts) (void)hipFree(s->Ts);
   delete s;
   c->mg = nullptr;
 }
 
 // the KB table changed in place: its FP32 copy is made again on next use
 void mg_forget_kb(Ctx* c) {
-  if (c->mg) state(c)->B32_src = nullptr;
+  if (c->mg) {
+    state(c)->B32_src = nullptr;
+    state(c)->Ts_src = nullptr;
+  }
+}
+
+// the KB table (npj rows of nplwex) split into bf16 hi | lo: row stride
+// round_up(nplwex, 64), rows padded to round_up(npj, 64), zeros in every pad,
+// a margin of TE elements after each half (the NN edge tiles read into it);
+// lo = the lo half's offset
+const unsigned short* mg_kb_split(Ctx* c, int* ld, int* rows_pad, long* lo) {
+  if (!c->d_B) return nullptr;
+  MgState* s = state(c);
+  *ld = (int)(cdivu(c->nplwex, KP) * KP);
+  *rows_pad = (int)(cdivu(c->npj, KP) * KP);
+  *lo = (long)*rows_pad * *ld + TE;
+  const size_t len = 2 * (size_t)*lo;
+  if (s->Ts_src != c->d_B || s->Ts_len != len || !s->Ts) {
+    buffers_changed(c);
+    if (s->Ts_len != len || !s->Ts) {
+      if (s->Ts) (void)hipFree(s->Ts);
+      s->Ts = nullptr;
+      s->Ts_len = 0;
+      if (hipMalloc(&s->Ts, sizeof(u16) * len) != hipSuccess) return nullptr;
+    }
+    hipLaunchKernelGGL(k_split_rows, dim3(cdivu(*lo / 2, 256)), dim3(256), 0, c->stream, c->d_B,
+                       c->npj, c->nplwex, c->nplwex, s->Ts, *ld, *rows_pad, *lo);
+    s->Ts_src = c->d_B;
+    s->Ts_len = len;
+  }
+  return s->Ts;
+}
+
+// zero-initialised u16 work array i (0, 1) of at least n elements; contents
+// persist between calls (the producers write only inside their pads)
+unsigned short* mg_u16_scratch(Ctx* c, int i, size_t n) {
+  if (i < 0 || i >= 2) return nullptr;
+  MgState* s = state(c);
+  if (s->u16_len[i] < n) {
+    buffers_changed(c);
+    if (s->u16s[i]) (void)hipFree(s->u16s[i]);
+    s->u16s[i] = nullptr;
+    s->u16_len[i] = 0;
+    if (hipMalloc(&s->u16s[i], sizeof(u16) * n) != hipSuccess) return nullptr;
+    if (hipMemsetAsync(s->u16s[i], 0, sizeof(u16) * n, c->stream) != hipSuccess) return nullptr;
+    s->u16_len[i] = n;
+  }
+  return s->u16s[i];
 }
 
 const float* mg_kb_b32(Ctx* c) {
@@ -658,18 +901,79 @@ float* mg_f32_scratch(Ctx* c, int i, size_t n) {
 
 int mg_tuned_shapes(Ctx* c) { return c->mg ? (int)state(c)->grid.size() : 0; }
 
+namespace {
+// FP64 product on the output tile chosen for its shape: the four tiles
+// (128 | 96) x (128 | 96) are timed on the first product of a shape (into a
+// scratch output; each with its own tuned grid) and the fastest is kept;
+// QX_GEMM_NOTUNE=1 takes the tile that pads the output least
+template <bool AK>
+int f64_product(Ctx* c, int kind, int M, int N, int K, double alpha, const double* A, int lda,
+                const double* B, int ldb, double beta, double* C, int ldc, int corr) {
+  using KF = void (*)(Sk, const double*, int, const double*, int, double*, double*);
+  static const KF ks[4] = {k_mg_f64<AK, 4, 4>, k_mg_f64<AK, 4, 3>, k_mg_f64<AK, 3, 4>,
+                           k_mg_f64<AK, 3, 3>};
+  static const int tms[4] = {128, 128, 96, 96}, tns[4] = {128, 96, 128, 96};
+  if (M <= 0 || N <= 0) return 0;
+  if ((lda | ldb) & 1) return -3;
+  MgState* s = state(c);
+  const auto key = std::make_tuple(kind, M, N, K, lda, ldb);
+  auto run = [&](int v, double b, double* out, int ldo) {
+    return run_product<double>(c, kind + 10 * (v + 1), KT64, 2, ks[v], M, N, K, alpha, A, lda, B,
+                               ldb, b, out, ldo, corr, 0, 0, tms[v], tns[v]);
+  };
+  auto it = s->tile.find(key);
+  int best = 0;
+  if (it != s->tile.end()) {
+    best = it->second;
+  } else if (tuning_off()) {
+    long least = -1;
+    for (int v = 0; v < 4; ++v) {
+      const long area = (long)cdivu(M, tms[v]) * tms[v] * (long)cdivu(N, tns[v]) * tns[v];
+      if (least < 0 || area < least) {
+        least = area;
+        best = v;
+      }
+    }
+    s->tile[key] = best;
+  } else {
+    TrialOut<double> out((size_t)M * N);
+    if (!out.p) return -3;
+    hipEvent_t e0, e1;
+    QX_CHECK(hipEventCreate(&e0));
+    QX_CHECK(hipEventCreate(&e1));
+    float bms = 1e30f;
+    for (int v = 0; v < 4; ++v) {
+      if (int rc = run(v, 0.0, out.p, M)) return rc;  // tunes the grid of this tile
+      QX_CHECK(hipEventRecord(e0, c->stream));
+      for (int r = 0; r < 2; ++r)
+        if (int rc = run(v, 0.0, out.p, M)) return rc;
+      QX_CHECK(hipEventRecord(e1, c->stream));
+      QX_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      QX_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < bms) {
+        bms = ms;
+        best = v;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    s->tile[key] = best;
+  }
+  return run(best, beta, C, ldc);
+}
+}  // namespace
+
 // FP64 TN: C = alpha (A^T B - corr) + beta C
 int mg_dgemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
                 const double* B, int ldb, double beta, double* C, int ldc, int corr) {
-  return run_product<double>(c, 0, KT64, 2, k_mg_f64<true>, M, N, K, alpha, A, lda, B, ldb, beta,
-                             C, ldc, corr);
+  return f64_product<true>(c, 0, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, corr);
 }
 
 // FP64 NN: C = alpha A B + beta C
 int mg_dgemm_nn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
                 const double* B, int ldb, double beta, double* C, int ldc) {
-  return run_product<double>(c, 1, KT64, 2, k_mg_f64<false>, M, N, K, alpha, A, lda, B, ldb,
-                             beta, C, ldc, 0);
+  return f64_product<false>(c, 1, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, 0);
 }
 
 // split bf16x3 TN on T operands into TC: C = alpha (A^T B - corr) + beta C
@@ -686,6 +990,23 @@ int mg_bf3_nn(Ctx* c, int M, int N, int K, double alpha, const T* A, int lda, co
               double beta, TC* C, int ldc) {
   return run_product<float>(c, 4 + (sizeof(T) == 8 ? 1 : 0), KT16, 2, k_mg_bf3<T, TC, false>, M,
                             N, K, alpha, A, lda, B, ldb, beta, C, ldc, 0);
+}
+
+// split bf16x3 on pre-split operands (hi then lo, K a multiple of 64 with
+// zero padding, leading dimensions and lo offsets multiples of 64): TN / NN
+// into FP32 C
+int mg_ps3_tn(Ctx* c, int M, int N, int K, const unsigned short* A, int lda, long alo,
+              const unsigned short* B, int ldb, long blo, float* C, int ldc) {
+  if (K % KP || (lda | ldb) % KP || alo % KP || blo % KP) return -3;
+  return run_product<float>(c, 6, KP, 2, k_mg_ps3<true, float>, M, N, K, 1.0, A, lda, B, ldb,
+                            0.0, C, ldc, 0, alo, blo);
+}
+
+int mg_ps3_nn(Ctx* c, int M, int N, int K, const unsigned short* A, int lda, long alo,
+              const unsigned short* B, int ldb, long blo, float* C, int ldc) {
+  if (K % KP || (lda | ldb) % KP || alo % KP || blo % KP) return -3;
+  return run_product<float>(c, 7, KP, 2, k_mg_ps3<false, float>, M, N, K, 1.0, A, lda, B, ldb,
+                            0.0, C, ldc, 0, alo, blo);
 }
 
 template int mg_bf3_tn<double, double>(Ctx*, int, int, int, double, const double*, int,

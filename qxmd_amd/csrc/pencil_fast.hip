@@ -101,11 +101,15 @@ __device__ inline int mirror_col_f(int col) {
 }
 
 // ---- z inverse: sphere -> T1[bp][z slot][col] ------------------------------
-// psi32 (optional): receives the FP32 copy of the coefficients read
+// psi32 (optional): receives the FP32 copy of the coefficients read;
+// psis (optional): their bf16 hi | lo split instead (row stride ldk, lo half
+// at offset slo), the operand of the split bf16x3 KB projection
 template <typename T, int NA, int NB>
 __global__ void __launch_bounds__(kFastThreads) k_zinv_f(FastArgs a, const double* __restrict__ psi,
                                                          cplx<T>* __restrict__ t1,
-                                                         float* __restrict__ psi32) {
+                                                         float* __restrict__ psi32,
+                                                         unsigned short* __restrict__ psis,
+                                                         int ldk, long slo) {
   constexpr int N = NA * NB;
   constexpr int PB = zpairs(N, (int)sizeof(cplx<T>));
   __shared__ cplx<T> tw[N];
@@ -133,6 +137,18 @@ __global__ void __launch_bounds__(kFastThreads) k_zinv_f(FastArgs a, const doubl
         qa[a.nplwex + 2 * p + 1] = (float)bi;
       }
     }
+    if (psis) {
+      unsigned h, l;
+      unsigned short* qa = psis + (size_t)ba * ldk + 2 * p;
+      split_bf16x2((float)ar, (float)ai, h, l);
+      *reinterpret_cast<unsigned*>(qa) = h;
+      *reinterpret_cast<unsigned*>(qa + slo) = l;
+      if (hasb) {
+        split_bf16x2((float)br, (float)bi, h, l);
+        *reinterpret_cast<unsigned*>(qa + ldk) = h;
+        *reinterpret_cast<unsigned*>(qa + ldk + slo) = l;
+      }
+    }
     int col = a.pt_col[p] - c0, z = a.pt_z[p];
     if (p == 0) {
       lines[col][0] = mk<T>((T)ar, (T)br);
@@ -149,8 +165,9 @@ __global__ void __launch_bounds__(kFastThreads) k_zinv_f(FastArgs a, const doubl
     np_stepB<T, NA, NB>(lines[it / NA], it % NA, +1);
   __syncthreads();
   cplx<T>* out = t1 + (size_t)bp * N * a.ncol;
+  const unsigned mc = div_magic((unsigned)ncl);
   for (int i = threadIdx.x; i < ncl * N; i += blockDim.x) {
-    int zs = i / ncl, cl = i % ncl;
+    const int zs = fdiv(i, mc), cl = i - zs * ncl;
     out[(size_t)zs * a.ncol + c0 + cl] = lines[cl][zs];
   }
 }
@@ -284,11 +301,16 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
   __syncthreads();
 
   // y inverse (line fastest across lanes)
-  for (int it = threadIdx.x; it < nH * NB; it += blockDim.x)
-    np_stepA<T, NA, NB>(A + (it % nH) * sa, it / nH, tw, +1);
+  const unsigned mH = div_magic((unsigned)nH);
+  for (int it = threadIdx.x; it < nH * NB; it += blockDim.x) {
+    const int q = fdiv(it, mH);
+    np_stepA<T, NA, NB>(A + (it - q * nH) * sa, q, tw, +1);
+  }
   __syncthreads();
-  for (int it = threadIdx.x; it < nH * NA; it += blockDim.x)
-    np_stepB<T, NA, NB>(A + (it % nH) * sa, it / nH, +1);
+  for (int it = threadIdx.x; it < nH * NA; it += blockDim.x) {
+    const int q = fdiv(it, mH);
+    np_stepB<T, NA, NB>(A + (it - q * nH) * sa, q, +1);
+  }
   __syncthreads();
 
   for (int g = wave; g < NG; g += nwave) {
@@ -329,11 +351,15 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
   }
   __syncthreads();
   // y forward (line fastest across lanes)
-  for (int it = threadIdx.x; it < nH * NA; it += blockDim.x)
-    pn_stepB<T, NA, NB>(A + (it % nH) * sa, it / nH, tw, -1);
+  for (int it = threadIdx.x; it < nH * NA; it += blockDim.x) {
+    const int q = fdiv(it, mH);
+    pn_stepB<T, NA, NB>(A + (it - q * nH) * sa, q, tw, -1);
+  }
   __syncthreads();
-  for (int it = threadIdx.x; it < nH * NB; it += blockDim.x)
-    pn_stepA<T, NA, NB>(A + (it % nH) * sa, it / nH, -1);
+  for (int it = threadIdx.x; it < nH * NB; it += blockDim.x) {
+    const int q = fdiv(it, mH);
+    pn_stepA<T, NA, NB>(A + (it - q * nH) * sa, q, -1);
+  }
   __syncthreads();
   cplx<T>* out = t2 + ((size_t)bp * N + zs) * a.ncol;
   for (int cl = threadIdx.x; cl < a.ncol; cl += blockDim.x)
@@ -358,8 +384,9 @@ __global__ void __launch_bounds__(kFastThreads) k_zfwd_f(FastArgs a, const doubl
   const int c0 = q0 == 0 ? 0 : 2 * q0 - 1, c1 = 2 * q1 - 1, ncl = c1 - c0;
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = mk<T>((T)a.tw[2 * k], (T)a.tw[2 * k + 1]);
   const cplx<T>* in = t2 + (size_t)bp * N * a.ncol;
+  const unsigned mc = div_magic((unsigned)ncl);
   for (int i = threadIdx.x; i < ncl * N; i += blockDim.x) {
-    int zs = i / ncl, cl = i % ncl;
+    const int zs = fdiv(i, mc), cl = i - zs * ncl;
     lines[cl][zs] = in[(size_t)zs * a.ncol + c0 + cl];
   }
   __syncthreads();
@@ -453,11 +480,16 @@ __global__ void __launch_bounds__(rho_max_threads(RG)) k_plane_rho_f(FastArgs a,
     for (int cl = threadIdx.x; cl < a.ncol; cl += blockDim.x)
       A[a.col_hi[cl] * sa + a.col_kmod[cl]] = in[cl];
     __syncthreads();
-    for (int it = threadIdx.x; it < nH * NB; it += blockDim.x)
-      np_stepA<T, NA, NB>(A + (it % nH) * sa, it / nH, tw, +1);
+    const unsigned mH = div_magic((unsigned)nH);
+    for (int it = threadIdx.x; it < nH * NB; it += blockDim.x) {
+      const int q = fdiv(it, mH);
+      np_stepA<T, NA, NB>(A + (it - q * nH) * sa, q, tw, +1);
+    }
     __syncthreads();
-    for (int it = threadIdx.x; it < nH * NA; it += blockDim.x)
-      np_stepB<T, NA, NB>(A + (it % nH) * sa, it / nH, +1);
+    for (int it = threadIdx.x; it < nH * NA; it += blockDim.x) {
+      const int q = fdiv(it, mH);
+      np_stepB<T, NA, NB>(A + (it - q * nH) * sa, q, +1);
+    }
     __syncthreads();
 #pragma unroll
     for (int gi = 0; gi < RG; ++gi) {
@@ -733,7 +765,10 @@ static void plane_strides(int NA, int NB, int nH, int nwave, int* psa, int* psr)
 // mid (optional): work launched between the plane pass and the z forward pass,
 // e.g. the KB contractions of the fused mixed band-apply (needs one batch)
 struct FastMid {
-  float* psi32;        // FP32 copy of psi, written by the z inverse pass
+  float* psi32;        // FP32 copy of psi, written by the z inverse pass, or
+  unsigned short* psis;  // its bf16 hi | lo split (row stride ldk, lo at slo)
+  int ldk;
+  long slo;
   const float* kb32;   // FP32 term the z forward pass adds
   int (*run)(Ctx*, const double*, int, const FastMid&);
 };
@@ -773,7 +808,7 @@ static PlaneCfg plane_config(Ctx* c, const double* psi, int nb, const T* Vp, cpl
     constexpr int PB = zpairs(N, (int)sizeof(cplx<T>));
     const unsigned gz = (unsigned)((c->pc.npair + PB - 1) / PB);
     hipLaunchKernelGGL((k_zinv_f<T, NA, NB>), dim3(gz, nbb), dim3(kFastThreads), 0, c->stream, a,
-                       psi, t1, nullptr);
+                       psi, t1, nullptr, nullptr, 0, 0L);
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
       float bms = 1e30f;
@@ -845,7 +880,8 @@ static int local_fast_nn(Ctx* c, const double* psi, double* out, int nb, int acc
     a.sr = sr;
     lt_mark(c, 1, false);
     hipLaunchKernelGGL((k_zinv_f<T, NA, NB>), dim3(gz, nbb), dim3(kFastThreads), 0, c->stream, a,
-                       psi, t1, mid ? mid->psi32 : nullptr);
+                       psi, t1, mid ? mid->psi32 : nullptr, mid ? mid->psis : nullptr,
+                       mid ? mid->ldk : 0, mid ? mid->slo : 0L);
     hipLaunchKernelGGL((k_plane_f<T, NA, NB>), dim3(N, nbb), dim3(64 * nwave), lp, c->stream, a,
                        Vp, t1, t2);
     if (mid) {  // the mid work is not part of the FFT stage's launch time
@@ -898,7 +934,7 @@ static int density_fast_nn(Ctx* c, const double* psi, int nb, const double* w) {
     a.sa = sa;
     a.sr = sr;
     hipLaunchKernelGGL((k_zinv_f<T, NA, NB>), dim3(gz, nbb), dim3(kFastThreads), 0, c->stream, a,
-                       psi, t1, nullptr);
+                       psi, t1, nullptr, nullptr, 0, 0L);
     int ng = std::min(groups_per_batch, nbb);
     int per = (nbb + ng - 1) / ng;
     ng = (nbb + per - 1) / per;
@@ -936,7 +972,7 @@ static int local_fast_t(Ctx* c, const double* psi, double* out, int nb, int accu
 }
 
 static int fused_kb_mid(Ctx* c, const double* psi, int nb, const FastMid& m) {
-  return kb_mixed_f32(c, psi, m.psi32, nb, const_cast<float*>(m.kb32));
+  return kb_mixed_f32(c, psi, m.psi32, m.psis, m.ldk, m.slo, nb, const_cast<float*>(m.kb32));
 }
 
 int hpsi_mixed_fused(Ctx* c, const double* psi, double* out, int nb) {
@@ -944,11 +980,21 @@ int hpsi_mixed_fused(Ctx* c, const double* psi, double* out, int nb) {
       gemm_path() == GEMM_OWN_F32 || !use_big(c->npj, nb) ||
       !use_big(c->nplwex, nb) || (c->nplwex & 1) || (c->npj & 1))
     return -3;
-  FastMid m;
-  m.psi32 = mg_f32_scratch(c, 0, (size_t)c->nplwex * nb);
+  FastMid m{};
+  if (gemm_path() == GEMM_OWN) {
+    // the bf16 split of psi, band rows of round_up(nplwex, 64) (zero pads),
+    // the lo half after room for nbmax bands, so the pads never move
+    m.ldk = (c->nplwex + 63) / 64 * 64;
+    m.slo = (long)c->nbmax * m.ldk;
+    m.psis = mg_u16_scratch(c, 0, 2 * (size_t)m.slo);
+    if (!m.psis) return -3;
+  } else {
+    m.psi32 = mg_f32_scratch(c, 0, (size_t)c->nplwex * nb);
+    if (!m.psi32) return -3;
+  }
   m.kb32 = mg_f32_scratch(c, 1, (size_t)c->nplwex * nb);
   m.run = fused_kb_mid;
-  if (!m.psi32 || !m.kb32) return -3;
+  if (!m.kb32) return -3;
   return local_fast_t<float>(c, psi, out, nb, 0, &m);
 }
 

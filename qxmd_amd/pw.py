@@ -228,9 +228,10 @@ class PlaneWave:
             return ("hipBLASLt FP32 GEMM, FAST_TF32 split products on bf16 MFMA"
                     if prec == MIXED else "hipBLASLt FP64 GEMM")
         if prec == MIXED:
-            return ("k_mg_bf3: split bf16x3 products on v_mfma_f32_16x16x32_bf16"
+            return ("k_mg_ps3: split bf16x3 products on v_mfma_f32_16x16x32_bf16, operands "
+                    "split beforehand (the KB table once, psi by the z-inverse pass)"
                     if path == GEMM_OWN else "k_gemm_big<float>: v_mfma_f32_16x16x4f32")
-        return "k_mg_f64: v_mfma_f64_16x16x4f64, 128 x 128 tiles"
+        return "k_mg_f64: v_mfma_f64_16x16x4f64, 128 or 96 tile edges per shape"
 
     def set_cg_graph(self, on):
         """Replay the batched CG's captured iteration graph (on by default)."""
