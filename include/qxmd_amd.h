@@ -158,7 +158,9 @@ int qx_eigen_bkl_batched_slot(qx_ctx *ctx, int psi, int hpsi, int nb, const doub
 int qx_ekib_slot(qx_ctx *ctx, int slot, int nb, double *ekib);
 /* one SCF band step with frozen potential (energy pass, rotation, CG,
  * Gram-Schmidt, density) -- see Sources/pwlda.f90:2087-3574; psi/hpsi are
- * slots 0 and 1 as for qx_eigen_bkl_slot */
+ * slots 0 and 1 as for qx_eigen_bkl_slot.  prec QX_MIXED: FP32 FFTs and
+ * split bf16x3 products for H psi, the subspace-H, rotation, overlap and CG
+ * products; the eigensolver and the Cholesky of the overlap stay FP64. */
 int qx_scf_band_step(qx_ctx *ctx, int psi, int hpsi, int nb, const double *occ, int noband,
                      double rvol, double *eig, double *rho, int *lcholesky, int prec);
 /* qx_scf_band_step with the CG form chosen: batched != 0 uses
@@ -459,6 +461,17 @@ int qx_kset_kb(qx_ctx *ctx, int ik, int npj, const double *beta, const double *D
  * per_band = 0: each stage over all (k, band) blocks in one launch; 1: one
  * launch per (k, band) block (the per-band loop the batched form replaces). */
 int qx_khpsi_slot(qx_ctx *ctx, int src, int dst, int nb, int per_band);
+/* Lowest nb eigenpairs of H_k at every k-point in the context's local
+ * potential (the band update of the k-point form; the reference has none):
+ * block Rayleigh-Ritz steps on [psi, W] with W the residuals preconditioned
+ * as prpcg does (Sources/eigen.f90:1093-1113) and scaled to unit norm, the
+ * small generalized Hermitian problems of all k-points in one batched
+ * rocSOLVER solve.  Starts from the bands of k-slot src; leaves the Ritz
+ * vectors in src, H of them in dst, eig(nb, nk) the Ritz values (Ry), *iters
+ * the steps taken; stops when no Ritz value moves by more than tol or after
+ * max_iter steps.  Returns -21 if a basis overlap is not positive definite. */
+int qx_ksolve_slot(qx_ctx *ctx, int src, int dst, int max_iter, double tol, double *eig,
+                   int *iters);
 
 #ifdef __cplusplus
 }

@@ -129,7 +129,7 @@ int hpsi_batched(Ctx* c, const double* x, double* hx, int nb, int prec) {
 }
 
 // Gram-Schmidt by Cholesky of the packed overlap (schmidt2)
-int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host) {
+int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host, int prec) {
   const int n = c->nplwex;
   double* S = c->d_small;                         // nb x nb overlap, then L
   double* Linv = c->d_small + (size_t)(nb + 1) * nb;  // row-major L^-1
@@ -137,7 +137,9 @@ int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host) {
   int* flag = ok + 2;
   // S[i][j] = <psi_j|psi_i> (C[n=i][m=j]); symmetric, so its storage is also
   // the column-major S(i,j).
-  int rc = gemm_tn(c, nb, nb, n, 2.0, psi, n, psi, n, 0.0, S, nb, 1);
+  // the overlap product in the given precision (split bf16x3 in the mixed
+  // form); the Cholesky and the inverse stay FP64
+  int rc = gemm_tn_p(c, nb, nb, n, 2.0, psi, n, psi, n, 0.0, S, nb, 1, prec);
   if (rc) return rc;
   if ((rc = dense_chol_inv(c, S, nb, Linv, ok, flag))) return rc;
   int okh[2] = {0, 0};
@@ -148,14 +150,16 @@ int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host) {
   if (!okh[0]) return 0;
   // psi'_ib = sum_jb Linv(ib,jb) psi_jb: gemm_nn's B[n*ldb + k] = Linv(n,k)
   double* out = c->d_slot[5];
-  if ((rc = gemm_nn(c, n, nb, nb, 1.0, psi, n, Linv, nb, 0.0, out, n))) return rc;
+  if ((rc = gemm_nn_p(c, n, nb, nb, 1.0, psi, n, Linv, nb, 0.0, out, n, prec))) return rc;
   QX_CHECK(hipMemcpyAsync(psi, out, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice,
                           c->stream));
   return 0;
 }
 
 // subspace rotation: H_sub, eigenvectors U, psi <- psi U, hpsi <- hpsi U
-static int rotate_dev(Ctx* c, double* psi, double* hpsi, int nb, double* eig_host) {
+// (prec: the three large products; the eigensolver stays FP64)
+static int rotate_dev(Ctx* c, double* psi, double* hpsi, int nb, double* eig_host,
+                      int prec = PREC_FP64) {
   const int n = c->nplwex;
   double* C = c->d_small;
   double* H = C + (size_t)nb * nb;
@@ -163,16 +167,16 @@ static int rotate_dev(Ctx* c, double* psi, double* hpsi, int nb, double* eig_hos
   double* work = Z + (size_t)nb * nb;
   double* w = c->d_scal + 40 * (size_t)c->nbmax;
   int* info = reinterpret_cast<int*>(c->d_scal + 63 * (size_t)c->nbmax);
-  int rc = gemm_tn(c, nb, nb, n, 2.0, psi, n, hpsi, n, 0.0, C, nb, 1);
+  int rc = gemm_tn_p(c, nb, nb, n, 2.0, psi, n, hpsi, n, 0.0, C, nb, 1, prec);
   if (rc) return rc;
   if ((rc = dense_symmetrize(c, C, nb, H, nullptr))) return rc;
   if ((rc = dense_eigh(c, H, nb, Z, w, work, info))) return rc;
   // psi' = psi U: B(n=b, k=j) = U(j,b) = Z[j + nb*b] -> ldb = nb with B[n*ldb+k]
   double* out = c->d_slot[5];
-  if ((rc = gemm_nn(c, n, nb, nb, 1.0, psi, n, Z, nb, 0.0, out, n))) return rc;
+  if ((rc = gemm_nn_p(c, n, nb, nb, 1.0, psi, n, Z, nb, 0.0, out, n, prec))) return rc;
   QX_CHECK(hipMemcpyAsync(psi, out, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice,
                           c->stream));
-  if ((rc = gemm_nn(c, n, nb, nb, 1.0, hpsi, n, Z, nb, 0.0, out, n))) return rc;
+  if ((rc = gemm_nn_p(c, n, nb, nb, 1.0, hpsi, n, Z, nb, 0.0, out, n, prec))) return rc;
   QX_CHECK(hipMemcpyAsync(hpsi, out, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice,
                           c->stream));
   if (eig_host) {
@@ -530,7 +534,7 @@ int qx_schmidt_slot(qx_ctx* h, int slot, int nb, int* lcholesky) {
   if (!h) return fail(-1, "qx_schmidt_slot: null context");
   Ctx* c = &h->c;
   if (int e = check_slots(c, "qx_schmidt_slot", nb, slot, -1, kSlotStaged)) return e;
-  return schmidt_dev(c, c->d_slot[slot], nb, lcholesky);
+  return schmidt_dev(c, c->d_slot[slot], nb, lcholesky, PREC_FP64);
 }
 
 int qx_rotate_slot(qx_ctx* h, int psi, int hpsi, int nb, double* eig) {
@@ -620,7 +624,9 @@ int qx_scf_band_step_ex(qx_ctx* h, int psi, int hpsi, int nb, const double* occ,
   if ((rc = qx_ekib_slot(h, psi, nb, ekib.data()))) return rc;
   QX_CHECK(hipEventRecord(c->pev[1], c->stream));
   // subspace rotation
-  if ((rc = rotate_dev(c, P, HP, nb, eig))) return rc;
+  // the mixed form runs the overlap, subspace-H and rotation products as
+  // split bf16x3 products (the eigensolver and Cholesky stay FP64)
+  if ((rc = rotate_dev(c, P, HP, nb, eig, prec))) return rc;
   QX_CHECK(hipEventRecord(c->pev[2], c->stream));
   // CG
   if (batched)
@@ -631,7 +637,7 @@ int qx_scf_band_step_ex(qx_ctx* h, int psi, int hpsi, int nb, const double* occ,
                           bz.data(), nr.data(), prec);
   if (rc) return rc;
   QX_CHECK(hipEventRecord(c->pev[3], c->stream));
-  if ((rc = schmidt_dev(c, P, nb, lcholesky))) return rc;
+  if ((rc = schmidt_dev(c, P, nb, lcholesky, prec))) return rc;
   QX_CHECK(hipEventRecord(c->pev[4], c->stream));
   if ((rc = qx_density_slot(h, psi, nb, occ, rvol, rho, prec))) return rc;
   QX_CHECK(hipEventRecord(c->pev[5], c->stream));
@@ -834,7 +840,7 @@ int qx_schmidt(qx_ctx* h, double* cgjr, int nband, int* lcholesky) {
   if (nband > c->nbmax) return fail(-1, "qx_schmidt: nband > nbmax");
   int rc = put_internal(c, c->d_slot[0], cgjr, nband);
   if (rc) return rc;
-  if ((rc = schmidt_dev(c, c->d_slot[0], nband, lcholesky))) return rc;
+  if ((rc = schmidt_dev(c, c->d_slot[0], nband, lcholesky, PREC_FP64))) return rc;
   if (!*lcholesky) return 0;
   return get_reference(c, c->d_slot[0], cgjr, nband);
 }
@@ -1195,6 +1201,15 @@ int qx_kslot_get(qx_ctx* h, int slot, int ik, double* psi, int nb) {
 int qx_kset_kb(qx_ctx* h, int ik, int npj, const double* beta, const double* D) {
   int rc = h && (npj == 0 || (beta && D)) ? kp_set_kb(&h->c, ik, npj, beta, D) : -1;
   return rc ? fail(rc, "qx_kset_kb: no k-points set, bad k-point or projector count (%d)", rc)
+            : 0;
+}
+
+int qx_ksolve_slot(qx_ctx* h, int src, int dst, int max_iter, double tol, double* eig,
+                   int* iters) {
+  int rc = h ? kp_solve(&h->c, src, dst, max_iter, tol, eig, iters) : -1;
+  if (rc == -21)
+    return fail(rc, "qx_ksolve_slot: the overlap of a Rayleigh-Ritz basis is not positive definite");
+  return rc ? fail(rc, "qx_ksolve_slot: bad slots, no k-points set, or a solver error (%d)", rc)
             : 0;
 }
 

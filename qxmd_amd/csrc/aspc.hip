@@ -254,7 +254,7 @@ int aspc_step(Ctx* c, double* psi, int nb, int nstep, int nstep_ini, int* iaspc_
                             c->stream));
   *iaspc_out = iaspc;
   if (S.ihest != 0) {
-    if ((rc = schmidt_dev(c, psi, nb, lchol))) return rc;
+    if ((rc = schmidt_dev(c, psi, nb, lchol, PREC_FP64))) return rc;
   }
   QX_CHECK(hipStreamSynchronize(c->stream));
   return 0;

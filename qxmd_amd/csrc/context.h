@@ -235,6 +235,7 @@ int kp_slot_put(Ctx* c, int slot, int ik, const double* psi, int nb);
 int kp_slot_get(Ctx* c, int slot, int ik, double* psi, int nb);
 int kp_set_kb(Ctx* c, int ik, int npj, const double* beta, const double* D);
 int kp_hpsi(Ctx* c, int src, int dst, int nb, int per_band);
+int kp_solve(Ctx* c, int src, int dst, int max_iter, double tol, double* eig, int* iters);
 int cg_graph_entries(Ctx* c);
 int mg_dgemm_tn(Ctx* c, int M, int N, int K, double alpha, const double* A, int lda,
                 const double* B, int ldb, double beta, double* C, int ldc, int corr);
@@ -306,7 +307,7 @@ int kb_forces(Ctx* c, const double* psi, int nb, const double* occ_host, const d
 int lopp_set(Ctx* c, int ntype, int nplw5, const double* svco, const double* gx,
              const double* gy, const double* gz);
 void lopp_destroy(Ctx* c);
-int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host);
+int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host, int prec);
 // wave-function extrapolation between MD steps (aspc.hip)
 void aspc_destroy(Ctx* c);
 int aspc_init(Ctx* c, int nprvmx, int ihest);
