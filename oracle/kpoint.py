@@ -13,6 +13,10 @@ definitions with numpy's FFT:
     u(r)  = sum_G c(G) e^{iGr}                 (numpy ifftn x N)
     H c   = |k+G|^2 c + FFT[V u] / N + sum_p beta_p D_p <beta_p|c>
 
+hk_dense builds the same operator as a matrix from the convolution theorem
+(no band transforms): the two must agree, which pins the FFT index and sign
+conventions of hk_apply.
+
 Pinning (tests/test_kpoints_cpu.py): at k = 0 a coefficient set with
 c(-G) = conj c(G) reproduces the C oracle's Gamma H (pinned to the reference
 build); the labels of a k-point shifted by a reciprocal vector give the same
@@ -44,3 +48,20 @@ def hk_apply(c, hkl, kin, V, beta=None, D=None):
         P = np.conj(beta) @ c.T  # (npj, nb)
         out += ((np.asarray(D)[:, None] * P).T) @ beta
     return out
+
+
+def hk_dense(hkl, kin, V, beta=None, D=None):
+    """H_k as a dense matrix from its definition, without transforming bands:
+    |k+G|^2 on the diagonal, V^(G - G') (the grid's discrete transform of V,
+    indices modulo the grid) and sum_p beta_p(G) D_p conj(beta_p(G'))."""
+    n3, n2, n1 = V.shape
+    vh = np.fft.fftn(V) / V.size
+    h = np.asarray(hkl)
+    dz = np.mod(h[:, None, 2] - h[None, :, 2], n3)
+    dy = np.mod(h[:, None, 1] - h[None, :, 1], n2)
+    dx = np.mod(h[:, None, 0] - h[None, :, 0], n1)
+    H = vh[dz, dy, dx]
+    H[np.diag_indices_from(H)] += kin
+    if beta is not None and len(beta):
+        H = H + (np.asarray(beta).T * np.asarray(D)[None, :]) @ np.conj(beta)
+    return H

@@ -74,7 +74,7 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
                 print(o)
     tmp = OUT + ".tmp"
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs,
-          "-L/opt/rocm/lib", "-lhipblaslt", "-Wl,-rpath,/opt/rocm/lib"])
+          "-L/opt/rocm/lib", "-lhipblaslt", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, OUT)
     build_fortran(verbose)
     return OUT

@@ -40,6 +40,7 @@
 #include <map>
 #include <tuple>
 #include <vector>
+#include <rocsolver/rocsolver.h>
 #include "context.h"
 
 namespace qx {
@@ -51,6 +52,7 @@ constexpr int kKCols = 16;  // columns per block of the z kernels
 
 struct KArgs {
   int n1, n2, n3, nb, npw_max, ncol_max, kb0;
+  int rpk, r0;        // band rows of a k-point in the src/dst arrays, first row used
   const int* npw;     // [k]
   const int* ncol;    // [k]
   const int* col_p0;  // [k][ncol_max + 1]: first point of column c
@@ -79,7 +81,7 @@ __global__ void __launch_bounds__(kKThreads)
   __syncthreads();
   const int* cp = a.col_p0 + (size_t)k * (a.ncol_max + 1);
   const int* pz = a.pt_z + (size_t)k * a.npw_max;
-  const double* src = psi + (size_t)kb * 2 * a.npw_max;
+  const double* src = psi + ((size_t)k * a.rpk + a.r0 + (kb - k * a.nb)) * 2 * a.npw_max;
   const int p0 = cp[c0], p1 = cp[c0 + cn];
   for (int p = p0 + (int)threadIdx.x; p < p1; p += blockDim.x) {
     // the column of p: the last c with cp[c] <= p (columns are short)
@@ -144,7 +146,8 @@ __global__ void __launch_bounds__(kKThreads)
   const int kb = a.kb0 + (int)blockIdx.y;
   const int k = kb / a.nb;
   const int nc = a.ncol[k], np = a.npw[k];
-  double* dst = out + (size_t)kb * 2 * a.npw_max;
+  const size_t row = (size_t)k * a.rpk + a.r0 + (kb - k * a.nb);
+  double* dst = out + row * 2 * a.npw_max;
   if (blockIdx.x == 0)  // the padding beyond npw_k stays zero
     for (int i = 2 * np + (int)threadIdx.x; i < 2 * a.npw_max; i += blockDim.x) dst[i] = 0.0;
   const int c0 = (int)blockIdx.x * kKCols;
@@ -163,7 +166,7 @@ __global__ void __launch_bounds__(kKThreads)
   const int* cp = a.col_p0 + (size_t)k * (a.ncol_max + 1);
   const int* pz = a.pt_z + (size_t)k * a.npw_max;
   const double* kin = a.kin + (size_t)k * a.npw_max;
-  const double* src = psi + (size_t)kb * 2 * a.npw_max;
+  const double* src = psi + row * 2 * a.npw_max;
   const int p0 = cp[c0], p1 = cp[c0 + cn];
   for (int p = p0 + (int)threadIdx.x; p < p1; p += blockDim.x) {
     int c = c0;
@@ -187,6 +190,121 @@ __global__ void k_kscale(double* P, const double* D, int npj, int nb) {
   P[i] *= D[(int)(i % (2 * npj)) % npj];
 }
 
+// ---- band solver kernels (rows of 2 L doubles, L = npw_max) -----------------
+// copy nb rows per k-point: dst row k drpk + dr0 + b <- src row k srpk + sr0 + b
+__global__ void k_krows(const double* __restrict__ src, int srpk, int sr0, double* __restrict__ dst,
+                        int drpk, int dr0, int nb, long L2) {
+  const int kb = blockIdx.y, k = kb / nb, b = kb % nb;
+  const double* s = src + ((size_t)k * srpk + sr0 + b) * L2;
+  double* d = dst + ((size_t)k * drpk + dr0 + b) * L2;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < L2; i += (long)gridDim.x * blockDim.x)
+    d[i] = s[i];
+}
+
+// J c = i c: rows [0, nbs) of each k-point's block -> rows [nbs, 2 nbs)
+__global__ void k_kjrows(double* __restrict__ Z, int rpk, int nbs, long L2) {
+  const int kb = blockIdx.y, k = kb / nbs, b = kb % nbs;
+  const double2* s = reinterpret_cast<const double2*>(Z + ((size_t)k * rpk + b) * L2);
+  double2* d = reinterpret_cast<double2*>(Z + ((size_t)k * rpk + nbs + b) * L2);
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < L2 / 2; i += (long)gridDim.x * blockDim.x) {
+    const double2 v = s[i];
+    d[i] = make_double2(-v.y, v.x);
+  }
+}
+
+// one block per (k, band): ekin = <c|T|c> / <c|c> over the full sphere (the
+// kinetic energy the preconditioner scales with, the ekib of a real Gamma
+// band) -- for bands of rows [0, nb)
+__global__ void k_kekin(const double* __restrict__ Z, int rpk, int nb, int L,
+                        const double* __restrict__ kin, double* __restrict__ ekin) {
+  const int kb = blockIdx.x, k = kb / nb, b = kb % nb;
+  const double* z = Z + ((size_t)k * rpk + b) * 2 * L;
+  const double* q = kin + (size_t)k * L;
+  double t = 0.0, n = 0.0;
+  for (int i = threadIdx.x; i < L; i += blockDim.x) {
+    const double a = z[2 * i] * z[2 * i] + z[2 * i + 1] * z[2 * i + 1];
+    t += q[i] * a;
+    n += a;
+  }
+  __shared__ double st[256], sn[256];
+  st[threadIdx.x] = t;
+  sn[threadIdx.x] = n;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      st[threadIdx.x] += st[threadIdx.x + o];
+      sn[threadIdx.x] += sn[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ekin[kb] = st[0] / sn[0];
+}
+
+// W = K (H psi - lambda psi), the Teter-Payne-Allan preconditioner of prpcg
+// (Sources/eigen.f90:1093-1113) on |k+G|^2, then W scaled to unit norm; one
+// block per (k, band); rows: psi b, H psi b in Z / HZ, W -> Z row nb + b
+__global__ void k_kresid(double* __restrict__ Z, const double* __restrict__ HZ, int rpk, int nb,
+                         int L, const int* __restrict__ npw, const double* __restrict__ kin,
+                         const double* __restrict__ lam, const double* __restrict__ ekin) {
+  const int kb = blockIdx.x, k = kb / nb, b = kb % nb;
+  const double* z = Z + ((size_t)k * rpk + b) * 2 * L;
+  const double* h = HZ + ((size_t)k * rpk + b) * 2 * L;
+  double* w = Z + ((size_t)k * rpk + nb + b) * 2 * L;
+  const double* q = kin + (size_t)k * L;
+  const double l = lam[kb], e15 = 1.5 * ekin[kb];
+  const int np = npw[k];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < L; i += blockDim.x) {
+    double re = 0.0, im = 0.0;
+    if (i < np) {
+      const double x = q[i] / e15;
+      double tesu = 27.0 + x * (18.0 + x * (12.0 + 8.0 * x));
+      const double x2 = x * x;
+      tesu = tesu / (tesu + 16.0 * x2 * x2) * 2.0 / e15;
+      re = tesu * (h[2 * i] - l * z[2 * i]);
+      im = tesu * (h[2 * i + 1] - l * z[2 * i + 1]);
+    }
+    w[2 * i] = re;
+    w[2 * i + 1] = im;
+    s += re * re + im * im;
+  }
+  __shared__ double sr[256];
+  sr[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sr[threadIdx.x] += sr[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double f = sr[0] > 0.0 ? 1.0 / sqrt(sr[0]) : 0.0;
+  for (int i = threadIdx.x; i < 2 * L; i += blockDim.x) w[i] *= f;
+}
+
+// the stacked products G[n][m] (m < 2 nbs: rows of [Y; J Y]) -> complex
+// column-major A(m, n) = G[n][m] + i G[n][nbs + m], m, n < nbs
+__global__ void k_kpack(const double* __restrict__ G, int ldg, int nbs, int ldc,
+                        rocblas_double_complex* __restrict__ A, long strideG, long strideA) {
+  const int k = blockIdx.y;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)nbs * nbs) return;
+  const int m = (int)(i % nbs), n = (int)(i / nbs);
+  const double* g = G + k * strideG + (size_t)n * ldg;
+  A[k * strideA + m + (size_t)n * ldc] = rocblas_double_complex(g[m], g[nbs + m]);
+}
+
+// the lowest nb eigenvectors (columns of A, nbs long) -> Cs[n][j] = [Re; Im]
+// (4 nb rows per column, the first 2 nbs used)
+__global__ void k_kunpack(const rocblas_double_complex* __restrict__ A, int lda, int nbs, int nb,
+                          double* __restrict__ Cs, int ldcs, long strideA, long strideC) {
+  const int k = blockIdx.y;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)nbs * nb) return;
+  const int j = (int)(i % nbs), n = (int)(i / nbs);
+  const rocblas_double_complex v = A[k * strideA + j + (size_t)n * lda];
+  double* c = Cs + k * strideC + (size_t)n * ldcs;
+  c[j] = v.real();
+  c[nbs + j] = v.imag();
+}
+
 }  // namespace
 
 struct KPoints {
@@ -207,23 +325,36 @@ struct KPoints {
   double* d_A = nullptr;
   double* d_D = nullptr;
   double* d_P = nullptr;                   // 2 npj x nb
+  // band solver: per k-point the basis Y = [psi, W] and its partner J Y (J c
+  // = i c in the interleaved layout), [k][4 nb][2 npw_max], and H of it
+  double *d_Z = nullptr, *d_HZ = nullptr;
+  double* d_G = nullptr;                   // [k][2 nb][4 nb]: the stacked products
+  rocblas_double_complex *d_Hs = nullptr, *d_Ss = nullptr;  // [k][2 nb][2 nb]
+  double *d_w = nullptr, *d_E = nullptr;   // eigenvalues, solver work
+  int* d_info = nullptr;
+  double* d_Cs = nullptr;                  // [k][nb][4 nb]: [Re C; Im C]
+  double *d_ekin = nullptr, *d_lam = nullptr;  // per (k, band)
+  rocblas_handle rb = nullptr;
 };
 
 namespace {
 
 void kp_free(KPoints* k) {
   void* ptrs[] = {k->d_npw, k->d_ncol, k->d_col_p0, k->d_col_xy, k->d_pt_z, k->d_kin,
-                  k->d_T, k->d_A, k->d_D, k->d_P};
+                  k->d_T, k->d_A, k->d_D, k->d_P, k->d_Z, k->d_HZ, k->d_G, k->d_Hs, k->d_Ss,
+                  k->d_w, k->d_E, k->d_info, k->d_Cs, k->d_ekin, k->d_lam};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (k->rb) (void)rocblas_destroy_handle(k->rb);
   for (auto* p : k->d_slot)
     if (p) (void)hipFree(p);
 }
 
-KArgs kargs(const Ctx* c, const KPoints* k, int kb0) {
+KArgs kargs(const Ctx* c, const KPoints* k, int kb0, int rpk, int r0) {
   KArgs a{};
   a.n1 = c->n1; a.n2 = c->n2; a.n3 = c->n3;
   a.nb = k->nb; a.npw_max = k->npw_max; a.ncol_max = k->ncol_max; a.kb0 = kb0;
+  a.rpk = rpk; a.r0 = r0;
   a.npw = k->d_npw; a.ncol = k->d_ncol; a.col_p0 = k->d_col_p0; a.col_xy = k->d_col_xy;
   a.pt_z = k->d_pt_z; a.kin = k->d_kin; a.tw = c->d_tw;
   a.px = c->plx; a.py = c->ply; a.pz = c->plz;
@@ -465,9 +596,11 @@ int kp_set_kb(Ctx* c, int ik, int npj, const double* beta, const double* D) {
   return 0;
 }
 
-// local part of H for blocks kb0 .. kb0 + nkb - 1 (all within one batch of T)
-static int kp_local_run(Ctx* c, KPoints* k, const double* src, double* dst, int kb0, int nkb) {
-  const KArgs a = kargs(c, k, kb0);
+// local part of H for blocks kb0 .. kb0 + nkb - 1 (all within one batch of T);
+// band b of k-point k is row k rpk + r0 + b of src and dst
+static int kp_local_run(Ctx* c, KPoints* k, const double* src, double* dst, int kb0, int nkb,
+                        int rpk, int r0) {
+  const KArgs a = kargs(c, k, kb0, rpk, r0);
   const dim3 gz(cdivu(k->ncol_max, kKCols), nkb);
   hipLaunchKernelGGL(k_kzinv, gz, dim3(kKThreads), 0, c->stream, a, src, k->d_T);
   hipLaunchKernelGGL(k_kplane, dim3(c->n3, nkb), dim3(kKThreads), plane_lds(c), c->stream, a,
@@ -483,34 +616,31 @@ static int kp_local_run(Ctx* c, KPoints* k, const double* src, double* dst, int 
 // (one launch per batch; one batch when all blocks fit), the KB products one
 // pair per k-point; per_band = 1: the comparison loop, one (k, band) block per
 // launch of each stage and the KB products per band.
-int kp_hpsi(Ctx* c, int src, int dst, int nb, int per_band) {
-  auto* k = static_cast<KPoints*>(c->kp);
-  if (!k || src < 0 || dst < 0 || src >= KPoints::kSlots || dst >= KPoints::kSlots || src == dst ||
-      nb < 1 || nb > k->nb)
-    return -1;
-  const double* S = k->d_slot[src];
-  double* H = k->d_slot[dst];
+static int kp_apply(Ctx* c, KPoints* k, const double* S, double* H, int rpk, int r0, int nb,
+                    int per_band) {
   const size_t bstride = (size_t)2 * k->npw_max;
   const int total = k->nk * k->nb;
   if (per_band) {
     for (int ik = 0; ik < k->nk; ++ik)
       for (int b = 0; b < nb; ++b)
-        if (int rc = kp_local_run(c, k, S, H, ik * k->nb + b, 1)) return rc;
+        if (int rc = kp_local_run(c, k, S, H, ik * k->nb + b, 1, rpk, r0)) return rc;
   } else if (nb == k->nb) {
     for (int kb0 = 0; kb0 < total; kb0 += k->batch)
-      if (int rc = kp_local_run(c, k, S, H, kb0, std::min(k->batch, total - kb0))) return rc;
+      if (int rc = kp_local_run(c, k, S, H, kb0, std::min(k->batch, total - kb0), rpk, r0))
+        return rc;
   } else {
     for (int ik = 0; ik < k->nk; ++ik)
       for (int b0 = 0; b0 < nb; b0 += k->batch)
-        if (int rc = kp_local_run(c, k, S, H, ik * k->nb + b0, std::min(k->batch, nb - b0)))
+        if (int rc = kp_local_run(c, k, S, H, ik * k->nb + b0, std::min(k->batch, nb - b0), rpk,
+                                  r0))
           return rc;
   }
   if (k->npj == 0) return 0;
   const int npj2 = 2 * k->npj, K = (int)bstride;
   for (int ik = 0; ik < k->nk; ++ik) {
     const double* A = k->d_A + (size_t)ik * npj2 * bstride;
-    const double* Sk = S + (size_t)ik * k->nb * bstride;
-    double* Hk = H + (size_t)ik * k->nb * bstride;
+    const double* Sk = S + ((size_t)ik * rpk + r0) * bstride;
+    double* Hk = H + ((size_t)ik * rpk + r0) * bstride;
     const int step = per_band ? 1 : nb;
     for (int b0 = 0; b0 < nb; b0 += step) {
       const int n = std::min(step, nb - b0);
@@ -524,6 +654,151 @@ int kp_hpsi(Ctx* c, int src, int dst, int nb, int per_band) {
   }
   QX_CHECK(hipGetLastError());
   return 0;
+}
+
+// ---- band solver ------------------------------------------------------------
+static int kp_solver_alloc(Ctx* c, KPoints* k) {
+  if (k->d_Z) return 0;
+  const size_t L2 = 2 * (size_t)k->npw_max, nb = k->nb, nk = k->nk;
+  const size_t zbytes = sizeof(double) * nk * 4 * nb * L2;
+  QX_CHECK(hipMalloc(&k->d_Z, zbytes));
+  QX_CHECK(hipMalloc(&k->d_HZ, zbytes));
+  QX_CHECK(hipMemset(k->d_Z, 0, zbytes));
+  QX_CHECK(hipMemset(k->d_HZ, 0, zbytes));
+  QX_CHECK(hipMalloc(&k->d_G, sizeof(double) * nk * 2 * (2 * nb) * (4 * nb)));
+  QX_CHECK(hipMalloc(&k->d_Hs, sizeof(rocblas_double_complex) * nk * 4 * nb * nb));
+  QX_CHECK(hipMalloc(&k->d_Ss, sizeof(rocblas_double_complex) * nk * 4 * nb * nb));
+  QX_CHECK(hipMalloc(&k->d_w, sizeof(double) * nk * 2 * nb));
+  QX_CHECK(hipMalloc(&k->d_E, sizeof(double) * nk * 2 * nb));
+  QX_CHECK(hipMalloc(&k->d_info, sizeof(int) * nk));
+  QX_CHECK(hipMalloc(&k->d_Cs, sizeof(double) * nk * nb * 4 * nb));
+  QX_CHECK(hipMalloc(&k->d_ekin, sizeof(double) * nk * nb));
+  QX_CHECK(hipMalloc(&k->d_lam, sizeof(double) * nk * nb));
+  if (rocblas_create_handle(&k->rb) != rocblas_status_success) return -3;
+  buffers_changed(c);
+  return 0;
+}
+
+// Rayleigh-Ritz on the basis Y = rows [0, nbs) of each k-point's Z block (its
+// J Y in rows [nbs, 2 nbs), H Y in HZ likewise): the generalized Hermitian
+// problem (Y^H H Y) x = lambda (Y^H Y) x of every k-point in one batched
+// solve; the lowest nb Ritz vectors and H of them go to rows [0, nb) of the
+// k-slots psi / hpsi (nb rows per k-point), the Ritz values to d_lam and lam.
+static int kp_rayleigh_ritz(Ctx* c, KPoints* k, int nbs, double* psi, double* hpsi,
+                            double* lam_host) {
+  const int nb = k->nb, rpk = 4 * nb, ldg = 4 * nb, ld2 = 2 * nb;
+  const long L2 = 2L * k->npw_max;
+  const int K = (int)L2;
+  // J Y and J H Y
+  hipLaunchKernelGGL(k_kjrows, dim3(64, k->nk * nbs), dim3(256), 0, c->stream, k->d_Z, rpk, nbs, L2);
+  hipLaunchKernelGGL(k_kjrows, dim3(64, k->nk * nbs), dim3(256), 0, c->stream, k->d_HZ, rpk, nbs,
+                     L2);
+  const long strideG = 2L * ld2 * ldg, strideA = (long)ld2 * ld2;
+  for (int ik = 0; ik < k->nk; ++ik) {
+    const double* Zk = k->d_Z + (size_t)ik * rpk * L2;
+    const double* HZk = k->d_HZ + (size_t)ik * rpk * L2;
+    double* Gs = k->d_G + ik * strideG;
+    double* Gh = Gs + (size_t)ld2 * ldg;
+    // G[n][m] = <z_m, y_n> over [Y; J Y] x Y: Re and Im of Y^H Y (and Y^H H Y)
+    if (int rc = gemm_tn(c, 2 * nbs, nbs, K, 1.0, Zk, K, Zk, K, 0.0, Gs, ldg, 0)) return rc;
+    if (int rc = gemm_tn(c, 2 * nbs, nbs, K, 1.0, Zk, K, HZk, K, 0.0, Gh, ldg, 0)) return rc;
+  }
+  const dim3 gp(cdivu((long)nbs * nbs, 256), k->nk);
+  hipLaunchKernelGGL(k_kpack, gp, dim3(256), 0, c->stream, k->d_G, ldg, nbs, ld2, k->d_Ss,
+                     strideG, strideA);
+  hipLaunchKernelGGL(k_kpack, gp, dim3(256), 0, c->stream, k->d_G + (size_t)ld2 * ldg, ldg, nbs,
+                     ld2, k->d_Hs, strideG, strideA);
+  QX_CHECK(hipGetLastError());
+  if (rocblas_set_stream(k->rb, c->stream) != rocblas_status_success) return -3;
+  if (rocsolver_zhegvd_strided_batched(k->rb, rocblas_eform_ax, rocblas_evect_original,
+                                       rocblas_fill_lower, nbs, k->d_Hs, ld2, strideA, k->d_Ss,
+                                       ld2, strideA, k->d_w, ld2, k->d_E, ld2, k->d_info,
+                                       k->nk) != rocblas_status_success)
+    return -3;
+  std::vector<int> info(k->nk);
+  QX_CHECK(hipMemcpyAsync(info.data(), k->d_info, sizeof(int) * k->nk, hipMemcpyDeviceToHost,
+                          c->stream));
+  std::vector<double> w((size_t)k->nk * ld2);
+  QX_CHECK(hipMemcpyAsync(w.data(), k->d_w, sizeof(double) * w.size(), hipMemcpyDeviceToHost,
+                          c->stream));
+  QX_CHECK(hipStreamSynchronize(c->stream));
+  for (int ik = 0; ik < k->nk; ++ik)
+    if (info[ik] != 0) return -21;  // the overlap of the basis is not positive definite
+  std::vector<double> lam((size_t)k->nk * nb);
+  for (int ik = 0; ik < k->nk; ++ik)
+    for (int b = 0; b < nb; ++b) lam[(size_t)ik * nb + b] = w[(size_t)ik * ld2 + b];
+  if (lam_host) std::memcpy(lam_host, lam.data(), sizeof(double) * lam.size());
+  QX_CHECK(hipMemcpyAsync(k->d_lam, lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice,
+                          c->stream));
+  // psi = Y C, H psi = (H Y) C: real NN products on [Y; J Y] with [Re C; Im C]
+  const long strideC = (long)nb * ldg;
+  hipLaunchKernelGGL(k_kunpack, dim3(cdivu((long)nbs * nb, 256), k->nk), dim3(256), 0, c->stream,
+                     k->d_Hs, ld2, nbs, nb, k->d_Cs, ldg, strideA, strideC);
+  for (int ik = 0; ik < k->nk; ++ik) {
+    const double* Zk = k->d_Z + (size_t)ik * rpk * L2;
+    const double* HZk = k->d_HZ + (size_t)ik * rpk * L2;
+    const double* Ck = k->d_Cs + ik * strideC;
+    double* Pk = psi + (size_t)ik * nb * L2;
+    double* Hk = hpsi + (size_t)ik * nb * L2;
+    if (int rc = gemm_nn(c, K, nb, 2 * nbs, 1.0, Zk, K, Ck, ldg, 0.0, Pk, K)) return rc;
+    if (int rc = gemm_nn(c, K, nb, 2 * nbs, 1.0, HZk, K, Ck, ldg, 0.0, Hk, K)) return rc;
+  }
+  QX_CHECK(hipGetLastError());
+  return 0;
+}
+
+// Lowest nb eigenpairs of H_k at every k-point in the fixed local potential:
+// block Rayleigh-Ritz on [psi, W], W the preconditioned residuals (Teter-
+// Payne-Allan, as prpcg) scaled to unit norm, all k-points per step.  Starts
+// from the bands in k-slot src (nb per k-point), leaves the Ritz vectors in
+// src and H of them in dst; eig (nk x nb) the Ritz values; stops when no Ritz
+// value moves by more than tol (Ry) or after max_iter steps.
+int kp_solve(Ctx* c, int src, int dst, int max_iter, double tol, double* eig, int* iters) {
+  auto* k = static_cast<KPoints*>(c->kp);
+  if (!k || src < 0 || dst < 0 || src >= KPoints::kSlots || dst >= KPoints::kSlots || src == dst ||
+      max_iter < 0)
+    return -1;
+  if (int rc = kp_solver_alloc(c, k)) return rc;
+  const int nb = k->nb, rpk = 4 * nb;
+  const long L2 = 2L * k->npw_max;
+  double* P = k->d_slot[src];
+  double* HP = k->d_slot[dst];
+  const dim3 gr(64, k->nk * nb);
+  // Ritz vectors of the starting bands
+  if (int rc = kp_apply(c, k, P, HP, nb, 0, nb, 0)) return rc;
+  hipLaunchKernelGGL(k_krows, gr, dim3(256), 0, c->stream, P, nb, 0, k->d_Z, rpk, 0, nb, L2);
+  hipLaunchKernelGGL(k_krows, gr, dim3(256), 0, c->stream, HP, nb, 0, k->d_HZ, rpk, 0, nb, L2);
+  std::vector<double> lam((size_t)k->nk * nb), prev;
+  if (int rc = kp_rayleigh_ritz(c, k, nb, P, HP, lam.data())) return rc;
+  int it = 0;
+  for (; it < max_iter; ++it) {
+    hipLaunchKernelGGL(k_krows, gr, dim3(256), 0, c->stream, P, nb, 0, k->d_Z, rpk, 0, nb, L2);
+    hipLaunchKernelGGL(k_krows, gr, dim3(256), 0, c->stream, HP, nb, 0, k->d_HZ, rpk, 0, nb, L2);
+    hipLaunchKernelGGL(k_kekin, dim3(k->nk * nb), dim3(256), 0, c->stream, k->d_Z, rpk, nb,
+                       k->npw_max, k->d_kin, k->d_ekin);
+    hipLaunchKernelGGL(k_kresid, dim3(k->nk * nb), dim3(256), 0, c->stream, k->d_Z, k->d_HZ, rpk,
+                       nb, k->npw_max, k->d_npw, k->d_kin, k->d_lam, k->d_ekin);
+    if (int rc = kp_apply(c, k, k->d_Z, k->d_HZ, rpk, nb, nb, 0)) return rc;
+    prev = lam;
+    if (int rc = kp_rayleigh_ritz(c, k, 2 * nb, P, HP, lam.data())) return rc;
+    double dmax = 0.0;
+    for (size_t i = 0; i < lam.size(); ++i) dmax = std::max(dmax, std::fabs(lam[i] - prev[i]));
+    if (dmax < tol) {
+      ++it;
+      break;
+    }
+  }
+  if (eig) std::memcpy(eig, lam.data(), sizeof(double) * lam.size());
+  if (iters) *iters = it;
+  return 0;
+}
+
+int kp_hpsi(Ctx* c, int src, int dst, int nb, int per_band) {
+  auto* k = static_cast<KPoints*>(c->kp);
+  if (!k || src < 0 || dst < 0 || src >= KPoints::kSlots || dst >= KPoints::kSlots || src == dst ||
+      nb < 1 || nb > k->nb)
+    return -1;
+  return kp_apply(c, k, k->d_slot[src], k->d_slot[dst], k->nb, 0, nb, per_band);
 }
 
 }  // namespace qx

@@ -159,6 +159,7 @@ SIGNATURES = {
     "qx_kslot_get": [_vp, _i, _i, _dp, _i],
     "qx_kset_kb": [_vp, _i, _i, _dp, _dp],
     "qx_khpsi_slot": [_vp, _i, _i, _i, _i],
+    "qx_ksolve_slot": [_vp, _i, _i, _i, _d, _dp, _ip],
 }
 # entry points beyond the header's core set (batched CG, kinetic energies)
 EXTRA = {

@@ -303,6 +303,17 @@ class PlaneWave:
         check(load().qx_khpsi_slot(self.h, int(src), int(dst), nb, int(bool(per_band))),
               "qx_khpsi_slot")
 
+    def ksolve(self, src=0, dst=1, max_iter=100, tol=1e-10):
+        """Lowest bands of every k-point in the fixed potential (block
+        Rayleigh-Ritz from the bands of k-slot src).  Returns (eig (nk, nb),
+        steps)."""
+        nk = len(self._knpw)
+        eig = np.zeros((nk, self._knb))
+        it = np.zeros(1, np.int32)
+        check(load().qx_ksolve_slot(self.h, int(src), int(dst), int(max_iter), float(tol), eig, it),
+              "qx_ksolve_slot")
+        return eig, int(it[0])
+
     # ---- operations on resident slots
     def slot_copy(self, src, dst, nb):
         check(load().qx_slot_copy(self.h, int(src), int(dst), int(nb)), "qx_slot_copy")

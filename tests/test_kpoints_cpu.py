@@ -159,3 +159,24 @@ def test_oracle_hk_hermitian_and_relabel():
     beta2, _ = kpoints.kb_projectors(t, tau, np.zeros(8, int), b, k2, hkl2, a[0, 0] ** 3)
     hx2 = OK.hk_apply(x, hkl2, kin, V, beta2[:, 0::2] + 1j * beta2[:, 1::2], D)
     assert np.abs(hx2 - hx).max() < 1e-11 * np.abs(hx).max()
+
+
+def test_oracle_hk_apply_equals_dense_matrix():
+    """hk_apply (band transforms) equals hk_dense (the convolution theorem,
+    no transforms) on random bands at a general k-point."""
+    a = np.eye(3) * 10.26
+    b = _bvec(a)
+    nd = tuple(int(v) for v in O.get_mesh(a, 80.0))
+    n1, n2, n3 = nd
+    rng = np.random.default_rng(4)
+    V = -0.4 + 0.1 * rng.standard_normal((n3, n2, n1))
+    k = np.array([-0.25, 0.5, 0.125])
+    hkl, kin = pw.ksphere(b, k, 20.0, nd)
+    import conftest
+    t, _, _ = ncpp.silicon_cell_inputs(conftest.Sphere(1), NCPP, "si512")
+    beta, D = kpoints.kb_projectors(t, diamond_si(1).tau, np.zeros(8, int), b, k, hkl, a[0, 0] ** 3)
+    bc = beta[:, 0::2] + 1j * beta[:, 1::2]
+    x = rng.standard_normal((3, len(kin))) + 1j * rng.standard_normal((3, len(kin)))
+    got = OK.hk_apply(x, hkl, kin, V, bc, D)
+    ref = x @ OK.hk_dense(hkl, kin, V, bc, D).T
+    assert np.abs(got - ref).max() < 1e-11 * np.abs(ref).max()
