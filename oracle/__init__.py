@@ -37,6 +37,9 @@ PINNING = {
     "untrycore": "reference untrycore (oracle/_ref)",
     "softchg": "integral = sum occ; numpy FFT of psi(r)^2",
     "prpcg": "reference prpcg (oracle/_ref)",
+    "clmrcp": "bit-exact vs reference clmrcp (oracle/_ref); lattice closed form",
+    "kpoint.hk_apply/hk_dense": "k = 0 on a real state equals the C Gamma H (pinned to _ref); "
+                                "hermiticity; k -> k + b1 relabelling; apply = dense matrix",
     "eigen_bkl": "per-iteration trace against numpy: the 2x2 Rayleigh-Ritz eigenvalue on "
                  "span{x, h} of a dense H, the direction h = P g + beta h_prev up to the "
                  "projected span, beta = zansap/betk, h orthogonal to the updated bands, exits at "

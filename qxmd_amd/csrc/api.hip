@@ -263,7 +263,10 @@ int qx_create(qx_ctx** out, int n1, int n2, int n3, int nplw, const int* nga, co
   // pencil intermediates: a batch of band pairs, FP64 complex
   const size_t per_bp = 16 * (size_t)n3 * c->pc.ncol;
   const int nbp = c->nbmax / 2;
-  size_t cap_bytes = (size_t)512 << 20;  // pencil intermediates per buffer
+  // pencil intermediates per buffer: 4 GiB holds every band pair of the
+  // 512-atom cell (1280 bands) in one batch, so the fused mixed band-apply
+  // (one batch) covers it; two buffers of that size are 3% of the HBM
+  size_t cap_bytes = (size_t)4096 << 20;
   if (const char* e = std::getenv("QX_PENCIL_MB")) cap_bytes = (size_t)std::max(1, std::atoi(e)) << 20;
   c->batch_bp = std::max(1, std::min(nbp, (int)std::max<size_t>(1, cap_bytes / per_bp)));
   c->t_bytes = per_bp * c->batch_bp;
