@@ -128,8 +128,9 @@ int hpsi_batched(Ctx* c, const double* x, double* hx, int nb, int prec) {
   return kb_apply(c, x, hx, nb, 1, nullptr, prec);
 }
 
-// Gram-Schmidt by Cholesky of the packed overlap (schmidt2)
-int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host, int prec) {
+// Gram-Schmidt by Cholesky of the packed overlap (schmidt2).  split_ready:
+// the mixed form's split planes of psi (band split 2) are current.
+int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host, int prec, bool split_ready) {
   const int n = c->nplwex;
   double* S = c->d_small;                         // nb x nb overlap, then L
   double* Linv = c->d_small + (size_t)(nb + 1) * nb;  // row-major L^-1
@@ -139,7 +140,15 @@ int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host, int prec) {
   // the column-major S(i,j).
   // the overlap product in the given precision (split bf16x3 in the mixed
   // form); the Cholesky and the inverse stay FP64
-  int rc = gemm_tn_p(c, nb, nb, n, 2.0, psi, n, psi, n, 0.0, S, nb, 1, prec);
+  const bool split = band_split_wanted(c, nb, prec);
+  BandSplit ps;
+  int rc = 0;
+  if (split) {
+    rc = split_ready ? band_split_buf(c, 2, &ps) : band_split(c, 2, psi, nb, &ps);
+    if (!rc) rc = band_tn(c, nb, nb, 2.0, ps, psi, ps, psi, S, nb);
+  } else {
+    rc = gemm_tn_p(c, nb, nb, n, 2.0, psi, n, psi, n, 0.0, S, nb, 1, prec);
+  }
   if (rc) return rc;
   if ((rc = dense_chol_inv(c, S, nb, Linv, ok, flag))) return rc;
   int okh[2] = {0, 0};
@@ -149,6 +158,7 @@ int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host, int prec) {
   *ok_host = okh[0];
   if (!okh[0]) return 0;
   // psi'_ib = sum_jb Linv(ib,jb) psi_jb: gemm_nn's B[n*ldb + k] = Linv(n,k)
+  if (split) return band_nn(c, nb, 1.0, ps, Linv, nb, 0.0, psi);  // reads the split planes
   double* out = c->d_slot[5];
   if ((rc = gemm_nn_p(c, n, nb, nb, 1.0, psi, n, Linv, nb, 0.0, out, n, prec))) return rc;
   QX_CHECK(hipMemcpyAsync(psi, out, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice,
@@ -167,18 +177,33 @@ static int rotate_dev(Ctx* c, double* psi, double* hpsi, int nb, double* eig_hos
   double* work = Z + (size_t)nb * nb;
   double* w = c->d_scal + 40 * (size_t)c->nbmax;
   int* info = reinterpret_cast<int*>(c->d_scal + 63 * (size_t)c->nbmax);
-  int rc = gemm_tn_p(c, nb, nb, n, 2.0, psi, n, hpsi, n, 0.0, C, nb, 1, prec);
+  // the mixed form's products on split band slots (split 2: psi, 3: hpsi)
+  const bool split = band_split_wanted(c, nb, prec);
+  BandSplit ps, hs;
+  int rc = 0;
+  if (split) {
+    rc = band_split(c, 2, psi, nb, &ps);
+    if (!rc) rc = band_split(c, 3, hpsi, nb, &hs);
+    if (!rc) rc = band_tn(c, nb, nb, 2.0, ps, psi, hs, hpsi, C, nb);
+  } else {
+    rc = gemm_tn_p(c, nb, nb, n, 2.0, psi, n, hpsi, n, 0.0, C, nb, 1, prec);
+  }
   if (rc) return rc;
   if ((rc = dense_symmetrize(c, C, nb, H, nullptr))) return rc;
   if ((rc = dense_eigh(c, H, nb, Z, w, work, info))) return rc;
   // psi' = psi U: B(n=b, k=j) = U(j,b) = Z[j + nb*b] -> ldb = nb with B[n*ldb+k]
-  double* out = c->d_slot[5];
-  if ((rc = gemm_nn_p(c, n, nb, nb, 1.0, psi, n, Z, nb, 0.0, out, n, prec))) return rc;
-  QX_CHECK(hipMemcpyAsync(psi, out, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice,
-                          c->stream));
-  if ((rc = gemm_nn_p(c, n, nb, nb, 1.0, hpsi, n, Z, nb, 0.0, out, n, prec))) return rc;
-  QX_CHECK(hipMemcpyAsync(hpsi, out, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice,
-                          c->stream));
+  if (split) {  // the products read the split planes: written in place
+    if ((rc = band_nn(c, nb, 1.0, ps, Z, nb, 0.0, psi))) return rc;
+    if ((rc = band_nn(c, nb, 1.0, hs, Z, nb, 0.0, hpsi))) return rc;
+  } else {
+    double* out = c->d_slot[5];
+    if ((rc = gemm_nn_p(c, n, nb, nb, 1.0, psi, n, Z, nb, 0.0, out, n, prec))) return rc;
+    QX_CHECK(hipMemcpyAsync(psi, out, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice,
+                            c->stream));
+    if ((rc = gemm_nn_p(c, n, nb, nb, 1.0, hpsi, n, Z, nb, 0.0, out, n, prec))) return rc;
+    QX_CHECK(hipMemcpyAsync(hpsi, out, sizeof(double) * (size_t)n * nb, hipMemcpyDeviceToDevice,
+                            c->stream));
+  }
   if (eig_host) {
     QX_CHECK(hipMemcpyAsync(eig_host, w, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
     QX_CHECK(hipStreamSynchronize(c->stream));
@@ -640,7 +665,8 @@ int qx_scf_band_step_ex(qx_ctx* h, int psi, int hpsi, int nb, const double* occ,
                           bz.data(), nr.data(), prec);
   if (rc) return rc;
   QX_CHECK(hipEventRecord(c->pev[3], c->stream));
-  if ((rc = schmidt_dev(c, P, nb, lcholesky, prec))) return rc;
+  // the batched CG leaves the split planes of psi current for the overlap
+  if ((rc = schmidt_dev(c, P, nb, lcholesky, prec, batched != 0))) return rc;
   QX_CHECK(hipEventRecord(c->pev[4], c->stream));
   if ((rc = qx_density_slot(h, psi, nb, occ, rvol, rho, prec))) return rc;
   QX_CHECK(hipEventRecord(c->pev[5], c->stream));

@@ -189,6 +189,69 @@ __global__ void __launch_bounds__(kDotThreads) k_cg_update(double* __restrict__ 
   dot_store<1>(s, part, b, ch, nch);
 }
 
+// one value split into bf16 hi and lo = bf16(a - hi) (as split_bf16x2)
+__device__ inline void split_bf16(float a, unsigned short& hi, unsigned short& lo) {
+  const __bf16 h = (__bf16)a;
+  const __bf16 l = (__bf16)(a - (float)h);
+  hi = __builtin_bit_cast(unsigned short, h);
+  lo = __builtin_bit_cast(unsigned short, l);
+}
+
+// k_cg_update of the mixed form's split products: the updated x of the
+// active bands is also written to its bf16 hi | lo planes (xs: band stride
+// ldk, lo plane at xs + slo), which the next projection and the overlap read
+__global__ void __launch_bounds__(kDotThreads) k_cg_update_split(
+    double* __restrict__ x, double* __restrict__ hx, double* __restrict__ g,
+    const double* __restrict__ h, const double* __restrict__ hh, const double* __restrict__ a,
+    const double* __restrict__ t, const double* __restrict__ eig, int n,
+    const int* __restrict__ mask, int nch, double* __restrict__ part,
+    unsigned short* __restrict__ xs, int ldk, long slo) {
+  const int b = blockIdx.y, ch = blockIdx.x;
+  const bool act = !mask || mask[b];
+  const double av = a[b], tv = t[b], ev = eig[b];
+  const size_t o = (size_t)b * n, os = (size_t)b * ldk;
+  const int i0 = ch * kDotChunk, i1 = min(n, i0 + kDotChunk);
+  double s[1] = {0.0};
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    double gv;
+    if (act) {
+      const double xv = av * x[o + i] + tv * h[o + i];
+      const double hv = av * hx[o + i] + tv * hh[o + i];
+      x[o + i] = xv;
+      hx[o + i] = hv;
+      gv = ev * xv + -1.0 * hv;
+      g[o + i] = gv;
+      unsigned short sh, sl;
+      split_bf16((float)xv, sh, sl);
+      xs[os + i] = sh;
+      xs[slo + os + i] = sl;
+    } else {
+      gv = g[o + i];
+    }
+    s[0] += gv * gv;
+  }
+  dot_store<1>(s, part, b, ch, nch);
+}
+
+// CG direction of the active bands, h = g (beta null) or g + beta h, and its
+// bf16 hi | lo planes (hs, as xs above)
+__global__ void k_cg_dir_split(double* __restrict__ h, const double* __restrict__ g,
+                               const double* __restrict__ beta, int n, int nb,
+                               const int* __restrict__ mask, unsigned short* __restrict__ hs,
+                               int ldk, long slo) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)n * nb) return;
+  const int b = (int)(i / n), k = (int)(i % n);
+  if (mask && !mask[b]) return;
+  const size_t o = (size_t)b * n + k, os = (size_t)b * ldk + k;
+  const double v = beta ? g[o] + beta[b] * h[o] : g[o];
+  h[o] = v;
+  unsigned short sh, sl;
+  split_bf16((float)v, sh, sl);
+  hs[os] = sh;
+  hs[slo + os] = sl;
+}
+
 // out_q[b] = 2 sum_q - x_b[0] y_qb[0] for the NQ sums of the fused passes
 __global__ void k_dotq_final(const double* __restrict__ part, int nch, int nb, int nq,
                              const double* __restrict__ x, const double* y0,
@@ -833,6 +896,16 @@ int bkl_batched(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib_ho
   const int nch = (n + kDotChunk - 1) / kDotChunk;
   double* part = c->d_scal + 64 * (size_t)c->nbmax;  // 3 x nch x nb partial sums
   const dim3 dgrid(nch, nb);
+  // the mixed form's products on split operands: psi split here, kept current
+  // by the update pass (the overlap of the Gram-Schmidt step that follows
+  // reads it too), h split by the direction pass
+  const bool split = band_split_wanted(c, nb, prec);
+  BandSplit xs, hs;
+  if (split) {
+    int rc = band_split(c, 2, psi, nb, &xs);
+    if (!rc) rc = band_split_buf(c, 3, &hs);
+    if (rc) return rc;
+  }
   // one iteration; the first takes h = P g (beta = 0 and h may hold anything)
   auto iteration = [&](bool first) -> int {
     // precondition, CG direction
@@ -841,13 +914,22 @@ int bkl_batched(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib_ho
     hipLaunchKernelGGL(k_dotq_final, dim3(gb), dim3(256), 0, c->stream, part, nch, nb, 1, gnk, gnk,
                        nullptr, nullptr, n, s.zansap, nullptr, nullptr);
     hipLaunchKernelGGL(k_bkl_beta, dim3(gb), dim3(256), 0, c->stream, s, nb);
-    if (first) axpby(c, hnk, gnk, nullptr, nullptr, nullptr, 1.0, 0.0, nb, s.active);
-    else axpby(c, hnk, gnk, hnk, nullptr, s.beta, 1.0, 0.0, nb, s.active);
-    // project out all current bands: h <- h - Psi (Psi^T W h)
-    int rc = gemm_tn_p(c, nb, nb, n, 2.0, psi, n, hnk, n, 0.0, proj, nb, 1, prec);
-    if (rc) return rc;
-    rc = gemm_nn_p(c, n, nb, nb, -1.0, psi, n, proj, nb, 1.0, hnk, n, prec);
-    if (rc) return rc;
+    int rc = 0;
+    if (split) {
+      hipLaunchKernelGGL(k_cg_dir_split, dim3(cdivu((long)n * nb, 256)), dim3(256), 0, c->stream,
+                         hnk, gnk, first ? nullptr : s.beta, n, nb, s.active, hs.p, hs.ld, hs.lo);
+      // project out all current bands: h <- h - Psi (Psi^T W h)
+      if ((rc = band_tn(c, nb, nb, 2.0, xs, psi, hs, hnk, proj, nb))) return rc;
+      if ((rc = band_nn(c, nb, -1.0, xs, proj, nb, 1.0, hnk))) return rc;
+    } else {
+      if (first) axpby(c, hnk, gnk, nullptr, nullptr, nullptr, 1.0, 0.0, nb, s.active);
+      else axpby(c, hnk, gnk, hnk, nullptr, s.beta, 1.0, 0.0, nb, s.active);
+      // project out all current bands: h <- h - Psi (Psi^T W h)
+      rc = gemm_tn_p(c, nb, nb, n, 2.0, psi, n, hnk, n, 0.0, proj, nb, 1, prec);
+      if (rc) return rc;
+      rc = gemm_nn_p(c, n, nb, nb, -1.0, psi, n, proj, nb, 1.0, hnk, n, prec);
+      if (rc) return rc;
+    }
     rc = hpsi_batched(c, hnk, hh, nb, prec);
     if (rc) return rc;
     hipLaunchKernelGGL(k_dot3<kDotChunk>, dgrid, dim3(kDotThreads), 0, c->stream, hnk, hpsi, hh,
@@ -857,8 +939,13 @@ int bkl_batched(Ctx* c, double* psi, double* hpsi, int nb, const double* ekib_ho
     hipLaunchKernelGGL(k_bkl_step, dim3(gb), dim3(256), 0, c->stream, s, nb);
     // x = a (x + dt h); Hx = a (Hx + dt Hh); g = eig x - Hx
     hipLaunchKernelGGL(k_mul, dim3(gb), dim3(256), 0, c->stream, tmp, s.alpha, s.dt, nb);
-    hipLaunchKernelGGL(k_cg_update, dgrid, dim3(kDotThreads), 0, c->stream, psi, hpsi, gnk, hnk,
-                       hh, s.alpha, tmp, s.eig, n, s.active, nch, part);
+    if (split)
+      hipLaunchKernelGGL(k_cg_update_split, dgrid, dim3(kDotThreads), 0, c->stream, psi, hpsi,
+                         gnk, hnk, hh, s.alpha, tmp, s.eig, n, s.active, nch, part, xs.p, xs.ld,
+                         xs.lo);
+    else
+      hipLaunchKernelGGL(k_cg_update, dgrid, dim3(kDotThreads), 0, c->stream, psi, hpsi, gnk,
+                         hnk, hh, s.alpha, tmp, s.eig, n, s.active, nch, part);
     hipLaunchKernelGGL(k_dotq_final, dim3(gb), dim3(256), 0, c->stream, part, nch, nb, 1, gnk,
                        gnk, nullptr, nullptr, n, tmp2, nullptr, nullptr);
     hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, c->stream, s.zansa, tmp2, nb);

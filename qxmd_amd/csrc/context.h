@@ -8,6 +8,10 @@ namespace qx {
 
 enum Prec { PREC_FP64 = 0, PREC_FP32 = 1 };
 constexpr int kBbChunk = 256;  // vector elements per chunk of the single-band CG passes
+// bf16 split work arrays of the GEMM layer: 0 psi of the fused mixed
+// band-apply, 1 its KB projections, 2 and 3 band slots of the mixed band step,
+// 4 the nb x nb factor of a band product
+constexpr int kU16Scratch = 5;
 
 // Pencil layout of the half sphere.  Columns are the (h,k) lines along z that
 // the full sphere (G and -G) touches; they come in mirror pairs (c, -c), with
@@ -257,7 +261,30 @@ int mg_ps3_tn(Ctx* c, int M, int N, int K, const unsigned short* A, int lda, lon
               const unsigned short* B, int ldb, long blo, float* C, int ldc);
 int mg_ps3_nn(Ctx* c, int M, int N, int K, const unsigned short* A, int lda, long alo,
               const unsigned short* B, int ldb, long blo, float* C, int ldc);
+// FP64-output forms (alpha, beta; no packed-sum correction)
+int mg_ps3_tn_d(Ctx* c, int M, int N, int K, double alpha, const unsigned short* A, int lda,
+                long alo, const unsigned short* B, int ldb, long blo, double beta, double* C,
+                int ldc);
+int mg_ps3_nn_d(Ctx* c, int M, int N, int K, double alpha, const unsigned short* A, int lda,
+                long alo, const unsigned short* B, int ldb, long blo, double beta, double* C,
+                int ldc);
+int mg_split_rows(Ctx* c, const double* src, int rows, int cols, int lds, unsigned short* dst,
+                  int ld, long lo);
 void mg_forget_kb(Ctx* c);
+// a band slot split into bf16 hi | lo planes (band stride ld, lo = offset of
+// the lo plane) for the mixed band step's products
+struct BandSplit {
+  unsigned short* p = nullptr;
+  int ld = 0;
+  long lo = 0;
+};
+bool band_split_wanted(Ctx* c, int nb, int prec);
+int band_split_buf(Ctx* c, int which, BandSplit* s);
+int band_split(Ctx* c, int which, const double* src, int nb, BandSplit* s);
+int band_tn(Ctx* c, int M, int N, double alpha, const BandSplit& A, const double* a,
+            const BandSplit& B, const double* b, double* C, int ldc);
+int band_nn(Ctx* c, int nb, double alpha, const BandSplit& A, const double* Z, int ldz,
+            double beta, double* C);
 int mg_tuned_shapes(Ctx* c);
 bool use_big(int M, int N);
 // KB term of the mixed band-apply from the FP32 copy psi32 of psi: kb32 =
@@ -307,7 +334,7 @@ int kb_forces(Ctx* c, const double* psi, int nb, const double* occ_host, const d
 int lopp_set(Ctx* c, int ntype, int nplw5, const double* svco, const double* gx,
              const double* gy, const double* gz);
 void lopp_destroy(Ctx* c);
-int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host, int prec);
+int schmidt_dev(Ctx* c, double* psi, int nb, int* ok_host, int prec, bool split_ready = false);
 // wave-function extrapolation between MD steps (aspc.hip)
 void aspc_destroy(Ctx* c);
 int aspc_init(Ctx* c, int nprvmx, int ihest);

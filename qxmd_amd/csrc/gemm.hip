@@ -834,4 +834,60 @@ int kb_mixed_f32(Ctx* c, const double* psi, const float* psi32, const unsigned s
                                                 n);
 }
 
+// ---- the mixed band step's products on split band slots ---------------------
+// A band slot (nb bands of nplwex FP64) is split into bf16 hi | lo planes:
+// band stride round_up(nplwex, 64), round_up(nbmax, 64) bands, zeros in
+// every pad, each plane followed by a margin of 128 elements (the NN edge
+// tiles read into it).  The overlap, subspace-H, projection and rotation
+// products then run the split bf16x3 MFMA kernel on operands it reads at 2 x 2
+// bytes per element, split once per producer instead of once per tile.
+bool band_split_wanted(Ctx* c, int nb, int prec) {
+  static const bool off = std::getenv("QX_BAND_SPLIT_OFF") != nullptr;
+  return !off && prec == PREC_FP32 && gemm_path() == GEMM_OWN && !lt_enabled() &&
+         use_big(nb, nb);
+}
+
+int band_split_buf(Ctx* c, int which, BandSplit* s) {
+  const int ld = (int)(cdiv(c->nplwex, 64) * 64);
+  const long rows = (long)cdiv(c->nbmax, 64) * 64;
+  s->ld = ld;
+  s->lo = rows * ld + 128;
+  s->p = mg_u16_scratch(c, which, 2 * (size_t)s->lo);
+  return s->p ? 0 : -3;
+}
+
+int band_split(Ctx* c, int which, const double* src, int nb, BandSplit* s) {
+  int rc = band_split_buf(c, which, s);
+  if (rc) return rc;
+  return mg_split_rows(c, src, nb, c->nplwex, c->nplwex, s->p, s->ld, s->lo);
+}
+
+// C(m, n) = alpha (<a_m|b_n>_packed) for m < M, n < N: the split product
+// over the whole padded band, then the G = 0 term of the packed sum taken out
+// with the FP64 values of the slots (corr = 1 of gemm_tn)
+int band_tn(Ctx* c, int M, int N, double alpha, const BandSplit& A, const double* a,
+            const BandSplit& B, const double* b, double* C, int ldc) {
+  int rc = mg_ps3_tn_d(c, M, N, A.ld, alpha, A.p, A.ld, A.lo, B.p, B.ld, B.lo, 0.0, C, ldc);
+  if (rc) return rc;
+  const int n = c->nplwex;
+  hipLaunchKernelGGL(k_corr, dim3(cdiv((long)M * N, 256)), dim3(256), 0, c->stream, C, M, N, ldc,
+                     a, n, b, n, alpha, 0);
+  return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+// C(:, j) = alpha sum_k A_k Z(k, j) + beta C(:, j) for j < nb (bands of
+// nplwex).  The product reads A's split planes only, so C may be the slot A
+// was split from.
+int band_nn(Ctx* c, int nb, double alpha, const BandSplit& A, const double* Z, int ldz,
+            double beta, double* C) {
+  const int kp = (int)(cdiv(nb, 64) * 64);
+  const long zlo = (long)nb * kp;
+  unsigned short* Zs = mg_u16_scratch(c, 4, 2 * (size_t)zlo);
+  if (!Zs) return -3;
+  int rc = mg_split_rows(c, Z, nb, nb, ldz, Zs, kp, zlo);
+  if (rc) return rc;
+  return mg_ps3_nn_d(c, c->nplwex, nb, kp, alpha, A.p, A.ld, A.lo, Zs, kp, zlo, beta, C,
+                     c->nplwex);
+}
+
 }  // namespace qx

@@ -630,8 +630,8 @@ struct MgState {
   u16* Ts = nullptr;
   const double* Ts_src = nullptr;
   size_t Ts_len = 0;
-  u16* u16s[2] = {};  // zero-initialised split work arrays
-  size_t u16_len[2] = {};
+  u16* u16s[kU16Scratch] = {};  // zero-initialised split work arrays
+  size_t u16_len[kU16Scratch] = {};
 };
 
 __global__ void k_split_rows(const double* __restrict__ src, int rows, int cols, int lds,
@@ -848,10 +848,11 @@ const unsigned short* mg_kb_split(Ctx* c, int* ld, int* rows_pad, long* lo) {
   return s->Ts;
 }
 
-// zero-initialised u16 work array i (0, 1) of at least n elements; contents
-// persist between calls (the producers write only inside their pads)
+// zero-initialised u16 work array i (0 .. kU16Scratch - 1) of at least n
+// elements; contents persist between calls (the producers write only inside
+// their pads)
 unsigned short* mg_u16_scratch(Ctx* c, int i, size_t n) {
-  if (i < 0 || i >= 2) return nullptr;
+  if (i < 0 || i >= kU16Scratch) return nullptr;
   MgState* s = state(c);
   if (s->u16_len[i] < n) {
     buffers_changed(c);
@@ -1007,6 +1008,33 @@ int mg_ps3_nn(Ctx* c, int M, int N, int K, const unsigned short* A, int lda, lon
   if (K % KP || (lda | ldb) % KP || alo % KP || blo % KP) return -3;
   return run_product<float>(c, 7, KP, 2, k_mg_ps3<false, float>, M, N, K, 1.0, A, lda, B, ldb,
                             0.0, C, ldc, 0, alo, blo);
+}
+
+// the same on FP64 C: C = alpha A^T B + beta C (TN), C = alpha A B + beta C (NN)
+int mg_ps3_tn_d(Ctx* c, int M, int N, int K, double alpha, const unsigned short* A, int lda,
+                long alo, const unsigned short* B, int ldb, long blo, double beta, double* C,
+                int ldc) {
+  if (K % KP || (lda | ldb) % KP || alo % KP || blo % KP) return -3;
+  return run_product<float>(c, 8, KP, 2, k_mg_ps3<true, double>, M, N, K, alpha, A, lda, B, ldb,
+                            beta, C, ldc, 0, alo, blo);
+}
+
+int mg_ps3_nn_d(Ctx* c, int M, int N, int K, double alpha, const unsigned short* A, int lda,
+                long alo, const unsigned short* B, int ldb, long blo, double beta, double* C,
+                int ldc) {
+  if (K % KP || (lda | ldb) % KP || alo % KP || blo % KP) return -3;
+  return run_product<float>(c, 9, KP, 2, k_mg_ps3<false, double>, M, N, K, alpha, A, lda, B, ldb,
+                            beta, C, ldc, 0, alo, blo);
+}
+
+// rows x cols FP64 (row stride lds) -> bf16 hi at dst, lo at dst + lo: row
+// stride ld, zeros in every element of [0, lo) outside rows x cols
+int mg_split_rows(Ctx* c, const double* src, int rows, int cols, int lds, unsigned short* dst,
+                  int ld, long lo) {
+  if (ld % 2 || lo % 2 || cols > ld || (long)rows * ld > lo) return -3;
+  hipLaunchKernelGGL(k_split_rows, dim3(cdivu(lo / 2, 256)), dim3(256), 0, c->stream, src, rows,
+                     cols, lds, dst, ld, 0, lo);
+  return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
 template int mg_bf3_tn<double, double>(Ctx*, int, int, int, double, const double*, int,
