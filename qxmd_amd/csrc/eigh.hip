@@ -633,6 +633,159 @@ static int backtrans(Ctx* c, const double* a, int n, double* z) {
   return 0;
 }
 
+// ---- 4b. blocked back-transformation on MFMA ----------------------------------
+// The reflectors in blocks of 16: Q_b = H_k0 ... H_k0+15 = I - V T V^T with T
+// upper triangular, T(j, j) = 2 and T(0:j, j) = -2 T(0:j, 0:j) V(:, 0:j)^T v_j
+// (the forward recurrence of LAPACK's dlarft for unit reflectors, tau = 2).
+// Z = Q_0 Q_1 ... Z_T is formed from the last block to the first as
+// Z <- Z - V (T (V^T Z)), three chains of v_mfma_f64_16x16x4f64.
+constexpr int kWyB = 16;
+
+// Vt[b][i][r] = v_{16 b + r}(i): zero outside the reflector (rows <= k, rows
+// >= n, blocks past the last reflector n - 3)
+__global__ void k_wy_pack(const double* __restrict__ a, int n, int npad, int nblk,
+                          double* __restrict__ Vt) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)nblk * npad * kWyB) return;
+  const int r = (int)(idx % kWyB);
+  const long q = idx / kWyB;
+  const int i = (int)(q % npad), b = (int)(q / npad);
+  const int k = b * kWyB + r;
+  Vt[idx] = (k <= n - 3 && i > k && i < n) ? a[(size_t)k * n + i] : 0.0;
+}
+
+// T of block b (one workgroup each): Gram matrix G = V^T V, then row i of T
+// by thread i (row i of the recurrence needs row i only)
+__global__ void __launch_bounds__(256) k_wy_t(const double* __restrict__ Vt, int npad,
+                                              double* __restrict__ Tb) {
+  __shared__ double G[kWyB][kWyB + 1], T[kWyB][kWyB + 1];
+  const int b = blockIdx.x, t = threadIdx.x, r = t / kWyB, s = t % kWyB;
+  const double* V = Vt + (size_t)b * npad * kWyB;
+  double acc = 0.0;
+  for (int i = 0; i < npad; ++i) acc += V[(size_t)i * kWyB + r] * V[(size_t)i * kWyB + s];
+  G[r][s] = acc;
+  T[r][s] = 0.0;
+  __syncthreads();
+  if (t < kWyB) {
+    T[t][t] = 2.0;
+    for (int j = t + 1; j < kWyB; ++j) {
+      double x = 0.0;
+      for (int l = t; l < j; ++l) x += T[t][l] * G[l][j];
+      T[t][j] = -2.0 * x;
+    }
+  }
+  __syncthreads();
+  Tb[(size_t)b * kWyB * kWyB + t] = T[r][s];
+}
+
+// One workgroup per 16 columns of Z, held in registers for the whole pass as
+// 16 x 16 tiles in the MFMA accumulator layout (D(i, j) in lane j + 16 (i % 4),
+// register i / 4).  That layout is also the B-operand layout of four k steps
+// (register s = rows 4 s .. 4 s + 3), so V^T Z reads Z from registers, and
+// T W and V W' take their B operand straight from the previous product.
+// Tile t of the column panel belongs to wavefront t % 4; the four partial
+// V^T Z are summed through LDS in wavefront order.  TPW: tiles per wavefront.
+template <int TPW>
+__global__ void __launch_bounds__(256) k_backtrans_wy(const double* __restrict__ Vt,
+                                                      const double* __restrict__ Tb, int n,
+                                                      int npad, int nblk, double* __restrict__ z) {
+  __shared__ double red[4][4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, cl = lane & 15;
+  const int col = blockIdx.x * kWyB + cl;
+  const int ntile = npad / kWyB;
+  dv4 zt[TPW];
+#pragma unroll
+  for (int q = 0; q < TPW; ++q) {
+    const int t = wave + 4 * q;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = kWyB * t + g + 4 * v;
+      zt[q][v] = (t < ntile && row < n && col < n) ? z[(size_t)col * n + row] : 0.0;
+    }
+  }
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int k0 = b * kWyB;
+    const double* V = Vt + (size_t)b * npad * kWyB;
+    // W = V^T Z over this wavefront's tiles (tiles wholly at rows <= k0 hold
+    // no reflector entry)
+    dv4 w = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) {
+      const int t = wave + 4 * q;
+      if (t >= ntile || kWyB * t + kWyB - 1 <= k0) continue;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const double av = V[(size_t)(kWyB * t + 4 * s + g) * kWyB + cl];
+        w = __builtin_amdgcn_mfma_f64_16x16x4f64(av, zt[q][s], w, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red[wave][v][lane] = w[v];
+    __syncthreads();
+    dv4 W;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      W[v] = red[0][v][lane] + red[1][v][lane] + red[2][v][lane] + red[3][v][lane];
+    __syncthreads();
+    // W' = T W
+    const double* T = Tb + (size_t)b * kWyB * kWyB;
+    dv4 wp = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      wp = __builtin_amdgcn_mfma_f64_16x16x4f64(T[cl * kWyB + 4 * s + g], W[s], wp, 0, 0, 0);
+    // Z <- Z - V W'
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) {
+      const int t = wave + 4 * q;
+      if (t >= ntile || kWyB * t + kWyB - 1 <= k0) continue;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const double av = -V[(size_t)(kWyB * t + cl) * kWyB + 4 * s + g];
+        zt[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, wp[s], zt[q], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < TPW; ++q) {
+    const int t = wave + 4 * q;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = kWyB * t + g + 4 * v;
+      if (t < ntile && row < n && col < n) z[(size_t)col * n + row] = zt[q][v];
+    }
+  }
+}
+
+// the blocked back-transformation for 64 <= n <= 2048 (ws: n^2 + 32 n + 512
+// doubles at least); -3 outside that range
+static int backtrans_wy(Ctx* c, const double* a, int n, double* z, double* ws) {
+  if (n < 64 || n > 2048 || std::getenv("QX_BACKTRANS_LDS")) return -3;
+  const int npad = (n + kWyB - 1) / kWyB * kWyB;
+  const int nblk = (n - 2 + kWyB - 1) / kWyB;
+  double* Vt = ws;
+  double* Tb = ws + (size_t)nblk * npad * kWyB;
+  const long nv = (long)nblk * npad * kWyB;
+  hipStream_t st = c->stream;
+  hipLaunchKernelGGL(k_wy_pack, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, a, n, npad,
+                     nblk, Vt);
+  hipLaunchKernelGGL(k_wy_t, dim3(nblk), dim3(256), 0, st, Vt, npad, Tb);
+  const int tpw = (npad / kWyB + 3) / 4;
+  const dim3 grid((n + kWyB - 1) / kWyB);
+#define QX_WY(T_)                                                                               \
+  if (tpw <= T_) {                                                                              \
+    hipLaunchKernelGGL(k_backtrans_wy<T_>, grid, dim3(256), 0, st, Vt, Tb, n, npad, nblk, z); \
+    return hipGetLastError() == hipSuccess ? 0 : -4;                                            \
+  }
+  QX_WY(3)
+  QX_WY(5)
+  QX_WY(9)
+  QX_WY(13)
+  QX_WY(20)
+  QX_WY(32)
+#undef QX_WY
+  return -3;
+}
+
 __global__ void k_sync_take(int* sync, int* out) {
   out[0] = sync[1] | sync[2];
   sync[2] = 0;
@@ -735,9 +888,10 @@ int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int
     int rc = chol_qr(c, z, n, gram, linv, X, ok + 8 * pass, pass == 0);
     if (rc) return rc;
   }
-  // 4. back-transformation
+  // 4. back-transformation (the LU scratch is free by now)
   {
-    int rc = n <= 2000 ? backtrans<8>(c, a, n, z) : backtrans<4>(c, a, n, z);
+    int rc = backtrans_wy(c, a, n, z, lu);
+    if (rc == -3) rc = n <= 2000 ? backtrans<8>(c, a, n, z) : backtrans<4>(c, a, n, z);
     if (rc) return rc;
   }
   hipLaunchKernelGGL(k_sync_take, dim3(1), dim3(1), 0, st, c->d_sync, sflag);

@@ -266,6 +266,27 @@ def test_rseigqr(si8, case, path, monkeypatch):
     assert np.abs(a @ z - z * w).max() < 1e-10 * np.abs(wr).max()
 
 
+@pytest.mark.parametrize("backtrans", ["wy", "lds"])
+@pytest.mark.parametrize("n", [64, 65, 540, 1280])
+def test_rseigqr_sizes(si8, n, backtrans, monkeypatch):
+    """The eigensolver at the band counts of the silicon cells and around the
+    16-row blocks of the back-transformation: the blocked form on MFMA (the
+    column panel in registers, one tile-count variant per size range) and the
+    reflector-by-reflector form, each against numpy's eigh by residual and
+    orthonormality."""
+    if backtrans == "lds":
+        monkeypatch.setenv("QX_BACKTRANS_LDS", "1")
+    gs, ctx, H, v = _setup(si8, n, kb=False)
+    rng = np.random.default_rng(n + 7)
+    x = rng.standard_normal((n, n))
+    a = x + x.T
+    w, z = ctx.rseigqr(a)
+    wr = np.linalg.eigvalsh(a)
+    np.testing.assert_allclose(w, wr, rtol=0, atol=1e-11 * np.abs(wr).max())
+    assert np.abs(z.T @ z - np.eye(n)).max() < 1e-10
+    assert np.abs(a @ z - z * w).max() < 1e-10 * np.abs(wr).max()
+
+
 @pytest.mark.parametrize("n", [3, 4, 5, 17, 64, 65, 300, 540, 640, 641, 1280])
 def test_rseigqr_sizes(si8, n):
     """The cooperative eigensolver over the band counts the workloads use
