@@ -301,6 +301,25 @@ __device__ inline int sturm_count_ms(const double* d, const double* e, int n, do
   return cnt;
 }
 
+// the same count with d and the squared couplings e2[i] = e[i]^2 in LDS (the
+// same arithmetic: e[i-1] * e[i-1] rounds to e2[i-1])
+__device__ inline int sturm_count_lds(const double* d, const double* e2, int n, double sigma,
+                                      double pivmin) {
+  int cnt = 0;
+  double q = d[0] - sigma;
+  if (fabs(q) < pivmin) q = -pivmin;
+  cnt += q < 0.0;
+#pragma unroll 8
+  for (int i = 1; i < n; ++i) {
+    q = d[i] - sigma - e2[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += q < 0.0;
+  }
+  return cnt;
+}
+
+constexpr int kBisectLds = 2048;  // d and e^2 staged in LDS up to this order
+
 // one wavefront per eigenvalue j (0-based, ascending): the bracket [lo, hi]
 // holds lambda_j; lane l tests lo + (l+1)(hi-lo)/65, the first point with more
 // than j eigenvalues below it closes the new bracket.
@@ -308,6 +327,15 @@ __global__ void __launch_bounds__(256) k_bisect_ms(const double* __restrict__ d,
                                                    const double* __restrict__ e, int n,
                                                    const double* __restrict__ bounds,
                                                    double* __restrict__ w) {
+  __shared__ double ds[kBisectLds], e2s[kBisectLds];
+  const bool staged = n <= kBisectLds;
+  if (staged) {
+    for (int i = threadIdx.x; i < n; i += 256) {
+      ds[i] = d[i];
+      e2s[i] = i < n - 1 ? e[i] * e[i] : 0.0;
+    }
+    __syncthreads();
+  }
   const int lane = threadIdx.x & 63;
   const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (j >= n) return;
@@ -318,7 +346,8 @@ __global__ void __launch_bounds__(256) k_bisect_ms(const double* __restrict__ d,
     if (hi - lo <= 4.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin) break;
     const double step = (hi - lo) / 65.0;
     const double x = lo + step * (double)(lane + 1);
-    const int cnt = sturm_count_ms(d, e, n, x, pivmin);
+    const int cnt = staged ? sturm_count_lds(ds, e2s, n, x, pivmin)
+                           : sturm_count_ms(d, e, n, x, pivmin);
     const unsigned long long m = __ballot(cnt > j);
     const int l = m ? __ffsll((long long)m) - 1 : 64;
     const double nlo = l > 0 ? lo + step * (double)l : lo;
@@ -494,19 +523,57 @@ constexpr double kClusterTol = 1e-3;
 constexpr int kClusterMax = 64;  // larger clusters take the CholeskyQR path
 
 // cl[j] = first index of the cluster holding eigenvalue j; cl[n] = size of the
-// largest cluster.  One thread: n is at most a few thousand.
-__global__ void k_cluster_scan(const double* __restrict__ w, int n,
-                               const double* __restrict__ bounds, int* __restrict__ cl) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// largest cluster.  One workgroup of 1024 threads: the cluster starts are an
+// inclusive max-scan of the indices that open a cluster (n <= 4096), one
+// thread for larger n.
+constexpr int kScanMax = 4096;
+__global__ void __launch_bounds__(1024) k_cluster_scan(const double* __restrict__ w, int n,
+                                                       const double* __restrict__ bounds,
+                                                       int* __restrict__ cl) {
+  __shared__ int st[kScanMax];
+  __shared__ int wmx[16];
+  const int t = threadIdx.x;
   const double tol = kClusterTol * bounds[3];
-  int s = 0, mx = 1;
-  cl[0] = 0;
-  for (int j = 1; j < n; ++j) {
-    if (w[j] - w[j - 1] > tol) s = j;
-    cl[j] = s;
-    mx = max(mx, j - s + 1);
+  if (n > kScanMax) {
+    if (t != 0) return;
+    int s = 0, mx = 1;
+    cl[0] = 0;
+    for (int j = 1; j < n; ++j) {
+      if (w[j] - w[j - 1] > tol) s = j;
+      cl[j] = s;
+      mx = max(mx, j - s + 1);
+    }
+    cl[n] = mx;
+    return;
   }
-  cl[n] = mx;
+  for (int j = t; j < n; j += 1024) st[j] = (j == 0 || w[j] - w[j - 1] > tol) ? j : 0;
+  __syncthreads();
+  for (int off = 1; off < n; off <<= 1) {
+    int v[kScanMax / 1024];
+#pragma unroll
+    for (int u = 0; u < kScanMax / 1024; ++u) {
+      const int j = t + 1024 * u;
+      v[u] = j < n ? (j >= off ? max(st[j], st[j - off]) : st[j]) : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kScanMax / 1024; ++u)
+      if (t + 1024 * u < n) st[t + 1024 * u] = v[u];
+    __syncthreads();
+  }
+  int mx = 1;
+  for (int j = t; j < n; j += 1024) {
+    cl[j] = st[j];
+    mx = max(mx, j - st[j] + 1);
+  }
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+  if ((t & 63) == 0) wmx[t >> 6] = mx;
+  __syncthreads();
+  if (t == 0) {
+    int m = wmx[0];
+    for (int q = 1; q < 16; ++q) m = max(m, wmx[q]);
+    cl[n] = m;
+  }
 }
 
 // One workgroup per cluster start: classical Gram-Schmidt applied twice to
@@ -658,11 +725,20 @@ __global__ void k_wy_pack(const double* __restrict__ a, int n, int npad, int nbl
 // by thread i (row i of the recurrence needs row i only)
 __global__ void __launch_bounds__(256) k_wy_t(const double* __restrict__ Vt, int npad,
                                               double* __restrict__ Tb) {
-  __shared__ double G[kWyB][kWyB + 1], T[kWyB][kWyB + 1];
+  __shared__ double G[kWyB][kWyB + 1], T[kWyB][kWyB + 1], Vs[64][kWyB + 1];
   const int b = blockIdx.x, t = threadIdx.x, r = t / kWyB, s = t % kWyB;
   const double* V = Vt + (size_t)b * npad * kWyB;
   double acc = 0.0;
-  for (int i = 0; i < npad; ++i) acc += V[(size_t)i * kWyB + r] * V[(size_t)i * kWyB + s];
+  for (int i0 = 0; i0 < npad; i0 += 64) {  // 64 rows of the block at a time through LDS
+    for (int idx = t; idx < 64 * kWyB; idx += 256) {
+      const int ri = idx / kWyB, ci = idx % kWyB;
+      Vs[ri][ci] = i0 + ri < npad ? V[(size_t)(i0 + ri) * kWyB + ci] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int i = 0; i < 64; ++i) acc += Vs[i][r] * Vs[i][s];
+    __syncthreads();
+  }
   G[r][s] = acc;
   T[r][s] = 0.0;
   __syncthreads();
@@ -868,7 +944,7 @@ int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int
   // factors of pass 0 are reused by pass 1.  Vectors are orthonormalised
   // within eigenvalue clusters, or, when a cluster is too large for that, all
   // together by CholeskyQR (scratch X).
-  hipLaunchKernelGGL(k_cluster_scan, dim3(1), dim3(64), 0, st, w, n, bounds, cl);
+  hipLaunchKernelGGL(k_cluster_scan, dim3(1), dim3(1024), 0, st, w, n, bounds, cl);
   int maxcl = 0;
   QX_CHECK(hipMemcpyAsync(&maxcl, cl + n, sizeof(int), hipMemcpyDeviceToHost, st));
   QX_CHECK(hipStreamSynchronize(st));
