@@ -190,6 +190,14 @@ int qx_launch_timer(qx_ctx *ctx, int id, int mode, double *ms, int *count);
 int qx_set_gemm_path(int path);
 int qx_gemm_path(void);
 
+/* work schedules of our contraction kernels, process-wide: 0 = each product
+ * shape takes the faster of stream-K and K-sliced on its first use (the
+ * default), 1 = stream-K only, 2 = K-sliced only.  Results of the schedules
+ * agree to rounding of the partial-sum order; tests run each.  (No
+ * counterpart in the reference.) */
+int qx_set_gemm_schedule(int mode);
+int qx_gemm_schedule(void);
+
 /* ---------------- Fortran-shaped drop-ins (host arrays) -------------------- */
 /* rfft3 (Sources/fft3.f90:113-232): key=1 r2c x1/N with full Hermitian grid
  * out, else c2r from the half grid; ierror = 0 on success. */

@@ -783,6 +783,14 @@ int qx_set_gemm_path(int path) {
 
 int qx_gemm_path(void) { return gemm_path(); }
 
+int qx_set_gemm_schedule(int mode) {
+  if (mode < 0 || mode > 2) return fail(-1, "qx_set_gemm_schedule: bad mode %d", mode);
+  mg_set_schedule(mode);
+  return 0;
+}
+
+int qx_gemm_schedule(void) { return mg_schedule(); }
+
 int qx_set_cg_graph(qx_ctx* h, int on) {
   Ctx* c = &h->c;
   c->cg_graph_on = on != 0;

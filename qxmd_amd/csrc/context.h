@@ -227,6 +227,10 @@ int lt_sgemm_f32(Ctx* c, bool ta, int M, int N, int K, const float* A, int lda, 
 // our MFMA contractions (mfma_gemm.hip); -3 = shape not taken (odd leading
 // dimension), the caller falls back to the small-tile kernels
 void mg_destroy(Ctx* c);
+// schedules the first-use timing of a product shape may choose from:
+// 0 stream-K and K-sliced, 1 stream-K only, 2 K-sliced only
+void mg_set_schedule(int mode);
+int mg_schedule();
 void cg_graphs_destroy(Ctx* c);
 // k-points (kpoint.hip)
 void kp_destroy(Ctx* c);

@@ -33,6 +33,7 @@
 // chosen per shape by timing the candidates on first use (into a scratch
 // output) and cached.
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <map>
 #include <tuple>
@@ -53,12 +54,14 @@ constexpr int kXcd = 8;                    // XCDs the dispatcher round-robins o
 
 __host__ __device__ inline unsigned cdivu(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
-// the stream-K decomposition of one product
+// the work decomposition of one product: stream-K (S == 0) or K-sliced (S > 0)
 struct Sk {
   int M, N, K;
-  int tiles_m, nk;  // tiles along M; K stages per tile
-  long L;           // iterations (tile, K stage) per workgroup
-  int G;            // workgroups
+  int tiles_m, tiles_n, nk;  // tiles along M and N; K stages per tile
+  long L;                    // stream-K: iterations (tile, K stage) per workgroup
+  int S, nks;                // K-sliced: slices, K stages per slice
+  int nfast;                 // tiles numbered with the N index fastest (NN products)
+  int G;                     // workgroups
   double alpha, beta;
   int ldc, corr;
   long alo, blo;  // pre-split operands: element offset of the lo half after the hi half
@@ -69,6 +72,43 @@ struct Sk {
 __device__ inline int logical_wg(int G) {
   const int b = (int)blockIdx.x;
   return (b % kXcd) * (G / kXcd) + b / kXcd;
+}
+
+// The (tile, K stage) iterations of logical workgroup lb, [g0, g1) with
+// iteration g = tile * nk + stage.  Stream-K: a contiguous range of L
+// iterations, tile-major.  K-sliced: the single segment of tile lb % T over
+// slice lb / T, stages [s nks, min(nk, (s + 1) nks)); the slices are
+// numbered slowest, so the workgroups of one XCD (a contiguous stretch of
+// logical workgroups) run the same K-slice of every tile at the same time and
+// share each K stage of both operands in their L2.  Empty for workgroups past
+// the last non-empty slice.
+__device__ inline void sk_range(const Sk& sk, int lb, long& g0, long& g1) {
+  const int T = sk.tiles_m * sk.tiles_n;
+  if (sk.S > 0) {
+    const int s = lb / T, t = lb - s * T;
+    const int a = s * sk.nks, b = min(sk.nk, a + sk.nks);
+    if (s >= sk.S || a >= b) {
+      g0 = g1 = 0;
+      return;
+    }
+    g0 = (long)t * sk.nk + a;
+    g1 = (long)t * sk.nk + b;
+  } else {
+    g0 = (long)lb * sk.L;
+    g1 = min((long)T * sk.nk, g0 + sk.L);
+  }
+}
+
+// origin of output tile t: M index fastest, or N index fastest (NN products:
+// consecutive tiles share the rows of the long operand A)
+__device__ inline void tile_origin(const Sk& sk, int t, int& m0, int& n0) {
+  if (sk.nfast) {
+    m0 = (t / sk.tiles_n) * sk.tm;
+    n0 = (t % sk.tiles_n) * sk.tn;
+  } else {
+    m0 = (t % sk.tiles_m) * sk.tm;
+    n0 = (t / sk.tiles_m) * sk.tn;
+  }
 }
 
 // packed-sum correction of C(m, n) (TN only): 1 -> 0.5 A0 B0, 2 -> + A1 B1
@@ -99,15 +139,16 @@ __global__ void __launch_bounds__(256, 2)
   __shared__ double Bs[2][KT64][LPN];
   const int M = sk.M, N = sk.N, K = sk.K;
   const int lb = logical_wg(sk.G);
-  const long I = (long)sk.tiles_m * cdivu(N, TN) * sk.nk;
-  const long g0 = lb * sk.L, g1 = min(I, g0 + sk.L);
+  long g0, g1;
+  sk_range(sk, lb, g0, g1);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * (TM / 2), wn = (wave >> 1) * (TN / 2);
   for (long g = g0; g < g1;) {
     const int t = (int)(g / sk.nk);
     const int s0 = (int)(g - (long)t * sk.nk);
     const int s1 = (int)min((long)sk.nk, g1 - (long)t * sk.nk);
-    const int m0 = (t % sk.tiles_m) * TM, n0 = (t / sk.tiles_m) * TN;
+    int m0, n0;
+    tile_origin(sk, t, m0, n0);
     const int kb = s0 * KT64, ke = min(K, s1 * KT64);
     d4 acc[BI][BJ];
 #pragma unroll
@@ -232,19 +273,29 @@ template <typename TW, typename T, typename TC>
 __global__ void k_mg_fixup(Sk sk, const TW* __restrict__ Wp, const T* __restrict__ A, int lda,
                            const T* __restrict__ B, int ldb, TC* __restrict__ C) {
   const int t = (int)blockIdx.y;
-  const long lo = (long)t * sk.nk, hi = lo + sk.nk - 1;
-  const int b_lo = (int)(lo / sk.L), b_hi = (int)(hi / sk.L);
-  if (b_lo == b_hi) return;
   const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (e >= sk.tm * sk.tn) return;
   const int ml = e % sk.tm, nl = e / sk.tm;
-  const int m = (t % sk.tiles_m) * sk.tm + ml, n = (t / sk.tiles_m) * sk.tn + nl;
+  int m0, n0;
+  tile_origin(sk, t, m0, n0);
+  const int m = m0 + ml, n = n0 + nl;
   if (m >= sk.M || n >= sk.N) return;
   double s = 0.0;
-  for (int b = b_lo; b <= b_hi; ++b) {
-    const int first = (int)((long)b * sk.L / sk.nk);  // the workgroup's first tile
-    const size_t slot = (size_t)b * 2 + (first == t ? 0 : 1);
-    s += (double)Wp[slot * TE * TE + (size_t)nl * sk.tm + ml];
+  if (sk.S > 0) {
+    // K-sliced: the first partial slot of workgroup (slice q, tile t), slices in order
+    const int nt = sk.tiles_m * sk.tiles_n, ns = (int)cdivu(sk.nk, sk.nks);
+    if (ns <= 1) return;  // one slice covered the tile whole
+    for (int q = 0; q < ns; ++q)
+      s += (double)Wp[(size_t)(q * nt + t) * 2 * TE * TE + (size_t)nl * sk.tm + ml];
+  } else {
+    const long lo = (long)t * sk.nk, hi = lo + sk.nk - 1;
+    const int b_lo = (int)(lo / sk.L), b_hi = (int)(hi / sk.L);
+    if (b_lo == b_hi) return;
+    for (int b = b_lo; b <= b_hi; ++b) {
+      const int first = (int)((long)b * sk.L / sk.nk);  // the workgroup's first tile
+      const size_t slot = (size_t)b * 2 + (first == t ? 0 : 1);
+      s += (double)Wp[slot * TE * TE + (size_t)nl * sk.tm + ml];
+    }
   }
   s -= corr_term(sk.corr, A, lda, B, ldb, m, n);
   TC* p = C + (size_t)n * sk.ldc + m;
@@ -305,8 +356,8 @@ __global__ void __launch_bounds__(256, 2)
   __shared__ unsigned short Ah[TE][LP16], Al[TE][LP16], Bh[TE][LP16], Bl[TE][LP16];
   const int M = sk.M, N = sk.N, K = sk.K;
   const int lb = logical_wg(sk.G);
-  const long I = (long)sk.tiles_m * cdivu(N, TE) * sk.nk;
-  const long g0 = lb * sk.L, g1 = min(I, g0 + sk.L);
+  long g0, g1;
+  sk_range(sk, lb, g0, g1);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
   // k-contiguous rows: thread -> row tid >> 1, k run (tid & 1) * 16
@@ -317,7 +368,8 @@ __global__ void __launch_bounds__(256, 2)
     const int t = (int)(g / sk.nk);
     const int s0 = (int)(g - (long)t * sk.nk);
     const int s1 = (int)min((long)sk.nk, g1 - (long)t * sk.nk);
-    const int m0 = (t % sk.tiles_m) * TE, n0 = (t / sk.tiles_m) * TE;
+    int m0, n0;
+    tile_origin(sk, t, m0, n0);
     const int kb = s0 * KT16, ke = min(K, s1 * KT16);
     f4 acc[4][4];
 #pragma unroll
@@ -476,8 +528,8 @@ __global__ void __launch_bounds__(256, 2)
   __shared__ u16 Ah[TE][LPP], Al[TE][LPP], Bh[TE][LPP], Bl[TE][LPP];
   const int M = sk.M, N = sk.N;
   const int lb = logical_wg(sk.G);
-  const long I = (long)sk.tiles_m * cdivu(N, TE) * sk.nk;
-  const long g0 = lb * sk.L, g1 = min(I, g0 + sk.L);
+  long g0, g1;
+  sk_range(sk, lb, g0, g1);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
   // k-contiguous rows: thread -> row tid >> 1, k run (tid & 1) * 32 .. + 31
@@ -488,7 +540,8 @@ __global__ void __launch_bounds__(256, 2)
     const int t = (int)(g / sk.nk);
     const int s0 = (int)(g - (long)t * sk.nk);
     const int s1 = (int)min((long)sk.nk, g1 - (long)t * sk.nk);
-    const int m0 = (t % sk.tiles_m) * TE, n0 = (t / sk.tiles_m) * TE;
+    int m0, n0;
+    tile_origin(sk, t, m0, n0);
     const int kb = s0 * KP, ke = s1 * KP;
     f4 acc[4][4];
 #pragma unroll
@@ -664,6 +717,7 @@ MgState* state(Ctx* c) {
 
 void* workspace(Ctx* c, size_t bytes) {
   MgState* s = state(c);
+  bytes = std::max(bytes, (size_t)256);  // a product with no partial tiles still gets a pointer
   if (s->ws_bytes >= bytes) return s->ws;
   buffers_changed(c);
   if (s->ws) (void)hipFree(s->ws);
@@ -674,8 +728,11 @@ void* workspace(Ctx* c, size_t bytes) {
   return s->ws;
 }
 
-Sk make_sk(int M, int N, int K, int kt, int G, double alpha, double beta, int ldc, int corr,
-           int tm = TE, int tn = TE) {
+// A schedule code: > 0 = stream-K on that many workgroups, < 0 = K-sliced
+// with -code slices (on the multiple of 8 workgroups that covers slices x
+// tiles; -1 is one workgroup per output tile)
+Sk make_sk(int M, int N, int K, int kt, int code, double alpha, double beta, int ldc, int corr,
+           int tm = TE, int tn = TE, bool nfast = false) {
   Sk sk;
   sk.M = M;
   sk.N = N;
@@ -683,10 +740,21 @@ Sk make_sk(int M, int N, int K, int kt, int G, double alpha, double beta, int ld
   sk.tm = tm;
   sk.tn = tn;
   sk.tiles_m = (int)cdivu(M, tm);
+  sk.tiles_n = (int)cdivu(N, tn);
   sk.nk = (int)cdivu(K, kt);
-  const long I = (long)sk.tiles_m * cdivu(N, tn) * sk.nk;
-  sk.G = G;
-  sk.L = (long)cdivu(I, G);
+  sk.nfast = nfast ? 1 : 0;
+  const long T = (long)sk.tiles_m * sk.tiles_n;
+  if (code > 0) {
+    sk.S = 0;
+    sk.nks = 0;
+    sk.G = code;
+    sk.L = (long)cdivu(T * sk.nk, code);
+  } else {
+    sk.S = std::min(-code, std::max(sk.nk, 1));
+    sk.nks = (int)cdivu(sk.nk, sk.S);
+    sk.G = (int)(cdivu(sk.S * T, kXcd) * kXcd);
+    sk.L = 0;
+  }
   sk.alpha = alpha;
   sk.beta = beta;
   sk.ldc = ldc;
@@ -695,51 +763,92 @@ Sk make_sk(int M, int N, int K, int kt, int G, double alpha, double beta, int ld
   return sk;
 }
 
+// partial-tile slots of the workspace a product needs (none when every
+// tile is covered whole by one workgroup)
+size_t sk_slots(const Sk& sk) {
+  if (sk.S > 0) return cdivu(sk.nk, sk.nks) <= 1 ? 0 : 2 * (size_t)sk.G;
+  return 2 * (size_t)sk.G;
+}
+
+// which schedules the timed choice may take: 0 both, 1 stream-K only,
+// 2 K-sliced only (qx_set_gemm_schedule; tests run each on its own)
+std::atomic<int> g_sched{0};
+
+// candidate schedule codes of a product with T output tiles on `cus` CUs:
+// stream-K on one and two workgroups per CU; K-sliced with 8 slices (one per
+// XCD) and with as many slices as fill one or two workgroups per CU, all
+// slices x tiles resident at once; one workgroup per tile where the tiles
+// alone fill the chip
+std::vector<int> sched_candidates(long T, int nk, int cus, int per_cu_max) {
+  std::vector<int> v;
+  const int mode = g_sched.load(std::memory_order_relaxed);
+  if (mode != 2)
+    for (int k = 1; k <= per_cu_max; ++k) v.push_back(k * cus);
+  if (mode != 1) {
+    const int before = (int)v.size();
+    auto add = [&](long S) {
+      if (S < 1 || S > nk) return;
+      const int code = -(int)S;
+      if (std::find(v.begin() + before, v.end(), code) == v.end()) v.push_back(code);
+    };
+    if (8 * T <= (long)per_cu_max * cus) add(8);
+    for (int k = 1; k <= per_cu_max; ++k)
+      if ((long)k * cus / T >= 2) add((long)k * cus / T);
+    if (T >= cus || (int)v.size() == before) add(1);
+  }
+  return v;
+}
+
 bool tuning_off() {
   static const bool off = std::getenv("QX_GEMM_NOTUNE") != nullptr;
   return off;
 }
 
-// workgroup count for a product: candidates of 1 .. per_cu_max workgroups
-// per CU, timed on first use of a shape with `trial(G)` (which writes a
-// scratch output), then cached
+// schedule of a product (a code of make_sk): the candidates of
+// sched_candidates, timed on first use of a shape with `trial(code)` (which
+// writes a scratch output), then cached; untimed, stream-K on two
+// workgroups per CU; 0 on failure
 template <typename F>
-int choose_grid(Ctx* c, int kind, int M, int N, int K, int lda, int ldb, int per_cu_max,
-                size_t part_elem, F trial) {
+int choose_grid(Ctx* c, int kind, int M, int N, int K, int kt, int tm, int tn, int lda, int ldb,
+                int per_cu_max, size_t part_elem, F trial) {
   MgState* s = state(c);
-  const auto key = std::make_tuple(kind, M, N, K, lda, ldb);
+  const int mode = g_sched.load(std::memory_order_relaxed);
+  const auto key = std::make_tuple(kind + 1000 * mode, M, N, K, lda, ldb);
   auto it = s->grid.find(key);
   if (it != s->grid.end()) return it->second;
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->dev);
   cus = std::max(kXcd, cus / kXcd * kXcd);
-  int best_g = 2 * cus;
-  if (!tuning_off()) {
+  const long T = (long)cdivu(M, tm) * cdivu(N, tn);
+  const std::vector<int> cand = sched_candidates(T, (int)cdivu(K, kt), cus, per_cu_max);
+  int best_code = cand.empty() ? 2 * cus : cand.back();
+  if (mode == 0 && !cand.empty()) best_code = cand[per_cu_max > 1 ? 1 : 0];
+  if (!tuning_off() && cand.size() > 1) {
     hipEvent_t e0, e1;
     QX_CHECK(hipEventCreate(&e0));
     QX_CHECK(hipEventCreate(&e1));
     float best = 1e30f;
-    for (int k = 1; k <= per_cu_max; ++k) {
-      const int G = k * cus;
-      if (!workspace(c, part_elem * 2 * TE * TE * (size_t)G)) return -3;
-      if (trial(G)) return -3;  // warm
+    for (const int code : cand) {
+      const Sk probe = make_sk(M, N, K, kt, code, 1.0, 0.0, M, 0, tm, tn);
+      if (!workspace(c, part_elem * TE * TE * sk_slots(probe))) return 0;
+      if (trial(code)) return 0;  // warm
       QX_CHECK(hipEventRecord(e0, c->stream));
       for (int r = 0; r < 2; ++r)
-        if (trial(G)) return -3;
+        if (trial(code)) return 0;
       QX_CHECK(hipEventRecord(e1, c->stream));
       QX_CHECK(hipEventSynchronize(e1));
       float ms = 0.f;
       QX_CHECK(hipEventElapsedTime(&ms, e0, e1));
       if (ms < best) {
         best = ms;
-        best_g = G;
+        best_code = code;
       }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }
-  s->grid[key] = best_g;
-  return best_g;
+  s->grid[key] = best_code;
+  return best_code;
 }
 
 // scratch output of a tuning trial (M x N of TC), released after the trials
@@ -758,37 +867,43 @@ template <typename TW, typename T, typename TC, typename K1>
 int launch_sk(Ctx* c, const Sk& sk, K1 kernel, const T* A, int lda, const T* B, int ldb, TC* C) {
   TW* W = static_cast<TW*>(state(c)->ws);
   hipLaunchKernelGGL(kernel, dim3(sk.G), dim3(256), 0, c->stream, sk, A, lda, B, ldb, C, W);
-  const int tiles = sk.tiles_m * (int)cdivu(sk.N, sk.tn);
+  const int tiles = sk.tiles_m * sk.tiles_n;
   hipLaunchKernelGGL((k_mg_fixup<TW, T, TC>), dim3(TE * TE / 256, tiles), dim3(256), 0,
                      c->stream, sk, W, A, lda, B, ldb, C);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
+// nfast: number the output tiles with the N index fastest (the NN products,
+// whose long operand A is shared by the tiles of one row of tiles)
 template <typename TW, typename T, typename TC, typename K1>
 int run_product(Ctx* c, int kind, int kt, int per_cu_max, K1 kernel, int M, int N, int K,
                 double alpha, const T* A, int lda, const T* B, int ldb, double beta, TC* C,
-                int ldc, int corr, long alo = 0, long blo = 0, int tm = TE, int tn = TE) {
+                int ldc, int corr, long alo = 0, long blo = 0, int tm = TE, int tn = TE,
+                bool nfast = false) {
   if (M <= 0 || N <= 0) return 0;
   if ((lda | ldb) & 1) return -3;  // 8 / 16-byte row loads need even leading dimensions
   MgState* s = state(c);
-  const bool known = s->grid.count(std::make_tuple(kind, M, N, K, lda, ldb)) != 0;
-  int G;
+  const int mode = g_sched.load(std::memory_order_relaxed);
+  const bool known = s->grid.count(std::make_tuple(kind + 1000 * mode, M, N, K, lda, ldb)) != 0;
+  int code;
   if (known || tuning_off()) {
-    G = choose_grid(c, kind, M, N, K, lda, ldb, per_cu_max, sizeof(TW), [](int) { return 0; });
+    code = choose_grid(c, kind, M, N, K, kt, tm, tn, lda, ldb, per_cu_max, sizeof(TW),
+                       [](int) { return 0; });
   } else {
     TrialOut<TC> out((size_t)M * N);
     if (!out.p) return -3;
-    G = choose_grid(c, kind, M, N, K, lda, ldb, per_cu_max, sizeof(TW), [&](int g) {
-      Sk tk = make_sk(M, N, K, kt, g, alpha, 0.0, M, corr, tm, tn);
-      tk.alo = alo;
-      tk.blo = blo;
-      return launch_sk<TW>(c, tk, kernel, A, lda, B, ldb, out.p);
-    });
+    code = choose_grid(c, kind, M, N, K, kt, tm, tn, lda, ldb, per_cu_max, sizeof(TW),
+                       [&](int cd) {
+                         Sk tk = make_sk(M, N, K, kt, cd, alpha, 0.0, M, corr, tm, tn, nfast);
+                         tk.alo = alo;
+                         tk.blo = blo;
+                         return launch_sk<TW>(c, tk, kernel, A, lda, B, ldb, out.p);
+                       });
     QX_CHECK(hipStreamSynchronize(c->stream));  // trials done before the scratch goes
   }
-  if (G < 0) return G;
-  if (!workspace(c, sizeof(TW) * 2 * TE * TE * (size_t)G)) return -3;
-  Sk sk = make_sk(M, N, K, kt, G, alpha, beta, ldc, corr, tm, tn);
+  if (code == 0) return -3;
+  Sk sk = make_sk(M, N, K, kt, code, alpha, beta, ldc, corr, tm, tn, nfast);
+  if (!workspace(c, sizeof(TW) * TE * TE * sk_slots(sk))) return -3;
   sk.alo = alo;
   sk.blo = blo;
   lt_mark(c, c->lt_gemm_id, false);
@@ -798,6 +913,9 @@ int run_product(Ctx* c, int kind, int kt, int per_cu_max, K1 kernel, int M, int 
 }
 
 }  // namespace
+
+void mg_set_schedule(int mode) { g_sched.store(mode, std::memory_order_relaxed); }
+int mg_schedule() { return g_sched.load(std::memory_order_relaxed); }
 
 void mg_destroy(Ctx* c) {
   if (!c->mg) return;
@@ -920,7 +1038,7 @@ int f64_product(Ctx* c, int kind, int M, int N, int K, double alpha, const doubl
   const auto key = std::make_tuple(kind, M, N, K, lda, ldb);
   auto run = [&](int v, double b, double* out, int ldo) {
     return run_product<double>(c, kind + 10 * (v + 1), KT64, 2, ks[v], M, N, K, alpha, A, lda, B,
-                               ldb, b, out, ldo, corr, 0, 0, tms[v], tns[v]);
+                               ldb, b, out, ldo, corr, 0, 0, tms[v], tns[v], !AK);
   };
   auto it = s->tile.find(key);
   int best = 0;
@@ -990,7 +1108,7 @@ template <typename T, typename TC>
 int mg_bf3_nn(Ctx* c, int M, int N, int K, double alpha, const T* A, int lda, const T* B, int ldb,
               double beta, TC* C, int ldc) {
   return run_product<float>(c, 4 + (sizeof(T) == 8 ? 1 : 0), KT16, 2, k_mg_bf3<T, TC, false>, M,
-                            N, K, alpha, A, lda, B, ldb, beta, C, ldc, 0);
+                            N, K, alpha, A, lda, B, ldb, beta, C, ldc, 0, 0, 0, TE, TE, true);
 }
 
 // split bf16x3 on pre-split operands (hi then lo, K a multiple of 64 with
@@ -1007,7 +1125,7 @@ int mg_ps3_nn(Ctx* c, int M, int N, int K, const unsigned short* A, int lda, lon
               const unsigned short* B, int ldb, long blo, float* C, int ldc) {
   if (K % KP || (lda | ldb) % KP || alo % KP || blo % KP) return -3;
   return run_product<float>(c, 7, KP, 2, k_mg_ps3<false, float>, M, N, K, 1.0, A, lda, B, ldb,
-                            0.0, C, ldc, 0, alo, blo);
+                            0.0, C, ldc, 0, alo, blo, TE, TE, true);
 }
 
 // the same on FP64 C: C = alpha A^T B + beta C (TN), C = alpha A B + beta C (NN)
@@ -1024,7 +1142,7 @@ int mg_ps3_nn_d(Ctx* c, int M, int N, int K, double alpha, const unsigned short*
                 int ldc) {
   if (K % KP || (lda | ldb) % KP || alo % KP || blo % KP) return -3;
   return run_product<float>(c, 9, KP, 2, k_mg_ps3<false, double>, M, N, K, alpha, A, lda, B, ldb,
-                            beta, C, ldc, 0, alo, blo);
+                            beta, C, ldc, 0, alo, blo, TE, TE, true);
 }
 
 // rows x cols FP64 (row stride lds) -> bf16 hi at dst, lo at dst + lo: row

@@ -191,20 +191,36 @@ __device__ inline void x_offsets(int b, int hmax, int sa, int (&off)[NA], unsign
   *inside = m;
 }
 
-// x step A for row slot ys, inputs gathered from the sphere tile
+// twiddles of an x-pass lane: the lane's index j is the same for every row it
+// transforms (b of step A, ka of step B'), so W^(j m), m = 1 .. max(NA, NB) - 1,
+// are read from the table once and kept in registers
+template <int NA, int NB>
+__host__ __device__ constexpr int x_ntw() {
+  return (NA > NB ? NA : NB) - 1;
+}
+template <typename T, int NA, int NB>
+__device__ inline void x_twiddles(const cplx<T>* tw, int j, cplx<T> (&w)[x_ntw<NA, NB>()]) {
+  constexpr int N = NA * NB;
+  int k = 0;
+#pragma unroll
+  for (int m = 0; m < x_ntw<NA, NB>(); ++m) {
+    k += j;
+    if (k >= N) k -= N;  // only lanes idle in both steps get here
+    w[m] = tw[k];
+  }
+}
+
+// x step A for row slot ys, inputs gathered from the sphere tile; xw: the
+// lane's twiddles W^(b ka)
 template <typename T, int NA, int NB>
 __device__ inline void x_stepA(const cplx<T>* A, const int (&off)[NA], cplx<T>* R, int ys, int b,
-                               const cplx<T>* tw) {
+                               const cplx<T> (&xw)[x_ntw<NA, NB>()]) {
   cplx<T> v[NA];
 #pragma unroll
   for (int a = 0; a < NA; ++a) v[a] = A[off[a] + ys];
   Dft<T, NA>::run(v, +1);
-  int k = 0;
 #pragma unroll
-  for (int ka = 1; ka < NA; ++ka) {
-    k += b;  // b * ka
-    v[ka] = cmul(v[ka], twid(tw, k, +1));
-  }
+  for (int ka = 1; ka < NA; ++ka) v[ka] = cmul(v[ka], xw[ka - 1]);
 #pragma unroll
   for (int ka = 0; ka < NA; ++ka) R[b + NB * ka] = v[ka];
 }
@@ -293,6 +309,8 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = mk<T>((T)a.tw[2 * k], (T)a.tw[2 * k + 1]);
   for (int i = threadIdx.x; i < (nH + 1) * sa; i += blockDim.x) A[i] = mk<T>(0, 0);
   __syncthreads();
+  cplx<T> xw[x_ntw<NA, NB>()];
+  x_twiddles<T, NA, NB>(tw, jx, xw);
 #pragma unroll
   for (int j = 0; j < kPlaneLoads; ++j)
     if (ci[j] >= 0) A[ci[j]] = cv[j];
@@ -317,7 +335,7 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
     const int y0 = g * RW;
     const bool row_ok = y0 + rx < N;
     cplx<T>* Rr = Rw + rx * sr;
-    if (actA && row_ok) x_stepA<T, NA, NB>(A, xoff, Rr, y0 + rx, jx, tw);
+    if (actA && row_ok) x_stepA<T, NA, NB>(A, xoff, Rr, y0 + rx, jx, xw);
     wave_lds_sync();
     if (actB && row_ok) {
       cplx<T>* L = Rr + NB * jx;
@@ -331,12 +349,8 @@ __global__ void __launch_bounds__(kPlaneMaxThreads) k_plane_f(FastArgs a, const 
         v[kb].y *= vv[kb];
       }
       Dft<T, NB>::run(v, -1);
-      int k = 0;
 #pragma unroll
-      for (int b = 1; b < NB; ++b) {
-        k += jx;  // b * jx
-        v[b] = cmul(v[b], twid(tw, k, -1));
-      }
+      for (int b = 1; b < NB; ++b) v[b] = cmul(v[b], mk<T>(xw[b - 1].x, -xw[b - 1].y));  // W^-(b jx)
 #pragma unroll
       for (int b = 0; b < NB; ++b) L[b] = v[b];
     }
@@ -464,6 +478,9 @@ __global__ void __launch_bounds__(rho_max_threads(RG)) k_plane_rho_f(FastArgs a,
   unsigned xin;
   x_offsets<NA, NB>(min(jx, NB - 1), a.hmax, sa, xoff, &xin);
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = mk<T>((T)a.tw[2 * k], (T)a.tw[2 * k + 1]);
+  __syncthreads();
+  cplx<T> xw[x_ntw<NA, NB>()];
+  x_twiddles<T, NA, NB>(tw, jx, xw);
   double acc[RG][NB];
 #pragma unroll
   for (int gi = 0; gi < RG; ++gi)
@@ -498,7 +515,7 @@ __global__ void __launch_bounds__(rho_max_threads(RG)) k_plane_rho_f(FastArgs a,
         const int y0 = g * RW;
         const bool row_ok = y0 + rx < N;
         cplx<T>* Rr = Rw + rx * sr;
-        if (actA && row_ok) x_stepA<T, NA, NB>(A, xoff, Rr, y0 + rx, jx, tw);
+        if (actA && row_ok) x_stepA<T, NA, NB>(A, xoff, Rr, y0 + rx, jx, xw);
         wave_lds_sync();
         if (actB && row_ok) {
           const cplx<T>* L = Rr + NB * jx;

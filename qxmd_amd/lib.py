@@ -107,6 +107,8 @@ SIGNATURES = {
     "qx_set_cg_graph": [_vp, _i],
     "qx_cg_graph_count": [_vp],
     "qx_gemm_path": [],
+    "qx_set_gemm_schedule": [_i],
+    "qx_gemm_schedule": [],
     "qx_hcnonloc": [_vp, _dp, _dp],
     "qx_hckbpp_g": [_vp, _dp, _dp, _i, _vp],
     "qx_hckbpp_ib_g": [_vp, _dp, _dp],

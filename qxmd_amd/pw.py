@@ -28,6 +28,20 @@ def gemm_path():
     return int(load().qx_gemm_path())
 
 
+SCHED_TIMED, SCHED_STREAMK, SCHED_SLICED = 0, 1, 2
+
+
+def set_gemm_schedule(mode):
+    """Work schedules of our contraction kernels, process-wide: SCHED_TIMED
+    (each shape takes the faster of stream-K and K-sliced on first use),
+    SCHED_STREAMK or SCHED_SLICED only."""
+    check(load().qx_set_gemm_schedule(int(mode)), "qx_set_gemm_schedule")
+
+
+def gemm_schedule():
+    return int(load().qx_gemm_schedule())
+
+
 def clmrcp(b, dkmax2):
     """The Ewald reciprocal-lattice list (clmrcp, Sources/bulk.f90:556) for
     reciprocal vectors b (3, 3; columns, without 2 pi) and |G|^2 <= dkmax2:

@@ -125,3 +125,32 @@ def test_own_kernels_match_hipblaslt(case):
     assert _relerr(a[0], b[0]) < 1e-12
     assert np.abs(a[1] - b[1]).max() < 1e-11
     assert _relerr(a[2], b[2]) < 1e-10
+
+
+@pytest.mark.parametrize("sched", [pw.SCHED_STREAMK, pw.SCHED_SLICED], ids=["streamk", "sliced"])
+def test_schedules_vs_oracle(case, sched):
+    """Each work schedule of our kernels on its own (stream-K; K-sliced, which
+    for the NN products here is one workgroup per output tile): FP64 KB
+    apply, split bf16x3 KB apply, rotation and Gram-Schmidt against the
+    oracle to the bounds of the tests above."""
+    ctx, H, psi = case["ctx"], case["H"], case["psi"]
+    pw.set_gemm_schedule(sched)
+    try:
+        print("schedule", sched, flush=True)
+        got64 = _kb(ctx, psi)
+        got32 = _kb(ctx, psi, prec=pw.MIXED)
+        ctx.put(0, psi)
+        ctx.hpsi(0, 1, NB)
+        eig = ctx.rotate(0, 1, NB)
+        s, ok = ctx.schmidt_host(case["raw"])
+        assert ok
+    finally:
+        pw.set_gemm_schedule(pw.SCHED_TIMED)
+    ref, _ = O.hckbpp_g(case["eq"], case["lmx"], case["vcl"], psi, np.zeros_like(psi))
+    assert _relerr(got64, ref) < 1e-11
+    assert _relerr(got32, ref) < 1.5e-5
+    hm, _ = O.hmatrix(psi, H.hpsi(psi))
+    w, _ = O.eigh(hm)
+    np.testing.assert_allclose(eig, w, rtol=0, atol=1e-10)
+    sref, okr = O.schmidt(case["raw"])
+    assert okr and _relerr(s, sref) < 1e-10

@@ -19,6 +19,23 @@ def test_library_exports_header_symbols():
         assert hasattr(L, n), n
 
 
+def test_gemm_schedule_switch():
+    """The schedule switch is host state: every mode is taken back, a bad one
+    is refused without changing it."""
+    assert pw.gemm_schedule() == pw.SCHED_TIMED
+    try:
+        for m in (pw.SCHED_STREAMK, pw.SCHED_SLICED, pw.SCHED_TIMED):
+            pw.set_gemm_schedule(m)
+            assert pw.gemm_schedule() == m
+        pw.set_gemm_schedule(pw.SCHED_SLICED)
+        for bad in (-1, 3):
+            with pytest.raises(lib.QxError):
+                pw.set_gemm_schedule(bad)
+        assert pw.gemm_schedule() == pw.SCHED_SLICED
+    finally:
+        pw.set_gemm_schedule(pw.SCHED_TIMED)
+
+
 def test_no_cpu_fallback_without_device():
     import torch
     if torch.cuda.is_available():
