@@ -15,6 +15,7 @@
 // the forward step B' act on the same registers.
 #include <algorithm>
 #include <array>
+#include <cstdio>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -654,9 +655,10 @@ static int plane_waves(int N, int nH, int rw) {
 // banks of 4 bytes serve 64/el lanes per pass, and a pass lasts as many
 // cycles as the most distinct addresses one bank is asked for (lanes reading
 // the same address share it).  e[l] < 0 marks an idle lane.
-static int lds_passes(const int* e, int el) {
+// (banks = 64; the variant with 32 banks serves 32/el lanes per pass)
+static int lds_passes(const int* e, int el, int banks = 64) {
   int tot = 0;
-  const int lpp = 64 / el;
+  const int lpp = banks / el;
   for (int g = 0; g < 64; g += lpp) {
     int d[64], nd = 0;
     for (int l = g; l < g + lpp; ++l)
@@ -666,7 +668,7 @@ static int lds_passes(const int* e, int el) {
     std::sort(d, d + nd);
     nd = (int)(std::unique(d, d + nd) - d);
     int cnt[64] = {0}, mx = 0;
-    for (int i = 0; i < nd; ++i) mx = std::max(mx, ++cnt[d[i] & 63]);
+    for (int i = 0; i < nd; ++i) mx = std::max(mx, ++cnt[d[i] & (banks - 1)]);
     tot += mx;
   }
   return tot;
@@ -674,7 +676,7 @@ static int lds_passes(const int* e, int el) {
 
 // modelled LDS passes of the accesses that depend on the sphere tile stride
 // sa (y passes and the x pass's column reads/writes) ...
-static long plane_cost_sa(int NA, int NB, int nH, int sa, int el) {
+static long plane_cost_sa(int NA, int NB, int nH, int sa, int el, int banks) {
   const int N = NA * NB, hmax = (nH - 1) / 2, RW = plane_rows_per_wave(NA, NB);
   long c = 0;
   int e[64];
@@ -686,7 +688,7 @@ static long plane_cost_sa(int NA, int NB, int nH, int sa, int el) {
           const int it = w0 + l, line = it % nH, j = it / nH;
           e[l] = it < n ? line * sa + (step == 0 ? j + NB * q : NB * j + q) : -1;
         }
-        c += 4 * lds_passes(e, el);  // inverse and forward, read and write
+        c += 4 * lds_passes(e, el, banks);  // inverse and forward, read and write
       }
   }
   for (int y0 = 0; y0 < N; y0 += RW)
@@ -701,13 +703,13 @@ static long plane_cost_sa(int NA, int NB, int nH, int sa, int el) {
         e[l] = ok ? (h < N ? h + hmax : nH) * sa + y0 + rx : -1;
         e2[l] = ok && h < N ? (h + hmax) * sa + y0 + rx : -1;
       }
-      c += lds_passes(e, el) + lds_passes(e2, el);
+      c += lds_passes(e, el, banks) + lds_passes(e2, el, banks);
     }
   return c;
 }
 
 // ... and on the x strip stride sr
-static long plane_cost_sr(int NA, int NB, int sr, int el) {
+static long plane_cost_sr(int NA, int NB, int sr, int el, int banks) {
   const int N = NA * NB, RW = plane_rows_per_wave(NA, NB);
   long c = 0;
   int e[64];
@@ -717,14 +719,14 @@ static long plane_cost_sr(int NA, int NB, int sr, int el) {
         const int rx = l % RW, jx = l / RW;
         e[l] = (jx < NB && y0 + rx < N) ? rx * sr + jx + NB * q : -1;
       }
-      c += 2 * lds_passes(e, el);
+      c += 2 * lds_passes(e, el, banks);
     }
     for (int q = 0; q < NB; ++q) {
       for (int l = 0; l < 64; ++l) {
         const int rx = l % RW, jx = l / RW;
         e[l] = (jx < NA && y0 + rx < N) ? rx * sr + NB * jx + q : -1;
       }
-      c += 2 * lds_passes(e, el);
+      c += 2 * lds_passes(e, el, banks);
     }
   }
   return c;
@@ -736,20 +738,20 @@ static long plane_cost_sr(int NA, int NB, int sr, int el) {
 // model's choice).  Cached per shape.
 template <typename T>
 static std::vector<std::pair<int, int>> plane_stride_ranks(int NA, int NB, int nH, int nwave,
-                                                           int keep) {
+                                                           int keep, int banks = 64) {
   static std::mutex mu;
-  static std::map<std::array<int, 5>, std::vector<std::pair<int, int>>> cache;
-  const std::array<int, 5> key{NA * 100 + NB, nH, (int)sizeof(T), nwave, keep};
+  static std::map<std::array<int, 6>, std::vector<std::pair<int, int>>> cache;
+  const std::array<int, 6> key{NA * 100 + NB, nH, (int)sizeof(T), nwave, keep, banks};
   std::lock_guard<std::mutex> lk(mu);
   auto it = cache.find(key);
   if (it == cache.end()) {
     const int N = NA * NB, RW = plane_rows_per_wave(NA, NB), el = (int)sizeof(cplx<T>) / 4;
     const size_t occ0 = kCuLds / plane_lds<T>(N, nH, N, N, RW, nwave);
     long csr[33];
-    for (int d = 0; d <= 32; ++d) csr[d] = plane_cost_sr(NA, NB, N + d, el);
+    for (int d = 0; d <= 32; ++d) csr[d] = plane_cost_sr(NA, NB, N + d, el, banks);
     std::vector<std::pair<long, std::pair<int, int>>> all;
     for (int da = 0; da <= 32; ++da) {
-      const long ca = plane_cost_sa(NA, NB, nH, N + da, el);
+      const long ca = plane_cost_sa(NA, NB, nH, N + da, el, banks);
       for (int dr = 0; dr <= 32; ++dr) {
         const size_t l = plane_lds<T>(N, nH, N + da, N + dr, RW, nwave);
         if (l > kCuLds || kCuLds / l < occ0) continue;
@@ -777,6 +779,28 @@ static void plane_strides(int NA, int NB, int nH, int nwave, int* psa, int* psr)
   *psr = r[0].second;
   if (const char* e = std::getenv("QX_PLANE_SA")) *psa = std::max(NA * NB, std::atoi(e));
   if (const char* e = std::getenv("QX_PLANE_SR")) *psr = std::max(NA * NB, std::atoi(e));
+}
+
+// Stride pairs the plane tuner times: the three best of the bank model with
+// 64 banks and with 32, and a coarse grid of odd paddings (the model's
+// conflict counts have not matched the measured ones, so time decides), all
+// keeping the blocks per CU of unpadded rows.
+template <typename T>
+static std::vector<std::pair<int, int>> plane_stride_candidates(int NA, int NB, int nH,
+                                                                int nwave) {
+  const int N = NA * NB, RW = plane_rows_per_wave(NA, NB);
+  std::vector<std::pair<int, int>> v = plane_stride_ranks<T>(NA, NB, nH, nwave, 3, 64);
+  for (const auto& p : plane_stride_ranks<T>(NA, NB, nH, nwave, 3, 32))
+    if (std::find(v.begin(), v.end(), p) == v.end()) v.push_back(p);
+  const size_t occ0 = kCuLds / plane_lds<T>(N, nH, N, N, RW, nwave);
+  for (int da : {1, 3, 5, 9, 17})
+    for (int dr : {1, 5, 9, 17}) {
+      const std::pair<int, int> p{N + da, N + dr};
+      const size_t l = plane_lds<T>(N, nH, p.first, p.second, RW, nwave);
+      if (l > kCuLds || kCuLds / l < occ0) continue;
+      if (std::find(v.begin(), v.end(), p) == v.end()) v.push_back(p);
+    }
+  return v;
 }
 
 // mid (optional): work launched between the plane pass and the z forward pass,
@@ -831,7 +855,7 @@ static PlaneCfg plane_config(Ctx* c, const double* psi, int nb, const T* Vp, cpl
       float bms = 1e30f;
       for (int nw = kPlaneMaxThreads / 64; nw >= 4; nw /= 2) {
         if (plane_lds<T>(N, c->pc.nH, N, N, RW, nw) > kCuLds) continue;
-        for (const auto& st : plane_stride_ranks<T>(NA, NB, c->pc.nH, nw, 3)) {
+        for (const auto& st : plane_stride_candidates<T>(NA, NB, c->pc.nH, nw)) {
           const size_t lp = plane_lds<T>(N, c->pc.nH, st.first, st.second, RW, nw);
           if (lp > kCuLds) continue;
           if (lp > 65536)
@@ -849,6 +873,9 @@ static PlaneCfg plane_config(Ctx* c, const double* psi, int nb, const T* Vp, cpl
           (void)hipEventSynchronize(e1);
           float ms = 0.f;
           (void)hipEventElapsedTime(&ms, e0, e1);
+          if (std::getenv("QX_FFT_VERBOSE"))
+            fprintf(stderr, "plane tune N=%d nH=%d T%zu: %d waves, sa %d sr %d: %.4f ms\n", N,
+                    c->pc.nH, sizeof(T), nw, st.first, st.second, ms / 2);
           if (hipGetLastError() == hipSuccess && ms > 0.f && ms < bms) {
             bms = ms;
             best.nwave = nw;
