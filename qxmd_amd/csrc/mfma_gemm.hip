@@ -285,8 +285,17 @@ __global__ void k_mg_fixup(Sk sk, const TW* __restrict__ Wp, const T* __restrict
     // K-sliced: the first partial slot of workgroup (slice q, tile t), slices in order
     const int nt = sk.tiles_m * sk.tiles_n, ns = (int)cdivu(sk.nk, sk.nks);
     if (ns <= 1) return;  // one slice covered the tile whole
-    for (int q = 0; q < ns; ++q)
-      s += (double)Wp[(size_t)(q * nt + t) * 2 * TE * TE + (size_t)nl * sk.tm + ml];
+    const TW* w0 = Wp + (size_t)t * 2 * TE * TE + (size_t)nl * sk.tm + ml;
+    const size_t qs = (size_t)nt * 2 * TE * TE;  // slice to slice
+    int q = 0;
+    for (; q + 8 <= ns; q += 8) {  // eight loads in flight, summed in slice order
+      TW v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = w0[(size_t)(q + u) * qs];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += (double)v[u];
+    }
+    for (; q < ns; ++q) s += (double)w0[(size_t)q * qs];
   } else {
     const long lo = (long)t * sk.nk, hi = lo + sk.nk - 1;
     const int b_lo = (int)(lo / sk.L), b_hi = (int)(hi / sk.L);
@@ -868,8 +877,9 @@ int launch_sk(Ctx* c, const Sk& sk, K1 kernel, const T* A, int lda, const T* B, 
   TW* W = static_cast<TW*>(state(c)->ws);
   hipLaunchKernelGGL(kernel, dim3(sk.G), dim3(256), 0, c->stream, sk, A, lda, B, ldb, C, W);
   const int tiles = sk.tiles_m * sk.tiles_n;
-  hipLaunchKernelGGL((k_mg_fixup<TW, T, TC>), dim3(TE * TE / 256, tiles), dim3(256), 0,
-                     c->stream, sk, W, A, lda, B, ldb, C);
+  if (sk_slots(sk))  // else every tile was written whole by its workgroup
+    hipLaunchKernelGGL((k_mg_fixup<TW, T, TC>), dim3(TE * TE / 256, tiles), dim3(256), 0,
+                       c->stream, sk, W, A, lda, B, ldb, C);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
