@@ -170,23 +170,34 @@ def scf_errors(r, ref, natom):
             "max_force_ha_bohr": float(np.abs(r["forces"] - ref["forces"]).max() / 2)}
 
 
-def md_run(ctx, w, nsteps=3, max_scf=40, tol_ha_atom=1e-6, dt=40.0):
-    """Converged MD steps at constant energy (Si masses, dt in a.u.), each SCF
-    in mixed precision with the final FP64 refinement sweep, device-resident
-    except the O(natom) Verlet updates.  Per step: half kick and position
-    update (host); structure factors, KB table and local potential at the new
-    positions (device); sbalinaspc; SCF iterations until the Harris-Foulkes
-    energy changes by less than tol_ha_atom per atom; KB, local and
-    reciprocal-space Ewald forces (the Ewald real-space sum is not part of the
-    path); second half kick.  Step 0 converges from the starting bands."""
+def md_run(ctx, w, nsteps=3, max_scf=40, tol_ha_atom=1e-6, dt=41.341, temperature=300.0,
+           seed=7):
+    """Converged MD steps at constant energy (Si masses, dt in a.u.: 1 fs),
+    from seeded Maxwell-Boltzmann velocities at `temperature` (K, centre of
+    mass at rest), each SCF in mixed precision with the final FP64 refinement
+    sweep, device-resident except the O(natom) Verlet updates.  Per step:
+    half kick and position update (host); structure factors, KB table and
+    local potential at the new positions (device); sbalinaspc; SCF
+    iterations until the Harris-Foulkes energy changes by less than
+    tol_ha_atom per atom; KB, local and reciprocal-space Ewald forces (the
+    Ewald real-space sum is not part of the path); second half kick.  Step 0
+    converges from the starting bands.  After the last step an FP64 batched
+    SCF is continued at the same positions to 1e-8 Ha/atom, and the last
+    step's forces and energy are compared with its."""
     from qxmd_amd import md, pw
     natom, nb, occ = w.natom, w.nband, w.occ
     hcell = w.cell.hcell
     species = np.ones(natom, np.int32)
     lfix = np.zeros(1, bool)
-    acon = md.acon([28.0855 * 1822.888486], dt)
+    mass = 28.0855 * 1822.888486
+    acon = md.acon([mass], dt)
     x = w.frac.copy()
-    v = np.zeros_like(x)
+    # velocities: Cartesian components N(0, kT/m) in a.u., the mean removed,
+    # as displacements per step in cell units (the Verlet routines' v)
+    rng = np.random.default_rng(seed)
+    vc = rng.normal(0.0, np.sqrt(3.166811563e-6 * temperature / mass), (natom, 3))
+    vc -= vc.mean(axis=0)
+    v = (vc * dt) @ np.linalg.inv(hcell).T
     sc = w.scf(ctx, x)
     ctx.put(0, w.psi_orth)
     ctx.aspc_init(3, 2)
@@ -201,6 +212,7 @@ def md_run(ctx, w, nsteps=3, max_scf=40, tol_ha_atom=1e-6, dt=40.0):
     first = {"scf_iterations": n0, "converged": conv0,
              "ms": round((time.perf_counter() - t0) * 1e3, 2)}
     steps, iters, convs = [], [], []
+    f = e = None
     for k in range(1, nsteps + 1):
         ctx.sync()
         t0 = time.perf_counter()
@@ -208,17 +220,29 @@ def md_run(ctx, w, nsteps=3, max_scf=40, tol_ha_atom=1e-6, dt=40.0):
         x = md.updtps_pbc(md.updtps(x, v, species, lfix))
         sc.set_positions(x)
         ctx.sbalinaspc(0, nb, k)
-        n, conv, _ = sc.iterate(occ, tol_ha_atom, max_scf, pw.MIXED, True,
-                                refine_ha_atom=tol_ha_atom)
-        a = md.normalized_a(sc.forces(occ), species, hcell, acon)
+        n, conv, hist = sc.iterate(occ, tol_ha_atom, max_scf, pw.MIXED, True,
+                                   refine_ha_atom=tol_ha_atom)
+        f = sc.forces(occ)
+        e = hist[-1]
+        a = md.normalized_a(f, species, hcell, acon)
         v = md.vkick(v, a, species, lfix)
         ctx.sync()
         steps.append((time.perf_counter() - t0) * 1e3)
         iters.append(n)
         convs.append(conv)
-    return {"step_ms": round(float(np.mean(steps)), 2), "steps": nsteps,
-            "scf_iterations": iters, "converged": all(convs), "tol_ha_atom": tol_ha_atom,
-            "scf_prec": "mixed + FP64 refinement", "first_step": first}
+    out = {"step_ms": round(float(np.mean(steps)), 2), "steps": nsteps, "dt_fs": 1.0,
+           "temperature_k": temperature, "scf_iterations": iters, "converged": all(convs),
+           "tol_ha_atom": tol_ha_atom, "scf_prec": "mixed + FP64 refinement",
+           "first_step": first}
+    if f is not None:
+        n_ref, conv_ref, hist_ref = sc.iterate(occ, 1e-8, max_scf, pw.FP64, True)
+        f_ref = sc.forces(occ)
+        out["last_step_vs_fp64"] = {
+            "reference": "FP64 batched SCF continued at the last step's positions to 1e-8 Ha/atom",
+            "reference_scf_iterations": n_ref, "reference_converged": conv_ref,
+            "max_force_err_ha_bohr": float(np.abs(f - f_ref).max()) / 2.0,
+            "energy_err_ha_atom": float(abs(e - hist_ref[-1])) / 2.0 / natom}
+    return out
 
 
 def fft_stage_lower_bytes(gs, ncol, nb):
@@ -553,6 +577,7 @@ def main():
     ap.add_argument("--cpu-worker", action="store_true")
     ap.add_argument("--pmc-worker", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--md-steps", type=int, default=3, help="converged MD steps of the MD run")
     args = ap.parse_args()
     natom = 8 * args.ncell ** 3
     nband = args.nband or int(2.5 * natom)  # 20 / 160 / 540 / 1280
@@ -712,7 +737,7 @@ def main():
         print(f"# MD-step pieces: {extra['md_parts']}", file=sys.stderr, flush=True)
         extra["sbalinaspc_ms"] = extrapolation(ctx, w.psi_orth, nband)
         print(f"# sbalinaspc: {extra['sbalinaspc_ms']}", file=sys.stderr, flush=True)
-        extra["md_step"] = md_run(ctx, w)
+        extra["md_step"] = md_run(ctx, w, nsteps=args.md_steps)
         print(f"# converged MD step: {extra['md_step']}", file=sys.stderr, flush=True)
     if not args.no_kpoints:
         extra["kpoints"] = kpoint_bench(args.steps)
