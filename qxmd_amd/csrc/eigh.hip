@@ -870,10 +870,11 @@ __global__ void k_sync_take(int* sync, int* out) {
 // ---- host driver ----------------------------------------------------------------
 // workgroups of the tridiagonalisation: four rows each up to n = 640 (at
 // n = 540 on one MI355X: 135 groups 7.68 ms per rotation phase, 68 groups
-// 7.83 ms, 256 groups 8.0 ms), eight above
+// 7.83 ms, 256 groups 8.0 ms), five above (n = 1280: 256 groups 22.3 ms per
+// eigensolve, 160 groups of eight rows 23.3 ms)
 static int tridiag_groups(int n) {
   if (const char* s = std::getenv("QX_TRID_WG")) return std::max(1, std::atoi(s));
-  return std::max(1, n <= 640 ? (n + 3) / 4 : (n + 7) / 8);
+  return std::max(1, n <= 640 ? (n + 3) / 4 : (n + 4) / 5);
 }
 
 // z (n x n, column-major, vectors in columns) <- QR factor of z by CholeskyQR
