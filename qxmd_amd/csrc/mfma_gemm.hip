@@ -520,6 +520,13 @@ __global__ void __launch_bounds__(256, 2)
 typedef unsigned short u16;
 constexpr int KP = 64, LPP = KP + 8;  // pre-split K stage, LDS row pitch (bf16)
 
+// NN products stage A transposed (k-contiguous rows of m): the 16-byte
+// k-chunk c of row r sits at chunk c ^ ((r >> 4) & 7), which spreads the
+// transposing stores of 32 lanes (rows 4j + u) over the banks (8-way -> 2-way
+// conflicts in a 64-bank model) and keeps the MFMA fragment reads, whose 16
+// rows share one swizzle, conflict-free
+__device__ inline int nn_col(int r, int k) { return ((((k >> 3) ^ (r >> 4)) & 7) << 3) | (k & 7); }
+
 // 4 k rows of 4 m values (each row a pair of packed bf16 pairs) -> the run
 // of 4 k values of m = u (16-bit transpose)
 __device__ inline uint2 t4(uint2 r0, uint2 r1, uint2 r2, uint2 r3, int u) {
@@ -608,10 +615,11 @@ __global__ void __launch_bounds__(256, 2)
       } else {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          *reinterpret_cast<uint2*>(&Ah[am + u][ak]) = t4(nh0, nh1, nh2, nh3, u);
-          *reinterpret_cast<uint2*>(&Ah[am + u][ak + 4]) = t4(nh4, nh5, nh6, nh7, u);
-          *reinterpret_cast<uint2*>(&Al[am + u][ak]) = t4(nl0, nl1, nl2, nl3, u);
-          *reinterpret_cast<uint2*>(&Al[am + u][ak + 4]) = t4(nl4, nl5, nl6, nl7, u);
+          const int c = nn_col(am + u, ak);
+          *reinterpret_cast<uint2*>(&Ah[am + u][c]) = t4(nh0, nh1, nh2, nh3, u);
+          *reinterpret_cast<uint2*>(&Ah[am + u][c + 4]) = t4(nh4, nh5, nh6, nh7, u);
+          *reinterpret_cast<uint2*>(&Al[am + u][c]) = t4(nl0, nl1, nl2, nl3, u);
+          *reinterpret_cast<uint2*>(&Al[am + u][c + 4]) = t4(nl4, nl5, nl6, nl7, u);
         }
       }
       *reinterpret_cast<uint4*>(&Bh[lr][lk + 0]) = bh0;
@@ -636,8 +644,9 @@ __global__ void __launch_bounds__(256, 2)
         bf8 xh[4], xl[4], yh[4], yl[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          xh[i] = *reinterpret_cast<const bf8*>(&Ah[wm + 16 * i + r][kk]);
-          xl[i] = *reinterpret_cast<const bf8*>(&Al[wm + 16 * i + r][kk]);
+          const int ra = wm + 16 * i + r, ka = AK ? kk : nn_col(ra, kk);
+          xh[i] = *reinterpret_cast<const bf8*>(&Ah[ra][ka]);
+          xl[i] = *reinterpret_cast<const bf8*>(&Al[ra][ka]);
           yh[i] = *reinterpret_cast<const bf8*>(&Bh[wn + 16 * i + r][kk]);
           yl[i] = *reinterpret_cast<const bf8*>(&Bl[wn + 16 * i + r][kk]);
         }
