@@ -356,6 +356,12 @@ __device__ inline void load4<double>(const double* p, float* r) {
   r[0] = (float)a.x; r[1] = (float)a.y; r[2] = (float)b.x; r[3] = (float)b.y;
 }
 
+// NN: the 16-byte k-chunk c of staged row r sits at chunk c ^ ((r >> 4) & 3)
+// (four chunks per 32-deep stage), which spreads the transposing stores of
+// 32 lanes (rows 4j + u) over the banks (8-way -> 2-way conflicts in a
+// 64-bank model) and keeps the fragment reads of 16 rows conflict-free
+__device__ inline int nn_col4(int r, int k) { return ((((k >> 3) ^ (r >> 4)) & 3) << 3) | (k & 7); }
+
 // AK: TN (A rows k-contiguous), else NN (A m-contiguous); T the operands'
 // type, TC the output's; partial segments go to Wp as floats
 template <typename T, typename TC, bool AK>
@@ -451,8 +457,9 @@ __global__ void __launch_bounds__(256, 2)
           unsigned h[2], l[2];
           split2(ra[u], ra[4 + u], h[0], l[0]);
           split2(ra[8 + u], ra[12 + u], h[1], l[1]);
-          *reinterpret_cast<uint2*>(&Ah[am + u][ak]) = make_uint2(h[0], h[1]);
-          *reinterpret_cast<uint2*>(&Al[am + u][ak]) = make_uint2(l[0], l[1]);
+          const int c = nn_col4(am + u, ak);
+          *reinterpret_cast<uint2*>(&Ah[am + u][c]) = make_uint2(h[0], h[1]);
+          *reinterpret_cast<uint2*>(&Al[am + u][c]) = make_uint2(l[0], l[1]);
         }
       }
       split16(rb, h0, h1, l0, l1);
@@ -471,8 +478,9 @@ __global__ void __launch_bounds__(256, 2)
       bf8 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        ah[i] = *reinterpret_cast<const bf8*>(&Ah[wm + 16 * i + r][kk]);
-        al[i] = *reinterpret_cast<const bf8*>(&Al[wm + 16 * i + r][kk]);
+        const int ra = wm + 16 * i + r, ka = AK ? kk : nn_col4(ra, kk);
+        ah[i] = *reinterpret_cast<const bf8*>(&Ah[ra][ka]);
+        al[i] = *reinterpret_cast<const bf8*>(&Al[ra][ka]);
         bh[i] = *reinterpret_cast<const bf8*>(&Bh[wn + 16 * i + r][kk]);
         bl[i] = *reinterpret_cast<const bf8*>(&Bl[wn + 16 * i + r][kk]);
       }
