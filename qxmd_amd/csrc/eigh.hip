@@ -62,9 +62,13 @@ __device__ inline double block_sum256(double v, double* red) {
 // workgroup, and each workgroup publishes that step only after it has read all
 // of its inputs of the step before.  Waits are bounded (err raised, as in
 // grid_barrier), so a launch that cannot complete still ends.
-constexpr int kTrdM = 8;   // columns per thread: n <= 256 * kTrdM
+// M: columns per thread, n <= 256 M; instantiated for M = 3, 5, 8 (the
+// smallest that covers n), so a step issues and polls only the loads of the
+// columns that exist
+constexpr int kTrdM = 8;   // largest M: n <= 256 * kTrdM
 constexpr int kTrdKq = 4;  // K shares read per lane: groups <= 64 * kTrdKq
 
+template <int M>
 __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, int n,
                                                       double* __restrict__ d,
                                                       double* __restrict__ e, double* rowbuf,
@@ -91,9 +95,9 @@ __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, in
   if (g == 0)  // row 0 is column 0: publish it (row 0 lives in workgroup 0)
     for (int j = t; j < n; j += 256) st_agent(&rowbuf[j], R[j]);
   grid_barrier(bar, G, err);
-  double x[kTrdM], vr[kTrdM], w[kTrdM];
+  double x[M], vr[M], w[M];
 #pragma unroll
-  for (int m = 0; m < kTrdM; ++m) {
+  for (int m = 0; m < M; ++m) {
     const int j = t + 256 * m;
     x[m] = j < n ? ld_agent(&rowbuf[j]) : 0.0;
     vr[m] = 0.0;
@@ -111,7 +115,7 @@ __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, in
     // reflector of column k, formed redundantly by every workgroup
     double part = 0.0;
 #pragma unroll
-    for (int m = 0; m < kTrdM; ++m) {
+    for (int m = 0; m < M; ++m) {
       const int j = t + 256 * m;
       if (j > k && j < n) part += x[m] * x[m];
       if (j == k + 1) rd[4] = x[m];
@@ -126,7 +130,7 @@ __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, in
     const bool skip = !(vn2 > 1e-300);
     const double inv = skip ? 0.0 : 1.0 / sqrt(vn2);
 #pragma unroll
-    for (int m = 0; m < kTrdM; ++m) {
+    for (int m = 0; m < M; ++m) {
       const int j = t + 256 * m;
       if (j > k && j < n) {
         vr[m] = (j == k + 1 ? x[m] - alpha : x[m]) * inv;
@@ -160,7 +164,7 @@ __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, in
     if (t == 0) st_tagged(kq + 2 * g, kw[0] + kw[1] + kw[2] + kw[3], tag);
     // wait for this step's values: every load is issued before any tag is
     // tested (entries clamped to published ones), so their latencies overlap
-    unsigned long long kw0[kTrdKq], kw1[kTrdKq], pw0[kTrdM], pw1[kTrdM], rw0[kTrdM], rw1[kTrdM];
+    unsigned long long kw0[kTrdKq], kw1[kTrdKq], pw0[M], pw1[M], rw0[M], rw1[M];
     unsigned long long p10, p11, r10, r11;
     for (int spins = 0;; ++spins) {
 #pragma unroll
@@ -174,7 +178,7 @@ __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, in
       r10 = ld_u64_agent(rq + 2 * (k + 1));
       r11 = ld_u64_agent(rq + 2 * (k + 1) + 1);
 #pragma unroll
-      for (int m = 0; m < kTrdM; ++m) {
+      for (int m = 0; m < M; ++m) {
         const int j = min(max(t + 256 * m, k + 1), n - 1);
         pw0[m] = ld_u64_agent(pq + 2 * j);
         pw1[m] = ld_u64_agent(pq + 2 * j + 1);
@@ -185,7 +189,7 @@ __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, in
 #pragma unroll
       for (int u = 0; u < kTrdKq; ++u) all = all && tag_ok(kw0[u], kw1[u], tag);
 #pragma unroll
-      for (int m = 0; m < kTrdM; ++m)
+      for (int m = 0; m < M; ++m)
         all = all && tag_ok(pw0[m], pw1[m], tag) && tag_ok(rw0[m], rw1[m], tag);
       if (all || failed) break;
       if ((spins & 1023) == 1023 &&
@@ -206,7 +210,7 @@ __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, in
       const double v1 = v[k + 1];
       const double w1 = 2.0 * (p1 - K * v1);
 #pragma unroll
-      for (int m = 0; m < kTrdM; ++m) {
+      for (int m = 0; m < M; ++m) {
         const int j = t + 256 * m;
         if (j > k && j < n) w[m] = 2.0 * (tag_value(pw0[m], pw1[m]) - K * vr[m]);
         if (j > k + 1 && j < n) x[m] = tag_value(rw0[m], rw1[m]) - v1 * w[m] - w1 * vr[m];
@@ -218,14 +222,14 @@ __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, in
         const double vi = v[i], wi = 2.0 * (prow[r] - K * vi);
         double* Rr = R + (size_t)r * n;
 #pragma unroll
-        for (int m = 0; m < kTrdM; ++m) {
+        for (int m = 0; m < M; ++m) {
           const int j = t + 256 * m;
           if (j > k + 1 && j < n) Rr[j] -= vi * w[m] + wi * vr[m];
         }
       }
     } else {
 #pragma unroll
-      for (int m = 0; m < kTrdM; ++m) {
+      for (int m = 0; m < M; ++m) {
         const int j = t + 256 * m;
         if (j > k + 1 && j < n) x[m] = tag_value(rw0[m], rw1[m]);
       }
@@ -234,7 +238,7 @@ __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, in
   }
   if (g == 0) {
 #pragma unroll
-    for (int m = 0; m < kTrdM; ++m)
+    for (int m = 0; m < M; ++m)
       if (t + 256 * m == n - 1) e[n - 2] = x[m];
   }
   // R(n-1, n-1) was last updated by the thread owning column n-1: it reads it
@@ -936,8 +940,10 @@ int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int
   const long npub = 8 * (long)n + 4 * (long)G;
   hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((npub + 255) / 256)), dim3(256), 0, st, pub, npub);
   void* args[] = {&a, &n, &d, &e, &rowbuf, &pub, &bar, &err};
-  if (coop_launch(c, reinterpret_cast<const void*>(k_tridiag_coop), G, lds, args, true) < 0)
-    return dense_eigh_legacy(c, a, n, z, w, work);
+  const void* trd = n <= 256 * 3   ? reinterpret_cast<const void*>(k_tridiag_coop<3>)
+                    : n <= 256 * 5 ? reinterpret_cast<const void*>(k_tridiag_coop<5>)
+                                   : reinterpret_cast<const void*>(k_tridiag_coop<kTrdM>);
+  if (coop_launch(c, trd, G, lds, args, true) < 0) return dense_eigh_legacy(c, a, n, z, w, work);
   // 2. eigenvalues
   hipLaunchKernelGGL(k_gersh, dim3(1), dim3(256), 0, st, d, e, n, bounds);
   hipLaunchKernelGGL(k_bisect_ms, dim3((n + 3) / 4), dim3(256), 0, st, d, e, n, bounds, w);
