@@ -62,13 +62,13 @@ __device__ inline double block_sum256(double v, double* red) {
 // workgroup, and each workgroup publishes that step only after it has read all
 // of its inputs of the step before.  Waits are bounded (err raised, as in
 // grid_barrier), so a launch that cannot complete still ends.
-// M: columns per thread, n <= 256 M; instantiated for M = 3, 5, 8 (the
-// smallest that covers n), so a step issues and polls only the loads of the
-// columns that exist
+// M: columns per thread, n <= 256 M, and KQ: K shares per lane, G <= 64 KQ;
+// instantiated for the smallest that cover n and G (M = 3, 5, 8; KQ = 3, 4),
+// so a step issues and polls only the loads of columns and shares that exist
 constexpr int kTrdM = 8;   // largest M: n <= 256 * kTrdM
 constexpr int kTrdKq = 4;  // K shares read per lane: groups <= 64 * kTrdKq
 
-template <int M>
+template <int M, int KQ>
 __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, int n,
                                                       double* __restrict__ d,
                                                       double* __restrict__ e, double* rowbuf,
@@ -164,11 +164,11 @@ __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, in
     if (t == 0) st_tagged(kq + 2 * g, kw[0] + kw[1] + kw[2] + kw[3], tag);
     // wait for this step's values: every load is issued before any tag is
     // tested (entries clamped to published ones), so their latencies overlap
-    unsigned long long kw0[kTrdKq], kw1[kTrdKq], pw0[M], pw1[M], rw0[M], rw1[M];
+    unsigned long long kw0[KQ], kw1[KQ], pw0[M], pw1[M], rw0[M], rw1[M];
     unsigned long long p10, p11, r10, r11;
     for (int spins = 0;; ++spins) {
 #pragma unroll
-      for (int u = 0; u < kTrdKq; ++u) {
+      for (int u = 0; u < KQ; ++u) {
         const unsigned long long* s = kq + 2 * min(lane + 64 * u, G - 1);
         kw0[u] = ld_u64_agent(s);
         kw1[u] = ld_u64_agent(s + 1);
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, in
       }
       bool all = tag_ok(p10, p11, tag) && tag_ok(r10, r11, tag);
 #pragma unroll
-      for (int u = 0; u < kTrdKq; ++u) all = all && tag_ok(kw0[u], kw1[u], tag);
+      for (int u = 0; u < KQ; ++u) all = all && tag_ok(kw0[u], kw1[u], tag);
 #pragma unroll
       for (int m = 0; m < M; ++m)
         all = all && tag_ok(pw0[m], pw1[m], tag) && tag_ok(rw0[m], rw1[m], tag);
@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(256) k_tridiag_coop(double* __restrict__ a, in
     const double p1 = tag_value(p10, p11), r1 = tag_value(r10, r11);
     double ks = 0.0;
 #pragma unroll
-    for (int u = 0; u < kTrdKq; ++u) ks += lane + 64 * u < G ? tag_value(kw0[u], kw1[u]) : 0.0;
+    for (int u = 0; u < KQ; ++u) ks += lane + 64 * u < G ? tag_value(kw0[u], kw1[u]) : 0.0;
     const double K = wave_sum(ks);
     if (!skip) {
       const double v1 = v[k + 1];
@@ -940,9 +940,11 @@ int dense_eigh(Ctx* c, double* a, int n, double* z, double* w, double* work, int
   const long npub = 8 * (long)n + 4 * (long)G;
   hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((npub + 255) / 256)), dim3(256), 0, st, pub, npub);
   void* args[] = {&a, &n, &d, &e, &rowbuf, &pub, &bar, &err};
-  const void* trd = n <= 256 * 3   ? reinterpret_cast<const void*>(k_tridiag_coop<3>)
-                    : n <= 256 * 5 ? reinterpret_cast<const void*>(k_tridiag_coop<5>)
-                                   : reinterpret_cast<const void*>(k_tridiag_coop<kTrdM>);
+  const void* trd =
+      n <= 256 * 3 ? (G <= 64 * 3 ? reinterpret_cast<const void*>(k_tridiag_coop<3, 3>)
+                                  : reinterpret_cast<const void*>(k_tridiag_coop<3, kTrdKq>))
+      : n <= 256 * 5 ? reinterpret_cast<const void*>(k_tridiag_coop<5, kTrdKq>)
+                     : reinterpret_cast<const void*>(k_tridiag_coop<kTrdM, kTrdKq>);
   if (coop_launch(c, trd, G, lds, args, true) < 0) return dense_eigh_legacy(c, a, n, z, w, work);
   // 2. eigenvalues
   hipLaunchKernelGGL(k_gersh, dim3(1), dim3(256), 0, st, d, e, n, bounds);
