@@ -29,9 +29,15 @@
 // partial first / last segments of its range in a workspace; one fix-up pass
 // sums the partials of each shared tile in workgroup order (deterministic).
 // Workgroups are numbered so that each XCD (dispatch round-robin over 8)
-// works a contiguous stretch of the iteration space, sharing its L2.  G is
-// chosen per shape by timing the candidates on first use (into a scratch
-// output) and cached.
+// works a contiguous stretch of the iteration space, sharing its L2.
+//
+// K-sliced alternative: one workgroup per (K-slice, tile), slices numbered
+// slowest, so an XCD's workgroups sweep the same K-slice of every tile in
+// step and fetch each operand stage once into its L2 (stream-K fetched it
+// once per tile); the fix-up sums the slices in order.  With one slice it is
+// one workgroup per tile and no fix-up.  Per shape the stream-K grid or the
+// slice count is chosen by timing the candidates on first use (into a
+// scratch output) and cached.
 #include <algorithm>
 #include <atomic>
 #include <cstdlib>
